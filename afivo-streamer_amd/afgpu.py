@@ -1,0 +1,439 @@
+"""ctypes bindings for the afivo-streamer hot path on MI355X.
+
+Three shared libraries are involved:
+
+* ``libafgpu.so`` -- the product: HIP kernels for gfx950 behind the C-ABI in
+  ``include/afgpu.h``.
+* ``libafhost.so`` -- host-side stand-in for what stays on the CPU in the
+  reference (AMR tree, initial conditions, lookup tables).
+* ``oracle/liboracle_afcpu.so`` -- the CPU oracle (test infrastructure; only
+  tests, the smoke check and the CPU-baseline leg of bench.py load it).
+
+``Path`` wraps one handle of either the device library or the oracle with
+the same method names as the reference's routines (``field_compute``,
+``mg_fas_vcycle``, ``forward_euler`` ...), so that parity tests read like the
+reference's own call sequence.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+REPO_DIR = os.path.dirname(PKG_DIR)
+GPU_LIB = os.path.join(PKG_DIR, "libafgpu.so")
+HOST_LIB = os.path.join(PKG_DIR, "libafhost.so")
+ORACLE_LIB = os.path.join(REPO_DIR, "oracle", "liboracle_afcpu.so")
+
+# ---- constants mirrored from include/afgpu.h -------------------------------
+OK = 0
+XYZ, CYL = 1, 2
+PREC_FP64, PREC_MIXED = 0, 1
+FORWARD_EULER, HEUNS_METHOD = 1, 2
+CC_ELECTRON, CC_POS_ION, CC_NEG_ION = 0, 3, 6
+CC_PHI, CC_PHI_2, CC_EFIELD, CC_RHS, CC_TMP, CC_PHOTO, CC_HELMH = 9, 10, 11, 12, 13, 14, 15
+N_CC = 18
+FC_FLUX, FC_FIELD = 0, 1
+OP_POISSON, OP_HELMH = 0, 1
+MAX_MODES = 3
+
+
+class ParamsT(C.Structure):
+    _fields_ = [
+        ("gas_number_density", C.c_double),
+        ("time_integrator", C.c_int),
+        ("dt_max", C.c_double),
+        ("dt_min", C.c_double),
+        ("dt_safety_factor", C.c_double),
+        ("dt_cfl_number", C.c_double),
+        ("dt_max_growth_factor", C.c_double),
+        ("dt_chemistry_limit_loss", C.c_int),
+        ("dt_chemistry_nmin", C.c_double),
+        ("mg_num_vcycles", C.c_int),
+        ("mg_max_rel_residual", C.c_double),
+        ("mg_n_cycle_down", C.c_int),
+        ("mg_n_cycle_up", C.c_int),
+        ("domain_len_last", C.c_double),
+        ("species_bc", C.c_int),
+        ("photoi_enabled", C.c_int),
+        ("photoi_per_steps", C.c_int),
+        ("photoi_species", C.c_int),
+        ("photoi_eta_quench", C.c_double),
+        ("photoi_n_modes", C.c_int),
+        ("photoi_lambda", C.c_double * MAX_MODES),
+        ("photoi_coeff", C.c_double * MAX_MODES),
+        ("photoi_max_rel_residual", C.c_double),
+        ("photoi_max_fmg_cycles", C.c_int),
+    ]
+
+
+class StepInfoT(C.Structure):
+    _fields_ = [
+        ("it", C.c_int),
+        ("time", C.c_double),
+        ("dt_used", C.c_double),
+        ("dt_next", C.c_double),
+        ("dt_lim", C.c_double),
+        ("n_rejected", C.c_int),
+        ("n_vcycles", C.c_int),
+        ("residual", C.c_double),
+        ("rel_residual", C.c_double),
+        ("photoi_done", C.c_int),
+        ("leaf_cells", C.c_longlong),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_P = C.c_void_p
+_I = C.c_int
+_D = C.c_double
+_IP = C.POINTER(C.c_int)
+_DP = C.POINTER(C.c_double)
+
+# name -> (restype, argtypes) for the functions common to afgpu_/afcpu_
+_COMMON = {
+    "create": (_I, [_I, _I, _I, _I, C.POINTER(_P)]),
+    "destroy": (_I, [_P]),
+    "last_error": (C.c_char_p, []),
+    "sync_tree": (_I, [_P, _I, _IP, _IP, _IP, _IP, _IP, _IP, _DP, _DP]),
+    "put_cc": (_I, [_P, _I, _I, _IP, _DP]),
+    "get_cc": (_I, [_P, _I, _I, _IP, _DP]),
+    "put_fc": (_I, [_P, _I, _I, _IP, _DP]),
+    "get_fc": (_I, [_P, _I, _I, _IP, _DP]),
+    "set_params": (_I, [_P, C.POINTER(ParamsT)]),
+    "set_tables": (_I, [_P, _I, _D, _D, _DP, _D, _D, _DP]),
+    "set_voltage": (_I, [_P, _D]),
+    "mg_fas_vcycle": (_I, [_P, _I, _I, _I]),
+    "mg_fas_fmg": (_I, [_P, _I, _I, _I]),
+    "tree_maxabs_cc": (_I, [_P, _I, _DP]),
+    "gc_tree": (_I, [_P, _I]),
+    "restrict_tree": (_I, [_P, _I]),
+    "field_compute": (_I, [_P, _I, _I, _IP, _DP]),
+    "field_from_potential": (_I, [_P]),
+    "forward_euler": (_I, [_P, _D, _I, _I, _IP, _DP, _I, _I, _I, _DP]),
+    "advance": (_I, [_P, _D, _DP]),
+    "photoi_set_src": (_I, [_P]),
+    "copy_current_state": (_I, [_P]),
+    "restore_previous_state": (_I, [_P]),
+    "step": (_I, [_P, C.POINTER(StepInfoT)]),
+    "num_leaf_cells": (C.c_longlong, [_P]),
+}
+_GPU_ONLY = {
+    "coarse_factors": (_I, [_I, _I, _I, _DP, _DP, _IP, _DP, _DP, _DP]),
+    "kernel_timing": (_I, [_P, _I]),
+    "kernel_stats": (_I, [_P, _I, _I, _DP, C.POINTER(C.c_longlong), _DP]),
+}
+_ORACLE_ONLY = {
+    "set_coarse_factors": (_I, [_P, _I, _DP, _DP, _DP]),
+    "num_threads": (_I, []),
+}
+
+_libs = {}
+
+
+class AfError(RuntimeError):
+    pass
+
+
+def _bind(lib, prefix, table):
+    for name, (res, args) in table.items():
+        f = getattr(lib, prefix + name)
+        f.restype = res
+        f.argtypes = args
+
+
+def load_gpu():
+    """Load the device library; raises if it is missing (no fallback)."""
+    if "gpu" not in _libs:
+        if not os.path.exists(GPU_LIB):
+            raise AfError("libafgpu.so not built: run __graft_entry__.build()")
+        lib = C.CDLL(GPU_LIB)
+        _bind(lib, "afgpu_", _COMMON)
+        _bind(lib, "afgpu_", _GPU_ONLY)
+        _libs["gpu"] = lib
+    return _libs["gpu"]
+
+
+def load_oracle():
+    if "oracle" not in _libs:
+        if not os.path.exists(ORACLE_LIB):
+            raise AfError("oracle library not built: run __graft_entry__.build()")
+        lib = C.CDLL(ORACLE_LIB)
+        _bind(lib, "afcpu_", _COMMON)
+        _bind(lib, "afcpu_", _ORACLE_ONLY)
+        _libs["oracle"] = lib
+    return _libs["oracle"]
+
+
+def load_host():
+    if "host" not in _libs:
+        if not os.path.exists(HOST_LIB):
+            raise AfError("libafhost.so not built: run __graft_entry__.build()")
+        lib = C.CDLL(HOST_LIB)
+        lib.afhost_tree_create.restype = _I
+        lib.afhost_tree_create.argtypes = [_I, _I, _I, _DP, _DP, C.POINTER(_P)]
+        lib.afhost_tree_destroy.argtypes = [_P]
+        lib.afhost_refine.restype = _I
+        lib.afhost_refine.argtypes = [_P, _I, _IP]
+        lib.afhost_refine_uniform.restype = _I
+        lib.afhost_refine_uniform.argtypes = [_P, _I]
+        lib.afhost_tree_info.restype = _I
+        lib.afhost_tree_info.argtypes = [_P, _IP, _IP, C.POINTER(C.c_longlong), _IP]
+        lib.afhost_tree_export.restype = _I
+        lib.afhost_tree_export.argtypes = [_P, _IP, _IP, _IP, _IP, _IP, _IP, _DP, _DP]
+        lib.afhost_init_cond.restype = _I
+        lib.afhost_init_cond.argtypes = [_P, _I, _IP, _D, _D, C.c_ulonglong, _D, _D, _DP, _DP]
+        lib.afhost_transport_tables.restype = _I
+        lib.afhost_transport_tables.argtypes = [_I, _DP, _DP, _DP, _DP, _DP, _D, _I, _D,
+                                                _DP, _DP, _DP, _DP, _DP, _DP]
+        _libs["host"] = lib
+    return _libs["host"]
+
+
+def _ip(a):
+    return a.ctypes.data_as(_IP)
+
+
+def _dp(a):
+    return a.ctypes.data_as(_DP)
+
+
+# ---- host tree --------------------------------------------------------------
+class Tree:
+    """The box tree (stand-in for afivo's af_t on the host)."""
+
+    def __init__(self, ndim, nc, coord, r_min, domain_len):
+        self.lib = load_host()
+        self.D, self.nc, self.coord = ndim, nc, coord
+        self.h = _P()
+        r = np.ascontiguousarray(r_min, dtype=np.float64)
+        L = np.ascontiguousarray(domain_len, dtype=np.float64)
+        if self.lib.afhost_tree_create(ndim, nc, coord, _dp(r), _dp(L), C.byref(self.h)):
+            raise AfError("afhost_tree_create failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.afhost_tree_destroy(self.h)
+            self.h = None
+
+    def refine_uniform(self, lvl):
+        if self.lib.afhost_refine_uniform(self.h, lvl) < 0:
+            raise AfError("refine_uniform failed")
+
+    def refine(self, ids):
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        n = self.lib.afhost_refine(self.h, len(ids), _ip(ids))
+        if n < 0:
+            raise AfError("refine failed")
+        return n
+
+    def info(self):
+        nb, hl, nl = C.c_int(), C.c_int(), C.c_int()
+        cells = C.c_longlong()
+        self.lib.afhost_tree_info(self.h, C.byref(nb), C.byref(hl), C.byref(cells), C.byref(nl))
+        return {"n_boxes": nb.value, "highest_lvl": hl.value, "leaf_cells": cells.value,
+                "n_leaves": nl.value}
+
+    def export(self):
+        n = self.info()["n_boxes"]
+        D = self.D
+        a = {
+            "lvl": np.zeros(n, np.int32),
+            "ix": np.zeros(n * D, np.int32),
+            "parent": np.zeros(n, np.int32),
+            "children": np.zeros(n * 2 ** D, np.int32),
+            "neighbors": np.zeros(n * 2 * D, np.int32),
+            "nbmat": np.zeros(n * 3 ** D, np.int32),
+            "r_min": np.zeros(n * D, np.float64),
+            "dr": np.zeros(n * D, np.float64),
+        }
+        self.lib.afhost_tree_export(self.h, _ip(a["lvl"]), _ip(a["ix"]), _ip(a["parent"]),
+                                    _ip(a["children"]), _ip(a["neighbors"]), _ip(a["nbmat"]),
+                                    _dp(a["r_min"]), _dp(a["dr"]))
+        a["n_boxes"] = n
+        return a
+
+    def init_cond(self, ids, background, noise, seed, seed_density, seed_width, seed_r0):
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        ncell = (self.nc + 2) ** self.D
+        out = np.zeros((len(ids), ncell), np.float64)
+        r0 = np.ascontiguousarray(seed_r0, dtype=np.float64)
+        if self.lib.afhost_init_cond(self.h, len(ids), _ip(ids), background, noise, seed,
+                                     seed_density, seed_width, _dp(r0), _dp(out)):
+            raise AfError("init_cond failed")
+        return out
+
+
+def transport_tables(E, mu, dif, alpha, eta, N, table_size=1000, table_max_td=-1.0):
+    lib = load_host()
+    E = np.ascontiguousarray(E, np.float64)
+    args = [np.ascontiguousarray(x, np.float64) for x in (mu, dif, alpha, eta)]
+    td = np.zeros(4 * table_size)
+    ch = np.zeros(2 * table_size)
+    tx0, tif, cx0, cif = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+    if lib.afhost_transport_tables(len(E), _dp(E), *[_dp(x) for x in args], N, table_size,
+                                   table_max_td, C.byref(tx0), C.byref(tif), _dp(td),
+                                   C.byref(cx0), C.byref(cif), _dp(ch)):
+        raise AfError("transport_tables failed")
+    return {"n": table_size, "td_x_min": tx0.value, "td_inv_fac": tif.value, "td": td,
+            "chem_x_min": cx0.value, "chem_inv_fac": cif.value, "chem": ch}
+
+
+def coarse_factors(ndim, nc, coord, r_min, dr, bc_dirichlet):
+    lib = load_gpu()
+    r = np.ascontiguousarray(r_min, np.float64)
+    d = np.ascontiguousarray(dr, np.float64)
+    bc = np.ascontiguousarray(bc_dirichlet, np.int32)
+    vinv = np.zeros(ndim * nc * nc)
+    v = np.zeros(ndim * nc * nc)
+    eig = np.zeros(ndim * nc)
+    rc = lib.afgpu_coarse_factors(ndim, nc, coord, _dp(r), _dp(d), _ip(bc), _dp(vinv), _dp(v), _dp(eig))
+    if rc:
+        raise AfError("afgpu_coarse_factors failed (%d)" % rc)
+    return vinv, v, eig
+
+
+# ---- one handle of the device path or the oracle --------------------------
+class Path:
+    """A handle of the device library (kind='gpu') or of the oracle."""
+
+    def __init__(self, kind, ndim, nc, coord, precision=PREC_FP64):
+        self.kind = kind
+        self.lib = load_gpu() if kind == "gpu" else load_oracle()
+        self.pre = "afgpu_" if kind == "gpu" else "afcpu_"
+        self.D, self.nc, self.coord = ndim, nc, coord
+        self.ncell = (nc + 2) ** ndim
+        self.nface = ndim * (nc + 1) ** ndim
+        self.h = _P()
+        self._call("create", ndim, nc, coord, precision, C.byref(self.h), with_handle=False)
+        self.n_boxes = 0
+
+    def _fn(self, name):
+        return getattr(self.lib, self.pre + name)
+
+    def _call(self, name, *args, with_handle=True):
+        f = self._fn(name)
+        rc = f(self.h, *args) if with_handle else f(*args)
+        if rc != OK:
+            msg = self._fn("last_error")()
+            raise AfError("%s%s failed (%d): %s" % (self.pre, name, rc, msg.decode() if msg else ""))
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._fn("destroy")(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # tree / data -----------------------------------------------------------
+    def sync_tree(self, t):
+        self._call("sync_tree", t["n_boxes"], _ip(t["lvl"]), _ip(t["ix"]), _ip(t["parent"]),
+                   _ip(t["children"]), _ip(t["neighbors"]), _ip(t["nbmat"]),
+                   _dp(t["r_min"]), _dp(t["dr"]))
+        self.n_boxes = t["n_boxes"]
+        self.tree = t
+
+    def _ids(self, ids):
+        if ids is None:
+            ids = np.arange(1, self.n_boxes + 1, dtype=np.int32)
+        return np.ascontiguousarray(ids, dtype=np.int32)
+
+    def put_cc(self, iv, data, ids=None):
+        ids = self._ids(ids)
+        data = np.ascontiguousarray(data, np.float64).reshape(len(ids), self.ncell)
+        self._call("put_cc", iv, len(ids), _ip(ids), _dp(data))
+
+    def get_cc(self, iv, ids=None):
+        ids = self._ids(ids)
+        out = np.zeros((len(ids), self.ncell))
+        self._call("get_cc", iv, len(ids), _ip(ids), _dp(out))
+        return out
+
+    def put_fc(self, ifc, data, ids=None):
+        ids = self._ids(ids)
+        data = np.ascontiguousarray(data, np.float64).reshape(len(ids), self.nface)
+        self._call("put_fc", ifc, len(ids), _ip(ids), _dp(data))
+
+    def get_fc(self, ifc, ids=None):
+        ids = self._ids(ids)
+        out = np.zeros((len(ids), self.nface))
+        self._call("get_fc", ifc, len(ids), _ip(ids), _dp(out))
+        return out
+
+    def set_params(self, p):
+        self.params = p
+        self._call("set_params", C.byref(p))
+
+    def set_tables(self, tb):
+        self._call("set_tables", tb["n"], tb["td_x_min"], tb["td_inv_fac"], _dp(tb["td"]),
+                   tb["chem_x_min"], tb["chem_inv_fac"], _dp(tb["chem"]))
+
+    def set_voltage(self, v):
+        self._call("set_voltage", v)
+
+    def set_coarse_factors(self, op, vinv, v, eig):
+        """Oracle only: supply the exact level-1 solve factors."""
+        self._call("set_coarse_factors", op, _dp(vinv), _dp(v), _dp(eig))
+
+    # reference operator API -------------------------------------------------
+    def mg_fas_vcycle(self, op, set_residual=True, max_lvl=0):
+        self._call("mg_fas_vcycle", op, int(set_residual), max_lvl)
+
+    def mg_fas_fmg(self, op, set_residual=True, have_guess=True):
+        self._call("mg_fas_fmg", op, int(set_residual), int(have_guess))
+
+    def tree_maxabs_cc(self, iv):
+        out = C.c_double()
+        self._call("tree_maxabs_cc", iv, C.byref(out))
+        return out.value
+
+    def gc_tree(self, iv):
+        self._call("gc_tree", iv)
+
+    def restrict_tree(self, iv):
+        self._call("restrict_tree", iv)
+
+    def field_compute(self, s_in=0, have_guess=True):
+        n, r = C.c_int(), C.c_double()
+        self._call("field_compute", s_in, int(have_guess), C.byref(n), C.byref(r))
+        return n.value, r.value
+
+    def field_from_potential(self):
+        self._call("field_from_potential")
+
+    def forward_euler(self, dt, s_deriv, s_prev, w_prev, s_out, i_step, n_steps):
+        sp = np.ascontiguousarray(s_prev, np.int32)
+        wp = np.ascontiguousarray(w_prev, np.float64)
+        lim = C.c_double()
+        self._call("forward_euler", dt, s_deriv, len(sp), _ip(sp), _dp(wp), s_out, i_step,
+                   n_steps, C.byref(lim))
+        return lim.value
+
+    def advance(self, dt):
+        lim = C.c_double()
+        self._call("advance", dt, C.byref(lim))
+        return lim.value
+
+    def photoi_set_src(self):
+        self._call("photoi_set_src")
+
+    def copy_current_state(self):
+        self._call("copy_current_state")
+
+    def restore_previous_state(self):
+        self._call("restore_previous_state")
+
+    def step(self):
+        info = StepInfoT()
+        self._call("step", C.byref(info))
+        return info.as_dict()
+
+    def num_leaf_cells(self):
+        return self._fn("num_leaf_cells")(self.h)

@@ -1,0 +1,70 @@
+"""Build the native libraries in-tree.
+
+* libafgpu.so  -- HIP for gfx950 (hipcc), the product.
+* libafhost.so -- host-side tree / initial-condition / table stand-in (g++).
+* oracle/liboracle_afcpu.so -- the CPU oracle (test infrastructure).
+
+All floating-point code is built with -ffp-contract=off so that no fused
+multiply-adds are formed: the FP64 device path then evaluates every cell
+value with the same operations, in the same order, as the CPU path.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+REPO_DIR = os.path.dirname(PKG_DIR)
+INC = os.path.join(REPO_DIR, "include")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("AFGPU_ARCH", "gfx950")
+
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=" + ARCH,
+    "-ffp-contract=off",
+    # no packed FP32 instructions (see the build notes in DESIGN.md)
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result",
+]
+
+
+def _run(cmd, cwd=None):
+    print("+ " + " ".join(cmd), flush=True)
+    subprocess.run(cmd, cwd=cwd, check=True)
+
+
+def build_gpu():
+    out = os.path.join(PKG_DIR, "libafgpu.so")
+    srcs = [os.path.join(PKG_DIR, "csrc", "afgpu.hip"),
+            os.path.join(PKG_DIR, "csrc", "coarse_factors.cpp")]
+    tmp = out + ".tmp"
+    _run([HIPCC] + HIP_FLAGS + ["-I" + INC, "-I" + os.path.join(PKG_DIR, "csrc")] + srcs + ["-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_host():
+    out = os.path.join(PKG_DIR, "libafhost.so")
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+          os.path.join(PKG_DIR, "host", "afhost.cpp"), "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_oracle():
+    _run(["make", "-B", "-C", os.path.join(REPO_DIR, "oracle")])
+    return os.path.join(REPO_DIR, "oracle", "liboracle_afcpu.so")
+
+
+def build_all():
+    build_host()
+    build_oracle()
+    build_gpu()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1:] or ["all"]
+    for w in what:
+        {"all": build_all, "gpu": build_gpu, "host": build_host, "oracle": build_oracle}[w]()

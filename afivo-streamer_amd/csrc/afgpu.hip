@@ -1,0 +1,1091 @@
+// afgpu: MI355X (gfx950) implementation of afivo-streamer's per-time-step
+// hot path behind the C-ABI of include/afgpu.h.
+//
+// Host side of the library: box pools in HBM, per-level box lists, and the
+// call sequences of the reference routines (mg_fas_vcycle, mg_fas_fmg,
+// field_compute, forward_euler, photoi_set_src and the time loop).  All
+// work is enqueued on one HIP stream per handle; the host reads back only
+// the scalars the reference's control flow branches on (max |residual|,
+// time-step limits).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "afgpu.h"
+#include "afgpu_fluid.h"
+
+using namespace afg;
+
+namespace {
+
+thread_local std::string g_err;
+
+void set_err(const std::string &s) { g_err = s; }
+
+#define HIPCHK(expr)                                                               \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      set_err(std::string(#expr) + ": " + hipGetErrorString(e_));                  \
+      return AFGPU_ERR_HIP;                                                        \
+    }                                                                              \
+  } while (0)
+
+#define RCHK(expr)              \
+  do {                          \
+    int r_ = (expr);            \
+    if (r_ != AFGPU_OK) return r_; \
+  } while (0)
+
+struct List {
+  int off = 0, n = 0;
+};
+
+// Timing of one kernel class: event pairs recorded around its launches.
+struct KTimer {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  double algo_bytes = 0;
+  long long launches = 0;
+};
+
+}  // namespace
+
+struct afgpu_ctx;
+
+struct ImplBase {
+  virtual ~ImplBase() {}
+  virtual int gc_lvl(int lvl, int iv, bool corners) = 0;
+  virtual int vcycle(int op, bool set_residual, int max_lvl) = 0;
+  virtual int fmg(int op, bool set_residual, bool have_guess) = 0;
+  virtual int maxabs(int iv, double *out) = 0;
+  virtual int field_compute(int s_in, bool have_guess, int *n, double *res) = 0;
+  virtual int field_from_potential() = 0;
+  virtual int forward_euler(double dt, int s_deriv, int n_prev, const int *s_prev, const double *w_prev,
+                            int s_out, int i_step, int n_steps, double *dt_lim) = 0;
+  virtual int photoi() = 0;
+  virtual int copy_states(bool backup) = 0;
+  virtual int restrict_tree(int iv) = 0;
+};
+
+struct afgpu_ctx {
+  int D = 2, nc = 8, coord = AFGPU_XYZ, precision = AFGPU_PREC_FP64;
+  size_t ncell = 0, nface = 0;  // nface: all D blocks of one box
+  hipStream_t stream = nullptr;
+  // tree
+  int n_boxes = 0, cap_boxes = 0, highest_lvl = 0;
+  std::vector<DBox> hboxes;
+  DBox *d_boxes = nullptr;
+  std::vector<int> h_lists;
+  int *d_lists = nullptr;
+  std::vector<List> l_ids, l_leaves, l_parents;
+  List l_all_leaves, l_all_ids, l_refb;
+  // pools
+  double *cc[AFGPU_N_CC] = {nullptr};
+  double *fc[AFGPU_N_FC] = {nullptr};
+  float *cc32[3] = {nullptr};  // mixed precision: correction, residual, scratch
+  // tables
+  double *d_td = nullptr, *d_chem = nullptr;
+  DTable td{}, chem{};
+  bool have_tables = false;
+  // settings
+  afgpu_params_t p{};
+  bool have_params = false;
+  double voltage = 0;
+  struct Op {
+    double lambda2 = 0;
+    bool ok = false;
+    double *d_vinv = nullptr, *d_v = nullptr, *d_eig = nullptr;
+  } ops[1 + AFGPU_MAX_MODES];
+  // reductions
+  unsigned long long *d_red = nullptr, *h_red = nullptr;
+  // time loop
+  int it = 0;
+  double time = 0, global_time = 0, global_dt = 0, frac_rejected = 0, dt = 0;
+  bool started = false;
+  // timing
+  bool timing = false;
+  KTimer timers[3];
+  std::unique_ptr<ImplBase> impl;
+
+  int n_states() const { return p.time_integrator == AFGPU_HEUNS_METHOD ? 2 : 1; }
+  const int *list(const List &l) const { return d_lists + l.off; }
+};
+
+namespace {
+
+VarBc var_methods(const afgpu_ctx &c, int iv, bool homogeneous = false) {
+  VarBc m{};
+  m.limiter = (c.D < 3) ? LIM_MC : LIM_GMINMOD43;
+  for (int nb = 0; nb < 6; nb++) {
+    m.type[nb] = BC_NEUMANN;
+    m.val[nb] = 0.0;
+  }
+  const int last = c.D - 1;
+  if (iv == AFGPU_CC_PHI) {
+    m.rb = RB_MG;
+    m.type[2 * last] = BC_DIRICHLET;
+    m.type[2 * last + 1] = BC_DIRICHLET;
+    m.val[2 * last + 1] = homogeneous ? 0.0 : c.voltage;
+  } else if (iv >= AFGPU_CC_HELMH && iv < AFGPU_CC_HELMH + AFGPU_MAX_MODES) {
+    m.rb = RB_MG;
+    m.type[2 * last] = BC_DIRICHLET;
+    m.type[2 * last + 1] = BC_DIRICHLET;
+  } else if (iv == AFGPU_CC_EFIELD) {
+    m.rb = RB_INTERP;
+  } else if (iv == AFGPU_CC_PHOTO) {
+    m.rb = RB_INTERP;
+    m.type[2 * last] = BC_DIRICHLET;
+    m.type[2 * last + 1] = BC_DIRICHLET;
+  } else if (iv >= 0 && iv < 9) {
+    m.rb = RB_INTERP_LIM;
+    if (c.p.species_bc == AFGPU_SPECIES_BC_DIRICHLET_ZERO) {
+      m.type[2 * last] = BC_DIRICHLET_COPY;
+      m.type[2 * last + 1] = BC_DIRICHLET_COPY;
+    }
+  } else {
+    m.rb = RB_NONE;
+  }
+  return m;
+}
+
+int op_phi(int op) { return op == AFGPU_OP_POISSON ? AFGPU_CC_PHI : AFGPU_CC_HELMH + (op - AFGPU_OP_HELMH); }
+
+int timer_start(afgpu_ctx &c, int kind, double bytes) {
+  if (!c.timing) return AFGPU_OK;
+  KTimer &t = c.timers[kind];
+  while (t.pool.size() < t.used + 2) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    t.pool.push_back(e);
+  }
+  hipEvent_t a = t.pool[t.used], b = t.pool[t.used + 1];
+  t.used += 2;
+  t.ev.push_back({a, b});
+  t.algo_bytes += bytes;
+  t.launches++;
+  HIPCHK(hipEventRecord(a, c.stream));
+  return AFGPU_OK;
+}
+int timer_stop(afgpu_ctx &c, int kind) {
+  if (!c.timing) return AFGPU_OK;
+  HIPCHK(hipEventRecord(c.timers[kind].ev.back().second, c.stream));
+  return AFGPU_OK;
+}
+
+template <int D, int NC, int BS>
+struct Impl : ImplBase {
+  using G = Geo<D, NC>;
+  afgpu_ctx &c;
+  explicit Impl(afgpu_ctx &cc_) : c(cc_) {}
+
+  template <typename K, typename... Args>
+  int launch(K kernel, int nblocks, Args... args) {
+    if (nblocks <= 0) return AFGPU_OK;
+    hipLaunchKernelGGL(kernel, dim3(nblocks), dim3(BS), 0, c.stream, args...);
+    HIPCHK(hipGetLastError());
+    return AFGPU_OK;
+  }
+
+  int boxop(const List &l, const double *a, double *b, int op, double s = 0.0) {
+    return launch(k_boxop<G::NCELL, BS, double>, l.n, c.list(l), a, b, op, s);
+  }
+
+  int gc_lvl(int lvl, int iv, bool corners) override {
+    const List &l = c.l_ids[lvl];
+    return launch(k_gc<D, NC, BS, double>, l.n, c.list(l), (const DBox *)c.d_boxes, c.cc[iv], var_methods(c, iv),
+                  (int)corners);
+  }
+  int gc_tree(int iv) {
+    for (int lvl = 1; lvl <= c.highest_lvl; lvl++) RCHK(gc_lvl(lvl, iv, true));
+    return AFGPU_OK;
+  }
+
+  // --- multigrid (afivo/src/m_af_multigrid.f90) ---------------------------
+  int gsrb_boxes(int lvl, int op, bool down) {
+    const int n_cycle = down ? c.p.mg_n_cycle_down : c.p.mg_n_cycle_up;
+    const List &l = c.l_ids[lvl];
+    const int iphi = op_phi(op);
+    const double bytes = 12.0 * l.n * G::NINT;
+    for (int n = 1; n <= 2 * n_cycle; n++) {
+      RCHK(timer_start(c, 0, bytes));
+      RCHK(launch(k_gsrb<D, NC, BS, double>, l.n, c.list(l), (const DBox *)c.d_boxes, c.cc[iphi],
+                  (const double *)c.cc[AFGPU_CC_RHS], n, c.ops[op].lambda2, (int)(c.coord == AFGPU_CYL)));
+      RCHK(timer_stop(c, 0));
+      RCHK(gc_lvl(lvl, iphi, !down && n == 2 * n_cycle));
+    }
+    return AFGPU_OK;
+  }
+
+  int update_coarse(int lvl, int op, bool v_cycle) {
+    const int iphi = op_phi(op);
+    const int cyl = c.coord == AFGPU_CYL;
+    if (!v_cycle && lvl == c.highest_lvl) RCHK(gc_lvl(lvl, iphi, true));
+    const List &l = c.l_ids[lvl];
+    RCHK(launch(k_restrict_residual<D, NC, BS, double>, l.n, c.list(l), (const DBox *)c.d_boxes, c.cc[iphi],
+                (const double *)c.cc[AFGPU_CC_RHS], c.cc[AFGPU_CC_TMP], c.ops[op].lambda2, cyl, (int)!v_cycle));
+    RCHK(gc_lvl(lvl - 1, iphi, true));
+    const List &pl = c.l_parents[lvl - 1];
+    return launch(k_coarse_rhs<D, NC, BS, double>, pl.n, c.list(pl), (const DBox *)c.d_boxes,
+                  (const double *)c.cc[iphi], c.cc[AFGPU_CC_RHS], c.cc[AFGPU_CC_TMP], c.ops[op].lambda2, cyl,
+                  (int)v_cycle);
+  }
+
+  int correct_children(int lvl_p, int op) {
+    const int iphi = op_phi(op);
+    RCHK(boxop(c.l_parents[lvl_p], c.cc[iphi], c.cc[AFGPU_CC_TMP], OP_SUB_FROM));
+    const List &l = c.l_ids[lvl_p + 1];
+    return launch(k_prolong_add<D, NC, BS, double>, l.n, c.list(l), (const DBox *)c.d_boxes,
+                  (const double *)c.cc[AFGPU_CC_TMP], c.cc[iphi]);
+  }
+
+  int ensure_factors(int op) {
+    afgpu_ctx::Op &o = c.ops[op];
+    if (o.ok) return AFGPU_OK;
+    const DBox &b1 = c.hboxes[c.h_lists[c.l_ids[1].off] - 1];
+    int bcd[6] = {0, 0, 0, 0, 0, 0};
+    bcd[2 * (D - 1)] = 1;
+    bcd[2 * (D - 1) + 1] = 1;
+    const size_t nn = (size_t)D * NC * NC;
+    std::vector<double> vinv(nn), v(nn), eig((size_t)D * NC);
+    RCHK(afgpu_coarse_factors(D, NC, c.coord, b1.r_min, b1.dr, bcd, vinv.data(), v.data(), eig.data()));
+    if (!o.d_vinv) {
+      HIPCHK(hipMalloc(&o.d_vinv, nn * sizeof(double)));
+      HIPCHK(hipMalloc(&o.d_v, nn * sizeof(double)));
+      HIPCHK(hipMalloc(&o.d_eig, (size_t)D * NC * sizeof(double)));
+    }
+    HIPCHK(hipMemcpy(o.d_vinv, vinv.data(), nn * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(o.d_v, v.data(), nn * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(o.d_eig, eig.data(), (size_t)D * NC * sizeof(double), hipMemcpyHostToDevice));
+    o.ok = true;
+    return AFGPU_OK;
+  }
+
+  int solve_coarse(int op) {
+    RCHK(ensure_factors(op));
+    const int iphi = op_phi(op);
+    const int id = c.h_lists[c.l_ids[1].off];
+    hipLaunchKernelGGL((k_coarse_solve<D, NC, BS>), dim3(1), dim3(BS), 0, c.stream, id, (const DBox *)c.d_boxes,
+                       (const double *)c.cc[AFGPU_CC_RHS], c.cc[iphi], var_methods(c, iphi), c.ops[op].lambda2,
+                       (int)(c.coord == AFGPU_CYL), (const double *)c.ops[op].d_vinv,
+                       (const double *)c.ops[op].d_v, (const double *)c.ops[op].d_eig);
+    HIPCHK(hipGetLastError());
+    return gc_lvl(1, iphi, true);
+  }
+
+  int vcycle(int op, bool set_residual, int max_lvl) override {
+    for (int lvl = max_lvl; lvl >= 2; lvl--) {
+      RCHK(gsrb_boxes(lvl, op, true));
+      RCHK(update_coarse(lvl, op, true));
+    }
+    RCHK(solve_coarse(op));
+    for (int lvl = 2; lvl <= max_lvl; lvl++) {
+      RCHK(correct_children(lvl - 1, op));
+      RCHK(gc_lvl(lvl, op_phi(op), true));
+      RCHK(gsrb_boxes(lvl, op, false));
+    }
+    if (set_residual) {
+      const int cyl = c.coord == AFGPU_CYL;
+      for (int lvl = 1; lvl <= max_lvl; lvl++) {
+        const List &l = c.l_ids[lvl];
+        RCHK(launch(k_residual<D, NC, BS, double>, l.n, c.list(l), (const DBox *)c.d_boxes,
+                    (const double *)c.cc[op_phi(op)], (const double *)c.cc[AFGPU_CC_RHS], c.cc[AFGPU_CC_TMP],
+                    c.ops[op].lambda2, cyl));
+      }
+    }
+    return AFGPU_OK;
+  }
+
+  int fmg(int op, bool set_residual, bool have_guess) override {
+    const int iphi = op_phi(op);
+    if (have_guess) {
+      for (int lvl = c.highest_lvl; lvl >= 2; lvl--) RCHK(update_coarse(lvl, op, false));
+    } else {
+      for (int lvl = c.highest_lvl; lvl >= 2; lvl--) {  // init_phi_rhs
+        const List &l = c.l_ids[lvl];
+        RCHK(boxop(l, c.cc[iphi], c.cc[iphi], OP_CLEAR));
+        RCHK(launch(k_restrict<D, NC, BS, double>, l.n, c.list(l), (const DBox *)c.d_boxes,
+                    (const double *)c.cc[AFGPU_CC_RHS], c.cc[AFGPU_CC_RHS], 1, (int)(c.coord == AFGPU_CYL)));
+      }
+    }
+    RCHK(boxop(c.l_ids[1], c.cc[iphi], c.cc[AFGPU_CC_TMP], OP_COPY));
+    RCHK(vcycle(op, set_residual && c.highest_lvl == 1, 1));
+    for (int lvl = 2; lvl <= c.highest_lvl; lvl++) {
+      RCHK(boxop(c.l_ids[lvl], c.cc[iphi], c.cc[AFGPU_CC_TMP], OP_COPY));
+      RCHK(correct_children(lvl - 1, op));
+      RCHK(gc_lvl(lvl, iphi, true));
+      RCHK(vcycle(op, set_residual && lvl == c.highest_lvl, lvl));
+    }
+    return AFGPU_OK;
+  }
+
+  int read_red(int n) {
+    HIPCHK(hipMemcpyAsync(c.h_red, c.d_red, n * sizeof(unsigned long long), hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    return AFGPU_OK;
+  }
+  static double as_double(unsigned long long u) {
+    double d;
+    std::memcpy(&d, &u, sizeof(d));
+    return d;
+  }
+
+  int maxabs(int iv, double *out) override {
+    HIPCHK(hipMemsetAsync(c.d_red, 0, sizeof(unsigned long long), c.stream));
+    RCHK(launch(k_maxabs<D, NC, BS>, c.l_all_leaves.n, c.list(c.l_all_leaves), (const double *)c.cc[iv], c.d_red));
+    RCHK(read_red(1));
+    *out = c.l_all_leaves.n > 0 ? as_double(c.h_red[0]) : -std::numeric_limits<double>::max() / 10;
+    return AFGPU_OK;
+  }
+
+  // --- field (src/m_field.f90) ----------------------------------------------
+  int field_from_potential() override {
+    RCHK(launch(k_gradient<D, NC, BS>, c.l_all_ids.n, c.list(c.l_all_ids), (const DBox *)c.d_boxes,
+                (const double *)c.cc[AFGPU_CC_PHI], c.fc[AFGPU_FC_FIELD], c.cc[AFGPU_CC_EFIELD]));
+    return gc_tree(AFGPU_CC_EFIELD);
+  }
+
+  double min_dr() const {
+    const DBox &b = c.hboxes[c.h_lists[c.l_ids[c.highest_lvl].off] - 1];
+    double m = b.dr[0];
+    for (int d = 1; d < D; d++) m = std::min(m, b.dr[d]);
+    return m;
+  }
+
+  int field_compute(int s_in, bool have_guess, int *n_out, double *res_out) override {
+    RCHK(launch(k_set_rhs<G::NCELL, BS>, c.l_all_leaves.n, c.list(c.l_all_leaves),
+                (const double *)c.cc[AFGPU_CC_ELECTRON + s_in], (const double *)c.cc[AFGPU_CC_POS_ION + s_in],
+                (const double *)c.cc[AFGPU_CC_NEG_ION + s_in], c.cc[AFGPU_CC_RHS]));
+    double max_rhs;
+    RCHK(maxabs(AFGPU_CC_RHS, &max_rhs));
+    const double conv_fac = 1e-10, min_residual = 1e-6, max_residual = 1e8;
+    const double threshold = std::max(std::max(min_residual, max_rhs * c.p.mg_max_rel_residual),
+                                      conv_fac * std::fabs(c.voltage) / (c.p.domain_len_last * min_dr()));
+    int count = 0;
+    double last = 0;
+    if (!have_guess) {
+      std::vector<double> res(101, 0.0);
+      int i;
+      for (i = 1; i <= 100; i++) {
+        RCHK(fmg(AFGPU_OP_POISSON, true, true));
+        RCHK(maxabs(AFGPU_CC_TMP, &res[i]));
+        count++;
+        last = res[i];
+        if (res[i] < threshold) break;
+        if (i > 2) {
+          double mn = std::min(std::min(res[i - 2], res[i - 1]), res[i]);
+          double mx = std::max(std::max(res[i - 2], res[i - 1]), res[i]);
+          double ratio = mn / mx;
+          if (ratio < 2.0 && ratio > 0.5 && res[i] < max_residual) break;
+        }
+      }
+      if (i == 101) {
+        set_err("No convergence in initial field computation");
+        return AFGPU_ERR_NO_CONVERGENCE;
+      }
+    }
+    if (c.precision == AFGPU_PREC_MIXED) {
+      RCHK(mixed_vcycles(threshold, &count, &last));
+    } else {
+      for (int i = 1; i <= c.p.mg_num_vcycles; i++) {
+        RCHK(vcycle(AFGPU_OP_POISSON, true, c.highest_lvl));
+        RCHK(maxabs(AFGPU_CC_TMP, &last));
+        count++;
+        if (last < threshold) break;
+      }
+    }
+    RCHK(field_from_potential());
+    if (n_out) *n_out = count;
+    if (res_out) *res_out = last;
+    return AFGPU_OK;
+  }
+
+  // Mixed precision: FP64 defect correction around FP32 V-cycles.
+  int mixed_vcycles(double threshold, int *count, double *last) {
+    set_err("mixed precision is not built in this library version");
+    (void)threshold; (void)count; (void)last;
+    return AFGPU_ERR_UNSUPPORTED;
+  }
+
+  // --- fluid (src/m_fluid.f90, afivo/src/m_af_flux_schemes.f90) -------------
+  int forward_euler(double dt, int s_deriv, int n_prev, const int *s_prev, const double *w_prev, int s_out,
+                    int i_step, int n_steps, double *dt_lim) override {
+    const bool last_step = (i_step == n_steps);
+    if (i_step > 1) RCHK(field_compute(s_deriv, true, nullptr, nullptr));
+    const int cyl = c.coord == AFGPU_CYL;
+    const int iv_e = AFGPU_CC_ELECTRON + s_deriv;
+    // af_restrict_ref_boundary (afivo/src/m_af_restrict.f90:140-161)
+    RCHK(launch(k_restrict<D, NC, BS, double>, c.l_refb.n, c.list(c.l_refb), (const DBox *)c.d_boxes,
+                (const double *)c.cc[iv_e], c.cc[iv_e], 1, cyl));
+    HIPCHK(hipMemsetAsync(c.d_red + 1, 0, 2 * sizeof(unsigned long long), c.stream));
+    const double big = 1e100;
+    unsigned long long bigbits;
+    std::memcpy(&bigbits, &big, sizeof(big));
+    c.h_red[3] = bigbits;
+    HIPCHK(hipMemcpyAsync(c.d_red + 3, c.h_red + 3, sizeof(unsigned long long), hipMemcpyHostToDevice, c.stream));
+    const double N_inv = 1 / c.p.gas_number_density;
+    const VarBc msp = var_methods(c, iv_e);
+    for (int lvl = 1; lvl <= c.highest_lvl; lvl++) {
+      const List &l = c.l_leaves[lvl];
+      RCHK(timer_start(c, 1, (2.0 + 2 * D) * 8.0 * l.n * G::NINT));
+      RCHK(launch(k_flux<D, NC, BS>, l.n, c.list(l), (const DBox *)c.d_boxes, c.cc[iv_e],
+                  (const double *)c.cc[AFGPU_CC_EFIELD], (const double *)c.fc[AFGPU_FC_FIELD], c.fc[AFGPU_FC_FLUX],
+                  msp, c.td, N_inv, c.d_red + 1, c.d_red + 2));
+      RCHK(timer_stop(c, 1));
+    }
+    for (int lvl = 1; lvl <= c.highest_lvl - 1; lvl++) {
+      const List &l = c.l_parents[lvl];
+      RCHK(launch(k_consistent_fluxes<D, NC, BS>, l.n, c.list(l), (const DBox *)c.d_boxes, c.fc[AFGPU_FC_FLUX],
+                  cyl));
+    }
+    UpdateArgs A{};
+    for (int sp = 0; sp < 3; sp++)
+      for (int s = 0; s < 3; s++) A.sp[sp][s] = c.cc[AFGPU_CC_SPECIES(sp) + s];
+    A.efield = c.cc[AFGPU_CC_EFIELD];
+    A.photo = c.cc[AFGPU_CC_PHOTO];
+    A.flux = c.fc[AFGPU_FC_FLUX];
+    A.chem = c.chem;
+    A.dt = dt;
+    A.N_inv = N_inv;
+    A.s_deriv = s_deriv;
+    A.s_out = s_out;
+    A.n_prev = n_prev;
+    for (int m = 0; m < n_prev; m++) {
+      A.s_prev[m] = s_prev[m];
+      A.w_prev[m] = w_prev[m];
+    }
+    A.last_step = last_step;
+    A.chem_nmin_on = c.p.dt_chemistry_nmin > 0;
+    A.chem_nmin = c.p.dt_chemistry_nmin;
+    A.chem_limit_loss = c.p.dt_chemistry_limit_loss;
+    A.photoi_enabled = c.p.photoi_enabled;
+    A.photoi_species = c.p.photoi_species;
+    A.cyl = cyl;
+    RCHK(timer_start(c, 2, (7.0 + D + (c.p.photoi_enabled ? 1 : 0)) * 8.0 * c.l_all_leaves.n * G::NINT));
+    RCHK(launch(k_update<D, NC, BS>, c.l_all_leaves.n, c.list(c.l_all_leaves), (const DBox *)c.d_boxes, A,
+                c.d_red + 3));
+    RCHK(timer_stop(c, 2));
+    RCHK(read_red(4));
+    double lim[4];
+    const double cfl_max = as_double(c.h_red[1]);
+    const double sig_max = as_double(c.h_red[2]);
+    lim[0] = c.l_all_leaves.n > 0 ? 1 / cfl_max : 1e100;
+    lim[1] = c.l_all_leaves.n > 0 ? UC_eps0 / (UC_elem_charge * std::max(sig_max, 1e-100)) : 1e100;
+    lim[2] = last_step ? as_double(c.h_red[3]) : 1e100;
+    lim[3] = 1e100;
+    lim[0] = lim[0] * c.p.dt_cfl_number;
+    const double mv = std::min(std::min(std::min(lim[0], lim[1]), lim[2]), lim[3]);
+    *dt_lim = std::min(c.p.dt_max, mv);
+    return AFGPU_OK;
+  }
+
+  // --- photoionization (src/m_photoi.f90, src/m_photoi_helmh.f90) -----------
+  int photoi() override {
+    RCHK(launch(k_photoi_src<D, NC, BS>, c.l_all_leaves.n, c.list(c.l_all_leaves),
+                (const double *)c.cc[AFGPU_CC_EFIELD], (const double *)c.cc[AFGPU_CC_ELECTRON], c.cc[AFGPU_CC_RHS],
+                c.td, c.p.gas_number_density, c.p.photoi_eta_quench));
+    RCHK(boxop(c.l_all_ids, c.cc[AFGPU_CC_PHOTO], c.cc[AFGPU_CC_PHOTO], OP_CLEAR));
+    double max_rhs;
+    RCHK(maxabs(AFGPU_CC_RHS, &max_rhs));
+    max_rhs = std::max(max_rhs, std::sqrt(std::numeric_limits<double>::epsilon()));
+    for (int n = 0; n < c.p.photoi_n_modes; n++) {
+      const int op = AFGPU_OP_HELMH + n;
+      for (int i = 1; i <= c.p.photoi_max_fmg_cycles; i++) {
+        RCHK(fmg(op, true, true));
+        double residu;
+        RCHK(maxabs(AFGPU_CC_TMP, &residu));
+        if (residu / max_rhs < c.p.photoi_max_rel_residual) break;
+      }
+      RCHK(boxop(c.l_all_leaves, c.cc[op_phi(op)], c.cc[AFGPU_CC_PHOTO], OP_AXPY_NEG, c.p.photoi_coeff[n]));
+    }
+    return AFGPU_OK;
+  }
+
+  int copy_states(bool backup) override {
+    const int ns = c.n_states();
+    for (int sp = 0; sp < 3; sp++) {
+      const int iv = AFGPU_CC_SPECIES(sp);
+      if (backup)
+        RCHK(boxop(c.l_all_ids, c.cc[iv], c.cc[iv + ns], OP_COPY));
+      else
+        RCHK(boxop(c.l_all_ids, c.cc[iv + ns], c.cc[iv], OP_COPY));
+    }
+    if (backup)
+      RCHK(boxop(c.l_all_ids, c.cc[AFGPU_CC_PHI], c.cc[AFGPU_CC_PHI_2], OP_COPY));
+    else {
+      RCHK(boxop(c.l_all_ids, c.cc[AFGPU_CC_PHI_2], c.cc[AFGPU_CC_PHI], OP_COPY));
+      RCHK(field_from_potential());
+    }
+    return AFGPU_OK;
+  }
+
+  int restrict_tree(int iv) override {
+    for (int lvl = c.highest_lvl; lvl >= 2; lvl--) {
+      const List &l = c.l_ids[lvl];
+      RCHK(launch(k_restrict<D, NC, BS, double>, l.n, c.list(l), (const DBox *)c.d_boxes, (const double *)c.cc[iv],
+                  c.cc[iv], 1, (int)(c.coord == AFGPU_CYL)));
+    }
+    return AFGPU_OK;
+  }
+};
+
+int make_impl(afgpu_ctx &c) {
+  if (c.D == 2 && c.nc == 8)
+    c.impl.reset(new Impl<2, 8, 64>(c));
+  else if (c.D == 2 && c.nc == 16)
+    c.impl.reset(new Impl<2, 16, 256>(c));
+  else if (c.D == 3 && c.nc == 8)
+    c.impl.reset(new Impl<3, 8, 256>(c));
+  else if (c.D == 3 && c.nc == 16)
+    c.impl.reset(new Impl<3, 16, 256>(c));
+  else {
+    set_err("box size must be 8 or 16");
+    return AFGPU_ERR_UNSUPPORTED;
+  }
+  return AFGPU_OK;
+}
+
+int ready(afgpu_t *h, bool need_tables = false) {
+  if (!h) {
+    set_err("null handle");
+    return AFGPU_ERR_ARG;
+  }
+  if (h->highest_lvl < 1 || !h->have_params) {
+    set_err("tree or parameters not set");
+    return AFGPU_ERR_STATE;
+  }
+  if (need_tables && !h->have_tables) {
+    set_err("lookup tables not set");
+    return AFGPU_ERR_STATE;
+  }
+  return AFGPU_OK;
+}
+
+int sync_stream(afgpu_t *h) {
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return AFGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *afgpu_last_error(void) { return g_err.c_str(); }
+
+int afgpu_create(int ndim, int n_cell, int coord, int precision, afgpu_t **out) {
+  if (!out || (ndim != 2 && ndim != 3) || (coord != AFGPU_XYZ && coord != AFGPU_CYL) ||
+      (coord == AFGPU_CYL && ndim != 2) || (precision != AFGPU_PREC_FP64 && precision != AFGPU_PREC_MIXED)) {
+    set_err("afgpu_create: invalid arguments");
+    return AFGPU_ERR_ARG;
+  }
+  std::unique_ptr<afgpu_t> h(new afgpu_t());
+  h->D = ndim;
+  h->nc = n_cell;
+  h->coord = coord;
+  h->precision = precision;
+  h->ncell = 1;
+  size_t nf = 1;
+  for (int d = 0; d < ndim; d++) {
+    h->ncell *= (n_cell + 2);
+    nf *= (n_cell + 1);
+  }
+  h->nface = nf * ndim;
+  RCHK(make_impl(*h));
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (ndev < 1) {
+    set_err("no GPU device");
+    return AFGPU_ERR_HIP;
+  }
+  HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  HIPCHK(hipMalloc(&h->d_red, 8 * sizeof(unsigned long long)));
+  HIPCHK(hipHostMalloc(&h->h_red, 8 * sizeof(unsigned long long), hipHostMallocDefault));
+  *out = h.release();
+  return AFGPU_OK;
+}
+
+int afgpu_destroy(afgpu_t *h) {
+  if (!h) return AFGPU_OK;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  for (int i = 0; i < AFGPU_N_CC; i++) hipFree(h->cc[i]);
+  for (int i = 0; i < AFGPU_N_FC; i++) hipFree(h->fc[i]);
+  for (int i = 0; i < 3; i++) hipFree(h->cc32[i]);
+  hipFree(h->d_boxes);
+  hipFree(h->d_lists);
+  hipFree(h->d_td);
+  hipFree(h->d_chem);
+  hipFree(h->d_red);
+  if (h->h_red) hipHostFree(h->h_red);
+  for (auto &o : h->ops) {
+    hipFree(o.d_vinv);
+    hipFree(o.d_v);
+    hipFree(o.d_eig);
+  }
+  for (auto &t : h->timers)
+    for (auto e : t.pool) hipEventDestroy(e);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+  return AFGPU_OK;
+}
+
+int afgpu_sync_tree(afgpu_t *h, int n_boxes, const int *lvl, const int *ix, const int *parent, const int *children,
+                    const int *neighbors, const int *neighbor_mat, const double *r_min, const double *dr) {
+  if (!h || n_boxes < 1 || !lvl || !ix || !parent || !children || !neighbors || !neighbor_mat || !r_min || !dr) {
+    set_err("afgpu_sync_tree: invalid arguments");
+    return AFGPU_ERR_ARG;
+  }
+  const int D = h->D, nch = 1 << D, nnb = 2 * D, nm = (D == 3) ? 27 : 9;
+  std::vector<DBox> hb(n_boxes);
+  int hl = 0;
+  for (int b = 0; b < n_boxes; b++) {
+    DBox &x = hb[b];
+    std::memset(&x, 0, sizeof(x));
+    x.lvl = lvl[b];
+    x.parent = parent[b];
+    for (int d = 0; d < D; d++) {
+      x.ix[d] = ix[b * D + d];
+      x.r_min[d] = r_min[b * D + d];
+      x.dr[d] = dr[b * D + d];
+    }
+    for (int q = 0; q < nch; q++) x.children[q] = children[b * nch + q];
+    for (int q = 0; q < nnb; q++) x.nb[q] = neighbors[b * nnb + q];
+    for (int q = 0; q < nm; q++) x.nbmat[q] = neighbor_mat[b * nm + q];
+    // validate references
+    auto bad = [&](int v) { return v > n_boxes; };
+    bool err = bad(x.parent);
+    for (int q = 0; q < nch; q++) err |= bad(x.children[q]) || x.children[q] < 0;
+    for (int q = 0; q < nnb; q++) err |= bad(x.nb[q]);
+    for (int q = 0; q < nm; q++) err |= bad(x.nbmat[q]);
+    if (err || x.lvl < 0) {
+      set_err("afgpu_sync_tree: box reference out of range");
+      return AFGPU_ERR_ARG;
+    }
+    hl = std::max(hl, x.lvl);
+  }
+  // Level lists (lvl_t ids / leaves / parents) and derived lists.
+  std::vector<std::vector<int>> ids(hl + 1), leaves(hl + 1), parents(hl + 1);
+  std::vector<int> refb;
+  for (int b = 0; b < n_boxes; b++) {
+    const DBox &x = hb[b];
+    if (x.lvl <= 0) continue;
+    ids[x.lvl].push_back(b + 1);
+    if (x.children[0] > 0)
+      parents[x.lvl].push_back(b + 1);
+    else
+      leaves[x.lvl].push_back(b + 1);
+  }
+  for (int l = 1; l <= hl; l++)
+    for (int id : leaves[l]) {
+      const DBox &x = hb[id - 1];
+      bool rb = false;
+      for (int q = 0; q < nnb; q++) rb |= (x.nb[q] == 0);
+      if (x.parent > 0 && rb) refb.push_back(id);
+    }
+  if (hl < 1 || ids[1].size() != 1) {
+    set_err("level 1 must consist of exactly one box (coarse_grid_size = box_size)");
+    return AFGPU_ERR_UNSUPPORTED;
+  }
+  // Structural checks that every kernel relies on: a parent has all 2^D
+  // children (and they point back), level-1 faces are physical, and every
+  // refinement-boundary face has the parent's neighbour.
+  for (int b = 0; b < n_boxes; b++) {
+    const DBox &x = hb[b];
+    if (x.lvl <= 0) continue;
+    if (x.children[0] > 0)
+      for (int q = 0; q < nch; q++) {
+        const int cid = x.children[q];
+        if (cid < 1 || hb[cid - 1].parent != b + 1 || hb[cid - 1].lvl != x.lvl + 1) {
+          set_err("afgpu_sync_tree: inconsistent children");
+          return AFGPU_ERR_ARG;
+        }
+      }
+    for (int q = 0; q < nnb; q++) {
+      const int nid = x.nb[q];
+      if (nid > 0 && hb[nid - 1].lvl != x.lvl) {
+        set_err("afgpu_sync_tree: neighbour on another level");
+        return AFGPU_ERR_ARG;
+      }
+      if (x.lvl == 1 && nid == 0) {
+        set_err("afgpu_sync_tree: level-1 box without neighbour or boundary");
+        return AFGPU_ERR_ARG;
+      }
+    }
+    for (int q = 0; q < nm; q++) {
+      const int nid = x.nbmat[q];
+      if (nid > 0 && hb[nid - 1].lvl != x.lvl) {
+        set_err("afgpu_sync_tree: diagonal neighbour on another level");
+        return AFGPU_ERR_ARG;
+      }
+    }
+  }
+  for (int l = 2; l <= hl; l++)
+    for (int id : ids[l]) {
+      const DBox &x = hb[id - 1];
+      if (x.parent < 1) {
+        set_err("box above level 1 without parent");
+        return AFGPU_ERR_ARG;
+      }
+      for (int q = 0; q < nnb; q++)
+        if (x.nb[q] == 0 && hb[x.parent - 1].nb[q] <= 0) {
+          set_err("tree violates 2:1 balance at a refinement boundary");
+          return AFGPU_ERR_ARG;
+        }
+    }
+  std::vector<int> all;
+  auto add = [&](const std::vector<int> &v) {
+    List l;
+    l.off = (int)all.size();
+    l.n = (int)v.size();
+    all.insert(all.end(), v.begin(), v.end());
+    return l;
+  };
+  h->l_ids.assign(hl + 1, List());
+  h->l_leaves.assign(hl + 1, List());
+  h->l_parents.assign(hl + 1, List());
+  std::vector<int> all_leaves, all_ids;
+  for (int l = 1; l <= hl; l++) {
+    h->l_ids[l] = add(ids[l]);
+    h->l_leaves[l] = add(leaves[l]);
+    h->l_parents[l] = add(parents[l]);
+    all_leaves.insert(all_leaves.end(), leaves[l].begin(), leaves[l].end());
+    all_ids.insert(all_ids.end(), ids[l].begin(), ids[l].end());
+  }
+  h->l_all_leaves = add(all_leaves);
+  h->l_all_ids = add(all_ids);
+  h->l_refb = add(refb);
+  HIPCHK(hipStreamSynchronize(h->stream));
+  // Grow the pools, keeping the data of existing box ids.
+  if (n_boxes > h->cap_boxes) {
+    const int cap = std::max(n_boxes, h->cap_boxes + h->cap_boxes / 4);
+    auto grow = [&](double *&ptr, size_t per_box) -> int {
+      double *np = nullptr;
+      HIPCHK(hipMalloc(&np, (size_t)cap * per_box * sizeof(double)));
+      HIPCHK(hipMemsetAsync(np, 0, (size_t)cap * per_box * sizeof(double), h->stream));
+      if (ptr) {
+        HIPCHK(hipMemcpyAsync(np, ptr, (size_t)h->cap_boxes * per_box * sizeof(double), hipMemcpyDeviceToDevice,
+                              h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        HIPCHK(hipFree(ptr));
+      }
+      ptr = np;
+      return AFGPU_OK;
+    };
+    for (int i = 0; i < AFGPU_N_CC; i++) RCHK(grow(h->cc[i], h->ncell));
+    for (int i = 0; i < AFGPU_N_FC; i++) RCHK(grow(h->fc[i], h->nface));
+    if (h->precision == AFGPU_PREC_MIXED) {
+      for (int i = 0; i < 3; i++) {
+        hipFree(h->cc32[i]);
+        HIPCHK(hipMalloc(&h->cc32[i], (size_t)cap * h->ncell * sizeof(float)));
+        HIPCHK(hipMemsetAsync(h->cc32[i], 0, (size_t)cap * h->ncell * sizeof(float), h->stream));
+      }
+    }
+    h->cap_boxes = cap;
+  }
+  hipFree(h->d_boxes);
+  hipFree(h->d_lists);
+  h->d_boxes = nullptr;
+  h->d_lists = nullptr;
+  HIPCHK(hipMalloc(&h->d_boxes, (size_t)n_boxes * sizeof(DBox)));
+  HIPCHK(hipMemcpy(h->d_boxes, hb.data(), (size_t)n_boxes * sizeof(DBox), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&h->d_lists, std::max<size_t>(1, all.size()) * sizeof(int)));
+  if (!all.empty()) HIPCHK(hipMemcpy(h->d_lists, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice));
+  h->hboxes = hb;
+  h->h_lists = all;
+  h->n_boxes = n_boxes;
+  h->highest_lvl = hl;
+  for (auto &o : h->ops) o.ok = false;  // level-1 box may have changed
+  return AFGPU_OK;
+}
+
+static int xfer(afgpu_t *h, bool cc, int iv, int n_ids, const int *ids, double *data, bool put) {
+  if (!h || (cc && (iv < 0 || iv >= AFGPU_N_CC)) || (!cc && (iv < 0 || iv >= AFGPU_N_FC)) || n_ids < 0 ||
+      (n_ids > 0 && (!ids || !data))) {
+    set_err("transfer: invalid arguments");
+    return AFGPU_ERR_ARG;
+  }
+  if (n_ids == 0) return AFGPU_OK;
+  const size_t blk = cc ? h->ncell : h->nface;
+  double *pool = cc ? h->cc[iv] : h->fc[iv];
+  for (int n = 0; n < n_ids; n++)
+    if (ids[n] < 1 || ids[n] > h->n_boxes) {
+      set_err("transfer: box id out of range");
+      return AFGPU_ERR_ARG;
+    }
+  // Contiguous runs of ids move in one copy each.
+  int n = 0;
+  while (n < n_ids) {
+    int m = n + 1;
+    while (m < n_ids && ids[m] == ids[m - 1] + 1) m++;
+    double *dev = pool + (size_t)(ids[n] - 1) * blk;
+    double *host = data + (size_t)n * blk;
+    const size_t bytes = (size_t)(m - n) * blk * sizeof(double);
+    if (put)
+      HIPCHK(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, h->stream));
+    else
+      HIPCHK(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, h->stream));
+    n = m;
+  }
+  return sync_stream(h);
+}
+
+int afgpu_put_cc(afgpu_t *h, int iv, int n, const int *ids, const double *d) {
+  return xfer(h, true, iv, n, ids, const_cast<double *>(d), true);
+}
+int afgpu_get_cc(afgpu_t *h, int iv, int n, const int *ids, double *d) { return xfer(h, true, iv, n, ids, d, false); }
+int afgpu_put_fc(afgpu_t *h, int iv, int n, const int *ids, const double *d) {
+  return xfer(h, false, iv, n, ids, const_cast<double *>(d), true);
+}
+int afgpu_get_fc(afgpu_t *h, int iv, int n, const int *ids, double *d) { return xfer(h, false, iv, n, ids, d, false); }
+
+int afgpu_set_params(afgpu_t *h, const afgpu_params_t *p) {
+  if (!h || !p || p->photoi_n_modes < 0 || p->photoi_n_modes > AFGPU_MAX_MODES ||
+      (p->time_integrator != AFGPU_FORWARD_EULER && p->time_integrator != AFGPU_HEUNS_METHOD) ||
+      p->photoi_species < 0 || p->photoi_species > 2 || p->mg_n_cycle_down < 1 || p->mg_n_cycle_up < 1) {
+    set_err("afgpu_set_params: invalid arguments");
+    return AFGPU_ERR_ARG;
+  }
+  h->p = *p;
+  h->have_params = true;
+  h->ops[AFGPU_OP_POISSON].lambda2 = 0.0;
+  for (int n = 0; n < p->photoi_n_modes; n++) h->ops[AFGPU_OP_HELMH + n].lambda2 = p->photoi_lambda[n] * p->photoi_lambda[n];
+  return AFGPU_OK;
+}
+
+int afgpu_set_tables(afgpu_t *h, int n_rows, double td_x_min, double td_inv_fac, const double *td_cols,
+                     double chem_x_min, double chem_inv_fac, const double *chem_cols) {
+  if (!h || n_rows < 2 || !td_cols || !chem_cols) {
+    set_err("afgpu_set_tables: invalid arguments");
+    return AFGPU_ERR_ARG;
+  }
+  hipFree(h->d_td);
+  hipFree(h->d_chem);
+  h->d_td = h->d_chem = nullptr;
+  HIPCHK(hipMalloc(&h->d_td, 4 * (size_t)n_rows * sizeof(double)));
+  HIPCHK(hipMalloc(&h->d_chem, 2 * (size_t)n_rows * sizeof(double)));
+  HIPCHK(hipMemcpy(h->d_td, td_cols, 4 * (size_t)n_rows * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->d_chem, chem_cols, 2 * (size_t)n_rows * sizeof(double), hipMemcpyHostToDevice));
+  h->td = DTable{h->d_td, n_rows, td_x_min, td_inv_fac};
+  h->chem = DTable{h->d_chem, n_rows, chem_x_min, chem_inv_fac};
+  h->have_tables = true;
+  return AFGPU_OK;
+}
+
+int afgpu_set_voltage(afgpu_t *h, double v) {
+  if (!h) return AFGPU_ERR_ARG;
+  h->voltage = v;
+  return AFGPU_OK;
+}
+
+int afgpu_mg_fas_vcycle(afgpu_t *h, int op, int set_residual, int max_lvl) {
+  RCHK(ready(h));
+  if (op < 0 || op > AFGPU_MAX_MODES) return AFGPU_ERR_ARG;
+  if (max_lvl <= 0 || max_lvl > h->highest_lvl) max_lvl = h->highest_lvl;
+  RCHK(h->impl->vcycle(op, set_residual != 0, max_lvl));
+  return sync_stream(h);
+}
+
+int afgpu_mg_fas_fmg(afgpu_t *h, int op, int set_residual, int have_guess) {
+  RCHK(ready(h));
+  if (op < 0 || op > AFGPU_MAX_MODES) return AFGPU_ERR_ARG;
+  RCHK(h->impl->fmg(op, set_residual != 0, have_guess != 0));
+  return sync_stream(h);
+}
+
+int afgpu_tree_maxabs_cc(afgpu_t *h, int iv, double *out) {
+  RCHK(ready(h));
+  if (iv < 0 || iv >= AFGPU_N_CC || !out) return AFGPU_ERR_ARG;
+  return h->impl->maxabs(iv, out);
+}
+
+int afgpu_gc_tree(afgpu_t *h, int iv) {
+  RCHK(ready(h));
+  if (iv < 0 || iv >= AFGPU_N_CC || var_methods(*h, iv).rb == RB_NONE) return AFGPU_ERR_ARG;
+  for (int lvl = 1; lvl <= h->highest_lvl; lvl++) RCHK(h->impl->gc_lvl(lvl, iv, true));
+  return sync_stream(h);
+}
+
+int afgpu_restrict_tree(afgpu_t *h, int iv) {
+  RCHK(ready(h));
+  if (iv < 0 || iv >= AFGPU_N_CC) return AFGPU_ERR_ARG;
+  RCHK(h->impl->restrict_tree(iv));
+  return sync_stream(h);
+}
+
+int afgpu_field_compute(afgpu_t *h, int s_in, int have_guess, int *n_cycles, double *residual) {
+  RCHK(ready(h));
+  if (s_in < 0 || s_in > 2) return AFGPU_ERR_ARG;
+  RCHK(h->impl->field_compute(s_in, have_guess != 0, n_cycles, residual));
+  return sync_stream(h);
+}
+
+int afgpu_field_from_potential(afgpu_t *h) {
+  RCHK(ready(h));
+  RCHK(h->impl->field_from_potential());
+  return sync_stream(h);
+}
+
+int afgpu_forward_euler(afgpu_t *h, double dt, int s_deriv, int n_prev, const int *s_prev, const double *w_prev,
+                        int s_out, int i_step, int n_steps, double *dt_lim) {
+  RCHK(ready(h, true));
+  if (!dt_lim || n_prev < 1 || n_prev > 4 || !s_prev || !w_prev || s_deriv < 0 || s_deriv > 2 || s_out < 0 ||
+      s_out > 2)
+    return AFGPU_ERR_ARG;
+  for (int m = 0; m < n_prev; m++)
+    if (s_prev[m] < 0 || s_prev[m] > 2) return AFGPU_ERR_ARG;
+  return h->impl->forward_euler(dt, s_deriv, n_prev, s_prev, w_prev, s_out, i_step, n_steps, dt_lim);
+}
+
+static int advance(afgpu_t *h, double dt, double *dt_lim) {
+  *dt_lim = 1e100;
+  if (h->p.time_integrator == AFGPU_FORWARD_EULER) {
+    const int sp[1] = {0};
+    const double w[1] = {1.0};
+    RCHK(h->impl->forward_euler(dt, 0, 1, sp, w, 0, 1, 1, dt_lim));
+  } else {
+    const int sp1[1] = {0};
+    const double w1[1] = {1.0};
+    RCHK(h->impl->forward_euler(dt, 0, 1, sp1, w1, 1, 1, 2, dt_lim));
+    const int sp2[2] = {0, 1};
+    const double w2[2] = {0.5, 0.5};
+    RCHK(h->impl->forward_euler(0.5 * dt, 1, 2, sp2, w2, 0, 2, 2, dt_lim));
+  }
+  h->time = h->time + dt;
+  return AFGPU_OK;
+}
+
+int afgpu_advance(afgpu_t *h, double dt, double *dt_lim) {
+  RCHK(ready(h, true));
+  if (!dt_lim) return AFGPU_ERR_ARG;
+  return advance(h, dt, dt_lim);
+}
+
+int afgpu_photoi_set_src(afgpu_t *h) {
+  RCHK(ready(h, true));
+  RCHK(h->impl->photoi());
+  return sync_stream(h);
+}
+
+int afgpu_copy_current_state(afgpu_t *h) {
+  RCHK(ready(h));
+  RCHK(h->impl->copy_states(true));
+  return sync_stream(h);
+}
+
+int afgpu_restore_previous_state(afgpu_t *h) {
+  RCHK(ready(h));
+  RCHK(h->impl->copy_states(false));
+  return sync_stream(h);
+}
+
+int afgpu_step(afgpu_t *h, afgpu_step_info_t *info) {
+  RCHK(ready(h, true));
+  if (!h->started) {
+    h->global_dt = h->p.dt_min;
+    h->dt = h->global_dt;
+    h->started = true;
+  }
+  h->it = h->it + 1;
+  int photoi_done = 0;
+  if (h->p.photoi_enabled && h->p.photoi_per_steps > 0 && h->it % h->p.photoi_per_steps == 0) {
+    RCHK(h->impl->photoi());
+    photoi_done = 1;
+  }
+  double dt_lim = std::numeric_limits<double>::max();
+  double dt = h->dt;
+  const int max_attempts = 10;
+  int n, n_rej = 0;
+  for (n = 1; n <= max_attempts; n++) {
+    RCHK(h->impl->copy_states(true));
+    double dt_lim_step;
+    RCHK(advance(h, dt, &dt_lim_step));
+    dt_lim = std::min(dt_lim, dt_lim_step);
+    if (dt <= dt_lim_step) break;
+    n_rej++;
+    dt = h->p.dt_safety_factor * dt_lim_step;
+    h->time = h->global_time;
+    RCHK(h->impl->copy_states(false));
+  }
+  h->frac_rejected = 0.99 * h->frac_rejected;
+  if (n > 1) h->frac_rejected = h->frac_rejected + 0.01;
+  if (n == max_attempts + 1) {
+    set_err("All time steps were rejected");
+    return AFGPU_ERR_ALL_REJECTED;
+  }
+  const double dt_used = dt;
+  int n_cyc = 0;
+  double res = 0;
+  RCHK(h->impl->field_compute(0, true, &n_cyc, &res));
+  double tmp = h->p.dt_max_growth_factor;
+  if (h->frac_rejected > 0.1) tmp = 1.0;
+  dt = std::min(tmp * h->global_dt, h->p.dt_safety_factor * std::min(dt_lim, h->p.dt_max));
+  h->global_dt = dt;
+  h->global_time = h->time;
+  h->dt = dt;
+  if (info) {
+    info->it = h->it;
+    info->time = h->time;
+    info->dt_used = dt_used;
+    info->dt_next = dt;
+    info->dt_lim = dt_lim;
+    info->n_rejected = n_rej;
+    info->n_vcycles = n_cyc;
+    info->residual = res;
+    double mr;
+    RCHK(h->impl->maxabs(AFGPU_CC_RHS, &mr));
+    info->rel_residual = mr > 0 ? res / mr : 0;
+    info->photoi_done = photoi_done;
+    info->leaf_cells = afgpu_num_leaf_cells(h);
+  }
+  RCHK(sync_stream(h));
+  if (h->global_dt < h->p.dt_min) {
+    set_err("dt too small");
+    return AFGPU_ERR_DT_TOO_SMALL;
+  }
+  return AFGPU_OK;
+}
+
+long long afgpu_num_leaf_cells(afgpu_t *h) {
+  if (!h) return 0;
+  long long n = h->l_all_leaves.n;
+  for (int d = 0; d < h->D; d++) n *= h->nc;
+  return n;
+}
+
+int afgpu_kernel_timing(afgpu_t *h, int enable) {
+  if (!h) return AFGPU_ERR_ARG;
+  h->timing = enable != 0;
+  return AFGPU_OK;
+}
+
+int afgpu_kernel_stats(afgpu_t *h, int kind, int reset, double *total_ms, long long *launches, double *algo_bytes) {
+  if (!h || kind < 0 || kind > 2) return AFGPU_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  KTimer &t = h->timers[kind];
+  double ms = 0;
+  for (auto &pr : t.ev) {
+    float e = 0;
+    HIPCHK(hipEventElapsedTime(&e, pr.first, pr.second));
+    ms += e;
+  }
+  if (total_ms) *total_ms = ms;
+  if (launches) *launches = t.launches;
+  if (algo_bytes) *algo_bytes = t.algo_bytes;
+  if (reset) {
+    t.ev.clear();
+    t.used = 0;
+    t.algo_bytes = 0;
+    t.launches = 0;
+  }
+  return AFGPU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,150 @@
+// Shared device-side definitions of the afivo-streamer hot path for gfx950.
+//
+// Data layout in HBM: one pool per cell-centred variable,
+//   pool[(id-1) * NG^D + i + NG*(j + NG*k)],  NG = NC + 2,
+// i.e. exactly afivo's box%cc(DTIMES(0:nc+1), iv) per box
+// (afivo/src/m_af_types.f90:302), so a box can be copied to/from the Fortran
+// array without repacking.  Face variables use box%fc's layout, D blocks of
+// (NC+1)^D per box.  Topology lives in an array of DBox records.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace afg {
+
+struct DBox {
+  int lvl;
+  int parent;
+  int ix[3];
+  int children[8];
+  int nb[6];
+  int nbmat[27];
+  double r_min[3];
+  double dr[3];
+};
+
+enum BcType { BC_DIRICHLET = 0, BC_NEUMANN = 1, BC_DIRICHLET_COPY = 2 };
+enum RbType { RB_NONE = 0, RB_MG = 1, RB_INTERP = 2, RB_INTERP_LIM = 3 };
+enum LimType { LIM_MC = 0, LIM_GMINMOD43 = 1 };
+
+// Boundary methods of one variable (bc per face + refinement-boundary rule).
+struct VarBc {
+  int type[6];
+  double val[6];
+  int rb;
+  int limiter;
+};
+
+// Lookup table (src/lookup_table_fortran/m_lookup_table.f90): columns [col][row].
+struct DTable {
+  const double *cols;
+  int n;
+  double x_min, inv_fac;
+};
+
+constexpr double UC_eps0 = 8.8541878128e-12;
+constexpr double UC_elem_charge = 1.602176634e-19;
+constexpr double SI_to_Townsend = 1e21;
+
+template <int D, int NC>
+struct Geo {
+  static constexpr int NG = NC + 2;
+  static constexpr int NCELL = (D == 3) ? NG * NG * NG : NG * NG;
+  static constexpr int NF = NC + 1;
+  static constexpr int NFACE = (D == 3) ? NF * NF * NF : NF * NF;  // per dimension
+  static constexpr int NINT = (D == 3) ? NC * NC * NC : NC * NC;
+  static constexpr int NTAN = (D == 3) ? NC * NC : NC;              // cells on a face
+  __host__ __device__ static constexpr int idx(int i, int j, int k) { return i + NG * (j + NG * k); }
+  __host__ __device__ static constexpr int fidx(int i, int j, int k, int d) {
+    return (i - 1) + NF * ((j - 1) + NF * (k - 1)) + d * NFACE;
+  }
+};
+
+__host__ __device__ inline void tangential(int D, int d, int &t0, int &t1) {
+  if (D == 2) {
+    t0 = 1 - d;
+    t1 = -1;
+  } else {
+    t0 = (d == 0) ? 1 : 0;
+    t1 = (d == 2) ? 1 : 2;
+  }
+}
+
+// Lookup: LT_get_loc / LT_get_col_at_loc (m_lookup_table.f90:330-406).
+__device__ inline void lt_loc(const DTable &t, double x, int &ix, double &w) {
+  double frac = (x - t.x_min) * t.inv_fac;
+  if (frac <= 0) {
+    ix = 1;
+    w = 1;
+  } else if (frac >= t.n - 1) {
+    ix = t.n - 1;
+    w = 0;
+  } else {
+    ix = (int)ceil(frac);
+    w = ix - frac;
+  }
+}
+__device__ inline double lt_at(const DTable &t, int c, int ix, double w) {
+  const double *y = t.cols + (size_t)c * t.n;
+  return w * y[ix - 1] + (1 - w) * y[ix];
+}
+
+// Limiters (afivo/src/m_af_limiters.f90).
+template <typename T>
+__device__ inline T lim_gminmod(T a, T b, T theta) {
+  if (a * b > 0) {
+    T x = fabs(theta * a), y = fabs(theta * b), z = fabs(T(0.5) * (a + b));
+    T m = fmin(fmin(x, y), z);
+    return copysign(m, a);
+  }
+  return T(0);
+}
+template <typename T>
+__device__ inline T lim_apply(T a, T b, int lim) {
+  return (lim == LIM_MC) ? lim_gminmod<T>(a, b, T(2.0)) : lim_gminmod<T>(a, b, T(4 / 3.0));
+}
+template <typename T>
+__device__ inline T lim_koren(T a, T b) {
+  const T third = T(1 / 3.0);
+  T aa = a * a, ab = a * b;
+  if (ab <= 0) return T(0);
+  if (aa <= T(0.25) * ab) return 2 * a;
+  if (aa <= T(2.5) * ab) return third * (b + 2 * a);
+  return 2 * b;
+}
+
+// Atomic max/min of non-negative doubles through their bit patterns (the
+// IEEE order of non-negative values equals the unsigned-integer order).
+__device__ inline void atomic_max_nonneg(unsigned long long *addr, double v) {
+  atomicMax(addr, (unsigned long long)__double_as_longlong(v));
+}
+__device__ inline void atomic_min_nonneg(unsigned long long *addr, double v) {
+  atomicMin(addr, (unsigned long long)__double_as_longlong(v));
+}
+
+template <int BS>
+__device__ inline double block_max(double v, double *sh) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double r = sh[0];
+  for (int w = 1; w < BS / 64; w++) r = fmax(r, sh[w]);
+  __syncthreads();
+  return r;
+}
+template <int BS>
+__device__ inline double block_min(double v, double *sh) {
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double r = sh[0];
+  for (int w = 1; w < BS / 64; w++) r = fmin(r, sh[w]);
+  __syncthreads();
+  return r;
+}
+
+}  // namespace afg

@@ -1,0 +1,470 @@
+// Drift-diffusion, source and field kernels:
+//   flux_upwind_box + af_gc2_box + flux_upwind   (k_flux)
+//   af_consistent_fluxes                          (k_consistent_fluxes)
+//   flux_update_densities + add_source_terms      (k_update)
+//   field_set_rhs, mg_compute_phi_gradient, photoi source/accumulation.
+#pragma once
+
+#include "afgpu_mg.h"
+
+namespace afg {
+
+// Two-layer halo value of n_e for face nb, layer (0 = adjacent, 1 = outer),
+// tangential (a, b): af_gc2_box (afivo/src/m_af_ghostcell.f90:672-744) with
+// gc2_prolong_rb (:753-856) and bc_to_gc2 (:282-375).
+template <int D, int NC>
+__device__ inline double gc2_value(const DBox *__restrict__ boxes, int id, const double *var, const VarBc &m, int nb,
+                                   int layer, int a, int b) {
+  using G = Geo<D, NC>;
+  constexpr int H = NC / 2;
+  const DBox &bx = boxes[id - 1];
+  const double *u = var + (size_t)(id - 1) * G::NCELL;
+  const int d = nb / 2;
+  const bool low = (nb % 2) == 0;
+  int t0, t1;
+  tangential(D, d, t0, t1);
+  int p[3] = {0, 0, 0};
+  p[t0] = a;
+  if (D == 3) p[t1] = b;
+  const int nb_id = bx.nb[nb];
+  if (nb_id > 0) {
+    const double *w = var + (size_t)(nb_id - 1) * G::NCELL;
+    p[d] = low ? NC - layer : 1 + layer;
+    return w[G::idx(p[0], p[1], p[2])];
+  }
+  if (nb_id < 0) {
+    double c0, c1, c2;
+    if (m.type[nb] == BC_DIRICHLET) {
+      c0 = 2; c1 = -1; c2 = c0;
+    } else if (m.type[nb] == BC_NEUMANN) {
+      c0 = bx.dr[d] * (low ? -1 : 1); c1 = 1; c2 = 3 * c0;
+    } else {
+      c0 = 1; c1 = 0; c2 = c0;
+    }
+    if (D == 3 && nb == 3) c2 = c0;  // the reference's high-y branch (:361)
+    const double bv = m.val[nb];
+    p[d] = low ? 1 + layer : NC - layer;
+    const double x = u[G::idx(p[0], p[1], p[2])];
+    return (layer == 0) ? c0 * bv + c1 * x : c2 * bv + c1 * x;
+  }
+  // gc2_prolong_rb: limited linear prolongation from the parent's neighbour
+  const int p_nb_id = boxes[bx.parent - 1].nb[nb];
+  const double *P = var + (size_t)(p_nb_id - 1) * G::NCELL;
+  int off[3] = {0, 0, 0};
+  for (int dd = 0; dd < D; dd++) off[dd] = ((bx.ix[dd] - 1) & 1) * H;
+  // fine index along every dimension, and the coarse cell containing it
+  int fine[3] = {p[0], p[1], p[2]};
+  fine[d] = low ? -1 + layer : NC + 1 + layer;
+  int cc[3] = {0, 0, 0};
+  bool lower[3] = {false, false, false};
+  for (int dd = 0; dd < D; dd++) {
+    if (dd == d) {
+      // normal direction: the low face's fine cells are -1 (lower) and 0,
+      // the high face's are nc+1 (lower) and nc+2
+      cc[dd] = low ? NC : 1;
+      lower[dd] = low ? (layer == 1) : (layer == 0);
+    } else {
+      cc[dd] = off[dd] + ((fine[dd] + 1) >> 1);
+      lower[dd] = (fine[dd] & 1) != 0;
+    }
+  }
+  const int c = G::idx(cc[0], cc[1], cc[2]);
+  constexpr int st[3] = {1, G::NG, G::NG * G::NG};
+  const double f0 = P[c];
+  double v = f0;
+  for (int dd = 0; dd < D; dd++) {
+    const double sa = P[c] - P[c - st[dd]];
+    const double sb = P[c + st[dd]] - P[c];
+    const double fd = 0.25 * lim_apply<double>(sa, sb, m.limiter);
+    v = lower[dd] ? v - fd : v + fd;
+  }
+  return v;
+}
+
+// One leaf box: n_e halo, face fluxes, CFL sum and conductivity maxima.
+// Launched per refinement level, lowest level first (the reference's level
+// loop makes a coarse leaf's written-back ghost layer visible to the finer
+// level's halo, m_af_flux_schemes.f90:696-707).
+template <int D, int NC, int BS>
+__global__ void __launch_bounds__(BS)
+    k_flux(const int *__restrict__ ids, const DBox *__restrict__ boxes, double *ne, const double *__restrict__ efield,
+           const double *__restrict__ efc, double *flux, VarBc m, DTable td, double N_inv,
+           unsigned long long *cfl_max, unsigned long long *sig_max) {
+  using G = Geo<D, NC>;
+  constexpr int NT = G::NTAN;
+  __shared__ double halo[2 * D][2][NT];
+  __shared__ double sh[BS / 64];
+  const int id = ids[blockIdx.x];
+  const DBox &bx = boxes[id - 1];
+  const size_t base = (size_t)(id - 1) * G::NCELL;
+  const size_t fbase = (size_t)(id - 1) * D * G::NFACE;
+
+  for (int q = threadIdx.x; q < 2 * D * 2 * NT; q += BS) {
+    const int nb = q / (2 * NT), layer = (q / NT) % 2, r = q % NT;
+    const int a = r % NC + 1, b = (D == 3) ? r / NC + 1 : 0;
+    halo[nb][layer][r] = gc2_value<D, NC>(boxes, id, ne, m, nb, layer, a, b);
+  }
+  __syncthreads();
+  // write the first layer back into the box (m_af_ghostcell.f90:740-742)
+  for (int q = threadIdx.x; q < 2 * D * NT; q += BS) {
+    const int nb = q / NT, r = q % NT;
+    const int d = nb / 2;
+    int t0, t1;
+    tangential(D, d, t0, t1);
+    int p[3] = {0, 0, 0};
+    p[t0] = r % NC + 1;
+    if (D == 3) p[t1] = r / NC + 1;
+    p[d] = (nb % 2) ? NC + 1 : 0;
+    ne[base + G::idx(p[0], p[1], p[2])] = halo[nb][0][r];
+  }
+
+  auto line = [&](int d, int r, const int *p, int q) -> double {
+    if (q >= 1 && q <= NC) {
+      int pp[3] = {p[0], p[1], p[2]};
+      pp[d] = q;
+      return ne[base + G::idx(pp[0], pp[1], pp[2])];
+    }
+    if (q <= 0) return halo[2 * d][-q][r];
+    return halo[2 * d + 1][q - NC - 1][r];
+  };
+
+  double smax = -1e300;
+  // face fluxes (flux_upwind, src/m_fluid.f90:102-209)
+  for (int q = threadIdx.x; q < D * (NC + 1) * NT; q += BS) {
+    const int d = q / ((NC + 1) * NT);
+    const int f = (q / NT) % (NC + 1) + 1;
+    const int r = q % NT;
+    int t0, t1;
+    tangential(D, d, t0, t1);
+    int p[3] = {0, 0, 0};
+    p[t0] = r % NC + 1;
+    if (D == 3) p[t1] = r / NC + 1;
+    const double inv_dx = 1 / bx.dr[d];
+    int fp[3] = {p[0], p[1], (D == 3) ? p[2] : 1};
+    fp[d] = f;
+    const double ex = efc[fbase + G::fidx(fp[0], fp[1], fp[2], d)];
+    const double Lm2 = line(d, r, p, f - 2), Lm1 = line(d, r, p, f - 1), L0 = line(d, r, p, f),
+                 Lp1 = line(d, r, p, f + 1);
+    double u;
+    if (-1 * ex > 0) {
+      u = Lm1 + 0.5 * lim_koren<double>(L0 - Lm1, Lm1 - Lm2);
+    } else {
+      u = L0 - 0.5 * lim_koren<double>(L0 - Lm1, Lp1 - L0);
+    }
+    int e0[3] = {p[0], p[1], p[2]}, e1[3] = {p[0], p[1], p[2]};
+    e0[d] = f - 1;
+    e1[d] = f;
+    const double tmp_fc = 0.5 * (efield[base + G::idx(e0[0], e0[1], e0[2])] +
+                                 efield[base + G::idx(e1[0], e1[1], e1[2])]) *
+                          SI_to_Townsend * N_inv;
+    int lix;
+    double lw;
+    lt_loc(td, tmp_fc, lix, lw);
+    const double mu = lt_at(td, 0, lix, lw) * N_inv;
+    const double dc = lt_at(td, 1, lix, lw) * N_inv;
+    const double v = -mu * ex;
+    flux[fbase + G::fidx(fp[0], fp[1], fp[2], d)] = v * u - dc * inv_dx * (L0 - Lm1);
+    smax = fmax(smax, mu * u);
+  }
+  smax = block_max<BS>(smax, sh);
+  if (threadIdx.x == 0) atomic_max_nonneg(sig_max, fmax(smax, 1e-100));
+
+  // CFL sum per cell (m_fluid.f90:194-195 summed over dimensions,
+  // m_af_flux_schemes.f90:779-848)
+  double cmax = 0;
+  for (int q = threadIdx.x; q < G::NINT; q += BS) {
+    int i, j, k;
+    interior_ijk<D, NC>(q, i, j, k);
+    double cfl = 0.0;
+    for (int d = 0; d < D; d++) {
+      const double inv_dx = 1 / bx.dr[d];
+      double vv[2], dd[2];
+      for (int s = 0; s < 2; s++) {
+        int c[3] = {i, j, k};
+        int fp[3] = {i, j, (D == 3) ? k : 1};
+        fp[d] += s;  // faces c and c+1
+        const double ex = efc[fbase + G::fidx(fp[0], fp[1], fp[2], d)];
+        int e0[3] = {c[0], c[1], c[2]};
+        e0[d] = c[d] - 1 + s;
+        int e1[3] = {c[0], c[1], c[2]};
+        e1[d] = c[d] + s;
+        const double tmp_fc = 0.5 * (efield[base + G::idx(e0[0], e0[1], e0[2])] +
+                                     efield[base + G::idx(e1[0], e1[1], e1[2])]) *
+                              SI_to_Townsend * N_inv;
+        int lix;
+        double lw;
+        lt_loc(td, tmp_fc, lix, lw);
+        const double mu = lt_at(td, 0, lix, lw) * N_inv;
+        dd[s] = lt_at(td, 1, lix, lw) * N_inv;
+        vv[s] = -mu * ex;
+      }
+      const double cl = 1.0 * fmax(fabs(vv[1]), fabs(vv[0])) * inv_dx + 2 * fmax(dd[1], dd[0]) * (inv_dx * inv_dx);
+      cfl = cfl + cl;
+    }
+    cmax = fmax(cmax, cfl);
+  }
+  cmax = block_max<BS>(cmax, sh);
+  if (threadIdx.x == 0) atomic_max_nonneg(cfl_max, cmax);
+}
+
+// Coarse-leaf faces next to a refined neighbour get the (cylindrically
+// weighted) average of the fine fluxes: af_consistent_fluxes +
+// flux_from_children (afivo/src/m_af_core.f90:1257-1402).
+template <int D, int NC, int BS>
+__global__ void __launch_bounds__(BS) k_consistent_fluxes(const int *__restrict__ ids, const DBox *__restrict__ boxes,
+                                                          double *flux, int cyl) {
+  using G = Geo<D, NC>;
+  constexpr int H = NC / 2;
+  const int id = ids[blockIdx.x];
+  const DBox &bx = boxes[id - 1];
+  for (int q = threadIdx.x; q < 2 * D * G::NTAN; q += BS) {
+    const int nb = q / G::NTAN, r = q % G::NTAN;
+    const int nb_id = bx.nb[nb];
+    if (nb_id <= 0 || boxes[nb_id - 1].children[0] > 0) continue;
+    const int d = nb / 2;
+    const bool low = (nb % 2) == 0;
+    int t0, t1;
+    tangential(D, d, t0, t1);
+    const int a = r % NC + 1, b = (D == 3) ? r / NC + 1 : 0;
+    int cd[3] = {0, 0, 0};
+    cd[d] = low ? 0 : 1;
+    cd[t0] = (a > H) ? 1 : 0;
+    if (D == 3) cd[t1] = (b > H) ? 1 : 0;
+    const int i_ch = cd[0] + 2 * cd[1] + 4 * cd[2];
+    const int c_id = bx.children[i_ch];
+    const double *fc = flux + (size_t)(c_id - 1) * D * G::NFACE;
+    double *fn = flux + (size_t)(nb_id - 1) * D * G::NFACE;
+    const int fi = low ? 1 : NC + 1, fi_nb = low ? NC + 1 : 1;
+    const int pa = a - cd[t0] * H;  // index on the child's half
+    if (D == 2) {
+      int pn[2], s0[2], s1[2];
+      pn[d] = fi_nb;
+      pn[t0] = a;
+      s0[d] = s1[d] = fi;
+      s0[t0] = 2 * pa - 1;
+      s1[t0] = 2 * pa;
+      const double A = fc[G::fidx(s0[0], s0[1], 1, d)], B = fc[G::fidx(s1[0], s1[1], 1, d)];
+      double val;
+      if (d == 1 && cyl) {
+        const DBox &bn = boxes[nb_id - 1];
+        double rr = bn.r_min[0] + (a - 0.5) * bn.dr[0];
+        double tmp = 0.25 * bn.dr[0] / rr;
+        double w1 = 1 - tmp, w2 = 1 + tmp;
+        val = 0.5 * (w1 * A + w2 * B);
+      } else {
+        val = 0.5 * (A + B);
+      }
+      fn[G::fidx(pn[0], pn[1], 1, d)] = val;
+    } else {
+      const int pb = b - cd[t1] * H;
+      int pn[3], s00[3], s10[3], s01[3], s11[3];
+      pn[d] = fi_nb;
+      pn[t0] = a;
+      pn[t1] = b;
+      s00[d] = s10[d] = s01[d] = s11[d] = fi;
+      s00[t0] = 2 * pa - 1; s00[t1] = 2 * pb - 1;
+      s10[t0] = 2 * pa;     s10[t1] = 2 * pb - 1;
+      s01[t0] = 2 * pa - 1; s01[t1] = 2 * pb;
+      s11[t0] = 2 * pa;     s11[t1] = 2 * pb;
+      fn[G::fidx(pn[0], pn[1], pn[2], d)] =
+          0.25 * (fc[G::fidx(s00[0], s00[1], s00[2], d)] + fc[G::fidx(s10[0], s10[1], s10[2], d)] +
+                  fc[G::fidx(s01[0], s01[1], s01[2], d)] + fc[G::fidx(s11[0], s11[1], s11[2], d)]);
+    }
+  }
+}
+
+struct UpdateArgs {
+  double *sp[3][3];   // species k, state s
+  const double *efield;
+  const double *photo;
+  const double *flux;
+  DTable chem;
+  double dt, N_inv;
+  int s_deriv, s_out, n_prev;
+  int s_prev[4];
+  double w_prev[4];
+  int last_step;
+  int chem_nmin_on;
+  double chem_nmin;
+  int chem_limit_loss;
+  int photoi_enabled, photoi_species;
+  int cyl;
+};
+
+// flux_update_densities (afivo/src/m_af_flux_schemes.f90:320-436) with the
+// standard-model add_source_terms (src/m_fluid.f90:298-466,
+// src/m_chemistry.f90:672-804) over all leaves.
+template <int D, int NC, int BS>
+__global__ void __launch_bounds__(BS) k_update(const int *__restrict__ ids, const DBox *__restrict__ boxes,
+                                               UpdateArgs A, unsigned long long *chem_min) {
+  using G = Geo<D, NC>;
+  __shared__ double sh[BS / 64];
+  const int id = ids[blockIdx.x];
+  const DBox &bx = boxes[id - 1];
+  const size_t base = (size_t)(id - 1) * G::NCELL;
+  const size_t fbase = (size_t)(id - 1) * D * G::NFACE;
+  double dt_dr[3];
+  for (int d = 0; d < D; d++) dt_dr[d] = A.dt / bx.dr[d];
+  const double eps = 1e-100;
+  double bmin = 1e100;
+  for (int q = threadIdx.x; q < G::NINT; q += BS) {
+    int i, j, k;
+    interior_ijk<D, NC>(q, i, j, k);
+    const size_t x = base + G::idx(i, j, k);
+    double out[3];
+    for (int sp = 0; sp < 3; sp++) {
+      double tmp = 0.0;
+      for (int m = 0; m < A.n_prev; m++) tmp = tmp + A.w_prev[m] * A.sp[sp][A.s_prev[m]][x];
+      out[sp] = tmp;
+    }
+    double dens[3];
+    for (int sp = 0; sp < 3; sp++) {
+      const double cur = (A.s_deriv == A.s_out) ? out[sp] : A.sp[sp][A.s_deriv][x];
+      dens[sp] = fmax(cur, 0.0);
+    }
+    const double field = SI_to_Townsend * A.N_inv * A.efield[x];
+    int lix;
+    double lw;
+    lt_loc(A.chem, field, lix, lw);
+    const double k_ion = 1.0 * lt_at(A.chem, 0, lix, lw);
+    const double k_att = 1.0 * lt_at(A.chem, 1, lix, lw);
+    const double r1 = k_ion * dens[0];
+    const double r2 = k_att * dens[0];
+    double de = 0.0, dp = 0.0, dm = 0.0;
+    de = de - r1;
+    de = de + r1 * 2;
+    dp = dp + r1 * 1;
+    de = de - r2;
+    dm = dm + r2 * 1;
+    double dd[3] = {de, dp, dm};
+    if (A.last_step) {
+      for (int sp = 0; sp < 3; sp++) {
+        double t;
+        if (A.chem_nmin_on)
+          t = (dens[sp] + A.chem_nmin) / fmax(fabs(dd[sp]), eps);
+        else if (A.chem_limit_loss)
+          t = fmax(dens[sp], eps) / fmax(-dd[sp], eps);
+        else
+          t = 1e100;
+        bmin = fmin(bmin, t);
+      }
+    }
+    if (A.photoi_enabled) {
+      const double ph = A.photo[x];
+      dd[0] = dd[0] + ph;
+      dd[A.photoi_species] = dd[A.photoi_species] + ph;
+    }
+    for (int sp = 0; sp < 3; sp++) out[sp] = out[sp] + A.dt * dd[sp];
+    // conservative flux divergence of the electrons
+    const double *fl = A.flux + fbase;
+    double ne = out[0];
+    if (D == 2) {
+      if (A.cyl) {
+        const double dr = bx.dr[0];
+        const double r = bx.r_min[0] + (i - 0.5) * dr;
+        const double inv_r = 1 / r;
+        const double rf1 = (r - 0.5 * dr) * inv_r, rf2 = (r + 0.5 * dr) * inv_r;
+        ne = ne + dt_dr[0] * (rf1 * fl[G::fidx(i, j, 1, 0)] - rf2 * fl[G::fidx(i + 1, j, 1, 0)]) +
+             dt_dr[1] * (fl[G::fidx(i, j, 1, 1)] - fl[G::fidx(i, j + 1, 1, 1)]);
+      } else {
+        ne = ne + dt_dr[0] * (fl[G::fidx(i, j, 1, 0)] - fl[G::fidx(i + 1, j, 1, 0)]) +
+             dt_dr[1] * (fl[G::fidx(i, j, 1, 1)] - fl[G::fidx(i, j + 1, 1, 1)]);
+      }
+    } else {
+      ne = ne + dt_dr[0] * (fl[G::fidx(i, j, k, 0)] - fl[G::fidx(i + 1, j, k, 0)]) +
+           dt_dr[1] * (fl[G::fidx(i, j, k, 1)] - fl[G::fidx(i, j + 1, k, 1)]) +
+           dt_dr[2] * (fl[G::fidx(i, j, k, 2)] - fl[G::fidx(i, j, k + 1, 2)]);
+    }
+    out[0] = ne;
+    for (int sp = 0; sp < 3; sp++) A.sp[sp][A.s_out][x] = out[sp];
+  }
+  if (A.last_step) {
+    bmin = block_min<BS>(bmin, sh);
+    if (threadIdx.x == 0) atomic_min_nonneg(chem_min, bmin);
+  }
+}
+
+// field_set_rhs (src/m_field.f90:406-444) on leaves, all cells.
+template <int NCELL, int BS>
+__global__ void __launch_bounds__(BS) k_set_rhs(const int *__restrict__ ids, const double *n0, const double *n1,
+                                                const double *n2, double *rhs) {
+  const size_t base = (size_t)(ids[blockIdx.x] - 1) * NCELL;
+  const double fac = -UC_elem_charge / UC_eps0;
+  const double q0 = -1 * fac, q1 = 1 * fac, q2 = -1 * fac;
+  for (int x = threadIdx.x; x < NCELL; x += BS) {
+    double r = 0.0;
+    r = r + q0 * n0[base + x];
+    r = r + q1 * n1[base + x];
+    r = r + q2 * n2[base + x];
+    rhs[base + x] = r;
+  }
+}
+
+// E = -grad phi on all faces and |E| at cell centres
+// (mg_compute_phi_gradient / mg_box_lpl_gradient / mg_box_field_norm,
+// afivo/src/m_af_multigrid.f90:1857-2045).
+template <int D, int NC, int BS>
+__global__ void __launch_bounds__(BS) k_gradient(const int *__restrict__ ids, const DBox *__restrict__ boxes,
+                                                 const double *__restrict__ phi, double *efc, double *enorm) {
+  using G = Geo<D, NC>;
+  const int id = ids[blockIdx.x];
+  const DBox &bx = boxes[id - 1];
+  const double *u = phi + (size_t)(id - 1) * G::NCELL;
+  double *f = efc + (size_t)(id - 1) * D * G::NFACE;
+  double inv_dr[3];
+  for (int d = 0; d < D; d++) inv_dr[d] = -1.0 / bx.dr[d];
+  constexpr int NFD = (NC + 1) * G::NTAN;
+  constexpr int st[3] = {1, G::NG, G::NG * G::NG};
+  for (int q = threadIdx.x; q < D * NFD; q += BS) {
+    const int d = q / NFD, r = q % NFD;
+    int t0, t1;
+    tangential(D, d, t0, t1);
+    int p[3] = {0, 0, 1};
+    p[d] = r % (NC + 1) + 1;
+    p[t0] = (r / (NC + 1)) % NC + 1;
+    if (D == 3) p[t1] = r / ((NC + 1) * NC) + 1;
+    const int x = G::idx(p[0], p[1], D == 3 ? p[2] : 0);
+    f[G::fidx(p[0], p[1], p[2], d)] = inv_dr[d] * (u[x] - u[x - st[d]]);
+  }
+  double *e = enorm + (size_t)(id - 1) * G::NCELL;
+  for (int q = threadIdx.x; q < G::NINT; q += BS) {
+    int i, j, k;
+    interior_ijk<D, NC>(q, i, j, k);
+    const int x = G::idx(i, j, k);
+    double s = 0;
+    for (int d = 0; d < D; d++) {
+      const double lo = inv_dr[d] * (u[x] - u[x - st[d]]);
+      const double hi = inv_dr[d] * (u[x + st[d]] - u[x]);
+      const double a = lo + hi;
+      s = (d == 0) ? a * a : s + a * a;
+    }
+    e[x] = 0.5 * sqrt(s);
+  }
+}
+
+// Photon production on leaves (photoionization_rate_from_alpha,
+// src/m_photoi.f90:259-291).
+template <int D, int NC, int BS>
+__global__ void __launch_bounds__(BS) k_photoi_src(const int *__restrict__ ids, const double *__restrict__ efield,
+                                                   const double *__restrict__ ne, double *rhs, DTable td, double N,
+                                                   double coeff) {
+  using G = Geo<D, NC>;
+  const size_t base = (size_t)(ids[blockIdx.x] - 1) * G::NCELL;
+  for (int q = threadIdx.x; q < G::NINT; q += BS) {
+    int i, j, k;
+    interior_ijk<D, NC>(q, i, j, k);
+    const size_t x = base + G::idx(i, j, k);
+    const double fld = efield[x];
+    const double Td = fld * SI_to_Townsend / N;
+    int lix;
+    double lw;
+    lt_loc(td, Td, lix, lw);
+    const double alpha = lt_at(td, 2, lix, lw);
+    const double mob = lt_at(td, 0, lix, lw);
+    double tmp = fld * mob * alpha * ne[x] * coeff;
+    if (tmp < 0) tmp = 0;
+    rhs[x] = tmp;
+  }
+}
+
+}  // namespace afg

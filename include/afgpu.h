@@ -1,0 +1,259 @@
+/*
+ * afgpu.h -- C-ABI of the MI355X (gfx950) implementation of afivo-streamer's
+ * per-time-step hot path: the afivo FAS multigrid V-cycle/FMG for the
+ * electrostatic potential and the Helmholtz photoionization modes, the
+ * Koren-limited drift-diffusion flux with transport-table lookup, and the
+ * explicit density/source update.
+ *
+ * Plain C types only (no torch, no HIP types), so that a Fortran
+ * `bind(C)` interface block, ctypes or any other FFI can bind it.
+ *
+ * Conventions, identical to afivo's box tree (afivo/src/m_af_types.f90):
+ *   - box ids are 1-based, 0 (af_no_box) means "no box", a negative
+ *     neighbour id (af_phys_boundary) means a physical boundary;
+ *   - neighbour numbering lowx=1, highx=2, lowy=3, highy=4, lowz=5, highz=6
+ *     (m_af_types.f90:164-172, 196-207); child numbering follows
+ *     af_child_dix (m_af_types.f90:148, 181-183);
+ *   - cell-centred box data is (nc+2)^D doubles with one ghost layer,
+ *     column-major with i fastest, exactly box%cc(DTIMES(0:nc+1), iv)
+ *     (m_af_types.f90:302, m_af_core.f90:549-551);
+ *   - face data for one variable is D blocks of (nc+1)^D doubles,
+ *     exactly box%fc(DTIMES(1:nc+1), 1:NDIM, ifc) (m_af_core.f90:551).
+ *
+ * Every entry returns an int status: AFGPU_OK on success, otherwise one of
+ * the AFGPU_ERR_* codes; afgpu_last_error() returns the message.  The
+ * Fortran wrapper turns a non-zero status into `error stop`, as the
+ * reference does after its hypre calls (m_coarse_solver.f90:187-189).
+ *
+ * Threading: calls are made from serial driver code (as the reference
+ * calls field_compute / af_advance outside OpenMP regions,
+ * src/streamer.f90:263,327); a handle is not re-entrant.
+ */
+#ifndef AFGPU_H
+#define AFGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct afgpu_ctx afgpu_t;
+
+/* ---- status codes ---------------------------------------------------- */
+#define AFGPU_OK                 0
+#define AFGPU_ERR_ARG            1 /* invalid argument / size            */
+#define AFGPU_ERR_HIP            2 /* HIP runtime failure                */
+#define AFGPU_ERR_STATE          3 /* call out of order (no tree, ...)   */
+#define AFGPU_ERR_NO_CONVERGENCE 4 /* "No convergence in initial field computation" */
+#define AFGPU_ERR_ALL_REJECTED   5 /* "All time steps were rejected"     */
+#define AFGPU_ERR_DT_TOO_SMALL   6 /* "dt too small"                     */
+#define AFGPU_ERR_UNSUPPORTED    7 /* configuration outside the built path */
+
+/* ---- coordinate systems (af_xyz, af_cyl; m_af_types.f90:48-51) ------- */
+#define AFGPU_XYZ 1
+#define AFGPU_CYL 2
+
+/* ---- precision forms ------------------------------------------------- */
+#define AFGPU_PREC_FP64  0 /* plain FP64, same arithmetic as the CPU path  */
+#define AFGPU_PREC_MIXED 1 /* FP32 multigrid inside FP64 defect correction */
+
+/* ---- time integrators (m_af_advance.f90:10-11) ----------------------- */
+#define AFGPU_FORWARD_EULER 1
+#define AFGPU_HEUNS_METHOD  2
+
+/* ---- species boundary conditions (src/m_streamer.f90:324-336) -------- */
+#define AFGPU_SPECIES_BC_NEUMANN_ZERO   0
+#define AFGPU_SPECIES_BC_DIRICHLET_ZERO 1
+
+/* ---- cell-centred variables of the device box pool --------------------
+ * Densities keep afivo's "iv + state" numbering of copies
+ * (af_add_cc_variable n_copies, src/m_chemistry.f90:285-287): state s of
+ * species k is AFGPU_CC_SPECIES(k) + s.  With forward Euler, state 1 is the
+ * backup used by copy_current_state; with Heun, state 1 is the predictor
+ * and state 2 the backup (src/streamer.f90:659-688). */
+#define AFGPU_CC_ELECTRON   0  /* e, e_2, e_3    */
+#define AFGPU_CC_POS_ION    3  /* M+, ...        */
+#define AFGPU_CC_NEG_ION    6  /* M-, ...        */
+#define AFGPU_CC_PHI        9  /* phi            */
+#define AFGPU_CC_PHI_2      10 /* phi backup     */
+#define AFGPU_CC_EFIELD     11 /* electric_fld   */
+#define AFGPU_CC_RHS        12 /* rhs            */
+#define AFGPU_CC_TMP        13 /* tmp            */
+#define AFGPU_CC_PHOTO      14 /* photo          */
+#define AFGPU_CC_HELMH      15 /* helmh_1..helmh_3 (15, 16, 17) */
+#define AFGPU_N_CC          18
+#define AFGPU_CC_SPECIES(k) (3 * (k))
+
+/* ---- face-centred variables ------------------------------------------ */
+#define AFGPU_FC_FLUX  0 /* flux_elec                      */
+#define AFGPU_FC_FIELD 1 /* electric_fld (face components) */
+#define AFGPU_N_FC     2
+
+/* ---- multigrid operators --------------------------------------------- */
+#define AFGPU_OP_POISSON 0 /* mg for phi, bc field_bc_homogeneous         */
+#define AFGPU_OP_HELMH   1 /* mg_helm(n) is AFGPU_OP_HELMH + n - 1         */
+#define AFGPU_MAX_MODES  3
+
+/* Scalar settings of the path: the values the reference reads from its
+ * config (m_streamer.f90, m_dt.f90, m_photoi*.f90, m_gas.f90). */
+typedef struct {
+  double gas_number_density;   /* N = 1e5 p / (k_B T)  (m_gas.f90:174-176)   */
+  int    time_integrator;      /* AFGPU_FORWARD_EULER or AFGPU_HEUNS_METHOD   */
+  double dt_max;               /* m_dt.f90:39                                 */
+  double dt_min;               /* m_dt.f90:42                                 */
+  double dt_safety_factor;     /* m_dt.f90:27                                 */
+  double dt_cfl_number;        /* m_dt.f90:30 (0.5 default)                   */
+  double dt_max_growth_factor; /* m_dt.f90:45                                 */
+  int    dt_chemistry_limit_loss; /* m_dt.f90:36                              */
+  double dt_chemistry_nmin;    /* m_dt.f90:33 (<= 0: unused)                  */
+  int    mg_num_vcycles;       /* ST_multigrid_num_vcycles (m_streamer.f90:151)*/
+  double mg_max_rel_residual;  /* ST_multigrid_max_rel_residual               */
+  int    mg_n_cycle_down;      /* mg_t%n_cycle_down (2)                       */
+  int    mg_n_cycle_up;        /* mg_t%n_cycle_up (2)                         */
+  double domain_len_last;      /* ST_domain_len(NDIM), for the residual floor */
+  int    species_bc;           /* AFGPU_SPECIES_BC_*                          */
+  int    photoi_enabled;
+  int    photoi_per_steps;     /* photoi%per_steps                            */
+  int    photoi_species;       /* 0 = e, 1 = M+, 2 = M- (photoi%species)      */
+  double photoi_eta_quench;    /* photoi_eta * quench_fac (m_photoi.f90:175-184) */
+  int    photoi_n_modes;       /* <= AFGPU_MAX_MODES                          */
+  double photoi_lambda[AFGPU_MAX_MODES]; /* 1/m, after pressure scaling       */
+  double photoi_coeff[AFGPU_MAX_MODES];  /* 1/m^2, after pressure scaling     */
+  double photoi_max_rel_residual; /* photoi_helmh%max_rel_residual (1e-2)     */
+  int    photoi_max_fmg_cycles;   /* max_fmg_cycles (10)                      */
+} afgpu_params_t;
+
+/* Result of one time step (afgpu_step). */
+typedef struct {
+  int    it;            /* iteration number after the step                */
+  double time;          /* simulation time after the step                 */
+  double dt_used;       /* accepted dt                                    */
+  double dt_next;       /* global_dt for the next step                    */
+  double dt_lim;        /* min over stages of the stage dt limits         */
+  int    n_rejected;    /* rejected attempts in this step                 */
+  int    n_vcycles;     /* V-cycles of the final field_compute            */
+  double residual;      /* max |residual| of the final field_compute      */
+  double rel_residual;  /* residual / max |rhs|                           */
+  int    photoi_done;   /* photoionization recomputed in this step        */
+  long long leaf_cells; /* leaf cells advanced (af_num_cells_used)        */
+} afgpu_step_info_t;
+
+/* ---- lifetime ---------------------------------------------------------
+ * Replaces tree/mg set-up: af_init (m_af_core.f90:138) and mg_init
+ * (m_af_multigrid.f90:43).  ndim 2 or 3; n_cell even, 8 or 16;
+ * coord AFGPU_XYZ or AFGPU_CYL (2D only). */
+int afgpu_create(int ndim, int n_cell, int coord, int precision, afgpu_t **out);
+int afgpu_destroy(afgpu_t *h);
+const char *afgpu_last_error(void);
+
+/* Mirror of the AMR tree after af_init / af_adjust_refinement / restart
+ * (box_t fields, m_af_types.f90:286-324).  Arrays are indexed by id-1 and
+ * hold n_boxes entries of: lvl (0 = unused slot), ix[D], parent,
+ * children[2^D], neighbors[2D], neighbor_mat[3^D] (i fastest, as
+ * box%neighbor_mat(DTIMES(-1:1))), r_min[D], dr[D].  Level lists
+ * (lvl_t ids/leaves/parents) are derived.  Box data is kept for box ids
+ * whose slot survives; new slots start at zero. */
+int afgpu_sync_tree(afgpu_t *h, int n_boxes, const int *lvl, const int *ix,
+                    const int *parent, const int *children,
+                    const int *neighbors, const int *neighbor_mat,
+                    const double *r_min, const double *dr);
+
+/* Box data transfer: n_ids boxes, data[n][(nc+2)^D] (cc) or
+ * data[n][D][(nc+1)^D] (fc).  Replaces reading/writing box%cc / box%fc. */
+int afgpu_put_cc(afgpu_t *h, int iv, int n_ids, const int *ids, const double *data);
+int afgpu_get_cc(afgpu_t *h, int iv, int n_ids, const int *ids, double *data);
+int afgpu_put_fc(afgpu_t *h, int ifc, int n_ids, const int *ids, const double *data);
+int afgpu_get_fc(afgpu_t *h, int ifc, int n_ids, const int *ids, double *data);
+
+/* Settings, transport tables and applied voltage.
+ * td_cols: [4][n_rows] columns mobility*N, diffusion*N, alpha/N, eta/N of
+ * td_tbl (src/m_transport_data.f90:85-126); chem_cols: [2][n_rows]
+ * ionization and attachment rate columns of chemtbl_fld
+ * (src/m_chemistry.f90:233-256, 345-372).  Lookup as LT_get_col
+ * (src/lookup_table_fortran/m_lookup_table.f90:330-406). */
+int afgpu_set_params(afgpu_t *h, const afgpu_params_t *p);
+int afgpu_set_tables(afgpu_t *h, int n_rows,
+                     double td_x_min, double td_inv_fac, const double *td_cols,
+                     double chem_x_min, double chem_inv_fac, const double *chem_cols);
+/* field_set_voltage result (src/m_field.f90:551-586): Dirichlet value at
+ * the high boundary of the last dimension. */
+int afgpu_set_voltage(afgpu_t *h, double voltage);
+
+/* ---- multigrid (afivo/src/m_af_multigrid.f90) --------------------------
+ * op: AFGPU_OP_POISSON or AFGPU_OP_HELMH + n.  Replaces
+ * mg_fas_vcycle(tree, mg, set_residual, highest_lvl) (:185) and
+ * mg_fas_fmg(tree, mg, set_residual, have_guess) (:137).  max_lvl <= 0
+ * means tree%highest_lvl. */
+int afgpu_mg_fas_vcycle(afgpu_t *h, int op, int set_residual, int max_lvl);
+int afgpu_mg_fas_fmg(afgpu_t *h, int op, int set_residual, int have_guess);
+/* af_tree_maxabs_cc (afivo/src/m_af_utils.f90:773): max |cc| over leaf
+ * interior cells. */
+int afgpu_tree_maxabs_cc(afgpu_t *h, int iv, double *out);
+/* af_gc_tree (afivo/src/m_af_ghostcell.f90:24) and af_restrict_tree
+ * (afivo/src/m_af_restrict.f90:50) for one variable, with the variable's
+ * own methods; used before refinement (src/streamer.f90:399-400). */
+int afgpu_gc_tree(afgpu_t *h, int iv);
+int afgpu_restrict_tree(afgpu_t *h, int iv);
+
+/* ---- field (src/m_field.f90) -------------------------------------------
+ * field_compute(tree, mg, s_in, time, have_guess) (:448): rhs from the
+ * charged species of state s_in, V-cycles (or FMG without guess) to the
+ * residual threshold, then field_from_potential.  Outputs the number of
+ * V-cycles (or FMG cycles) and the final max |residual|. */
+int afgpu_field_compute(afgpu_t *h, int s_in, int have_guess,
+                        int *n_cycles, double *residual);
+/* field_from_potential (:531): E_fc = -grad phi, |E| and its ghost cells. */
+int afgpu_field_from_potential(afgpu_t *h);
+
+/* ---- fluid (src/m_fluid.f90, afivo/src/m_af_flux_schemes.f90) ----------
+ * forward_euler (src/m_fluid.f90:21): flux_upwind_tree (Koren, table
+ * lookup, af_consistent_fluxes) + flux_update_densities with
+ * add_source_terms; returns dt_lim = min(dt_max, minval(dt_limits)). */
+int afgpu_forward_euler(afgpu_t *h, double dt, int s_deriv, int n_prev,
+                        const int *s_prev, const double *w_prev, int s_out,
+                        int i_step, int n_steps, double *dt_lim);
+/* af_advance(tree, dt, dt_lim, time, all_densities, integrator,
+ * forward_euler) (afivo/src/m_af_advance.f90:121). */
+int afgpu_advance(afgpu_t *h, double dt, double *dt_lim);
+
+/* photoi_set_src (src/m_photoi.f90:153) with the Helmholtz method
+ * (src/m_photoi_helmh.f90:162). */
+int afgpu_photoi_set_src(afgpu_t *h);
+
+/* copy_current_state / restore_previous_state (src/streamer.f90:659-688). */
+int afgpu_copy_current_state(afgpu_t *h);
+int afgpu_restore_previous_state(afgpu_t *h);
+
+/* One iteration of the time loop of src/streamer.f90:181-365 without
+ * output and refinement: photoionization every photoi_per_steps, step
+ * attempts with rejection/restore, field_compute, next dt.  The first call
+ * takes dt = dt_min (src/streamer.f90:459). */
+int afgpu_step(afgpu_t *h, afgpu_step_info_t *info);
+
+/* Host-only helper (no GPU use): factors of the exact level-1 solve used in
+ * place of hypre PFMG (afivo/src/m_coarse_solver.f90:421).  Level 1 must be
+ * one box with physical boundaries on all faces; r_min/dr are that box's,
+ * bc_dirichlet[2D] is 1 for a Dirichlet face and 0 for a Neumann face.
+ * Outputs per dimension d: vinv/v (nc x nc, row major) and eig (nc), with
+ * T_d = V_d diag(eig_d) V_d^-1 the 1-D operator including boundary
+ * folding. */
+int afgpu_coarse_factors(int ndim, int n_cell, int coord, const double *r_min,
+                         const double *dr, const int *bc_dirichlet,
+                         double *vinv, double *v, double *eig);
+
+/* Leaf-cell count of the current tree (af_num_cells_used). */
+long long afgpu_num_leaf_cells(afgpu_t *h);
+
+/* Kernel-level timing: when enabled, every launch of kernel class `kind`
+ * (0 = GSRB half-sweep, 1 = flux, 2 = density update) is bracketed by HIP
+ * events on the library's stream.  Reports the summed duration (ms), the
+ * launch count and the algorithmic bytes (interior words moved) of those
+ * launches since the last reset. */
+int afgpu_kernel_timing(afgpu_t *h, int enable);
+int afgpu_kernel_stats(afgpu_t *h, int kind, int reset, double *total_ms,
+                       long long *launches, double *algo_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AFGPU_H */
