@@ -119,6 +119,8 @@ _COMMON = {
     "restore_previous_state": (_I, [_P]),
     "step": (_I, [_P, C.POINTER(StepInfoT)]),
     "num_leaf_cells": (C.c_longlong, [_P]),
+    "get_time_state": (_I, [_P, _IP, _DP, _DP]),
+    "set_time_state": (_I, [_P, _I, _D, _D]),
 }
 _GPU_ONLY = {
     "coarse_factors": (_I, [_I, _I, _I, _DP, _DP, _IP, _DP, _DP, _DP]),
@@ -434,6 +436,14 @@ class Path:
         info = StepInfoT()
         self._call("step", C.byref(info))
         return info.as_dict()
+
+    def get_time_state(self):
+        it, t, dt = C.c_int(), C.c_double(), C.c_double()
+        self._call("get_time_state", C.byref(it), C.byref(t), C.byref(dt))
+        return it.value, t.value, dt.value
+
+    def set_time_state(self, it, time, global_dt):
+        self._call("set_time_state", it, time, global_dt)
 
     def num_leaf_cells(self):
         return self._fn("num_leaf_cells")(self.h)

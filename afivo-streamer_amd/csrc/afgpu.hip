@@ -615,24 +615,24 @@ int afgpu_create(int ndim, int n_cell, int coord, int precision, afgpu_t **out) 
 
 int afgpu_destroy(afgpu_t *h) {
   if (!h) return AFGPU_OK;
-  if (h->stream) hipStreamSynchronize(h->stream);
-  for (int i = 0; i < AFGPU_N_CC; i++) hipFree(h->cc[i]);
-  for (int i = 0; i < AFGPU_N_FC; i++) hipFree(h->fc[i]);
-  for (int i = 0; i < 3; i++) hipFree(h->cc32[i]);
-  hipFree(h->d_boxes);
-  hipFree(h->d_lists);
-  hipFree(h->d_td);
-  hipFree(h->d_chem);
-  hipFree(h->d_red);
-  if (h->h_red) hipHostFree(h->h_red);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (int i = 0; i < AFGPU_N_CC; i++) (void)hipFree(h->cc[i]);
+  for (int i = 0; i < AFGPU_N_FC; i++) (void)hipFree(h->fc[i]);
+  for (int i = 0; i < 3; i++) (void)hipFree(h->cc32[i]);
+  (void)hipFree(h->d_boxes);
+  (void)hipFree(h->d_lists);
+  (void)hipFree(h->d_td);
+  (void)hipFree(h->d_chem);
+  (void)hipFree(h->d_red);
+  if (h->h_red) (void)hipHostFree(h->h_red);
   for (auto &o : h->ops) {
-    hipFree(o.d_vinv);
-    hipFree(o.d_v);
-    hipFree(o.d_eig);
+    (void)hipFree(o.d_vinv);
+    (void)hipFree(o.d_v);
+    (void)hipFree(o.d_eig);
   }
   for (auto &t : h->timers)
-    for (auto e : t.pool) hipEventDestroy(e);
-  if (h->stream) hipStreamDestroy(h->stream);
+    for (auto e : t.pool) (void)hipEventDestroy(e);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return AFGPU_OK;
 }
@@ -783,15 +783,15 @@ int afgpu_sync_tree(afgpu_t *h, int n_boxes, const int *lvl, const int *ix, cons
     for (int i = 0; i < AFGPU_N_FC; i++) RCHK(grow(h->fc[i], h->nface));
     if (h->precision == AFGPU_PREC_MIXED) {
       for (int i = 0; i < 3; i++) {
-        hipFree(h->cc32[i]);
+        (void)hipFree(h->cc32[i]);
         HIPCHK(hipMalloc(&h->cc32[i], (size_t)cap * h->ncell * sizeof(float)));
         HIPCHK(hipMemsetAsync(h->cc32[i], 0, (size_t)cap * h->ncell * sizeof(float), h->stream));
       }
     }
     h->cap_boxes = cap;
   }
-  hipFree(h->d_boxes);
-  hipFree(h->d_lists);
+  (void)hipFree(h->d_boxes);
+  (void)hipFree(h->d_lists);
   h->d_boxes = nullptr;
   h->d_lists = nullptr;
   HIPCHK(hipMalloc(&h->d_boxes, (size_t)n_boxes * sizeof(DBox)));
@@ -866,8 +866,8 @@ int afgpu_set_tables(afgpu_t *h, int n_rows, double td_x_min, double td_inv_fac,
     set_err("afgpu_set_tables: invalid arguments");
     return AFGPU_ERR_ARG;
   }
-  hipFree(h->d_td);
-  hipFree(h->d_chem);
+  (void)hipFree(h->d_td);
+  (void)hipFree(h->d_chem);
   h->d_td = h->d_chem = nullptr;
   HIPCHK(hipMalloc(&h->d_td, 4 * (size_t)n_rows * sizeof(double)));
   HIPCHK(hipMalloc(&h->d_chem, 2 * (size_t)n_rows * sizeof(double)));
@@ -1050,6 +1050,23 @@ int afgpu_step(afgpu_t *h, afgpu_step_info_t *info) {
     set_err("dt too small");
     return AFGPU_ERR_DT_TOO_SMALL;
   }
+  return AFGPU_OK;
+}
+
+int afgpu_get_time_state(afgpu_t *h, int *it, double *time, double *global_dt) {
+  if (!h) return AFGPU_ERR_ARG;
+  if (it) *it = h->it;
+  if (time) *time = h->time;
+  if (global_dt) *global_dt = h->started ? h->global_dt : h->p.dt_min;
+  return AFGPU_OK;
+}
+
+int afgpu_set_time_state(afgpu_t *h, int it, double time, double global_dt) {
+  if (!h || it < 0 || !(global_dt > 0)) return AFGPU_ERR_ARG;
+  h->it = it;
+  h->time = h->global_time = time;
+  h->global_dt = h->dt = global_dt;
+  h->started = true;
   return AFGPU_OK;
 }
 

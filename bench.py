@@ -1,0 +1,191 @@
+"""Benchmark: cell-steps/s of the full 3D AMR streamer step (drift-diffusion
++ multigrid Poisson to 1e-8 relative residual + 3-term Helmholtz
+photoionization) on one MI355X.
+
+One "step" is one iteration of the reference's time loop
+(src/streamer.f90:181-365 without output/refinement): photoionization
+source and three Helmholtz FMG solves, backup copy, forward-Euler flux and
+source update, then the Poisson solve to the residual threshold and the
+field.  Cell-steps = leaf cells advanced per step (af_num_cells_used),
+summed over the timed steps.
+
+Prints one JSON line.  The default run measures the selected precision form
+on the GPU, the plain FP64 GPU form on the same workload, and the CPU
+oracle (the reference algorithm restated in C++/OpenMP) on a bounded sample
+(one step of the same workload from the same state) on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "afivo-streamer_amd"))
+
+import numpy as np  # noqa: E402
+
+import afgpu as A  # noqa: E402
+import setups as S  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_case(args):
+    case = S.config1(uniform_lvl=args.uniform_lvl, max_lvl=args.max_lvl, radius=args.radius * 1e-3,
+                     photoi=True, num_vcycles=args.max_vcycles, rel_residual=args.rel_residual)
+    case.params.photoi_per_steps = args.photoi_per_steps
+    return case
+
+
+def run_gpu(case, precision, warmup, steps, timing):
+    p = A.Path("gpu", case.D, case.nc, case.coord, precision)
+    S.apply(p, case)
+    t0 = time.time()
+    p.field_compute(0, have_guess=False)
+    log("  initial field solve %.2f s" % (time.time() - t0))
+    for i in range(warmup):
+        info = p.step()
+        log("  warmup", i, info["n_vcycles"], "%.3e" % info["rel_residual"])
+    if timing:
+        p.lib.afgpu_kernel_timing(p.h, 1)
+        for k in range(3):
+            p.lib.afgpu_kernel_stats(p.h, k, 1, None, None, None)
+    infos = []
+    t0 = time.perf_counter()
+    for i in range(steps):
+        infos.append(p.step())
+    t1 = time.perf_counter()
+    stats = {}
+    if timing:
+        import ctypes as C
+        for k, name in enumerate(("gsrb", "flux", "update")):
+            ms, n, b = C.c_double(), C.c_longlong(), C.c_double()
+            p.lib.afgpu_kernel_stats(p.h, k, 1, C.byref(ms), C.byref(n), C.byref(b))
+            stats[name] = {"total_ms": ms.value, "launches": n.value, "algo_bytes": b.value}
+        p.lib.afgpu_kernel_timing(p.h, 0)
+    return p, infos, t1 - t0, stats
+
+
+def rel_err(a, b):
+    d = a - b
+    return float(np.abs(d).max() / max(np.abs(b).max(), 1e-300)), \
+        float(np.sqrt((d * d).sum() / max((b * b).sum(), 1e-300)))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--uniform-lvl", type=int, default=4)
+    ap.add_argument("--max-lvl", type=int, default=5)
+    ap.add_argument("--radius", type=float, default=5.2, help="refined channel radius (mm)")
+    ap.add_argument("--rel-residual", type=float, default=1e-8)
+    ap.add_argument("--max-vcycles", type=int, default=100)
+    ap.add_argument("--photoi-per-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64-ref", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box in this setup has a single MI355X "
+              "(the multi-GPU sharding of the path is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    case = make_case(args)
+    info = case.tree.info()
+    log("workload", case.name, info)
+    prec = A.PREC_MIXED if args.precision == "mixed" else A.PREC_FP64
+    log("GPU %s run" % args.precision)
+    p, infos, secs, stats = run_gpu(case, prec, args.warmup, args.steps, timing=True)
+    cell_steps = sum(i["leaf_cells"] for i in infos)
+    value = cell_steps / secs
+    nv = [i["n_vcycles"] for i in infos]
+    log("  %.3f s for %d steps: %.4e cell-steps/s, vcycles/step %s" % (secs, args.steps, value, nv))
+
+    # roofline of the dominant kernel class
+    dom = max(stats, key=lambda k: stats[k]["total_ms"]) if stats else None
+    roof = None
+    if dom:
+        st = stats[dom]
+        avg_ms = st["total_ms"] / max(st["launches"], 1)
+        achieved = st["algo_bytes"] / (st["total_ms"] * 1e-3) / 1e9 if st["total_ms"] > 0 else 0.0
+        roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "avg_launch_ms": round(avg_ms, 5), "launches": st["launches"],
+                "algo_bytes_per_launch": st["algo_bytes"] / max(st["launches"], 1),
+                "kernel_share_of_step": round(st["total_ms"] / (secs * 1e3), 3)}
+        log("  kernel stats", json.dumps(stats))
+
+    extra = {}
+    ref = None
+    if prec == A.PREC_MIXED and not args.no_fp64_ref:
+        log("GPU fp64 reference run")
+        ref, rinfos, rsecs, _ = run_gpu(case, A.PREC_FP64, args.warmup, args.steps, timing=False)
+        fp64_value = sum(i["leaf_cells"] for i in rinfos) / rsecs
+        extra["fp64_gpu_value"] = fp64_value
+        extra["mixed_over_fp64"] = value / fp64_value
+        errs = {}
+        for name, iv in (("phi", A.CC_PHI), ("efield", A.CC_EFIELD), ("electron", A.CC_ELECTRON)):
+            mx, l2 = rel_err(p.get_cc(iv), ref.get_cc(iv))
+            errs[name] = {"max_rel": mx, "l2_rel": l2}
+        extra["error_vs_fp64"] = errs
+        log("  fp64 %.4e cell-steps/s; mixed/fp64 = %.3f; errors %s" % (fp64_value, value / fp64_value, errs))
+
+    cpu = None
+    if not args.no_cpu:
+        src = ref if ref is not None else p
+        threads = min(16, os.cpu_count() or 1)
+        os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+        c = A.Path("oracle", case.D, case.nc, case.coord)
+        S.apply(c, case)
+        for iv in list(range(9)) + [A.CC_PHI, A.CC_EFIELD, A.CC_PHOTO] + [A.CC_HELMH + k for k in range(3)]:
+            c.put_cc(iv, src.get_cc(iv))
+        c.put_fc(A.FC_FIELD, src.get_fc(A.FC_FIELD))
+        # continue the time loop from the GPU state: same dt and iteration
+        c.set_time_state(*src.get_time_state())
+        c.lib.afcpu_num_threads.restype = A.C.c_int
+        nthreads = c.lib.afcpu_num_threads()
+        log("CPU oracle sample: %d step(s), %d threads" % (args.cpu_steps, nthreads))
+        t0 = time.perf_counter()
+        ccells = 0
+        for _ in range(args.cpu_steps):
+            ccells += c.step()["leaf_cells"]
+        csecs = time.perf_counter() - t0
+        cpu = {"value": ccells / csecs, "unit": "cell-steps/s", "cores": nthreads, "kind": "port",
+               "sample": "%d step(s) of the same 3D AMR workload (%d leaf cells) from the GPU state "
+                         "after warmup, %.1f s" % (args.cpu_steps, info["leaf_cells"], csecs)}
+        log("  cpu %.4e cell-steps/s (%.1f s)" % (cpu["value"], csecs))
+
+    out = {
+        "metric": "cell-steps/s, 3D AMR step (DD+MG Poisson+photoi), FP64 vs mixed at 1e-8",
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": secs * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if prec == A.PREC_MIXED else "f64",
+        "data": "synthetic",
+        "config": {"workload": case.name, "leaf_boxes": info["n_leaves"], "leaf_cells": info["leaf_cells"],
+                   "boxes": info["n_boxes"], "levels": info["highest_lvl"], "box_cells": "16^3",
+                   "precision": args.precision, "rel_residual": args.rel_residual,
+                   "photoi_per_steps": args.photoi_per_steps, "mean_vcycles": float(np.mean(nv))},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

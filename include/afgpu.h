@@ -240,6 +240,11 @@ int afgpu_coarse_factors(int ndim, int n_cell, int coord, const double *r_min,
                          const double *dr, const int *bc_dirichlet,
                          double *vinv, double *v, double *eig);
 
+/* Time-loop scalars (it, time, global_dt), as written and read for a
+ * restart by write_sim_data / read_sim_data (src/streamer.f90:541-577). */
+int afgpu_get_time_state(afgpu_t *h, int *it, double *time, double *global_dt);
+int afgpu_set_time_state(afgpu_t *h, int it, double time, double global_dt);
+
 /* Leaf-cell count of the current tree (af_num_cells_used). */
 long long afgpu_num_leaf_cells(afgpu_t *h);
 

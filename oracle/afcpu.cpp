@@ -2000,6 +2000,26 @@ int afcpu_step(afcpu_t *h, afgpu_step_info_t *info) {
   return step(h->c, info);
 }
 
+int afcpu_get_time_state(afcpu_t *h, int *it, double *time, double *global_dt) {
+  CHECK_H(h);
+  Ctx &c = h->c;
+  if (it) *it = c.it;
+  if (time) *time = c.time;
+  if (global_dt) *global_dt = c.started ? c.global_dt : c.p.dt_min;
+  return AFGPU_OK;
+}
+
+int afcpu_set_time_state(afcpu_t *h, int it, double time, double global_dt) {
+  CHECK_H(h);
+  if (it < 0 || !(global_dt > 0)) return AFGPU_ERR_ARG;
+  Ctx &c = h->c;
+  c.it = it;
+  c.time = c.global_time = time;
+  c.global_dt = c.dt = global_dt;
+  c.started = true;
+  return AFGPU_OK;
+}
+
 long long afcpu_num_leaf_cells(afcpu_t *h) {
   if (!h) return 0;
   long long n = 0;
