@@ -196,14 +196,25 @@ struct Impl : ImplBase {
     return AFGPU_OK;
   }
 
+  // Workgroups per box: 1 on large levels; on levels with few boxes the
+  // box's work items are spread over several workgroups.
+  static int bpb(int n_boxes, int items) {
+    if (n_boxes >= 256) return 1;
+    int b = (512 + n_boxes - 1) / n_boxes;
+    const int cap = std::max(1, items / 64);
+    return std::max(1, std::min(b, cap));
+  }
+
   int boxop(const List &l, const double *a, double *b, int op, double s = 0.0) {
-    return launch(k_boxop<G::NCELL, BS, double>, l.n, c.list(l), a, b, op, s);
+    const int k = bpb(l.n, G::NCELL);
+    return launch(k_boxop<G::NCELL, BS, double>, l.n * k, c.list(l), a, b, op, s, k);
   }
 
   int gc_lvl(int lvl, int iv, bool corners) override {
     const List &l = c.l_ids[lvl];
-    return launch(k_gc<D, NC, BS, double>, l.n, c.list(l), (const DBox *)c.d_boxes, c.cc[iv], var_methods(c, iv),
-                  (int)corners);
+    const int k = corners ? 1 : bpb(l.n, 2 * D * G::NTAN);
+    return launch(k_gc<D, NC, BS, double>, l.n * k, c.list(l), (const DBox *)c.d_boxes, c.cc[iv],
+                  var_methods(c, iv), (int)corners, k);
   }
   int gc_tree(int iv) {
     for (int lvl = 1; lvl <= c.highest_lvl; lvl++) RCHK(gc_lvl(lvl, iv, true));
@@ -216,14 +227,16 @@ struct Impl : ImplBase {
     const List &l = c.l_ids[lvl];
     const int iphi = op_phi(op);
     const double bytes = 12.0 * l.n * G::NINT;
+    const VarBc m = var_methods(c, iphi);
+    const int k = bpb(l.n, G::NINT / 2);
     for (int n = 1; n <= 2 * n_cycle; n++) {
       RCHK(timer_start(c, 0, bytes));
-      RCHK(launch(k_gsrb<D, NC, BS, double>, l.n, c.list(l), (const DBox *)c.d_boxes, c.cc[iphi],
-                  (const double *)c.cc[AFGPU_CC_RHS], n, c.ops[op].lambda2, (int)(c.coord == AFGPU_CYL)));
+      RCHK(launch(k_gsrb<D, NC, BS, double>, l.n * k, c.list(l), (const DBox *)c.d_boxes, c.cc[iphi],
+                  (const double *)c.cc[AFGPU_CC_RHS], n, c.ops[op].lambda2, (int)(c.coord == AFGPU_CYL), m, k));
       RCHK(timer_stop(c, 0));
-      RCHK(gc_lvl(lvl, iphi, !down && n == 2 * n_cycle));
     }
-    return AFGPU_OK;
+    // ghost layer after the last half-sweep (corners on the final up-sweep)
+    return gc_lvl(lvl, iphi, !down);
   }
 
   int update_coarse(int lvl, int op, bool v_cycle) {
@@ -231,21 +244,25 @@ struct Impl : ImplBase {
     const int cyl = c.coord == AFGPU_CYL;
     if (!v_cycle && lvl == c.highest_lvl) RCHK(gc_lvl(lvl, iphi, true));
     const List &l = c.l_ids[lvl];
-    RCHK(launch(k_restrict_residual<D, NC, BS, double>, l.n, c.list(l), (const DBox *)c.d_boxes, c.cc[iphi],
-                (const double *)c.cc[AFGPU_CC_RHS], c.cc[AFGPU_CC_TMP], c.ops[op].lambda2, cyl, (int)!v_cycle));
+    constexpr int NQ = G::NINT / (1 << D);
+    const int k = bpb(l.n, NQ);
+    RCHK(launch(k_restrict_residual<D, NC, BS, double>, l.n * k, c.list(l), (const DBox *)c.d_boxes, c.cc[iphi],
+                (const double *)c.cc[AFGPU_CC_RHS], c.cc[AFGPU_CC_TMP], c.ops[op].lambda2, cyl, (int)!v_cycle, k));
     RCHK(gc_lvl(lvl - 1, iphi, true));
     const List &pl = c.l_parents[lvl - 1];
-    return launch(k_coarse_rhs<D, NC, BS, double>, pl.n, c.list(pl), (const DBox *)c.d_boxes,
+    const int k2 = bpb(pl.n, G::NCELL);
+    return launch(k_coarse_rhs<D, NC, BS, double>, pl.n * k2, c.list(pl), (const DBox *)c.d_boxes,
                   (const double *)c.cc[iphi], c.cc[AFGPU_CC_RHS], c.cc[AFGPU_CC_TMP], c.ops[op].lambda2, cyl,
-                  (int)v_cycle);
+                  (int)v_cycle, k2);
   }
 
   int correct_children(int lvl_p, int op) {
     const int iphi = op_phi(op);
     RCHK(boxop(c.l_parents[lvl_p], c.cc[iphi], c.cc[AFGPU_CC_TMP], OP_SUB_FROM));
     const List &l = c.l_ids[lvl_p + 1];
-    return launch(k_prolong_add<D, NC, BS, double>, l.n, c.list(l), (const DBox *)c.d_boxes,
-                  (const double *)c.cc[AFGPU_CC_TMP], c.cc[iphi]);
+    const int k = bpb(l.n, G::NINT);
+    return launch(k_prolong_add<D, NC, BS, double>, l.n * k, c.list(l), (const DBox *)c.d_boxes,
+                  (const double *)c.cc[AFGPU_CC_TMP], c.cc[iphi], k);
   }
 
   int ensure_factors(int op) {
@@ -274,7 +291,7 @@ struct Impl : ImplBase {
     RCHK(ensure_factors(op));
     const int iphi = op_phi(op);
     const int id = c.h_lists[c.l_ids[1].off];
-    hipLaunchKernelGGL((k_coarse_solve<D, NC, BS>), dim3(1), dim3(BS), 0, c.stream, id, (const DBox *)c.d_boxes,
+    hipLaunchKernelGGL((k_coarse_solve<D, NC, 1024>), dim3(1), dim3(1024), 0, c.stream, id, (const DBox *)c.d_boxes,
                        (const double *)c.cc[AFGPU_CC_RHS], c.cc[iphi], var_methods(c, iphi), c.ops[op].lambda2,
                        (int)(c.coord == AFGPU_CYL), (const double *)c.ops[op].d_vinv,
                        (const double *)c.ops[op].d_v, (const double *)c.ops[op].d_eig);
@@ -294,14 +311,25 @@ struct Impl : ImplBase {
       RCHK(gsrb_boxes(lvl, op, false));
     }
     if (set_residual) {
+      // residual on every box; the max |residual| over leaves is reduced on
+      // the fly (read back with leaf_residual_max)
       const int cyl = c.coord == AFGPU_CYL;
+      HIPCHK(hipMemsetAsync(c.d_red, 0, sizeof(unsigned long long), c.stream));
       for (int lvl = 1; lvl <= max_lvl; lvl++) {
         const List &l = c.l_ids[lvl];
-        RCHK(launch(k_residual<D, NC, BS, double>, l.n, c.list(l), (const DBox *)c.d_boxes,
+        const int k = bpb(l.n, G::NINT);
+        RCHK(launch(k_residual<D, NC, BS, double>, l.n * k, c.list(l), (const DBox *)c.d_boxes,
                     (const double *)c.cc[op_phi(op)], (const double *)c.cc[AFGPU_CC_RHS], c.cc[AFGPU_CC_TMP],
-                    c.ops[op].lambda2, cyl));
+                    c.ops[op].lambda2, cyl, c.d_red, k));
       }
     }
+    return AFGPU_OK;
+  }
+
+  // max |tmp| over leaf interiors as reduced by the last set_residual pass
+  int leaf_residual_max(double *out) {
+    RCHK(read_red(1));
+    *out = as_double(c.h_red[0]);
     return AFGPU_OK;
   }
 
@@ -313,8 +341,9 @@ struct Impl : ImplBase {
       for (int lvl = c.highest_lvl; lvl >= 2; lvl--) {  // init_phi_rhs
         const List &l = c.l_ids[lvl];
         RCHK(boxop(l, c.cc[iphi], c.cc[iphi], OP_CLEAR));
-        RCHK(launch(k_restrict<D, NC, BS, double>, l.n, c.list(l), (const DBox *)c.d_boxes,
-                    (const double *)c.cc[AFGPU_CC_RHS], c.cc[AFGPU_CC_RHS], 1, (int)(c.coord == AFGPU_CYL)));
+        const int k = bpb(l.n, G::NINT / (1 << D));
+        RCHK(launch(k_restrict<D, NC, BS, double>, l.n * k, c.list(l), (const DBox *)c.d_boxes,
+                    (const double *)c.cc[AFGPU_CC_RHS], c.cc[AFGPU_CC_RHS], 1, (int)(c.coord == AFGPU_CYL), k));
       }
     }
     RCHK(boxop(c.l_ids[1], c.cc[iphi], c.cc[AFGPU_CC_TMP], OP_COPY));
@@ -341,7 +370,9 @@ struct Impl : ImplBase {
 
   int maxabs(int iv, double *out) override {
     HIPCHK(hipMemsetAsync(c.d_red, 0, sizeof(unsigned long long), c.stream));
-    RCHK(launch(k_maxabs<D, NC, BS>, c.l_all_leaves.n, c.list(c.l_all_leaves), (const double *)c.cc[iv], c.d_red));
+    const int k = bpb(c.l_all_leaves.n, G::NINT);
+    RCHK(launch(k_maxabs<D, NC, BS>, c.l_all_leaves.n * k, c.list(c.l_all_leaves), (const double *)c.cc[iv], c.d_red,
+                k));
     RCHK(read_red(1));
     *out = c.l_all_leaves.n > 0 ? as_double(c.h_red[0]) : -std::numeric_limits<double>::max() / 10;
     return AFGPU_OK;
@@ -377,7 +408,7 @@ struct Impl : ImplBase {
       int i;
       for (i = 1; i <= 100; i++) {
         RCHK(fmg(AFGPU_OP_POISSON, true, true));
-        RCHK(maxabs(AFGPU_CC_TMP, &res[i]));
+        RCHK(leaf_residual_max(&res[i]));
         count++;
         last = res[i];
         if (res[i] < threshold) break;
@@ -398,7 +429,7 @@ struct Impl : ImplBase {
     } else {
       for (int i = 1; i <= c.p.mg_num_vcycles; i++) {
         RCHK(vcycle(AFGPU_OP_POISSON, true, c.highest_lvl));
-        RCHK(maxabs(AFGPU_CC_TMP, &last));
+        RCHK(leaf_residual_max(&last));
         count++;
         if (last < threshold) break;
       }
@@ -425,7 +456,7 @@ struct Impl : ImplBase {
     const int iv_e = AFGPU_CC_ELECTRON + s_deriv;
     // af_restrict_ref_boundary (afivo/src/m_af_restrict.f90:140-161)
     RCHK(launch(k_restrict<D, NC, BS, double>, c.l_refb.n, c.list(c.l_refb), (const DBox *)c.d_boxes,
-                (const double *)c.cc[iv_e], c.cc[iv_e], 1, cyl));
+                (const double *)c.cc[iv_e], c.cc[iv_e], 1, cyl, 1));
     HIPCHK(hipMemsetAsync(c.d_red + 1, 0, 2 * sizeof(unsigned long long), c.stream));
     const double big = 1e100;
     unsigned long long bigbits;
@@ -502,7 +533,7 @@ struct Impl : ImplBase {
       for (int i = 1; i <= c.p.photoi_max_fmg_cycles; i++) {
         RCHK(fmg(op, true, true));
         double residu;
-        RCHK(maxabs(AFGPU_CC_TMP, &residu));
+        RCHK(leaf_residual_max(&residu));
         if (residu / max_rhs < c.p.photoi_max_rel_residual) break;
       }
       RCHK(boxop(c.l_all_leaves, c.cc[op_phi(op)], c.cc[AFGPU_CC_PHOTO], OP_AXPY_NEG, c.p.photoi_coeff[n]));
@@ -532,7 +563,7 @@ struct Impl : ImplBase {
     for (int lvl = c.highest_lvl; lvl >= 2; lvl--) {
       const List &l = c.l_ids[lvl];
       RCHK(launch(k_restrict<D, NC, BS, double>, l.n, c.list(l), (const DBox *)c.d_boxes, (const double *)c.cc[iv],
-                  c.cc[iv], 1, (int)(c.coord == AFGPU_CYL)));
+                  c.cc[iv], 1, (int)(c.coord == AFGPU_CYL), 1));
     }
     return AFGPU_OK;
   }

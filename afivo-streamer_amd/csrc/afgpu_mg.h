@@ -1,16 +1,34 @@
 // Multigrid and ghost-cell kernels (afivo/src/m_af_multigrid.f90,
 // m_af_stencil.f90, m_af_ghostcell.f90, m_af_restrict.f90).
 //
-// One workgroup per box of a box list; every cell value is computed with
-// the reference's operation order, so the FP64 instantiation reproduces the
-// CPU path bit for bit (hipcc is run with -ffp-contract=off).  The same
-// templates instantiated with T = float form the inner cycle of the
-// mixed-precision solver.
+// Work decomposition: a launch covers a box list; each box is handled by
+// `bpb` workgroups ("blocks per box", > 1 on the coarse levels that have few
+// boxes, so that those launches still spread over the CUs).  Every cell
+// value is computed with the reference's operation order, so the FP64
+// instantiation reproduces the CPU path bit for bit (hipcc is run with
+// -ffp-contract=off).  The same templates instantiated with T = float form
+// the inner cycle of the mixed-precision solver.
 #pragma once
 
 #include "afgpu_common.h"
 
 namespace afg {
+
+// Box of this workgroup and the first work item / stride of this thread.
+struct Part {
+  int id;
+  int q0;
+  int stride;
+};
+template <int BS>
+__device__ inline Part part_of(const int *__restrict__ ids, int bpb) {
+  Part p;
+  const int slot = blockIdx.x / bpb;
+  p.id = ids[slot];
+  p.q0 = (blockIdx.x - slot * bpb) * BS + threadIdx.x;
+  p.stride = bpb * BS;
+  return p;
+}
 
 // Constant 3/5/7-point stencil of a box (mg_box_lpl_stencil,
 // afivo/src/m_af_multigrid.f90:1246-1264).
@@ -74,61 +92,302 @@ __device__ inline void interior_ijk(int q, int &i, int &j, int &k) {
   k = (D == 3) ? q / (NC * NC) + 1 : 0;
 }
 
-// Red-black Gauss-Seidel half-sweep (stencil_gsrb_357, m_af_stencil.f90:838-998)
+// ---- ghost cells --------------------------------------------------------------
+// Value of the ghost cell of face nb at tangential position (a, b) of box
+// `id`, as af_gc_box (afivo/src/m_af_ghostcell.f90:64-120) sets it: copy of
+// the same-level neighbour (copy_from_nb :654), physical boundary (bc_to_gc
+// :173-279), or refinement boundary (mg_sides_rb, m_af_multigrid.f90:294-461,
+// or af_gc_interp / af_gc_interp_lim, m_af_ghostcell.f90:394-605).  x1 and x2
+// are the box's first and second interior cells along the face normal.
+template <int D, int NC, typename T>
+__device__ inline T ghost_value(const DBox *__restrict__ boxes, const DBox &bx, const T *var, const VarBc &m, int nb,
+                                int a, int b, T x1, T x2) {
+  using G = Geo<D, NC>;
+  constexpr int H = NC / 2;
+  const int d = nb / 2;
+  const bool low = (nb % 2) == 0;
+  int t0, t1;
+  tangential(D, d, t0, t1);
+  const int nb_id = bx.nb[nb];
+  if (nb_id > 0) {
+    const T *w = var + (size_t)(nb_id - 1) * G::NCELL;
+    int q[3] = {0, 0, 0};
+    q[t0] = a;
+    if (D == 3) q[t1] = b;
+    q[d] = low ? NC : 1;
+    return w[G::idx(q[0], q[1], q[2])];
+  }
+  if (nb_id < 0) {
+    T c0, c1, c2;
+    if (m.type[nb] == BC_DIRICHLET) {
+      c0 = 2; c1 = -1; c2 = 0;
+    } else if (m.type[nb] == BC_NEUMANN) {
+      c0 = (T)bx.dr[d] * T(low ? -1 : 1); c1 = 1; c2 = 0;
+    } else {
+      c0 = 1; c1 = 0; c2 = 0;
+    }
+    return c0 * (T)m.val[nb] + c1 * x1 + c2 * x2;
+  }
+  const int p_nb_id = boxes[bx.parent - 1].nb[nb];
+  const T *P = var + (size_t)(p_nb_id - 1) * G::NCELL;
+  int co[3] = {0, 0, 0};
+  for (int dd = 0; dd < D; dd++) co[dd] = ((bx.ix[dd] - 1) & 1) * H;
+  if (m.rb == RB_MG) {
+    const int cn = low ? NC : 1;
+    T gcv;
+    if (D == 2) {
+      const int i = (a + 1) >> 1;
+      int q[3] = {0, 0, 0};
+      q[d] = cn;
+      q[t0] = co[t0] + i + 1;
+      const T tp = P[G::idx(q[0], q[1], q[2])];
+      q[t0] = co[t0] + i - 1;
+      const T tm = P[G::idx(q[0], q[1], q[2])];
+      q[t0] = co[t0] + i;
+      const T tc = P[G::idx(q[0], q[1], q[2])];
+      const T grad = T(0.125) * (tp - tm);
+      gcv = (a & 1) ? tc - grad : tc + grad;
+    } else {
+      const int i = (a + 1) >> 1, j = (b + 1) >> 1;
+      auto TT = [&](int qa, int qb) {
+        int q[3];
+        q[d] = cn;
+        q[t0] = co[t0] + qa;
+        q[t1] = co[t1] + qb;
+        return P[G::idx(q[0], q[1], q[2])];
+      };
+      const T g1 = T(0.125) * (TT(i + 1, j) - TT(i - 1, j));
+      const T g2 = T(0.125) * (TT(i, j + 1) - TT(i, j - 1));
+      const T tc = TT(i, j);
+      const T h = (a & 1) ? tc - g1 : tc + g1;
+      gcv = (b & 1) ? h - g2 : h + g2;
+    }
+    return T(0.5) * gcv + T(0.75) * x1 - T(0.25) * x2;
+  }
+  const T third = T(1 / 3.0), sixth = T(1 / 6.0);
+  const int ix_c = low ? NC : 1;
+  const bool limited = (m.rb == RB_INTERP_LIM);
+  T v;
+  if (D == 2) {
+    const int a1 = co[t0] + ((a + 1) >> 1), a2 = a1 + 1 - 2 * (a & 1);
+    int q[3] = {0, 0, 0};
+    q[d] = ix_c;
+    q[t0] = a1;
+    const T c1 = P[G::idx(q[0], q[1], q[2])];
+    q[t0] = a2;
+    const T c2 = P[G::idx(q[0], q[1], q[2])];
+    v = T(0.5) * c1 + sixth * c2 + third * x1;
+    if (limited && v > 2 * c1) v = 2 * c1;
+  } else {
+    const int a1 = co[t0] + ((a + 1) >> 1), a2 = a1 + 1 - 2 * (a & 1);
+    const int b1 = co[t1] + ((b + 1) >> 1), b2 = b1 + 1 - 2 * (b & 1);
+    int q[3];
+    q[d] = ix_c;
+    q[t0] = a1; q[t1] = b1;
+    const T c1 = P[G::idx(q[0], q[1], q[2])];
+    q[t0] = a2; q[t1] = b1;
+    const T ca = P[G::idx(q[0], q[1], q[2])];
+    q[t0] = a1; q[t1] = b2;
+    const T cb = P[G::idx(q[0], q[1], q[2])];
+    // the z-face case sums the off-diagonal coarse values in the other order
+    const T c2 = (d == 2) ? cb : ca, c3 = (d == 2) ? ca : cb;
+    v = third * c1 + sixth * c2 + sixth * c3 + third * x1;
+    if (limited && v > 2 * c1) v = 2 * c1;
+  }
+  return v;
+}
+
+// Position of face cell (nb, a, b) at normal index p.
+template <int D>
+__device__ inline void face_pos(int nb, int a, int b, int pn, int *p) {
+  const int d = nb / 2;
+  int t0, t1;
+  tangential(D, d, t0, t1);
+  p[0] = p[1] = p[2] = 0;
+  p[t0] = a;
+  if (D == 3) p[t1] = b;
+  p[d] = pn;
+}
+
+// Corner (2D) / edge + corner (3D) ghost cells (af_gc_box_corner :125-170,
+// af_edge_gc_extrap :885-924, af_corner_gc_extrap :860-879).
+__constant__ int c_edge_dim3[12] = {0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 2};
+__constant__ int c_edge_dir3[12][3] = {{0, -1, -1}, {0, 1, -1}, {0, -1, 1}, {0, 1, 1},
+                                       {-1, 0, -1}, {1, 0, -1}, {-1, 0, 1}, {1, 0, 1},
+                                       {-1, -1, 0}, {1, -1, 0}, {-1, 1, 0}, {1, 1, 0}};
+
+template <int D, int NC, typename T>
+__device__ inline void gc_edge_cell(const DBox &bx, T *var, int id, int n, int qq) {
+  using G = Geo<D, NC>;
+  T *u = var + (size_t)(id - 1) * G::NCELL;
+  const int dim = c_edge_dim3[n];
+  const int *dir = c_edge_dir3[n];
+  int lo[3];
+  for (int dd = 0; dd < 3; dd++) lo[dd] = (dir[dd] > 0) ? NC + 1 : 0;
+  lo[dim] = qq;
+  const int nb_id = bx.nbmat[(dir[0] + 1) + 3 * (dir[1] + 1) + 9 * (dir[2] + 1)];
+  if (nb_id > 0) {
+    const T *w = var + (size_t)(nb_id - 1) * G::NCELL;
+    u[G::idx(lo[0], lo[1], lo[2])] = w[G::idx(lo[0] - dir[0] * NC, lo[1] - dir[1] * NC, lo[2] - dir[2] * NC)];
+  } else {
+    const int o0 = (dim + 1) % 3, o1 = (dim + 2) % 3;
+    int di[3];
+    for (int dd = 0; dd < 3; dd++) di[dd] = 1 - 2 * (lo[dd] & 1);
+    di[dim] = 0;
+    int ia[3] = {lo[0], lo[1], lo[2]}, ib[3] = {lo[0], lo[1], lo[2]};
+    ia[o0] += di[o0];
+    ib[o1] += di[o1];
+    u[G::idx(lo[0], lo[1], lo[2])] = u[G::idx(ia[0], ia[1], ia[2])] + u[G::idx(ib[0], ib[1], ib[2])] -
+                                     u[G::idx(lo[0] + di[0], lo[1] + di[1], lo[2] + di[2])];
+  }
+}
+
+template <int D, int NC, typename T>
+__device__ inline void gc_corner_cell(const DBox &bx, T *var, int id, int n) {
+  using G = Geo<D, NC>;
+  T *u = var + (size_t)(id - 1) * G::NCELL;
+  const int cx = n & 1, cy = (n >> 1) & 1, cz = (n >> 2) & 1;
+  const int dnb[3] = {2 * cx - 1, 2 * cy - 1, (D == 3) ? 2 * cz - 1 : 0};
+  const int lo[3] = {cx * (NC + 1), cy * (NC + 1), (D == 3) ? cz * (NC + 1) : 0};
+  const int nb_id = bx.nbmat[(dnb[0] + 1) + 3 * (dnb[1] + 1) + 9 * (dnb[2] + 1)];
+  if (nb_id > 0) {
+    const T *w = var + (size_t)(nb_id - 1) * G::NCELL;
+    u[G::idx(lo[0], lo[1], lo[2])] = w[G::idx(lo[0] - dnb[0] * NC, lo[1] - dnb[1] * NC, lo[2] - dnb[2] * NC)];
+  } else if (D == 2) {
+    const int di = 1 - 2 * (lo[0] & 1), dj = 1 - 2 * (lo[1] & 1);
+    u[G::idx(lo[0], lo[1], 0)] =
+        u[G::idx(lo[0] + di, lo[1], 0)] + u[G::idx(lo[0], lo[1] + dj, 0)] - u[G::idx(lo[0] + di, lo[1] + dj, 0)];
+  } else {
+    const int di = 1 - 2 * (lo[0] & 1), dj = 1 - 2 * (lo[1] & 1), dk = 1 - 2 * (lo[2] & 1);
+    const int x = lo[0], y = lo[1], z = lo[2];
+    u[G::idx(x, y, z)] = u[G::idx(x, y + dj, z + dk)] + u[G::idx(x + di, y, z + dk)] +
+                         u[G::idx(x + di, y + dj, z)] - 2 * u[G::idx(x + di, y + dj, z + dk)];
+  }
+}
+
+// af_gc_box for all boxes of a list.  Corners need the face ghosts of the
+// same box, so with corners the box must be handled by one workgroup
+// (bpb = 1); face-only fills may be split over several workgroups.
+template <int D, int NC, int BS, typename T>
+__global__ void __launch_bounds__(BS) k_gc(const int *__restrict__ ids, const DBox *__restrict__ boxes, T *var,
+                                           VarBc m, int corners, int bpb) {
+  using G = Geo<D, NC>;
+  const Part pt = part_of<BS>(ids, bpb);
+  const int id = pt.id;
+  const DBox &bx = boxes[id - 1];
+  T *u = var + (size_t)(id - 1) * G::NCELL;
+  constexpr int NFC = 2 * D * G::NTAN;
+  for (int q = pt.q0; q < NFC; q += pt.stride) {
+    const int nb = q / G::NTAN, r = q % G::NTAN;
+    const int a = r % NC + 1, b = (D == 3) ? r / NC + 1 : 0;
+    const bool low = (nb % 2) == 0;
+    int p[3];
+    face_pos<D>(nb, a, b, low ? 1 : NC, p);
+    const T x1 = u[G::idx(p[0], p[1], p[2])];
+    face_pos<D>(nb, a, b, low ? 2 : NC - 1, p);
+    const T x2 = u[G::idx(p[0], p[1], p[2])];
+    const T g = ghost_value<D, NC, T>(boxes, bx, var, m, nb, a, b, x1, x2);
+    face_pos<D>(nb, a, b, low ? 0 : NC + 1, p);
+    u[G::idx(p[0], p[1], p[2])] = g;
+  }
+  if (!corners) return;
+  __syncthreads();
+  if (D == 3) {
+    for (int q = threadIdx.x; q < 12 * NC; q += BS) gc_edge_cell<D, NC, T>(bx, var, id, q / NC, q % NC + 1);
+    __syncthreads();
+  }
+  for (int q = threadIdx.x; q < (1 << D); q += BS) gc_corner_cell<D, NC, T>(bx, var, id, q);
+}
+
+// Red-black Gauss-Seidel half-sweep (stencil_gsrb_357,
+// m_af_stencil.f90:838-998) fused with the ghost-cell fill that the
+// reference performs before it (gsrb_boxes, m_af_multigrid.f90:648-687):
+// a neighbour outside the box is taken from ghost_value() with the same
+// inputs af_gc_box would use, so the updated cells are bit-identical while
+// no separate ghost-fill pass is needed between half-sweeps.  (The stored
+// ghost layer is refreshed once after the last half-sweep.)
 template <int D, int NC, int BS, typename T>
 __global__ void __launch_bounds__(BS) k_gsrb(const int *__restrict__ ids, const DBox *__restrict__ boxes, T *phi,
-                                             const T *__restrict__ rhs, int redblack, T lambda2, int cyl) {
+                                             const T *__restrict__ rhs, int redblack, T lambda2, int cyl, VarBc m,
+                                             int bpb) {
   using G = Geo<D, NC>;
-  const int id = ids[blockIdx.x];
+  const Part pt = part_of<BS>(ids, bpb);
+  const int id = pt.id;
   const DBox &bx = boxes[id - 1];
   const Sten<D, T> s = lpl_stencil<D, T>(bx, lambda2);
   T *u = phi + (size_t)(id - 1) * G::NCELL;
   const T *r = rhs + (size_t)(id - 1) * G::NCELL;
   constexpr int HALF = G::NINT / 2;
-  for (int q = threadIdx.x; q < HALF; q += BS) {
-    const int m = q % (NC / 2);
+  constexpr int st[3] = {1, G::NG, G::NG * G::NG};
+  for (int q = pt.q0; q < HALF; q += pt.stride) {
+    const int mm = q % (NC / 2);
     const int j = (q / (NC / 2)) % NC + 1;
     const int k = (D == 3) ? q / ((NC / 2) * NC) + 1 : 0;
     const int i0 = 2 - ((redblack ^ (D == 3 ? k + j : j)) & 1);
-    const int i = i0 + 2 * m;
+    const int i = i0 + 2 * mm;
     const int x = G::idx(i, j, k);
+    const int pos[3] = {i, j, k};
+    const T uc = u[x];
+    T nbv[6];
+    for (int nb = 0; nb < 2 * D; nb++) {
+      const int d = nb / 2;
+      const bool low = (nb % 2) == 0;
+      const bool edge = low ? pos[d] == 1 : pos[d] == NC;
+      if (!edge) {
+        nbv[nb] = u[low ? x - st[d] : x + st[d]];
+      } else {
+        int t0, t1;
+        tangential(D, d, t0, t1);
+        const T x2 = u[low ? x + st[d] : x - st[d]];
+        nbv[nb] = ghost_value<D, NC, T>(boxes, bx, phi, m, nb, pos[t0], (D == 3) ? pos[t1] : 0, uc, x2);
+      }
+    }
     if (D == 2) {
       if (cyl) {
         T c1, c2, c3;
         cyl_coeffs<T>(bx, i, s.c, c1, c2, c3);
         T inv1 = T(1) / c1;
-        u[x] = (r[x] - c2 * u[x - 1] - c3 * u[x + 1] - s.c[3] * u[x - G::NG] - s.c[4] * u[x + G::NG]) * inv1;
+        u[x] = (r[x] - c2 * nbv[0] - c3 * nbv[1] - s.c[3] * nbv[2] - s.c[4] * nbv[3]) * inv1;
       } else {
         T inv_c1 = T(1) / s.c[0];
-        u[x] = (r[x] - s.c[1] * u[x - 1] - s.c[2] * u[x + 1] - s.c[3] * u[x - G::NG] - s.c[4] * u[x + G::NG]) *
-               inv_c1;
+        u[x] = (r[x] - s.c[1] * nbv[0] - s.c[2] * nbv[1] - s.c[3] * nbv[2] - s.c[4] * nbv[3]) * inv_c1;
       }
     } else {
-      constexpr int NG2 = G::NG * G::NG;
       T inv_c1 = T(1) / s.c[0];
-      u[x] = (r[x] - s.c[1] * u[x - 1] - s.c[2] * u[x + 1] - s.c[3] * u[x - G::NG] - s.c[4] * u[x + G::NG] -
-              s.c[5] * u[x - NG2] - s.c[6] * u[x + NG2]) *
+      u[x] = (r[x] - s.c[1] * nbv[0] - s.c[2] * nbv[1] - s.c[3] * nbv[2] - s.c[4] * nbv[3] - s.c[5] * nbv[4] -
+              s.c[6] * nbv[5]) *
              inv_c1;
     }
   }
 }
 
-// tmp = rhs - L phi on interior cells (residual_box, m_af_multigrid.f90:801-810)
+// tmp = rhs - L phi on interior cells (residual_box, m_af_multigrid.f90:801-810);
+// for leaf boxes the max |tmp| is also reduced into *leafmax
+// (af_tree_maxabs_cc, afivo/src/m_af_utils.f90:773).
 template <int D, int NC, int BS, typename T>
 __global__ void __launch_bounds__(BS) k_residual(const int *__restrict__ ids, const DBox *__restrict__ boxes,
                                                  const T *__restrict__ phi, const T *__restrict__ rhs, T *tmp,
-                                                 T lambda2, int cyl) {
+                                                 T lambda2, int cyl, unsigned long long *leafmax, int bpb) {
   using G = Geo<D, NC>;
-  const int id = ids[blockIdx.x];
+  __shared__ double sh[BS / 64];
+  const Part pt = part_of<BS>(ids, bpb);
+  const int id = pt.id;
   const DBox &bx = boxes[id - 1];
   const Sten<D, T> s = lpl_stencil<D, T>(bx, lambda2);
   const size_t base = (size_t)(id - 1) * G::NCELL;
-  for (int q = threadIdx.x; q < G::NINT; q += BS) {
+  double mx = 0;
+  for (int q = pt.q0; q < G::NINT; q += pt.stride) {
     int i, j, k;
     interior_ijk<D, NC>(q, i, j, k);
     const int x = G::idx(i, j, k);
     T lp = apply_cell<D, NC, T>(phi + base, s, cyl != 0, bx, i, j, k);
-    tmp[base + x] = rhs[base + x] - lp;
+    const T rv = rhs[base + x] - lp;
+    tmp[base + x] = rv;
+    mx = fmax(mx, (double)fabs(rv));
+  }
+  if (leafmax && bx.children[0] == 0) {
+    mx = block_max<BS>(mx, sh);
+    if (threadIdx.x == 0) atomic_max_nonneg(leafmax, mx);
   }
 }
 
@@ -166,18 +425,19 @@ __device__ inline T restrict_cell(const T *f, int i_f, int jf, int kf, bool cyl_
 // Restrict variable `src` of child boxes into `dst` of their parents.
 template <int D, int NC, int BS, typename T>
 __global__ void __launch_bounds__(BS) k_restrict(const int *__restrict__ ids, const DBox *__restrict__ boxes,
-                                                 const T *src, T *dst, int use_geom, int cyl) {
+                                                 const T *src, T *dst, int use_geom, int cyl, int bpb) {
   using G = Geo<D, NC>;
   constexpr int H = NC / 2;
   constexpr int NQ = (D == 3) ? H * H * H : H * H;
-  const int id = ids[blockIdx.x];
+  const Part pt = part_of<BS>(ids, bpb);
+  const int id = pt.id;
   const DBox &bc = boxes[id - 1];
   const DBox &bp = boxes[bc.parent - 1];
   int off[3] = {0, 0, 0};
   for (int d = 0; d < D; d++) off[d] = ((bc.ix[d] - 1) & 1) * H;
   const T *f = src + (size_t)(id - 1) * G::NCELL;
   T *u = dst + (size_t)(bc.parent - 1) * G::NCELL;
-  for (int q = threadIdx.x; q < NQ; q += BS) {
+  for (int q = pt.q0; q < NQ; q += pt.stride) {
     const int i = q % H + 1, j = (q / H) % H + 1, k = (D == 3) ? q / (H * H) + 1 : 0;
     const int ic = off[0] + i, jc = off[1] + j, kc = (D == 3) ? off[2] + k : 0;
     u[G::idx(ic, jc, kc)] =
@@ -194,18 +454,19 @@ __global__ void __launch_bounds__(BS) k_restrict(const int *__restrict__ ids, co
 template <int D, int NC, int BS, typename T>
 __global__ void __launch_bounds__(BS) k_restrict_residual(const int *__restrict__ ids, const DBox *__restrict__ boxes,
                                                           T *phi, const T *__restrict__ rhs, T *tmp, T lambda2,
-                                                          int cyl, int store_residual) {
+                                                          int cyl, int store_residual, int bpb) {
   using G = Geo<D, NC>;
   constexpr int H = NC / 2;
   constexpr int NQ = (D == 3) ? H * H * H : H * H;
-  const int id = ids[blockIdx.x];
+  const Part pt = part_of<BS>(ids, bpb);
+  const int id = pt.id;
   const DBox &bc = boxes[id - 1];
   const DBox &bp = boxes[bc.parent - 1];
   const Sten<D, T> s = lpl_stencil<D, T>(bc, lambda2);
   const size_t cb = (size_t)(id - 1) * G::NCELL, pb = (size_t)(bc.parent - 1) * G::NCELL;
   int off[3] = {0, 0, 0};
   for (int d = 0; d < D; d++) off[d] = ((bc.ix[d] - 1) & 1) * H;
-  for (int q = threadIdx.x; q < NQ; q += BS) {
+  for (int q = pt.q0; q < NQ; q += pt.stride) {
     const int i = q % H + 1, j = (q / H) % H + 1, k = (D == 3) ? q / (H * H) + 1 : 0;
     const int i_f = 2 * i - 1, jf = 2 * j - 1, kf = (D == 3) ? 2 * k - 1 : 0;
     // residuals of the 2^D children cells, in the summation order of
@@ -248,13 +509,14 @@ __global__ void __launch_bounds__(BS) k_restrict_residual(const int *__restrict_
 template <int D, int NC, int BS, typename T>
 __global__ void __launch_bounds__(BS) k_coarse_rhs(const int *__restrict__ ids, const DBox *__restrict__ boxes,
                                                    const T *__restrict__ phi, T *rhs, T *tmp, T lambda2, int cyl,
-                                                   int copy_phi) {
+                                                   int copy_phi, int bpb) {
   using G = Geo<D, NC>;
-  const int id = ids[blockIdx.x];
+  const Part pt = part_of<BS>(ids, bpb);
+  const int id = pt.id;
   const DBox &bx = boxes[id - 1];
   const Sten<D, T> s = lpl_stencil<D, T>(bx, lambda2);
   const size_t base = (size_t)(id - 1) * G::NCELL;
-  for (int x = threadIdx.x; x < G::NCELL; x += BS) {
+  for (int x = pt.q0; x < G::NCELL; x += pt.stride) {
     const int i = x % G::NG, j = (x / G::NG) % G::NG, k = (D == 3) ? x / (G::NG * G::NG) : 0;
     const bool inner = i >= 1 && i <= NC && j >= 1 && j <= NC && (D == 2 || (k >= 1 && k <= NC));
     T rv = inner ? apply_cell<D, NC, T>(phi + base, s, cyl != 0, bx, i, j, k) : rhs[base + x];
@@ -267,9 +529,10 @@ __global__ void __launch_bounds__(BS) k_coarse_rhs(const int *__restrict__ ids, 
 enum BoxOp { OP_COPY = 0, OP_CLEAR = 1, OP_SUB_FROM = 2, OP_AXPY_NEG = 3 };
 // OP_COPY: b = a; OP_CLEAR: b = 0; OP_SUB_FROM: b = a - b; OP_AXPY_NEG: b = b - s*a
 template <int NCELL, int BS, typename T>
-__global__ void __launch_bounds__(BS) k_boxop(const int *__restrict__ ids, const T *a, T *b, int op, T s) {
-  const size_t base = (size_t)(ids[blockIdx.x] - 1) * NCELL;
-  for (int x = threadIdx.x; x < NCELL; x += BS) {
+__global__ void __launch_bounds__(BS) k_boxop(const int *__restrict__ ids, const T *a, T *b, int op, T s, int bpb) {
+  const Part pt = part_of<BS>(ids, bpb);
+  const size_t base = (size_t)(pt.id - 1) * NCELL;
+  for (int x = pt.q0; x < NCELL; x += pt.stride) {
     switch (op) {
       case OP_COPY: b[base + x] = a[base + x]; break;
       case OP_CLEAR: b[base + x] = T(0); break;
@@ -284,16 +547,17 @@ __global__ void __launch_bounds__(BS) k_boxop(const int *__restrict__ ids, const
 // mg_box_prolong_linear_stencil (m_af_multigrid.f90:1267-1284).
 template <int D, int NC, int BS, typename T>
 __global__ void __launch_bounds__(BS) k_prolong_add(const int *__restrict__ ids, const DBox *__restrict__ boxes,
-                                                    const T *__restrict__ src, T *dst) {
+                                                    const T *__restrict__ src, T *dst, int bpb) {
   using G = Geo<D, NC>;
   constexpr int H = NC / 2;
-  const int id = ids[blockIdx.x];
+  const Part pt = part_of<BS>(ids, bpb);
+  const int id = pt.id;
   const DBox &bc = boxes[id - 1];
   const T *P = src + (size_t)(bc.parent - 1) * G::NCELL;
   T *u = dst + (size_t)(id - 1) * G::NCELL;
   int off[3] = {0, 0, 0};
   for (int d = 0; d < D; d++) off[d] = ((bc.ix[d] - 1) & 1) * H;
-  for (int q = threadIdx.x; q < G::NINT; q += BS) {
+  for (int q = pt.q0; q < G::NINT; q += pt.stride) {
     int i, j, k;
     interior_ijk<D, NC>(q, i, j, k);
     const int i1 = off[0] + ((i + 1) >> 1), i2 = i1 + 1 - 2 * (i & 1);
@@ -313,220 +577,17 @@ __global__ void __launch_bounds__(BS) k_prolong_add(const int *__restrict__ ids,
   }
 }
 
-// ---- ghost cells --------------------------------------------------------------
-// Ghost value of face nb at tangential position (a, b) of box `id`
-// (af_gc_box, afivo/src/m_af_ghostcell.f90:64-120).
-template <int D, int NC, typename T>
-__device__ inline void gc_face_cell(const DBox *__restrict__ boxes, int id, T *var, const VarBc &m, int nb, int a,
-                                    int b) {
-  using G = Geo<D, NC>;
-  constexpr int H = NC / 2;
-  const DBox &bx = boxes[id - 1];
-  T *u = var + (size_t)(id - 1) * G::NCELL;
-  const int d = nb / 2;
-  const bool low = (nb % 2) == 0;
-  int t0, t1;
-  tangential(D, d, t0, t1);
-  int p[3] = {0, 0, 0};
-  p[t0] = a;
-  if (D == 3) p[t1] = b;
-  const int g = low ? 0 : NC + 1;
-  const int nb_id = bx.nb[nb];
-  if (nb_id > 0) {  // copy_from_nb (:654-669)
-    const T *w = var + (size_t)(nb_id - 1) * G::NCELL;
-    int q[3] = {p[0], p[1], p[2]};
-    q[d] = low ? NC : 1;
-    p[d] = g;
-    u[G::idx(p[0], p[1], p[2])] = w[G::idx(q[0], q[1], q[2])];
-    return;
-  }
-  if (nb_id < 0) {  // bc_to_gc (:173-279)
-    T c0, c1, c2;
-    if (m.type[nb] == BC_DIRICHLET) {
-      c0 = 2; c1 = -1; c2 = 0;
-    } else if (m.type[nb] == BC_NEUMANN) {
-      c0 = (T)bx.dr[d] * T(low ? -1 : 1); c1 = 1; c2 = 0;
-    } else {
-      c0 = 1; c1 = 0; c2 = 0;
-    }
-    const T bv = (T)m.val[nb];
-    p[d] = low ? 1 : NC;
-    const T x1 = u[G::idx(p[0], p[1], p[2])];
-    p[d] = low ? 2 : NC - 1;
-    const T x2 = u[G::idx(p[0], p[1], p[2])];
-    p[d] = g;
-    u[G::idx(p[0], p[1], p[2])] = c0 * bv + c1 * x1 + c2 * x2;
-    return;
-  }
-  // refinement boundary
-  const int p_nb_id = boxes[bx.parent - 1].nb[nb];
-  const T *P = var + (size_t)(p_nb_id - 1) * G::NCELL;
-  int co[3] = {0, 0, 0};
-  for (int dd = 0; dd < D; dd++) co[dd] = ((bx.ix[dd] - 1) & 1) * H;
-  if (m.rb == RB_MG) {  // mg_sides_rb (m_af_multigrid.f90:294-461)
-    const int cn = low ? NC : 1;
-    T gcv;
-    if (D == 2) {
-      const int i = (a + 1) >> 1;
-      int q[3] = {0, 0, 0};
-      q[d] = cn;
-      q[t0] = co[t0] + i + 1;
-      T tp = P[G::idx(q[0], q[1], q[2])];
-      q[t0] = co[t0] + i - 1;
-      T tm = P[G::idx(q[0], q[1], q[2])];
-      q[t0] = co[t0] + i;
-      T tc = P[G::idx(q[0], q[1], q[2])];
-      T grad = T(0.125) * (tp - tm);
-      gcv = (a & 1) ? tc - grad : tc + grad;
-    } else {
-      const int i = (a + 1) >> 1, j = (b + 1) >> 1;
-      auto TT = [&](int qa, int qb) {
-        int q[3];
-        q[d] = cn;
-        q[t0] = co[t0] + qa;
-        q[t1] = co[t1] + qb;
-        return P[G::idx(q[0], q[1], q[2])];
-      };
-      T g1 = T(0.125) * (TT(i + 1, j) - TT(i - 1, j));
-      T g2 = T(0.125) * (TT(i, j + 1) - TT(i, j - 1));
-      T tc = TT(i, j);
-      T h = (a & 1) ? tc - g1 : tc + g1;
-      gcv = (b & 1) ? h - g2 : h + g2;
-    }
-    const int ix = low ? 1 : NC, di = low ? 1 : -1;
-    p[d] = ix;
-    const T x1 = u[G::idx(p[0], p[1], p[2])];
-    p[d] = ix + di;
-    const T x2 = u[G::idx(p[0], p[1], p[2])];
-    p[d] = ix - di;
-    u[G::idx(p[0], p[1], p[2])] = T(0.5) * gcv + T(0.75) * x1 - T(0.25) * x2;
-    return;
-  }
-  // af_gc_interp / af_gc_interp_lim (m_af_ghostcell.f90:394-605)
-  const T third = T(1 / 3.0), sixth = T(1 / 6.0);
-  const int ix_f = low ? 1 : NC, ix_c = low ? NC : 1;
-  const bool limited = (m.rb == RB_INTERP_LIM);
-  p[d] = ix_f;
-  const T f = u[G::idx(p[0], p[1], p[2])];
-  p[d] = g;
-  T v;
-  if (D == 2) {
-    const int a1 = co[t0] + ((a + 1) >> 1), a2 = a1 + 1 - 2 * (a & 1);
-    int q[3] = {0, 0, 0};
-    q[d] = ix_c;
-    q[t0] = a1;
-    const T c1 = P[G::idx(q[0], q[1], q[2])];
-    q[t0] = a2;
-    const T c2 = P[G::idx(q[0], q[1], q[2])];
-    v = T(0.5) * c1 + sixth * c2 + third * f;
-    if (limited && v > 2 * c1) v = 2 * c1;
-  } else {
-    const int a1 = co[t0] + ((a + 1) >> 1), a2 = a1 + 1 - 2 * (a & 1);
-    const int b1 = co[t1] + ((b + 1) >> 1), b2 = b1 + 1 - 2 * (b & 1);
-    int q[3];
-    q[d] = ix_c;
-    q[t0] = a1; q[t1] = b1;
-    const T c1 = P[G::idx(q[0], q[1], q[2])];
-    q[t0] = a2; q[t1] = b1;
-    const T ca = P[G::idx(q[0], q[1], q[2])];
-    q[t0] = a1; q[t1] = b2;
-    const T cb = P[G::idx(q[0], q[1], q[2])];
-    // the z-face case sums the off-diagonal coarse values in the other order
-    const T c2 = (d == 2) ? cb : ca, c3 = (d == 2) ? ca : cb;
-    v = third * c1 + sixth * c2 + sixth * c3 + third * f;
-    if (limited && v > 2 * c1) v = 2 * c1;
-  }
-  u[G::idx(p[0], p[1], p[2])] = v;
-}
-
-// Corner (2D) / edge + corner (3D) ghost cells (af_gc_box_corner :125-170,
-// af_edge_gc_extrap :885-924, af_corner_gc_extrap :860-879).
-__constant__ int c_edge_dim3[12] = {0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 2};
-__constant__ int c_edge_dir3[12][3] = {{0, -1, -1}, {0, 1, -1}, {0, -1, 1}, {0, 1, 1},
-                                       {-1, 0, -1}, {1, 0, -1}, {-1, 0, 1}, {1, 0, 1},
-                                       {-1, -1, 0}, {1, -1, 0}, {-1, 1, 0}, {1, 1, 0}};
-
-template <int D, int NC, typename T>
-__device__ inline void gc_edge_cell(const DBox *__restrict__ boxes, int id, T *var, int n, int qq) {
-  using G = Geo<D, NC>;
-  const DBox &bx = boxes[id - 1];
-  T *u = var + (size_t)(id - 1) * G::NCELL;
-  const int dim = c_edge_dim3[n];
-  const int *dir = c_edge_dir3[n];
-  int lo[3];
-  for (int dd = 0; dd < 3; dd++) lo[dd] = (dir[dd] > 0) ? NC + 1 : 0;
-  lo[dim] = qq;
-  const int nb_id = bx.nbmat[(dir[0] + 1) + 3 * (dir[1] + 1) + 9 * (dir[2] + 1)];
-  if (nb_id > 0) {
-    const T *w = var + (size_t)(nb_id - 1) * G::NCELL;
-    u[G::idx(lo[0], lo[1], lo[2])] = w[G::idx(lo[0] - dir[0] * NC, lo[1] - dir[1] * NC, lo[2] - dir[2] * NC)];
-  } else {
-    const int o0 = (dim + 1) % 3, o1 = (dim + 2) % 3;
-    int di[3];
-    for (int dd = 0; dd < 3; dd++) di[dd] = 1 - 2 * (lo[dd] & 1);
-    di[dim] = 0;
-    int ia[3] = {lo[0], lo[1], lo[2]}, ib[3] = {lo[0], lo[1], lo[2]};
-    ia[o0] += di[o0];
-    ib[o1] += di[o1];
-    u[G::idx(lo[0], lo[1], lo[2])] = u[G::idx(ia[0], ia[1], ia[2])] + u[G::idx(ib[0], ib[1], ib[2])] -
-                                     u[G::idx(lo[0] + di[0], lo[1] + di[1], lo[2] + di[2])];
-  }
-}
-
-template <int D, int NC, typename T>
-__device__ inline void gc_corner_cell(const DBox *__restrict__ boxes, int id, T *var, int n) {
-  using G = Geo<D, NC>;
-  const DBox &bx = boxes[id - 1];
-  T *u = var + (size_t)(id - 1) * G::NCELL;
-  const int cx = n & 1, cy = (n >> 1) & 1, cz = (n >> 2) & 1;
-  const int dnb[3] = {2 * cx - 1, 2 * cy - 1, (D == 3) ? 2 * cz - 1 : 0};
-  const int lo[3] = {cx * (NC + 1), cy * (NC + 1), (D == 3) ? cz * (NC + 1) : 0};
-  const int nb_id = bx.nbmat[(dnb[0] + 1) + 3 * (dnb[1] + 1) + 9 * (dnb[2] + 1)];
-  if (nb_id > 0) {
-    const T *w = var + (size_t)(nb_id - 1) * G::NCELL;
-    u[G::idx(lo[0], lo[1], lo[2])] = w[G::idx(lo[0] - dnb[0] * NC, lo[1] - dnb[1] * NC, lo[2] - dnb[2] * NC)];
-  } else if (D == 2) {
-    const int di = 1 - 2 * (lo[0] & 1), dj = 1 - 2 * (lo[1] & 1);
-    u[G::idx(lo[0], lo[1], 0)] =
-        u[G::idx(lo[0] + di, lo[1], 0)] + u[G::idx(lo[0], lo[1] + dj, 0)] - u[G::idx(lo[0] + di, lo[1] + dj, 0)];
-  } else {
-    const int di = 1 - 2 * (lo[0] & 1), dj = 1 - 2 * (lo[1] & 1), dk = 1 - 2 * (lo[2] & 1);
-    const int x = lo[0], y = lo[1], z = lo[2];
-    u[G::idx(x, y, z)] = u[G::idx(x, y + dj, z + dk)] + u[G::idx(x + di, y, z + dk)] +
-                         u[G::idx(x + di, y + dj, z)] - 2 * u[G::idx(x + di, y + dj, z + dk)];
-  }
-}
-
-template <int D, int NC, int BS, typename T>
-__global__ void __launch_bounds__(BS) k_gc(const int *__restrict__ ids, const DBox *__restrict__ boxes, T *var,
-                                           VarBc m, int corners) {
-  using G = Geo<D, NC>;
-  const int id = ids[blockIdx.x];
-  constexpr int NFC = 2 * D * G::NTAN;
-  for (int q = threadIdx.x; q < NFC; q += BS) {
-    const int nb = q / G::NTAN, r = q % G::NTAN;
-    const int a = r % NC + 1, b = (D == 3) ? r / NC + 1 : 0;
-    gc_face_cell<D, NC, T>(boxes, id, var, m, nb, a, b);
-  }
-  if (!corners) return;
-  __syncthreads();
-  if (D == 3) {
-    for (int q = threadIdx.x; q < 12 * NC; q += BS) gc_edge_cell<D, NC, T>(boxes, id, var, q / NC, q % NC + 1);
-    __syncthreads();
-  }
-  for (int q = threadIdx.x; q < (1 << D); q += BS) gc_corner_cell<D, NC, T>(boxes, id, var, q);
-}
-
 // max |u| over interior cells of a box list (af_tree_maxabs_cc,
 // afivo/src/m_af_utils.f90:773).
 template <int D, int NC, int BS>
 __global__ void __launch_bounds__(BS) k_maxabs(const int *__restrict__ ids, const double *__restrict__ u,
-                                               unsigned long long *out) {
+                                               unsigned long long *out, int bpb) {
   using G = Geo<D, NC>;
   __shared__ double sh[BS / 64];
-  const size_t base = (size_t)(ids[blockIdx.x] - 1) * G::NCELL;
+  const Part pt = part_of<BS>(ids, bpb);
+  const size_t base = (size_t)(pt.id - 1) * G::NCELL;
   double m = 0;
-  for (int q = threadIdx.x; q < G::NINT; q += BS) {
+  for (int q = pt.q0; q < G::NINT; q += pt.stride) {
     int i, j, k;
     interior_ijk<D, NC>(q, i, j, k);
     m = fmax(m, fabs(u[base + G::idx(i, j, k)]));
@@ -538,20 +599,23 @@ __global__ void __launch_bounds__(BS) k_maxabs(const int *__restrict__ ids, cons
 // Exact level-1 solve by fast diagonalisation (replaces hypre PFMG,
 // afivo/src/m_coarse_solver.f90:421): fold boundary values into the rhs as
 // coarse_solver_set_rhs_phi (:286-338) does, transform with V^-1 along each
-// axis, divide by the eigenvalue sums, transform back with V.
-template <int D, int NC, int BS>
-__global__ void __launch_bounds__(BS) k_coarse_solve(int id, const DBox *__restrict__ boxes, const double *rhs,
-                                                     double *phi, VarBc m, double lambda2, int cyl,
-                                                     const double *__restrict__ vinv, const double *__restrict__ v,
-                                                     const double *__restrict__ eig) {
+// axis, divide by the eigenvalue sums, transform back with V.  One
+// workgroup of CBS threads; the box lives in LDS.
+template <int D, int NC, int CBS>
+__global__ void __launch_bounds__(CBS) k_coarse_solve(int id, const DBox *__restrict__ boxes, const double *rhs,
+                                                      double *phi, VarBc m, double lambda2, int cyl,
+                                                      const double *__restrict__ vinv,
+                                                      const double *__restrict__ v,
+                                                      const double *__restrict__ eig) {
   using G = Geo<D, NC>;
   constexpr int N = G::NINT;
-  __shared__ double bufA[N], bufB[N];
+  constexpr int PER = (N + CBS - 1) / CBS;
+  __shared__ double buf[N];
   const DBox &bx = boxes[id - 1];
   const Sten<D, double> s = lpl_stencil<D, double>(bx, lambda2);
   const double *r = rhs + (size_t)(id - 1) * G::NCELL;
   auto lin = [](int i, int j, int k) { return (i - 1) + NC * ((j - 1) + NC * (D == 3 ? k - 1 : 0)); };
-  for (int q = threadIdx.x; q < N; q += BS) {
+  for (int q = threadIdx.x; q < N; q += CBS) {
     int i, j, k;
     interior_ijk<D, NC>(q, i, j, k);
     double bval = r[G::idx(i, j, k)];
@@ -576,25 +640,27 @@ __global__ void __launch_bounds__(BS) k_coarse_solve(int id, const DBox *__restr
         f = -(cnb * bx.dr[d]) * (double)(low ? -1 : 1);
       bval = bval + f * m.val[nb];
     }
-    bufA[lin(i, j, k)] = bval;
+    buf[lin(i, j, k)] = bval;
   }
   __syncthreads();
-  double *in = bufA, *out = bufB;
   for (int pass = 0; pass < 2 * D; pass++) {
     const int d = pass % D;
     const double *M = ((pass < D) ? vinv : v) + (size_t)d * NC * NC;
     if (pass == D) {
-      for (int q = threadIdx.x; q < N; q += BS) {
+      for (int q = threadIdx.x; q < N; q += CBS) {
         int i, j, k;
         interior_ijk<D, NC>(q, i, j, k);
         double den = eig[i - 1] + eig[NC + j - 1];
         if (D == 3) den = den + eig[2 * NC + k - 1];
         den = den - lambda2;
-        in[q] = in[q] / den;
+        buf[q] = buf[q] / den;
       }
       __syncthreads();
     }
-    for (int q = threadIdx.x; q < N; q += BS) {
+    double out[PER];
+    for (int t = 0; t < PER; t++) {
+      const int q = threadIdx.x + t * CBS;
+      if (q >= N) break;
       int i, j, k;
       interior_ijk<D, NC>(q, i, j, k);
       int p[3] = {i, j, (D == 3) ? k : 1};
@@ -603,20 +669,22 @@ __global__ void __launch_bounds__(BS) k_coarse_solve(int id, const DBox *__restr
       for (int mm = 1; mm <= NC; mm++) {
         int qq[3] = {p[0], p[1], p[2]};
         qq[d] = mm;
-        sum = sum + M[(row - 1) * NC + (mm - 1)] * in[lin(qq[0], qq[1], qq[2])];
+        sum = sum + M[(row - 1) * NC + (mm - 1)] * buf[lin(qq[0], qq[1], qq[2])];
       }
-      out[q] = sum;
+      out[t] = sum;
     }
     __syncthreads();
-    double *t = in;
-    in = out;
-    out = t;
+    for (int t = 0; t < PER; t++) {
+      const int q = threadIdx.x + t * CBS;
+      if (q < N) buf[q] = out[t];
+    }
+    __syncthreads();
   }
   double *u = phi + (size_t)(id - 1) * G::NCELL;
-  for (int q = threadIdx.x; q < N; q += BS) {
+  for (int q = threadIdx.x; q < N; q += CBS) {
     int i, j, k;
     interior_ijk<D, NC>(q, i, j, k);
-    u[G::idx(i, j, k)] = in[q];
+    u[G::idx(i, j, k)] = buf[q];
   }
 }
 
