@@ -1,0 +1,443 @@
+"""Pins of the CPU oracle against what the mathematics fixes (no GPU).
+
+Each test checks one part of the restated reference algorithm against an
+independent statement: a dense direct solve of the textbook discretisation,
+a manufactured solution with its expected second-order convergence, exact
+reproduction of linear fields by the ghost-cell and prolongation rules
+(the invariant of afivo/examples/check_ghostcells.f90), conservation of
+the electron number by the flux update including refinement boundaries,
+and closed forms of the table lookup and the chemistry source.
+"""
+import numpy as np
+import pytest
+from scipy.special import j0
+
+R_J1 = 3.8317059702075125  # first zero of J1
+
+
+# ---- helpers -------------------------------------------------------------------
+def uniform_case(S, A, D, coord, lvl, L=4e-3, V=1e3, nc=8):
+    r0 = [L / 2] * (D - 1) + [0.7 * L]
+    if coord == A.CYL:
+        r0[0] = 0.0
+    return S.make_case(D, nc, coord, [L] * D, V, lvl, r0, seed_width=0.3e-3)
+
+
+def oracle(A, S, case):
+    p = A.Path("oracle", case.D, case.nc, case.coord)
+    return S.apply(p, case)
+
+
+def leaf_info(case):
+    t = case.extra["export"]
+    D = case.D
+    out = []
+    for b in range(t["n_boxes"]):
+        if t["children"][b * 2 ** D] == 0:
+            out.append((b + 1, t["lvl"][b], t["r_min"][b * D:(b + 1) * D], t["dr"][b * D:(b + 1) * D],
+                        t["ix"][b * D:(b + 1) * D]))
+    return out
+
+
+def centres(nc, D, rmin, dr, ghosts=True):
+    """Cell-centre coordinates of a box as arrays in (k, j, i) C order."""
+    rng = np.arange(0, nc + 2) if ghosts else np.arange(1, nc + 1)
+    axes = [rmin[d] + (rng - 0.5) * dr[d] for d in range(D)]
+    grids = np.meshgrid(*axes[::-1], indexing="ij")
+    return grids[::-1]  # x, y(, z) arrays shaped (k, j, i)
+
+
+def box_view(arr_row, nc, D):
+    return arr_row.reshape((nc + 2,) * D)
+
+
+def interior(v, D):
+    return v[(slice(1, -1),) * D]
+
+
+def gather_uniform(p, case, iv, lvl):
+    """Leaf interiors of a uniform level as one global array (k, j, i)."""
+    D, nc = case.D, case.nc
+    n = nc * 2 ** (lvl - 1)
+    out = np.zeros((n,) * D)
+    data = p.get_cc(iv)
+    t = case.extra["export"]
+    for b in range(t["n_boxes"]):
+        if t["lvl"][b] != lvl:
+            continue
+        ix = t["ix"][b * D:(b + 1) * D]
+        v = interior(box_view(data[b], nc, D), D)
+        sl = tuple(slice((ix[d] - 1) * nc, ix[d] * nc) for d in reversed(range(D)))
+        out[sl] = v
+    return out
+
+
+def dense_operator(D, n, dr, coord, lam2, r0=0.0):
+    """Textbook 3/5/7-point (cylindrical: (1/r) d/dr (r d/dr)) operator on a
+    uniform n^D grid, Neumann on all faces except Dirichlet on both faces of
+    the last dimension (ghost = 2b - x1), as a dense matrix on (k, j, i)
+    C-ordered unknowns.  Returns A and the rhs contribution of b_hi = 1."""
+    N = n ** D
+    Amat = np.zeros((N, N))
+    bc_hi = np.zeros(N)
+    shape = (n,) * D
+
+    def flat(idx):
+        return np.ravel_multi_index(tuple(idx[::-1]), shape)
+
+    for q in range(N):
+        idx = list(np.unravel_index(q, shape))[::-1]  # (i, j, k)
+        row = flat(idx)
+        Amat[row, row] -= lam2
+        for d in range(D):
+            h2 = dr[d] ** 2
+            if coord == 2 and d == 0:
+                r = r0 + (idx[0] + 0.5) * dr[0]
+                wlo, whi = (r - 0.5 * dr[0]) / r, (r + 0.5 * dr[0]) / r
+            else:
+                wlo = whi = 1.0
+            for side, w in ((-1, wlo), (1, whi)):
+                c = w / h2
+                j = list(idx)
+                j[d] += side
+                Amat[row, row] -= c
+                if 0 <= j[d] < n:
+                    Amat[row, flat(j)] += c
+                elif d == D - 1:  # Dirichlet: ghost = 2b - x1
+                    Amat[row, row] -= c
+                    if side == 1:
+                        bc_hi[row] += 2 * c
+                else:  # Neumann zero: ghost = x1
+                    Amat[row, row] += c
+    return Amat, bc_hi
+
+
+# ---- exact level-1 solve and multigrid fixed point ---------------------------
+@pytest.mark.parametrize("D,coord,lvl", [(2, 2, 1), (2, 1, 1), (3, 1, 1), (2, 2, 2), (2, 1, 3), (3, 1, 2)])
+def test_multigrid_solution_equals_dense_solve(A, S, D, coord, lvl):
+    """On a uniform tree the composite problem is the uniform fine-grid
+    problem: the coarse solve (lvl 1) or repeated V-cycles (lvl > 1) must
+    converge to numpy's dense solution of the textbook discretisation."""
+    case = uniform_case(S, A, D, coord, lvl)
+    p = oracle(A, S, case)
+    rng = np.random.default_rng(1)
+    nb = case.extra["export"]["n_boxes"]
+    rhs = rng.standard_normal((nb, p.ncell)) * 1e9
+    p.put_cc(A.CC_RHS, rhs)
+    for _ in range(1 if lvl == 1 else 30):
+        p.mg_fas_vcycle(A.OP_POISSON, True, 0)
+    n = case.nc * 2 ** (lvl - 1)
+    dr = np.array([4e-3 / n] * D)
+    Amat, bhi = dense_operator(D, n, dr, coord, 0.0)
+    f = gather_uniform(p, case, A.CC_RHS, lvl).ravel()
+    sol = np.linalg.solve(Amat, f - bhi * case.voltage)
+    got = gather_uniform(p, case, A.CC_PHI, lvl).ravel()
+    assert np.abs(got - sol).max() <= 1e-9 * np.abs(sol).max()
+
+
+def test_helmholtz_coarse_solve_equals_dense_solve(A, S):
+    case = uniform_case(S, A, 3, 1, 1)
+    p = oracle(A, S, case)
+    rng = np.random.default_rng(2)
+    nb = case.extra["export"]["n_boxes"]
+    p.put_cc(A.CC_RHS, rng.standard_normal((nb, p.ncell)))
+    p.mg_fas_vcycle(A.OP_HELMH + 2, True, 0)
+    lam = case.params.photoi_lambda[2]
+    Amat, _ = dense_operator(3, 8, np.array([5e-4] * 3), 1, lam * lam)
+    sol = np.linalg.solve(Amat, gather_uniform(p, case, A.CC_RHS, 1).ravel())
+    got = gather_uniform(p, case, A.CC_HELMH + 2, 1).ravel()
+    assert np.abs(got - sol).max() <= 1e-10 * np.abs(sol).max()
+
+
+# ---- manufactured solutions: second-order convergence ------------------------
+def mms_error(A, S, D, coord, lvl, refine=None):
+    L, V, amp = 4e-3, 1e3, 500.0
+    if refine is None:
+        case = uniform_case(S, A, D, coord, lvl, L=L, V=V)
+    else:
+        case = S.make_case(D, 8, coord, [L] * D, V, lvl, [L / 2] * D, refine_fn=refine)
+    p = oracle(A, S, case)
+    t = case.extra["export"]
+    rhs = np.zeros((t["n_boxes"], p.ncell))
+    exact = {}
+    kz = np.pi / L
+    for b in range(t["n_boxes"]):
+        rmin, dr = t["r_min"][b * D:(b + 1) * D], t["dr"][b * D:(b + 1) * D]
+        X = centres(8, D, rmin, dr)
+        z = X[-1]
+        if coord == A.CYL:
+            kr = R_J1 / L
+            ex = V * z / L + amp * j0(kr * X[0]) * np.sin(kz * z)
+            lap = amp * (-kr ** 2 - kz ** 2) * j0(kr * X[0]) * np.sin(kz * z)
+        else:
+            kx = np.pi / L
+            tr = np.cos(kx * X[0])
+            if D == 3:
+                tr = tr * np.cos(kx * X[1])
+            ex = V * z / L + amp * tr * np.sin(kz * z)
+            lap = -amp * ((D - 1) * kx ** 2 + kz ** 2) * tr * np.sin(kz * z)
+        rhs[b] = lap.ravel()
+        exact[b + 1] = ex
+    p.put_cc(A.CC_RHS, rhs)
+    p.mg_fas_fmg(A.OP_POISSON, True, False)
+    for _ in range(15):
+        p.mg_fas_vcycle(A.OP_POISSON, True, 0)
+    phi = p.get_cc(A.CC_PHI)
+    err = 0.0
+    for bid, _, _, _, _ in leaf_info(case):
+        v = interior(box_view(phi[bid - 1], 8, D), D)
+        err = max(err, np.abs(v - interior(exact[bid], D)).max())
+    return err
+
+
+@pytest.mark.parametrize("D,coord", [(2, 2), (2, 1), (3, 1)])
+def test_poisson_mms_second_order(A, S, D, coord):
+    lvls = (2, 3, 4) if D == 2 else (1, 2, 3)
+    errs = [mms_error(A, S, D, coord, l) for l in lvls]
+    rates = [np.log2(errs[i] / errs[i + 1]) for i in range(len(errs) - 1)]
+    assert all(1.8 < r < 2.3 for r in rates), (errs, rates)
+
+
+@pytest.mark.parametrize("D,coord", [(2, 2), (3, 1)])
+def test_poisson_mms_on_amr_tree(A, S, D, coord):
+    """A refined tree must not be less accurate than its coarsest uniform
+    part: refinement boundaries (mg_sides_rb, restriction, prolongation)
+    keep the composite solution second-order consistent."""
+    base = 2
+    e_uniform = mms_error(A, S, D, coord, base)
+    e_amr = mms_error(A, S, D, coord, base, refine=S.random_refine(base + 2, 0.4, 5))
+    assert e_amr < 2.0 * e_uniform, (e_uniform, e_amr)
+
+
+# ---- ghost cells and prolongation reproduce linear fields ---------------------
+def linear_field(X, D, slope, offset):
+    return offset + sum(slope[d] * X[d] for d in range(D))
+
+
+def check_linear_ghosts(A, S, D, iv, slope, physical_ok, offset=3.0, voltage=1e4):
+    case = S.small_amr(D, nc=8, coord=A.XYZ, uniform_lvl=1 if D == 3 else 2, max_lvl=4 if D == 2 else 3,
+                       frac=0.5, seed=3, voltage=voltage)
+    p = oracle(A, S, case)
+    t = case.extra["export"]
+    vals = np.zeros((t["n_boxes"], p.ncell))
+    exact = {}
+    for b in range(t["n_boxes"]):
+        X = centres(8, D, t["r_min"][b * D:(b + 1) * D], t["dr"][b * D:(b + 1) * D])
+        ex = linear_field(X, D, slope, offset)
+        exact[b] = ex
+        vals[b] = ex.ravel()
+    # interior only, with the ghost layer poisoned
+    poisoned = vals.copy().reshape((t["n_boxes"],) + (10,) * D)
+    for d in range(D):
+        sl0 = [slice(None)] * (D + 1)
+        sl0[1 + d] = 0
+        poisoned[tuple(sl0)] = -1e30
+        sl0[1 + d] = -1
+        poisoned[tuple(sl0)] = -1e30
+    p.put_cc(iv, poisoned.reshape(t["n_boxes"], -1))
+    p.gc_tree(iv)
+    got = p.get_cc(iv)
+    checked = 0
+    for b in range(t["n_boxes"]):
+        g = box_view(got[b], 8, D)
+        e = exact[b]
+        nbs = t["neighbors"][b * 2 * D:(b + 1) * 2 * D]
+        for nb in range(2 * D):
+            if nbs[nb] < 0 and not physical_ok:
+                continue
+            d = nb // 2
+            sl = [slice(1, -1)] * D
+            sl[D - 1 - d] = 0 if nb % 2 == 0 else -1
+            gg, ee = g[tuple(sl)], e[tuple(sl)]
+            ok = np.ones(gg.shape, bool)
+            if nbs[nb] == 0 and not physical_ok:
+                # the interpolation reads coarse ghost cells in the tangential
+                # directions; skip those that lie on a physical boundary,
+                # where the coarse ghost follows the boundary condition
+                par = t["parent"][b] - 1
+                pnb = t["neighbors"][par * 2 * D + nb] - 1
+                tdims = [q for q in range(D) if q != d]
+                ix = t["ix"][b * D:(b + 1) * D]
+                for ax, td in enumerate(tdims):
+                    co = ((ix[td] - 1) & 1) * 4
+                    for a in range(1, 9):
+                        a1 = co + (a + 1) // 2
+                        a2 = a1 + 1 - 2 * (a & 1)
+                        side = 0 if a2 < 1 else (1 if a2 > 8 else -1)
+                        if side >= 0 and t["neighbors"][pnb * 2 * D + 2 * td + side] < 0:
+                            idx = [slice(None)] * gg.ndim
+                            idx[gg.ndim - 1 - ax] = a - 1
+                            ok[tuple(idx)] = False
+            diff = np.abs(gg - ee)[ok].max() if ok.any() else 0.0
+            assert diff <= 1e-12 * np.abs(e).max(), (b, nb, nbs[nb], diff)
+            checked += int(ok.sum())
+    assert checked > 0
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_ghostcells_reproduce_linear_field(A, S, D):
+    """af_gc_interp (|E|) and af_gc_interp_lim (species, unclamped) fill
+    refinement-boundary and same-level ghosts exactly for linear data."""
+    slope = [0.3e3, -0.2e3, 0.1e3][:D]
+    check_linear_ghosts(A, S, D, A.CC_EFIELD, slope, physical_ok=False)
+    check_linear_ghosts(A, S, D, A.CC_ELECTRON, slope, physical_ok=False)
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_mg_ghostcells_reproduce_linear_potential(A, S, D):
+    """mg_sides_rb at refinement boundaries and the Dirichlet/Neumann rules at
+    the domain boundary reproduce phi = V z / L (which satisfies the
+    potential's boundary conditions) exactly."""
+    V, L = 1e4, 4e-3
+    slope = [0.0] * (D - 1) + [V / L]
+    check_linear_ghosts(A, S, D, A.CC_PHI, slope, physical_ok=True, offset=0.0, voltage=V)
+
+
+# ---- conservation of the flux update across refinement boundaries ----------
+@pytest.mark.parametrize("D,coord", [(2, 2), (2, 1), (3, 1)])
+def test_electron_number_conserved_without_field(A, S, D, coord):
+    """With zero field there is no ionization and only diffusion: the total
+    electron number (sum of n_e times cell volume over leaves, with the
+    2 pi r weight in cylindrical coordinates) is conserved by the
+    conservative update, including consistent fluxes at refinement
+    boundaries (af_consistent_fluxes)."""
+    case = S.small_amr(D, nc=8, coord=coord, uniform_lvl=2, max_lvl=4 if D == 2 else 3, frac=0.5, seed=13,
+                       voltage=0.0)
+    p = oracle(A, S, case)
+    p.field_compute(0, have_guess=False)  # neutral, V = 0: E = 0
+    leaves = leaf_info(case)
+
+    def total():
+        ne = p.get_cc(A.CC_ELECTRON)
+        s = 0.0
+        for bid, lvl, rmin, dr, ix in leaves:
+            v = interior(box_view(ne[bid - 1], 8, D), D)
+            w = np.prod(dr)
+            if coord == A.CYL:
+                X = centres(8, D, rmin, dr, ghosts=False)
+                s += (v * X[0]).sum() * w
+            else:
+                s += v.sum() * w
+        return s
+
+    n0 = total()
+    for _ in range(5):
+        p.forward_euler(2e-12, 0, [0], [1.0], 0, 1, 1)
+    n1 = total()
+    assert abs(n1 - n0) <= 1e-12 * n0, (n0, n1)
+    # diffusion did change the profile
+    assert np.abs(p.get_cc(A.CC_ELECTRON) - case.ne0).max() > 0
+
+
+# ---- flux of a linear profile ------------------------------------------------
+def test_upwind_flux_of_linear_profile(A, S):
+    """Uniform field, linear n_e along z: the Koren reconstruction is exact
+    (ratio 1 -> limiter (b+2a)/3 = a), so every interior z-face carries
+    v n(face) - D dn/dz with v = -mu E."""
+    case = uniform_case(S, A, 3, 1, 2, V=0.0)
+    p = oracle(A, S, case)
+    t = case.extra["export"]
+    N = case.params.gas_number_density
+    E0 = -2.0e6
+    Lz = 4e-3
+    grad = 1e20
+    ne = np.zeros((t["n_boxes"], p.ncell))
+    efc = np.zeros((t["n_boxes"], p.nface))
+    ecc = np.full((t["n_boxes"], p.ncell), abs(E0))
+    for b in range(t["n_boxes"]):
+        X = centres(8, 3, t["r_min"][b * 3:(b + 1) * 3], t["dr"][b * 3:(b + 1) * 3])
+        ne[b] = (1e18 + grad * (X[2] - 0.0)).ravel()
+        f = efc[b].reshape(3, 9, 9, 9)
+        f[2] = E0
+        efc[b] = f.ravel()
+    p.put_cc(A.CC_ELECTRON, ne)
+    p.put_cc(A.CC_EFIELD, ecc)
+    p.put_fc(A.FC_FIELD, efc)
+    p.forward_euler(1e-15, 0, [0], [1.0], 0, 1, 1)
+    flux = p.get_fc(A.FC_FLUX)
+    tb = case.tdata
+    x = np.linspace(0, 1, tb["n"]) * ((tb["n"] - 1) / tb["td_inv_fac"])
+    Td = abs(E0) * 1e21 / N
+    mu = np.interp(Td, x, tb["td"][:tb["n"]]) / N
+    dif = np.interp(Td, x, tb["td"][tb["n"]:2 * tb["n"]]) / N
+    v = -mu * E0
+    for b in range(t["n_boxes"]):
+        if t["lvl"][b] != 2:
+            continue
+        rmin, dr = t["r_min"][b * 3:(b + 1) * 3], t["dr"][b * 3:(b + 1) * 3]
+        fz = flux[b].reshape(3, 9, 9, 9)[2]  # (k, j, i), face k between cells k-1, k
+        for kf in range(9):
+            zf = rmin[2] + kf * dr[2]
+            if zf < 2 * dr[2] or zf > Lz - 2 * dr[2]:
+                continue  # faces influenced by the physical boundary
+            expect = v * (1e18 + grad * zf) - dif * grad
+            got = fz[kf, :8, :8]
+            assert np.abs(got - expect).max() <= 1e-9 * abs(expect), (b, kf)
+
+
+# ---- table lookup and chemistry closed forms ----------------------------------
+def table_x(tb):
+    return tb["td_x_min"] + np.arange(tb["n"]) / tb["td_inv_fac"]
+
+
+def test_photoionization_source_matches_np_interp(A, S):
+    """photoionization_rate_from_alpha with LT_get_loc semantics (linear
+    interpolation, clamping at both ends) equals np.interp on the table."""
+    case = uniform_case(S, A, 2, 2, 2)
+    p = oracle(A, S, case)
+    t = case.extra["export"]
+    rng = np.random.default_rng(4)
+    E = rng.uniform(-1e5, 4e7, (t["n_boxes"], p.ncell))  # beyond both table ends
+    ne = rng.uniform(1e12, 1e20, (t["n_boxes"], p.ncell))
+    p.put_cc(A.CC_EFIELD, E)
+    p.put_cc(A.CC_ELECTRON, ne)
+    p.photoi_set_src()
+    rhs = p.get_cc(A.CC_RHS)
+    tb = case.tdata
+    N = case.params.gas_number_density
+    x = table_x(tb)
+    n = tb["n"]
+    Td = E * 1e21 / N
+    mob = np.interp(Td, x, tb["td"][:n])
+    alpha = np.interp(Td, x, tb["td"][2 * n:3 * n])
+    expect = np.maximum(E * mob * alpha * ne * case.params.photoi_eta_quench, 0.0)
+    for bid, _, _, _, _ in leaf_info(case):
+        g = interior(box_view(rhs[bid - 1], 8, 2), 2)
+        e = interior(box_view(expect[bid - 1], 8, 2), 2)
+        assert np.abs(g - e).max() <= 1e-13 * np.abs(e).max()
+
+
+def test_chemistry_source_closed_form(A, S):
+    """Standard model e + M -> 2e + M+, e + M -> M- with tabulated rates:
+    with zero face field and uniform densities there is no transport, so one
+    forward-Euler step gives n_e' = n_e + dt (k_i - k_a) n_e,
+    n_+' = n_+ + dt k_i n_e, n_-' = n_- + dt k_a n_e."""
+    case = uniform_case(S, A, 2, 1, 2, V=0.0)
+    p = oracle(A, S, case)
+    t = case.extra["export"]
+    rng = np.random.default_rng(6)
+    E = rng.uniform(0, 3.2e7, (t["n_boxes"], p.ncell))
+    n0 = np.full((t["n_boxes"], p.ncell), 3e18)
+    for iv in (A.CC_ELECTRON, A.CC_POS_ION, A.CC_NEG_ION):
+        p.put_cc(iv, n0)
+    p.put_cc(A.CC_EFIELD, E)
+    p.put_fc(A.FC_FIELD, np.zeros((t["n_boxes"], p.nface)))
+    dt = 1e-13
+    p.forward_euler(dt, 0, [0], [1.0], 0, 1, 1)
+    tb = case.tdata
+    N = case.params.gas_number_density
+    x = tb["chem_x_min"] + np.arange(tb["n"]) / tb["chem_inv_fac"]
+    Td = 1e21 / N * E
+    ki = np.interp(Td, x, tb["chem"][:tb["n"]])
+    ka = np.interp(Td, x, tb["chem"][tb["n"]:])
+    expect = {A.CC_ELECTRON: n0 + dt * (ki - ka) * n0, A.CC_POS_ION: n0 + dt * ki * n0,
+              A.CC_NEG_ION: n0 + dt * ka * n0}
+    for iv, e in expect.items():
+        g = p.get_cc(iv)
+        for bid, _, _, _, _ in leaf_info(case):
+            gi = interior(box_view(g[bid - 1], 8, 2), 2)
+            ei = interior(box_view(e[bid - 1], 8, 2), 2)
+            assert np.abs(gi - ei).max() <= 1e-13 * np.abs(ei).max(), iv
+    # net charge is conserved by the chemistry
+    q = sum(sg * p.get_cc(iv) for sg, iv in ((-1, A.CC_ELECTRON), (1, A.CC_POS_ION), (-1, A.CC_NEG_ION)))
+    assert np.abs(q - (-n0)).max() <= 1e-15 * n0.max() * 10
