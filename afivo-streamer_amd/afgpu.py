@@ -36,6 +36,7 @@ N_CC = 18
 FC_FLUX, FC_FIELD = 0, 1
 OP_POISSON, OP_HELMH = 0, 1
 MAX_MODES = 3
+OPT_GRAPHS = 1
 
 
 class ParamsT(C.Structure):
@@ -126,6 +127,7 @@ _GPU_ONLY = {
     "coarse_factors": (_I, [_I, _I, _I, _DP, _DP, _IP, _DP, _DP, _DP]),
     "kernel_timing": (_I, [_P, _I]),
     "kernel_stats": (_I, [_P, _I, _I, _DP, C.POINTER(C.c_longlong), _DP]),
+    "set_option": (_I, [_P, _I, _I]),
 }
 _ORACLE_ONLY = {
     "set_coarse_factors": (_I, [_P, _I, _DP, _DP, _DP]),
@@ -444,6 +446,10 @@ class Path:
 
     def set_time_state(self, it, time, global_dt):
         self._call("set_time_state", it, time, global_dt)
+
+    def set_option(self, option, value):
+        """Device library only (e.g. OPT_GRAPHS)."""
+        self._call("set_option", option, int(value))
 
     def num_leaf_cells(self):
         return self._fn("num_leaf_cells")(self.h)

@@ -14,7 +14,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <array>
 #include <limits>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -49,13 +51,23 @@ struct List {
   int off = 0, n = 0;
 };
 
-// Timing of one kernel class: event pairs recorded around its launches.
+// Timing of one kernel class: accumulated from HIP event pairs recorded on
+// the library's stream around each launch (also inside captured graphs,
+// where the pair becomes two event-record nodes replayed with the graph).
 struct KTimer {
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
-  std::vector<hipEvent_t> pool;
-  size_t used = 0;
+  double total_ms = 0;
   double algo_bytes = 0;
   long long launches = 0;
+};
+struct TPair {
+  hipEvent_t a, b;
+  int kind;
+  double bytes;
+};
+// A captured launch sequence and the timing pairs it contains.
+struct GraphRec {
+  hipGraphExec_t exec = nullptr;
+  std::vector<TPair> pairs;
 };
 
 }  // namespace
@@ -115,6 +127,13 @@ struct afgpu_ctx {
   // timing
   bool timing = false;
   KTimer timers[3];
+  std::vector<hipEvent_t> ev_pool;  // events of eager launches, recycled
+  size_t ev_used = 0;
+  std::vector<TPair> pending;       // recorded, not yet read
+  std::vector<TPair> *capture_pairs = nullptr;
+  // captured launch sequences (HIP graphs), keyed by kind/operator/flags
+  bool use_graphs = true;
+  std::map<std::array<int, 4>, GraphRec> graphs;
   std::unique_ptr<ImplBase> impl;
 
   int n_states() const { return p.time_integrator == AFGPU_HEUNS_METHOD ? 2 : 1; }
@@ -158,27 +177,95 @@ VarBc var_methods(const afgpu_ctx &c, int iv, bool homogeneous = false) {
   return m;
 }
 
+enum GraphKind { G_VCYCLE = 1, G_FMG = 2, G_FFP = 3 };
+
 int op_phi(int op) { return op == AFGPU_OP_POISSON ? AFGPU_CC_PHI : AFGPU_CC_HELMH + (op - AFGPU_OP_HELMH); }
 
 int timer_start(afgpu_ctx &c, int kind, double bytes) {
   if (!c.timing) return AFGPU_OK;
-  KTimer &t = c.timers[kind];
-  while (t.pool.size() < t.used + 2) {
-    hipEvent_t e;
-    HIPCHK(hipEventCreate(&e));
-    t.pool.push_back(e);
+  TPair tp{nullptr, nullptr, kind, bytes};
+  if (c.capture_pairs) {  // events owned by the graph being captured
+    HIPCHK(hipEventCreate(&tp.a));
+    HIPCHK(hipEventCreate(&tp.b));
+    c.capture_pairs->push_back(tp);
+  } else {
+    while (c.ev_pool.size() < c.ev_used + 2) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      c.ev_pool.push_back(e);
+    }
+    tp.a = c.ev_pool[c.ev_used];
+    tp.b = c.ev_pool[c.ev_used + 1];
+    c.ev_used += 2;
+    c.pending.push_back(tp);
   }
-  hipEvent_t a = t.pool[t.used], b = t.pool[t.used + 1];
-  t.used += 2;
-  t.ev.push_back({a, b});
-  t.algo_bytes += bytes;
-  t.launches++;
-  HIPCHK(hipEventRecord(a, c.stream));
+  HIPCHK(hipEventRecord(tp.a, c.stream));
   return AFGPU_OK;
 }
 int timer_stop(afgpu_ctx &c, int kind) {
   if (!c.timing) return AFGPU_OK;
-  HIPCHK(hipEventRecord(c.timers[kind].ev.back().second, c.stream));
+  const TPair &tp = c.capture_pairs ? c.capture_pairs->back() : c.pending.back();
+  (void)kind;
+  HIPCHK(hipEventRecord(tp.b, c.stream));
+  return AFGPU_OK;
+}
+// Accumulate the pairs recorded so far (the stream has been synchronised).
+int timer_harvest(afgpu_ctx &c) {
+  for (const TPair &tp : c.pending) {
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, tp.a, tp.b));
+    KTimer &t = c.timers[tp.kind];
+    t.total_ms += ms;
+    t.algo_bytes += tp.bytes;
+    t.launches++;
+  }
+  c.pending.clear();
+  c.ev_used = 0;
+  return AFGPU_OK;
+}
+void drop_graphs(afgpu_ctx &c) {
+  for (auto &kv : c.graphs) {
+    if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+    for (auto &tp : kv.second.pairs) {
+      (void)hipEventDestroy(tp.a);
+      (void)hipEventDestroy(tp.b);
+    }
+  }
+  c.graphs.clear();
+}
+// Run `body` (a sequence of kernel launches on c.stream with no host
+// synchronisation and no memset/memcpy) through a HIP graph captured on its
+// first use and replayed afterwards.
+template <typename F>
+int run_graph(afgpu_ctx &c, std::array<int, 4> key, F body) {
+  // HIP does not report elapsed times of events recorded by graph nodes, so
+  // kernel timing runs the same sequences eagerly.
+  if (!c.use_graphs || c.timing) return body();
+  auto it = c.graphs.find(key);
+  if (it == c.graphs.end()) {
+    GraphRec rec;
+    HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    c.capture_pairs = &rec.pairs;
+    const int rc = body();
+    c.capture_pairs = nullptr;
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c.stream, &g);
+    if (rc != AFGPU_OK || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      for (auto &tp : rec.pairs) {
+        (void)hipEventDestroy(tp.a);
+        (void)hipEventDestroy(tp.b);
+      }
+      if (rc != AFGPU_OK) return rc;
+      HIPCHK(e);
+    }
+    const hipError_t ei = hipGraphInstantiate(&rec.exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHK(ei);
+    it = c.graphs.emplace(key, rec).first;
+  }
+  HIPCHK(hipGraphLaunch(it->second.exec, c.stream));
+  if (c.timing) c.pending.insert(c.pending.end(), it->second.pairs.begin(), it->second.pairs.end());
   return AFGPU_OK;
 }
 
@@ -300,6 +387,12 @@ struct Impl : ImplBase {
   }
 
   int vcycle(int op, bool set_residual, int max_lvl) override {
+    RCHK(ensure_factors(op));  // allocation and upload happen outside any capture
+    return run_graph(c, {G_VCYCLE, op, (int)set_residual, max_lvl},
+                     [&] { return vcycle_body(op, set_residual, max_lvl); });
+  }
+
+  int vcycle_body(int op, bool set_residual, int max_lvl) {
     for (int lvl = max_lvl; lvl >= 2; lvl--) {
       RCHK(gsrb_boxes(lvl, op, true));
       RCHK(update_coarse(lvl, op, true));
@@ -314,7 +407,7 @@ struct Impl : ImplBase {
       // residual on every box; the max |residual| over leaves is reduced on
       // the fly (read back with leaf_residual_max)
       const int cyl = c.coord == AFGPU_CYL;
-      HIPCHK(hipMemsetAsync(c.d_red, 0, sizeof(unsigned long long), c.stream));
+      RCHK(set_red(0, 0ull));
       for (int lvl = 1; lvl <= max_lvl; lvl++) {
         const List &l = c.l_ids[lvl];
         const int k = bpb(l.n, G::NINT);
@@ -334,6 +427,12 @@ struct Impl : ImplBase {
   }
 
   int fmg(int op, bool set_residual, bool have_guess) override {
+    RCHK(ensure_factors(op));
+    return run_graph(c, {G_FMG, op, (int)set_residual, (int)have_guess},
+                     [&] { return fmg_body(op, set_residual, have_guess); });
+  }
+
+  int fmg_body(int op, bool set_residual, bool have_guess) {
     const int iphi = op_phi(op);
     if (have_guess) {
       for (int lvl = c.highest_lvl; lvl >= 2; lvl--) RCHK(update_coarse(lvl, op, false));
@@ -347,12 +446,12 @@ struct Impl : ImplBase {
       }
     }
     RCHK(boxop(c.l_ids[1], c.cc[iphi], c.cc[AFGPU_CC_TMP], OP_COPY));
-    RCHK(vcycle(op, set_residual && c.highest_lvl == 1, 1));
+    RCHK(vcycle_body(op, set_residual && c.highest_lvl == 1, 1));
     for (int lvl = 2; lvl <= c.highest_lvl; lvl++) {
       RCHK(boxop(c.l_ids[lvl], c.cc[iphi], c.cc[AFGPU_CC_TMP], OP_COPY));
       RCHK(correct_children(lvl - 1, op));
       RCHK(gc_lvl(lvl, iphi, true));
-      RCHK(vcycle(op, set_residual && lvl == c.highest_lvl, lvl));
+      RCHK(vcycle_body(op, set_residual && lvl == c.highest_lvl, lvl));
     }
     return AFGPU_OK;
   }
@@ -360,6 +459,12 @@ struct Impl : ImplBase {
   int read_red(int n) {
     HIPCHK(hipMemcpyAsync(c.h_red, c.d_red, n * sizeof(unsigned long long), hipMemcpyDeviceToHost, c.stream));
     HIPCHK(hipStreamSynchronize(c.stream));
+    return timer_harvest(c);
+  }
+  // reduction slot i := value (a kernel, so that it can live in a graph)
+  int set_red(int i, unsigned long long value) {
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, c.stream, c.d_red + i, value);
+    HIPCHK(hipGetLastError());
     return AFGPU_OK;
   }
   static double as_double(unsigned long long u) {
@@ -369,7 +474,7 @@ struct Impl : ImplBase {
   }
 
   int maxabs(int iv, double *out) override {
-    HIPCHK(hipMemsetAsync(c.d_red, 0, sizeof(unsigned long long), c.stream));
+    RCHK(set_red(0, 0ull));
     const int k = bpb(c.l_all_leaves.n, G::NINT);
     RCHK(launch(k_maxabs<D, NC, BS>, c.l_all_leaves.n * k, c.list(c.l_all_leaves), (const double *)c.cc[iv], c.d_red,
                 k));
@@ -380,9 +485,11 @@ struct Impl : ImplBase {
 
   // --- field (src/m_field.f90) ----------------------------------------------
   int field_from_potential() override {
-    RCHK(launch(k_gradient<D, NC, BS>, c.l_all_ids.n, c.list(c.l_all_ids), (const DBox *)c.d_boxes,
-                (const double *)c.cc[AFGPU_CC_PHI], c.fc[AFGPU_FC_FIELD], c.cc[AFGPU_CC_EFIELD]));
-    return gc_tree(AFGPU_CC_EFIELD);
+    return run_graph(c, {G_FFP, 0, 0, 0}, [&] {
+      RCHK(launch(k_gradient<D, NC, BS>, c.l_all_ids.n, c.list(c.l_all_ids), (const DBox *)c.d_boxes,
+                  (const double *)c.cc[AFGPU_CC_PHI], c.fc[AFGPU_FC_FIELD], c.cc[AFGPU_CC_EFIELD]));
+      return gc_tree(AFGPU_CC_EFIELD);
+    });
   }
 
   double min_dr() const {
@@ -457,12 +564,12 @@ struct Impl : ImplBase {
     // af_restrict_ref_boundary (afivo/src/m_af_restrict.f90:140-161)
     RCHK(launch(k_restrict<D, NC, BS, double>, c.l_refb.n, c.list(c.l_refb), (const DBox *)c.d_boxes,
                 (const double *)c.cc[iv_e], c.cc[iv_e], 1, cyl, 1));
-    HIPCHK(hipMemsetAsync(c.d_red + 1, 0, 2 * sizeof(unsigned long long), c.stream));
     const double big = 1e100;
     unsigned long long bigbits;
     std::memcpy(&bigbits, &big, sizeof(big));
-    c.h_red[3] = bigbits;
-    HIPCHK(hipMemcpyAsync(c.d_red + 3, c.h_red + 3, sizeof(unsigned long long), hipMemcpyHostToDevice, c.stream));
+    RCHK(set_red(1, 0ull));
+    RCHK(set_red(2, 0ull));
+    RCHK(set_red(3, bigbits));
     const double N_inv = 1 / c.p.gas_number_density;
     const VarBc msp = var_methods(c, iv_e);
     for (int lvl = 1; lvl <= c.highest_lvl; lvl++) {
@@ -603,7 +710,7 @@ int ready(afgpu_t *h, bool need_tables = false) {
 
 int sync_stream(afgpu_t *h) {
   HIPCHK(hipStreamSynchronize(h->stream));
-  return AFGPU_OK;
+  return timer_harvest(*h);
 }
 
 }  // namespace
@@ -647,6 +754,8 @@ int afgpu_create(int ndim, int n_cell, int coord, int precision, afgpu_t **out) 
 int afgpu_destroy(afgpu_t *h) {
   if (!h) return AFGPU_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  drop_graphs(*h);
+  for (auto e : h->ev_pool) (void)hipEventDestroy(e);
   for (int i = 0; i < AFGPU_N_CC; i++) (void)hipFree(h->cc[i]);
   for (int i = 0; i < AFGPU_N_FC; i++) (void)hipFree(h->fc[i]);
   for (int i = 0; i < 3; i++) (void)hipFree(h->cc32[i]);
@@ -661,8 +770,6 @@ int afgpu_destroy(afgpu_t *h) {
     (void)hipFree(o.d_v);
     (void)hipFree(o.d_eig);
   }
-  for (auto &t : h->timers)
-    for (auto e : t.pool) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return AFGPU_OK;
@@ -834,6 +941,7 @@ int afgpu_sync_tree(afgpu_t *h, int n_boxes, const int *lvl, const int *ix, cons
   h->n_boxes = n_boxes;
   h->highest_lvl = hl;
   for (auto &o : h->ops) o.ok = false;  // level-1 box may have changed
+  drop_graphs(*h);
   return AFGPU_OK;
 }
 
@@ -884,6 +992,7 @@ int afgpu_set_params(afgpu_t *h, const afgpu_params_t *p) {
     set_err("afgpu_set_params: invalid arguments");
     return AFGPU_ERR_ARG;
   }
+  drop_graphs(*h);
   h->p = *p;
   h->have_params = true;
   h->ops[AFGPU_OP_POISSON].lambda2 = 0.0;
@@ -904,6 +1013,7 @@ int afgpu_set_tables(afgpu_t *h, int n_rows, double td_x_min, double td_inv_fac,
   HIPCHK(hipMalloc(&h->d_chem, 2 * (size_t)n_rows * sizeof(double)));
   HIPCHK(hipMemcpy(h->d_td, td_cols, 4 * (size_t)n_rows * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(h->d_chem, chem_cols, 2 * (size_t)n_rows * sizeof(double), hipMemcpyHostToDevice));
+  drop_graphs(*h);
   h->td = DTable{h->d_td, n_rows, td_x_min, td_inv_fac};
   h->chem = DTable{h->d_chem, n_rows, chem_x_min, chem_inv_fac};
   h->have_tables = true;
@@ -912,6 +1022,7 @@ int afgpu_set_tables(afgpu_t *h, int n_rows, double td_x_min, double td_inv_fac,
 
 int afgpu_set_voltage(afgpu_t *h, double v) {
   if (!h) return AFGPU_ERR_ARG;
+  if (v != h->voltage) drop_graphs(*h);  // the boundary value is a captured argument
   h->voltage = v;
   return AFGPU_OK;
 }
@@ -1110,30 +1221,35 @@ long long afgpu_num_leaf_cells(afgpu_t *h) {
 
 int afgpu_kernel_timing(afgpu_t *h, int enable) {
   if (!h) return AFGPU_ERR_ARG;
+  RCHK(sync_stream(h));
+  if ((enable != 0) != h->timing) drop_graphs(*h);  // recapture with/without event nodes
   h->timing = enable != 0;
   return AFGPU_OK;
 }
 
 int afgpu_kernel_stats(afgpu_t *h, int kind, int reset, double *total_ms, long long *launches, double *algo_bytes) {
   if (!h || kind < 0 || kind > 2) return AFGPU_ERR_ARG;
-  HIPCHK(hipStreamSynchronize(h->stream));
+  RCHK(sync_stream(h));
   KTimer &t = h->timers[kind];
-  double ms = 0;
-  for (auto &pr : t.ev) {
-    float e = 0;
-    HIPCHK(hipEventElapsedTime(&e, pr.first, pr.second));
-    ms += e;
-  }
-  if (total_ms) *total_ms = ms;
+  if (total_ms) *total_ms = t.total_ms;
   if (launches) *launches = t.launches;
   if (algo_bytes) *algo_bytes = t.algo_bytes;
-  if (reset) {
-    t.ev.clear();
-    t.used = 0;
-    t.algo_bytes = 0;
-    t.launches = 0;
-  }
+  if (reset) t = KTimer();
   return AFGPU_OK;
+}
+
+int afgpu_set_option(afgpu_t *h, int option, int value) {
+  if (!h) return AFGPU_ERR_ARG;
+  switch (option) {
+    case AFGPU_OPT_GRAPHS:
+      RCHK(sync_stream(h));
+      drop_graphs(*h);
+      h->use_graphs = value != 0;
+      return AFGPU_OK;
+    default:
+      set_err("afgpu_set_option: unknown option");
+      return AFGPU_ERR_ARG;
+  }
 }
 
 }  // extern "C"

@@ -124,6 +124,10 @@ __device__ inline void atomic_min_nonneg(unsigned long long *addr, double v) {
   atomicMin(addr, (unsigned long long)__double_as_longlong(v));
 }
 
+__global__ void k_set_u64(unsigned long long *p, unsigned long long v) {
+  if (threadIdx.x == 0) *p = v;
+}
+
 template <int BS>
 __device__ inline double block_max(double v, double *sh) {
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));

@@ -257,6 +257,10 @@ int afgpu_kernel_timing(afgpu_t *h, int enable);
 int afgpu_kernel_stats(afgpu_t *h, int kind, int reset, double *total_ms,
                        long long *launches, double *algo_bytes);
 
+/* Execution options. */
+#define AFGPU_OPT_GRAPHS 1 /* capture V-cycle / FMG launch sequences as HIP graphs (default 1) */
+int afgpu_set_option(afgpu_t *h, int option, int value);
+
 #ifdef __cplusplus
 }
 #endif

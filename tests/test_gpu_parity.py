@@ -87,3 +87,21 @@ def test_photoi_parity(A, S, name):
         p.field_compute(0, have_guess=False)
         p.photoi_set_src()
     compare(A, g, c, [A.CC_PHOTO, A.CC_HELMH, A.CC_HELMH + 1, A.CC_HELMH + 2], what=name + " photoi")
+
+
+def test_graph_replays_match_eager(A, S):
+    """The HIP-graph path (captured V-cycle / FMG / field sequences, replayed
+    many times with host synchronisation in between) gives exactly the eager
+    results, step after step."""
+    case = CASES["xyz3d_amr"](S)
+    case.params.photoi_enabled = 1
+    case.params.photoi_per_steps = 2
+    g1 = S.apply(A.Path("gpu", case.D, case.nc, case.coord), case)
+    g2 = S.apply(A.Path("gpu", case.D, case.nc, case.coord), case)
+    g2.set_option(A.OPT_GRAPHS, 0)
+    assert g1.field_compute(0, False) == g2.field_compute(0, False)
+    for it in range(6):
+        a, b = g1.step(), g2.step()
+        assert a == b
+        for iv in (A.CC_ELECTRON, A.CC_PHI, A.CC_PHOTO, A.CC_TMP):
+            assert np.array_equal(g1.get_cc(iv), g2.get_cc(iv)), (it, iv)
