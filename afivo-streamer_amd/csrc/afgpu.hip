@@ -177,7 +177,7 @@ VarBc var_methods(const afgpu_ctx &c, int iv, bool homogeneous = false) {
   return m;
 }
 
-enum GraphKind { G_VCYCLE = 1, G_FMG = 2, G_FFP = 3 };
+enum GraphKind { G_VCYCLE = 1, G_FMG = 2, G_FFP = 3, G_MP_RES = 4, G_MP_VCYCLE = 5, G_MP_GC = 6, G_MP_FMG = 7 };
 
 int op_phi(int op) { return op == AFGPU_OP_POISSON ? AFGPU_CC_PHI : AFGPU_CC_HELMH + (op - AFGPU_OP_HELMH); }
 
@@ -292,64 +292,85 @@ struct Impl : ImplBase {
     return std::max(1, std::min(b, cap));
   }
 
-  int boxop(const List &l, const double *a, double *b, int op, double s = 0.0) {
+  template <typename T>
+  int boxop(const List &l, const T *a, T *b, int op, T s = T(0)) {
     const int k = bpb(l.n, G::NCELL);
-    return launch(k_boxop<G::NCELL, BS, double>, l.n * k, c.list(l), a, b, op, s, k);
+    return launch(k_boxop<G::NCELL, BS, T>, l.n * k, c.list(l), a, b, op, s, k);
   }
 
-  int gc_lvl(int lvl, int iv, bool corners) override {
+  template <typename T>
+  int gc_var(int lvl, T *var, const VarBc &m, bool corners) {
     const List &l = c.l_ids[lvl];
     const int k = corners ? 1 : bpb(l.n, 2 * D * G::NTAN);
-    return launch(k_gc<D, NC, BS, double>, l.n * k, c.list(l), (const DBox *)c.d_boxes, c.cc[iv],
-                  var_methods(c, iv), (int)corners, k);
+    return launch(k_gc<D, NC, BS, T>, l.n * k, c.list(l), (const DBox *)c.d_boxes, var, m, (int)corners, k);
   }
+  int gc_lvl(int lvl, int iv, bool corners) override { return gc_var(lvl, c.cc[iv], var_methods(c, iv), corners); }
   int gc_tree(int iv) {
     for (int lvl = 1; lvl <= c.highest_lvl; lvl++) RCHK(gc_lvl(lvl, iv, true));
     return AFGPU_OK;
   }
 
   // --- multigrid (afivo/src/m_af_multigrid.f90) ---------------------------
-  int gsrb_boxes(int lvl, int op, bool down) {
+  // The variables one multigrid solve works on.  FP64: the operator's own
+  // phi with the shared rhs/tmp.  Mixed precision: the single-precision
+  // correction e, its rhs (the FP64 residual, rounded) and scratch, with the
+  // boundary conditions made homogeneous.
+  template <typename T>
+  struct Mg {
+    int op;
+    T *phi, *rhs, *tmp;
+    VarBc m;
+    T lambda2;
+  };
+  Mg<double> mg64(int op) {
+    const int iphi = op_phi(op);
+    return {op, c.cc[iphi], c.cc[AFGPU_CC_RHS], c.cc[AFGPU_CC_TMP], var_methods(c, iphi), c.ops[op].lambda2};
+  }
+  Mg<float> mg32(int op) {
+    return {op, c.cc32[0], c.cc32[1], c.cc32[2], var_methods(c, op_phi(op), true), (float)c.ops[op].lambda2};
+  }
+
+  template <typename T>
+  int gsrb_boxes(const Mg<T> &v, int lvl, bool down) {
     const int n_cycle = down ? c.p.mg_n_cycle_down : c.p.mg_n_cycle_up;
     const List &l = c.l_ids[lvl];
-    const int iphi = op_phi(op);
-    const double bytes = 12.0 * l.n * G::NINT;
-    const VarBc m = var_methods(c, iphi);
+    // algorithmic bytes of a half-sweep: the other colour and the rhs of
+    // the updated colour are read, the updated colour is written
+    const double bytes = 1.5 * sizeof(T) * l.n * G::NINT;
     const int k = bpb(l.n, G::NINT / 2);
     for (int n = 1; n <= 2 * n_cycle; n++) {
       RCHK(timer_start(c, 0, bytes));
-      RCHK(launch(k_gsrb<D, NC, BS, double>, l.n * k, c.list(l), (const DBox *)c.d_boxes, c.cc[iphi],
-                  (const double *)c.cc[AFGPU_CC_RHS], n, c.ops[op].lambda2, (int)(c.coord == AFGPU_CYL), m, k));
+      RCHK(launch(k_gsrb<D, NC, BS, T>, l.n * k, c.list(l), (const DBox *)c.d_boxes, v.phi, (const T *)v.rhs, n,
+                  v.lambda2, (int)(c.coord == AFGPU_CYL), v.m, k));
       RCHK(timer_stop(c, 0));
     }
     // ghost layer after the last half-sweep (corners on the final up-sweep)
-    return gc_lvl(lvl, iphi, !down);
+    return gc_var(lvl, v.phi, v.m, !down);
   }
 
-  int update_coarse(int lvl, int op, bool v_cycle) {
-    const int iphi = op_phi(op);
+  template <typename T>
+  int update_coarse(const Mg<T> &v, int lvl, bool v_cycle) {
     const int cyl = c.coord == AFGPU_CYL;
-    if (!v_cycle && lvl == c.highest_lvl) RCHK(gc_lvl(lvl, iphi, true));
+    if (!v_cycle && lvl == c.highest_lvl) RCHK(gc_var(lvl, v.phi, v.m, true));
     const List &l = c.l_ids[lvl];
     constexpr int NQ = G::NINT / (1 << D);
     const int k = bpb(l.n, NQ);
-    RCHK(launch(k_restrict_residual<D, NC, BS, double>, l.n * k, c.list(l), (const DBox *)c.d_boxes, c.cc[iphi],
-                (const double *)c.cc[AFGPU_CC_RHS], c.cc[AFGPU_CC_TMP], c.ops[op].lambda2, cyl, (int)!v_cycle, k));
-    RCHK(gc_lvl(lvl - 1, iphi, true));
+    RCHK(launch(k_restrict_residual<D, NC, BS, T>, l.n * k, c.list(l), (const DBox *)c.d_boxes, v.phi,
+                (const T *)v.rhs, v.tmp, v.lambda2, cyl, (int)!v_cycle, k));
+    RCHK(gc_var(lvl - 1, v.phi, v.m, true));
     const List &pl = c.l_parents[lvl - 1];
     const int k2 = bpb(pl.n, G::NCELL);
-    return launch(k_coarse_rhs<D, NC, BS, double>, pl.n * k2, c.list(pl), (const DBox *)c.d_boxes,
-                  (const double *)c.cc[iphi], c.cc[AFGPU_CC_RHS], c.cc[AFGPU_CC_TMP], c.ops[op].lambda2, cyl,
-                  (int)v_cycle, k2);
+    return launch(k_coarse_rhs<D, NC, BS, T>, pl.n * k2, c.list(pl), (const DBox *)c.d_boxes, (const T *)v.phi,
+                  v.rhs, v.tmp, v.lambda2, cyl, (int)v_cycle, k2);
   }
 
-  int correct_children(int lvl_p, int op) {
-    const int iphi = op_phi(op);
-    RCHK(boxop(c.l_parents[lvl_p], c.cc[iphi], c.cc[AFGPU_CC_TMP], OP_SUB_FROM));
+  template <typename T>
+  int correct_children(const Mg<T> &v, int lvl_p) {
+    RCHK(boxop<T>(c.l_parents[lvl_p], v.phi, v.tmp, OP_SUB_FROM));
     const List &l = c.l_ids[lvl_p + 1];
     const int k = bpb(l.n, G::NINT);
-    return launch(k_prolong_add<D, NC, BS, double>, l.n * k, c.list(l), (const DBox *)c.d_boxes,
-                  (const double *)c.cc[AFGPU_CC_TMP], c.cc[iphi], k);
+    return launch(k_prolong_add<D, NC, BS, T>, l.n * k, c.list(l), (const DBox *)c.d_boxes, (const T *)v.tmp, v.phi,
+                  k);
   }
 
   int ensure_factors(int op) {
@@ -374,34 +395,37 @@ struct Impl : ImplBase {
     return AFGPU_OK;
   }
 
-  int solve_coarse(int op) {
-    RCHK(ensure_factors(op));
-    const int iphi = op_phi(op);
+  // The level-1 solve always runs in FP64 (single-precision storage is
+  // widened on load and rounded on store).
+  template <typename T>
+  int solve_coarse(const Mg<T> &v) {
     const int id = c.h_lists[c.l_ids[1].off];
-    hipLaunchKernelGGL((k_coarse_solve<D, NC, 1024>), dim3(1), dim3(1024), 0, c.stream, id, (const DBox *)c.d_boxes,
-                       (const double *)c.cc[AFGPU_CC_RHS], c.cc[iphi], var_methods(c, iphi), c.ops[op].lambda2,
-                       (int)(c.coord == AFGPU_CYL), (const double *)c.ops[op].d_vinv,
-                       (const double *)c.ops[op].d_v, (const double *)c.ops[op].d_eig);
+    const afgpu_ctx::Op &o = c.ops[v.op];
+    hipLaunchKernelGGL((k_coarse_solve<D, NC, 1024, T>), dim3(1), dim3(1024), 0, c.stream, id,
+                       (const DBox *)c.d_boxes, (const T *)v.rhs, v.phi, v.m, o.lambda2,
+                       (int)(c.coord == AFGPU_CYL), (const double *)o.d_vinv, (const double *)o.d_v,
+                       (const double *)o.d_eig);
     HIPCHK(hipGetLastError());
-    return gc_lvl(1, iphi, true);
+    return gc_var(1, v.phi, v.m, true);
   }
 
   int vcycle(int op, bool set_residual, int max_lvl) override {
     RCHK(ensure_factors(op));  // allocation and upload happen outside any capture
     return run_graph(c, {G_VCYCLE, op, (int)set_residual, max_lvl},
-                     [&] { return vcycle_body(op, set_residual, max_lvl); });
+                     [&] { return vcycle_body(mg64(op), set_residual, max_lvl); });
   }
 
-  int vcycle_body(int op, bool set_residual, int max_lvl) {
+  template <typename T>
+  int vcycle_body(const Mg<T> &v, bool set_residual, int max_lvl) {
     for (int lvl = max_lvl; lvl >= 2; lvl--) {
-      RCHK(gsrb_boxes(lvl, op, true));
-      RCHK(update_coarse(lvl, op, true));
+      RCHK(gsrb_boxes(v, lvl, true));
+      RCHK(update_coarse(v, lvl, true));
     }
-    RCHK(solve_coarse(op));
+    RCHK(solve_coarse(v));
     for (int lvl = 2; lvl <= max_lvl; lvl++) {
-      RCHK(correct_children(lvl - 1, op));
-      RCHK(gc_lvl(lvl, op_phi(op), true));
-      RCHK(gsrb_boxes(lvl, op, false));
+      RCHK(correct_children(v, lvl - 1));
+      RCHK(gc_var(lvl, v.phi, v.m, true));
+      RCHK(gsrb_boxes(v, lvl, false));
     }
     if (set_residual) {
       // residual on every box; the max |residual| over leaves is reduced on
@@ -411,9 +435,8 @@ struct Impl : ImplBase {
       for (int lvl = 1; lvl <= max_lvl; lvl++) {
         const List &l = c.l_ids[lvl];
         const int k = bpb(l.n, G::NINT);
-        RCHK(launch(k_residual<D, NC, BS, double>, l.n * k, c.list(l), (const DBox *)c.d_boxes,
-                    (const double *)c.cc[op_phi(op)], (const double *)c.cc[AFGPU_CC_RHS], c.cc[AFGPU_CC_TMP],
-                    c.ops[op].lambda2, cyl, c.d_red, k));
+        RCHK(launch(k_residual<D, NC, BS, T>, l.n * k, c.list(l), (const DBox *)c.d_boxes, (const T *)v.phi,
+                    (const T *)v.rhs, v.tmp, v.lambda2, cyl, c.d_red, k));
       }
     }
     return AFGPU_OK;
@@ -429,31 +452,67 @@ struct Impl : ImplBase {
   int fmg(int op, bool set_residual, bool have_guess) override {
     RCHK(ensure_factors(op));
     return run_graph(c, {G_FMG, op, (int)set_residual, (int)have_guess},
-                     [&] { return fmg_body(op, set_residual, have_guess); });
+                     [&] { return fmg_body(mg64(op), set_residual, have_guess); });
   }
 
-  int fmg_body(int op, bool set_residual, bool have_guess) {
-    const int iphi = op_phi(op);
+  template <typename T>
+  int fmg_body(const Mg<T> &v, bool set_residual, bool have_guess) {
     if (have_guess) {
-      for (int lvl = c.highest_lvl; lvl >= 2; lvl--) RCHK(update_coarse(lvl, op, false));
+      for (int lvl = c.highest_lvl; lvl >= 2; lvl--) RCHK(update_coarse(v, lvl, false));
     } else {
       for (int lvl = c.highest_lvl; lvl >= 2; lvl--) {  // init_phi_rhs
         const List &l = c.l_ids[lvl];
-        RCHK(boxop(l, c.cc[iphi], c.cc[iphi], OP_CLEAR));
+        RCHK(boxop<T>(l, v.phi, v.phi, OP_CLEAR));
         const int k = bpb(l.n, G::NINT / (1 << D));
-        RCHK(launch(k_restrict<D, NC, BS, double>, l.n * k, c.list(l), (const DBox *)c.d_boxes,
-                    (const double *)c.cc[AFGPU_CC_RHS], c.cc[AFGPU_CC_RHS], 1, (int)(c.coord == AFGPU_CYL), k));
+        RCHK(launch(k_restrict<D, NC, BS, T>, l.n * k, c.list(l), (const DBox *)c.d_boxes, (const T *)v.rhs, v.rhs,
+                    1, (int)(c.coord == AFGPU_CYL), k));
       }
     }
-    RCHK(boxop(c.l_ids[1], c.cc[iphi], c.cc[AFGPU_CC_TMP], OP_COPY));
-    RCHK(vcycle_body(op, set_residual && c.highest_lvl == 1, 1));
+    RCHK(boxop<T>(c.l_ids[1], v.phi, v.tmp, OP_COPY));
+    RCHK(vcycle_body(v, set_residual && c.highest_lvl == 1, 1));
     for (int lvl = 2; lvl <= c.highest_lvl; lvl++) {
-      RCHK(boxop(c.l_ids[lvl], c.cc[iphi], c.cc[AFGPU_CC_TMP], OP_COPY));
-      RCHK(correct_children(lvl - 1, op));
-      RCHK(gc_lvl(lvl, iphi, true));
-      RCHK(vcycle_body(op, set_residual && lvl == c.highest_lvl, lvl));
+      RCHK(boxop<T>(c.l_ids[lvl], v.phi, v.tmp, OP_COPY));
+      RCHK(correct_children(v, lvl - 1));
+      RCHK(gc_var(lvl, v.phi, v.m, true));
+      RCHK(vcycle_body(v, set_residual && lvl == c.highest_lvl, lvl));
     }
     return AFGPU_OK;
+  }
+
+  // --- mixed precision: FP64 defect correction around FP32 cycles ---------
+  // One closing pass of a correction cycle: (optionally) phi += e, e = 0 on
+  // every box; then phi on the refined (covered) cells := restriction of
+  // the finer phi, level by level, which is what every FAS cycle starts
+  // with (update_coarse, m_af_multigrid.f90:691-740) and what the composite
+  // residual of the coarse leaf cells next to refined boxes depends on, so
+  // the iteration has the plain form's fixed point; then the FP64 residual
+  // of phi on the leaves into the FP32 rhs of the next cycle, with its max
+  // reduced into reduction slot 0.
+  int mp_close(int op, bool correct) {
+    const int iphi = op_phi(op);
+    if (correct) {
+      const List &l = c.l_all_ids;
+      RCHK(launch(k_mp_correct<G::NCELL, BS>, l.n * bpb(l.n, G::NCELL), c.list(l), c.cc[iphi], c.cc32[0],
+                  bpb(l.n, G::NCELL)));
+    }
+    for (int lvl = c.highest_lvl; lvl >= 2; lvl--) {
+      const List &l = c.l_ids[lvl];
+      const int k = bpb(l.n, G::NINT / (1 << D));
+      RCHK(launch(k_restrict<D, NC, BS, double>, l.n * k, c.list(l), (const DBox *)c.d_boxes,
+                  (const double *)c.cc[iphi], c.cc[iphi], 0, (int)(c.coord == AFGPU_CYL), k));
+    }
+    // The refinement-boundary ghosts of a level-l box (mg_sides_rb) read
+    // the level l-1 neighbour including its stored ghost layer, so that
+    // layer is refreshed on every coarser level, from level 1 up.  The
+    // finest leaves' own ghosts are formed inside the residual kernel.
+    const VarBc m = var_methods(c, iphi);
+    for (int lvl = 1; lvl < c.highest_lvl; lvl++) RCHK(gc_var(lvl, c.cc[iphi], m, true));
+    RCHK(set_red(0, 0ull));
+    const List &l = c.l_all_leaves;
+    const int k = bpb(l.n, G::NINT);
+    return launch(k_mp_residual<D, NC, BS>, l.n * k, c.list(l), (const DBox *)c.d_boxes, (const double *)c.cc[iphi],
+                  (const double *)c.cc[AFGPU_CC_RHS], c.cc32[1], c.ops[op].lambda2, (int)(c.coord == AFGPU_CYL),
+                  var_methods(c, iphi), c.d_red, k);
   }
 
   int read_red(int n) {
@@ -547,11 +606,27 @@ struct Impl : ImplBase {
     return AFGPU_OK;
   }
 
-  // Mixed precision: FP64 defect correction around FP32 V-cycles.
+  // Mixed precision: FP64 defect correction around FP32 V-cycles.  Each
+  // iteration solves L e = r approximately with one single-precision FAS
+  // V-cycle from e = 0 (smoothing, restriction and prolongation in FP32, the
+  // level-1 solve in FP64), adds e to phi in FP64 and forms the new residual
+  // in FP64; the loop stops on the same FP64 leaf-residual test as the
+  // plain form (m_field.f90:515-521).
   int mixed_vcycles(double threshold, int *count, double *last) {
-    set_err("mixed precision is not built in this library version");
-    (void)threshold; (void)count; (void)last;
-    return AFGPU_ERR_UNSUPPORTED;
+    const int op = AFGPU_OP_POISSON;
+    RCHK(ensure_factors(op));
+    RCHK(run_graph(c, {G_MP_RES, op, 0, 0}, [&] { return mp_close(op, false); }));
+    for (int i = 1; i <= c.p.mg_num_vcycles; i++) {
+      RCHK(run_graph(c, {G_MP_VCYCLE, op, 0, c.highest_lvl}, [&] {
+        RCHK(vcycle_body(mg32(op), false, c.highest_lvl));
+        return mp_close(op, true);
+      }));
+      RCHK(leaf_residual_max(last));
+      (*count)++;
+      if (*last < threshold) break;
+    }
+    // the stored ghost layer of phi, as the plain form leaves it
+    return run_graph(c, {G_MP_GC, op, 0, 0}, [&] { return gc_tree(op_phi(op)); });
   }
 
   // --- fluid (src/m_fluid.f90, afivo/src/m_af_flux_schemes.f90) -------------
@@ -637,11 +712,29 @@ struct Impl : ImplBase {
     max_rhs = std::max(max_rhs, std::sqrt(std::numeric_limits<double>::epsilon()));
     for (int n = 0; n < c.p.photoi_n_modes; n++) {
       const int op = AFGPU_OP_HELMH + n;
-      for (int i = 1; i <= c.p.photoi_max_fmg_cycles; i++) {
-        RCHK(fmg(op, true, true));
-        double residu;
-        RCHK(leaf_residual_max(&residu));
-        if (residu / max_rhs < c.p.photoi_max_rel_residual) break;
+      if (c.precision == AFGPU_PREC_MIXED) {
+        // FP64 defect correction around single-precision FMG cycles on the
+        // correction (from e = 0, rhs = the FP64 residual of the previous
+        // solution of this mode)
+        RCHK(ensure_factors(op));
+        RCHK(run_graph(c, {G_MP_RES, op, 0, 0}, [&] { return mp_close(op, false); }));
+        for (int i = 1; i <= c.p.photoi_max_fmg_cycles; i++) {
+          RCHK(run_graph(c, {G_MP_FMG, op, 0, 0}, [&] {
+            RCHK(fmg_body(mg32(op), false, false));
+            return mp_close(op, true);
+          }));
+          double residu;
+          RCHK(leaf_residual_max(&residu));
+          if (residu / max_rhs < c.p.photoi_max_rel_residual) break;
+        }
+        RCHK(run_graph(c, {G_MP_GC, op, 0, 0}, [&] { return gc_tree(op_phi(op)); }));
+      } else {
+        for (int i = 1; i <= c.p.photoi_max_fmg_cycles; i++) {
+          RCHK(fmg(op, true, true));
+          double residu;
+          RCHK(leaf_residual_max(&residu));
+          if (residu / max_rhs < c.p.photoi_max_rel_residual) break;
+        }
       }
       RCHK(boxop(c.l_all_leaves, c.cc[op_phi(op)], c.cc[AFGPU_CC_PHOTO], OP_AXPY_NEG, c.p.photoi_coeff[n]));
     }

@@ -601,9 +601,10 @@ __global__ void __launch_bounds__(BS) k_maxabs(const int *__restrict__ ids, cons
 // coarse_solver_set_rhs_phi (:286-338) does, transform with V^-1 along each
 // axis, divide by the eigenvalue sums, transform back with V.  One
 // workgroup of CBS threads; the box lives in LDS.
-template <int D, int NC, int CBS>
-__global__ void __launch_bounds__(CBS) k_coarse_solve(int id, const DBox *__restrict__ boxes, const double *rhs,
-                                                      double *phi, VarBc m, double lambda2, int cyl,
+// TS is the storage type of rhs and phi; the transforms always run in FP64.
+template <int D, int NC, int CBS, typename TS>
+__global__ void __launch_bounds__(CBS) k_coarse_solve(int id, const DBox *__restrict__ boxes, const TS *rhs,
+                                                      TS *phi, VarBc m, double lambda2, int cyl,
                                                       const double *__restrict__ vinv,
                                                       const double *__restrict__ v,
                                                       const double *__restrict__ eig) {
@@ -613,12 +614,12 @@ __global__ void __launch_bounds__(CBS) k_coarse_solve(int id, const DBox *__rest
   __shared__ double buf[N];
   const DBox &bx = boxes[id - 1];
   const Sten<D, double> s = lpl_stencil<D, double>(bx, lambda2);
-  const double *r = rhs + (size_t)(id - 1) * G::NCELL;
+  const TS *r = rhs + (size_t)(id - 1) * G::NCELL;
   auto lin = [](int i, int j, int k) { return (i - 1) + NC * ((j - 1) + NC * (D == 3 ? k - 1 : 0)); };
   for (int q = threadIdx.x; q < N; q += CBS) {
     int i, j, k;
     interior_ijk<D, NC>(q, i, j, k);
-    double bval = r[G::idx(i, j, k)];
+    double bval = (double)r[G::idx(i, j, k)];
     int p[3] = {i, j, (D == 3) ? k : 1};
     for (int nb = 0; nb < 2 * D; nb++) {
       if (bx.nb[nb] >= 0) continue;
@@ -680,11 +681,91 @@ __global__ void __launch_bounds__(CBS) k_coarse_solve(int id, const DBox *__rest
     }
     __syncthreads();
   }
-  double *u = phi + (size_t)(id - 1) * G::NCELL;
+  TS *u = phi + (size_t)(id - 1) * G::NCELL;
   for (int q = threadIdx.x; q < N; q += CBS) {
     int i, j, k;
     interior_ijk<D, NC>(q, i, j, k);
-    u[G::idx(i, j, k)] = buf[q];
+    u[G::idx(i, j, k)] = (TS)buf[q];
+  }
+}
+
+// ---- mixed precision: the FP64 side of the defect correction ---------------
+// The inner multigrid cycle works on the correction e (single precision,
+// homogeneous boundary conditions) of the FP64 potential; these two kernels
+// close each cycle in FP64.
+
+// phi += e on whole boxes, then e = 0 so the next cycle starts from a zero
+// correction.
+template <int NCELL, int BS>
+__global__ void __launch_bounds__(BS) k_mp_correct(const int *__restrict__ ids, double *phi, float *e, int bpb) {
+  const Part pt = part_of<BS>(ids, bpb);
+  const size_t base = (size_t)(pt.id - 1) * NCELL;
+  for (int x = pt.q0; x < NCELL; x += pt.stride) {
+    phi[base + x] = phi[base + x] + (double)e[base + x];
+    e[base + x] = 0.0f;
+  }
+}
+
+// FP64 residual rhs - L phi on the interior cells of a box list, with every
+// neighbour outside the box taken from ghost_value() (so the stored ghost
+// layer of phi need not be current), stored in single precision as the rhs
+// of the next correction cycle; max |residual| over leaf boxes is reduced
+// into *leafmax.
+template <int D, int NC, int BS>
+__global__ void __launch_bounds__(BS) k_mp_residual(const int *__restrict__ ids, const DBox *__restrict__ boxes,
+                                                    const double *__restrict__ phi, const double *__restrict__ rhs,
+                                                    float *r32, double lambda2, int cyl, VarBc m,
+                                                    unsigned long long *leafmax, int bpb) {
+  using G = Geo<D, NC>;
+  __shared__ double sh[BS / 64];
+  const Part pt = part_of<BS>(ids, bpb);
+  const int id = pt.id;
+  const DBox &bx = boxes[id - 1];
+  const Sten<D, double> s = lpl_stencil<D, double>(bx, lambda2);
+  const size_t base = (size_t)(id - 1) * G::NCELL;
+  const double *u = phi + base;
+  constexpr int st[3] = {1, G::NG, G::NG * G::NG};
+  double mx = 0;
+  for (int q = pt.q0; q < G::NINT; q += pt.stride) {
+    int i, j, k;
+    interior_ijk<D, NC>(q, i, j, k);
+    const int x = G::idx(i, j, k);
+    const int pos[3] = {i, j, k};
+    const double uc = u[x];
+    double nbv[6];
+    for (int nb = 0; nb < 2 * D; nb++) {
+      const int d = nb / 2;
+      const bool low = (nb % 2) == 0;
+      const bool edge = low ? pos[d] == 1 : pos[d] == NC;
+      if (!edge) {
+        nbv[nb] = u[low ? x - st[d] : x + st[d]];
+      } else {
+        int t0, t1;
+        tangential(D, d, t0, t1);
+        const double x2 = u[low ? x + st[d] : x - st[d]];
+        nbv[nb] = ghost_value<D, NC, double>(boxes, bx, phi, m, nb, pos[t0], (D == 3) ? pos[t1] : 0, uc, x2);
+      }
+    }
+    double lp;
+    if (D == 2) {
+      if (cyl) {
+        double c1, c2, c3;
+        cyl_coeffs<double>(bx, i, s.c, c1, c2, c3);
+        lp = c1 * uc + c2 * nbv[0] + c3 * nbv[1] + s.c[3] * nbv[2] + s.c[4] * nbv[3];
+      } else {
+        lp = s.c[0] * uc + s.c[1] * nbv[0] + s.c[2] * nbv[1] + s.c[3] * nbv[2] + s.c[4] * nbv[3];
+      }
+    } else {
+      lp = s.c[0] * uc + s.c[1] * nbv[0] + s.c[2] * nbv[1] + s.c[3] * nbv[2] + s.c[4] * nbv[3] +
+           s.c[5] * nbv[4] + s.c[6] * nbv[5];
+    }
+    const double rv = rhs[base + x] - lp;
+    r32[base + x] = (float)rv;
+    mx = fmax(mx, fabs(rv));
+  }
+  if (bx.children[0] == 0) {
+    mx = block_max<BS>(mx, sh);
+    if (threadIdx.x == 0) atomic_max_nonneg(leafmax, mx);
   }
 }
 

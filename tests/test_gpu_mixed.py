@@ -1,0 +1,130 @@
+"""The mixed-precision form (FP32 multigrid cycles inside FP64 defect
+correction) against the plain FP64 form, which equals the CPU oracle bit for
+bit (test_gpu_parity.py).
+
+The two forms are different iterations with the same fixed point: the
+composite leaf equations with the refined coarse cells equal to the
+restriction of the finer solution.  Both are compared on leaf-box interiors
+(the values the physics reads); refined coarse cells hold FAS auxiliary
+values in the plain form.
+
+* fixed point: both forms iterated to their round-off floor agree to
+  ~1e-13 (asserted 1e-11);
+* time steps at the production threshold (leaf residual 1e-8 of max |rhs|,
+  or the absolute floor of m_field.f90:500-504 where that is larger): the
+  potential, field and densities after several steps agree to the
+  accuracy the threshold leaves (asserted 1e-6 to 1e-5);
+* photoionization to a 1e-8 relative residual: the photoionization source
+  agrees to 1e-7.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+CASES = {
+    "cyl2d_amr": lambda S: S.small_amr(2, coord=2, uniform_lvl=2, max_lvl=4, seed=3),
+    "xyz3d_amr": lambda S: S.small_amr(3, nc=8, uniform_lvl=2, max_lvl=3, frac=0.3, seed=11),
+    "xyz3d_16": lambda S: S.small_amr(3, nc=16, uniform_lvl=2, max_lvl=3, frac=0.3, seed=7),
+}
+
+
+def leaf_interior(case, p, iv):
+    ex = case.extra["export"]
+    D, nc = case.D, case.nc
+    ch = ex["children"].reshape(ex["n_boxes"], 2 ** D)
+    leaves = np.nonzero(ch[:, 0] == 0)[0] + 1
+    a = p.get_cc(iv, ids=leaves).reshape((len(leaves),) + (nc + 2,) * D)
+    sl = (slice(None),) + (slice(1, nc + 1),) * D
+    return a[sl]
+
+
+def rel_max(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def setup(case, photoi=False):
+    case.params.mg_num_vcycles = 100
+    case.params.mg_max_rel_residual = 1e-8
+    if photoi:
+        case.params.photoi_enabled = 1
+        case.params.photoi_per_steps = 1
+        case.params.photoi_max_rel_residual = 1e-8
+    return case
+
+
+def pair(A, S, case):
+    f = S.apply(A.Path("gpu", case.D, case.nc, case.coord, A.PREC_FP64), case)
+    m = S.apply(A.Path("gpu", case.D, case.nc, case.coord, A.PREC_MIXED), case)
+    return f, m
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_mixed_poisson_fixed_point(A, S, name):
+    case = setup(CASES[name](S))
+    f, m = pair(A, S, case)
+    nf, rf = f.field_compute(0, False)
+    nm, rm = m.field_compute(0, False)
+    print(name, "initial solve fp64", nf, rf, "mixed", nm, rm, flush=True)
+    assert nm <= nf + 2
+    for _ in range(10):
+        f.mg_fas_vcycle(A.OP_POISSON, True, 0)
+        m.field_compute(0, True)  # at least one more mixed cycle each call
+    e = rel_max(leaf_interior(case, m, A.CC_PHI), leaf_interior(case, f, A.CC_PHI))
+    print(name, "fixed point phi %.2e" % e, flush=True)
+    assert e < 1e-11
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_mixed_steps_match_fp64(A, S, name):
+    case = setup(CASES[name](S))
+    f, m = pair(A, S, case)
+    f.field_compute(0, False)
+    m.field_compute(0, False)
+    for it in range(5):
+        a, b = f.step(), m.step()
+        print(name, it, "fp64", a["n_vcycles"], "%.2e" % a["rel_residual"],
+              "mixed", b["n_vcycles"], "%.2e" % b["rel_residual"], flush=True)
+        assert b["n_vcycles"] <= a["n_vcycles"] + 2
+        # both stop on the same absolute threshold
+        assert b["rel_residual"] < 10 * max(1e-8, a["rel_residual"])
+    errs = {}
+    for key, iv in (("phi", A.CC_PHI), ("field", A.CC_EFIELD), ("electron", A.CC_ELECTRON),
+                    ("pos_ion", A.CC_POS_ION)):
+        errs[key] = rel_max(leaf_interior(case, m, iv), leaf_interior(case, f, iv))
+    print(name, "errors", {k: "%.2e" % v for k, v in errs.items()}, flush=True)
+    assert errs["phi"] < 1e-6
+    assert errs["field"] < 1e-5
+    assert errs["electron"] < 1e-6
+    assert errs["pos_ion"] < 1e-6
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_mixed_photoi_matches_fp64(A, S, name):
+    case = setup(CASES[name](S), photoi=True)
+    f, m = pair(A, S, case)
+    f.field_compute(0, have_guess=False)
+    m.put_cc(A.CC_PHI, f.get_cc(A.CC_PHI))
+    m.field_from_potential()
+    for rep in range(2):  # the second solve starts from the first one's modes
+        for p in (f, m):
+            p.photoi_set_src()
+        e = rel_max(leaf_interior(case, m, A.CC_PHOTO), leaf_interior(case, f, A.CC_PHOTO))
+        print(name, rep, "photo error %.2e" % e, flush=True)
+        assert e < 1e-7
+
+
+def test_mixed_graph_replays_match_eager(A, S):
+    case = setup(CASES["xyz3d_amr"](S), photoi=True)
+    case.params.photoi_max_rel_residual = 1e-3
+    g1 = S.apply(A.Path("gpu", case.D, case.nc, case.coord, A.PREC_MIXED), case)
+    g2 = S.apply(A.Path("gpu", case.D, case.nc, case.coord, A.PREC_MIXED), case)
+    g2.set_option(A.OPT_GRAPHS, 0)
+    assert g1.field_compute(0, False) == g2.field_compute(0, False)
+    assert np.array_equal(g1.get_cc(A.CC_PHI), g2.get_cc(A.CC_PHI))
+    for it in range(5):
+        a, b = g1.step(), g2.step()
+        assert a == b
+        for iv in (A.CC_ELECTRON, A.CC_PHI, A.CC_PHOTO):
+            assert np.array_equal(g1.get_cc(iv), g2.get_cc(iv)), (it, iv)

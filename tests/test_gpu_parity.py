@@ -40,6 +40,7 @@ CASES = {
     "cyl2d_amr": lambda S: S.small_amr(2, coord=2, uniform_lvl=2, max_lvl=4, seed=3),
     "xyz2d_amr": lambda S: S.small_amr(2, coord=1, uniform_lvl=2, max_lvl=4, seed=5),
     "xyz3d_amr": lambda S: S.small_amr(3, nc=8, uniform_lvl=2, max_lvl=3, frac=0.3, seed=11),
+    "xyz3d_16": lambda S: S.small_amr(3, nc=16, uniform_lvl=2, max_lvl=3, frac=0.3, seed=7),
 }
 
 
@@ -79,7 +80,7 @@ def test_field_and_steps_parity(A, S, name):
     assert np.array_equal(fg, fc_)
 
 
-@pytest.mark.parametrize("name", ["cyl2d_amr", "xyz3d_amr"])
+@pytest.mark.parametrize("name", ["cyl2d_amr", "xyz3d_amr", "xyz3d_16"])
 def test_photoi_parity(A, S, name):
     case = CASES[name](S)
     g, c = make_pair(A, S, case)
