@@ -37,6 +37,7 @@ FC_FLUX, FC_FIELD = 0, 1
 OP_POISSON, OP_HELMH = 0, 1
 MAX_MODES = 3
 OPT_GRAPHS = 1
+OPT_DIRECT_N = 2
 
 
 class ParamsT(C.Structure):

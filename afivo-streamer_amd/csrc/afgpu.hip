@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "afgpu.h"
+#include "afgpu_direct.h"
 #include "afgpu_fluid.h"
 
 using namespace afg;
@@ -118,6 +119,19 @@ struct afgpu_ctx {
     bool ok = false;
     double *d_vinv = nullptr, *d_v = nullptr, *d_eig = nullptr;
   } ops[1 + AFGPU_MAX_MODES];
+  // mixed precision: exact solve on the finest uniformly refined level
+  int direct_max_n = 128;
+  struct Direct {
+    bool set = false;  // lc and the map below are current for the tree
+    int lc = 1, N = 0;
+    int *d_map = nullptr;
+    double *A0 = nullptr, *A1 = nullptr;
+    size_t cap = 0;  // doubles in A0 / A1
+    struct F {
+      bool ok = false;
+      double *vinv = nullptr, *v = nullptr, *vinvT = nullptr, *vT = nullptr, *eig = nullptr;
+    } f[1 + AFGPU_MAX_MODES];
+  } dir;
   // reductions
   unsigned long long *d_red = nullptr, *h_red = nullptr;
   // time loop
@@ -415,14 +429,20 @@ struct Impl : ImplBase {
                      [&] { return vcycle_body(mg64(op), set_residual, max_lvl); });
   }
 
+  // FAS V-cycle from max_lvl down to the coarsest level lc (1: the level-1
+  // box solve of the reference; > 1: the exact uniform-level solve of the
+  // mixed-precision inner cycle) and back.
   template <typename T>
-  int vcycle_body(const Mg<T> &v, bool set_residual, int max_lvl) {
-    for (int lvl = max_lvl; lvl >= 2; lvl--) {
+  int vcycle_body(const Mg<T> &v, bool set_residual, int max_lvl, int lc = 1) {
+    for (int lvl = max_lvl; lvl >= lc + 1; lvl--) {
       RCHK(gsrb_boxes(v, lvl, true));
       RCHK(update_coarse(v, lvl, true));
     }
-    RCHK(solve_coarse(v));
-    for (int lvl = 2; lvl <= max_lvl; lvl++) {
+    if (lc == 1)
+      RCHK(solve_coarse(v));
+    else
+      RCHK(solve_level(v, lc));
+    for (int lvl = lc + 1; lvl <= max_lvl; lvl++) {
       RCHK(correct_children(v, lvl - 1));
       RCHK(gc_var(lvl, v.phi, v.m, true));
       RCHK(gsrb_boxes(v, lvl, false));
@@ -456,11 +476,11 @@ struct Impl : ImplBase {
   }
 
   template <typename T>
-  int fmg_body(const Mg<T> &v, bool set_residual, bool have_guess) {
+  int fmg_body(const Mg<T> &v, bool set_residual, bool have_guess, int lc = 1) {
     if (have_guess) {
-      for (int lvl = c.highest_lvl; lvl >= 2; lvl--) RCHK(update_coarse(v, lvl, false));
+      for (int lvl = c.highest_lvl; lvl >= lc + 1; lvl--) RCHK(update_coarse(v, lvl, false));
     } else {
-      for (int lvl = c.highest_lvl; lvl >= 2; lvl--) {  // init_phi_rhs
+      for (int lvl = c.highest_lvl; lvl >= lc + 1; lvl--) {  // init_phi_rhs
         const List &l = c.l_ids[lvl];
         RCHK(boxop<T>(l, v.phi, v.phi, OP_CLEAR));
         const int k = bpb(l.n, G::NINT / (1 << D));
@@ -468,15 +488,169 @@ struct Impl : ImplBase {
                     1, (int)(c.coord == AFGPU_CYL), k));
       }
     }
-    RCHK(boxop<T>(c.l_ids[1], v.phi, v.tmp, OP_COPY));
-    RCHK(vcycle_body(v, set_residual && c.highest_lvl == 1, 1));
-    for (int lvl = 2; lvl <= c.highest_lvl; lvl++) {
+    RCHK(boxop<T>(c.l_ids[lc], v.phi, v.tmp, OP_COPY));
+    RCHK(vcycle_body(v, set_residual && c.highest_lvl == lc, lc, lc));
+    for (int lvl = lc + 1; lvl <= c.highest_lvl; lvl++) {
       RCHK(boxop<T>(c.l_ids[lvl], v.phi, v.tmp, OP_COPY));
       RCHK(correct_children(v, lvl - 1));
       RCHK(gc_var(lvl, v.phi, v.m, true));
-      RCHK(vcycle_body(v, set_residual && lvl == c.highest_lvl, lvl));
+      RCHK(vcycle_body(v, set_residual && lvl == c.highest_lvl, lvl, lc));
     }
     return AFGPU_OK;
+  }
+
+  // --- exact solve on a uniformly refined level (mixed precision) ---------
+  // The coarsest level of the inner cycle: the highest level lc such that
+  // every level up to lc covers the domain and lc has at most direct_max_n
+  // cells per axis (1 when no such level above 1 exists).
+  int direct_level(int *lc_out) {
+    afgpu_ctx::Direct &d = c.dir;
+    if (!d.set) {
+      int lc = 1;
+      for (int l = 2; l <= c.highest_lvl; l++) {
+        const long long want = 1ll << (D * (l - 1));
+        const int n = NC << (l - 1);
+        if (c.l_ids[l].n != want || n > c.direct_max_n || n < 16) break;
+        lc = l;
+      }
+      d.lc = lc;
+      d.N = NC << (lc - 1);
+      if (lc > 1) {
+        const int nb = 1 << (lc - 1);
+        const int nbox = c.l_ids[lc].n;
+        std::vector<int> map(nbox, 0);
+        for (int q = 0; q < nbox; q++) {
+          const int id = c.h_lists[c.l_ids[lc].off + q];
+          const DBox &b = c.hboxes[id - 1];
+          int pos = 0, mul = 1;
+          for (int dd = 0; dd < D; dd++) {
+            pos += (b.ix[dd] - 1) * mul;
+            mul *= nb;
+          }
+          if (pos < 0 || pos >= nbox || map[pos] != 0) {
+            set_err("uniform level: inconsistent box positions");
+            return AFGPU_ERR_STATE;
+          }
+          map[pos] = id;
+        }
+        (void)hipFree(d.d_map);
+        d.d_map = nullptr;
+        HIPCHK(hipMalloc(&d.d_map, nbox * sizeof(int)));
+        HIPCHK(hipMemcpy(d.d_map, map.data(), nbox * sizeof(int), hipMemcpyHostToDevice));
+        size_t cells = 1;
+        for (int dd = 0; dd < D; dd++) cells *= (size_t)d.N;
+        if (cells > d.cap) {
+          (void)hipFree(d.A0);
+          (void)hipFree(d.A1);
+          d.A0 = d.A1 = nullptr;
+          HIPCHK(hipMalloc(&d.A0, cells * sizeof(double)));
+          HIPCHK(hipMalloc(&d.A1, cells * sizeof(double)));
+          d.cap = cells;
+        }
+      }
+      for (auto &f : d.f) f.ok = false;
+      d.set = true;
+    }
+    *lc_out = d.lc;
+    return AFGPU_OK;
+  }
+
+  // 1-D eigen-factors of the level-lc operator (allocation and upload, so
+  // this runs before any capture).
+  int ensure_direct_factors(int op) {
+    afgpu_ctx::Direct &d = c.dir;
+    afgpu_ctx::Direct::F &f = d.f[op];
+    if (f.ok || d.lc <= 1) return AFGPU_OK;
+    const int N = d.N;
+    const DBox &b1 = c.hboxes[c.h_lists[c.l_ids[1].off] - 1];
+    const DBox &bl = c.hboxes[c.h_lists[c.l_ids[d.lc].off] - 1];
+    int bcd[6] = {0, 0, 0, 0, 0, 0};
+    bcd[2 * (D - 1)] = 1;
+    bcd[2 * (D - 1) + 1] = 1;
+    const size_t nn = (size_t)D * N * N;
+    std::vector<double> vinv(nn), v(nn), eig((size_t)D * N), vinvT(nn), vT(nn);
+    RCHK(afgpu_coarse_factors(D, N, c.coord, b1.r_min, bl.dr, bcd, vinv.data(), v.data(), eig.data()));
+    for (int dd = 0; dd < D; dd++)
+      for (int i = 0; i < N; i++)
+        for (int m = 0; m < N; m++) {
+          const size_t o = (size_t)dd * N * N;
+          vinvT[o + (size_t)m * N + i] = vinv[o + (size_t)i * N + m];
+          vT[o + (size_t)m * N + i] = v[o + (size_t)i * N + m];
+        }
+    double **dst[5] = {&f.vinv, &f.v, &f.vinvT, &f.vT, &f.eig};
+    const std::vector<double> *srcs[5] = {&vinv, &v, &vinvT, &vT, &eig};
+    for (int q = 0; q < 5; q++) {
+      (void)hipFree(*dst[q]);
+      *dst[q] = nullptr;
+      HIPCHK(hipMalloc(dst[q], srcs[q]->size() * sizeof(double)));
+      HIPCHK(hipMemcpy(*dst[q], srcs[q]->data(), srcs[q]->size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    f.ok = true;
+    return AFGPU_OK;
+  }
+
+  template <int N>
+  int direct_transforms(const afgpu_ctx::Direct::F &f, double lambda2) {
+    afgpu_ctx::Direct &d = c.dir;
+    const size_t NN = (size_t)N * N;
+    const int nrows = (D == 3) ? N * N : N;
+    constexpr int TI = (N < 64) ? N : 64;
+    constexpr int TR = (256 / TI) / (N / TI) * 16;
+    const int row_blocks = (nrows + TR - 1) / TR;
+    constexpr int TC = (N < 64) ? N : (N == 64 ? 64 : 32);
+    const int line_blocks = (N / TC) * ((D == 3) ? N : 1);
+    auto rows = [&](const double *in, double *out, const double *MT) {
+      hipLaunchKernelGGL((k_lc_rows<D, N>), dim3(row_blocks), dim3(256), 0, c.stream, in, out, MT, nrows);
+      return hipGetLastError();
+    };
+    auto lines = [&](const double *in, double *out, const double *M, int axis, int div) {
+      hipLaunchKernelGGL((k_lc_lines<D, N>), dim3(line_blocks), dim3(256), 0, c.stream, in, out, M, axis, div,
+                         (const double *)f.eig, lambda2);
+      return hipGetLastError();
+    };
+    if (D == 3) {
+      HIPCHK(rows(d.A0, d.A1, f.vinvT));
+      HIPCHK(lines(d.A1, d.A0, f.vinv + NN, 1, 0));
+      HIPCHK(lines(d.A0, d.A1, f.vinv + 2 * NN, 2, 1));
+      HIPCHK(lines(d.A1, d.A0, f.v + 2 * NN, 2, 0));
+      HIPCHK(lines(d.A0, d.A1, f.v + NN, 1, 0));
+      HIPCHK(rows(d.A1, d.A0, f.vT));
+    } else {
+      HIPCHK(rows(d.A0, d.A1, f.vinvT));
+      HIPCHK(lines(d.A1, d.A0, f.vinv + NN, 1, 1));
+      HIPCHK(lines(d.A0, d.A1, f.v + NN, 1, 0));
+      HIPCHK(rows(d.A1, d.A0, f.vT));
+    }
+    return AFGPU_OK;
+  }
+
+  // phi(lc) := L^-1 rhs(lc) on the whole uniform level (homogeneous
+  // boundary conditions: the correction equation), then its ghost layer.
+  template <typename T>
+  int solve_level(const Mg<T> &v, int lc) {
+    afgpu_ctx::Direct &d = c.dir;
+    const afgpu_ctx::Direct::F &f = d.f[v.op];
+    if (d.lc != lc || !f.ok) {
+      set_err("uniform-level solve used before its set-up");
+      return AFGPU_ERR_STATE;
+    }
+    const int nb = d.N / NC;
+    const int nbox = c.l_ids[lc].n;
+    hipLaunchKernelGGL((k_lc_gather<D, NC, BS, T>), dim3(nbox), dim3(BS), 0, c.stream, (const int *)d.d_map, nb,
+                       (const T *)v.rhs, d.A0);
+    HIPCHK(hipGetLastError());
+    const double l2 = c.ops[v.op].lambda2;
+    switch (d.N) {
+      case 16: RCHK(direct_transforms<16>(f, l2)); break;
+      case 32: RCHK(direct_transforms<32>(f, l2)); break;
+      case 64: RCHK(direct_transforms<64>(f, l2)); break;
+      case 128: RCHK(direct_transforms<128>(f, l2)); break;
+      default: set_err("uniform-level solve: unsupported size"); return AFGPU_ERR_UNSUPPORTED;
+    }
+    hipLaunchKernelGGL((k_lc_scatter<D, NC, BS, T>), dim3(nbox), dim3(BS), 0, c.stream, (const int *)d.d_map, nb,
+                       (const double *)d.A0, v.phi);
+    HIPCHK(hipGetLastError());
+    return gc_var(lc, v.phi, v.m, true);
   }
 
   // --- mixed precision: FP64 defect correction around FP32 cycles ---------
@@ -488,31 +662,49 @@ struct Impl : ImplBase {
   // the iteration has the plain form's fixed point; then the FP64 residual
   // of phi on the leaves into the FP32 rhs of the next cycle, with its max
   // reduced into reduction slot 0.
-  int mp_close(int op, bool correct) {
+  int mp_close(int op, bool correct, int lc) {
     const int iphi = op_phi(op);
     if (correct) {
       const List &l = c.l_all_ids;
       RCHK(launch(k_mp_correct<G::NCELL, BS>, l.n * bpb(l.n, G::NCELL), c.list(l), c.cc[iphi], c.cc32[0],
                   bpb(l.n, G::NCELL)));
     }
-    for (int lvl = c.highest_lvl; lvl >= 2; lvl--) {
-      const List &l = c.l_ids[lvl];
-      const int k = bpb(l.n, G::NINT / (1 << D));
-      RCHK(launch(k_restrict<D, NC, BS, double>, l.n * k, c.list(l), (const DBox *)c.d_boxes,
-                  (const double *)c.cc[iphi], c.cc[iphi], 0, (int)(c.coord == AFGPU_CYL), k));
-    }
+    // (levels below lc are covered everywhere and not part of the iteration)
+    RCHK(restrict_levels(iphi, c.highest_lvl, lc + 1));
     // The refinement-boundary ghosts of a level-l box (mg_sides_rb) read
     // the level l-1 neighbour including its stored ghost layer, so that
-    // layer is refreshed on every coarser level, from level 1 up.  The
-    // finest leaves' own ghosts are formed inside the residual kernel.
+    // layer is refreshed on every coarser level, from lc up.  The finest
+    // leaves' own ghosts are formed inside the residual kernel.
     const VarBc m = var_methods(c, iphi);
-    for (int lvl = 1; lvl < c.highest_lvl; lvl++) RCHK(gc_var(lvl, c.cc[iphi], m, true));
+    for (int lvl = lc; lvl < c.highest_lvl; lvl++) RCHK(gc_var(lvl, c.cc[iphi], m, true));
     RCHK(set_red(0, 0ull));
     const List &l = c.l_all_leaves;
     const int k = bpb(l.n, G::NINT);
     return launch(k_mp_residual<D, NC, BS>, l.n * k, c.list(l), (const DBox *)c.d_boxes, (const double *)c.cc[iphi],
                   (const double *)c.cc[AFGPU_CC_RHS], c.cc32[1], c.ops[op].lambda2, (int)(c.coord == AFGPU_CYL),
-                  var_methods(c, iphi), c.d_red, k);
+                  m, c.d_red, k);
+  }
+
+  // phi on refined cells := restriction of the finer phi (no geometry
+  // weights, as update_coarse restricts phi), levels hi -> lo-1.
+  int restrict_levels(int iphi, int hi, int lo) {
+    for (int lvl = hi; lvl >= lo; lvl--) {
+      const List &l = c.l_ids[lvl];
+      const int k = bpb(l.n, G::NINT / (1 << D));
+      RCHK(launch(k_restrict<D, NC, BS, double>, l.n * k, c.list(l), (const DBox *)c.d_boxes,
+                  (const double *)c.cc[iphi], c.cc[iphi], 0, (int)(c.coord == AFGPU_CYL), k));
+    }
+    return AFGPU_OK;
+  }
+
+  // End of a mixed-precision solve: the levels below lc get the restricted
+  // solution too, and the stored ghost layers are refreshed everywhere.
+  int mp_finish(int op, int lc) {
+    const int iphi = op_phi(op);
+    return run_graph(c, {G_MP_GC, op, lc, 0}, [&] {
+      RCHK(restrict_levels(iphi, lc, 2));
+      return gc_tree(iphi);
+    });
   }
 
   int read_red(int n) {
@@ -608,25 +800,28 @@ struct Impl : ImplBase {
 
   // Mixed precision: FP64 defect correction around FP32 V-cycles.  Each
   // iteration solves L e = r approximately with one single-precision FAS
-  // V-cycle from e = 0 (smoothing, restriction and prolongation in FP32, the
-  // level-1 solve in FP64), adds e to phi in FP64 and forms the new residual
-  // in FP64; the loop stops on the same FP64 leaf-residual test as the
-  // plain form (m_field.f90:515-521).
+  // V-cycle from e = 0 (smoothing, restriction and prolongation in FP32;
+  // the coarsest level solved exactly in FP64: the finest uniformly refined
+  // level when there is one, else the level-1 box), adds e to phi in FP64
+  // and forms the new residual in FP64; the loop stops on the same FP64
+  // leaf-residual test as the plain form (m_field.f90:515-521).
   int mixed_vcycles(double threshold, int *count, double *last) {
     const int op = AFGPU_OP_POISSON;
+    int lc;
     RCHK(ensure_factors(op));
-    RCHK(run_graph(c, {G_MP_RES, op, 0, 0}, [&] { return mp_close(op, false); }));
+    RCHK(direct_level(&lc));
+    RCHK(ensure_direct_factors(op));
+    RCHK(run_graph(c, {G_MP_RES, op, lc, 0}, [&] { return mp_close(op, false, lc); }));
     for (int i = 1; i <= c.p.mg_num_vcycles; i++) {
-      RCHK(run_graph(c, {G_MP_VCYCLE, op, 0, c.highest_lvl}, [&] {
-        RCHK(vcycle_body(mg32(op), false, c.highest_lvl));
-        return mp_close(op, true);
+      RCHK(run_graph(c, {G_MP_VCYCLE, op, lc, c.highest_lvl}, [&] {
+        RCHK(vcycle_body(mg32(op), false, c.highest_lvl, lc));
+        return mp_close(op, true, lc);
       }));
       RCHK(leaf_residual_max(last));
       (*count)++;
       if (*last < threshold) break;
     }
-    // the stored ghost layer of phi, as the plain form leaves it
-    return run_graph(c, {G_MP_GC, op, 0, 0}, [&] { return gc_tree(op_phi(op)); });
+    return mp_finish(op, lc);
   }
 
   // --- fluid (src/m_fluid.f90, afivo/src/m_af_flux_schemes.f90) -------------
@@ -716,18 +911,21 @@ struct Impl : ImplBase {
         // FP64 defect correction around single-precision FMG cycles on the
         // correction (from e = 0, rhs = the FP64 residual of the previous
         // solution of this mode)
+        int lc;
         RCHK(ensure_factors(op));
-        RCHK(run_graph(c, {G_MP_RES, op, 0, 0}, [&] { return mp_close(op, false); }));
+        RCHK(direct_level(&lc));
+        RCHK(ensure_direct_factors(op));
+        RCHK(run_graph(c, {G_MP_RES, op, lc, 0}, [&] { return mp_close(op, false, lc); }));
         for (int i = 1; i <= c.p.photoi_max_fmg_cycles; i++) {
-          RCHK(run_graph(c, {G_MP_FMG, op, 0, 0}, [&] {
-            RCHK(fmg_body(mg32(op), false, false));
-            return mp_close(op, true);
+          RCHK(run_graph(c, {G_MP_FMG, op, lc, 0}, [&] {
+            RCHK(fmg_body(mg32(op), false, false, lc));
+            return mp_close(op, true, lc);
           }));
           double residu;
           RCHK(leaf_residual_max(&residu));
           if (residu / max_rhs < c.p.photoi_max_rel_residual) break;
         }
-        RCHK(run_graph(c, {G_MP_GC, op, 0, 0}, [&] { return gc_tree(op_phi(op)); }));
+        RCHK(mp_finish(op, lc));
       } else {
         for (int i = 1; i <= c.p.photoi_max_fmg_cycles; i++) {
           RCHK(fmg(op, true, true));
@@ -852,6 +1050,16 @@ int afgpu_destroy(afgpu_t *h) {
   for (int i = 0; i < AFGPU_N_CC; i++) (void)hipFree(h->cc[i]);
   for (int i = 0; i < AFGPU_N_FC; i++) (void)hipFree(h->fc[i]);
   for (int i = 0; i < 3; i++) (void)hipFree(h->cc32[i]);
+  (void)hipFree(h->dir.d_map);
+  (void)hipFree(h->dir.A0);
+  (void)hipFree(h->dir.A1);
+  for (auto &f : h->dir.f) {
+    (void)hipFree(f.vinv);
+    (void)hipFree(f.v);
+    (void)hipFree(f.vinvT);
+    (void)hipFree(f.vT);
+    (void)hipFree(f.eig);
+  }
   (void)hipFree(h->d_boxes);
   (void)hipFree(h->d_lists);
   (void)hipFree(h->d_td);
@@ -1034,6 +1242,8 @@ int afgpu_sync_tree(afgpu_t *h, int n_boxes, const int *lvl, const int *ix, cons
   h->n_boxes = n_boxes;
   h->highest_lvl = hl;
   for (auto &o : h->ops) o.ok = false;  // level-1 box may have changed
+  h->dir.set = false;
+  for (auto &f : h->dir.f) f.ok = false;
   drop_graphs(*h);
   return AFGPU_OK;
 }
@@ -1338,6 +1548,17 @@ int afgpu_set_option(afgpu_t *h, int option, int value) {
       RCHK(sync_stream(h));
       drop_graphs(*h);
       h->use_graphs = value != 0;
+      return AFGPU_OK;
+    case AFGPU_OPT_DIRECT_N:
+      if (value > 128) {
+        set_err("afgpu_set_option: direct solve limited to 128 cells per axis");
+        return AFGPU_ERR_ARG;
+      }
+      RCHK(sync_stream(h));
+      drop_graphs(*h);
+      h->direct_max_n = value;
+      h->dir.set = false;
+      for (auto &f : h->dir.f) f.ok = false;
       return AFGPU_OK;
     default:
       set_err("afgpu_set_option: unknown option");

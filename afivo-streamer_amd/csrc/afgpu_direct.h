@@ -1,0 +1,153 @@
+// Exact solve on a uniformly refined level (mixed-precision form only).
+//
+// When every level up to lc covers the whole domain, the level-lc operator
+// (afivo's constant-coefficient 3/5/7-point stencil with the physical
+// boundary conditions folded in, m_af_stencil.f90:367-496,
+// m_af_ghostcell.f90:173-279) is a Kronecker sum of 1-D operators, exactly
+// like the level-1 box handled by k_coarse_solve.  The mixed-precision
+// inner cycle solves its correction equation exactly on that level with the
+// same fast diagonalisation, applied to the whole level at once: the
+// level's boxes are gathered into one contiguous N^D grid (x fastest), each
+// axis is transformed with the 1-D eigenvector matrix (small dense
+// products, tiled through LDS), divided by the eigenvalue sums, and
+// transformed back.  All transforms run in FP64.
+#pragma once
+
+#include "afgpu_common.h"
+
+namespace afg {
+
+// Interior cells of the level's boxes <-> the contiguous grid.  `map` lists
+// the box id of every box position (bx + nb*(by + nb*bz)), one workgroup
+// per box.
+template <int D, int NC, int BS, typename T>
+__global__ void __launch_bounds__(BS) k_lc_gather(const int *__restrict__ map, int nb, const T *__restrict__ src,
+                                                  double *__restrict__ A) {
+  using G = Geo<D, NC>;
+  const int b = blockIdx.x;
+  const int id = map[b];
+  const int bx = b % nb, by = (b / nb) % nb, bz = (D == 3) ? b / (nb * nb) : 0;
+  const int N = nb * NC;
+  const T *u = src + (size_t)(id - 1) * G::NCELL;
+  for (int q = threadIdx.x; q < G::NINT; q += BS) {
+    const int i = q % NC, j = (q / NC) % NC, k = (D == 3) ? q / (NC * NC) : 0;
+    const size_t g = (size_t)(bx * NC + i) + (size_t)N * ((by * NC + j) + (size_t)N * (bz * NC + k));
+    A[g] = (double)u[G::idx(i + 1, j + 1, (D == 3) ? k + 1 : 0)];
+  }
+}
+
+template <int D, int NC, int BS, typename T>
+__global__ void __launch_bounds__(BS) k_lc_scatter(const int *__restrict__ map, int nb, const double *__restrict__ A,
+                                                   T *__restrict__ dst) {
+  using G = Geo<D, NC>;
+  const int b = blockIdx.x;
+  const int id = map[b];
+  const int bx = b % nb, by = (b / nb) % nb, bz = (D == 3) ? b / (nb * nb) : 0;
+  const int N = nb * NC;
+  T *u = dst + (size_t)(id - 1) * G::NCELL;
+  for (int q = threadIdx.x; q < G::NINT; q += BS) {
+    const int i = q % NC, j = (q / NC) % NC, k = (D == 3) ? q / (NC * NC) : 0;
+    const size_t g = (size_t)(bx * NC + i) + (size_t)N * ((by * NC + j) + (size_t)N * (bz * NC + k));
+    u[G::idx(i + 1, j + 1, (D == 3) ? k + 1 : 0)] = (T)A[g];
+  }
+}
+
+// Transform along axis a >= 1 (y, or z in 3D):
+//   out[.., i, .., x] = sum_m M[i][m] in[.., m, .., x]
+// A workgroup takes TC consecutive x columns at one fixed value of the
+// remaining coordinate; the N x TC input tile is staged in LDS, lanes run
+// over the columns (coalesced) and the thread groups over output rows.
+// With `div`, the result is divided by the eigenvalue sum of its cell
+// (the last forward transform).
+template <int D, int N>
+__global__ void __launch_bounds__(256) k_lc_lines(const double *__restrict__ in, double *__restrict__ out,
+                                                  const double *__restrict__ M, int axis, int div,
+                                                  const double *__restrict__ eig, double lambda2) {
+  constexpr int TC = (N < 64) ? N : (N == 64 ? 64 : 32);
+  constexpr int G = 256 / TC;
+  constexpr int RPT = (N + G - 1) / G;
+  __shared__ double tile[N * TC];
+  const int nxc = N / TC;
+  const int xc = blockIdx.x % nxc, o = blockIdx.x / nxc;  // o: the other coordinate
+  const int x0 = xc * TC;
+  const size_t sa = (axis == 1) ? (size_t)N : (size_t)N * N;            // stride of the axis
+  const size_t so = (D == 2) ? 0 : ((axis == 1) ? (size_t)N * N : (size_t)N);  // stride of o
+  const size_t base = (size_t)o * so + x0;
+  for (int q = threadIdx.x; q < N * TC; q += 256) {
+    const int m = q / TC, c = q % TC;
+    tile[q] = in[base + (size_t)m * sa + c];
+  }
+  __syncthreads();
+  const int c = threadIdx.x % TC, g = threadIdx.x / TC;
+  double acc[RPT];
+  for (int r = 0; r < RPT; r++) acc[r] = 0.0;
+  for (int m = 0; m < N; m++) {
+    const double t = tile[m * TC + c];
+    for (int r = 0; r < RPT; r++) {
+      const int i = g + r * G;
+      if (i < N) acc[r] = acc[r] + M[i * N + m] * t;
+    }
+  }
+  for (int r = 0; r < RPT; r++) {
+    const int i = g + r * G;
+    if (i >= N) break;
+    double v = acc[r];
+    if (div) {
+      const int x = x0 + c;
+      int y, z;
+      if (D == 2) {
+        y = i;
+        z = 0;
+      } else if (axis == 1) {
+        y = i;
+        z = o;
+      } else {
+        y = o;
+        z = i;
+      }
+      double den = eig[x] + eig[N + y];
+      if (D == 3) den = den + eig[2 * N + z];
+      den = den - lambda2;
+      v = v / den;
+    }
+    out[base + (size_t)i * sa + c] = v;
+  }
+}
+
+// Transform along x (contiguous lines): out[r][i] = sum_m in[r][m] MT[m][i]
+// with MT the transposed matrix.  A workgroup takes TR consecutive rows,
+// staged in LDS (padded rows); lanes run over i (coalesced stores and
+// matrix loads), thread groups over (i chunk, row subset).
+template <int D, int N>
+__global__ void __launch_bounds__(256) k_lc_rows(const double *__restrict__ in, double *__restrict__ out,
+                                                 const double *__restrict__ MT, int nrows) {
+  constexpr int TI = (N < 64) ? N : 64;
+  constexpr int NIC = N / TI;
+  constexpr int G = 256 / TI;
+  constexpr int RG = G / NIC;
+  constexpr int RPT = 16;
+  constexpr int TR = RG * RPT;
+  constexpr int LD = N + 1;
+  __shared__ double tile[TR * LD];
+  const int r0 = blockIdx.x * TR;
+  for (int q = threadIdx.x; q < TR * N; q += 256) {
+    const int rr = q / N, m = q % N;
+    tile[rr * LD + m] = (r0 + rr < nrows) ? in[(size_t)(r0 + rr) * N + m] : 0.0;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x % TI, g = threadIdx.x / TI;
+  const int ic = g % NIC, rg = g / NIC;
+  const int i = ic * TI + lane;
+  double acc[RPT];
+  for (int r = 0; r < RPT; r++) acc[r] = 0.0;
+  for (int m = 0; m < N; m++) {
+    const double w = MT[m * N + i];
+    for (int r = 0; r < RPT; r++) acc[r] = acc[r] + tile[(rg * RPT + r) * LD + m] * w;
+  }
+  for (int r = 0; r < RPT; r++) {
+    const int row = r0 + rg * RPT + r;
+    if (row < nrows) out[(size_t)row * N + i] = acc[r];
+  }
+}
+
+}  // namespace afg

@@ -72,7 +72,7 @@ extern "C" int afgpu_coarse_factors(int ndim, int nc, int coord,
                                     const double *r_min, const double *dr,
                                     const int *bc_dirichlet, double *vinv,
                                     double *v, double *eig) {
-  if ((ndim != 2 && ndim != 3) || nc < 2 || nc > 64 || !r_min || !dr ||
+  if ((ndim != 2 && ndim != 3) || nc < 2 || nc > 256 || !r_min || !dr ||
       !bc_dirichlet || !vinv || !v || !eig)
     return AFGPU_ERR_ARG;
   if (coord == AFGPU_CYL && ndim != 2) return AFGPU_ERR_ARG;

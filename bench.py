@@ -42,9 +42,11 @@ def make_case(args):
     return case
 
 
-def run_gpu(case, precision, warmup, steps, timing_steps):
+def run_gpu(case, precision, warmup, steps, timing_steps, direct_n=None):
     p = A.Path("gpu", case.D, case.nc, case.coord, precision)
     S.apply(p, case)
+    if direct_n is not None:
+        p.set_option(A.OPT_DIRECT_N, direct_n)
     t0 = time.time()
     p.field_compute(0, have_guess=False)
     log("  initial field solve %.2f s" % (time.time() - t0))
@@ -108,6 +110,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--timing-steps", type=int, default=2,
                     help="eager steps after the timed region with per-kernel HIP event timing")
+    ap.add_argument("--direct-n", type=int, default=128,
+                    help="mixed precision: largest uniform level (cells per axis) solved exactly "
+                         "by the inner cycle; 0 = descend to level 1")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64-ref", action="store_true")
     args = ap.parse_args()
@@ -121,7 +126,8 @@ def main():
     log("workload", case.name, info)
     prec = A.PREC_MIXED if args.precision == "mixed" else A.PREC_FP64
     log("GPU %s run" % args.precision)
-    p, infos, secs, stats = run_gpu(case, prec, args.warmup, args.steps, args.timing_steps)
+    p, infos, secs, stats = run_gpu(case, prec, args.warmup, args.steps, args.timing_steps,
+                                    args.direct_n if prec == A.PREC_MIXED else None)
     cell_steps = sum(i["leaf_cells"] for i in infos)
     value = cell_steps / secs
     nv = [i["n_vcycles"] for i in infos]
@@ -203,7 +209,8 @@ def main():
         "config": {"workload": case.name, "leaf_boxes": info["n_leaves"], "leaf_cells": info["leaf_cells"],
                    "boxes": info["n_boxes"], "levels": info["highest_lvl"], "box_cells": "16^3",
                    "precision": args.precision, "rel_residual": args.rel_residual,
-                   "photoi_per_steps": args.photoi_per_steps, "mean_vcycles": float(np.mean(nv))},
+                   "photoi_per_steps": args.photoi_per_steps, "mean_vcycles": float(np.mean(nv)),
+                   "direct_n": args.direct_n if prec == A.PREC_MIXED else None},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -259,6 +259,9 @@ int afgpu_kernel_stats(afgpu_t *h, int kind, int reset, double *total_ms,
 
 /* Execution options. */
 #define AFGPU_OPT_GRAPHS 1 /* capture V-cycle / FMG launch sequences as HIP graphs (default 1) */
+#define AFGPU_OPT_DIRECT_N 2 /* mixed precision: largest uniformly refined level (cells per axis,
+                                16..128) that the inner cycle solves exactly; below 16 the
+                                inner cycle descends to level 1 (default 128) */
 int afgpu_set_option(afgpu_t *h, int option, int value);
 
 #ifdef __cplusplus

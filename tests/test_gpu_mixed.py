@@ -115,6 +115,39 @@ def test_mixed_photoi_matches_fp64(A, S, name):
         assert e < 1e-7
 
 
+@pytest.mark.parametrize("direct_n", [0, 16, 32, 64])
+def test_mixed_inner_coarse_level(A, S, direct_n):
+    """The inner cycle's coarsest level: the level-1 box (direct_n = 0) or
+    the exact solve on the finest uniformly refined level with at most
+    direct_n cells per axis.  All reach the FP64 fixed point."""
+    case = setup(S.small_amr(2, coord=2, uniform_lvl=4, max_lvl=5, frac=0.3, seed=5))
+    f, m = pair(A, S, case)
+    m.set_option(A.OPT_DIRECT_N, direct_n)
+    f.field_compute(0, False)
+    n0, r0 = m.field_compute(0, False)
+    for _ in range(8):
+        f.mg_fas_vcycle(A.OP_POISSON, True, 0)
+        m.field_compute(0, True)
+    e = rel_max(leaf_interior(case, m, A.CC_PHI), leaf_interior(case, f, A.CC_PHI))
+    print("direct_n", direct_n, "initial cycles", n0, "fixed point phi %.2e" % e, flush=True)
+    assert e < 1e-11
+
+
+def test_mixed_uniform_tree_is_one_direct_solve(A, S):
+    """A uniformly refined tree is solved by the exact level solve alone:
+    one correction cycle reaches the threshold."""
+    case = setup(S.config0(uniform_lvl=4))  # 2D cylindrical, 64 x 64 cells
+    f, m = pair(A, S, case)
+    f.field_compute(0, False)
+    m.field_compute(0, False)
+    for it in range(3):
+        a, b = f.step(), m.step()
+        print(it, a["n_vcycles"], b["n_vcycles"], b["rel_residual"], flush=True)
+        assert b["n_vcycles"] <= 2
+    e = rel_max(leaf_interior(case, m, A.CC_ELECTRON), leaf_interior(case, f, A.CC_ELECTRON))
+    assert e < 1e-9
+
+
 def test_mixed_graph_replays_match_eager(A, S):
     case = setup(CASES["xyz3d_amr"](S), photoi=True)
     case.params.photoi_max_rel_residual = 1e-3
