@@ -441,3 +441,167 @@ def test_chemistry_source_closed_form(A, S):
     # net charge is conserved by the chemistry
     q = sum(sg * p.get_cc(iv) for sg, iv in ((-1, A.CC_ELECTRON), (1, A.CC_POS_ION), (-1, A.CC_NEG_ION)))
     assert np.abs(q - (-n0)).max() <= 1e-15 * n0.max() * 10
+
+
+# ---- flux of a linear profile across refinement boundaries ---------------------
+def test_upwind_flux_of_linear_profile_on_amr_tree(A, S):
+    """As above on a refined 3D tree: the two-layer halo at refinement
+    boundaries (gc2_prolong_rb with the gminmod limiter, unclamped for
+    linear data), af_restrict_ref_boundary and af_consistent_fluxes all
+    preserve a linear n_e, so every leaf z-face away from the physical
+    boundary carries v n(face) - D dn/dz and every x/y-face carries 0."""
+    case = S.small_amr(3, nc=8, coord=A.XYZ, uniform_lvl=1, max_lvl=3, frac=0.5, seed=3, voltage=0.0)
+    p = oracle(A, S, case)
+    t = case.extra["export"]
+    N = case.params.gas_number_density
+    E0, grad, Lz = -2.0e6, 1e20, 4e-3
+    ne = np.zeros((t["n_boxes"], p.ncell))
+    efc = np.zeros((t["n_boxes"], p.nface))
+    for b in range(t["n_boxes"]):
+        X = centres(8, 3, t["r_min"][b * 3:(b + 1) * 3], t["dr"][b * 3:(b + 1) * 3])
+        ne[b] = (1e18 + grad * X[2]).ravel()
+        f = efc[b].reshape(3, 9, 9, 9)
+        f[2] = E0
+        efc[b] = f.ravel()
+    p.put_cc(A.CC_ELECTRON, ne)
+    p.put_cc(A.CC_EFIELD, np.full((t["n_boxes"], p.ncell), abs(E0)))
+    p.put_fc(A.FC_FIELD, efc)
+    p.forward_euler(1e-15, 0, [0], [1.0], 0, 1, 1)
+    flux = p.get_fc(A.FC_FLUX)
+    tb = case.tdata
+    x = table_x(tb)
+    Td = abs(E0) * 1e21 / N
+    mu = np.interp(Td, x, tb["td"][:tb["n"]]) / N
+    dif = np.interp(Td, x, tb["td"][tb["n"]:2 * tb["n"]]) / N
+    v = -mu * E0
+    margin = 2 * Lz / 8  # two level-1 cells: no physical-boundary influence
+    checked = 0
+    for bid, lvl, rmin, dr, ix in leaf_info(case):
+        fb = flux[bid - 1].reshape(3, 9, 9, 9)  # [dim][k][j][i]
+        for kf in range(9):
+            zf = rmin[2] + kf * dr[2]
+            if zf < margin or zf > Lz - margin:
+                continue
+            expect = v * (1e18 + grad * zf) - dif * grad
+            got = fb[2][kf, :8, :8]
+            assert np.abs(got - expect).max() <= 1e-9 * abs(expect), (bid, lvl, kf)
+            checked += got.size
+        zc = rmin[2] + (np.arange(8) + 0.5) * dr[2]
+        inner = (zc > margin) & (zc < Lz - margin)
+        for d in (0, 1):
+            g = fb[d][:8, :8, :8][inner]
+            if g.size:
+                assert np.abs(g).max() <= 1e-9 * abs(v * 1e18), (bid, d)
+    assert checked > 0
+
+
+# ---- field from the potential ---------------------------------------------------
+@pytest.mark.parametrize("D,coord", [(2, 2), (3, 1)])
+def test_field_of_linear_potential(A, S, D, coord):
+    """mg_compute_phi_gradient + mg_box_field_norm + the |E| ghost fill:
+    phi = V z / L (which meets the potential's boundary conditions) gives
+    E_z = -V/L on every face of every leaf, no other component, and
+    |E| = V/L in every leaf cell including the ghost layer."""
+    V, L = 1e4, 4e-3
+    case = S.small_amr(D, nc=8, coord=coord, uniform_lvl=2, max_lvl=4 if D == 2 else 3, frac=0.4, seed=9,
+                       voltage=V)
+    p = oracle(A, S, case)
+    t = case.extra["export"]
+    phi = np.zeros((t["n_boxes"], p.ncell))
+    for b in range(t["n_boxes"]):
+        X = centres(8, D, t["r_min"][b * D:(b + 1) * D], t["dr"][b * D:(b + 1) * D])
+        phi[b] = (V / L * X[D - 1]).ravel()
+    p.put_cc(A.CC_PHI, phi)
+    p.field_from_potential()
+    fc = p.get_fc(A.FC_FIELD)
+    ecc = p.get_cc(A.CC_EFIELD)
+    def faces(f, d):  # the (nc+1) faces along d times nc cells along the others
+        idx = [slice(0, 8)] * D
+        idx[D - 1 - d] = slice(0, 9)
+        return f[d][tuple(idx)]
+
+    for bid, lvl, rmin, dr, ix in leaf_info(case):
+        f = fc[bid - 1].reshape((D,) + (9,) * D)
+        assert np.abs(faces(f, D - 1) + V / L).max() <= 1e-9 * V / L, bid
+        for d in range(D - 1):
+            assert np.abs(faces(f, d)).max() <= 1e-9 * V / L, (bid, d)
+        e = box_view(ecc[bid - 1], 8, D)
+        core = e[(slice(None),) * D]
+        # faces of the ghost layer (the corners are extrapolated)
+        sl_in = interior(core, D)
+        assert np.abs(sl_in - V / L).max() <= 1e-9 * V / L, bid
+        for d in range(D):
+            for side in (0, -1):
+                idx = [slice(1, -1)] * D
+                idx[D - 1 - d] = side
+                assert np.abs(core[tuple(idx)] - V / L).max() <= 1e-9 * V / L, (bid, d, side)
+
+
+# ---- Heun's method and the time-step limits -------------------------------------
+def uniform_chemistry_case(A, S, integrator):
+    """Neutral uniform plasma in a uniform field: the charge density stays
+    zero (ionization and attachment create charge pairs), so the field stays
+    V/L, the fluxes have zero divergence and the densities follow the
+    constant-rate chemistry ODE exactly."""
+    V, L = 3e4, 4e-3
+    case = S.make_case(2, 8, A.XYZ, [L, L], V, 2, [L / 2, L / 2], seed_density=0.0, background=1e18, noise=0.0,
+                       integrator=integrator, num_vcycles=50, rel_residual=1e-12)
+    p = oracle(A, S, case)
+    p.field_compute(0, have_guess=False)
+    return case, p, V / L
+
+
+def chem_rates(case, E):
+    tb = case.tdata
+    N = case.params.gas_number_density
+    x = tb["chem_x_min"] + np.arange(tb["n"]) / tb["chem_inv_fac"]
+    Td = 1e21 / N * E
+    return np.interp(Td, x, tb["chem"][:tb["n"]]), np.interp(Td, x, tb["chem"][tb["n"]:])
+
+
+def test_heun_step_is_second_order_taylor(A, S):
+    """af_advance with Heun's method (m_af_advance.f90): for dn/dt = a n with
+    constant a it must give n (1 + a dt + (a dt)^2 / 2), and for the ions
+    n_+ + k_i dt n_e (1 + a dt / 2)."""
+    case, p, E = uniform_chemistry_case(A, S, A.HEUNS_METHOD)
+    ki, ka = chem_rates(case, E)
+    assert ki > 0
+    dt = 2e-12
+    n0 = 1e18
+    p.advance(dt)
+    a = (ki - ka) * dt
+    expect_e = n0 * (1 + a + 0.5 * a * a)
+    expect_p = n0 + ki * dt * n0 * (1 + 0.5 * a)
+    for iv, ex in ((A.CC_ELECTRON, expect_e), (A.CC_POS_ION, expect_p)):
+        g = p.get_cc(iv)
+        for bid, _, _, _, _ in leaf_info(case):
+            gi = interior(box_view(g[bid - 1], 8, 2), 2)
+            assert np.abs(gi - ex).max() <= 1e-11 * ex, (iv, bid)
+    # a forward-Euler step differs from it by the second-order term
+    case_fe, q, _ = uniform_chemistry_case(A, S, A.FORWARD_EULER)
+    q.advance(dt)
+    leaf = leaf_info(case_fe)[0][0]
+    ge = interior(box_view(q.get_cc(A.CC_ELECTRON)[leaf - 1], 8, 2), 2)
+    assert np.abs(ge - n0 * (1 + a)).max() <= 1e-11 * n0
+
+
+def test_time_step_limits_closed_form(A, S):
+    """dt limit of forward_euler (src/m_fluid.f90:96-98): min(dt_max,
+    cfl_number / max_cells sum_d (|v|/dx_d + 2 D/dx_d^2), eps0 / (e max mu n))
+    for a uniform plasma in a uniform field."""
+    case, p, E = uniform_chemistry_case(A, S, A.FORWARD_EULER)
+    case.params.dt_max = 1.0  # let the physical limits decide
+    p.set_params(case.params)
+    lim = p.advance(1e-15)
+    tb = case.tdata
+    N = case.params.gas_number_density
+    x = table_x(tb)
+    Td = E * 1e21 / N
+    mu = np.interp(Td, x, tb["td"][:tb["n"]]) / N
+    dif = np.interp(Td, x, tb["td"][tb["n"]:2 * tb["n"]]) / N
+    dx = 4e-3 / 16
+    cfl = mu * E / dx + 2 * dif / dx ** 2 + 2 * dif / dx ** 2
+    n_max = p.tree_maxabs_cc(A.CC_ELECTRON)
+    drt = 8.8541878128e-12 / (1.602176634e-19 * mu * n_max)
+    expect = min(case.params.dt_cfl_number / cfl, drt)
+    assert abs(lim - expect) <= 1e-6 * expect, (lim, expect, case.params.dt_cfl_number / cfl, drt)
