@@ -38,6 +38,7 @@ OP_POISSON, OP_HELMH = 0, 1
 MAX_MODES = 3
 OPT_GRAPHS = 1
 OPT_DIRECT_N = 2
+OPT_MIXED_SMOOTH = 3
 
 
 class ParamsT(C.Structure):

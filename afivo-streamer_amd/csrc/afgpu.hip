@@ -19,6 +19,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "afgpu.h"
@@ -121,6 +122,7 @@ struct afgpu_ctx {
   } ops[1 + AFGPU_MAX_MODES];
   // mixed precision: exact solve on the finest uniformly refined level
   int direct_max_n = 128;
+  int mixed_smooth = 0;  // > 0: smoothing sweeps of the inner cycle (down = up)
   struct Direct {
     bool set = false;  // lc and the map below are current for the tree
     int lc = 1, N = 0;
@@ -345,32 +347,53 @@ struct Impl : ImplBase {
   }
 
   template <typename T>
-  int gsrb_boxes(const Mg<T> &v, int lvl, bool down) {
-    const int n_cycle = down ? c.p.mg_n_cycle_down : c.p.mg_n_cycle_up;
+  int gsrb_boxes(const Mg<T> &v, int lvl, bool down, bool final_gc = true) {
+    int n_cycle = down ? c.p.mg_n_cycle_down : c.p.mg_n_cycle_up;
+    if (std::is_same<T, float>::value && c.mixed_smooth > 0) n_cycle = c.mixed_smooth;
     const List &l = c.l_ids[lvl];
     // algorithmic bytes of a half-sweep: the other colour and the rhs of
     // the updated colour are read, the updated colour is written
     const double bytes = 1.5 * sizeof(T) * l.n * G::NINT;
     const int k = bpb(l.n, G::NINT / 2);
+    const int cyl = c.coord == AFGPU_CYL;
     for (int n = 1; n <= 2 * n_cycle; n++) {
       RCHK(timer_start(c, 0, bytes));
-      RCHK(launch(k_gsrb<D, NC, BS, T>, l.n * k, c.list(l), (const DBox *)c.d_boxes, v.phi, (const T *)v.rhs, n,
-                  v.lambda2, (int)(c.coord == AFGPU_CYL), v.m, k));
+      if (k == 1)
+        RCHK(launch(k_gsrb_lds<D, NC, BS, T>, l.n, c.list(l), (const DBox *)c.d_boxes, v.phi, (const T *)v.rhs, n,
+                    v.lambda2, cyl, v.m));
+      else
+        RCHK(launch(k_gsrb<D, NC, BS, T>, l.n * k, c.list(l), (const DBox *)c.d_boxes, v.phi, (const T *)v.rhs, n,
+                    v.lambda2, cyl, v.m, k));
       RCHK(timer_stop(c, 0));
     }
     // ghost layer after the last half-sweep (corners on the final up-sweep)
-    return gc_var(lvl, v.phi, v.m, !down);
+    return final_gc ? gc_var(lvl, v.phi, v.m, !down) : AFGPU_OK;
+  }
+
+  // The single-precision inner cycle restricts with the child box staged in
+  // LDS and its face ghosts formed in-kernel, so it needs no ghost fill
+  // after the down-sweep.  (The FP64 form keeps the reference's fills: its
+  // stored ghost layers are part of the state compared with the oracle.)
+  template <typename T>
+  bool restrict_in_lds(int lvl) const {
+    constexpr int NQ = G::NINT / (1 << D);
+    return std::is_same<T, float>::value && bpb(c.l_ids[lvl].n, NQ) == 1;
   }
 
   template <typename T>
   int update_coarse(const Mg<T> &v, int lvl, bool v_cycle) {
     const int cyl = c.coord == AFGPU_CYL;
-    if (!v_cycle && lvl == c.highest_lvl) RCHK(gc_var(lvl, v.phi, v.m, true));
     const List &l = c.l_ids[lvl];
     constexpr int NQ = G::NINT / (1 << D);
     const int k = bpb(l.n, NQ);
-    RCHK(launch(k_restrict_residual<D, NC, BS, T>, l.n * k, c.list(l), (const DBox *)c.d_boxes, v.phi,
-                (const T *)v.rhs, v.tmp, v.lambda2, cyl, (int)!v_cycle, k));
+    if (restrict_in_lds<T>(lvl)) {
+      RCHK(launch(k_restrict_residual_lds<D, NC, BS, T>, l.n, c.list(l), (const DBox *)c.d_boxes, v.phi,
+                  (const T *)v.rhs, v.tmp, v.lambda2, cyl, (int)!v_cycle, v.m));
+    } else {
+      if (!v_cycle && lvl == c.highest_lvl) RCHK(gc_var(lvl, v.phi, v.m, true));
+      RCHK(launch(k_restrict_residual<D, NC, BS, T>, l.n * k, c.list(l), (const DBox *)c.d_boxes, v.phi,
+                  (const T *)v.rhs, v.tmp, v.lambda2, cyl, (int)!v_cycle, k));
+    }
     RCHK(gc_var(lvl - 1, v.phi, v.m, true));
     const List &pl = c.l_parents[lvl - 1];
     const int k2 = bpb(pl.n, G::NCELL);
@@ -435,17 +458,22 @@ struct Impl : ImplBase {
   template <typename T>
   int vcycle_body(const Mg<T> &v, bool set_residual, int max_lvl, int lc = 1) {
     for (int lvl = max_lvl; lvl >= lc + 1; lvl--) {
-      RCHK(gsrb_boxes(v, lvl, true));
+      RCHK(gsrb_boxes(v, lvl, true, !restrict_in_lds<T>(lvl)));
       RCHK(update_coarse(v, lvl, true));
     }
     if (lc == 1)
       RCHK(solve_coarse(v));
     else
       RCHK(solve_level(v, lc));
+    // (no ghost fill between the correction and the smoother: the fused
+    // half-sweeps form every out-of-box neighbour themselves, and the
+    // stored layer is rewritten after the last half-sweep)
     for (int lvl = lc + 1; lvl <= max_lvl; lvl++) {
       RCHK(correct_children(v, lvl - 1));
-      RCHK(gc_var(lvl, v.phi, v.m, true));
-      RCHK(gsrb_boxes(v, lvl, false));
+      // the single-precision correction's ghost layer on the finest level
+      // is read by nothing after the cycle
+      const bool final_gc = !(std::is_same<T, float>::value && lvl == c.highest_lvl);
+      RCHK(gsrb_boxes(v, lvl, false, final_gc));
     }
     if (set_residual) {
       // residual on every box; the max |residual| over leaves is reduced on
@@ -493,7 +521,6 @@ struct Impl : ImplBase {
     for (int lvl = lc + 1; lvl <= c.highest_lvl; lvl++) {
       RCHK(boxop<T>(c.l_ids[lvl], v.phi, v.tmp, OP_COPY));
       RCHK(correct_children(v, lvl - 1));
-      RCHK(gc_var(lvl, v.phi, v.m, true));
       RCHK(vcycle_body(v, set_residual && lvl == c.highest_lvl, lvl, lc));
     }
     return AFGPU_OK;
@@ -599,27 +626,40 @@ struct Impl : ImplBase {
     const int row_blocks = (nrows + TR - 1) / TR;
     constexpr int TC = (N < 64) ? N : (N == 64 ? 64 : 32);
     const int line_blocks = (N / TC) * ((D == 3) ? N : 1);
-    auto rows = [&](const double *in, double *out, const double *MT) {
-      hipLaunchKernelGGL((k_lc_rows<D, N>), dim3(row_blocks), dim3(256), 0, c.stream, in, out, MT, nrows);
+    // rows(): the x transform (MT: transposed matrix for the small-N
+    // kernel, M itself for the tiled one); lines(): the y / z transforms
+    constexpr bool TILED = (N == 64 || N == 128);
+    const int gemm_row_blocks = (N / 64) * (nrows / 64);
+    const int gemm_line_blocks = (N / 64) * (N / 64) * ((D == 3) ? N : 1);
+    auto rows = [&](const double *in, double *out, const double *Mx, const double *MTx) {
+      if (TILED)
+        hipLaunchKernelGGL((k_lc_gemm<D, (TILED ? N : 64), true>), dim3(gemm_row_blocks), dim3(256), 0, c.stream, in,
+                           out, Mx, 0, 0, (const double *)f.eig, lambda2);
+      else
+        hipLaunchKernelGGL((k_lc_rows<D, N>), dim3(row_blocks), dim3(256), 0, c.stream, in, out, MTx, nrows);
       return hipGetLastError();
     };
     auto lines = [&](const double *in, double *out, const double *M, int axis, int div) {
-      hipLaunchKernelGGL((k_lc_lines<D, N>), dim3(line_blocks), dim3(256), 0, c.stream, in, out, M, axis, div,
-                         (const double *)f.eig, lambda2);
+      if (TILED)
+        hipLaunchKernelGGL((k_lc_gemm<D, (TILED ? N : 64), false>), dim3(gemm_line_blocks), dim3(256), 0, c.stream,
+                           in, out, M, axis, div, (const double *)f.eig, lambda2);
+      else
+        hipLaunchKernelGGL((k_lc_lines<D, N>), dim3(line_blocks), dim3(256), 0, c.stream, in, out, M, axis, div,
+                           (const double *)f.eig, lambda2);
       return hipGetLastError();
     };
     if (D == 3) {
-      HIPCHK(rows(d.A0, d.A1, f.vinvT));
+      HIPCHK(rows(d.A0, d.A1, f.vinv, f.vinvT));
       HIPCHK(lines(d.A1, d.A0, f.vinv + NN, 1, 0));
       HIPCHK(lines(d.A0, d.A1, f.vinv + 2 * NN, 2, 1));
       HIPCHK(lines(d.A1, d.A0, f.v + 2 * NN, 2, 0));
       HIPCHK(lines(d.A0, d.A1, f.v + NN, 1, 0));
-      HIPCHK(rows(d.A1, d.A0, f.vT));
+      HIPCHK(rows(d.A1, d.A0, f.v, f.vT));
     } else {
-      HIPCHK(rows(d.A0, d.A1, f.vinvT));
+      HIPCHK(rows(d.A0, d.A1, f.vinv, f.vinvT));
       HIPCHK(lines(d.A1, d.A0, f.vinv + NN, 1, 1));
       HIPCHK(lines(d.A0, d.A1, f.v + NN, 1, 0));
-      HIPCHK(rows(d.A1, d.A0, f.vT));
+      HIPCHK(rows(d.A1, d.A0, f.v, f.vT));
     }
     return AFGPU_OK;
   }
@@ -680,6 +720,10 @@ struct Impl : ImplBase {
     RCHK(set_red(0, 0ull));
     const List &l = c.l_all_leaves;
     const int k = bpb(l.n, G::NINT);
+    if (k == 1)
+      return launch(k_mp_residual_lds<D, NC, BS>, l.n, c.list(l), (const DBox *)c.d_boxes,
+                    (const double *)c.cc[iphi], (const double *)c.cc[AFGPU_CC_RHS], c.cc32[1], c.ops[op].lambda2,
+                    (int)(c.coord == AFGPU_CYL), m, c.d_red);
     return launch(k_mp_residual<D, NC, BS>, l.n * k, c.list(l), (const DBox *)c.d_boxes, (const double *)c.cc[iphi],
                   (const double *)c.cc[AFGPU_CC_RHS], c.cc32[1], c.ops[op].lambda2, (int)(c.coord == AFGPU_CYL),
                   m, c.d_red, k);
@@ -925,7 +969,9 @@ struct Impl : ImplBase {
           RCHK(leaf_residual_max(&residu));
           if (residu / max_rhs < c.p.photoi_max_rel_residual) break;
         }
-        RCHK(mp_finish(op, lc));
+        // No final ghost refresh of the mode: the next solve forms its
+        // residual with in-kernel ghosts, and the photoionization source is
+        // read on interior cells only (src/m_fluid.f90:437-439).
       } else {
         for (int i = 1; i <= c.p.photoi_max_fmg_cycles; i++) {
           RCHK(fmg(op, true, true));
@@ -1559,6 +1605,15 @@ int afgpu_set_option(afgpu_t *h, int option, int value) {
       h->direct_max_n = value;
       h->dir.set = false;
       for (auto &f : h->dir.f) f.ok = false;
+      return AFGPU_OK;
+    case AFGPU_OPT_MIXED_SMOOTH:
+      if (value < 0 || value > 8) {
+        set_err("afgpu_set_option: inner smoothing sweeps must be 0..8");
+        return AFGPU_ERR_ARG;
+      }
+      RCHK(sync_stream(h));
+      drop_graphs(*h);
+      h->mixed_smooth = value;
       return AFGPU_OK;
     default:
       set_err("afgpu_set_option: unknown option");

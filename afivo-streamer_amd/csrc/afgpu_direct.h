@@ -150,4 +150,112 @@ __global__ void __launch_bounds__(256) k_lc_rows(const double *__restrict__ in, 
   }
 }
 
+// The same transforms for N = 64 and 128 as a tiled product
+//   out(i, col) = sum_m M[i][m] X(m, col)
+// over 64 x 64 output tiles (i0.., col0..): chunks of 16 m are staged in LDS
+// for both M and X (the next chunk is prefetched into registers while the
+// current one is used), and each thread accumulates a 4 x 4 block.
+// ROWS: X(m, col) = in[col*N + m] (the x transform, col = row of the grid);
+// else X(m, col) = in[o*so + m*sa + x0 + col] (the y / z transforms).  With
+// `div` (y/z forms only) the result is divided by the eigenvalue sum.
+template <int D, int N, bool ROWS>
+__global__ void __launch_bounds__(256) k_lc_gemm(const double *__restrict__ in, double *__restrict__ out,
+                                                 const double *__restrict__ M, int axis, int div,
+                                                 const double *__restrict__ eig, double lambda2) {
+  static_assert(N == 64 || N == 128, "tiled transform for N = 64, 128");
+  constexpr int T = 64, KC = 16, KP = KC + 1, NIT = N / T;
+  __shared__ double Ms[T * KP];
+  __shared__ double Xs[KC * T];
+  const int it = blockIdx.x % NIT, rest = blockIdx.x / NIT;
+  const int i0 = it * T;
+  size_t sm, sc, base;
+  int o = 0, x0 = 0;
+  if (ROWS) {
+    sm = 1;
+    sc = N;
+    base = (size_t)rest * T * N;
+  } else {
+    constexpr int nxc = N / T;
+    const int xc = rest % nxc;
+    o = rest / nxc;
+    x0 = xc * T;
+    sm = (axis == 1) ? (size_t)N : (size_t)N * N;
+    const size_t so = (D == 2) ? 0 : ((axis == 1) ? (size_t)N * N : (size_t)N);
+    sc = 1;
+    base = (size_t)o * so + x0;
+  }
+  const int tid = threadIdx.x;
+  const int ti = ROWS ? (tid % 16) : (tid / 16);
+  const int tc = ROWS ? (tid / 16) : (tid % 16);
+  // global -> register staging of one chunk: 4 M and 4 X values per thread
+  double pm[4], px[4];
+  auto fetch = [&](int k0) {
+    for (int u = 0; u < 4; u++) {
+      const int q = tid + 256 * u;
+      const int i = q / KC, kk = q % KC;
+      pm[u] = M[(size_t)(i0 + i) * N + k0 + kk];
+      int kx, c;
+      if (ROWS) {
+        kx = q % KC;
+        c = q / KC;
+      } else {
+        c = q % T;
+        kx = q / T;
+      }
+      px[u] = in[base + (size_t)(k0 + kx) * sm + (size_t)c * sc];
+    }
+  };
+  auto stash = [&]() {
+    for (int u = 0; u < 4; u++) {
+      const int q = tid + 256 * u;
+      Ms[(q / KC) * KP + q % KC] = pm[u];
+      if (ROWS)
+        Xs[(q % KC) * T + q / KC] = px[u];
+      else
+        Xs[(q / T) * T + q % T] = px[u];
+    }
+  };
+  double acc[4][4];
+  for (int r = 0; r < 4; r++)
+    for (int q = 0; q < 4; q++) acc[r][q] = 0.0;
+  fetch(0);
+  for (int k0 = 0; k0 < N; k0 += KC) {
+    stash();
+    __syncthreads();
+    if (k0 + KC < N) fetch(k0 + KC);
+    for (int kk = 0; kk < KC; kk++) {
+      double a[4], b[4];
+      for (int r = 0; r < 4; r++) a[r] = Ms[(ti + 16 * r) * KP + kk];
+      for (int q = 0; q < 4; q++) b[q] = Xs[kk * T + tc + 16 * q];
+      for (int r = 0; r < 4; r++)
+        for (int q = 0; q < 4; q++) acc[r][q] = acc[r][q] + a[r] * b[q];
+    }
+    __syncthreads();
+  }
+  for (int r = 0; r < 4; r++)
+    for (int q = 0; q < 4; q++) {
+      const int i = i0 + ti + 16 * r, c = tc + 16 * q;
+      double v = acc[r][q];
+      if (!ROWS && div) {
+        const int x = x0 + c;
+        int y, z;
+        if (D == 2) {
+          y = i;
+          z = 0;
+        } else if (axis == 1) {
+          y = i;
+          z = o;
+        } else {
+          y = o;
+          z = i;
+        }
+        double den = eig[x] + eig[N + y];
+        if (D == 3) den = den + eig[2 * N + z];
+        den = den - lambda2;
+        v = v / den;
+      }
+      out[base + (size_t)i * sm + (size_t)c * sc] = v;
+    }
+}
+
 }  // namespace afg

@@ -132,8 +132,11 @@ __global__ void __launch_bounds__(BS)
   // face fluxes (flux_upwind, src/m_fluid.f90:102-209)
   for (int q = threadIdx.x; q < D * (NC + 1) * NT; q += BS) {
     const int d = q / ((NC + 1) * NT);
-    const int f = (q / NT) % (NC + 1) + 1;
-    const int r = q % NT;
+    // x faces: consecutive threads take consecutive faces of one grid row
+    // (contiguous in memory); y/z faces: consecutive rows of one face plane
+    const int qd = q % ((NC + 1) * NT);
+    const int f = (d == 0) ? qd % (NC + 1) + 1 : qd / NT + 1;
+    const int r = (d == 0) ? qd / (NC + 1) : qd % NT;
     int t0, t1;
     tangential(D, d, t0, t1);
     int p[3] = {0, 0, 0};

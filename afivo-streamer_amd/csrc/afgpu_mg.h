@@ -769,4 +769,182 @@ __global__ void __launch_bounds__(BS) k_mp_residual(const int *__restrict__ ids,
   }
 }
 
+// ---- LDS-staged forms (one workgroup per box) ---------------------------------
+// The box interior is loaded once, coalesced, into LDS; the face ghosts that
+// the pass needs are formed by ghost_value() into the LDS ghost layer (all
+// threads of an iteration on the same face, so without divergence); the
+// stencil then reads LDS only.  Same arithmetic, in the same order, as the
+// direct kernels above.
+
+template <int D, int NC, int BS, typename T>
+__device__ inline void lds_load_interior(T *s, const T *u) {
+  using G = Geo<D, NC>;
+  for (int q = threadIdx.x; q < G::NINT; q += BS) {
+    int i, j, k;
+    interior_ijk<D, NC>(q, i, j, k);
+    const int x = G::idx(i, j, k);
+    s[x] = u[x];
+  }
+}
+
+// Face ghosts into LDS.  colour >= 0: only the ghosts next to interior cells
+// of that GSRB colour ((i + j + k + colour) even), the only ones its
+// half-sweep reads; colour < 0: all face ghosts.
+template <int D, int NC, int BS, typename T>
+__device__ inline void lds_face_ghosts(T *s, const DBox *__restrict__ boxes, const DBox &bx, const T *var,
+                                       const VarBc &m, int colour) {
+  using G = Geo<D, NC>;
+  constexpr int NFC = 2 * D * G::NTAN;
+  for (int q = threadIdx.x; q < NFC; q += BS) {
+    const int nb = q / G::NTAN, r = q % G::NTAN;
+    const int a = r % NC + 1, b = (D == 3) ? r / NC + 1 : 0;
+    const bool low = (nb % 2) == 0;
+    int p[3];
+    face_pos<D>(nb, a, b, low ? 1 : NC, p);
+    if (colour >= 0 && ((p[0] + p[1] + p[2] + colour) & 1)) continue;
+    const T x1 = s[G::idx(p[0], p[1], p[2])];
+    face_pos<D>(nb, a, b, low ? 2 : NC - 1, p);
+    const T x2 = s[G::idx(p[0], p[1], p[2])];
+    const T g = ghost_value<D, NC, T>(boxes, bx, var, m, nb, a, b, x1, x2);
+    face_pos<D>(nb, a, b, low ? 0 : NC + 1, p);
+    s[G::idx(p[0], p[1], p[2])] = g;
+  }
+}
+
+// k_gsrb with the box staged in LDS (levels with one workgroup per box).
+template <int D, int NC, int BS, typename T>
+__global__ void __launch_bounds__(BS) k_gsrb_lds(const int *__restrict__ ids, const DBox *__restrict__ boxes, T *phi,
+                                                 const T *__restrict__ rhs, int redblack, T lambda2, int cyl, VarBc m) {
+  using G = Geo<D, NC>;
+  __shared__ T s[G::NCELL];
+  const int id = ids[blockIdx.x];
+  const DBox &bx = boxes[id - 1];
+  const Sten<D, T> st = lpl_stencil<D, T>(bx, lambda2);
+  T *u = phi + (size_t)(id - 1) * G::NCELL;
+  const T *r = rhs + (size_t)(id - 1) * G::NCELL;
+  lds_load_interior<D, NC, BS, T>(s, u);
+  __syncthreads();
+  // the updated cells satisfy (i + j + k + redblack) even
+  lds_face_ghosts<D, NC, BS, T>(s, boxes, bx, phi, m, redblack & 1);
+  __syncthreads();
+  constexpr int HALF = G::NINT / 2;
+  constexpr int sx = 1, sy = G::NG, sz = G::NG * G::NG;
+  for (int q = threadIdx.x; q < HALF; q += BS) {
+    const int mm = q % (NC / 2);
+    const int j = (q / (NC / 2)) % NC + 1;
+    const int k = (D == 3) ? q / ((NC / 2) * NC) + 1 : 0;
+    const int i0 = 2 - ((redblack ^ (D == 3 ? k + j : j)) & 1);
+    const int i = i0 + 2 * mm;
+    const int x = G::idx(i, j, k);
+    if (D == 2) {
+      if (cyl) {
+        T c1, c2, c3;
+        cyl_coeffs<T>(bx, i, st.c, c1, c2, c3);
+        T inv1 = T(1) / c1;
+        u[x] = (r[x] - c2 * s[x - sx] - c3 * s[x + sx] - st.c[3] * s[x - sy] - st.c[4] * s[x + sy]) * inv1;
+      } else {
+        T inv_c1 = T(1) / st.c[0];
+        u[x] = (r[x] - st.c[1] * s[x - sx] - st.c[2] * s[x + sx] - st.c[3] * s[x - sy] - st.c[4] * s[x + sy]) *
+               inv_c1;
+      }
+    } else {
+      T inv_c1 = T(1) / st.c[0];
+      u[x] = (r[x] - st.c[1] * s[x - sx] - st.c[2] * s[x + sx] - st.c[3] * s[x - sy] - st.c[4] * s[x + sy] -
+              st.c[5] * s[x - sz] - st.c[6] * s[x + sz]) *
+             inv_c1;
+    }
+  }
+}
+
+// k_mp_residual with the box staged in LDS.
+template <int D, int NC, int BS>
+__global__ void __launch_bounds__(BS) k_mp_residual_lds(const int *__restrict__ ids, const DBox *__restrict__ boxes,
+                                                        const double *__restrict__ phi,
+                                                        const double *__restrict__ rhs, float *r32, double lambda2,
+                                                        int cyl, VarBc m, unsigned long long *leafmax) {
+  using G = Geo<D, NC>;
+  __shared__ double s[G::NCELL];
+  __shared__ double sh[BS / 64];
+  const int id = ids[blockIdx.x];
+  const DBox &bx = boxes[id - 1];
+  const Sten<D, double> st = lpl_stencil<D, double>(bx, lambda2);
+  const size_t base = (size_t)(id - 1) * G::NCELL;
+  lds_load_interior<D, NC, BS, double>(s, phi + base);
+  __syncthreads();
+  lds_face_ghosts<D, NC, BS, double>(s, boxes, bx, phi, m, -1);
+  __syncthreads();
+  double mx = 0;
+  for (int q = threadIdx.x; q < G::NINT; q += BS) {
+    int i, j, k;
+    interior_ijk<D, NC>(q, i, j, k);
+    const int x = G::idx(i, j, k);
+    const double lp = apply_cell<D, NC, double>(s, st, cyl != 0, bx, i, j, k);
+    const double rv = rhs[base + x] - lp;
+    r32[base + x] = (float)rv;
+    mx = fmax(mx, fabs(rv));
+  }
+  if (bx.children[0] == 0) {
+    mx = block_max<BS>(mx, sh);
+    if (threadIdx.x == 0) atomic_max_nonneg(leafmax, mx);
+  }
+}
+
+// k_restrict_residual with the child box staged in LDS and its face ghosts
+// formed in-kernel, so no ghost fill has to precede it.
+template <int D, int NC, int BS, typename T>
+__global__ void __launch_bounds__(BS) k_restrict_residual_lds(const int *__restrict__ ids,
+                                                              const DBox *__restrict__ boxes, T *phi,
+                                                              const T *__restrict__ rhs, T *tmp, T lambda2, int cyl,
+                                                              int store_residual, VarBc m) {
+  using G = Geo<D, NC>;
+  constexpr int H = NC / 2;
+  constexpr int NQ = (D == 3) ? H * H * H : H * H;
+  __shared__ T s[G::NCELL];
+  const int id = ids[blockIdx.x];
+  const DBox &bc = boxes[id - 1];
+  const DBox &bp = boxes[bc.parent - 1];
+  const Sten<D, T> st = lpl_stencil<D, T>(bc, lambda2);
+  const size_t cb = (size_t)(id - 1) * G::NCELL, pb = (size_t)(bc.parent - 1) * G::NCELL;
+  lds_load_interior<D, NC, BS, T>(s, phi + cb);
+  __syncthreads();
+  lds_face_ghosts<D, NC, BS, T>(s, boxes, bc, phi, m, -1);
+  __syncthreads();
+  int off[3] = {0, 0, 0};
+  for (int d = 0; d < D; d++) off[d] = ((bc.ix[d] - 1) & 1) * H;
+  for (int q = threadIdx.x; q < NQ; q += BS) {
+    const int i = q % H + 1, j = (q / H) % H + 1, k = (D == 3) ? q / (H * H) + 1 : 0;
+    const int i_f = 2 * i - 1, jf = 2 * j - 1, kf = (D == 3) ? 2 * k - 1 : 0;
+    T rs[8];
+    const int nch = (D == 3) ? 8 : 4;
+    for (int c = 0; c < nch; c++) {
+      const int ii = i_f + (c & 1), jj = jf + ((c >> 1) & 1), kk = kf + ((c >> 2) & 1);
+      T lp = apply_cell<D, NC, T>(s, st, cyl != 0, bc, ii, jj, kk);
+      rs[c] = rhs[cb + G::idx(ii, jj, kk)] - lp;
+      if (store_residual) tmp[cb + G::idx(ii, jj, kk)] = rs[c];
+    }
+    const int ic = off[0] + i, jc = off[1] + j, kc = (D == 3) ? off[2] + k : 0;
+    T rv;
+    if (D == 2) {
+      if (cyl) {
+        T r = (T)bp.r_min[0] + (T(ic) - T(0.5)) * (T)bp.dr[0];
+        T t = T(0.25) * (T)bp.dr[0] / r;
+        T w1 = T(1) - t, w2 = T(1) + t;
+        rv = T(0.25) * (w1 * (rs[0] + rs[2]) + w2 * (rs[1] + rs[3]));
+      } else {
+        T sm = rs[0];
+        sm = sm + rs[1];
+        sm = sm + rs[2];
+        sm = sm + rs[3];
+        rv = T(0.25) * sm;
+      }
+    } else {
+      T sm = rs[0];
+      for (int c = 1; c < 8; c++) sm = sm + rs[c];
+      rv = T(0.125) * sm;
+    }
+    tmp[pb + G::idx(ic, jc, kc)] = rv;
+    phi[pb + G::idx(ic, jc, kc)] = restrict_cell<D, NC, T>(s, i_f, jf, kf, false, bp, ic);
+  }
+}
+
 }  // namespace afg

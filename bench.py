@@ -42,11 +42,13 @@ def make_case(args):
     return case
 
 
-def run_gpu(case, precision, warmup, steps, timing_steps, direct_n=None):
+def run_gpu(case, precision, warmup, steps, timing_steps, direct_n=None, smooth=None):
     p = A.Path("gpu", case.D, case.nc, case.coord, precision)
     S.apply(p, case)
     if direct_n is not None:
         p.set_option(A.OPT_DIRECT_N, direct_n)
+    if smooth is not None:
+        p.set_option(A.OPT_MIXED_SMOOTH, smooth)
     t0 = time.time()
     p.field_compute(0, have_guess=False)
     log("  initial field solve %.2f s" % (time.time() - t0))
@@ -95,6 +97,19 @@ def leaf_interior(case, p, iv):
     return a[(slice(None),) + (slice(1, case.nc + 1),) * case.D]
 
 
+def copy_probe(mib):
+    """A plain device-to-device copy of known size, for counter calibration:
+    each launch reads and writes exactly `mib` MiB."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_per_launch": n * 4, "launches": 5}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,9 +128,16 @@ def main():
     ap.add_argument("--direct-n", type=int, default=128,
                     help="mixed precision: largest uniform level (cells per axis) solved exactly "
                          "by the inner cycle; 0 = descend to level 1")
+    ap.add_argument("--mixed-smooth", type=int, default=0,
+                    help="mixed precision: GSRB sweeps per side in the inner cycle (0 = reference counts)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB a few times (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters against known bytes) and exit")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64-ref", action="store_true")
     args = ap.parse_args()
+    if args.copy_probe > 0:
+        return copy_probe(args.copy_probe)
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X "
               "(the multi-GPU sharding of the path is described in DESIGN.md)", file=sys.stderr)
@@ -126,8 +148,9 @@ def main():
     log("workload", case.name, info)
     prec = A.PREC_MIXED if args.precision == "mixed" else A.PREC_FP64
     log("GPU %s run" % args.precision)
+    mixed = prec == A.PREC_MIXED
     p, infos, secs, stats = run_gpu(case, prec, args.warmup, args.steps, args.timing_steps,
-                                    args.direct_n if prec == A.PREC_MIXED else None)
+                                    args.direct_n if mixed else None, args.mixed_smooth if mixed else None)
     cell_steps = sum(i["leaf_cells"] for i in infos)
     value = cell_steps / secs
     nv = [i["n_vcycles"] for i in infos]
@@ -210,7 +233,8 @@ def main():
                    "boxes": info["n_boxes"], "levels": info["highest_lvl"], "box_cells": "16^3",
                    "precision": args.precision, "rel_residual": args.rel_residual,
                    "photoi_per_steps": args.photoi_per_steps, "mean_vcycles": float(np.mean(nv)),
-                   "direct_n": args.direct_n if prec == A.PREC_MIXED else None},
+                   "direct_n": args.direct_n if prec == A.PREC_MIXED else None,
+                   "mixed_smooth": args.mixed_smooth if prec == A.PREC_MIXED else None},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

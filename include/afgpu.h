@@ -262,6 +262,9 @@ int afgpu_kernel_stats(afgpu_t *h, int kind, int reset, double *total_ms,
 #define AFGPU_OPT_DIRECT_N 2 /* mixed precision: largest uniformly refined level (cells per axis,
                                 16..128) that the inner cycle solves exactly; below 16 the
                                 inner cycle descends to level 1 (default 128) */
+#define AFGPU_OPT_MIXED_SMOOTH 3 /* mixed precision: GSRB sweeps before and after the coarse
+                                    correction in the inner cycle; 0 = the params' counts
+                                    (default 0) */
 int afgpu_set_option(afgpu_t *h, int option, int value);
 
 #ifdef __cplusplus
