@@ -39,6 +39,7 @@ MAX_MODES = 3
 OPT_GRAPHS = 1
 OPT_DIRECT_N = 2
 OPT_MIXED_SMOOTH = 3
+OPT_BLOCK_SMOOTH = 4
 
 
 class ParamsT(C.Structure):

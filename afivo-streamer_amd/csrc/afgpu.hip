@@ -106,7 +106,7 @@ struct afgpu_ctx {
   // pools
   double *cc[AFGPU_N_CC] = {nullptr};
   double *fc[AFGPU_N_FC] = {nullptr};
-  float *cc32[3] = {nullptr};  // mixed precision: correction, residual, scratch
+  float *cc32[4] = {nullptr};  // mixed precision: correction, residual, scratch, second correction buffer
   // tables
   double *d_td = nullptr, *d_chem = nullptr;
   DTable td{}, chem{};
@@ -123,6 +123,7 @@ struct afgpu_ctx {
   // mixed precision: exact solve on the finest uniformly refined level
   int direct_max_n = 128;
   int mixed_smooth = 0;  // > 0: smoothing sweeps of the inner cycle (down = up)
+  int block_smooth = 0;  // inner cycle: block smoother on levels with one workgroup per box
   struct Direct {
     bool set = false;  // lc and the map below are current for the tree
     int lc = 1, N = 0;
@@ -337,13 +338,28 @@ struct Impl : ImplBase {
     T *phi, *rhs, *tmp;
     VarBc m;
     T lambda2;
+    T *alt;  // second buffer of the block smoother (single precision only)
   };
   Mg<double> mg64(int op) {
     const int iphi = op_phi(op);
-    return {op, c.cc[iphi], c.cc[AFGPU_CC_RHS], c.cc[AFGPU_CC_TMP], var_methods(c, iphi), c.ops[op].lambda2};
+    return {op, c.cc[iphi], c.cc[AFGPU_CC_RHS], c.cc[AFGPU_CC_TMP], var_methods(c, iphi), c.ops[op].lambda2,
+            nullptr};
   }
   Mg<float> mg32(int op) {
-    return {op, c.cc32[0], c.cc32[1], c.cc32[2], var_methods(c, op_phi(op), true), (float)c.ops[op].lambda2};
+    return {op,        c.cc32[0], c.cc32[1], c.cc32[2], var_methods(c, op_phi(op), true), (float)c.ops[op].lambda2,
+            c.cc32[3]};
+  }
+
+  // Levels of the single-precision inner cycle that use the block smoother:
+  // their down-sweep writes the second buffer, which the restriction and the
+  // coarse correction then use, and their up-sweep writes phi back.
+  template <typename T>
+  bool use_block(int lvl) const {
+    return std::is_same<T, float>::value && c.block_smooth && bpb(c.l_ids[lvl].n, G::NINT / 2) == 1;
+  }
+  template <typename T>
+  T *level_buf(const Mg<T> &v, int lvl, bool after_down) const {
+    return (after_down && use_block<T>(lvl)) ? v.alt : v.phi;
   }
 
   template <typename T>
@@ -356,6 +372,15 @@ struct Impl : ImplBase {
     const double bytes = 1.5 * sizeof(T) * l.n * G::NINT;
     const int k = bpb(l.n, G::NINT / 2);
     const int cyl = c.coord == AFGPU_CYL;
+    if (use_block<T>(lvl)) {
+      T *in = down ? v.phi : v.alt, *out = down ? v.alt : v.phi;
+      // algorithmic bytes of a block phase: in and rhs read, out written
+      RCHK(timer_start(c, 0, 3.0 * sizeof(T) * l.n * G::NINT));
+      RCHK(launch(k_smooth_block<D, NC, BS, T>, l.n, c.list(l), (const DBox *)c.d_boxes, (const T *)in, out,
+                  (const T *)v.rhs, (const T *)v.phi, 2 * n_cycle, v.lambda2, cyl, v.m));
+      RCHK(timer_stop(c, 0));
+      return final_gc ? gc_var(lvl, out, v.m, !down) : AFGPU_OK;
+    }
     for (int n = 1; n <= 2 * n_cycle; n++) {
       RCHK(timer_start(c, 0, bytes));
       if (k == 1)
@@ -387,8 +412,9 @@ struct Impl : ImplBase {
     constexpr int NQ = G::NINT / (1 << D);
     const int k = bpb(l.n, NQ);
     if (restrict_in_lds<T>(lvl)) {
-      RCHK(launch(k_restrict_residual_lds<D, NC, BS, T>, l.n, c.list(l), (const DBox *)c.d_boxes, v.phi,
-                  (const T *)v.rhs, v.tmp, v.lambda2, cyl, (int)!v_cycle, v.m));
+      RCHK(launch(k_restrict_residual_lds<D, NC, BS, T>, l.n, c.list(l), (const DBox *)c.d_boxes,
+                  (const T *)level_buf(v, lvl, v_cycle), v.phi, (const T *)v.rhs, v.tmp, v.lambda2, cyl,
+                  (int)!v_cycle, v.m));
     } else {
       if (!v_cycle && lvl == c.highest_lvl) RCHK(gc_var(lvl, v.phi, v.m, true));
       RCHK(launch(k_restrict_residual<D, NC, BS, T>, l.n * k, c.list(l), (const DBox *)c.d_boxes, v.phi,
@@ -401,13 +427,14 @@ struct Impl : ImplBase {
                   v.rhs, v.tmp, v.lambda2, cyl, (int)v_cycle, k2);
   }
 
+  // (target: the buffer holding the children's level, see use_block)
   template <typename T>
-  int correct_children(const Mg<T> &v, int lvl_p) {
+  int correct_children(const Mg<T> &v, int lvl_p, T *target = nullptr) {
     RCHK(boxop<T>(c.l_parents[lvl_p], v.phi, v.tmp, OP_SUB_FROM));
     const List &l = c.l_ids[lvl_p + 1];
     const int k = bpb(l.n, G::NINT);
-    return launch(k_prolong_add<D, NC, BS, T>, l.n * k, c.list(l), (const DBox *)c.d_boxes, (const T *)v.tmp, v.phi,
-                  k);
+    return launch(k_prolong_add<D, NC, BS, T>, l.n * k, c.list(l), (const DBox *)c.d_boxes, (const T *)v.tmp,
+                  target ? target : v.phi, k);
   }
 
   int ensure_factors(int op) {
@@ -469,7 +496,7 @@ struct Impl : ImplBase {
     // half-sweeps form every out-of-box neighbour themselves, and the
     // stored layer is rewritten after the last half-sweep)
     for (int lvl = lc + 1; lvl <= max_lvl; lvl++) {
-      RCHK(correct_children(v, lvl - 1));
+      RCHK(correct_children(v, lvl - 1, level_buf(v, lvl, true)));
       // the single-precision correction's ghost layer on the finest level
       // is read by nothing after the cycle
       const bool final_gc = !(std::is_same<T, float>::value && lvl == c.highest_lvl);
@@ -516,6 +543,9 @@ struct Impl : ImplBase {
                     1, (int)(c.coord == AFGPU_CYL), k));
       }
     }
+    // the single-precision correction starts from zero on its coarsest level
+    // too (refined boxes there keep the last cycle's values otherwise)
+    if (std::is_same<T, float>::value && !have_guess) RCHK(boxop<T>(c.l_ids[lc], v.phi, v.phi, OP_CLEAR));
     RCHK(boxop<T>(c.l_ids[lc], v.phi, v.tmp, OP_COPY));
     RCHK(vcycle_body(v, set_residual && c.highest_lvl == lc, lc, lc));
     for (int lvl = lc + 1; lvl <= c.highest_lvl; lvl++) {
@@ -705,9 +735,8 @@ struct Impl : ImplBase {
   int mp_close(int op, bool correct, int lc) {
     const int iphi = op_phi(op);
     if (correct) {
-      const List &l = c.l_all_ids;
-      RCHK(launch(k_mp_correct<G::NCELL, BS>, l.n * bpb(l.n, G::NCELL), c.list(l), c.cc[iphi], c.cc32[0],
-                  bpb(l.n, G::NCELL)));
+      const List &l = c.l_all_leaves;
+      RCHK(launch(k_mp_correct_leaves<D, NC, BS>, l.n, c.list(l), c.cc[iphi], c.cc32[0]));
     }
     // (levels below lc are covered everywhere and not part of the iteration)
     RCHK(restrict_levels(iphi, c.highest_lvl, lc + 1));
@@ -1095,7 +1124,7 @@ int afgpu_destroy(afgpu_t *h) {
   for (auto e : h->ev_pool) (void)hipEventDestroy(e);
   for (int i = 0; i < AFGPU_N_CC; i++) (void)hipFree(h->cc[i]);
   for (int i = 0; i < AFGPU_N_FC; i++) (void)hipFree(h->fc[i]);
-  for (int i = 0; i < 3; i++) (void)hipFree(h->cc32[i]);
+  for (int i = 0; i < 4; i++) (void)hipFree(h->cc32[i]);
   (void)hipFree(h->dir.d_map);
   (void)hipFree(h->dir.A0);
   (void)hipFree(h->dir.A1);
@@ -1267,7 +1296,7 @@ int afgpu_sync_tree(afgpu_t *h, int n_boxes, const int *lvl, const int *ix, cons
     for (int i = 0; i < AFGPU_N_CC; i++) RCHK(grow(h->cc[i], h->ncell));
     for (int i = 0; i < AFGPU_N_FC; i++) RCHK(grow(h->fc[i], h->nface));
     if (h->precision == AFGPU_PREC_MIXED) {
-      for (int i = 0; i < 3; i++) {
+      for (int i = 0; i < 4; i++) {
         (void)hipFree(h->cc32[i]);
         HIPCHK(hipMalloc(&h->cc32[i], (size_t)cap * h->ncell * sizeof(float)));
         HIPCHK(hipMemsetAsync(h->cc32[i], 0, (size_t)cap * h->ncell * sizeof(float), h->stream));
@@ -1605,6 +1634,11 @@ int afgpu_set_option(afgpu_t *h, int option, int value) {
       h->direct_max_n = value;
       h->dir.set = false;
       for (auto &f : h->dir.f) f.ok = false;
+      return AFGPU_OK;
+    case AFGPU_OPT_BLOCK_SMOOTH:
+      RCHK(sync_stream(h));
+      drop_graphs(*h);
+      h->block_smooth = value != 0;
       return AFGPU_OK;
     case AFGPU_OPT_MIXED_SMOOTH:
       if (value < 0 || value > 8) {

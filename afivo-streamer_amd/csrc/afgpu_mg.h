@@ -99,9 +99,11 @@ __device__ inline void interior_ijk(int q, int &i, int &j, int &k) {
 // :173-279), or refinement boundary (mg_sides_rb, m_af_multigrid.f90:294-461,
 // or af_gc_interp / af_gc_interp_lim, m_af_ghostcell.f90:394-605).  x1 and x2
 // are the box's first and second interior cells along the face normal.
+// `coarse` (default: var) holds the coarser level's values when the box's
+// own level is kept in a different buffer.
 template <int D, int NC, typename T>
 __device__ inline T ghost_value(const DBox *__restrict__ boxes, const DBox &bx, const T *var, const VarBc &m, int nb,
-                                int a, int b, T x1, T x2) {
+                                int a, int b, T x1, T x2, const T *coarse = nullptr) {
   using G = Geo<D, NC>;
   constexpr int H = NC / 2;
   const int d = nb / 2;
@@ -129,7 +131,7 @@ __device__ inline T ghost_value(const DBox *__restrict__ boxes, const DBox &bx, 
     return c0 * (T)m.val[nb] + c1 * x1 + c2 * x2;
   }
   const int p_nb_id = boxes[bx.parent - 1].nb[nb];
-  const T *P = var + (size_t)(p_nb_id - 1) * G::NCELL;
+  const T *P = (coarse ? coarse : var) + (size_t)(p_nb_id - 1) * G::NCELL;
   int co[3] = {0, 0, 0};
   for (int dd = 0; dd < D; dd++) co[dd] = ((bx.ix[dd] - 1) & 1) * H;
   if (m.rb == RB_MG) {
@@ -792,7 +794,7 @@ __device__ inline void lds_load_interior(T *s, const T *u) {
 // half-sweep reads; colour < 0: all face ghosts.
 template <int D, int NC, int BS, typename T>
 __device__ inline void lds_face_ghosts(T *s, const DBox *__restrict__ boxes, const DBox &bx, const T *var,
-                                       const VarBc &m, int colour) {
+                                       const VarBc &m, int colour, const T *coarse = nullptr) {
   using G = Geo<D, NC>;
   constexpr int NFC = 2 * D * G::NTAN;
   for (int q = threadIdx.x; q < NFC; q += BS) {
@@ -805,7 +807,7 @@ __device__ inline void lds_face_ghosts(T *s, const DBox *__restrict__ boxes, con
     const T x1 = s[G::idx(p[0], p[1], p[2])];
     face_pos<D>(nb, a, b, low ? 2 : NC - 1, p);
     const T x2 = s[G::idx(p[0], p[1], p[2])];
-    const T g = ghost_value<D, NC, T>(boxes, bx, var, m, nb, a, b, x1, x2);
+    const T g = ghost_value<D, NC, T>(boxes, bx, var, m, nb, a, b, x1, x2, coarse);
     face_pos<D>(nb, a, b, low ? 0 : NC + 1, p);
     s[G::idx(p[0], p[1], p[2])] = g;
   }
@@ -893,9 +895,9 @@ __global__ void __launch_bounds__(BS) k_mp_residual_lds(const int *__restrict__ 
 // formed in-kernel, so no ghost fill has to precede it.
 template <int D, int NC, int BS, typename T>
 __global__ void __launch_bounds__(BS) k_restrict_residual_lds(const int *__restrict__ ids,
-                                                              const DBox *__restrict__ boxes, T *phi,
-                                                              const T *__restrict__ rhs, T *tmp, T lambda2, int cyl,
-                                                              int store_residual, VarBc m) {
+                                                              const DBox *__restrict__ boxes, const T *child,
+                                                              T *phi, const T *__restrict__ rhs, T *tmp, T lambda2,
+                                                              int cyl, int store_residual, VarBc m) {
   using G = Geo<D, NC>;
   constexpr int H = NC / 2;
   constexpr int NQ = (D == 3) ? H * H * H : H * H;
@@ -905,9 +907,9 @@ __global__ void __launch_bounds__(BS) k_restrict_residual_lds(const int *__restr
   const DBox &bp = boxes[bc.parent - 1];
   const Sten<D, T> st = lpl_stencil<D, T>(bc, lambda2);
   const size_t cb = (size_t)(id - 1) * G::NCELL, pb = (size_t)(bc.parent - 1) * G::NCELL;
-  lds_load_interior<D, NC, BS, T>(s, phi + cb);
+  lds_load_interior<D, NC, BS, T>(s, child + cb);
   __syncthreads();
-  lds_face_ghosts<D, NC, BS, T>(s, boxes, bc, phi, m, -1);
+  lds_face_ghosts<D, NC, BS, T>(s, boxes, bc, child, m, -1, (const T *)phi);
   __syncthreads();
   int off[3] = {0, 0, 0};
   for (int d = 0; d < D; d++) off[d] = ((bc.ix[d] - 1) & 1) * H;
@@ -944,6 +946,111 @@ __global__ void __launch_bounds__(BS) k_restrict_residual_lds(const int *__restr
     }
     tmp[pb + G::idx(ic, jc, kc)] = rv;
     phi[pb + G::idx(ic, jc, kc)] = restrict_cell<D, NC, T>(s, i_f, jf, kf, false, bp, ic);
+  }
+}
+
+// Block smoother (single-precision inner cycle only): all 2 n_cycle
+// half-sweeps of one smoothing phase on a box in LDS, in one launch.  The
+// values of same-level neighbour boxes are taken from `in` at the start and
+// held for the whole phase (the box couples to its neighbours as in a
+// block-Jacobi sweep), which lets every workgroup read `in` while writing
+// `out`; physical and refinement-boundary ghosts depend on the box's own
+// cells and are formed again before every half-sweep.  Inside the box the
+// half-sweeps are exactly those of k_gsrb.
+template <int D, int NC, int BS, typename T>
+__global__ void __launch_bounds__(BS) k_smooth_block(const int *__restrict__ ids, const DBox *__restrict__ boxes,
+                                                     const T *in, T *out, const T *__restrict__ rhs,
+                                                     const T *coarse, int n_half, T lambda2, int cyl, VarBc m) {
+  using G = Geo<D, NC>;
+  __shared__ T s[G::NCELL];
+  const int id = ids[blockIdx.x];
+  const DBox &bx = boxes[id - 1];
+  const Sten<D, T> st = lpl_stencil<D, T>(bx, lambda2);
+  const size_t base = (size_t)(id - 1) * G::NCELL;
+  const T *r = rhs + base;
+  lds_load_interior<D, NC, BS, T>(s, in + base);
+  constexpr int NFC = 2 * D * G::NTAN;
+  // same-level neighbour copies, once
+  for (int q = threadIdx.x; q < NFC; q += BS) {
+    const int nb = q / G::NTAN;
+    if (bx.nb[nb] <= 0) continue;
+    const int rr = q % G::NTAN;
+    const int a = rr % NC + 1, b = (D == 3) ? rr / NC + 1 : 0;
+    const bool low = (nb % 2) == 0;
+    int p[3];
+    face_pos<D>(nb, a, b, low ? 0 : NC + 1, p);
+    s[G::idx(p[0], p[1], p[2])] = ghost_value<D, NC, T>(boxes, bx, in, m, nb, a, b, T(0), T(0));
+  }
+  constexpr int HALF = G::NINT / 2;
+  constexpr int sx = 1, sy = G::NG, sz = G::NG * G::NG;
+  for (int n = 1; n <= n_half; n++) {
+    __syncthreads();
+    // ghosts that depend on the box's own cells, for the colour updated now
+    for (int q = threadIdx.x; q < NFC; q += BS) {
+      const int nb = q / G::NTAN;
+      if (bx.nb[nb] > 0) continue;
+      const int rr = q % G::NTAN;
+      const int a = rr % NC + 1, b = (D == 3) ? rr / NC + 1 : 0;
+      const bool low = (nb % 2) == 0;
+      int p[3];
+      face_pos<D>(nb, a, b, low ? 1 : NC, p);
+      if ((p[0] + p[1] + p[2] + n) & 1) continue;
+      const T x1 = s[G::idx(p[0], p[1], p[2])];
+      face_pos<D>(nb, a, b, low ? 2 : NC - 1, p);
+      const T x2 = s[G::idx(p[0], p[1], p[2])];
+      const T g = ghost_value<D, NC, T>(boxes, bx, in, m, nb, a, b, x1, x2, coarse);
+      face_pos<D>(nb, a, b, low ? 0 : NC + 1, p);
+      s[G::idx(p[0], p[1], p[2])] = g;
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < HALF; q += BS) {
+      const int mm = q % (NC / 2);
+      const int j = (q / (NC / 2)) % NC + 1;
+      const int k = (D == 3) ? q / ((NC / 2) * NC) + 1 : 0;
+      const int i0 = 2 - ((n ^ (D == 3 ? k + j : j)) & 1);
+      const int i = i0 + 2 * mm;
+      const int x = G::idx(i, j, k);
+      T v;
+      if (D == 2) {
+        if (cyl) {
+          T c1, c2, c3;
+          cyl_coeffs<T>(bx, i, st.c, c1, c2, c3);
+          v = (r[x] - c2 * s[x - sx] - c3 * s[x + sx] - st.c[3] * s[x - sy] - st.c[4] * s[x + sy]) / c1;
+        } else {
+          v = (r[x] - st.c[1] * s[x - sx] - st.c[2] * s[x + sx] - st.c[3] * s[x - sy] - st.c[4] * s[x + sy]) /
+              st.c[0];
+        }
+      } else {
+        v = (r[x] - st.c[1] * s[x - sx] - st.c[2] * s[x + sx] - st.c[3] * s[x - sy] - st.c[4] * s[x + sy] -
+             st.c[5] * s[x - sz] - st.c[6] * s[x + sz]) /
+            st.c[0];
+      }
+      s[x] = v;
+    }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < G::NINT; q += BS) {
+    int i, j, k;
+    interior_ijk<D, NC>(q, i, j, k);
+    const int x = G::idx(i, j, k);
+    out[base + x] = s[x];
+  }
+}
+
+// phi += e on the interior of leaf boxes and e = 0 on the whole leaf box,
+// ghost layer included: the refinement-boundary ghosts of finer boxes read
+// a coarse leaf's ghost cells during the next cycle's first sweeps.  (Refined
+// cells get the restriction of the finer phi instead, and e there is
+// overwritten before the next cycle reads it.)
+template <int D, int NC, int BS>
+__global__ void __launch_bounds__(BS) k_mp_correct_leaves(const int *__restrict__ ids, double *phi, float *e) {
+  using G = Geo<D, NC>;
+  const size_t base = (size_t)(ids[blockIdx.x] - 1) * G::NCELL;
+  for (int x = threadIdx.x; x < G::NCELL; x += BS) {
+    const int i = x % G::NG, j = (x / G::NG) % G::NG, k = (D == 3) ? x / (G::NG * G::NG) : 0;
+    const bool inner = i >= 1 && i <= NC && j >= 1 && j <= NC && (D == 2 || (k >= 1 && k <= NC));
+    if (inner) phi[base + x] = phi[base + x] + (double)e[base + x];
+    e[base + x] = 0.0f;
   }
 }
 

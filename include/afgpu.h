@@ -265,6 +265,9 @@ int afgpu_kernel_stats(afgpu_t *h, int kind, int reset, double *total_ms,
 #define AFGPU_OPT_MIXED_SMOOTH 3 /* mixed precision: GSRB sweeps before and after the coarse
                                     correction in the inner cycle; 0 = the params' counts
                                     (default 0) */
+#define AFGPU_OPT_BLOCK_SMOOTH 4 /* mixed precision: inner-cycle smoothing phases as one
+                                    launch per level, each box swept in LDS with its
+                                    same-level neighbours' values held (default 0) */
 int afgpu_set_option(afgpu_t *h, int option, int value);
 
 #ifdef __cplusplus

@@ -148,6 +148,43 @@ def test_mixed_uniform_tree_is_one_direct_solve(A, S):
     assert e < 1e-9
 
 
+BLOCK_CASES = {
+    "cyl2d_amr": CASES["cyl2d_amr"],
+    # level 4 has 512 boxes: one workgroup per box, so the block smoother runs there
+    "xyz3d_l4": lambda S: S.small_amr(3, nc=8, uniform_lvl=4, max_lvl=5, frac=0.15, seed=21),
+}
+
+
+@pytest.mark.parametrize("name", sorted(BLOCK_CASES))
+def test_mixed_block_smoother(A, S, name):
+    """The inner cycle with the block smoother reaches the same fixed point,
+    keeps the step results within the threshold's accuracy, and is
+    deterministic (graph replays equal eager runs)."""
+    case = setup(BLOCK_CASES[name](S))
+    f, m = pair(A, S, case)
+    m.set_option(A.OPT_BLOCK_SMOOTH, 1)
+    m2 = S.apply(A.Path("gpu", case.D, case.nc, case.coord, A.PREC_MIXED), case)
+    m2.set_option(A.OPT_BLOCK_SMOOTH, 1)
+    m2.set_option(A.OPT_GRAPHS, 0)
+    f.field_compute(0, False)
+    assert m.field_compute(0, False) == m2.field_compute(0, False)
+    for it in range(3):
+        a, b, b2 = f.step(), m.step(), m2.step()
+        print(name, it, "fp64", a["n_vcycles"], "block", b["n_vcycles"], "%.2e" % b["rel_residual"], flush=True)
+        assert b == b2
+        assert b["n_vcycles"] <= a["n_vcycles"] + 3
+    assert np.array_equal(m.get_cc(A.CC_PHI), m2.get_cc(A.CC_PHI))
+    e = rel_max(leaf_interior(case, m, A.CC_ELECTRON), leaf_interior(case, f, A.CC_ELECTRON))
+    print(name, "electron error %.2e" % e, flush=True)
+    assert e < 1e-6
+    for _ in range(10):
+        f.mg_fas_vcycle(A.OP_POISSON, True, 0)
+        m.field_compute(0, True)
+    e = rel_max(leaf_interior(case, m, A.CC_PHI), leaf_interior(case, f, A.CC_PHI))
+    print(name, "fixed point phi %.2e" % e, flush=True)
+    assert e < 1e-10
+
+
 def test_mixed_graph_replays_match_eager(A, S):
     case = setup(CASES["xyz3d_amr"](S), photoi=True)
     case.params.photoi_max_rel_residual = 1e-3
