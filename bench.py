@@ -30,6 +30,15 @@ import setups as S  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch of the dominant kernel (the single-precision
+# half-sweep on the finest level of this workload), from the two rocprofv3
+# counter passes in profiles/gsrb_f32_3d_amr_l5_counters.txt, each scaled by
+# the same counter's reading for a plain copy of known size
+# (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000).
+MEASURED_TRAFFIC = {
+    ("xyz3d_amr_l5_r5.2mm", "mixed", "gsrb"): (55301.0 * 2.000 + 39353.8 * 1.000) * 1024,
+}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -168,8 +177,11 @@ def main():
         st = stats[dom]
         avg_ms = st["total_ms"] / max(st["launches"], 1)
         achieved = st["algo_bytes"] / (st["total_ms"] * 1e-3) / 1e9 if st["total_ms"] > 0 else 0.0
+        traffic = None
+        if not args.block_smooth and args.direct_n == 128 and args.mixed_smooth == 0:
+            traffic = MEASURED_TRAFFIC.get((case.name, args.precision, dom))
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "avg_launch_ms": round(avg_ms, 5), "launches": st["launches"],
                 "algo_bytes_per_launch": st["algo_bytes"] / max(st["launches"], 1),
                 "timing": "HIP events around each launch, %d eager steps after the timed region"
