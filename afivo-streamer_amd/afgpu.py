@@ -40,6 +40,8 @@ OPT_GRAPHS = 1
 OPT_DIRECT_N = 2
 OPT_MIXED_SMOOTH = 3
 OPT_BLOCK_SMOOTH = 4
+OPT_BATCH_MODES = 5
+OPT_DIRECT_MFMA = 6
 
 
 class ParamsT(C.Structure):

@@ -95,6 +95,18 @@ struct afgpu_ctx {
   int D = 2, nc = 8, coord = AFGPU_XYZ, precision = AFGPU_PREC_FP64;
   size_t ncell = 0, nface = 0;  // nface: all D blocks of one box
   hipStream_t stream = nullptr;
+  hipStream_t cur = nullptr;  // stream of the launch helpers: `stream`, or a mode stream inside a fork
+  // photoionization modes of the mixed-precision form run as parallel
+  // branches (one stream each, default priority) inside one captured graph
+  bool batch_modes = true;
+  hipStream_t mstream[AFGPU_MAX_MODES] = {nullptr};
+  hipEvent_t fork_ev = nullptr, join_ev[AFGPU_MAX_MODES] = {nullptr};
+  struct ModeRes {
+    float *cc32[4] = {nullptr};  // correction, its rhs, scratch, second buffer
+    double *A0 = nullptr, *A1 = nullptr;  // uniform-level solve grids
+    size_t a_cap = 0;
+    int cap_boxes = 0;
+  } mres[AFGPU_MAX_MODES];
   // tree
   int n_boxes = 0, cap_boxes = 0, highest_lvl = 0;
   std::vector<DBox> hboxes;
@@ -124,6 +136,7 @@ struct afgpu_ctx {
   int direct_max_n = 128;
   int mixed_smooth = 0;  // > 0: smoothing sweeps of the inner cycle (down = up)
   int block_smooth = 0;  // inner cycle: block smoother on levels with one workgroup per box
+  bool direct_mfma = true;  // uniform-level transforms on the matrix cores
   struct Direct {
     bool set = false;  // lc and the map below are current for the tree
     int lc = 1, N = 0;
@@ -194,7 +207,20 @@ VarBc var_methods(const afgpu_ctx &c, int iv, bool homogeneous = false) {
   return m;
 }
 
-enum GraphKind { G_VCYCLE = 1, G_FMG = 2, G_FFP = 3, G_MP_RES = 4, G_MP_VCYCLE = 5, G_MP_GC = 6, G_MP_FMG = 7 };
+enum GraphKind {
+  G_VCYCLE = 1,
+  G_FMG = 2,
+  G_FFP = 3,
+  G_MP_RES = 4,
+  G_MP_VCYCLE = 5,
+  G_MP_GC = 6,
+  G_MP_FMG = 7,
+  G_MP_RES_MODES = 8,
+  G_MP_FMG_MODES = 9
+};
+// reduction slots: 0 residual max, 1-3 dt limits, 8 + n photoionization mode n
+constexpr int N_RED_SLOTS = 16;
+constexpr int RED_MODE = 8;
 
 int op_phi(int op) { return op == AFGPU_OP_POISSON ? AFGPU_CC_PHI : AFGPU_CC_HELMH + (op - AFGPU_OP_HELMH); }
 
@@ -216,14 +242,14 @@ int timer_start(afgpu_ctx &c, int kind, double bytes) {
     c.ev_used += 2;
     c.pending.push_back(tp);
   }
-  HIPCHK(hipEventRecord(tp.a, c.stream));
+  HIPCHK(hipEventRecord(tp.a, c.cur));
   return AFGPU_OK;
 }
 int timer_stop(afgpu_ctx &c, int kind) {
   if (!c.timing) return AFGPU_OK;
   const TPair &tp = c.capture_pairs ? c.capture_pairs->back() : c.pending.back();
   (void)kind;
-  HIPCHK(hipEventRecord(tp.b, c.stream));
+  HIPCHK(hipEventRecord(tp.b, c.cur));
   return AFGPU_OK;
 }
 // Accumulate the pairs recorded so far (the stream has been synchronised).
@@ -295,7 +321,7 @@ struct Impl : ImplBase {
   template <typename K, typename... Args>
   int launch(K kernel, int nblocks, Args... args) {
     if (nblocks <= 0) return AFGPU_OK;
-    hipLaunchKernelGGL(kernel, dim3(nblocks), dim3(BS), 0, c.stream, args...);
+    hipLaunchKernelGGL(kernel, dim3(nblocks), dim3(BS), 0, c.cur, args...);
     HIPCHK(hipGetLastError());
     return AFGPU_OK;
   }
@@ -345,9 +371,15 @@ struct Impl : ImplBase {
     return {op, c.cc[iphi], c.cc[AFGPU_CC_RHS], c.cc[AFGPU_CC_TMP], var_methods(c, iphi), c.ops[op].lambda2,
             nullptr};
   }
+  // single-precision pools of an operator: the photoionization modes each
+  // have their own, so that they can be solved side by side
+  float *f32(int op, int k) const {
+    return op >= AFGPU_OP_HELMH ? c.mres[op - AFGPU_OP_HELMH].cc32[k] : c.cc32[k];
+  }
+  static int red_slot(int op) { return op >= AFGPU_OP_HELMH ? RED_MODE + (op - AFGPU_OP_HELMH) : 0; }
   Mg<float> mg32(int op) {
-    return {op,        c.cc32[0], c.cc32[1], c.cc32[2], var_methods(c, op_phi(op), true), (float)c.ops[op].lambda2,
-            c.cc32[3]};
+    return {op, f32(op, 0), f32(op, 1), f32(op, 2), var_methods(c, op_phi(op), true), (float)c.ops[op].lambda2,
+            f32(op, 3)};
   }
 
   // Levels of the single-precision inner cycle that use the block smoother:
@@ -465,7 +497,7 @@ struct Impl : ImplBase {
   int solve_coarse(const Mg<T> &v) {
     const int id = c.h_lists[c.l_ids[1].off];
     const afgpu_ctx::Op &o = c.ops[v.op];
-    hipLaunchKernelGGL((k_coarse_solve<D, NC, 1024, T>), dim3(1), dim3(1024), 0, c.stream, id,
+    hipLaunchKernelGGL((k_coarse_solve<D, NC, 1024, T>), dim3(1), dim3(1024), 0, c.cur, id,
                        (const DBox *)c.d_boxes, (const T *)v.rhs, v.phi, v.m, o.lambda2,
                        (int)(c.coord == AFGPU_CYL), (const double *)o.d_vinv, (const double *)o.d_v,
                        (const double *)o.d_eig);
@@ -647,8 +679,7 @@ struct Impl : ImplBase {
   }
 
   template <int N>
-  int direct_transforms(const afgpu_ctx::Direct::F &f, double lambda2) {
-    afgpu_ctx::Direct &d = c.dir;
+  int direct_transforms(const afgpu_ctx::Direct::F &f, double lambda2, double *A0, double *A1) {
     const size_t NN = (size_t)N * N;
     const int nrows = (D == 3) ? N * N : N;
     constexpr int TI = (N < 64) ? N : 64;
@@ -661,35 +692,44 @@ struct Impl : ImplBase {
     constexpr bool TILED = (N == 64 || N == 128);
     const int gemm_row_blocks = (N / 64) * (nrows / 64);
     const int gemm_line_blocks = (N / 64) * (N / 64) * ((D == 3) ? N : 1);
+    const bool MC = TILED && c.direct_mfma;
+    const int mfma_row_blocks = (N / 64) * (nrows / 64);
+    const int mfma_line_blocks = (N / 64) * (N / 64) * ((D == 3) ? N : 1);
     auto rows = [&](const double *in, double *out, const double *Mx, const double *MTx) {
-      if (TILED)
-        hipLaunchKernelGGL((k_lc_gemm<D, (TILED ? N : 64), true>), dim3(gemm_row_blocks), dim3(256), 0, c.stream, in,
+      if (MC)
+        hipLaunchKernelGGL((k_lc_mfma<D, (TILED ? N : 64), true>), dim3(mfma_row_blocks), dim3(256), 0, c.cur, in,
+                           out, Mx, 0, 0, (const double *)f.eig, lambda2);
+      else if (TILED)
+        hipLaunchKernelGGL((k_lc_gemm<D, (TILED ? N : 64), true>), dim3(gemm_row_blocks), dim3(256), 0, c.cur, in,
                            out, Mx, 0, 0, (const double *)f.eig, lambda2);
       else
-        hipLaunchKernelGGL((k_lc_rows<D, N>), dim3(row_blocks), dim3(256), 0, c.stream, in, out, MTx, nrows);
+        hipLaunchKernelGGL((k_lc_rows<D, N>), dim3(row_blocks), dim3(256), 0, c.cur, in, out, MTx, nrows);
       return hipGetLastError();
     };
     auto lines = [&](const double *in, double *out, const double *M, int axis, int div) {
-      if (TILED)
-        hipLaunchKernelGGL((k_lc_gemm<D, (TILED ? N : 64), false>), dim3(gemm_line_blocks), dim3(256), 0, c.stream,
+      if (MC)
+        hipLaunchKernelGGL((k_lc_mfma<D, (TILED ? N : 64), false>), dim3(mfma_line_blocks), dim3(256), 0, c.cur,
+                           in, out, M, axis, div, (const double *)f.eig, lambda2);
+      else if (TILED)
+        hipLaunchKernelGGL((k_lc_gemm<D, (TILED ? N : 64), false>), dim3(gemm_line_blocks), dim3(256), 0, c.cur,
                            in, out, M, axis, div, (const double *)f.eig, lambda2);
       else
-        hipLaunchKernelGGL((k_lc_lines<D, N>), dim3(line_blocks), dim3(256), 0, c.stream, in, out, M, axis, div,
+        hipLaunchKernelGGL((k_lc_lines<D, N>), dim3(line_blocks), dim3(256), 0, c.cur, in, out, M, axis, div,
                            (const double *)f.eig, lambda2);
       return hipGetLastError();
     };
     if (D == 3) {
-      HIPCHK(rows(d.A0, d.A1, f.vinv, f.vinvT));
-      HIPCHK(lines(d.A1, d.A0, f.vinv + NN, 1, 0));
-      HIPCHK(lines(d.A0, d.A1, f.vinv + 2 * NN, 2, 1));
-      HIPCHK(lines(d.A1, d.A0, f.v + 2 * NN, 2, 0));
-      HIPCHK(lines(d.A0, d.A1, f.v + NN, 1, 0));
-      HIPCHK(rows(d.A1, d.A0, f.v, f.vT));
+      HIPCHK(rows(A0, A1, f.vinv, f.vinvT));
+      HIPCHK(lines(A1, A0, f.vinv + NN, 1, 0));
+      HIPCHK(lines(A0, A1, f.vinv + 2 * NN, 2, 1));
+      HIPCHK(lines(A1, A0, f.v + 2 * NN, 2, 0));
+      HIPCHK(lines(A0, A1, f.v + NN, 1, 0));
+      HIPCHK(rows(A1, A0, f.v, f.vT));
     } else {
-      HIPCHK(rows(d.A0, d.A1, f.vinv, f.vinvT));
-      HIPCHK(lines(d.A1, d.A0, f.vinv + NN, 1, 1));
-      HIPCHK(lines(d.A0, d.A1, f.v + NN, 1, 0));
-      HIPCHK(rows(d.A1, d.A0, f.v, f.vT));
+      HIPCHK(rows(A0, A1, f.vinv, f.vinvT));
+      HIPCHK(lines(A1, A0, f.vinv + NN, 1, 1));
+      HIPCHK(lines(A0, A1, f.v + NN, 1, 0));
+      HIPCHK(rows(A1, A0, f.v, f.vT));
     }
     return AFGPU_OK;
   }
@@ -706,19 +746,26 @@ struct Impl : ImplBase {
     }
     const int nb = d.N / NC;
     const int nbox = c.l_ids[lc].n;
-    hipLaunchKernelGGL((k_lc_gather<D, NC, BS, T>), dim3(nbox), dim3(BS), 0, c.stream, (const int *)d.d_map, nb,
-                       (const T *)v.rhs, d.A0);
+    const bool mode = v.op >= AFGPU_OP_HELMH;
+    double *A0 = mode ? c.mres[v.op - AFGPU_OP_HELMH].A0 : d.A0;
+    double *A1 = mode ? c.mres[v.op - AFGPU_OP_HELMH].A1 : d.A1;
+    if (!A0 || !A1) {
+      set_err("uniform-level solve: grids not allocated");
+      return AFGPU_ERR_STATE;
+    }
+    hipLaunchKernelGGL((k_lc_gather<D, NC, BS, T>), dim3(nbox), dim3(BS), 0, c.cur, (const int *)d.d_map, nb,
+                       (const T *)v.rhs, A0);
     HIPCHK(hipGetLastError());
     const double l2 = c.ops[v.op].lambda2;
     switch (d.N) {
-      case 16: RCHK(direct_transforms<16>(f, l2)); break;
-      case 32: RCHK(direct_transforms<32>(f, l2)); break;
-      case 64: RCHK(direct_transforms<64>(f, l2)); break;
-      case 128: RCHK(direct_transforms<128>(f, l2)); break;
+      case 16: RCHK(direct_transforms<16>(f, l2, A0, A1)); break;
+      case 32: RCHK(direct_transforms<32>(f, l2, A0, A1)); break;
+      case 64: RCHK(direct_transforms<64>(f, l2, A0, A1)); break;
+      case 128: RCHK(direct_transforms<128>(f, l2, A0, A1)); break;
       default: set_err("uniform-level solve: unsupported size"); return AFGPU_ERR_UNSUPPORTED;
     }
-    hipLaunchKernelGGL((k_lc_scatter<D, NC, BS, T>), dim3(nbox), dim3(BS), 0, c.stream, (const int *)d.d_map, nb,
-                       (const double *)d.A0, v.phi);
+    hipLaunchKernelGGL((k_lc_scatter<D, NC, BS, T>), dim3(nbox), dim3(BS), 0, c.cur, (const int *)d.d_map, nb,
+                       (const double *)A0, v.phi);
     HIPCHK(hipGetLastError());
     return gc_var(lc, v.phi, v.m, true);
   }
@@ -736,7 +783,7 @@ struct Impl : ImplBase {
     const int iphi = op_phi(op);
     if (correct) {
       const List &l = c.l_all_leaves;
-      RCHK(launch(k_mp_correct_leaves<D, NC, BS>, l.n, c.list(l), c.cc[iphi], c.cc32[0]));
+      RCHK(launch(k_mp_correct_leaves<D, NC, BS>, l.n, c.list(l), c.cc[iphi], f32(op, 0)));
     }
     // (levels below lc are covered everywhere and not part of the iteration)
     RCHK(restrict_levels(iphi, c.highest_lvl, lc + 1));
@@ -746,16 +793,17 @@ struct Impl : ImplBase {
     // leaves' own ghosts are formed inside the residual kernel.
     const VarBc m = var_methods(c, iphi);
     for (int lvl = lc; lvl < c.highest_lvl; lvl++) RCHK(gc_var(lvl, c.cc[iphi], m, true));
-    RCHK(set_red(0, 0ull));
+    const int slot = red_slot(op);
+    RCHK(set_red(slot, 0ull));
     const List &l = c.l_all_leaves;
     const int k = bpb(l.n, G::NINT);
     if (k == 1)
       return launch(k_mp_residual_lds<D, NC, BS>, l.n, c.list(l), (const DBox *)c.d_boxes,
-                    (const double *)c.cc[iphi], (const double *)c.cc[AFGPU_CC_RHS], c.cc32[1], c.ops[op].lambda2,
-                    (int)(c.coord == AFGPU_CYL), m, c.d_red);
+                    (const double *)c.cc[iphi], (const double *)c.cc[AFGPU_CC_RHS], f32(op, 1), c.ops[op].lambda2,
+                    (int)(c.coord == AFGPU_CYL), m, c.d_red + slot);
     return launch(k_mp_residual<D, NC, BS>, l.n * k, c.list(l), (const DBox *)c.d_boxes, (const double *)c.cc[iphi],
-                  (const double *)c.cc[AFGPU_CC_RHS], c.cc32[1], c.ops[op].lambda2, (int)(c.coord == AFGPU_CYL),
-                  m, c.d_red, k);
+                  (const double *)c.cc[AFGPU_CC_RHS], f32(op, 1), c.ops[op].lambda2, (int)(c.coord == AFGPU_CYL),
+                  m, c.d_red + slot, k);
   }
 
   // phi on refined cells := restriction of the finer phi (no geometry
@@ -787,7 +835,7 @@ struct Impl : ImplBase {
   }
   // reduction slot i := value (a kernel, so that it can live in a graph)
   int set_red(int i, unsigned long long value) {
-    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, c.stream, c.d_red + i, value);
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, c.cur, c.d_red + i, value);
     HIPCHK(hipGetLastError());
     return AFGPU_OK;
   }
@@ -970,6 +1018,60 @@ struct Impl : ImplBase {
   }
 
   // --- photoionization (src/m_photoi.f90, src/m_photoi_helmh.f90) -----------
+  // Per-mode single-precision pools and uniform-level grids (allocated
+  // outside any capture; zero-filled, as the correction must start at 0).
+  int ensure_mode_pools() {
+    const afgpu_ctx::Direct &d = c.dir;
+    for (int n = 0; n < c.p.photoi_n_modes; n++) {
+      afgpu_ctx::ModeRes &r = c.mres[n];
+      if (r.cap_boxes != c.cap_boxes) {
+        for (int k = 0; k < 4; k++) {
+          (void)hipFree(r.cc32[k]);
+          r.cc32[k] = nullptr;
+          HIPCHK(hipMalloc(&r.cc32[k], (size_t)c.cap_boxes * c.ncell * sizeof(float)));
+          HIPCHK(hipMemsetAsync(r.cc32[k], 0, (size_t)c.cap_boxes * c.ncell * sizeof(float), c.stream));
+        }
+        r.cap_boxes = c.cap_boxes;
+      }
+      if (d.lc > 1 && r.a_cap < d.cap) {
+        (void)hipFree(r.A0);
+        (void)hipFree(r.A1);
+        r.A0 = r.A1 = nullptr;
+        HIPCHK(hipMalloc(&r.A0, d.cap * sizeof(double)));
+        HIPCHK(hipMalloc(&r.A1, d.cap * sizeof(double)));
+        r.a_cap = d.cap;
+      }
+    }
+    return AFGPU_OK;
+  }
+
+  // Run body(op) for every photoionization mode in `mask`: as parallel
+  // branches on the mode streams (forked from and joined back into the
+  // handle's stream, so that a capture records them as branches of one
+  // graph), or one after the other on the handle's stream.
+  template <typename F>
+  int fork_modes(int mask, F body) {
+    int count = 0;
+    for (int n = 0; n < c.p.photoi_n_modes; n++) count += (mask >> n) & 1;
+    if (!c.batch_modes || count < 2) {
+      for (int n = 0; n < c.p.photoi_n_modes; n++)
+        if ((mask >> n) & 1) RCHK(body(AFGPU_OP_HELMH + n));
+      return AFGPU_OK;
+    }
+    HIPCHK(hipEventRecord(c.fork_ev, c.stream));
+    int rc = AFGPU_OK;
+    for (int n = 0; n < c.p.photoi_n_modes && rc == AFGPU_OK; n++) {
+      if (!((mask >> n) & 1)) continue;
+      HIPCHK(hipStreamWaitEvent(c.mstream[n], c.fork_ev, 0));
+      c.cur = c.mstream[n];
+      rc = body(AFGPU_OP_HELMH + n);
+      c.cur = c.stream;
+      HIPCHK(hipEventRecord(c.join_ev[n], c.mstream[n]));
+      HIPCHK(hipStreamWaitEvent(c.stream, c.join_ev[n], 0));
+    }
+    return rc;
+  }
+
   int photoi() override {
     RCHK(launch(k_photoi_src<D, NC, BS>, c.l_all_leaves.n, c.list(c.l_all_leaves),
                 (const double *)c.cc[AFGPU_CC_EFIELD], (const double *)c.cc[AFGPU_CC_ELECTRON], c.cc[AFGPU_CC_RHS],
@@ -978,36 +1080,52 @@ struct Impl : ImplBase {
     double max_rhs;
     RCHK(maxabs(AFGPU_CC_RHS, &max_rhs));
     max_rhs = std::max(max_rhs, std::sqrt(std::numeric_limits<double>::epsilon()));
-    for (int n = 0; n < c.p.photoi_n_modes; n++) {
-      const int op = AFGPU_OP_HELMH + n;
-      if (c.precision == AFGPU_PREC_MIXED) {
-        // FP64 defect correction around single-precision FMG cycles on the
-        // correction (from e = 0, rhs = the FP64 residual of the previous
-        // solution of this mode)
-        int lc;
-        RCHK(ensure_factors(op));
-        RCHK(direct_level(&lc));
-        RCHK(ensure_direct_factors(op));
-        RCHK(run_graph(c, {G_MP_RES, op, lc, 0}, [&] { return mp_close(op, false, lc); }));
-        for (int i = 1; i <= c.p.photoi_max_fmg_cycles; i++) {
-          RCHK(run_graph(c, {G_MP_FMG, op, lc, 0}, [&] {
+    const int n_modes = c.p.photoi_n_modes;
+    if (c.precision == AFGPU_PREC_MIXED && n_modes > 0) {
+      // Each mode: FP64 defect correction around single-precision FMG
+      // cycles on its correction (from e = 0, rhs = the FP64 residual of the
+      // previous solution of the mode), stopping on the reference's test
+      // (m_photoi_helmh.f90:184-189).  The modes are independent, so their
+      // cycles run side by side; each mode stops on its own.
+      int lc;
+      RCHK(direct_level(&lc));
+      for (int n = 0; n < n_modes; n++) {
+        RCHK(ensure_factors(AFGPU_OP_HELMH + n));
+        RCHK(ensure_direct_factors(AFGPU_OP_HELMH + n));
+      }
+      RCHK(ensure_mode_pools());
+      int active = (1 << n_modes) - 1;
+      RCHK(run_graph(c, {G_MP_RES_MODES, active, lc, 0},
+                     [&] { return fork_modes(active, [&](int op) { return mp_close(op, false, lc); }); }));
+      for (int i = 1; i <= c.p.photoi_max_fmg_cycles && active; i++) {
+        RCHK(run_graph(c, {G_MP_FMG_MODES, active, lc, 0}, [&] {
+          return fork_modes(active, [&](int op) {
             RCHK(fmg_body(mg32(op), false, false, lc));
             return mp_close(op, true, lc);
-          }));
-          double residu;
-          RCHK(leaf_residual_max(&residu));
-          if (residu / max_rhs < c.p.photoi_max_rel_residual) break;
-        }
-        // No final ghost refresh of the mode: the next solve forms its
-        // residual with in-kernel ghosts, and the photoionization source is
-        // read on interior cells only (src/m_fluid.f90:437-439).
-      } else {
-        for (int i = 1; i <= c.p.photoi_max_fmg_cycles; i++) {
-          RCHK(fmg(op, true, true));
-          double residu;
-          RCHK(leaf_residual_max(&residu));
-          if (residu / max_rhs < c.p.photoi_max_rel_residual) break;
-        }
+          });
+        }));
+        RCHK(read_red(N_RED_SLOTS));
+        for (int n = 0; n < n_modes; n++)
+          if ((active >> n) & 1) {
+            const double residu = as_double(c.h_red[RED_MODE + n]);
+            if (residu / max_rhs < c.p.photoi_max_rel_residual) active &= ~(1 << n);
+          }
+      }
+      // No final ghost refresh of the modes: the next solve forms its
+      // residual with in-kernel ghosts, and the photoionization source is
+      // read on interior cells only (src/m_fluid.f90:437-439).
+      for (int n = 0; n < n_modes; n++)
+        RCHK(boxop(c.l_all_leaves, c.cc[op_phi(AFGPU_OP_HELMH + n)], c.cc[AFGPU_CC_PHOTO], OP_AXPY_NEG,
+                   c.p.photoi_coeff[n]));
+      return AFGPU_OK;
+    }
+    for (int n = 0; n < n_modes; n++) {
+      const int op = AFGPU_OP_HELMH + n;
+      for (int i = 1; i <= c.p.photoi_max_fmg_cycles; i++) {
+        RCHK(fmg(op, true, true));
+        double residu;
+        RCHK(leaf_residual_max(&residu));
+        if (residu / max_rhs < c.p.photoi_max_rel_residual) break;
       }
       RCHK(boxop(c.l_all_leaves, c.cc[op_phi(op)], c.cc[AFGPU_CC_PHOTO], OP_AXPY_NEG, c.p.photoi_coeff[n]));
     }
@@ -1111,8 +1229,14 @@ int afgpu_create(int ndim, int n_cell, int coord, int precision, afgpu_t **out) 
     return AFGPU_ERR_HIP;
   }
   HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-  HIPCHK(hipMalloc(&h->d_red, 8 * sizeof(unsigned long long)));
-  HIPCHK(hipHostMalloc(&h->h_red, 8 * sizeof(unsigned long long), hipHostMallocDefault));
+  h->cur = h->stream;
+  for (int n = 0; n < AFGPU_MAX_MODES; n++) {
+    HIPCHK(hipStreamCreateWithFlags(&h->mstream[n], hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&h->join_ev[n], hipEventDisableTiming));
+  }
+  HIPCHK(hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming));
+  HIPCHK(hipMalloc(&h->d_red, N_RED_SLOTS * sizeof(unsigned long long)));
+  HIPCHK(hipHostMalloc(&h->h_red, N_RED_SLOTS * sizeof(unsigned long long), hipHostMallocDefault));
   *out = h.release();
   return AFGPU_OK;
 }
@@ -1146,6 +1270,14 @@ int afgpu_destroy(afgpu_t *h) {
     (void)hipFree(o.d_v);
     (void)hipFree(o.d_eig);
   }
+  for (int n = 0; n < AFGPU_MAX_MODES; n++) {
+    if (h->mstream[n]) (void)hipStreamDestroy(h->mstream[n]);
+    if (h->join_ev[n]) (void)hipEventDestroy(h->join_ev[n]);
+    for (int k = 0; k < 4; k++) (void)hipFree(h->mres[n].cc32[k]);
+    (void)hipFree(h->mres[n].A0);
+    (void)hipFree(h->mres[n].A1);
+  }
+  if (h->fork_ev) (void)hipEventDestroy(h->fork_ev);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return AFGPU_OK;
@@ -1254,12 +1386,22 @@ int afgpu_sync_tree(afgpu_t *h, int n_boxes, const int *lvl, const int *ix, cons
           return AFGPU_ERR_ARG;
         }
     }
+  // Lists are stored XCD-aware: the GPU hands consecutive workgroups to its
+  // 8 XCDs in turn, each XCD with its own L2, so workgroup j takes box
+  // v[start(j % 8) + j / 8] of the id-ordered list v.  Each XCD then works
+  // through a contiguous run of boxes (siblings and near neighbours), whose
+  // shared faces stay in its L2.
   std::vector<int> all;
   auto add = [&](const std::vector<int> &v) {
+    constexpr int X = 8;
     List l;
     l.off = (int)all.size();
     l.n = (int)v.size();
-    all.insert(all.end(), v.begin(), v.end());
+    const int n = l.n;
+    int start[X + 1];
+    start[0] = 0;
+    for (int x = 0; x < X; x++) start[x + 1] = start[x] + (n > x ? (n - x + X - 1) / X : 0);
+    for (int j = 0; j < n; j++) all.push_back(v[start[j % X] + j / X]);
     return l;
   };
   h->l_ids.assign(hl + 1, List());
@@ -1303,6 +1445,16 @@ int afgpu_sync_tree(afgpu_t *h, int n_boxes, const int *lvl, const int *ix, cons
       }
     }
     h->cap_boxes = cap;
+  }
+  // The single-precision corrections start every cycle from zero on the
+  // leaves; box roles may have changed, so clear them.
+  if (h->precision == AFGPU_PREC_MIXED) {
+    const size_t bytes = (size_t)h->cap_boxes * h->ncell * sizeof(float);
+    for (int i = 0; i < 4; i++)
+      if (h->cc32[i]) HIPCHK(hipMemsetAsync(h->cc32[i], 0, bytes, h->stream));
+    for (auto &r : h->mres)
+      for (int k = 0; k < 4; k++)
+        if (r.cc32[k] && r.cap_boxes == h->cap_boxes) HIPCHK(hipMemsetAsync(r.cc32[k], 0, bytes, h->stream));
   }
   (void)hipFree(h->d_boxes);
   (void)hipFree(h->d_lists);
@@ -1634,6 +1786,16 @@ int afgpu_set_option(afgpu_t *h, int option, int value) {
       h->direct_max_n = value;
       h->dir.set = false;
       for (auto &f : h->dir.f) f.ok = false;
+      return AFGPU_OK;
+    case AFGPU_OPT_DIRECT_MFMA:
+      RCHK(sync_stream(h));
+      drop_graphs(*h);
+      h->direct_mfma = value != 0;
+      return AFGPU_OK;
+    case AFGPU_OPT_BATCH_MODES:
+      RCHK(sync_stream(h));
+      drop_graphs(*h);
+      h->batch_modes = value != 0;
       return AFGPU_OK;
     case AFGPU_OPT_BLOCK_SMOOTH:
       RCHK(sync_stream(h));

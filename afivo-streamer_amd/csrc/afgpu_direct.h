@@ -258,4 +258,100 @@ __global__ void __launch_bounds__(256) k_lc_gemm(const double *__restrict__ in, 
     }
 }
 
+// The same transforms on the FP64 matrix cores (v_mfma_f64_16x16x4f64):
+//   D[p][q] = sum_k A[p][k] B[k][q]
+// y / z forms: p = output index i, q = column x, A = M, B = the input lines;
+// x form (ROWS): p = grid row r, q = output index i, A = the input rows,
+// B[k][i] = M[i][k].  A workgroup (4 waves) computes a 64 x 64 tile of D,
+// each wave a 32 x 32 block as 2 x 2 MFMA tiles; K runs in chunks of 16
+// staged in LDS.  Lane layout of the instruction: A operand A[l % 16][l / 16],
+// B operand B[l / 16][l % 16], result register j of lane l D[4 j + l / 16][l % 16].
+typedef double lc_double4 __attribute__((ext_vector_type(4)));
+
+template <int D, int N, bool ROWS>
+__global__ void __launch_bounds__(256) k_lc_mfma(const double *__restrict__ in, double *__restrict__ out,
+                                                 const double *__restrict__ M, int axis, int div,
+                                                 const double *__restrict__ eig, double lambda2) {
+  static_assert(N == 64 || N == 128, "matrix-core transform for N = 64, 128");
+  constexpr int T = 64, KC = 16, AP = KC + 1, BP = T + 1, NT = N / T;
+  __shared__ double As[T * AP];  // As[p][kk]
+  __shared__ double Bs[KC * BP];  // Bs[kk][q]
+  // blocks: ROWS: (output tiles) x (row tiles); else (output tiles) x
+  // (column tiles) x (the remaining coordinate)
+  const int t0 = blockIdx.x % NT, rest = blockIdx.x / NT;
+  size_t sm = 1, base = 0;
+  int o = 0, x0 = 0, p0 = 0, q0 = 0;
+  if (ROWS) {
+    p0 = rest * T;  // grid rows
+    q0 = t0 * T;    // output index i
+  } else {
+    p0 = t0 * T;  // output index i
+    x0 = (rest % NT) * T;
+    o = rest / NT;
+    q0 = 0;
+    sm = (axis == 1) ? (size_t)N : (size_t)N * N;
+    const size_t so = (D == 2) ? 0 : ((axis == 1) ? (size_t)N * N : (size_t)N);
+    base = (size_t)o * so + x0;
+  }
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wp = (w >> 1) * 32, wq = (w & 1) * 32;
+  lc_double4 acc[2][2];
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++) acc[a][b] = lc_double4{0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < N; k0 += KC) {
+    for (int q = tid; q < T * KC; q += 256) {
+      const int pp = q / KC, kk = q % KC;  // consecutive threads: consecutive k (contiguous in memory)
+      As[pp * AP + kk] = ROWS ? in[(size_t)(p0 + pp) * N + k0 + kk] : M[(size_t)(p0 + pp) * N + k0 + kk];
+      if (ROWS) Bs[kk * BP + pp] = M[(size_t)(q0 + pp) * N + k0 + kk];
+    }
+    if (!ROWS)
+      for (int q = tid; q < T * KC; q += 256) {
+        const int kk = q / T, qq = q % T;  // consecutive threads: consecutive columns
+        Bs[kk * BP + qq] = in[base + (size_t)(k0 + kk) * sm + qq];
+      }
+    __syncthreads();
+    for (int ks = 0; ks < KC; ks += 4) {
+      const int kk = ks + (lane >> 4);
+      double a[2], b[2];
+      for (int t = 0; t < 2; t++) {
+        a[t] = As[(wp + 16 * t + (lane & 15)) * AP + kk];
+        b[t] = Bs[kk * BP + wq + 16 * t + (lane & 15)];
+      }
+      for (int ta = 0; ta < 2; ta++)
+        for (int tb = 0; tb < 2; tb++) acc[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta], b[tb], acc[ta][tb], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  for (int ta = 0; ta < 2; ta++)
+    for (int tb = 0; tb < 2; tb++)
+      for (int j = 0; j < 4; j++) {
+        const int pp = p0 + wp + 16 * ta + 4 * j + (lane >> 4);
+        const int qq = q0 + wq + 16 * tb + (lane & 15);
+        double v = acc[ta][tb][j];
+        if (ROWS) {
+          out[(size_t)pp * N + qq] = v;
+        } else {
+          if (div) {
+            const int x = x0 + qq;
+            int y, z;
+            if (D == 2) {
+              y = pp;
+              z = 0;
+            } else if (axis == 1) {
+              y = pp;
+              z = o;
+            } else {
+              y = o;
+              z = pp;
+            }
+            double den = eig[x] + eig[N + y];
+            if (D == 3) den = den + eig[2 * N + z];
+            den = den - lambda2;
+            v = v / den;
+          }
+          out[base + (size_t)pp * sm + qq] = v;
+        }
+      }
+}
+
 }  // namespace afg

@@ -843,18 +843,29 @@ __global__ void __launch_bounds__(BS) k_gsrb_lds(const int *__restrict__ ids, co
         T c1, c2, c3;
         cyl_coeffs<T>(bx, i, st.c, c1, c2, c3);
         T inv1 = T(1) / c1;
-        u[x] = (r[x] - c2 * s[x - sx] - c3 * s[x + sx] - st.c[3] * s[x - sy] - st.c[4] * s[x + sy]) * inv1;
+        s[x] = (r[x] - c2 * s[x - sx] - c3 * s[x + sx] - st.c[3] * s[x - sy] - st.c[4] * s[x + sy]) * inv1;
       } else {
         T inv_c1 = T(1) / st.c[0];
-        u[x] = (r[x] - st.c[1] * s[x - sx] - st.c[2] * s[x + sx] - st.c[3] * s[x - sy] - st.c[4] * s[x + sy]) *
+        s[x] = (r[x] - st.c[1] * s[x - sx] - st.c[2] * s[x + sx] - st.c[3] * s[x - sy] - st.c[4] * s[x + sy]) *
                inv_c1;
       }
     } else {
       T inv_c1 = T(1) / st.c[0];
-      u[x] = (r[x] - st.c[1] * s[x - sx] - st.c[2] * s[x + sx] - st.c[3] * s[x - sy] - st.c[4] * s[x + sy] -
+      s[x] = (r[x] - st.c[1] * s[x - sx] - st.c[2] * s[x + sx] - st.c[3] * s[x - sy] - st.c[4] * s[x + sy] -
               st.c[5] * s[x - sz] - st.c[6] * s[x + sz]) *
              inv_c1;
     }
+  }
+  // Whole interior rows back (the other colour unchanged): full-line
+  // stores instead of every other word, so no line has to be fetched to
+  // merge a partial write.  Neighbouring boxes only read this box's other
+  // colour during the launch, which keeps its value.
+  __syncthreads();
+  for (int q = threadIdx.x; q < G::NINT; q += BS) {
+    int i, j, k;
+    interior_ijk<D, NC>(q, i, j, k);
+    const int x = G::idx(i, j, k);
+    u[x] = s[x];
   }
 }
 

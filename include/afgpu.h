@@ -268,6 +268,11 @@ int afgpu_kernel_stats(afgpu_t *h, int kind, int reset, double *total_ms,
 #define AFGPU_OPT_BLOCK_SMOOTH 4 /* mixed precision: inner-cycle smoothing phases as one
                                     launch per level, each box swept in LDS with its
                                     same-level neighbours' values held (default 0) */
+#define AFGPU_OPT_BATCH_MODES 5 /* mixed precision: the photoionization modes' cycles run as
+                                   parallel branches of one captured graph (default 1) */
+#define AFGPU_OPT_DIRECT_MFMA 6 /* mixed precision: uniform-level transforms (64 or 128 cells
+                                   per axis) on the FP64 matrix cores (1) or as vector FMA
+                                   tiles (0) (default 1) */
 int afgpu_set_option(afgpu_t *h, int option, int value);
 
 #ifdef __cplusplus

@@ -185,6 +185,47 @@ def test_mixed_block_smoother(A, S, name):
     assert e < 1e-10
 
 
+@pytest.mark.parametrize("name", ["cyl2d_amr", "xyz3d_16"])
+def test_mixed_batched_modes_equal_sequential(A, S, name):
+    """The photoionization modes solved side by side (parallel branches of
+    one captured graph) give bit for bit the results of solving them one
+    after the other, eagerly."""
+    case = setup(CASES[name](S), photoi=True)
+    case.params.photoi_max_rel_residual = 1e-4
+    m1 = S.apply(A.Path("gpu", case.D, case.nc, case.coord, A.PREC_MIXED), case)
+    m2 = S.apply(A.Path("gpu", case.D, case.nc, case.coord, A.PREC_MIXED), case)
+    m2.set_option(A.OPT_BATCH_MODES, 0)
+    m2.set_option(A.OPT_GRAPHS, 0)
+    for p in (m1, m2):
+        p.field_compute(0, have_guess=False)
+    for rep in range(3):
+        for p in (m1, m2):
+            p.photoi_set_src()
+        for iv in [A.CC_PHOTO] + [A.CC_HELMH + j for j in range(3)]:
+            assert np.array_equal(m1.get_cc(iv), m2.get_cc(iv)), (rep, iv)
+
+
+def test_mixed_direct_solve_128_matrix_cores(A, S):
+    """3D, 16^3 boxes, levels 1-4 uniform (level 4 = 128^3 cells, solved
+    exactly by the inner cycle): the matrix-core and the vector transforms
+    reach the FP64 fixed point."""
+    case = setup(S.small_amr(3, nc=16, uniform_lvl=4, max_lvl=5, frac=0.03, seed=4))
+    f = S.apply(A.Path("gpu", case.D, case.nc, case.coord, A.PREC_FP64), case)
+    f.field_compute(0, False)
+    for _ in range(6):
+        f.mg_fas_vcycle(A.OP_POISSON, True, 0)
+    ref = leaf_interior(case, f, A.CC_PHI)
+    for mfma in (1, 0):
+        m = S.apply(A.Path("gpu", case.D, case.nc, case.coord, A.PREC_MIXED), case)
+        m.set_option(A.OPT_DIRECT_MFMA, mfma)
+        n0, r0 = m.field_compute(0, False)
+        for _ in range(4):
+            m.field_compute(0, True)
+        e = rel_max(leaf_interior(case, m, A.CC_PHI), ref)
+        print("mfma", mfma, "initial cycles", n0, "phi vs fp64 fixed point %.2e" % e, flush=True)
+        assert e < 1e-11
+
+
 def test_mixed_graph_replays_match_eager(A, S):
     case = setup(CASES["xyz3d_amr"](S), photoi=True)
     case.params.photoi_max_rel_residual = 1e-3
