@@ -273,8 +273,9 @@ __global__ void __launch_bounds__(256) k_lc_mfma(const double *__restrict__ in, 
                                                  const double *__restrict__ M, int axis, int div,
                                                  const double *__restrict__ eig, double lambda2) {
   static_assert(N == 64 || N == 128, "matrix-core transform for N = 64, 128");
-  constexpr int T = 64, KC = 16, AP = KC + 1, BP = T + 1, NT = N / T;
-  __shared__ double As[T * AP];  // As[p][kk]
+  constexpr int T = 64, KC = 32, AP = KC + 1, BP = T + 1, NT = N / T;
+  constexpr int PER = T * KC / 256;  // values of each operand tile per thread
+  __shared__ double As[T * AP];   // As[p][kk]
   __shared__ double Bs[KC * BP];  // Bs[kk][q]
   // blocks: ROWS: (output tiles) x (row tiles); else (output tiles) x
   // (column tiles) x (the remaining coordinate)
@@ -295,21 +296,40 @@ __global__ void __launch_bounds__(256) k_lc_mfma(const double *__restrict__ in, 
   }
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wp = (w >> 1) * 32, wq = (w & 1) * 32;
+  // one K chunk of both operands, global -> registers (consecutive threads
+  // take consecutive addresses) and registers -> LDS
+  double pa[PER], pb[PER];
+  auto fetch = [&](int k0) {
+    for (int u = 0; u < PER; u++) {
+      const int q = tid + 256 * u;
+      const int pp = q / KC, kk = q % KC;
+      pa[u] = ROWS ? in[(size_t)(p0 + pp) * N + k0 + kk] : M[(size_t)(p0 + pp) * N + k0 + kk];
+      if (ROWS) {
+        pb[u] = M[(size_t)(q0 + pp) * N + k0 + kk];
+      } else {
+        const int kb = q / T, qq = q % T;
+        pb[u] = in[base + (size_t)(k0 + kb) * sm + qq];
+      }
+    }
+  };
+  auto stash = [&]() {
+    for (int u = 0; u < PER; u++) {
+      const int q = tid + 256 * u;
+      As[(q / KC) * AP + q % KC] = pa[u];
+      if (ROWS)
+        Bs[(q % KC) * BP + q / KC] = pb[u];
+      else
+        Bs[(q / T) * BP + q % T] = pb[u];
+    }
+  };
   lc_double4 acc[2][2];
   for (int a = 0; a < 2; a++)
     for (int b = 0; b < 2; b++) acc[a][b] = lc_double4{0.0, 0.0, 0.0, 0.0};
+  fetch(0);
   for (int k0 = 0; k0 < N; k0 += KC) {
-    for (int q = tid; q < T * KC; q += 256) {
-      const int pp = q / KC, kk = q % KC;  // consecutive threads: consecutive k (contiguous in memory)
-      As[pp * AP + kk] = ROWS ? in[(size_t)(p0 + pp) * N + k0 + kk] : M[(size_t)(p0 + pp) * N + k0 + kk];
-      if (ROWS) Bs[kk * BP + pp] = M[(size_t)(q0 + pp) * N + k0 + kk];
-    }
-    if (!ROWS)
-      for (int q = tid; q < T * KC; q += 256) {
-        const int kk = q / T, qq = q % T;  // consecutive threads: consecutive columns
-        Bs[kk * BP + qq] = in[base + (size_t)(k0 + kk) * sm + qq];
-      }
+    stash();
     __syncthreads();
+    if (k0 + KC < N) fetch(k0 + KC);
     for (int ks = 0; ks < KC; ks += 4) {
       const int kk = ks + (lane >> 4);
       double a[2], b[2];
@@ -318,7 +338,8 @@ __global__ void __launch_bounds__(256) k_lc_mfma(const double *__restrict__ in, 
         b[t] = Bs[kk * BP + wq + 16 * t + (lane & 15)];
       }
       for (int ta = 0; ta < 2; ta++)
-        for (int tb = 0; tb < 2; tb++) acc[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta], b[tb], acc[ta][tb], 0, 0, 0);
+        for (int tb = 0; tb < 2; tb++)
+          acc[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta], b[tb], acc[ta][tb], 0, 0, 0);
     }
     __syncthreads();
   }

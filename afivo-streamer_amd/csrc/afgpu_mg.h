@@ -813,6 +813,81 @@ __device__ inline void lds_face_ghosts(T *s, const DBox *__restrict__ boxes, con
   }
 }
 
+// Staging of a box and its face ghosts into LDS in two memory rounds: the
+// interior and the same-level neighbour copies are loaded together (into
+// registers, then LDS); after a barrier, the ghosts that depend on the box's
+// own cells (physical boundary, refinement boundary) are formed from LDS.
+// colour >= 0: only the ghosts next to interior cells of that GSRB colour.
+// Ends with a barrier.
+template <int D, int NC, int BS, typename T>
+__device__ inline void lds_stage(T *s, const DBox *__restrict__ boxes, const DBox &bx, int id, const T *var,
+                                 const VarBc &m, int colour, const T *coarse = nullptr) {
+  using G = Geo<D, NC>;
+  constexpr int NI = (G::NINT + BS - 1) / BS;
+  constexpr int NFC = 2 * D * G::NTAN;
+  constexpr int NF = (NFC + BS - 1) / BS;
+  const T *u = var + (size_t)(id - 1) * G::NCELL;
+  T vi[NI], vf[NF];
+  int gi[NF];
+  for (int t = 0; t < NI; t++) {
+    const int q = threadIdx.x + t * BS;
+    if (q < G::NINT) {
+      int i, j, k;
+      interior_ijk<D, NC>(q, i, j, k);
+      vi[t] = u[G::idx(i, j, k)];
+    }
+  }
+  for (int t = 0; t < NF; t++) {
+    gi[t] = -1;
+    const int q = threadIdx.x + t * BS;
+    if (q >= NFC) continue;
+    const int nb = q / G::NTAN, r = q % G::NTAN;
+    const int nb_id = bx.nb[nb];
+    if (nb_id <= 0) continue;
+    const int a = r % NC + 1, b = (D == 3) ? r / NC + 1 : 0;
+    const bool low = (nb % 2) == 0;
+    int p[3];
+    face_pos<D>(nb, a, b, low ? 1 : NC, p);
+    if (colour >= 0 && ((p[0] + p[1] + p[2] + colour) & 1)) continue;
+    int d = nb / 2;
+    p[d] = low ? 0 : NC + 1;
+    gi[t] = G::idx(p[0], p[1], p[2]);
+    p[d] = low ? NC : 1;  // the neighbour's cell across the face
+    vf[t] = var[(size_t)(nb_id - 1) * G::NCELL + G::idx(p[0], p[1], p[2])];
+  }
+  for (int t = 0; t < NI; t++) {
+    const int q = threadIdx.x + t * BS;
+    if (q < G::NINT) {
+      int i, j, k;
+      interior_ijk<D, NC>(q, i, j, k);
+      s[G::idx(i, j, k)] = vi[t];
+    }
+  }
+  for (int t = 0; t < NF; t++)
+    if (gi[t] >= 0) s[gi[t]] = vf[t];
+  __syncthreads();
+  bool any = false;
+  for (int nb = 0; nb < 2 * D; nb++) any = any || bx.nb[nb] <= 0;
+  if (any) {
+    for (int q = threadIdx.x; q < NFC; q += BS) {
+      const int nb = q / G::NTAN, r = q % G::NTAN;
+      if (bx.nb[nb] > 0) continue;
+      const int a = r % NC + 1, b = (D == 3) ? r / NC + 1 : 0;
+      const bool low = (nb % 2) == 0;
+      int p[3];
+      face_pos<D>(nb, a, b, low ? 1 : NC, p);
+      if (colour >= 0 && ((p[0] + p[1] + p[2] + colour) & 1)) continue;
+      const T x1 = s[G::idx(p[0], p[1], p[2])];
+      face_pos<D>(nb, a, b, low ? 2 : NC - 1, p);
+      const T x2 = s[G::idx(p[0], p[1], p[2])];
+      const T g = ghost_value<D, NC, T>(boxes, bx, var, m, nb, a, b, x1, x2, coarse);
+      face_pos<D>(nb, a, b, low ? 0 : NC + 1, p);
+      s[G::idx(p[0], p[1], p[2])] = g;
+    }
+    __syncthreads();
+  }
+}
+
 // k_gsrb with the box staged in LDS (levels with one workgroup per box).
 template <int D, int NC, int BS, typename T>
 __global__ void __launch_bounds__(BS) k_gsrb_lds(const int *__restrict__ ids, const DBox *__restrict__ boxes, T *phi,
@@ -824,34 +899,49 @@ __global__ void __launch_bounds__(BS) k_gsrb_lds(const int *__restrict__ ids, co
   const Sten<D, T> st = lpl_stencil<D, T>(bx, lambda2);
   T *u = phi + (size_t)(id - 1) * G::NCELL;
   const T *r = rhs + (size_t)(id - 1) * G::NCELL;
-  lds_load_interior<D, NC, BS, T>(s, u);
-  __syncthreads();
-  // the updated cells satisfy (i + j + k + redblack) even
-  lds_face_ghosts<D, NC, BS, T>(s, boxes, bx, phi, m, redblack & 1);
-  __syncthreads();
   constexpr int HALF = G::NINT / 2;
+  constexpr int NH = (HALF + BS - 1) / BS;
   constexpr int sx = 1, sy = G::NG, sz = G::NG * G::NG;
-  for (int q = threadIdx.x; q < HALF; q += BS) {
+  // updated cell t of this thread, and its rhs (loaded with the box)
+  auto cell = [&](int q, int &i, int &j, int &k) {
     const int mm = q % (NC / 2);
-    const int j = (q / (NC / 2)) % NC + 1;
-    const int k = (D == 3) ? q / ((NC / 2) * NC) + 1 : 0;
+    j = (q / (NC / 2)) % NC + 1;
+    k = (D == 3) ? q / ((NC / 2) * NC) + 1 : 0;
     const int i0 = 2 - ((redblack ^ (D == 3 ? k + j : j)) & 1);
-    const int i = i0 + 2 * mm;
+    i = i0 + 2 * mm;
+  };
+  T rr[NH];
+  for (int t = 0; t < NH; t++) {
+    const int q = threadIdx.x + t * BS;
+    if (q < HALF) {
+      int i, j, k;
+      cell(q, i, j, k);
+      rr[t] = r[G::idx(i, j, k)];
+    }
+  }
+  // the updated cells satisfy (i + j + k + redblack) even
+  lds_stage<D, NC, BS, T>(s, boxes, bx, id, phi, m, redblack & 1);
+  for (int t = 0; t < NH; t++) {
+    const int q = threadIdx.x + t * BS;
+    if (q >= HALF) break;
+    int i, j, k;
+    cell(q, i, j, k);
     const int x = G::idx(i, j, k);
+    const T rx = rr[t];
     if (D == 2) {
       if (cyl) {
         T c1, c2, c3;
         cyl_coeffs<T>(bx, i, st.c, c1, c2, c3);
         T inv1 = T(1) / c1;
-        s[x] = (r[x] - c2 * s[x - sx] - c3 * s[x + sx] - st.c[3] * s[x - sy] - st.c[4] * s[x + sy]) * inv1;
+        s[x] = (rx - c2 * s[x - sx] - c3 * s[x + sx] - st.c[3] * s[x - sy] - st.c[4] * s[x + sy]) * inv1;
       } else {
         T inv_c1 = T(1) / st.c[0];
-        s[x] = (r[x] - st.c[1] * s[x - sx] - st.c[2] * s[x + sx] - st.c[3] * s[x - sy] - st.c[4] * s[x + sy]) *
+        s[x] = (rx - st.c[1] * s[x - sx] - st.c[2] * s[x + sx] - st.c[3] * s[x - sy] - st.c[4] * s[x + sy]) *
                inv_c1;
       }
     } else {
       T inv_c1 = T(1) / st.c[0];
-      s[x] = (r[x] - st.c[1] * s[x - sx] - st.c[2] * s[x + sx] - st.c[3] * s[x - sy] - st.c[4] * s[x + sy] -
+      s[x] = (rx - st.c[1] * s[x - sx] - st.c[2] * s[x + sx] - st.c[3] * s[x - sy] - st.c[4] * s[x + sy] -
               st.c[5] * s[x - sz] - st.c[6] * s[x + sz]) *
              inv_c1;
     }
@@ -882,17 +972,26 @@ __global__ void __launch_bounds__(BS) k_mp_residual_lds(const int *__restrict__ 
   const DBox &bx = boxes[id - 1];
   const Sten<D, double> st = lpl_stencil<D, double>(bx, lambda2);
   const size_t base = (size_t)(id - 1) * G::NCELL;
-  lds_load_interior<D, NC, BS, double>(s, phi + base);
-  __syncthreads();
-  lds_face_ghosts<D, NC, BS, double>(s, boxes, bx, phi, m, -1);
-  __syncthreads();
+  constexpr int NI = (G::NINT + BS - 1) / BS;
+  double rr[NI];
+  for (int t = 0; t < NI; t++) {
+    const int q = threadIdx.x + t * BS;
+    if (q < G::NINT) {
+      int i, j, k;
+      interior_ijk<D, NC>(q, i, j, k);
+      rr[t] = rhs[base + G::idx(i, j, k)];
+    }
+  }
+  lds_stage<D, NC, BS, double>(s, boxes, bx, id, phi, m, -1);
   double mx = 0;
-  for (int q = threadIdx.x; q < G::NINT; q += BS) {
+  for (int t = 0; t < NI; t++) {
+    const int q = threadIdx.x + t * BS;
+    if (q >= G::NINT) break;
     int i, j, k;
     interior_ijk<D, NC>(q, i, j, k);
     const int x = G::idx(i, j, k);
     const double lp = apply_cell<D, NC, double>(s, st, cyl != 0, bx, i, j, k);
-    const double rv = rhs[base + x] - lp;
+    const double rv = rr[t] - lp;
     r32[base + x] = (float)rv;
     mx = fmax(mx, fabs(rv));
   }
@@ -918,10 +1017,7 @@ __global__ void __launch_bounds__(BS) k_restrict_residual_lds(const int *__restr
   const DBox &bp = boxes[bc.parent - 1];
   const Sten<D, T> st = lpl_stencil<D, T>(bc, lambda2);
   const size_t cb = (size_t)(id - 1) * G::NCELL, pb = (size_t)(bc.parent - 1) * G::NCELL;
-  lds_load_interior<D, NC, BS, T>(s, child + cb);
-  __syncthreads();
-  lds_face_ghosts<D, NC, BS, T>(s, boxes, bc, child, m, -1, (const T *)phi);
-  __syncthreads();
+  lds_stage<D, NC, BS, T>(s, boxes, bc, id, child, m, -1, (const T *)phi);
   int off[3] = {0, 0, 0};
   for (int d = 0; d < D; d++) off[d] = ((bc.ix[d] - 1) & 1) * H;
   for (int q = threadIdx.x; q < NQ; q += BS) {
