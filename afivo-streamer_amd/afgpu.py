@@ -42,6 +42,7 @@ OPT_MIXED_SMOOTH = 3
 OPT_BLOCK_SMOOTH = 4
 OPT_BATCH_MODES = 5
 OPT_DIRECT_MFMA = 6
+OPT_PHOTOI_VCYCLE = 7
 
 
 class ParamsT(C.Structure):

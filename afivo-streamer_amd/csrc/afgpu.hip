@@ -137,6 +137,7 @@ struct afgpu_ctx {
   int mixed_smooth = 0;  // > 0: smoothing sweeps of the inner cycle (down = up)
   int block_smooth = 0;  // inner cycle: block smoother on levels with one workgroup per box
   bool direct_mfma = true;  // uniform-level transforms on the matrix cores
+  bool photoi_inner_vcycle = false;  // photoionization inner cycle: V-cycle instead of FMG
   struct Direct {
     bool set = false;  // lc and the map below are current for the tree
     int lc = 1, N = 0;
@@ -345,6 +346,7 @@ struct Impl : ImplBase {
   int gc_var(int lvl, T *var, const VarBc &m, bool corners) {
     const List &l = c.l_ids[lvl];
     const int k = corners ? 1 : bpb(l.n, 2 * D * G::NTAN);
+    if (k == 1) return launch(k_gc_lds<D, NC, BS, T>, l.n, c.list(l), (const DBox *)c.d_boxes, var, m, (int)corners);
     return launch(k_gc<D, NC, BS, T>, l.n * k, c.list(l), (const DBox *)c.d_boxes, var, m, (int)corners, k);
   }
   int gc_lvl(int lvl, int iv, bool corners) override { return gc_var(lvl, c.cc[iv], var_methods(c, iv), corners); }
@@ -1100,7 +1102,10 @@ struct Impl : ImplBase {
       for (int i = 1; i <= c.p.photoi_max_fmg_cycles && active; i++) {
         RCHK(run_graph(c, {G_MP_FMG_MODES, active, lc, 0}, [&] {
           return fork_modes(active, [&](int op) {
-            RCHK(fmg_body(mg32(op), false, false, lc));
+            if (c.photoi_inner_vcycle)
+              RCHK(vcycle_body(mg32(op), false, c.highest_lvl, lc));
+            else
+              RCHK(fmg_body(mg32(op), false, false, lc));
             return mp_close(op, true, lc);
           });
         }));
@@ -1786,6 +1791,11 @@ int afgpu_set_option(afgpu_t *h, int option, int value) {
       h->direct_max_n = value;
       h->dir.set = false;
       for (auto &f : h->dir.f) f.ok = false;
+      return AFGPU_OK;
+    case AFGPU_OPT_PHOTOI_VCYCLE:
+      RCHK(sync_stream(h));
+      drop_graphs(*h);
+      h->photoi_inner_vcycle = value != 0;
       return AFGPU_OK;
     case AFGPU_OPT_DIRECT_MFMA:
       RCHK(sync_stream(h));

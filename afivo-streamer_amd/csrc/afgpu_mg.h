@@ -1161,4 +1161,76 @@ __global__ void __launch_bounds__(BS) k_mp_correct_leaves(const int *__restrict_
   }
 }
 
+// af_gc_box with the box staged in LDS: face ghosts as lds_stage forms them,
+// then (with corners) the edge and corner ghosts of gc_edge_cell /
+// gc_corner_cell from the LDS values, then only the ghost layer is written.
+// One workgroup per box.
+template <int D, int NC, int BS, typename T>
+__global__ void __launch_bounds__(BS) k_gc_lds(const int *__restrict__ ids, const DBox *__restrict__ boxes, T *var,
+                                               VarBc m, int corners) {
+  using G = Geo<D, NC>;
+  __shared__ T s[G::NCELL];
+  const int id = ids[blockIdx.x];
+  const DBox &bx = boxes[id - 1];
+  T *u = var + (size_t)(id - 1) * G::NCELL;
+  lds_stage<D, NC, BS, T>(s, boxes, bx, id, var, m, -1);
+  if (corners) {
+    if (D == 3) {
+      for (int q = threadIdx.x; q < 12 * NC; q += BS) {
+        const int n = q / NC, qq = q % NC + 1;
+        const int dim = c_edge_dim3[n];
+        const int *dir = c_edge_dir3[n];
+        int lo[3];
+        for (int dd = 0; dd < 3; dd++) lo[dd] = (dir[dd] > 0) ? NC + 1 : 0;
+        lo[dim] = qq;
+        const int nb_id = bx.nbmat[(dir[0] + 1) + 3 * (dir[1] + 1) + 9 * (dir[2] + 1)];
+        T v;
+        if (nb_id > 0) {
+          const T *w = var + (size_t)(nb_id - 1) * G::NCELL;
+          v = w[G::idx(lo[0] - dir[0] * NC, lo[1] - dir[1] * NC, lo[2] - dir[2] * NC)];
+        } else {
+          const int o0 = (dim + 1) % 3, o1 = (dim + 2) % 3;
+          int di[3];
+          for (int dd = 0; dd < 3; dd++) di[dd] = 1 - 2 * (lo[dd] & 1);
+          di[dim] = 0;
+          int ia[3] = {lo[0], lo[1], lo[2]}, ib[3] = {lo[0], lo[1], lo[2]};
+          ia[o0] += di[o0];
+          ib[o1] += di[o1];
+          v = s[G::idx(ia[0], ia[1], ia[2])] + s[G::idx(ib[0], ib[1], ib[2])] -
+              s[G::idx(lo[0] + di[0], lo[1] + di[1], lo[2] + di[2])];
+        }
+        s[G::idx(lo[0], lo[1], lo[2])] = v;
+      }
+      __syncthreads();
+    }
+    for (int n = threadIdx.x; n < (1 << D); n += BS) {
+      const int cx = n & 1, cy = (n >> 1) & 1, cz = (n >> 2) & 1;
+      const int dnb[3] = {2 * cx - 1, 2 * cy - 1, (D == 3) ? 2 * cz - 1 : 0};
+      const int lo[3] = {cx * (NC + 1), cy * (NC + 1), (D == 3) ? cz * (NC + 1) : 0};
+      const int nb_id = bx.nbmat[(dnb[0] + 1) + 3 * (dnb[1] + 1) + 9 * (dnb[2] + 1)];
+      T v;
+      if (nb_id > 0) {
+        const T *w = var + (size_t)(nb_id - 1) * G::NCELL;
+        v = w[G::idx(lo[0] - dnb[0] * NC, lo[1] - dnb[1] * NC, lo[2] - dnb[2] * NC)];
+      } else if (D == 2) {
+        const int di = 1 - 2 * (lo[0] & 1), dj = 1 - 2 * (lo[1] & 1);
+        v = s[G::idx(lo[0] + di, lo[1], 0)] + s[G::idx(lo[0], lo[1] + dj, 0)] - s[G::idx(lo[0] + di, lo[1] + dj, 0)];
+      } else {
+        const int di = 1 - 2 * (lo[0] & 1), dj = 1 - 2 * (lo[1] & 1), dk = 1 - 2 * (lo[2] & 1);
+        const int x = lo[0], y = lo[1], z = lo[2];
+        v = s[G::idx(x, y + dj, z + dk)] + s[G::idx(x + di, y, z + dk)] + s[G::idx(x + di, y + dj, z)] -
+            2 * s[G::idx(x + di, y + dj, z + dk)];
+      }
+      s[G::idx(lo[0], lo[1], lo[2])] = v;
+    }
+    __syncthreads();
+  }
+  // the ghost layer back: faces, and with corners the edges and corners
+  for (int x = threadIdx.x; x < G::NCELL; x += BS) {
+    const int i = x % G::NG, j = (x / G::NG) % G::NG, k = (D == 3) ? x / (G::NG * G::NG) : 0;
+    const int out = (i == 0 || i == NC + 1) + (j == 0 || j == NC + 1) + (D == 3 && (k == 0 || k == NC + 1));
+    if (out == 1 || (corners && out > 1)) u[x] = s[x];
+  }
+}
+
 }  // namespace afg

@@ -51,7 +51,8 @@ def make_case(args):
     return case
 
 
-def run_gpu(case, precision, warmup, steps, timing_steps, direct_n=None, smooth=None, block=None, batch=None):
+def run_gpu(case, precision, warmup, steps, timing_steps, direct_n=None, smooth=None, block=None, batch=None,
+            photoi_vcycle=None):
     p = A.Path("gpu", case.D, case.nc, case.coord, precision)
     S.apply(p, case)
     if direct_n is not None:
@@ -62,6 +63,8 @@ def run_gpu(case, precision, warmup, steps, timing_steps, direct_n=None, smooth=
         p.set_option(A.OPT_BLOCK_SMOOTH, block)
     if batch is not None:
         p.set_option(A.OPT_BATCH_MODES, batch)
+    if photoi_vcycle is not None:
+        p.set_option(A.OPT_PHOTOI_VCYCLE, photoi_vcycle)
     t0 = time.time()
     p.field_compute(0, have_guess=False)
     log("  initial field solve %.2f s" % (time.time() - t0))
@@ -147,6 +150,8 @@ def main():
                     help="mixed precision: block smoother in the inner cycle (1) or red-black sweeps (0)")
     ap.add_argument("--batch-modes", type=int, default=1,
                     help="mixed precision: photoionization modes as parallel graph branches (1) or in turn (0)")
+    ap.add_argument("--photoi-vcycle", type=int, default=0,
+                    help="mixed precision: photoionization inner cycle V-cycle (1) or FMG (0)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB a few times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes) and exit")
@@ -168,7 +173,8 @@ def main():
     mixed = prec == A.PREC_MIXED
     p, infos, secs, stats = run_gpu(case, prec, args.warmup, args.steps, args.timing_steps,
                                     args.direct_n if mixed else None, args.mixed_smooth if mixed else None,
-                                    args.block_smooth if mixed else None, args.batch_modes if mixed else None)
+                                    args.block_smooth if mixed else None, args.batch_modes if mixed else None,
+                                    args.photoi_vcycle if mixed else None)
     cell_steps = sum(i["leaf_cells"] for i in infos)
     value = cell_steps / secs
     nv = [i["n_vcycles"] for i in infos]
@@ -257,7 +263,8 @@ def main():
                    "direct_n": args.direct_n if prec == A.PREC_MIXED else None,
                    "mixed_smooth": args.mixed_smooth if prec == A.PREC_MIXED else None,
                    "block_smooth": args.block_smooth if prec == A.PREC_MIXED else None,
-                   "batch_modes": args.batch_modes if prec == A.PREC_MIXED else None},
+                   "batch_modes": args.batch_modes if prec == A.PREC_MIXED else None,
+                   "photoi_vcycle": args.photoi_vcycle if prec == A.PREC_MIXED else None},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -273,6 +273,9 @@ int afgpu_kernel_stats(afgpu_t *h, int kind, int reset, double *total_ms,
 #define AFGPU_OPT_DIRECT_MFMA 6 /* mixed precision: uniform-level transforms (64 or 128 cells
                                    per axis) on the FP64 matrix cores (1) or as vector FMA
                                    tiles (0) (default 1) */
+#define AFGPU_OPT_PHOTOI_VCYCLE 7 /* mixed precision: the photoionization modes' inner cycle is
+                                     one V-cycle (1) or one FMG cycle (0) per outer iteration
+                                     (default 0) */
 int afgpu_set_option(afgpu_t *h, int option, int value);
 
 #ifdef __cplusplus
