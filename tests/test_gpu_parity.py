@@ -106,3 +106,44 @@ def test_graph_replays_match_eager(A, S):
         assert a == b
         for iv in (A.CC_ELECTRON, A.CC_PHI, A.CC_PHOTO, A.CC_TMP):
             assert np.array_equal(g1.get_cc(iv), g2.get_cc(iv)), (it, iv)
+
+
+def test_heun_steps_parity(A, S):
+    """Heun's method (the reference's default integrator, m_dt.f90:80): two
+    forward-Euler stages with the field recomputed in between."""
+    case = S.small_amr(3, nc=8, uniform_lvl=2, max_lvl=3, frac=0.3, seed=11, integrator=A.HEUNS_METHOD)
+    g, c = make_pair(A, S, case)
+    assert g.field_compute(0, False) == c.field_compute(0, False)
+    for it in range(4):
+        ig, ic = g.step(), c.step()
+        assert ig == ic, (ig, ic)
+        compare(A, g, c, [A.CC_ELECTRON, A.CC_POS_ION, A.CC_PHI, A.CC_EFIELD], what="heun step %d" % it)
+
+
+def test_config0_full_size_parity(A, S):
+    """The 2D axisymmetric workload at its full size (uniform 256 x 256 grid in
+    1024 boxes of 8 x 8, 20 kV, Poisson to 1e-10 of max |rhs|): several steps
+    bit for bit."""
+    case = S.config0(uniform_lvl=6, num_vcycles=100, rel_residual=1e-10)
+    g, c = make_pair(A, S, case)
+    assert g.field_compute(0, False) == c.field_compute(0, False)
+    for it in range(8):
+        ig, ic = g.step(), c.step()
+        print(it, ig["n_vcycles"], ig["dt_used"], flush=True)
+        assert ig == ic, (ig, ic)
+    compare(A, g, c, [A.CC_ELECTRON, A.CC_POS_ION, A.CC_PHI, A.CC_EFIELD], what="config0")
+
+
+def test_config1_full_size_parity(A, S):
+    """The 3D AMR workload at its full size (2121 boxes of 16^3, 7,602,176 leaf
+    cells, 15 kV, Poisson to 1e-8): the initial solve and two steps bit for
+    bit against the CPU oracle."""
+    case = S.config1(num_vcycles=100, rel_residual=1e-8)
+    g, c = make_pair(A, S, case)
+    assert g.field_compute(0, False) == c.field_compute(0, False)
+    compare(A, g, c, [A.CC_PHI, A.CC_EFIELD], what="config1 field")
+    for it in range(2):
+        ig, ic = g.step(), c.step()
+        print(it, ig["n_vcycles"], ig["dt_used"], flush=True)
+        assert ig == ic, (ig, ic)
+    compare(A, g, c, [A.CC_ELECTRON, A.CC_POS_ION, A.CC_NEG_ION, A.CC_PHI, A.CC_EFIELD], what="config1 steps")
