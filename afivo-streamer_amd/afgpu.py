@@ -43,6 +43,7 @@ OPT_BLOCK_SMOOTH = 4
 OPT_BATCH_MODES = 5
 OPT_DIRECT_MFMA = 6
 OPT_PHOTOI_VCYCLE = 7
+OPT_FLUX32 = 8
 
 
 class ParamsT(C.Structure):

@@ -138,6 +138,7 @@ struct afgpu_ctx {
   int block_smooth = 0;  // inner cycle: block smoother on levels with one workgroup per box
   bool direct_mfma = true;  // uniform-level transforms on the matrix cores
   bool photoi_inner_vcycle = false;  // photoionization inner cycle: V-cycle instead of FMG
+  bool flux32 = true;  // mixed precision: FP32 flux stencil (FP64 accumulation)
   struct Direct {
     bool set = false;  // lc and the map below are current for the tree
     int lc = 1, N = 0;
@@ -968,9 +969,14 @@ struct Impl : ImplBase {
     for (int lvl = 1; lvl <= c.highest_lvl; lvl++) {
       const List &l = c.l_leaves[lvl];
       RCHK(timer_start(c, 1, (2.0 + 2 * D) * 8.0 * l.n * G::NINT));
-      RCHK(launch(k_flux<D, NC, BS>, l.n, c.list(l), (const DBox *)c.d_boxes, c.cc[iv_e],
-                  (const double *)c.cc[AFGPU_CC_EFIELD], (const double *)c.fc[AFGPU_FC_FIELD], c.fc[AFGPU_FC_FLUX],
-                  msp, c.td, N_inv, c.d_red + 1, c.d_red + 2));
+      if (c.precision == AFGPU_PREC_MIXED && c.flux32)
+        RCHK(launch(k_flux<D, NC, BS, float>, l.n, c.list(l), (const DBox *)c.d_boxes, c.cc[iv_e],
+                    (const double *)c.cc[AFGPU_CC_EFIELD], (const double *)c.fc[AFGPU_FC_FIELD],
+                    c.fc[AFGPU_FC_FLUX], msp, c.td, N_inv, c.d_red + 1, c.d_red + 2));
+      else
+        RCHK(launch(k_flux<D, NC, BS, double>, l.n, c.list(l), (const DBox *)c.d_boxes, c.cc[iv_e],
+                    (const double *)c.cc[AFGPU_CC_EFIELD], (const double *)c.fc[AFGPU_FC_FIELD],
+                    c.fc[AFGPU_FC_FLUX], msp, c.td, N_inv, c.d_red + 1, c.d_red + 2));
       RCHK(timer_stop(c, 1));
     }
     for (int lvl = 1; lvl <= c.highest_lvl - 1; lvl++) {
@@ -1791,6 +1797,11 @@ int afgpu_set_option(afgpu_t *h, int option, int value) {
       h->direct_max_n = value;
       h->dir.set = false;
       for (auto &f : h->dir.f) f.ok = false;
+      return AFGPU_OK;
+    case AFGPU_OPT_FLUX32:
+      RCHK(sync_stream(h));
+      drop_graphs(*h);
+      h->flux32 = value != 0;
       return AFGPU_OK;
     case AFGPU_OPT_PHOTOI_VCYCLE:
       RCHK(sync_stream(h));

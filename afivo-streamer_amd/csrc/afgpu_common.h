@@ -90,6 +90,26 @@ __device__ inline double lt_at(const DTable &t, int c, int ix, double w) {
   const double *y = t.cols + (size_t)c * t.n;
   return w * y[ix - 1] + (1 - w) * y[ix];
 }
+// The same in the arithmetic type TF (identical to the above for double).
+template <typename TF>
+__device__ inline void lt_loc_t(const DTable &t, TF x, int &ix, TF &w) {
+  TF frac = (x - (TF)t.x_min) * (TF)t.inv_fac;
+  if (frac <= 0) {
+    ix = 1;
+    w = 1;
+  } else if (frac >= t.n - 1) {
+    ix = t.n - 1;
+    w = 0;
+  } else {
+    ix = (int)ceil(frac);
+    w = ix - frac;
+  }
+}
+template <typename TF>
+__device__ inline TF lt_at_t(const DTable &t, int c, int ix, TF w) {
+  const double *y = t.cols + (size_t)c * t.n;
+  return w * (TF)y[ix - 1] + (1 - w) * (TF)y[ix];
+}
 
 // Limiters (afivo/src/m_af_limiters.f90).
 template <typename T>
@@ -108,6 +128,16 @@ __device__ inline T lim_apply(T a, T b, int lim) {
 template <typename T>
 __device__ inline T lim_koren(T a, T b) {
   const T third = T(1 / 3.0);
+  if (sizeof(T) < sizeof(double)) {
+    // single precision: densities of 1e20 square past the float range, so
+    // the same tests are made on magnitudes (a^2 <= c ab with ab > 0 is
+    // |a| <= c |b|)
+    if (!((a > 0 && b > 0) || (a < 0 && b < 0))) return T(0);
+    const T fa = fabs(a), fb = fabs(b);
+    if (fa <= T(0.25) * fb) return 2 * a;
+    if (fa <= T(2.5) * fb) return third * (b + 2 * a);
+    return 2 * b;
+  }
   T aa = a * a, ab = a * b;
   if (ab <= 0) return T(0);
   if (aa <= T(0.25) * ab) return 2 * a;

@@ -5,6 +5,8 @@
 //   field_set_rhs, mg_compute_phi_gradient, photoi source/accumulation.
 #pragma once
 
+#include <limits>
+
 #include "afgpu_mg.h"
 
 namespace afg {
@@ -85,50 +87,57 @@ __device__ inline double gc2_value(const DBox *__restrict__ boxes, int id, const
 // Launched per refinement level, lowest level first (the reference's level
 // loop makes a coarse leaf's written-back ghost layer visible to the finer
 // level's halo, m_af_flux_schemes.f90:696-707).
-template <int D, int NC, int BS>
+// TF: arithmetic type of the stencil (double: the reference's arithmetic;
+// float: the mixed-precision form's FP32 flux, accumulated in FP64 by
+// k_update).  The n_e halo is formed in FP64 and its first layer written
+// back at FP64 precision in either case.
+template <int D, int NC, int BS, typename TF>
 __global__ void __launch_bounds__(BS)
     k_flux(const int *__restrict__ ids, const DBox *__restrict__ boxes, double *ne, const double *__restrict__ efield,
-           const double *__restrict__ efc, double *flux, VarBc m, DTable td, double N_inv,
+           const double *__restrict__ efc, double *flux, VarBc m, DTable td, double N_inv_d,
            unsigned long long *cfl_max, unsigned long long *sig_max) {
   using G = Geo<D, NC>;
   constexpr int NT = G::NTAN;
-  __shared__ double halo[2 * D][2][NT];
+  __shared__ TF halo[2 * D][2][NT];
   __shared__ double sh[BS / 64];
   const int id = ids[blockIdx.x];
   const DBox &bx = boxes[id - 1];
   const size_t base = (size_t)(id - 1) * G::NCELL;
   const size_t fbase = (size_t)(id - 1) * D * G::NFACE;
+  const TF N_inv = (TF)N_inv_d;
 
+  // the halo; its first layer also goes back into the box
+  // (m_af_ghostcell.f90:740-742): no thread of any box reads a ghost cell
+  // of this level during the launch
   for (int q = threadIdx.x; q < 2 * D * 2 * NT; q += BS) {
     const int nb = q / (2 * NT), layer = (q / NT) % 2, r = q % NT;
     const int a = r % NC + 1, b = (D == 3) ? r / NC + 1 : 0;
-    halo[nb][layer][r] = gc2_value<D, NC>(boxes, id, ne, m, nb, layer, a, b);
+    const double hv = gc2_value<D, NC>(boxes, id, ne, m, nb, layer, a, b);
+    halo[nb][layer][r] = (TF)hv;
+    if (layer == 0) {
+      const int d = nb / 2;
+      int t0, t1;
+      tangential(D, d, t0, t1);
+      int p[3] = {0, 0, 0};
+      p[t0] = a;
+      if (D == 3) p[t1] = b;
+      p[d] = (nb % 2) ? NC + 1 : 0;
+      ne[base + G::idx(p[0], p[1], p[2])] = hv;
+    }
   }
   __syncthreads();
-  // write the first layer back into the box (m_af_ghostcell.f90:740-742)
-  for (int q = threadIdx.x; q < 2 * D * NT; q += BS) {
-    const int nb = q / NT, r = q % NT;
-    const int d = nb / 2;
-    int t0, t1;
-    tangential(D, d, t0, t1);
-    int p[3] = {0, 0, 0};
-    p[t0] = r % NC + 1;
-    if (D == 3) p[t1] = r / NC + 1;
-    p[d] = (nb % 2) ? NC + 1 : 0;
-    ne[base + G::idx(p[0], p[1], p[2])] = halo[nb][0][r];
-  }
 
-  auto line = [&](int d, int r, const int *p, int q) -> double {
+  auto line = [&](int d, int r, const int *p, int q) -> TF {
     if (q >= 1 && q <= NC) {
       int pp[3] = {p[0], p[1], p[2]};
       pp[d] = q;
-      return ne[base + G::idx(pp[0], pp[1], pp[2])];
+      return (TF)ne[base + G::idx(pp[0], pp[1], pp[2])];
     }
     if (q <= 0) return halo[2 * d][-q][r];
     return halo[2 * d + 1][q - NC - 1][r];
   };
 
-  double smax = -1e300;
+  TF smax = -std::numeric_limits<TF>::max();
   // face fluxes (flux_upwind, src/m_fluid.f90:102-209)
   for (int q = threadIdx.x; q < D * (NC + 1) * NT; q += BS) {
     const int d = q / ((NC + 1) * NT);
@@ -142,72 +151,72 @@ __global__ void __launch_bounds__(BS)
     int p[3] = {0, 0, 0};
     p[t0] = r % NC + 1;
     if (D == 3) p[t1] = r / NC + 1;
-    const double inv_dx = 1 / bx.dr[d];
+    const TF inv_dx = TF(1) / (TF)bx.dr[d];
     int fp[3] = {p[0], p[1], (D == 3) ? p[2] : 1};
     fp[d] = f;
-    const double ex = efc[fbase + G::fidx(fp[0], fp[1], fp[2], d)];
-    const double Lm2 = line(d, r, p, f - 2), Lm1 = line(d, r, p, f - 1), L0 = line(d, r, p, f),
-                 Lp1 = line(d, r, p, f + 1);
-    double u;
+    const TF ex = (TF)efc[fbase + G::fidx(fp[0], fp[1], fp[2], d)];
+    const TF Lm2 = line(d, r, p, f - 2), Lm1 = line(d, r, p, f - 1), L0 = line(d, r, p, f),
+             Lp1 = line(d, r, p, f + 1);
+    TF u;
     if (-1 * ex > 0) {
-      u = Lm1 + 0.5 * lim_koren<double>(L0 - Lm1, Lm1 - Lm2);
+      u = Lm1 + TF(0.5) * lim_koren<TF>(L0 - Lm1, Lm1 - Lm2);
     } else {
-      u = L0 - 0.5 * lim_koren<double>(L0 - Lm1, Lp1 - L0);
+      u = L0 - TF(0.5) * lim_koren<TF>(L0 - Lm1, Lp1 - L0);
     }
     int e0[3] = {p[0], p[1], p[2]}, e1[3] = {p[0], p[1], p[2]};
     e0[d] = f - 1;
     e1[d] = f;
-    const double tmp_fc = 0.5 * (efield[base + G::idx(e0[0], e0[1], e0[2])] +
-                                 efield[base + G::idx(e1[0], e1[1], e1[2])]) *
-                          SI_to_Townsend * N_inv;
+    const TF tmp_fc = TF(0.5) * ((TF)efield[base + G::idx(e0[0], e0[1], e0[2])] +
+                                 (TF)efield[base + G::idx(e1[0], e1[1], e1[2])]) *
+                      (TF)SI_to_Townsend * N_inv;
     int lix;
-    double lw;
-    lt_loc(td, tmp_fc, lix, lw);
-    const double mu = lt_at(td, 0, lix, lw) * N_inv;
-    const double dc = lt_at(td, 1, lix, lw) * N_inv;
-    const double v = -mu * ex;
-    flux[fbase + G::fidx(fp[0], fp[1], fp[2], d)] = v * u - dc * inv_dx * (L0 - Lm1);
+    TF lw;
+    lt_loc_t<TF>(td, tmp_fc, lix, lw);
+    const TF mu = lt_at_t<TF>(td, 0, lix, lw) * N_inv;
+    const TF dc = lt_at_t<TF>(td, 1, lix, lw) * N_inv;
+    const TF v = -mu * ex;
+    flux[fbase + G::fidx(fp[0], fp[1], fp[2], d)] = (double)(v * u - dc * inv_dx * (L0 - Lm1));
     smax = fmax(smax, mu * u);
   }
-  smax = block_max<BS>(smax, sh);
-  if (threadIdx.x == 0) atomic_max_nonneg(sig_max, fmax(smax, 1e-100));
+  double smax_d = block_max<BS>((double)smax, sh);
+  if (threadIdx.x == 0) atomic_max_nonneg(sig_max, fmax(smax_d, 1e-100));
 
   // CFL sum per cell (m_fluid.f90:194-195 summed over dimensions,
   // m_af_flux_schemes.f90:779-848)
-  double cmax = 0;
+  TF cmax = 0;
   for (int q = threadIdx.x; q < G::NINT; q += BS) {
     int i, j, k;
     interior_ijk<D, NC>(q, i, j, k);
-    double cfl = 0.0;
+    TF cfl = 0;
     for (int d = 0; d < D; d++) {
-      const double inv_dx = 1 / bx.dr[d];
-      double vv[2], dd[2];
+      const TF inv_dx = TF(1) / (TF)bx.dr[d];
+      TF vv[2], dd[2];
       for (int s = 0; s < 2; s++) {
         int c[3] = {i, j, k};
         int fp[3] = {i, j, (D == 3) ? k : 1};
         fp[d] += s;  // faces c and c+1
-        const double ex = efc[fbase + G::fidx(fp[0], fp[1], fp[2], d)];
+        const TF ex = (TF)efc[fbase + G::fidx(fp[0], fp[1], fp[2], d)];
         int e0[3] = {c[0], c[1], c[2]};
         e0[d] = c[d] - 1 + s;
         int e1[3] = {c[0], c[1], c[2]};
         e1[d] = c[d] + s;
-        const double tmp_fc = 0.5 * (efield[base + G::idx(e0[0], e0[1], e0[2])] +
-                                     efield[base + G::idx(e1[0], e1[1], e1[2])]) *
-                              SI_to_Townsend * N_inv;
+        const TF tmp_fc = TF(0.5) * ((TF)efield[base + G::idx(e0[0], e0[1], e0[2])] +
+                                     (TF)efield[base + G::idx(e1[0], e1[1], e1[2])]) *
+                          (TF)SI_to_Townsend * N_inv;
         int lix;
-        double lw;
-        lt_loc(td, tmp_fc, lix, lw);
-        const double mu = lt_at(td, 0, lix, lw) * N_inv;
-        dd[s] = lt_at(td, 1, lix, lw) * N_inv;
+        TF lw;
+        lt_loc_t<TF>(td, tmp_fc, lix, lw);
+        const TF mu = lt_at_t<TF>(td, 0, lix, lw) * N_inv;
+        dd[s] = lt_at_t<TF>(td, 1, lix, lw) * N_inv;
         vv[s] = -mu * ex;
       }
-      const double cl = 1.0 * fmax(fabs(vv[1]), fabs(vv[0])) * inv_dx + 2 * fmax(dd[1], dd[0]) * (inv_dx * inv_dx);
+      const TF cl = TF(1) * fmax(fabs(vv[1]), fabs(vv[0])) * inv_dx + TF(2) * fmax(dd[1], dd[0]) * (inv_dx * inv_dx);
       cfl = cfl + cl;
     }
     cmax = fmax(cmax, cfl);
   }
-  cmax = block_max<BS>(cmax, sh);
-  if (threadIdx.x == 0) atomic_max_nonneg(cfl_max, cmax);
+  const double cmax_d = block_max<BS>((double)cmax, sh);
+  if (threadIdx.x == 0) atomic_max_nonneg(cfl_max, cmax_d);
 }
 
 // Coarse-leaf faces next to a refined neighbour get the (cylindrically

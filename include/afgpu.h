@@ -276,6 +276,8 @@ int afgpu_kernel_stats(afgpu_t *h, int kind, int reset, double *total_ms,
 #define AFGPU_OPT_PHOTOI_VCYCLE 7 /* mixed precision: the photoionization modes' inner cycle is
                                      one V-cycle (1) or one FMG cycle (0) per outer iteration
                                      (default 0) */
+#define AFGPU_OPT_FLUX32 8 /* mixed precision: the drift-diffusion flux stencil in FP32, the
+                              density update accumulating in FP64 (default 1) */
 int afgpu_set_option(afgpu_t *h, int option, int value);
 
 #ifdef __cplusplus
