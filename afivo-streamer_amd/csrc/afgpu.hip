@@ -7,6 +7,7 @@
 // work is enqueued on one HIP stream per handle; the host reads back only
 // the scalars the reference's control flow branches on (max |residual|,
 // time-step limits).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -65,6 +66,7 @@ struct TPair {
   hipEvent_t a, b;
   int kind;
   double bytes;
+  int launches;  // back-to-back launches between the two events
 };
 // A captured launch sequence and the timing pairs it contains.
 struct GraphRec {
@@ -226,9 +228,9 @@ constexpr int RED_MODE = 8;
 
 int op_phi(int op) { return op == AFGPU_OP_POISSON ? AFGPU_CC_PHI : AFGPU_CC_HELMH + (op - AFGPU_OP_HELMH); }
 
-int timer_start(afgpu_ctx &c, int kind, double bytes) {
+int timer_start(afgpu_ctx &c, int kind, double bytes, int launches = 1) {
   if (!c.timing) return AFGPU_OK;
-  TPair tp{nullptr, nullptr, kind, bytes};
+  TPair tp{nullptr, nullptr, kind, bytes, launches};
   if (c.capture_pairs) {  // events owned by the graph being captured
     HIPCHK(hipEventCreate(&tp.a));
     HIPCHK(hipEventCreate(&tp.b));
@@ -247,6 +249,21 @@ int timer_start(afgpu_ctx &c, int kind, double bytes) {
   HIPCHK(hipEventRecord(tp.a, c.cur));
   return AFGPU_OK;
 }
+// A pair of events that the kernel dispatch itself records
+// (hipExtLaunchKernel): they bracket exactly the kernel's execution.
+int timer_events(afgpu_ctx &c, int kind, double bytes, hipEvent_t *a, hipEvent_t *b) {
+  while (c.ev_pool.size() < c.ev_used + 2) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    c.ev_pool.push_back(e);
+  }
+  TPair tp{c.ev_pool[c.ev_used], c.ev_pool[c.ev_used + 1], kind, bytes, 1};
+  c.ev_used += 2;
+  c.pending.push_back(tp);
+  *a = tp.a;
+  *b = tp.b;
+  return AFGPU_OK;
+}
 int timer_stop(afgpu_ctx &c, int kind) {
   if (!c.timing) return AFGPU_OK;
   const TPair &tp = c.capture_pairs ? c.capture_pairs->back() : c.pending.back();
@@ -262,7 +279,7 @@ int timer_harvest(afgpu_ctx &c) {
     KTimer &t = c.timers[tp.kind];
     t.total_ms += ms;
     t.algo_bytes += tp.bytes;
-    t.launches++;
+    t.launches += tp.launches;
   }
   c.pending.clear();
   c.ev_used = 0;
@@ -324,6 +341,18 @@ struct Impl : ImplBase {
   int launch(K kernel, int nblocks, Args... args) {
     if (nblocks <= 0) return AFGPU_OK;
     hipLaunchKernelGGL(kernel, dim3(nblocks), dim3(BS), 0, c.cur, args...);
+    HIPCHK(hipGetLastError());
+    return AFGPU_OK;
+  }
+  // launch() with the kernel's own execution timed under class `kind`
+  // when kernel timing is on (eager runs only)
+  template <typename K, typename... Args>
+  int launch_timed(int kind, double bytes, K kernel, int nblocks, Args... args) {
+    if (!c.timing || c.capture_pairs) return launch(kernel, nblocks, args...);
+    if (nblocks <= 0) return AFGPU_OK;
+    hipEvent_t a, b;
+    RCHK(timer_events(c, kind, bytes, &a, &b));
+    hipExtLaunchKernelGGL(kernel, dim3(nblocks), dim3(BS), 0, c.cur, a, b, 0, args...);
     HIPCHK(hipGetLastError());
     return AFGPU_OK;
   }
@@ -410,21 +439,18 @@ struct Impl : ImplBase {
     if (use_block<T>(lvl)) {
       T *in = down ? v.phi : v.alt, *out = down ? v.alt : v.phi;
       // algorithmic bytes of a block phase: in and rhs read, out written
-      RCHK(timer_start(c, 0, 3.0 * sizeof(T) * l.n * G::NINT));
-      RCHK(launch(k_smooth_block<D, NC, BS, T>, l.n, c.list(l), (const DBox *)c.d_boxes, (const T *)in, out,
-                  (const T *)v.rhs, (const T *)v.phi, 2 * n_cycle, v.lambda2, cyl, v.m));
-      RCHK(timer_stop(c, 0));
+      RCHK(launch_timed(0, 3.0 * sizeof(T) * l.n * G::NINT, k_smooth_block<D, NC, BS, T>, l.n, c.list(l),
+                        (const DBox *)c.d_boxes, (const T *)in, out, (const T *)v.rhs, (const T *)v.phi, 2 * n_cycle,
+                        v.lambda2, cyl, v.m));
       return final_gc ? gc_var(lvl, out, v.m, !down) : AFGPU_OK;
     }
     for (int n = 1; n <= 2 * n_cycle; n++) {
-      RCHK(timer_start(c, 0, bytes));
       if (k == 1)
-        RCHK(launch(k_gsrb_lds<D, NC, BS, T>, l.n, c.list(l), (const DBox *)c.d_boxes, v.phi, (const T *)v.rhs, n,
-                    v.lambda2, cyl, v.m));
+        RCHK(launch_timed(0, bytes, k_gsrb_lds<D, NC, BS, T>, l.n, c.list(l), (const DBox *)c.d_boxes, v.phi,
+                          (const T *)v.rhs, n, v.lambda2, cyl, v.m));
       else
-        RCHK(launch(k_gsrb<D, NC, BS, T>, l.n * k, c.list(l), (const DBox *)c.d_boxes, v.phi, (const T *)v.rhs, n,
-                    v.lambda2, cyl, v.m, k));
-      RCHK(timer_stop(c, 0));
+        RCHK(launch_timed(0, bytes, k_gsrb<D, NC, BS, T>, l.n * k, c.list(l), (const DBox *)c.d_boxes, v.phi,
+                          (const T *)v.rhs, n, v.lambda2, cyl, v.m, k));
     }
     // ghost layer after the last half-sweep (corners on the final up-sweep)
     return final_gc ? gc_var(lvl, v.phi, v.m, !down) : AFGPU_OK;
@@ -968,16 +994,15 @@ struct Impl : ImplBase {
     const VarBc msp = var_methods(c, iv_e);
     for (int lvl = 1; lvl <= c.highest_lvl; lvl++) {
       const List &l = c.l_leaves[lvl];
-      RCHK(timer_start(c, 1, (2.0 + 2 * D) * 8.0 * l.n * G::NINT));
+      const double fbytes = (2.0 + 2 * D) * 8.0 * l.n * G::NINT;
       if (c.precision == AFGPU_PREC_MIXED && c.flux32)
-        RCHK(launch(k_flux<D, NC, BS, float>, l.n, c.list(l), (const DBox *)c.d_boxes, c.cc[iv_e],
-                    (const double *)c.cc[AFGPU_CC_EFIELD], (const double *)c.fc[AFGPU_FC_FIELD],
-                    c.fc[AFGPU_FC_FLUX], msp, c.td, N_inv, c.d_red + 1, c.d_red + 2));
+        RCHK(launch_timed(1, fbytes, k_flux<D, NC, BS, float>, l.n, c.list(l), (const DBox *)c.d_boxes, c.cc[iv_e],
+                          (const double *)c.cc[AFGPU_CC_EFIELD], (const double *)c.fc[AFGPU_FC_FIELD],
+                          c.fc[AFGPU_FC_FLUX], msp, c.td, N_inv, c.d_red + 1, c.d_red + 2));
       else
-        RCHK(launch(k_flux<D, NC, BS, double>, l.n, c.list(l), (const DBox *)c.d_boxes, c.cc[iv_e],
-                    (const double *)c.cc[AFGPU_CC_EFIELD], (const double *)c.fc[AFGPU_FC_FIELD],
-                    c.fc[AFGPU_FC_FLUX], msp, c.td, N_inv, c.d_red + 1, c.d_red + 2));
-      RCHK(timer_stop(c, 1));
+        RCHK(launch_timed(1, fbytes, k_flux<D, NC, BS, double>, l.n, c.list(l), (const DBox *)c.d_boxes, c.cc[iv_e],
+                          (const double *)c.cc[AFGPU_CC_EFIELD], (const double *)c.fc[AFGPU_FC_FIELD],
+                          c.fc[AFGPU_FC_FLUX], msp, c.td, N_inv, c.d_red + 1, c.d_red + 2));
     }
     for (int lvl = 1; lvl <= c.highest_lvl - 1; lvl++) {
       const List &l = c.l_parents[lvl];
@@ -1007,10 +1032,9 @@ struct Impl : ImplBase {
     A.photoi_enabled = c.p.photoi_enabled;
     A.photoi_species = c.p.photoi_species;
     A.cyl = cyl;
-    RCHK(timer_start(c, 2, (7.0 + D + (c.p.photoi_enabled ? 1 : 0)) * 8.0 * c.l_all_leaves.n * G::NINT));
-    RCHK(launch(k_update<D, NC, BS>, c.l_all_leaves.n, c.list(c.l_all_leaves), (const DBox *)c.d_boxes, A,
-                c.d_red + 3));
-    RCHK(timer_stop(c, 2));
+    RCHK(launch_timed(2, (7.0 + D + (c.p.photoi_enabled ? 1 : 0)) * 8.0 * c.l_all_leaves.n * G::NINT,
+                      k_update<D, NC, BS>, c.l_all_leaves.n, c.list(c.l_all_leaves), (const DBox *)c.d_boxes, A,
+                      c.d_red + 3));
     RCHK(read_red(4));
     double lim[4];
     const double cfl_max = as_double(c.h_red[1]);

@@ -84,6 +84,8 @@ def run_gpu(case, precision, warmup, steps, timing_steps, direct_n=None, smooth=
     stats = {}
     if timing_steps > 0:
         import ctypes as C
+        # (modes in turn, so that no two timed kernels share the GPU)
+        p.set_option(A.OPT_BATCH_MODES, 0)
         p.lib.afgpu_kernel_timing(p.h, 1)
         for k in range(3):
             p.lib.afgpu_kernel_stats(p.h, k, 1, None, None, None)
@@ -94,6 +96,8 @@ def run_gpu(case, precision, warmup, steps, timing_steps, direct_n=None, smooth=
             p.lib.afgpu_kernel_stats(p.h, k, 1, C.byref(ms), C.byref(n), C.byref(b))
             stats[name] = {"total_ms": ms.value, "launches": n.value, "algo_bytes": b.value}
         p.lib.afgpu_kernel_timing(p.h, 0)
+        if batch is not None:
+            p.set_option(A.OPT_BATCH_MODES, batch)
     return p, infos, t1 - t0, stats
 
 
@@ -194,7 +198,8 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "avg_launch_ms": round(avg_ms, 5), "launches": st["launches"],
                 "algo_bytes_per_launch": st["algo_bytes"] / max(st["launches"], 1),
-                "timing": "HIP events around each launch, %d eager steps after the timed region"
+                "timing": "start/stop events recorded by each launch's dispatch (hipExtLaunchKernel) on the "
+                          "library's stream, %d eager steps after the timed region"
                           % args.timing_steps}
         log("  kernel stats", json.dumps(stats))
 
