@@ -135,6 +135,12 @@ _GPU_ONLY = {
     "kernel_timing": (_I, [_P, _I]),
     "kernel_stats": (_I, [_P, _I, _I, _DP, C.POINTER(C.c_longlong), _DP]),
     "set_option": (_I, [_P, _I, _I]),
+    "num_runs": (_I, [_P]),
+    "set_run_voltages": (_I, [_P, _I, _DP]),
+    "field_compute_runs": (_I, [_P, _I, _I, _I, _IP, _DP]),
+    "step_ensemble": (_I, [_P, _I, C.POINTER(StepInfoT)]),
+    "get_run_time_state": (_I, [_P, _I, _IP, _DP, _DP]),
+    "set_run_time_state": (_I, [_P, _I, _I, _D, _D]),
 }
 _ORACLE_ONLY = {
     "set_coarse_factors": (_I, [_P, _I, _DP, _DP, _DP]),
@@ -460,3 +466,32 @@ class Path:
 
     def num_leaf_cells(self):
         return self._fn("num_leaf_cells")(self.h)
+
+    # ensembles (device library only) ------------------------------------------
+    def num_runs(self):
+        return self._fn("num_runs")(self.h)
+
+    def set_run_voltages(self, volts):
+        v = np.ascontiguousarray(volts, np.float64)
+        self._call("set_run_voltages", len(v), _dp(v))
+
+    def field_compute_runs(self, s_in=0, have_guess=True):
+        R = self.num_runs()
+        n = np.zeros(R, np.int32)
+        r = np.zeros(R)
+        self._call("field_compute_runs", s_in, int(have_guess), R, _ip(n), _dp(r))
+        return n, r
+
+    def get_run_time_state(self, run):
+        it, t, dt = C.c_int(), C.c_double(), C.c_double()
+        self._call("get_run_time_state", run, C.byref(it), C.byref(t), C.byref(dt))
+        return it.value, t.value, dt.value
+
+    def set_run_time_state(self, run, it, time, global_dt):
+        self._call("set_run_time_state", run, it, time, global_dt)
+
+    def step_ensemble(self):
+        R = self.num_runs()
+        infos = (StepInfoT * R)()
+        self._call("step_ensemble", R, infos)
+        return [infos[r].as_dict() for r in range(R)]

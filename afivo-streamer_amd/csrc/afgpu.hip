@@ -91,6 +91,7 @@ struct ImplBase {
   virtual int photoi() = 0;
   virtual int copy_states(bool backup) = 0;
   virtual int restrict_tree(int iv) = 0;
+  virtual int step_runs(afgpu_step_info_t *info) = 0;
 };
 
 struct afgpu_ctx {
@@ -111,6 +112,34 @@ struct afgpu_ctx {
   } mres[AFGPU_MAX_MODES];
   // tree
   int n_boxes = 0, cap_boxes = 0, highest_lvl = 0;
+  // Forest of independent runs (a parameter study): every level-1 box roots
+  // a run of its own.  A plain tree is run 0.  Each run has its own applied
+  // voltage, reduction slots, convergence test and time step.
+  int n_runs = 1;
+  std::vector<double> run_volt;    // applied voltage per run
+  double *d_run_volt = nullptr;
+  std::vector<double> run_min_dr;  // finest cell size per run
+  int *d_run_of = nullptr;         // run of every box (index id - 1)
+  int *d_run_active = nullptr;     // runs the list kernels work on
+  int *d_run_host = nullptr;       // runs the host has made active
+  int *h_run_active = nullptr;     // pinned staging of the above
+  int *d_run_maxlvl = nullptr;     // highest level of every run
+  bool uneven = false;             // runs of different highest levels
+  bool masked = false;             // some list entries are negated
+  std::vector<int> base_active;    // runs taking part in the current step (empty: all)
+  int red_cap_runs = 0;            // runs the reduction buffers hold
+  double *d_dt_run = nullptr, *h_dt_run = nullptr;  // per-run dt of the stage advanced
+  bool use_run_dt = false;
+  std::vector<double> run_dt_lim;  // per-run dt limit of the last forward_euler
+  std::vector<long long> run_leaf_cells;  // leaf cells per run
+  std::vector<int> run_ncyc;       // per-run V-cycles of the last field solve
+  std::vector<double> run_res;     // per-run final max |residual| of that solve
+  struct RunTime {
+    int it = 0;
+    double time = 0, global_time = 0, global_dt = 0, dt = 0, frac_rejected = 0;
+    bool started = false;
+  };
+  std::vector<RunTime> rt;         // time-loop state per run (ensemble stepping)
   std::vector<DBox> hboxes;
   DBox *d_boxes = nullptr;
   std::vector<int> h_lists;
@@ -189,6 +218,10 @@ VarBc var_methods(const afgpu_ctx &c, int iv, bool homogeneous = false) {
     m.type[2 * last] = BC_DIRICHLET;
     m.type[2 * last + 1] = BC_DIRICHLET;
     m.val[2 * last + 1] = homogeneous ? 0.0 : c.voltage;
+    if (!homogeneous && c.n_runs > 1) {  // per-run applied voltage
+      m.run_face = 2 * last + 1;
+      m.run_val = c.d_run_volt;
+    }
   } else if (iv >= AFGPU_CC_HELMH && iv < AFGPU_CC_HELMH + AFGPU_MAX_MODES) {
     m.rb = RB_MG;
     m.type[2 * last] = BC_DIRICHLET;
@@ -222,8 +255,9 @@ enum GraphKind {
   G_MP_RES_MODES = 8,
   G_MP_FMG_MODES = 9
 };
-// reduction slots: 0 residual max, 1-3 dt limits, 8 + n photoionization mode n
-constexpr int N_RED_SLOTS = 16;
+// reduction slots (per run): 0 residual max, 1-3 dt limits, 8 + n
+// photoionization mode n
+constexpr int N_RED_SLOTS = RED_STRIDE;
 constexpr int RED_MODE = 8;
 
 int op_phi(int op) { return op == AFGPU_OP_POISSON ? AFGPU_CC_PHI : AFGPU_CC_HELMH + (op - AFGPU_OP_HELMH); }
@@ -477,7 +511,10 @@ struct Impl : ImplBase {
                   (const T *)level_buf(v, lvl, v_cycle), v.phi, (const T *)v.rhs, v.tmp, v.lambda2, cyl,
                   (int)!v_cycle, v.m));
     } else {
-      if (!v_cycle && lvl == c.highest_lvl) RCHK(gc_var(lvl, v.phi, v.m, true));
+      // (a forest whose runs end on different levels fills every level: a
+      // run's own finest level needs it, and for the other runs the fill
+      // repeats the one the level above has just made, with the same values)
+      if (!v_cycle && (lvl == c.highest_lvl || c.uneven)) RCHK(gc_var(lvl, v.phi, v.m, true));
       RCHK(launch(k_restrict_residual<D, NC, BS, T>, l.n * k, c.list(l), (const DBox *)c.d_boxes, v.phi,
                   (const T *)v.rhs, v.tmp, v.lambda2, cyl, (int)!v_cycle, k));
     }
@@ -524,9 +561,9 @@ struct Impl : ImplBase {
   // widened on load and rounded on store).
   template <typename T>
   int solve_coarse(const Mg<T> &v) {
-    const int id = c.h_lists[c.l_ids[1].off];
+    const List &l1 = c.l_ids[1];  // one box per run
     const afgpu_ctx::Op &o = c.ops[v.op];
-    hipLaunchKernelGGL((k_coarse_solve<D, NC, 1024, T>), dim3(1), dim3(1024), 0, c.cur, id,
+    hipLaunchKernelGGL((k_coarse_solve<D, NC, 1024, T>), dim3(l1.n), dim3(1024), 0, c.cur, c.list(l1),
                        (const DBox *)c.d_boxes, (const T *)v.rhs, v.phi, v.m, o.lambda2,
                        (int)(c.coord == AFGPU_CYL), (const double *)o.d_vinv, (const double *)o.d_v,
                        (const double *)o.d_eig);
@@ -563,25 +600,77 @@ struct Impl : ImplBase {
       const bool final_gc = !(std::is_same<T, float>::value && lvl == c.highest_lvl);
       RCHK(gsrb_boxes(v, lvl, false, final_gc));
     }
-    if (set_residual) {
-      // residual on every box; the max |residual| over leaves is reduced on
-      // the fly (read back with leaf_residual_max)
-      const int cyl = c.coord == AFGPU_CYL;
-      RCHK(set_red(0, 0ull));
-      for (int lvl = 1; lvl <= max_lvl; lvl++) {
-        const List &l = c.l_ids[lvl];
-        const int k = bpb(l.n, G::NINT);
-        RCHK(launch(k_residual<D, NC, BS, T>, l.n * k, c.list(l), (const DBox *)c.d_boxes, (const T *)v.phi,
-                    (const T *)v.rhs, v.tmp, v.lambda2, cyl, c.d_red, k));
-      }
+    if (set_residual) RCHK(residual_pass(v, max_lvl, true));
+    return AFGPU_OK;
+  }
+  // residual on every box of levels 1..max_lvl; the max |residual| over
+  // leaves is reduced on the fly into slot 0 of each run (read back with
+  // leaf_residual_max), cleared first with `reset`
+  template <typename T>
+  int residual_pass(const Mg<T> &v, int max_lvl, bool reset) {
+    const int cyl = c.coord == AFGPU_CYL;
+    if (reset) RCHK(set_red(0, 0ull));
+    for (int lvl = 1; lvl <= max_lvl; lvl++) {
+      const List &l = c.l_ids[lvl];
+      const int k = bpb(l.n, G::NINT);
+      RCHK(launch(k_residual<D, NC, BS, T>, l.n * k, c.list(l), (const DBox *)c.d_boxes, (const T *)v.phi,
+                  (const T *)v.rhs, v.tmp, v.lambda2, cyl, c.d_red, k));
     }
     return AFGPU_OK;
   }
 
   // max |tmp| over leaf interiors as reduced by the last set_residual pass
+  // (the max over the runs of a forest)
   int leaf_residual_max(double *out) {
     RCHK(read_red(1));
     *out = as_double(c.h_red[0]);
+    for (int r = 1; r < c.n_runs; r++) *out = std::max(*out, run_red(r, 0));
+    return AFGPU_OK;
+  }
+  // slot of run r as read by the last read_red
+  double run_red(int r, int slot) const { return as_double(c.h_red[(size_t)r * N_RED_SLOTS + slot]); }
+
+  // The runs taking part in the current step (all, unless a rejected step
+  // is being redone for some runs only).
+  std::vector<int> base_set() const {
+    return c.base_active.empty() ? std::vector<int>(c.n_runs, 1) : c.base_active;
+  }
+  // Let the list kernels work on the runs with active[r] != 0 only (null:
+  // the base set): the ids of the other runs are negated in every list and
+  // in the uniform-level map.  A no-op for a plain tree.
+  int set_active(const std::vector<int> *active) {
+    if (c.n_runs == 1) return AFGPU_OK;
+    const std::vector<int> base = base_set();
+    if (!active) active = &base;
+    bool all = true;
+    for (int r = 0; r < c.n_runs; r++) {
+      c.h_run_active[r] = (*active)[r] != 0;
+      all = all && c.h_run_active[r];
+    }
+    if (all && !c.masked) return AFGPU_OK;
+    HIPCHK(hipMemcpyAsync(c.d_run_host, c.h_run_active, c.n_runs * sizeof(int), hipMemcpyHostToDevice, c.stream));
+    RCHK(mask_levels(0, std::numeric_limits<int>::max()));
+    // (the staging buffer is rewritten by the next call)
+    HIPCHK(hipStreamSynchronize(c.stream));
+    c.masked = !all;
+    return AFGPU_OK;
+  }
+  // Device side of the masks (capturable): the list kernels work on the
+  // runs made active by the host whose highest level lies in [lo, hi].
+  int mask_levels(int lo, int hi) {
+    hipLaunchKernelGGL(k_runs_in_levels, dim3((c.n_runs + 63) / 64), dim3(64), 0, c.cur, (const int *)c.d_run_host,
+                       (const int *)c.d_run_maxlvl, c.d_run_active, lo, hi, c.n_runs);
+    HIPCHK(hipGetLastError());
+    const int n = (int)c.h_lists.size();
+    hipLaunchKernelGGL(k_mask_ids, dim3((n + 255) / 256), dim3(256), 0, c.cur, c.d_lists, n,
+                       (const int *)c.d_run_of, (const int *)c.d_run_active);
+    HIPCHK(hipGetLastError());
+    if (c.dir.set && c.dir.lc > 1) {
+      const int nm = c.l_ids[c.dir.lc].n;
+      hipLaunchKernelGGL(k_mask_ids, dim3((nm + 255) / 256), dim3(256), 0, c.cur, c.dir.d_map, nm,
+                         (const int *)c.d_run_of, (const int *)c.d_run_active);
+      HIPCHK(hipGetLastError());
+    }
     return AFGPU_OK;
   }
 
@@ -608,6 +697,7 @@ struct Impl : ImplBase {
     // too (refined boxes there keep the last cycle's values otherwise)
     if (std::is_same<T, float>::value && !have_guess) RCHK(boxop<T>(c.l_ids[lc], v.phi, v.phi, OP_CLEAR));
     RCHK(boxop<T>(c.l_ids[lc], v.phi, v.tmp, OP_COPY));
+    if (c.uneven) return fmg_up_uneven(v, set_residual, lc);
     RCHK(vcycle_body(v, set_residual && c.highest_lvl == lc, lc, lc));
     for (int lvl = lc + 1; lvl <= c.highest_lvl; lvl++) {
       RCHK(boxop<T>(c.l_ids[lvl], v.phi, v.tmp, OP_COPY));
@@ -615,6 +705,32 @@ struct Impl : ImplBase {
       RCHK(vcycle_body(v, set_residual && lvl == c.highest_lvl, lvl, lc));
     }
     return AFGPU_OK;
+  }
+  // The upward part of FMG for a forest whose runs end on different
+  // levels: the V-cycle ending on level l involves only the runs that reach
+  // l (a run's FMG ends on its own finest level), and each run's residual
+  // is formed after its own last V-cycle.  The masks are set on the device,
+  // so the sequence can be captured.
+  template <typename T>
+  int fmg_up_uneven(const Mg<T> &v, bool set_residual, int lc) {
+    const int top = std::numeric_limits<int>::max();
+    if (set_residual) RCHK(set_red(0, 0ull));
+    RCHK(vcycle_body(v, false, lc, lc));
+    if (set_residual) {
+      RCHK(mask_levels(lc, lc));
+      RCHK(residual_pass(v, lc, false));
+    }
+    for (int lvl = lc + 1; lvl <= c.highest_lvl; lvl++) {
+      RCHK(mask_levels(lvl, top));
+      RCHK(boxop<T>(c.l_ids[lvl], v.phi, v.tmp, OP_COPY));
+      RCHK(correct_children(v, lvl - 1));
+      RCHK(vcycle_body(v, false, lvl, lc));
+      if (set_residual) {
+        RCHK(mask_levels(lvl, lvl));
+        RCHK(residual_pass(v, lvl, false));
+      }
+    }
+    return mask_levels(0, top);
   }
 
   // --- exact solve on a uniformly refined level (mixed precision) ---------
@@ -624,9 +740,11 @@ struct Impl : ImplBase {
   int direct_level(int *lc_out) {
     afgpu_ctx::Direct &d = c.dir;
     if (!d.set) {
+      // (a forest of 2D runs is solved as a batch of equal grids, one per
+      // run; 3D forests use the level-1 solve)
       int lc = 1;
-      for (int l = 2; l <= c.highest_lvl; l++) {
-        const long long want = 1ll << (D * (l - 1));
+      for (int l = 2; l <= c.highest_lvl && (D == 2 || c.n_runs == 1); l++) {
+        const long long want = (long long)c.n_runs << (D * (l - 1));
         const int n = NC << (l - 1);
         if (c.l_ids[l].n != want || n > c.direct_max_n || n < 16) break;
         lc = l;
@@ -636,6 +754,7 @@ struct Impl : ImplBase {
       if (lc > 1) {
         const int nb = 1 << (lc - 1);
         const int nbox = c.l_ids[lc].n;
+        const int per = nbox / c.n_runs;  // boxes of one run's level
         std::vector<int> map(nbox, 0);
         for (int q = 0; q < nbox; q++) {
           const int id = c.h_lists[c.l_ids[lc].off + q];
@@ -645,6 +764,8 @@ struct Impl : ImplBase {
             pos += (b.ix[dd] - 1) * mul;
             mul *= nb;
           }
+          if (pos < 0 || pos >= per || b.run < 0 || b.run >= c.n_runs) pos = -1;
+          if (pos >= 0) pos += b.run * per;
           if (pos < 0 || pos >= nbox || map[pos] != 0) {
             set_err("uniform level: inconsistent box positions");
             return AFGPU_ERR_STATE;
@@ -655,7 +776,7 @@ struct Impl : ImplBase {
         d.d_map = nullptr;
         HIPCHK(hipMalloc(&d.d_map, nbox * sizeof(int)));
         HIPCHK(hipMemcpy(d.d_map, map.data(), nbox * sizeof(int), hipMemcpyHostToDevice));
-        size_t cells = 1;
+        size_t cells = c.n_runs;
         for (int dd = 0; dd < D; dd++) cells *= (size_t)d.N;
         if (cells > d.cap) {
           (void)hipFree(d.A0);
@@ -710,20 +831,23 @@ struct Impl : ImplBase {
   template <int N>
   int direct_transforms(const afgpu_ctx::Direct::F &f, double lambda2, double *A0, double *A1) {
     const size_t NN = (size_t)N * N;
-    const int nrows = (D == 3) ? N * N : N;
+    // the grids of all runs are transformed together: the x rows of every
+    // run, and the y lines with the run as the remaining coordinate
+    const int batch = (D == 3) ? N : c.n_runs;
+    const int nrows = (D == 3) ? N * N : N * c.n_runs;
     constexpr int TI = (N < 64) ? N : 64;
     constexpr int TR = (256 / TI) / (N / TI) * 16;
     const int row_blocks = (nrows + TR - 1) / TR;
     constexpr int TC = (N < 64) ? N : (N == 64 ? 64 : 32);
-    const int line_blocks = (N / TC) * ((D == 3) ? N : 1);
+    const int line_blocks = (N / TC) * batch;
     // rows(): the x transform (MT: transposed matrix for the small-N
     // kernel, M itself for the tiled one); lines(): the y / z transforms
     constexpr bool TILED = (N == 64 || N == 128);
     const int gemm_row_blocks = (N / 64) * (nrows / 64);
-    const int gemm_line_blocks = (N / 64) * (N / 64) * ((D == 3) ? N : 1);
+    const int gemm_line_blocks = (N / 64) * (N / 64) * batch;
     const bool MC = TILED && c.direct_mfma;
     const int mfma_row_blocks = (N / 64) * (nrows / 64);
-    const int mfma_line_blocks = (N / 64) * (N / 64) * ((D == 3) ? N : 1);
+    const int mfma_line_blocks = (N / 64) * (N / 64) * batch;
     auto rows = [&](const double *in, double *out, const double *Mx, const double *MTx) {
       if (MC)
         hipLaunchKernelGGL((k_lc_mfma<D, (TILED ? N : 64), true>), dim3(mfma_row_blocks), dim3(256), 0, c.cur, in,
@@ -857,14 +981,18 @@ struct Impl : ImplBase {
     });
   }
 
+  // read slots 0..n-1 of every run
   int read_red(int n) {
-    HIPCHK(hipMemcpyAsync(c.h_red, c.d_red, n * sizeof(unsigned long long), hipMemcpyDeviceToHost, c.stream));
+    const size_t cnt = (size_t)(c.n_runs - 1) * N_RED_SLOTS + n;
+    HIPCHK(hipMemcpyAsync(c.h_red, c.d_red, cnt * sizeof(unsigned long long), hipMemcpyDeviceToHost, c.stream));
     HIPCHK(hipStreamSynchronize(c.stream));
     return timer_harvest(c);
   }
-  // reduction slot i := value (a kernel, so that it can live in a graph)
+  // reduction slot i of every run := value (a kernel, so that it can live
+  // in a graph)
   int set_red(int i, unsigned long long value) {
-    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, c.cur, c.d_red + i, value);
+    hipLaunchKernelGGL(k_set_run_slots, dim3((c.n_runs + 63) / 64), dim3(64), 0, c.cur, c.d_red + i, value,
+                       c.n_runs);
     HIPCHK(hipGetLastError());
     return AFGPU_OK;
   }
@@ -874,13 +1002,22 @@ struct Impl : ImplBase {
     return d;
   }
 
-  int maxabs(int iv, double *out) override {
+  // max |u| over leaf interiors, per run (out[r])
+  int maxabs_runs(int iv, std::vector<double> &out) {
     RCHK(set_red(0, 0ull));
     const int k = bpb(c.l_all_leaves.n, G::NINT);
-    RCHK(launch(k_maxabs<D, NC, BS>, c.l_all_leaves.n * k, c.list(c.l_all_leaves), (const double *)c.cc[iv], c.d_red,
-                k));
+    RCHK(launch(k_maxabs<D, NC, BS>, c.l_all_leaves.n * k, c.list(c.l_all_leaves), (const DBox *)c.d_boxes,
+                (const double *)c.cc[iv], c.d_red, k));
     RCHK(read_red(1));
-    *out = c.l_all_leaves.n > 0 ? as_double(c.h_red[0]) : -std::numeric_limits<double>::max() / 10;
+    out.assign(c.n_runs, 0.0);
+    for (int r = 0; r < c.n_runs; r++)
+      out[r] = c.l_all_leaves.n > 0 ? run_red(r, 0) : -std::numeric_limits<double>::max() / 10;
+    return AFGPU_OK;
+  }
+  int maxabs(int iv, double *out) override {
+    std::vector<double> m;
+    RCHK(maxabs_runs(iv, m));
+    *out = *std::max_element(m.begin(), m.end());
     return AFGPU_OK;
   }
 
@@ -893,58 +1030,87 @@ struct Impl : ImplBase {
     });
   }
 
-  double min_dr() const {
-    const DBox &b = c.hboxes[c.h_lists[c.l_ids[c.highest_lvl].off] - 1];
-    double m = b.dr[0];
-    for (int d = 1; d < D; d++) m = std::min(m, b.dr[d]);
-    return m;
-  }
-
+  // field_compute (src/m_field.f90:448-528).  Each run of a forest has its
+  // own threshold and stops on its own test: the runs that have converged
+  // are masked out of the following cycles, so every run does exactly the
+  // cycles it would do alone.
   int field_compute(int s_in, bool have_guess, int *n_out, double *res_out) override {
+    const int R = c.n_runs;
+    RCHK(set_active(nullptr));
     RCHK(launch(k_set_rhs<G::NCELL, BS>, c.l_all_leaves.n, c.list(c.l_all_leaves),
                 (const double *)c.cc[AFGPU_CC_ELECTRON + s_in], (const double *)c.cc[AFGPU_CC_POS_ION + s_in],
                 (const double *)c.cc[AFGPU_CC_NEG_ION + s_in], c.cc[AFGPU_CC_RHS]));
-    double max_rhs;
-    RCHK(maxabs(AFGPU_CC_RHS, &max_rhs));
+    std::vector<double> max_rhs;
+    RCHK(maxabs_runs(AFGPU_CC_RHS, max_rhs));
     const double conv_fac = 1e-10, min_residual = 1e-6, max_residual = 1e8;
-    const double threshold = std::max(std::max(min_residual, max_rhs * c.p.mg_max_rel_residual),
-                                      conv_fac * std::fabs(c.voltage) / (c.p.domain_len_last * min_dr()));
-    int count = 0;
-    double last = 0;
+    std::vector<double> threshold(R);
+    for (int r = 0; r < R; r++) {
+      const double volt = (R > 1) ? c.run_volt[r] : c.voltage;
+      threshold[r] = std::max(std::max(min_residual, max_rhs[r] * c.p.mg_max_rel_residual),
+                              conv_fac * std::fabs(volt) / (c.p.domain_len_last * c.run_min_dr[r]));
+    }
+    c.run_ncyc.assign(R, 0);
+    c.run_res.assign(R, 0.0);
+    std::vector<int> active = base_set();
+    int n_active = 0;
+    for (int r = 0; r < R; r++) n_active += active[r] != 0;
+    const int n_part = n_active;
     if (!have_guess) {
-      std::vector<double> res(101, 0.0);
+      std::vector<std::vector<double>> res(R, std::vector<double>(101, 0.0));
       int i;
-      for (i = 1; i <= 100; i++) {
+      for (i = 1; i <= 100 && n_active > 0; i++) {
         RCHK(fmg(AFGPU_OP_POISSON, true, true));
-        RCHK(leaf_residual_max(&res[i]));
-        count++;
-        last = res[i];
-        if (res[i] < threshold) break;
-        if (i > 2) {
-          double mn = std::min(std::min(res[i - 2], res[i - 1]), res[i]);
-          double mx = std::max(std::max(res[i - 2], res[i - 1]), res[i]);
-          double ratio = mn / mx;
-          if (ratio < 2.0 && ratio > 0.5 && res[i] < max_residual) break;
+        RCHK(read_red(1));
+        for (int r = 0; r < R; r++) {
+          if (!active[r]) continue;
+          res[r][i] = run_red(r, 0);
+          c.run_ncyc[r]++;
+          c.run_res[r] = res[r][i];
+          bool done = res[r][i] < threshold[r];
+          if (!done && i > 2) {
+            const double mn = std::min(std::min(res[r][i - 2], res[r][i - 1]), res[r][i]);
+            const double mx = std::max(std::max(res[r][i - 2], res[r][i - 1]), res[r][i]);
+            const double ratio = mn / mx;
+            done = ratio < 2.0 && ratio > 0.5 && res[r][i] < max_residual;
+          }
+          if (done) {
+            active[r] = 0;
+            n_active--;
+          }
         }
+        RCHK(set_active(&active));
       }
-      if (i == 101) {
+      if (n_active > 0) {
+        RCHK(set_active(nullptr));
         set_err("No convergence in initial field computation");
         return AFGPU_ERR_NO_CONVERGENCE;
       }
+      active = base_set();
+      n_active = n_part;
+      RCHK(set_active(nullptr));
     }
     if (c.precision == AFGPU_PREC_MIXED) {
-      RCHK(mixed_vcycles(threshold, &count, &last));
+      RCHK(mixed_vcycles(threshold, active));
     } else {
-      for (int i = 1; i <= c.p.mg_num_vcycles; i++) {
+      for (int i = 1; i <= c.p.mg_num_vcycles && n_active > 0; i++) {
         RCHK(vcycle(AFGPU_OP_POISSON, true, c.highest_lvl));
-        RCHK(leaf_residual_max(&last));
-        count++;
-        if (last < threshold) break;
+        RCHK(read_red(1));
+        for (int r = 0; r < R; r++) {
+          if (!active[r]) continue;
+          c.run_res[r] = run_red(r, 0);
+          c.run_ncyc[r]++;
+          if (c.run_res[r] < threshold[r]) {
+            active[r] = 0;
+            n_active--;
+          }
+        }
+        RCHK(set_active(&active));
       }
+      RCHK(set_active(nullptr));
     }
     RCHK(field_from_potential());
-    if (n_out) *n_out = count;
-    if (res_out) *res_out = last;
+    if (n_out) *n_out = *std::max_element(c.run_ncyc.begin(), c.run_ncyc.end());
+    if (res_out) *res_out = *std::max_element(c.run_res.begin(), c.run_res.end());
     return AFGPU_OK;
   }
 
@@ -953,24 +1119,37 @@ struct Impl : ImplBase {
   // V-cycle from e = 0 (smoothing, restriction and prolongation in FP32;
   // the coarsest level solved exactly in FP64: the finest uniformly refined
   // level when there is one, else the level-1 box), adds e to phi in FP64
-  // and forms the new residual in FP64; the loop stops on the same FP64
-  // leaf-residual test as the plain form (m_field.f90:515-521).
-  int mixed_vcycles(double threshold, int *count, double *last) {
+  // and forms the new residual in FP64; each run stops on the same FP64
+  // leaf-residual test as the plain form (m_field.f90:515-521).  `active`
+  // holds the runs still iterating (all of them on entry).
+  int mixed_vcycles(const std::vector<double> &threshold, std::vector<int> &active) {
     const int op = AFGPU_OP_POISSON;
+    const int R = c.n_runs;
     int lc;
     RCHK(ensure_factors(op));
     RCHK(direct_level(&lc));
     RCHK(ensure_direct_factors(op));
     RCHK(run_graph(c, {G_MP_RES, op, lc, 0}, [&] { return mp_close(op, false, lc); }));
-    for (int i = 1; i <= c.p.mg_num_vcycles; i++) {
+    int n_active = 0;
+    for (int r = 0; r < R; r++) n_active += active[r] != 0;
+    for (int i = 1; i <= c.p.mg_num_vcycles && n_active > 0; i++) {
       RCHK(run_graph(c, {G_MP_VCYCLE, op, lc, c.highest_lvl}, [&] {
         RCHK(vcycle_body(mg32(op), false, c.highest_lvl, lc));
         return mp_close(op, true, lc);
       }));
-      RCHK(leaf_residual_max(last));
-      (*count)++;
-      if (*last < threshold) break;
+      RCHK(read_red(1));
+      for (int r = 0; r < R; r++) {
+        if (!active[r]) continue;
+        c.run_res[r] = run_red(r, 0);
+        c.run_ncyc[r]++;
+        if (c.run_res[r] < threshold[r]) {
+          active[r] = 0;
+          n_active--;
+        }
+      }
+      RCHK(set_active(&active));
     }
+    RCHK(set_active(nullptr));
     return mp_finish(op, lc);
   }
 
@@ -1017,6 +1196,8 @@ struct Impl : ImplBase {
     A.flux = c.fc[AFGPU_FC_FLUX];
     A.chem = c.chem;
     A.dt = dt;
+    A.dt_run = c.use_run_dt ? c.d_dt_run : nullptr;  // forest: dt_run[r] scaled by dt
+    A.dt_scale = dt;
     A.N_inv = N_inv;
     A.s_deriv = s_deriv;
     A.s_out = s_out;
@@ -1036,16 +1217,21 @@ struct Impl : ImplBase {
                       k_update<D, NC, BS>, c.l_all_leaves.n, c.list(c.l_all_leaves), (const DBox *)c.d_boxes, A,
                       c.d_red + 3));
     RCHK(read_red(4));
-    double lim[4];
-    const double cfl_max = as_double(c.h_red[1]);
-    const double sig_max = as_double(c.h_red[2]);
-    lim[0] = c.l_all_leaves.n > 0 ? 1 / cfl_max : 1e100;
-    lim[1] = c.l_all_leaves.n > 0 ? UC_eps0 / (UC_elem_charge * std::max(sig_max, 1e-100)) : 1e100;
-    lim[2] = last_step ? as_double(c.h_red[3]) : 1e100;
-    lim[3] = 1e100;
-    lim[0] = lim[0] * c.p.dt_cfl_number;
-    const double mv = std::min(std::min(std::min(lim[0], lim[1]), lim[2]), lim[3]);
-    *dt_lim = std::min(c.p.dt_max, mv);
+    // per run (only the runs not masked out are meaningful)
+    c.run_dt_lim.assign(c.n_runs, 1e100);
+    for (int r = 0; r < c.n_runs; r++) {
+      double lim[4];
+      const double cfl_max = run_red(r, 1);
+      const double sig_max = run_red(r, 2);
+      lim[0] = c.l_all_leaves.n > 0 ? 1 / cfl_max : 1e100;
+      lim[1] = c.l_all_leaves.n > 0 ? UC_eps0 / (UC_elem_charge * std::max(sig_max, 1e-100)) : 1e100;
+      lim[2] = last_step ? run_red(r, 3) : 1e100;
+      lim[3] = 1e100;
+      lim[0] = lim[0] * c.p.dt_cfl_number;
+      const double mv = std::min(std::min(std::min(lim[0], lim[1]), lim[2]), lim[3]);
+      c.run_dt_lim[r] = std::min(c.p.dt_max, mv);
+    }
+    *dt_lim = c.run_dt_lim[0];
     return AFGPU_OK;
   }
 
@@ -1185,6 +1371,124 @@ struct Impl : ImplBase {
     return AFGPU_OK;
   }
 
+  // --- ensembles: one time step of every run of a forest -------------------
+  // The time-loop body of src/streamer.f90:181-432 (as afgpu_step does it
+  // for one run) for every run at once: each run has its own dt, its own
+  // rejections and its own field-solve stopping test.  A rejected run is
+  // restored and advanced again with the other runs masked out.
+  int step_runs(afgpu_step_info_t *info) override {
+    const int R = c.n_runs;
+    for (auto &t : c.rt) {
+      if (!t.started) {
+        t.global_dt = c.p.dt_min;
+        t.dt = t.global_dt;
+        t.started = true;
+      }
+      t.it = t.it + 1;
+    }
+    std::vector<double> dt(R), dt_lim(R, std::numeric_limits<double>::max());
+    std::vector<int> todo(R, 1), n_att(R, 0), n_rej(R, 0);
+    for (int r = 0; r < R; r++) dt[r] = c.rt[r].dt;
+    const int max_attempts = 10;
+    int n_todo = R;
+    for (int n = 1; n <= max_attempts && n_todo > 0; n++) {
+      c.base_active = todo;
+      RCHK(set_active(nullptr));
+      RCHK(copy_states(true));
+      for (int r = 0; r < R; r++) c.h_dt_run[r] = dt[r];
+      // (the staging buffer is not rewritten before forward_euler's read-back
+      // has synchronised the stream)
+      HIPCHK(hipMemcpyAsync(c.d_dt_run, c.h_dt_run, R * sizeof(double), hipMemcpyHostToDevice, c.stream));
+      c.use_run_dt = true;
+      double dl;
+      int rc;
+      if (c.p.time_integrator == AFGPU_FORWARD_EULER) {
+        const int sp[1] = {0};
+        const double w[1] = {1.0};
+        rc = forward_euler(1.0, 0, 1, sp, w, 0, 1, 1, &dl);
+      } else {
+        const int sp1[1] = {0};
+        const double w1[1] = {1.0};
+        rc = forward_euler(1.0, 0, 1, sp1, w1, 1, 1, 2, &dl);
+        const int sp2[2] = {0, 1};
+        const double w2[2] = {0.5, 0.5};
+        if (rc == AFGPU_OK) rc = forward_euler(0.5, 1, 2, sp2, w2, 0, 2, 2, &dl);
+      }
+      c.use_run_dt = false;
+      if (rc != AFGPU_OK) {
+        c.base_active.clear();
+        return rc;
+      }
+      std::vector<int> rejected(R, 0);
+      int n_rejected = 0;
+      for (int r = 0; r < R; r++) {
+        if (!todo[r]) continue;
+        afgpu_ctx::RunTime &t = c.rt[r];
+        t.time = t.time + dt[r];
+        const double lim = c.run_dt_lim[r];
+        dt_lim[r] = std::min(dt_lim[r], lim);
+        n_att[r] = n;
+        if (dt[r] <= lim) {
+          todo[r] = 0;
+          n_todo--;
+        } else {
+          n_rej[r]++;
+          dt[r] = c.p.dt_safety_factor * lim;
+          t.time = t.global_time;
+          rejected[r] = 1;
+          n_rejected++;
+        }
+      }
+      if (n_rejected > 0) {
+        c.base_active = rejected;
+        RCHK(set_active(nullptr));
+        RCHK(copy_states(false));
+      }
+    }
+    c.base_active.clear();
+    RCHK(set_active(nullptr));
+    if (n_todo > 0) {
+      set_err("All time steps were rejected");
+      return AFGPU_ERR_ALL_REJECTED;
+    }
+    RCHK(field_compute(0, true, nullptr, nullptr));
+    std::vector<double> max_rhs;
+    if (info) RCHK(maxabs_runs(AFGPU_CC_RHS, max_rhs));
+    bool too_small = false;
+    for (int r = 0; r < R; r++) {
+      afgpu_ctx::RunTime &t = c.rt[r];
+      t.frac_rejected = 0.99 * t.frac_rejected;
+      if (n_att[r] > 1) t.frac_rejected = t.frac_rejected + 0.01;
+      const double dt_used = dt[r];
+      double tmp = c.p.dt_max_growth_factor;
+      if (t.frac_rejected > 0.1) tmp = 1.0;
+      const double dtn = std::min(tmp * t.global_dt, c.p.dt_safety_factor * std::min(dt_lim[r], c.p.dt_max));
+      t.global_dt = dtn;
+      t.global_time = t.time;
+      t.dt = dtn;
+      too_small = too_small || t.global_dt < c.p.dt_min;
+      if (info) {
+        afgpu_step_info_t &f = info[r];
+        f.it = t.it;
+        f.time = t.time;
+        f.dt_used = dt_used;
+        f.dt_next = dtn;
+        f.dt_lim = dt_lim[r];
+        f.n_rejected = n_rej[r];
+        f.n_vcycles = c.run_ncyc[r];
+        f.residual = c.run_res[r];
+        f.rel_residual = max_rhs[r] > 0 ? c.run_res[r] / max_rhs[r] : 0;
+        f.photoi_done = 0;
+        f.leaf_cells = c.run_leaf_cells[r];
+      }
+    }
+    if (too_small) {
+      set_err("dt too small");
+      return AFGPU_ERR_DT_TOO_SMALL;
+    }
+    return AFGPU_OK;
+  }
+
   int restrict_tree(int iv) override {
     for (int lvl = c.highest_lvl; lvl >= 2; lvl--) {
       const List &l = c.l_ids[lvl];
@@ -1270,8 +1574,6 @@ int afgpu_create(int ndim, int n_cell, int coord, int precision, afgpu_t **out) 
     HIPCHK(hipEventCreateWithFlags(&h->join_ev[n], hipEventDisableTiming));
   }
   HIPCHK(hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming));
-  HIPCHK(hipMalloc(&h->d_red, N_RED_SLOTS * sizeof(unsigned long long)));
-  HIPCHK(hipHostMalloc(&h->h_red, N_RED_SLOTS * sizeof(unsigned long long), hipHostMallocDefault));
   *out = h.release();
   return AFGPU_OK;
 }
@@ -1300,6 +1602,14 @@ int afgpu_destroy(afgpu_t *h) {
   (void)hipFree(h->d_chem);
   (void)hipFree(h->d_red);
   if (h->h_red) (void)hipHostFree(h->h_red);
+  (void)hipFree(h->d_run_of);
+  (void)hipFree(h->d_run_volt);
+  (void)hipFree(h->d_run_active);
+  (void)hipFree(h->d_run_host);
+  (void)hipFree(h->d_run_maxlvl);
+  (void)hipFree(h->d_dt_run);
+  if (h->h_run_active) (void)hipHostFree(h->h_run_active);
+  if (h->h_dt_run) (void)hipHostFree(h->h_dt_run);
   for (auto &o : h->ops) {
     (void)hipFree(o.d_vinv);
     (void)hipFree(o.d_v);
@@ -1371,9 +1681,46 @@ int afgpu_sync_tree(afgpu_t *h, int n_boxes, const int *lvl, const int *ix, cons
       for (int q = 0; q < nnb; q++) rb |= (x.nb[q] == 0);
       if (x.parent > 0 && rb) refb.push_back(id);
     }
-  if (hl < 1 || ids[1].size() != 1) {
-    set_err("level 1 must consist of exactly one box (coarse_grid_size = box_size)");
-    return AFGPU_ERR_UNSUPPORTED;
+  if (hl < 1 || ids[1].empty()) {
+    set_err("afgpu_sync_tree: no level-1 box");
+    return AFGPU_ERR_ARG;
+  }
+  // Runs: every level-1 box roots a run of its own (coarse_grid_size =
+  // box_size; several level-1 boxes form a forest of independent runs, all
+  // with the same level-1 geometry); boxes inherit their parent's run.
+  const int n_runs = (int)ids[1].size();
+  for (int q = 0; q < n_runs; q++) {
+    DBox &x = hb[ids[1][q] - 1];
+    for (int f = 0; f < nnb; f++)
+      if (x.nb[f] >= 0) {
+        set_err("level 1 must consist of one box per run (coarse_grid_size = box_size)");
+        return AFGPU_ERR_UNSUPPORTED;
+      }
+    const DBox &x0 = hb[ids[1][0] - 1];
+    for (int d = 0; d < D; d++)
+      if (x.r_min[d] != x0.r_min[d] || x.dr[d] != x0.dr[d]) {
+        set_err("afgpu_sync_tree: the runs of a forest must share the level-1 box geometry");
+        return AFGPU_ERR_UNSUPPORTED;
+      }
+    x.run = q;
+  }
+  for (int l = 2; l <= hl; l++)
+    for (int id : ids[l]) {
+      DBox &x = hb[id - 1];
+      if (x.parent < 1) {
+        set_err("box above level 1 without parent");
+        return AFGPU_ERR_ARG;
+      }
+      x.run = hb[x.parent - 1].run;
+    }
+  std::vector<double> min_dr(n_runs, std::numeric_limits<double>::max());
+  std::vector<int> run_of(n_boxes, 0), maxlvl(n_runs, 0);
+  for (int b = 0; b < n_boxes; b++) {
+    const DBox &x = hb[b];
+    run_of[b] = x.run;
+    if (x.lvl <= 0) continue;
+    maxlvl[x.run] = std::max(maxlvl[x.run], x.lvl);
+    for (int d = 0; d < D; d++) min_dr[x.run] = std::min(min_dr[x.run], x.dr[d]);
   }
   // Structural checks that every kernel relies on: a parent has all 2^D
   // children (and they point back), level-1 faces are physical, and every
@@ -1499,6 +1846,60 @@ int afgpu_sync_tree(afgpu_t *h, int n_boxes, const int *lvl, const int *ix, cons
   HIPCHK(hipMemcpy(h->d_boxes, hb.data(), (size_t)n_boxes * sizeof(DBox), hipMemcpyHostToDevice));
   HIPCHK(hipMalloc(&h->d_lists, std::max<size_t>(1, all.size()) * sizeof(int)));
   if (!all.empty()) HIPCHK(hipMemcpy(h->d_lists, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice));
+  // per-run buffers
+  (void)hipFree(h->d_run_of);
+  h->d_run_of = nullptr;
+  HIPCHK(hipMalloc(&h->d_run_of, (size_t)n_boxes * sizeof(int)));
+  HIPCHK(hipMemcpy(h->d_run_of, run_of.data(), (size_t)n_boxes * sizeof(int), hipMemcpyHostToDevice));
+  if (n_runs > h->red_cap_runs) {
+    (void)hipFree(h->d_red);
+    (void)hipFree(h->d_run_volt);
+    (void)hipFree(h->d_run_active);
+    (void)hipFree(h->d_run_host);
+    (void)hipFree(h->d_run_maxlvl);
+    (void)hipFree(h->d_dt_run);
+    if (h->h_red) (void)hipHostFree(h->h_red);
+    if (h->h_run_active) (void)hipHostFree(h->h_run_active);
+    if (h->h_dt_run) (void)hipHostFree(h->h_dt_run);
+    h->d_red = nullptr;
+    h->d_run_volt = h->d_dt_run = nullptr;
+    h->d_run_active = h->d_run_host = h->d_run_maxlvl = nullptr;
+    h->h_red = nullptr;
+    h->h_run_active = nullptr;
+    h->h_dt_run = nullptr;
+    h->red_cap_runs = 0;
+    const size_t nr = (size_t)n_runs;
+    HIPCHK(hipMalloc(&h->d_red, nr * N_RED_SLOTS * sizeof(unsigned long long)));
+    HIPCHK(hipHostMalloc(&h->h_red, nr * N_RED_SLOTS * sizeof(unsigned long long), hipHostMallocDefault));
+    HIPCHK(hipMalloc(&h->d_run_volt, nr * sizeof(double)));
+    HIPCHK(hipMalloc(&h->d_run_active, nr * sizeof(int)));
+    HIPCHK(hipMalloc(&h->d_run_host, nr * sizeof(int)));
+    HIPCHK(hipMalloc(&h->d_run_maxlvl, nr * sizeof(int)));
+    HIPCHK(hipHostMalloc(&h->h_run_active, nr * sizeof(int), hipHostMallocDefault));
+    HIPCHK(hipMalloc(&h->d_dt_run, nr * sizeof(double)));
+    HIPCHK(hipHostMalloc(&h->h_dt_run, nr * sizeof(double), hipHostMallocDefault));
+    h->red_cap_runs = n_runs;
+  }
+  if ((int)h->run_volt.size() != n_runs) h->run_volt.assign(n_runs, h->voltage);
+  HIPCHK(hipMemcpy(h->d_run_volt, h->run_volt.data(), (size_t)n_runs * sizeof(double), hipMemcpyHostToDevice));
+  if ((int)h->rt.size() != n_runs) h->rt.assign(n_runs, afgpu_ctx::RunTime());
+  HIPCHK(hipMemcpy(h->d_run_maxlvl, maxlvl.data(), (size_t)n_runs * sizeof(int), hipMemcpyHostToDevice));
+  {
+    std::vector<int> ones(n_runs, 1);
+    HIPCHK(hipMemcpy(h->d_run_host, ones.data(), (size_t)n_runs * sizeof(int), hipMemcpyHostToDevice));
+  }
+  h->uneven = *std::min_element(maxlvl.begin(), maxlvl.end()) != *std::max_element(maxlvl.begin(), maxlvl.end());
+  h->n_runs = n_runs;
+  h->run_min_dr = min_dr;
+  h->run_leaf_cells.assign(n_runs, 0);
+  for (int l = 1; l <= hl; l++)
+    for (int id : leaves[l]) {
+      long long cells = 1;
+      for (int d = 0; d < D; d++) cells *= h->nc;
+      h->run_leaf_cells[hb[id - 1].run] += cells;
+    }
+  h->masked = false;
+  h->base_active.clear();
   h->hboxes = hb;
   h->h_lists = all;
   h->n_boxes = n_boxes;
@@ -1589,7 +1990,68 @@ int afgpu_set_voltage(afgpu_t *h, double v) {
   if (!h) return AFGPU_ERR_ARG;
   if (v != h->voltage) drop_graphs(*h);  // the boundary value is a captured argument
   h->voltage = v;
+  if (!h->run_volt.empty()) {
+    std::vector<double> all(h->run_volt.size(), v);
+    return afgpu_set_run_voltages(h, (int)all.size(), all.data());
+  }
   return AFGPU_OK;
+}
+
+int afgpu_num_runs(afgpu_t *h) { return h ? h->n_runs : 0; }
+
+int afgpu_set_run_voltages(afgpu_t *h, int n_runs, const double *voltages) {
+  if (!h || !voltages || h->highest_lvl < 1 || n_runs != h->n_runs) {
+    set_err("afgpu_set_run_voltages: invalid arguments (tree synchronised? one value per run)");
+    return AFGPU_ERR_ARG;
+  }
+  RCHK(sync_stream(h));  // kernels in flight may read the previous values
+  h->run_volt.assign(voltages, voltages + n_runs);
+  HIPCHK(hipMemcpy(h->d_run_volt, h->run_volt.data(), (size_t)n_runs * sizeof(double), hipMemcpyHostToDevice));
+  return AFGPU_OK;
+}
+
+int afgpu_field_compute_runs(afgpu_t *h, int s_in, int have_guess, int n_runs, int *n_cycles, double *residual) {
+  RCHK(ready(h));
+  if (s_in < 0 || s_in > 2 || n_runs != h->n_runs) return AFGPU_ERR_ARG;
+  RCHK(h->impl->field_compute(s_in, have_guess != 0, nullptr, nullptr));
+  for (int r = 0; r < n_runs; r++) {
+    if (n_cycles) n_cycles[r] = h->run_ncyc[r];
+    if (residual) residual[r] = h->run_res[r];
+  }
+  return sync_stream(h);
+}
+
+int afgpu_get_run_time_state(afgpu_t *h, int run, int *it, double *time, double *global_dt) {
+  if (!h || run < 0 || run >= (int)h->rt.size()) return AFGPU_ERR_ARG;
+  const afgpu_ctx::RunTime &t = h->rt[run];
+  if (it) *it = t.it;
+  if (time) *time = t.time;
+  if (global_dt) *global_dt = t.started ? t.global_dt : h->p.dt_min;
+  return AFGPU_OK;
+}
+
+int afgpu_set_run_time_state(afgpu_t *h, int run, int it, double time, double global_dt) {
+  if (!h || run < 0 || run >= (int)h->rt.size() || it < 0 || !(global_dt > 0)) return AFGPU_ERR_ARG;
+  afgpu_ctx::RunTime &t = h->rt[run];
+  t.it = it;
+  t.time = t.global_time = time;
+  t.global_dt = t.dt = global_dt;
+  t.started = true;
+  return AFGPU_OK;
+}
+
+int afgpu_step_ensemble(afgpu_t *h, int n_runs, afgpu_step_info_t *info) {
+  RCHK(ready(h, true));
+  if (n_runs != h->n_runs) {
+    set_err("afgpu_step_ensemble: n_runs differs from the forest's run count");
+    return AFGPU_ERR_ARG;
+  }
+  if (h->p.photoi_enabled) {
+    set_err("afgpu_step_ensemble: photoionization is supported for single runs only");
+    return AFGPU_ERR_UNSUPPORTED;
+  }
+  RCHK(h->impl->step_runs(info));
+  return sync_stream(h);
 }
 
 int afgpu_mg_fas_vcycle(afgpu_t *h, int op, int set_residual, int max_lvl) {
@@ -1677,6 +2139,10 @@ int afgpu_advance(afgpu_t *h, double dt, double *dt_lim) {
 
 int afgpu_photoi_set_src(afgpu_t *h) {
   RCHK(ready(h, true));
+  if (h->n_runs > 1) {
+    set_err("afgpu_photoi_set_src: photoionization is supported for single runs only");
+    return AFGPU_ERR_UNSUPPORTED;
+  }
   RCHK(h->impl->photoi());
   return sync_stream(h);
 }
@@ -1695,6 +2161,10 @@ int afgpu_restore_previous_state(afgpu_t *h) {
 
 int afgpu_step(afgpu_t *h, afgpu_step_info_t *info) {
   RCHK(ready(h, true));
+  if (h->n_runs > 1) {
+    set_err("afgpu_step: the tree is a forest of runs (use afgpu_step_ensemble)");
+    return AFGPU_ERR_STATE;
+  }
   if (!h->started) {
     h->global_dt = h->p.dt_min;
     h->dt = h->global_dt;

@@ -16,6 +16,7 @@ namespace afg {
 
 struct DBox {
   int lvl;
+  int run;  // index of the independent run (tree of a forest) the box belongs to
   int parent;
   int ix[3];
   int children[8];
@@ -30,12 +31,25 @@ enum RbType { RB_NONE = 0, RB_MG = 1, RB_INTERP = 2, RB_INTERP_LIM = 3 };
 enum LimType { LIM_MC = 0, LIM_GMINMOD43 = 1 };
 
 // Boundary methods of one variable (bc per face + refinement-boundary rule).
+// In a forest of independent runs, the value of face `run_face` may differ
+// per run: it is then run_val[run] (a per-run applied voltage).
 struct VarBc {
   int type[6];
   double val[6];
   int rb;
   int limiter;
+  int run_face;
+  const double *run_val;
 };
+__device__ inline double bc_value(const VarBc &m, int nb, int run) {
+  return (m.run_val && nb == m.run_face) ? m.run_val[run] : m.val[nb];
+}
+
+// Reduction slots: each run of a forest has RED_STRIDE consecutive slots.
+constexpr int RED_STRIDE = 16;
+__device__ inline unsigned long long *run_slot(unsigned long long *base, const DBox &bx) {
+  return base + (size_t)bx.run * RED_STRIDE;
+}
 
 // Lookup table (src/lookup_table_fortran/m_lookup_table.f90): columns [col][row].
 struct DTable {
@@ -156,6 +170,27 @@ __device__ inline void atomic_min_nonneg(unsigned long long *addr, double v) {
 
 __global__ void k_set_u64(unsigned long long *p, unsigned long long v) {
   if (threadIdx.x == 0) *p = v;
+}
+// The same slot of every run := v.
+__global__ void k_set_run_slots(unsigned long long *p, unsigned long long v, int n_runs) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n_runs) p[(size_t)r * RED_STRIDE] = v;
+}
+// active[r] := host_active[r] and lo <= (highest level of run r) <= hi.
+__global__ void k_runs_in_levels(const int *__restrict__ host_active, const int *__restrict__ maxlvl, int *active,
+                                 int lo, int hi, int n_runs) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n_runs) active[r] = host_active[r] && maxlvl[r] >= lo && maxlvl[r] <= hi;
+}
+// Box lists of a forest: the id of a box whose run is not active is stored
+// negated, and every list kernel skips it; active runs get their ids back.
+__global__ void k_mask_ids(int *ids, int n, const int *__restrict__ run_of, const int *__restrict__ active) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int v = ids[q];
+  const int id = v < 0 ? -v : v;
+  if (id == 0) return;
+  ids[q] = active[run_of[id - 1]] ? id : -id;
 }
 
 template <int BS>

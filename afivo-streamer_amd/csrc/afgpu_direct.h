@@ -19,15 +19,19 @@ namespace afg {
 
 // Interior cells of the level's boxes <-> the contiguous grid.  `map` lists
 // the box id of every box position (bx + nb*(by + nb*bz)), one workgroup
-// per box.
+// per box; in a forest, run r's positions and grid follow those of run r-1.
 template <int D, int NC, int BS, typename T>
 __global__ void __launch_bounds__(BS) k_lc_gather(const int *__restrict__ map, int nb, const T *__restrict__ src,
                                                   double *__restrict__ A) {
   using G = Geo<D, NC>;
   const int b = blockIdx.x;
   const int id = map[b];
-  const int bx = b % nb, by = (b / nb) % nb, bz = (D == 3) ? b / (nb * nb) : 0;
+  if (id <= 0) return;  // run masked out
+  const int per = (D == 3) ? nb * nb * nb : nb * nb;  // boxes of one run's level
+  const int run = b / per, b0 = b % per;
+  const int bx = b0 % nb, by = (b0 / nb) % nb, bz = (D == 3) ? b0 / (nb * nb) : 0;
   const int N = nb * NC;
+  A += (size_t)run * ((D == 3) ? (size_t)N * N * N : (size_t)N * N);
   const T *u = src + (size_t)(id - 1) * G::NCELL;
   for (int q = threadIdx.x; q < G::NINT; q += BS) {
     const int i = q % NC, j = (q / NC) % NC, k = (D == 3) ? q / (NC * NC) : 0;
@@ -42,8 +46,12 @@ __global__ void __launch_bounds__(BS) k_lc_scatter(const int *__restrict__ map, 
   using G = Geo<D, NC>;
   const int b = blockIdx.x;
   const int id = map[b];
-  const int bx = b % nb, by = (b / nb) % nb, bz = (D == 3) ? b / (nb * nb) : 0;
+  if (id <= 0) return;  // run masked out
+  const int per = (D == 3) ? nb * nb * nb : nb * nb;
+  const int run = b / per, b0 = b % per;
+  const int bx = b0 % nb, by = (b0 / nb) % nb, bz = (D == 3) ? b0 / (nb * nb) : 0;
   const int N = nb * NC;
+  A += (size_t)run * ((D == 3) ? (size_t)N * N * N : (size_t)N * N);
   T *u = dst + (size_t)(id - 1) * G::NCELL;
   for (int q = threadIdx.x; q < G::NINT; q += BS) {
     const int i = q % NC, j = (q / NC) % NC, k = (D == 3) ? q / (NC * NC) : 0;
@@ -71,7 +79,7 @@ __global__ void __launch_bounds__(256) k_lc_lines(const double *__restrict__ in,
   const int xc = blockIdx.x % nxc, o = blockIdx.x / nxc;  // o: the other coordinate
   const int x0 = xc * TC;
   const size_t sa = (axis == 1) ? (size_t)N : (size_t)N * N;            // stride of the axis
-  const size_t so = (D == 2) ? 0 : ((axis == 1) ? (size_t)N * N : (size_t)N);  // stride of o
+  const size_t so = (D == 2) ? (size_t)N * N : ((axis == 1) ? (size_t)N * N : (size_t)N);  // stride of o (2D: the run of a forest)
   const size_t base = (size_t)o * so + x0;
   for (int q = threadIdx.x; q < N * TC; q += 256) {
     const int m = q / TC, c = q % TC;
@@ -180,7 +188,7 @@ __global__ void __launch_bounds__(256) k_lc_gemm(const double *__restrict__ in, 
     o = rest / nxc;
     x0 = xc * T;
     sm = (axis == 1) ? (size_t)N : (size_t)N * N;
-    const size_t so = (D == 2) ? 0 : ((axis == 1) ? (size_t)N * N : (size_t)N);
+    const size_t so = (D == 2) ? (size_t)N * N : ((axis == 1) ? (size_t)N * N : (size_t)N);
     sc = 1;
     base = (size_t)o * so + x0;
   }
@@ -291,7 +299,7 @@ __global__ void __launch_bounds__(256) k_lc_mfma(const double *__restrict__ in, 
     o = rest / NT;
     q0 = 0;
     sm = (axis == 1) ? (size_t)N : (size_t)N * N;
-    const size_t so = (D == 2) ? 0 : ((axis == 1) ? (size_t)N * N : (size_t)N);
+    const size_t so = (D == 2) ? (size_t)N * N : ((axis == 1) ? (size_t)N * N : (size_t)N);
     base = (size_t)o * so + x0;
   }
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;

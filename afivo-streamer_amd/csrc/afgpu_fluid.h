@@ -44,7 +44,7 @@ __device__ inline double gc2_value(const DBox *__restrict__ boxes, int id, const
       c0 = 1; c1 = 0; c2 = c0;
     }
     if (D == 3 && nb == 3) c2 = c0;  // the reference's high-y branch (:361)
-    const double bv = m.val[nb];
+    const double bv = bc_value(m, nb, bx.run);
     p[d] = low ? 1 + layer : NC - layer;
     const double x = u[G::idx(p[0], p[1], p[2])];
     return (layer == 0) ? c0 * bv + c1 * x : c2 * bv + c1 * x;
@@ -101,6 +101,7 @@ __global__ void __launch_bounds__(BS)
   __shared__ TF halo[2 * D][2][NT];
   __shared__ double sh[BS / 64];
   const int id = ids[blockIdx.x];
+  if (id <= 0) return;  // run masked out
   const DBox &bx = boxes[id - 1];
   const size_t base = (size_t)(id - 1) * G::NCELL;
   const size_t fbase = (size_t)(id - 1) * D * G::NFACE;
@@ -179,7 +180,7 @@ __global__ void __launch_bounds__(BS)
     smax = fmax(smax, mu * u);
   }
   double smax_d = block_max<BS>((double)smax, sh);
-  if (threadIdx.x == 0) atomic_max_nonneg(sig_max, fmax(smax_d, 1e-100));
+  if (threadIdx.x == 0) atomic_max_nonneg(run_slot(sig_max, bx), fmax(smax_d, 1e-100));
 
   // CFL sum per cell (m_fluid.f90:194-195 summed over dimensions,
   // m_af_flux_schemes.f90:779-848)
@@ -216,7 +217,7 @@ __global__ void __launch_bounds__(BS)
     cmax = fmax(cmax, cfl);
   }
   const double cmax_d = block_max<BS>((double)cmax, sh);
-  if (threadIdx.x == 0) atomic_max_nonneg(cfl_max, cmax_d);
+  if (threadIdx.x == 0) atomic_max_nonneg(run_slot(cfl_max, bx), cmax_d);
 }
 
 // Coarse-leaf faces next to a refined neighbour get the (cylindrically
@@ -228,6 +229,7 @@ __global__ void __launch_bounds__(BS) k_consistent_fluxes(const int *__restrict_
   using G = Geo<D, NC>;
   constexpr int H = NC / 2;
   const int id = ids[blockIdx.x];
+  if (id <= 0) return;  // run masked out
   const DBox &bx = boxes[id - 1];
   for (int q = threadIdx.x; q < 2 * D * G::NTAN; q += BS) {
     const int nb = q / G::NTAN, r = q % G::NTAN;
@@ -292,6 +294,9 @@ struct UpdateArgs {
   const double *flux;
   DTable chem;
   double dt, N_inv;
+  // forest of runs: run r advances by dt_scale * dt_run[r] (dt unused)
+  const double *dt_run;
+  double dt_scale;
   int s_deriv, s_out, n_prev;
   int s_prev[4];
   double w_prev[4];
@@ -312,11 +317,13 @@ __global__ void __launch_bounds__(BS) k_update(const int *__restrict__ ids, cons
   using G = Geo<D, NC>;
   __shared__ double sh[BS / 64];
   const int id = ids[blockIdx.x];
+  if (id <= 0) return;  // run masked out
   const DBox &bx = boxes[id - 1];
   const size_t base = (size_t)(id - 1) * G::NCELL;
   const size_t fbase = (size_t)(id - 1) * D * G::NFACE;
+  const double dt = A.dt_run ? A.dt_scale * A.dt_run[bx.run] : A.dt;
   double dt_dr[3];
-  for (int d = 0; d < D; d++) dt_dr[d] = A.dt / bx.dr[d];
+  for (int d = 0; d < D; d++) dt_dr[d] = dt / bx.dr[d];
   const double eps = 1e-100;
   double bmin = 1e100;
   for (int q = threadIdx.x; q < G::NINT; q += BS) {
@@ -366,7 +373,7 @@ __global__ void __launch_bounds__(BS) k_update(const int *__restrict__ ids, cons
       dd[0] = dd[0] + ph;
       dd[A.photoi_species] = dd[A.photoi_species] + ph;
     }
-    for (int sp = 0; sp < 3; sp++) out[sp] = out[sp] + A.dt * dd[sp];
+    for (int sp = 0; sp < 3; sp++) out[sp] = out[sp] + dt * dd[sp];
     // conservative flux divergence of the electrons
     const double *fl = A.flux + fbase;
     double ne = out[0];
@@ -392,7 +399,7 @@ __global__ void __launch_bounds__(BS) k_update(const int *__restrict__ ids, cons
   }
   if (A.last_step) {
     bmin = block_min<BS>(bmin, sh);
-    if (threadIdx.x == 0) atomic_min_nonneg(chem_min, bmin);
+    if (threadIdx.x == 0) atomic_min_nonneg(run_slot(chem_min, bx), bmin);
   }
 }
 
@@ -400,7 +407,9 @@ __global__ void __launch_bounds__(BS) k_update(const int *__restrict__ ids, cons
 template <int NCELL, int BS>
 __global__ void __launch_bounds__(BS) k_set_rhs(const int *__restrict__ ids, const double *n0, const double *n1,
                                                 const double *n2, double *rhs) {
-  const size_t base = (size_t)(ids[blockIdx.x] - 1) * NCELL;
+  const int id = ids[blockIdx.x];
+  if (id <= 0) return;  // run masked out
+  const size_t base = (size_t)(id - 1) * NCELL;
   const double fac = -UC_elem_charge / UC_eps0;
   const double q0 = -1 * fac, q1 = 1 * fac, q2 = -1 * fac;
   for (int x = threadIdx.x; x < NCELL; x += BS) {
@@ -420,6 +429,7 @@ __global__ void __launch_bounds__(BS) k_gradient(const int *__restrict__ ids, co
                                                  const double *__restrict__ phi, double *efc, double *enorm) {
   using G = Geo<D, NC>;
   const int id = ids[blockIdx.x];
+  if (id <= 0) return;  // run masked out
   const DBox &bx = boxes[id - 1];
   const double *u = phi + (size_t)(id - 1) * G::NCELL;
   double *f = efc + (size_t)(id - 1) * D * G::NFACE;
@@ -461,7 +471,9 @@ __global__ void __launch_bounds__(BS) k_photoi_src(const int *__restrict__ ids, 
                                                    const double *__restrict__ ne, double *rhs, DTable td, double N,
                                                    double coeff) {
   using G = Geo<D, NC>;
-  const size_t base = (size_t)(ids[blockIdx.x] - 1) * G::NCELL;
+  const int id = ids[blockIdx.x];
+  if (id <= 0) return;  // run masked out
+  const size_t base = (size_t)(id - 1) * G::NCELL;
   for (int q = threadIdx.x; q < G::NINT; q += BS) {
     int i, j, k;
     interior_ijk<D, NC>(q, i, j, k);

@@ -128,7 +128,7 @@ __device__ inline T ghost_value(const DBox *__restrict__ boxes, const DBox &bx, 
     } else {
       c0 = 1; c1 = 0; c2 = 0;
     }
-    return c0 * (T)m.val[nb] + c1 * x1 + c2 * x2;
+    return c0 * (T)bc_value(m, nb, bx.run) + c1 * x1 + c2 * x2;
   }
   const int p_nb_id = boxes[bx.parent - 1].nb[nb];
   const T *P = (coarse ? coarse : var) + (size_t)(p_nb_id - 1) * G::NCELL;
@@ -275,6 +275,7 @@ __global__ void __launch_bounds__(BS) k_gc(const int *__restrict__ ids, const DB
                                            VarBc m, int corners, int bpb) {
   using G = Geo<D, NC>;
   const Part pt = part_of<BS>(ids, bpb);
+  if (pt.id <= 0) return;  // run masked out
   const int id = pt.id;
   const DBox &bx = boxes[id - 1];
   T *u = var + (size_t)(id - 1) * G::NCELL;
@@ -314,6 +315,7 @@ __global__ void __launch_bounds__(BS) k_gsrb(const int *__restrict__ ids, const 
                                              int bpb) {
   using G = Geo<D, NC>;
   const Part pt = part_of<BS>(ids, bpb);
+  if (pt.id <= 0) return;  // run masked out
   const int id = pt.id;
   const DBox &bx = boxes[id - 1];
   const Sten<D, T> s = lpl_stencil<D, T>(bx, lambda2);
@@ -373,6 +375,7 @@ __global__ void __launch_bounds__(BS) k_residual(const int *__restrict__ ids, co
   using G = Geo<D, NC>;
   __shared__ double sh[BS / 64];
   const Part pt = part_of<BS>(ids, bpb);
+  if (pt.id <= 0) return;  // run masked out
   const int id = pt.id;
   const DBox &bx = boxes[id - 1];
   const Sten<D, T> s = lpl_stencil<D, T>(bx, lambda2);
@@ -389,7 +392,7 @@ __global__ void __launch_bounds__(BS) k_residual(const int *__restrict__ ids, co
   }
   if (leafmax && bx.children[0] == 0) {
     mx = block_max<BS>(mx, sh);
-    if (threadIdx.x == 0) atomic_max_nonneg(leafmax, mx);
+    if (threadIdx.x == 0) atomic_max_nonneg(run_slot(leafmax, bx), mx);
   }
 }
 
@@ -432,6 +435,7 @@ __global__ void __launch_bounds__(BS) k_restrict(const int *__restrict__ ids, co
   constexpr int H = NC / 2;
   constexpr int NQ = (D == 3) ? H * H * H : H * H;
   const Part pt = part_of<BS>(ids, bpb);
+  if (pt.id <= 0) return;  // run masked out
   const int id = pt.id;
   const DBox &bc = boxes[id - 1];
   const DBox &bp = boxes[bc.parent - 1];
@@ -461,6 +465,7 @@ __global__ void __launch_bounds__(BS) k_restrict_residual(const int *__restrict_
   constexpr int H = NC / 2;
   constexpr int NQ = (D == 3) ? H * H * H : H * H;
   const Part pt = part_of<BS>(ids, bpb);
+  if (pt.id <= 0) return;  // run masked out
   const int id = pt.id;
   const DBox &bc = boxes[id - 1];
   const DBox &bp = boxes[bc.parent - 1];
@@ -514,6 +519,7 @@ __global__ void __launch_bounds__(BS) k_coarse_rhs(const int *__restrict__ ids, 
                                                    int copy_phi, int bpb) {
   using G = Geo<D, NC>;
   const Part pt = part_of<BS>(ids, bpb);
+  if (pt.id <= 0) return;  // run masked out
   const int id = pt.id;
   const DBox &bx = boxes[id - 1];
   const Sten<D, T> s = lpl_stencil<D, T>(bx, lambda2);
@@ -533,6 +539,7 @@ enum BoxOp { OP_COPY = 0, OP_CLEAR = 1, OP_SUB_FROM = 2, OP_AXPY_NEG = 3 };
 template <int NCELL, int BS, typename T>
 __global__ void __launch_bounds__(BS) k_boxop(const int *__restrict__ ids, const T *a, T *b, int op, T s, int bpb) {
   const Part pt = part_of<BS>(ids, bpb);
+  if (pt.id <= 0) return;  // run masked out
   const size_t base = (size_t)(pt.id - 1) * NCELL;
   for (int x = pt.q0; x < NCELL; x += pt.stride) {
     switch (op) {
@@ -553,6 +560,7 @@ __global__ void __launch_bounds__(BS) k_prolong_add(const int *__restrict__ ids,
   using G = Geo<D, NC>;
   constexpr int H = NC / 2;
   const Part pt = part_of<BS>(ids, bpb);
+  if (pt.id <= 0) return;  // run masked out
   const int id = pt.id;
   const DBox &bc = boxes[id - 1];
   const T *P = src + (size_t)(bc.parent - 1) * G::NCELL;
@@ -582,11 +590,12 @@ __global__ void __launch_bounds__(BS) k_prolong_add(const int *__restrict__ ids,
 // max |u| over interior cells of a box list (af_tree_maxabs_cc,
 // afivo/src/m_af_utils.f90:773).
 template <int D, int NC, int BS>
-__global__ void __launch_bounds__(BS) k_maxabs(const int *__restrict__ ids, const double *__restrict__ u,
-                                               unsigned long long *out, int bpb) {
+__global__ void __launch_bounds__(BS) k_maxabs(const int *__restrict__ ids, const DBox *__restrict__ boxes,
+                                               const double *__restrict__ u, unsigned long long *out, int bpb) {
   using G = Geo<D, NC>;
   __shared__ double sh[BS / 64];
   const Part pt = part_of<BS>(ids, bpb);
+  if (pt.id <= 0) return;  // run masked out
   const size_t base = (size_t)(pt.id - 1) * G::NCELL;
   double m = 0;
   for (int q = pt.q0; q < G::NINT; q += pt.stride) {
@@ -595,17 +604,19 @@ __global__ void __launch_bounds__(BS) k_maxabs(const int *__restrict__ ids, cons
     m = fmax(m, fabs(u[base + G::idx(i, j, k)]));
   }
   m = block_max<BS>(m, sh);
-  if (threadIdx.x == 0) atomic_max_nonneg(out, m);
+  if (threadIdx.x == 0) atomic_max_nonneg(run_slot(out, boxes[pt.id - 1]), m);
 }
 
 // Exact level-1 solve by fast diagonalisation (replaces hypre PFMG,
 // afivo/src/m_coarse_solver.f90:421): fold boundary values into the rhs as
 // coarse_solver_set_rhs_phi (:286-338) does, transform with V^-1 along each
 // axis, divide by the eigenvalue sums, transform back with V.  One
-// workgroup of CBS threads; the box lives in LDS.
+// workgroup of CBS threads per level-1 box (one per run of a forest); the
+// box lives in LDS.
 // TS is the storage type of rhs and phi; the transforms always run in FP64.
 template <int D, int NC, int CBS, typename TS>
-__global__ void __launch_bounds__(CBS) k_coarse_solve(int id, const DBox *__restrict__ boxes, const TS *rhs,
+__global__ void __launch_bounds__(CBS) k_coarse_solve(const int *__restrict__ ids, const DBox *__restrict__ boxes,
+                                                      const TS *rhs,
                                                       TS *phi, VarBc m, double lambda2, int cyl,
                                                       const double *__restrict__ vinv,
                                                       const double *__restrict__ v,
@@ -614,6 +625,8 @@ __global__ void __launch_bounds__(CBS) k_coarse_solve(int id, const DBox *__rest
   constexpr int N = G::NINT;
   constexpr int PER = (N + CBS - 1) / CBS;
   __shared__ double buf[N];
+  const int id = ids[blockIdx.x];
+  if (id <= 0) return;  // run masked out
   const DBox &bx = boxes[id - 1];
   const Sten<D, double> s = lpl_stencil<D, double>(bx, lambda2);
   const TS *r = rhs + (size_t)(id - 1) * G::NCELL;
@@ -641,7 +654,7 @@ __global__ void __launch_bounds__(CBS) k_coarse_solve(int id, const DBox *__rest
         f = -2 * cnb;
       else
         f = -(cnb * bx.dr[d]) * (double)(low ? -1 : 1);
-      bval = bval + f * m.val[nb];
+      bval = bval + f * bc_value(m, nb, bx.run);
     }
     buf[lin(i, j, k)] = bval;
   }
@@ -701,6 +714,7 @@ __global__ void __launch_bounds__(CBS) k_coarse_solve(int id, const DBox *__rest
 template <int NCELL, int BS>
 __global__ void __launch_bounds__(BS) k_mp_correct(const int *__restrict__ ids, double *phi, float *e, int bpb) {
   const Part pt = part_of<BS>(ids, bpb);
+  if (pt.id <= 0) return;  // run masked out
   const size_t base = (size_t)(pt.id - 1) * NCELL;
   for (int x = pt.q0; x < NCELL; x += pt.stride) {
     phi[base + x] = phi[base + x] + (double)e[base + x];
@@ -721,6 +735,7 @@ __global__ void __launch_bounds__(BS) k_mp_residual(const int *__restrict__ ids,
   using G = Geo<D, NC>;
   __shared__ double sh[BS / 64];
   const Part pt = part_of<BS>(ids, bpb);
+  if (pt.id <= 0) return;  // run masked out
   const int id = pt.id;
   const DBox &bx = boxes[id - 1];
   const Sten<D, double> s = lpl_stencil<D, double>(bx, lambda2);
@@ -767,7 +782,7 @@ __global__ void __launch_bounds__(BS) k_mp_residual(const int *__restrict__ ids,
   }
   if (bx.children[0] == 0) {
     mx = block_max<BS>(mx, sh);
-    if (threadIdx.x == 0) atomic_max_nonneg(leafmax, mx);
+    if (threadIdx.x == 0) atomic_max_nonneg(run_slot(leafmax, bx), mx);
   }
 }
 
@@ -895,6 +910,7 @@ __global__ void __launch_bounds__(BS) k_gsrb_lds(const int *__restrict__ ids, co
   using G = Geo<D, NC>;
   __shared__ T s[G::NCELL];
   const int id = ids[blockIdx.x];
+  if (id <= 0) return;  // run masked out
   const DBox &bx = boxes[id - 1];
   const Sten<D, T> st = lpl_stencil<D, T>(bx, lambda2);
   T *u = phi + (size_t)(id - 1) * G::NCELL;
@@ -969,6 +985,7 @@ __global__ void __launch_bounds__(BS) k_mp_residual_lds(const int *__restrict__ 
   __shared__ double s[G::NCELL];
   __shared__ double sh[BS / 64];
   const int id = ids[blockIdx.x];
+  if (id <= 0) return;  // run masked out
   const DBox &bx = boxes[id - 1];
   const Sten<D, double> st = lpl_stencil<D, double>(bx, lambda2);
   const size_t base = (size_t)(id - 1) * G::NCELL;
@@ -997,7 +1014,7 @@ __global__ void __launch_bounds__(BS) k_mp_residual_lds(const int *__restrict__ 
   }
   if (bx.children[0] == 0) {
     mx = block_max<BS>(mx, sh);
-    if (threadIdx.x == 0) atomic_max_nonneg(leafmax, mx);
+    if (threadIdx.x == 0) atomic_max_nonneg(run_slot(leafmax, bx), mx);
   }
 }
 
@@ -1013,6 +1030,7 @@ __global__ void __launch_bounds__(BS) k_restrict_residual_lds(const int *__restr
   constexpr int NQ = (D == 3) ? H * H * H : H * H;
   __shared__ T s[G::NCELL];
   const int id = ids[blockIdx.x];
+  if (id <= 0) return;  // run masked out
   const DBox &bc = boxes[id - 1];
   const DBox &bp = boxes[bc.parent - 1];
   const Sten<D, T> st = lpl_stencil<D, T>(bc, lambda2);
@@ -1071,6 +1089,7 @@ __global__ void __launch_bounds__(BS) k_smooth_block(const int *__restrict__ ids
   using G = Geo<D, NC>;
   __shared__ T s[G::NCELL];
   const int id = ids[blockIdx.x];
+  if (id <= 0) return;  // run masked out
   const DBox &bx = boxes[id - 1];
   const Sten<D, T> st = lpl_stencil<D, T>(bx, lambda2);
   const size_t base = (size_t)(id - 1) * G::NCELL;
@@ -1152,7 +1171,9 @@ __global__ void __launch_bounds__(BS) k_smooth_block(const int *__restrict__ ids
 template <int D, int NC, int BS>
 __global__ void __launch_bounds__(BS) k_mp_correct_leaves(const int *__restrict__ ids, double *phi, float *e) {
   using G = Geo<D, NC>;
-  const size_t base = (size_t)(ids[blockIdx.x] - 1) * G::NCELL;
+  const int id = ids[blockIdx.x];
+  if (id <= 0) return;  // run masked out
+  const size_t base = (size_t)(id - 1) * G::NCELL;
   for (int x = threadIdx.x; x < G::NCELL; x += BS) {
     const int i = x % G::NG, j = (x / G::NG) % G::NG, k = (D == 3) ? x / (G::NG * G::NG) : 0;
     const bool inner = i >= 1 && i <= NC && j >= 1 && j <= NC && (D == 2 || (k >= 1 && k <= NC));
@@ -1171,6 +1192,7 @@ __global__ void __launch_bounds__(BS) k_gc_lds(const int *__restrict__ ids, cons
   using G = Geo<D, NC>;
   __shared__ T s[G::NCELL];
   const int id = ids[blockIdx.x];
+  if (id <= 0) return;  // run masked out
   const DBox &bx = boxes[id - 1];
   T *u = var + (size_t)(id - 1) * G::NCELL;
   lds_stage<D, NC, BS, T>(s, boxes, bx, id, var, m, -1);

@@ -205,3 +205,84 @@ def small_amr(D, nc=8, coord=None, uniform_lvl=2, max_lvl=4, frac=0.4, seed=7, v
                      refine_fn=random_refine(max_lvl, frac, seed),
                      name="amr%dd_%s_l%d_s%d" % (D, "cyl" if coord == A.CYL else "xyz", max_lvl, seed),
                      **kw)
+
+
+# ---- ensembles ----------------------------------------------------------------
+def merge_forest(cases):
+    """One forest of independent runs from single-run cases of equal
+    dimension, box size, geometry and level-1 box: the box ids of case r
+    follow those of case r-1 (tree references offset accordingly), so the
+    r-th level-1 box roots run r.  Returns (export, ne0, box offsets)."""
+    D, nc, coord = cases[0].D, cases[0].nc, cases[0].coord
+    keys = ("lvl", "ix", "parent", "children", "neighbors", "nbmat", "r_min", "dr")
+    parts = {k: [] for k in keys}
+    ne0, offs = [], []
+    off = 0
+    for c in cases:
+        if (c.D, c.nc, c.coord) != (D, nc, coord):
+            raise ValueError("runs of a forest must share dimension, box size and geometry")
+        t = c.extra["export"]
+        offs.append(off)
+        for k in keys:
+            a = np.array(t[k], copy=True)
+            if k in ("parent", "children", "neighbors", "nbmat"):
+                a[a > 0] += off  # 0: none / refinement boundary, < 0: physical boundary
+            parts[k].append(a)
+        ne0.append(c.ne0)
+        off += t["n_boxes"]
+    t = {k: np.ascontiguousarray(np.concatenate(parts[k])) for k in keys}
+    t["n_boxes"] = off
+    return t, np.concatenate(ne0), np.array(offs)
+
+
+@dataclass
+class Ensemble:
+    cases: list               # the single-run cases, in run order
+    export: dict              # the merged forest
+    ne0: np.ndarray
+    offsets: np.ndarray       # first box id - 1 of every run
+    voltages: np.ndarray
+    name: str = ""
+
+    def run_ids(self, r):
+        n = self.cases[r].extra["export"]["n_boxes"]
+        return np.arange(self.offsets[r] + 1, self.offsets[r] + n + 1, dtype=np.int32)
+
+
+def ensemble(n_runs=64, levels=(5, 6, 7, 8), v_range=(15e3, 25e3), z_range=(6e-3, 9e-3),
+             seed=20260101, num_vcycles=100, rel_residual=1e-8, level_list=None, **kw):
+    """Independent 2D axisymmetric runs of the config-0 physics (10 mm x
+    10 mm, 8^2 boxes) with uneven sizes: each run's uniform level is drawn
+    by seed from ``levels`` (5..8 -> 128^2 .. 1024^2 cells), its voltage
+    uniformly from ``v_range`` and its seed height uniformly from
+    ``z_range``; ``level_list`` fixes the levels instead.  Each run has its
+    own noise seed."""
+    rng = np.random.default_rng(seed)
+    if level_list is None:
+        level_list = rng.choice(np.asarray(levels), size=n_runs)
+    else:
+        level_list = np.asarray(level_list)
+        n_runs = len(level_list)
+    volts = rng.uniform(v_range[0], v_range[1], size=n_runs)
+    zs = rng.uniform(z_range[0], z_range[1], size=n_runs)
+    cases = []
+    for r in range(n_runs):
+        cases.append(make_case(2, 8, A.CYL, [10e-3, 10e-3], float(volts[r]), int(level_list[r]),
+                               [0.0, float(zs[r])], rng_seed=seed + 1 + r, num_vcycles=num_vcycles,
+                               rel_residual=rel_residual,
+                               name="cyl2d_lvl%d_run%d" % (level_list[r], r), **kw))
+    t, ne0, offs = merge_forest(cases)
+    name = "ensemble_%druns_%dcells" % (n_runs, sum(c.tree.info()["leaf_cells"] for c in cases))
+    return Ensemble(cases, t, ne0, offs, volts, name=name)
+
+
+def apply_ensemble(path, ens):
+    """Load an ensemble into a device-library handle (one forest)."""
+    c0 = ens.cases[0]
+    path.sync_tree(ens.export)
+    path.set_params(c0.params)
+    path.set_tables(c0.tdata)
+    path.set_run_voltages(ens.voltages)
+    for iv in (A.CC_ELECTRON, A.CC_POS_ION):
+        path.put_cc(iv, ens.ne0)
+    return path

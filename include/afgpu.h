@@ -280,6 +280,33 @@ int afgpu_kernel_stats(afgpu_t *h, int kind, int reset, double *total_ms,
                               density update accumulating in FP64 (default 1) */
 int afgpu_set_option(afgpu_t *h, int option, int value);
 
+/* ---- ensembles of independent runs --------------------------------------
+ * The reference runs a parameter study as separate processes, one per run
+ * (programs/parameter_study_2d/run_study.sh:1-5), each executing the time
+ * loop of src/streamer.f90:181-432.  Here a forest synchronised with
+ * afgpu_sync_tree holds many runs in one handle: every level-1 box roots a
+ * run of its own (run r = the r-th level-1 box in id order; the level-1
+ * boxes must share their geometry, their faces all physical boundaries).
+ * Every run keeps the reference's semantics: its own applied voltage,
+ * time step, rejections and stopping tests, so each run computes what it
+ * would compute alone.  Photoionization is supported for single runs only.
+ */
+int afgpu_num_runs(afgpu_t *h);
+/* Applied voltage of every run (afgpu_set_voltage sets all runs). */
+int afgpu_set_run_voltages(afgpu_t *h, int n_runs, const double *voltages);
+/* field_compute for every run; per-run V-cycle (or FMG) counts and final
+ * max |residual| (arrays of n_runs). */
+int afgpu_field_compute_runs(afgpu_t *h, int s_in, int have_guess, int n_runs,
+                             int *n_cycles, double *residual);
+/* One time step of every run (afgpu_step per run); info has n_runs
+ * entries.  afgpu_step itself takes single runs only. */
+int afgpu_step_ensemble(afgpu_t *h, int n_runs, afgpu_step_info_t *info);
+/* Time-loop scalars of run `run` as afgpu_step_ensemble advances them (a
+ * restart of an ensemble, write_sim_data / read_sim_data per run,
+ * src/streamer.f90:541-577). */
+int afgpu_get_run_time_state(afgpu_t *h, int run, int *it, double *time, double *global_dt);
+int afgpu_set_run_time_state(afgpu_t *h, int run, int it, double time, double global_dt);
+
 #ifdef __cplusplus
 }
 #endif

@@ -45,3 +45,32 @@ def test_tables_linear_interpolation_of_input(A, S):
         assert np.abs(col - ref).max() <= 1e-12 * max(np.abs(ref).max(), 1e-300)
     ion = tb["td"][2 * n:3 * n] * tb["td"][:n] * x * 1e-21 * N
     assert np.abs(tb["chem"][:n] - ion).max() <= 1e-12 * ion.max()
+
+
+def test_forest_merge_offsets_every_reference(A, S):
+    """An ensemble's forest is its runs' trees one after the other: box ids,
+    and every positive tree reference, offset by the boxes before the run;
+    boundary codes (0 refinement, < 0 physical) unchanged; the r-th level-1
+    box is run r's root."""
+    ens = S.ensemble(level_list=[2, 3, 1, 2], seed=5)
+    t = ens.export
+    D = 2
+    assert t["n_boxes"] == sum(c.extra["export"]["n_boxes"] for c in ens.cases)
+    roots = np.nonzero(t["lvl"] == 1)[0] + 1
+    assert list(roots) == [o + 1 for o in ens.offsets]
+    width = {"lvl": 1, "ix": D, "parent": 1, "children": 4, "neighbors": 4, "nbmat": 9,
+             "r_min": D, "dr": D}
+    for r, c in enumerate(ens.cases):
+        e = c.extra["export"]
+        o, n = ens.offsets[r], e["n_boxes"]
+        for k, w in width.items():
+            got = t[k][o * w:(o + n) * w]
+            exp = np.array(e[k], copy=True)
+            if k in ("parent", "children", "neighbors", "nbmat"):
+                exp[exp > 0] += o
+            assert np.array_equal(got, exp), (r, k)
+        assert np.array_equal(ens.ne0[o:o + n], c.ne0)
+    # distinct voltages and seeds per run, levels as requested
+    assert len(set(ens.voltages)) == len(ens.cases)
+    assert all(15e3 <= v <= 25e3 for v in ens.voltages)
+    assert [c.tree.info()["highest_lvl"] for c in ens.cases] == [2, 3, 1, 2]
