@@ -130,8 +130,116 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes_per_launch": n * 4, "launches": 5}), flush=True)
 
 
+def ensemble_leaf_interior(ens, r, p, iv):
+    """Interior cells of run r's leaf boxes in an ensemble handle."""
+    c = ens.cases[r]
+    ex = c.extra["export"]
+    ch = ex["children"].reshape(ex["n_boxes"], 4)
+    leaves = np.nonzero(ch[:, 0] == 0)[0] + 1 + ens.offsets[r]
+    a = p.get_cc(iv, ids=leaves).reshape((len(leaves), c.nc + 2, c.nc + 2))
+    return a[:, 1:c.nc + 1, 1:c.nc + 1]
+
+
+def run_ensemble(ens, precision, warmup, steps):
+    p = S.apply_ensemble(A.Path("gpu", 2, 8, A.CYL, precision), ens)
+    t0 = time.time()
+    p.field_compute_runs(0, have_guess=False)
+    log("  initial field solves %.2f s" % (time.time() - t0))
+    for i in range(warmup):
+        infos = p.step_ensemble()
+        log("  warmup", i, "vcycles", [f["n_vcycles"] for f in infos][:16], "...")
+    all_infos = []
+    t0 = time.perf_counter()
+    for i in range(steps):
+        all_infos.append(p.step_ensemble())
+    t1 = time.perf_counter()
+    return p, all_infos, t1 - t0
+
+
+def bench_ensemble(args):
+    """The ensemble workload: independent 2D axisymmetric runs of uneven
+    sizes advancing together in one forest (mixed precision at the given
+    residual), with each run's error against the same runs in FP64."""
+    t0 = time.time()
+    ens = S.ensemble(n_runs=args.runs, num_vcycles=args.max_vcycles, rel_residual=args.rel_residual)
+    sizes = [int(c.tree.info()["leaf_cells"]) for c in ens.cases]
+    log("workload", ens.name, "built in %.1f s; sizes" % (time.time() - t0),
+        {int(np.sqrt(k)): sizes.count(k) for k in sorted(set(sizes))})
+    prec = A.PREC_MIXED if args.precision == "mixed" else A.PREC_FP64
+    p, infos, secs = run_ensemble(ens, prec, args.warmup, args.steps)
+    cell_steps = sum(f["leaf_cells"] for step in infos for f in step)
+    value = cell_steps / secs
+    nv = np.array([[f["n_vcycles"] for f in step] for step in infos])
+    log("  %.3f s for %d steps: %.4e cell-steps/s; vcycles per run-step min %d mean %.2f max %d"
+        % (secs, args.steps, value, nv.min(), nv.mean(), nv.max()))
+    extra = {}
+    if prec == A.PREC_MIXED and not args.no_fp64_ref:
+        log("GPU fp64 ensemble")
+        ref, rinfos, rsecs = run_ensemble(ens, A.PREC_FP64, args.warmup, args.steps)
+        fp64_value = sum(f["leaf_cells"] for step in rinfos for f in step) / rsecs
+        extra["fp64_gpu_value"] = fp64_value
+        extra["mixed_over_fp64"] = value / fp64_value
+        worst = {}
+        per_run = []
+        for r in range(len(ens.cases)):
+            e = {}
+            for name, iv in (("phi", A.CC_PHI), ("efield", A.CC_EFIELD), ("electron", A.CC_ELECTRON)):
+                mx, l2 = rel_err(ensemble_leaf_interior(ens, r, p, iv), ensemble_leaf_interior(ens, r, ref, iv))
+                e[name] = mx
+                worst[name] = max(worst.get(name, 0.0), mx)
+            per_run.append(e)
+        extra["error_vs_fp64"] = {"max_rel_over_runs": worst,
+                                  "per_run_phi": [round(e["phi"], 14) for e in per_run],
+                                  "cells": "leaf interiors of every run after %d steps, max relative "
+                                           "to max |FP64| of that run" % (args.warmup + args.steps)}
+        log("  fp64 %.4e cell-steps/s; mixed/fp64 %.3f; worst errors %s" % (fp64_value, value / fp64_value, worst))
+    cpu = None
+    if not args.no_cpu:
+        # every run of the ensemble, one after the other as the reference's
+        # separate processes would, continued by the oracle for cpu_steps
+        # steps from the GPU state (the FP64 form's when it ran)
+        src = ref if "fp64_gpu_value" in extra else p
+        threads = min(16, os.cpu_count() or 1)
+        os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+        csecs, ccells, nthreads = 0.0, 0, 0
+        for r, case in enumerate(ens.cases):
+            c = A.Path("oracle", 2, 8, A.CYL)
+            S.apply(c, case)
+            ids = ens.run_ids(r)
+            for iv in list(range(9)) + [A.CC_PHI, A.CC_EFIELD]:
+                c.put_cc(iv, src.get_cc(iv, ids))
+            c.put_fc(A.FC_FIELD, src.get_fc(A.FC_FIELD, ids))
+            c.set_time_state(*src.get_run_time_state(r))
+            c.lib.afcpu_num_threads.restype = A.C.c_int
+            nthreads = c.lib.afcpu_num_threads()
+            t0 = time.perf_counter()
+            for _ in range(args.cpu_steps):
+                ccells += c.step()["leaf_cells"]
+            csecs += time.perf_counter() - t0
+            c.close()
+        cpu = {"value": ccells / csecs, "unit": "cell-steps/s", "cores": nthreads, "kind": "port",
+               "sample": "%d step(s) of every run (%d cells) from the GPU FP64 state, %.1f s"
+                         % (args.cpu_steps, int(sum(sizes)), csecs)}
+        log("  cpu %.4e cell-steps/s (%.1f s)" % (cpu["value"], csecs))
+    out = {
+        "metric": "cell-steps/s, ensemble of %d uneven 2D runs (mixed at %g)" % (len(ens.cases), args.rel_residual),
+        "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": secs * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64" if prec == A.PREC_MIXED else "f64", "data": "synthetic",
+        "config": {"workload": ens.name, "runs": len(ens.cases), "cells": int(sum(sizes)),
+                   "box_cells": "8^2", "precision": args.precision, "rel_residual": args.rel_residual,
+                   "mean_vcycles": float(nv.mean())},
+        "roofline": None, "cpu_baseline": cpu,
+    }
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=["amr3d", "ensemble"], default="amr3d",
+                    help="amr3d: the 3D AMR step (the headline metric); ensemble: independent 2D runs")
+    ap.add_argument("--runs", type=int, default=64, help="ensemble: number of runs")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
@@ -169,6 +277,8 @@ def main():
               "(the multi-GPU sharding of the path is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "ensemble":
+        return bench_ensemble(args)
     case = make_case(args)
     info = case.tree.info()
     log("workload", case.name, info)
