@@ -182,7 +182,8 @@ struct afgpu_ctx {
     } f[1 + AFGPU_MAX_MODES];
   } dir;
   // reductions
-  unsigned long long *d_red = nullptr, *h_red = nullptr;
+  unsigned long long *d_red = nullptr, *h_red = nullptr;  // [run][stripe][slot], host: [run][slot]
+  unsigned long long *d_red_out = nullptr;                // stripes folded, [run][slot]
   // time loop
   int it = 0;
   double time = 0, global_time = 0, global_dt = 0, frac_rejected = 0, dt = 0;
@@ -255,9 +256,10 @@ enum GraphKind {
   G_MP_RES_MODES = 8,
   G_MP_FMG_MODES = 9
 };
-// reduction slots (per run): 0 residual max, 1-3 dt limits, 8 + n
-// photoionization mode n
+// reduction slots (per run): 0 residual max, 1-3 dt limits (3: a minimum),
+// 8 + n photoionization mode n
 constexpr int N_RED_SLOTS = RED_STRIDE;
+constexpr unsigned RED_MIN_SLOTS = 1u << 3;
 constexpr int RED_MODE = 8;
 
 int op_phi(int op) { return op == AFGPU_OP_POISSON ? AFGPU_CC_PHI : AFGPU_CC_HELMH + (op - AFGPU_OP_HELMH); }
@@ -984,15 +986,19 @@ struct Impl : ImplBase {
   // read slots 0..n-1 of every run
   int read_red(int n) {
     const size_t cnt = (size_t)(c.n_runs - 1) * N_RED_SLOTS + n;
-    HIPCHK(hipMemcpyAsync(c.h_red, c.d_red, cnt * sizeof(unsigned long long), hipMemcpyDeviceToHost, c.stream));
+    const int nq = c.n_runs * N_RED_SLOTS;
+    hipLaunchKernelGGL(k_fold_stripes, dim3((nq + 255) / 256), dim3(256), 0, c.stream, (const unsigned long long *)c.d_red,
+                       c.d_red_out, c.n_runs, RED_MIN_SLOTS);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c.h_red, c.d_red_out, cnt * sizeof(unsigned long long), hipMemcpyDeviceToHost, c.stream));
     HIPCHK(hipStreamSynchronize(c.stream));
     return timer_harvest(c);
   }
   // reduction slot i of every run := value (a kernel, so that it can live
   // in a graph)
   int set_red(int i, unsigned long long value) {
-    hipLaunchKernelGGL(k_set_run_slots, dim3((c.n_runs + 63) / 64), dim3(64), 0, c.cur, c.d_red + i, value,
-                       c.n_runs);
+    hipLaunchKernelGGL(k_set_run_slots, dim3((c.n_runs * RED_STRIPES + 63) / 64), dim3(64), 0, c.cur, c.d_red + i,
+                       value, c.n_runs);
     HIPCHK(hipGetLastError());
     return AFGPU_OK;
   }
@@ -1601,6 +1607,7 @@ int afgpu_destroy(afgpu_t *h) {
   (void)hipFree(h->d_td);
   (void)hipFree(h->d_chem);
   (void)hipFree(h->d_red);
+  (void)hipFree(h->d_red_out);
   if (h->h_red) (void)hipHostFree(h->h_red);
   (void)hipFree(h->d_run_of);
   (void)hipFree(h->d_run_volt);
@@ -1853,6 +1860,7 @@ int afgpu_sync_tree(afgpu_t *h, int n_boxes, const int *lvl, const int *ix, cons
   HIPCHK(hipMemcpy(h->d_run_of, run_of.data(), (size_t)n_boxes * sizeof(int), hipMemcpyHostToDevice));
   if (n_runs > h->red_cap_runs) {
     (void)hipFree(h->d_red);
+    (void)hipFree(h->d_red_out);
     (void)hipFree(h->d_run_volt);
     (void)hipFree(h->d_run_active);
     (void)hipFree(h->d_run_host);
@@ -1861,7 +1869,7 @@ int afgpu_sync_tree(afgpu_t *h, int n_boxes, const int *lvl, const int *ix, cons
     if (h->h_red) (void)hipHostFree(h->h_red);
     if (h->h_run_active) (void)hipHostFree(h->h_run_active);
     if (h->h_dt_run) (void)hipHostFree(h->h_dt_run);
-    h->d_red = nullptr;
+    h->d_red = h->d_red_out = nullptr;
     h->d_run_volt = h->d_dt_run = nullptr;
     h->d_run_active = h->d_run_host = h->d_run_maxlvl = nullptr;
     h->h_red = nullptr;
@@ -1869,7 +1877,8 @@ int afgpu_sync_tree(afgpu_t *h, int n_boxes, const int *lvl, const int *ix, cons
     h->h_dt_run = nullptr;
     h->red_cap_runs = 0;
     const size_t nr = (size_t)n_runs;
-    HIPCHK(hipMalloc(&h->d_red, nr * N_RED_SLOTS * sizeof(unsigned long long)));
+    HIPCHK(hipMalloc(&h->d_red, nr * RED_STRIPES * N_RED_SLOTS * sizeof(unsigned long long)));
+    HIPCHK(hipMalloc(&h->d_red_out, nr * N_RED_SLOTS * sizeof(unsigned long long)));
     HIPCHK(hipHostMalloc(&h->h_red, nr * N_RED_SLOTS * sizeof(unsigned long long), hipHostMallocDefault));
     HIPCHK(hipMalloc(&h->d_run_volt, nr * sizeof(double)));
     HIPCHK(hipMalloc(&h->d_run_active, nr * sizeof(int)));

@@ -45,10 +45,15 @@ __device__ inline double bc_value(const VarBc &m, int nb, int run) {
   return (m.run_val && nb == m.run_face) ? m.run_val[run] : m.val[nb];
 }
 
-// Reduction slots: each run of a forest has RED_STRIDE consecutive slots.
+// Reduction slots: each run of a forest has RED_STRIDE slots, each kept in
+// RED_STRIPES copies that the workgroups address by their index, so that
+// the one atomic per box of a launch over 10^5 boxes does not serialise on
+// one address; k_fold_stripes combines the copies.  Layout
+// [run][stripe][slot].
 constexpr int RED_STRIDE = 16;
+constexpr int RED_STRIPES = 16;
 __device__ inline unsigned long long *run_slot(unsigned long long *base, const DBox &bx) {
-  return base + (size_t)bx.run * RED_STRIDE;
+  return base + ((size_t)bx.run * RED_STRIPES + (blockIdx.x & (RED_STRIPES - 1))) * RED_STRIDE;
 }
 
 // Lookup table (src/lookup_table_fortran/m_lookup_table.f90): columns [col][row].
@@ -171,10 +176,25 @@ __device__ inline void atomic_min_nonneg(unsigned long long *addr, double v) {
 __global__ void k_set_u64(unsigned long long *p, unsigned long long v) {
   if (threadIdx.x == 0) *p = v;
 }
-// The same slot of every run := v.
+// The same slot of every run (all stripes) := v.
 __global__ void k_set_run_slots(unsigned long long *p, unsigned long long v, int n_runs) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r < n_runs) p[(size_t)r * RED_STRIDE] = v;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n_runs * RED_STRIPES) p[(size_t)q * RED_STRIDE] = v;
+}
+// out[run][slot] := max (min for the slots in min_mask) over the stripes.
+__global__ void k_fold_stripes(const unsigned long long *__restrict__ red, unsigned long long *out, int n_runs,
+                               unsigned min_mask) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n_runs * RED_STRIDE) return;
+  const int r = q / RED_STRIDE, s = q % RED_STRIDE;
+  const bool mn = (min_mask >> s) & 1u;
+  const unsigned long long *p = red + (size_t)r * RED_STRIPES * RED_STRIDE + s;
+  unsigned long long v = p[0];
+  for (int t = 1; t < RED_STRIPES; t++) {
+    const unsigned long long w = p[(size_t)t * RED_STRIDE];
+    v = mn ? (w < v ? w : v) : (w > v ? w : v);
+  }
+  out[q] = v;
 }
 // active[r] := host_active[r] and lo <= (highest level of run r) <= hi.
 __global__ void k_runs_in_levels(const int *__restrict__ host_active, const int *__restrict__ maxlvl, int *active,
