@@ -145,6 +145,7 @@ struct afgpu_ctx {
   std::vector<int> h_lists;
   int *d_lists = nullptr;
   std::vector<List> l_ids, l_leaves, l_parents;
+  std::vector<char> lvl_rb;  // level has boxes with a refinement-boundary face
   List l_all_leaves, l_all_ids, l_refb;
   // pools
   double *cc[AFGPU_N_CC] = {nullptr};
@@ -940,14 +941,19 @@ struct Impl : ImplBase {
       const List &l = c.l_all_leaves;
       RCHK(launch(k_mp_correct_leaves<D, NC, BS>, l.n, c.list(l), c.cc[iphi], f32(op, 0)));
     }
-    // (levels below lc are covered everywhere and not part of the iteration)
-    RCHK(restrict_levels(iphi, c.highest_lvl, lc + 1));
+    // (levels below lc are covered everywhere and not part of the
+    // iteration).  The covered values are read by the residual of coarse
+    // leaf cells beside refined boxes and by the refinement-boundary ghosts
+    // of finer boxes; below the lowest level where either occurs they are
+    // not read before mp_finish (a uniformly refined run needs none).
+    RCHK(restrict_levels(iphi, c.highest_lvl, mp_restrict_lo(lc)));
     // The refinement-boundary ghosts of a level-l box (mg_sides_rb) read
     // the level l-1 neighbour including its stored ghost layer, so that
-    // layer is refreshed on every coarser level, from lc up.  The finest
-    // leaves' own ghosts are formed inside the residual kernel.
+    // layer is refreshed on the coarser levels below such boxes.  The
+    // finest leaves' own ghosts are formed inside the residual kernel.
     const VarBc m = var_methods(c, iphi);
-    for (int lvl = lc; lvl < c.highest_lvl; lvl++) RCHK(gc_var(lvl, c.cc[iphi], m, true));
+    for (int lvl = lc; lvl < c.highest_lvl; lvl++)
+      if (c.lvl_rb[lvl + 1]) RCHK(gc_var(lvl, c.cc[iphi], m, true));
     const int slot = red_slot(op);
     RCHK(set_red(slot, 0ull));
     const List &l = c.l_all_leaves;
@@ -959,6 +965,14 @@ struct Impl : ImplBase {
     return launch(k_mp_residual<D, NC, BS>, l.n * k, c.list(l), (const DBox *)c.d_boxes, (const double *)c.cc[iphi],
                   (const double *)c.cc[AFGPU_CC_RHS], f32(op, 1), c.ops[op].lambda2, (int)(c.coord == AFGPU_CYL),
                   m, c.d_red + slot, k);
+  }
+
+  // Lowest level whose phi mp_close restricts to the level below (levels
+  // lo..highest; highest + 1: none).
+  int mp_restrict_lo(int lc) const {
+    for (int lvl = lc + 1; lvl <= c.highest_lvl; lvl++)
+      if (c.l_leaves[lvl - 1].n > 0 || c.lvl_rb[lvl]) return lvl;
+    return c.highest_lvl + 1;
   }
 
   // phi on refined cells := restriction of the finer phi (no geometry
@@ -973,12 +987,13 @@ struct Impl : ImplBase {
     return AFGPU_OK;
   }
 
-  // End of a mixed-precision solve: the levels below lc get the restricted
-  // solution too, and the stored ghost layers are refreshed everywhere.
+  // End of a mixed-precision solve: the levels that mp_close leaves
+  // unrestricted get the restricted solution too, and the stored ghost
+  // layers are refreshed everywhere.
   int mp_finish(int op, int lc) {
     const int iphi = op_phi(op);
     return run_graph(c, {G_MP_GC, op, lc, 0}, [&] {
-      RCHK(restrict_levels(iphi, lc, 2));
+      RCHK(restrict_levels(iphi, mp_restrict_lo(lc) - 1, 2));
       return gc_tree(iphi);
     });
   }
@@ -1681,12 +1696,14 @@ int afgpu_sync_tree(afgpu_t *h, int n_boxes, const int *lvl, const int *ix, cons
     else
       leaves[x.lvl].push_back(b + 1);
   }
+  std::vector<char> lvl_rb(hl + 2, 0);
   for (int l = 1; l <= hl; l++)
-    for (int id : leaves[l]) {
+    for (int id : ids[l]) {
       const DBox &x = hb[id - 1];
       bool rb = false;
       for (int q = 0; q < nnb; q++) rb |= (x.nb[q] == 0);
-      if (x.parent > 0 && rb) refb.push_back(id);
+      if (rb) lvl_rb[l] = 1;
+      if (rb && x.parent > 0 && x.children[0] <= 0) refb.push_back(id);
     }
   if (hl < 1 || ids[1].empty()) {
     set_err("afgpu_sync_tree: no level-1 box");
@@ -1899,6 +1916,7 @@ int afgpu_sync_tree(afgpu_t *h, int n_boxes, const int *lvl, const int *ix, cons
   }
   h->uneven = *std::min_element(maxlvl.begin(), maxlvl.end()) != *std::max_element(maxlvl.begin(), maxlvl.end());
   h->n_runs = n_runs;
+  h->lvl_rb = lvl_rb;
   h->run_min_dr = min_dr;
   h->run_leaf_cells.assign(n_runs, 0);
   for (int l = 1; l <= hl; l++)

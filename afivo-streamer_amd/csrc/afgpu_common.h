@@ -49,11 +49,12 @@ __device__ inline double bc_value(const VarBc &m, int nb, int run) {
 // RED_STRIPES copies that the workgroups address by their index, so that
 // the one atomic per box of a launch over 10^5 boxes does not serialise on
 // one address; k_fold_stripes combines the copies.  Layout
-// [run][stripe][slot].
+// [run][stripe][slot].  Workgroups are dealt to the 8 XCDs in turn, so the
+// stripe is taken from the index within the XCD (blockIdx / 8).
 constexpr int RED_STRIDE = 16;
-constexpr int RED_STRIPES = 16;
+constexpr int RED_STRIPES = 32;
 __device__ inline unsigned long long *run_slot(unsigned long long *base, const DBox &bx) {
-  return base + ((size_t)bx.run * RED_STRIPES + (blockIdx.x & (RED_STRIPES - 1))) * RED_STRIDE;
+  return base + ((size_t)bx.run * RED_STRIPES + ((blockIdx.x >> 3) & (RED_STRIPES - 1))) * RED_STRIDE;
 }
 
 // Lookup table (src/lookup_table_fortran/m_lookup_table.f90): columns [col][row].
