@@ -94,6 +94,13 @@ class StepInfoT(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ShapeT(C.Structure):
+    _fields_ = [("boxes_per_workgroup", C.c_int), ("workgroups_per_cu", C.c_int), ("ms", C.c_double)]
+
+
+N_KERNEL_CLASSES = 8
+KERNEL_CLASSES = ("gsrb", "residual", "restrict", "prolong", "ghost", "boxop", "flux", "update")
+
 _P = C.c_void_p
 _I = C.c_int
 _D = C.c_double
@@ -140,6 +147,9 @@ _GPU_ONLY = {
     "field_compute_runs": (_I, [_P, _I, _I, _I, _IP, _DP]),
     "step_ensemble": (_I, [_P, _I, C.POINTER(StepInfoT)]),
     "get_run_time_state": (_I, [_P, _I, _IP, _DP, _DP]),
+    "autotune": (_I, [_P, _I, C.POINTER(ShapeT)]),
+    "get_shapes": (_I, [_P, _I, C.POINTER(ShapeT)]),
+    "set_shapes": (_I, [_P, _I, C.POINTER(ShapeT)]),
     "set_run_time_state": (_I, [_P, _I, _I, _D, _D]),
 }
 _ORACLE_ONLY = {
@@ -489,6 +499,26 @@ class Path:
 
     def set_run_time_state(self, run, it, time, global_dt):
         self._call("set_run_time_state", run, it, time, global_dt)
+
+    def autotune(self):
+        """Pick the work-decomposition shape of every kernel class by
+        measurement; returns {class: (boxes per workgroup, workgroups per
+        CU, ms)}."""
+        arr = (ShapeT * N_KERNEL_CLASSES)()
+        self._call("autotune", N_KERNEL_CLASSES, arr)
+        return {KERNEL_CLASSES[k]: (arr[k].boxes_per_workgroup, arr[k].workgroups_per_cu, arr[k].ms)
+                for k in range(N_KERNEL_CLASSES)}
+
+    def get_shapes(self):
+        arr = (ShapeT * N_KERNEL_CLASSES)()
+        self._call("get_shapes", N_KERNEL_CLASSES, arr)
+        return [(arr[k].boxes_per_workgroup, arr[k].workgroups_per_cu) for k in range(N_KERNEL_CLASSES)]
+
+    def set_shapes(self, shapes):
+        arr = (ShapeT * N_KERNEL_CLASSES)()
+        for k, (bb, pc) in enumerate(shapes):
+            arr[k].boxes_per_workgroup, arr[k].workgroups_per_cu = bb, pc
+        self._call("set_shapes", len(shapes), arr)
 
     def step_ensemble(self):
         R = self.num_runs()

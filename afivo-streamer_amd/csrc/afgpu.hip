@@ -54,6 +54,25 @@ struct List {
   int off = 0, n = 0;
 };
 
+// Classes of list kernels that share a work-decomposition shape.
+enum KClass {
+  KC_GSRB = 0,   // half-sweeps
+  KC_RESID,      // residuals (FP64 mixed-precision residual, set_residual)
+  KC_RESTRICT,   // restriction (with residual)
+  KC_PROLONG,    // prolongation, coarse rhs
+  KC_GC,         // ghost layers
+  KC_BOXOP,      // whole-box elementwise operations and reductions
+  KC_FLUX,       // drift-diffusion flux
+  KC_UPDATE,     // density update, field, sources
+  KC_N
+};
+// Shape of a class: BB boxes (work items) per workgroup, and a persistent
+// grid of per_cu workgroups per CU (0: one workgroup per BB items).
+struct Shape {
+  int bb = 1;
+  int per_cu = 0;
+};
+
 // Timing of one kernel class: accumulated from HIP event pairs recorded on
 // the library's stream around each launch (also inside captured graphs,
 // where the pair becomes two event-record nodes replayed with the graph).
@@ -144,6 +163,9 @@ struct afgpu_ctx {
   DBox *d_boxes = nullptr;
   std::vector<int> h_lists;
   int *d_lists = nullptr;
+  int lists_bb = 1;  // items per workgroup slot the list order is made for
+  std::vector<std::vector<int>> src_ids, src_leaves, src_parents;  // per level, id order
+  std::vector<int> src_refb;
   std::vector<List> l_ids, l_leaves, l_parents;
   std::vector<char> lvl_rb;  // level has boxes with a refinement-boundary face
   List l_all_leaves, l_all_ids, l_refb;
@@ -191,11 +213,16 @@ struct afgpu_ctx {
   bool started = false;
   // timing
   bool timing = false;
-  KTimer timers[3];
+  bool class_timing = false;  // time every list-kernel launch under 3 + its class
+  KTimer timers[3 + KC_N];
   std::vector<hipEvent_t> ev_pool;  // events of eager launches, recycled
   size_t ev_used = 0;
   std::vector<TPair> pending;       // recorded, not yet read
   std::vector<TPair> *capture_pairs = nullptr;
+  // work-decomposition shapes of the list-kernel classes (set from the
+  // box size at creation, chosen by afgpu_autotune) and the CU count
+  Shape shape[KC_N];
+  int n_cu = 256;
   // captured launch sequences (HIP graphs), keyed by kind/operator/flags
   bool use_graphs = true;
   std::map<std::array<int, 4>, GraphRec> graphs;
@@ -244,6 +271,48 @@ VarBc var_methods(const afgpu_ctx &c, int iv, bool homogeneous = false) {
     m.rb = RB_NONE;
   }
   return m;
+}
+
+// Device lists, stored XCD-aware: the GPU deals consecutive workgroups to
+// its 8 XCDs in turn, each XCD with its own L2, and a workgroup takes B
+// consecutive items, so item w runs on XCD (w / B) % 8.  The id-ordered
+// list v is permuted so that each XCD works through a contiguous run of
+// boxes (siblings and near neighbours), whose shared faces stay in its L2.
+void build_lists(afgpu_ctx &h, int B) {
+  std::vector<int> &all = h.h_lists;
+  all.clear();
+  auto add = [&](const std::vector<int> &v) {
+    constexpr int X = 8;
+    List l;
+    l.off = (int)all.size();
+    l.n = (int)v.size();
+    const int n = l.n;
+    const int nslots = (n + B - 1) / B;
+    int start[X + 1] = {0};
+    for (int J = 0; J < nslots; J++) start[J % X + 1] += std::min(B, n - J * B);
+    for (int x = 0; x < X; x++) start[x + 1] += start[x];
+    for (int w = 0; w < n; w++) {
+      const int J = w / B;
+      all.push_back(v[start[J % X] + (J / X) * B + w % B]);
+    }
+    return l;
+  };
+  const int hl = (int)h.src_ids.size() - 1;
+  h.l_ids.assign(hl + 1, List());
+  h.l_leaves.assign(hl + 1, List());
+  h.l_parents.assign(hl + 1, List());
+  std::vector<int> all_leaves, all_ids;
+  for (int l = 1; l <= hl; l++) {
+    h.l_ids[l] = add(h.src_ids[l]);
+    h.l_leaves[l] = add(h.src_leaves[l]);
+    h.l_parents[l] = add(h.src_parents[l]);
+    all_leaves.insert(all_leaves.end(), h.src_leaves[l].begin(), h.src_leaves[l].end());
+    all_ids.insert(all_ids.end(), h.src_ids[l].begin(), h.src_ids[l].end());
+  }
+  h.l_all_leaves = add(all_leaves);
+  h.l_all_ids = add(all_ids);
+  h.l_refb = add(h.src_refb);
+  h.lists_bb = B;
 }
 
 enum GraphKind {
@@ -394,6 +463,56 @@ struct Impl : ImplBase {
     return AFGPU_OK;
   }
 
+  // Run f with the boxes-per-workgroup count of class cls as a constant
+  // (several boxes per workgroup only when one box is one wavefront).
+  template <typename F>
+  int with_bb(int cls, F f) {
+    if constexpr (BS == 64) {
+      switch (c.shape[cls].bb) {
+        case 2: return f(std::integral_constant<int, 2>());
+        case 4: return f(std::integral_constant<int, 4>());
+        case 8: return f(std::integral_constant<int, 8>());
+        default: return f(std::integral_constant<int, 1>());
+      }
+    } else {
+      return f(std::integral_constant<int, 1>());
+    }
+  }
+  int grid_for(int cls, int nw, int bb) const {
+    int g = (nw + bb - 1) / bb;
+    const int cap = c.shape[cls].per_cu * c.n_cu;
+    return cap > 0 ? std::min(g, cap) : g;
+  }
+  // A list kernel (template argument BB) over nw work items, in the shape of
+  // its class; the item count is the kernel's last argument.
+  template <int BB, typename K, typename... Args>
+  int items(int cls, K kernel, int nw, Args... args) {
+    if (c.class_timing && c.timing && !c.capture_pairs) return items_timed<BB>(3 + cls, 0.0, cls, kernel, nw, args...);
+    if (nw <= 0) return AFGPU_OK;
+    hipLaunchKernelGGL(kernel, dim3(grid_for(cls, nw, BB)), dim3(BS * BB), 0, c.cur, args..., nw);
+    HIPCHK(hipGetLastError());
+    return AFGPU_OK;
+  }
+  // items() with the kernel's own execution timed under `kind` when kernel
+  // timing is on (eager runs only; under the kernel class instead while the
+  // classes are timed)
+  template <int BB, typename K, typename... Args>
+  int items_timed(int kind, double bytes, int cls, K kernel, int nw, Args... args) {
+    if (!c.timing || c.capture_pairs) {
+      if (nw <= 0) return AFGPU_OK;
+      hipLaunchKernelGGL(kernel, dim3(grid_for(cls, nw, BB)), dim3(BS * BB), 0, c.cur, args..., nw);
+      HIPCHK(hipGetLastError());
+      return AFGPU_OK;
+    }
+    if (nw <= 0) return AFGPU_OK;
+    if (c.class_timing) kind = 3 + cls;
+    hipEvent_t a, b;
+    RCHK(timer_events(c, kind, bytes, &a, &b));
+    hipExtLaunchKernelGGL(kernel, dim3(grid_for(cls, nw, BB)), dim3(BS * BB), 0, c.cur, a, b, 0, args..., nw);
+    HIPCHK(hipGetLastError());
+    return AFGPU_OK;
+  }
+
   // Workgroups per box: 1 on large levels; on levels with few boxes the
   // box's work items are spread over several workgroups.
   static int bpb(int n_boxes, int items) {
@@ -406,15 +525,21 @@ struct Impl : ImplBase {
   template <typename T>
   int boxop(const List &l, const T *a, T *b, int op, T s = T(0)) {
     const int k = bpb(l.n, G::NCELL);
-    return launch(k_boxop<G::NCELL, BS, T>, l.n * k, c.list(l), a, b, op, s, k);
+    return with_bb(KC_BOXOP, [&](auto B) {
+      constexpr int bb = decltype(B)::value;
+      return items<bb>(KC_BOXOP, k_boxop<G::NCELL, BS, T, bb>, l.n * k, c.list(l), a, b, op, s, k);
+    });
   }
 
   template <typename T>
   int gc_var(int lvl, T *var, const VarBc &m, bool corners) {
     const List &l = c.l_ids[lvl];
     const int k = corners ? 1 : bpb(l.n, 2 * D * G::NTAN);
-    if (k == 1) return launch(k_gc_lds<D, NC, BS, T>, l.n, c.list(l), (const DBox *)c.d_boxes, var, m, (int)corners);
-    return launch(k_gc<D, NC, BS, T>, l.n * k, c.list(l), (const DBox *)c.d_boxes, var, m, (int)corners, k);
+    if (k == 1) return with_bb(KC_GC, [&](auto B) {
+      constexpr int bb = decltype(B)::value;
+      return items<bb>(KC_GC, k_gc_lds<D, NC, BS, T, bb>, l.n, c.list(l), (const DBox *)c.d_boxes, var, m, (int)corners);
+    });
+    return items<1>(KC_GC, k_gc<D, NC, BS, T, 1>, l.n * k, c.list(l), (const DBox *)c.d_boxes, var, m, (int)corners, k);
   }
   int gc_lvl(int lvl, int iv, bool corners) override { return gc_var(lvl, c.cc[iv], var_methods(c, iv), corners); }
   int gc_tree(int iv) {
@@ -483,11 +608,17 @@ struct Impl : ImplBase {
     }
     for (int n = 1; n <= 2 * n_cycle; n++) {
       if (k == 1)
-        RCHK(launch_timed(0, bytes, k_gsrb_lds<D, NC, BS, T>, l.n, c.list(l), (const DBox *)c.d_boxes, v.phi,
-                          (const T *)v.rhs, n, v.lambda2, cyl, v.m));
+        RCHK(with_bb(KC_GSRB, [&](auto B) {
+      constexpr int bb = decltype(B)::value;
+      return items_timed<bb>(0, bytes, KC_GSRB, k_gsrb_lds<D, NC, BS, T, bb>, l.n, c.list(l), (const DBox *)c.d_boxes, v.phi,
+                          (const T *)v.rhs, n, v.lambda2, cyl, v.m);
+    }));
       else
-        RCHK(launch_timed(0, bytes, k_gsrb<D, NC, BS, T>, l.n * k, c.list(l), (const DBox *)c.d_boxes, v.phi,
-                          (const T *)v.rhs, n, v.lambda2, cyl, v.m, k));
+        RCHK(with_bb(KC_GSRB, [&](auto B) {
+      constexpr int bb = decltype(B)::value;
+      return items_timed<bb>(0, bytes, KC_GSRB, k_gsrb<D, NC, BS, T, bb>, l.n * k, c.list(l), (const DBox *)c.d_boxes, v.phi,
+                          (const T *)v.rhs, n, v.lambda2, cyl, v.m, k);
+    }));
     }
     // ghost layer after the last half-sweep (corners on the final up-sweep)
     return final_gc ? gc_var(lvl, v.phi, v.m, !down) : AFGPU_OK;
@@ -510,22 +641,31 @@ struct Impl : ImplBase {
     constexpr int NQ = G::NINT / (1 << D);
     const int k = bpb(l.n, NQ);
     if (restrict_in_lds<T>(lvl)) {
-      RCHK(launch(k_restrict_residual_lds<D, NC, BS, T>, l.n, c.list(l), (const DBox *)c.d_boxes,
+      RCHK(with_bb(KC_RESTRICT, [&](auto B) {
+      constexpr int bb = decltype(B)::value;
+      return items<bb>(KC_RESTRICT, k_restrict_residual_lds<D, NC, BS, T, bb>, l.n, c.list(l), (const DBox *)c.d_boxes,
                   (const T *)level_buf(v, lvl, v_cycle), v.phi, (const T *)v.rhs, v.tmp, v.lambda2, cyl,
-                  (int)!v_cycle, v.m));
+                  (int)!v_cycle, v.m);
+    }));
     } else {
       // (a forest whose runs end on different levels fills every level: a
       // run's own finest level needs it, and for the other runs the fill
       // repeats the one the level above has just made, with the same values)
       if (!v_cycle && (lvl == c.highest_lvl || c.uneven)) RCHK(gc_var(lvl, v.phi, v.m, true));
-      RCHK(launch(k_restrict_residual<D, NC, BS, T>, l.n * k, c.list(l), (const DBox *)c.d_boxes, v.phi,
-                  (const T *)v.rhs, v.tmp, v.lambda2, cyl, (int)!v_cycle, k));
+      RCHK(with_bb(KC_RESTRICT, [&](auto B) {
+      constexpr int bb = decltype(B)::value;
+      return items<bb>(KC_RESTRICT, k_restrict_residual<D, NC, BS, T, bb>, l.n * k, c.list(l), (const DBox *)c.d_boxes, v.phi,
+                  (const T *)v.rhs, v.tmp, v.lambda2, cyl, (int)!v_cycle, k);
+    }));
     }
     RCHK(gc_var(lvl - 1, v.phi, v.m, true));
     const List &pl = c.l_parents[lvl - 1];
     const int k2 = bpb(pl.n, G::NCELL);
-    return launch(k_coarse_rhs<D, NC, BS, T>, pl.n * k2, c.list(pl), (const DBox *)c.d_boxes, (const T *)v.phi,
+    return with_bb(KC_PROLONG, [&](auto B) {
+      constexpr int bb = decltype(B)::value;
+      return items<bb>(KC_PROLONG, k_coarse_rhs<D, NC, BS, T, bb>, pl.n * k2, c.list(pl), (const DBox *)c.d_boxes, (const T *)v.phi,
                   v.rhs, v.tmp, v.lambda2, cyl, (int)v_cycle, k2);
+    });
   }
 
   // (target: the buffer holding the children's level, see use_block)
@@ -534,8 +674,11 @@ struct Impl : ImplBase {
     RCHK(boxop<T>(c.l_parents[lvl_p], v.phi, v.tmp, OP_SUB_FROM));
     const List &l = c.l_ids[lvl_p + 1];
     const int k = bpb(l.n, G::NINT);
-    return launch(k_prolong_add<D, NC, BS, T>, l.n * k, c.list(l), (const DBox *)c.d_boxes, (const T *)v.tmp,
+    return with_bb(KC_PROLONG, [&](auto B) {
+      constexpr int bb = decltype(B)::value;
+      return items<bb>(KC_PROLONG, k_prolong_add<D, NC, BS, T, bb>, l.n * k, c.list(l), (const DBox *)c.d_boxes, (const T *)v.tmp,
                   target ? target : v.phi, k);
+    });
   }
 
   int ensure_factors(int op) {
@@ -616,8 +759,11 @@ struct Impl : ImplBase {
     for (int lvl = 1; lvl <= max_lvl; lvl++) {
       const List &l = c.l_ids[lvl];
       const int k = bpb(l.n, G::NINT);
-      RCHK(launch(k_residual<D, NC, BS, T>, l.n * k, c.list(l), (const DBox *)c.d_boxes, (const T *)v.phi,
-                  (const T *)v.rhs, v.tmp, v.lambda2, cyl, c.d_red, k));
+      RCHK(with_bb(KC_RESID, [&](auto B) {
+      constexpr int bb = decltype(B)::value;
+      return items<bb>(KC_RESID, k_residual<D, NC, BS, T, bb>, l.n * k, c.list(l), (const DBox *)c.d_boxes, (const T *)v.phi,
+                  (const T *)v.rhs, v.tmp, v.lambda2, cyl, c.d_red, k);
+    }));
     }
     return AFGPU_OK;
   }
@@ -692,8 +838,11 @@ struct Impl : ImplBase {
         const List &l = c.l_ids[lvl];
         RCHK(boxop<T>(l, v.phi, v.phi, OP_CLEAR));
         const int k = bpb(l.n, G::NINT / (1 << D));
-        RCHK(launch(k_restrict<D, NC, BS, T>, l.n * k, c.list(l), (const DBox *)c.d_boxes, (const T *)v.rhs, v.rhs,
-                    1, (int)(c.coord == AFGPU_CYL), k));
+        RCHK(with_bb(KC_RESTRICT, [&](auto B) {
+      constexpr int bb = decltype(B)::value;
+      return items<bb>(KC_RESTRICT, k_restrict<D, NC, BS, T, bb>, l.n * k, c.list(l), (const DBox *)c.d_boxes, (const T *)v.rhs, v.rhs,
+                    1, (int)(c.coord == AFGPU_CYL), k);
+    }));
       }
     }
     // the single-precision correction starts from zero on its coarsest level
@@ -939,7 +1088,10 @@ struct Impl : ImplBase {
     const int iphi = op_phi(op);
     if (correct) {
       const List &l = c.l_all_leaves;
-      RCHK(launch(k_mp_correct_leaves<D, NC, BS>, l.n, c.list(l), c.cc[iphi], f32(op, 0)));
+      RCHK(with_bb(KC_BOXOP, [&](auto B) {
+      constexpr int bb = decltype(B)::value;
+      return items<bb>(KC_BOXOP, k_mp_correct_leaves<D, NC, BS, bb>, l.n, c.list(l), c.cc[iphi], f32(op, 0));
+    }));
     }
     // (levels below lc are covered everywhere and not part of the
     // iteration).  The covered values are read by the residual of coarse
@@ -959,12 +1111,18 @@ struct Impl : ImplBase {
     const List &l = c.l_all_leaves;
     const int k = bpb(l.n, G::NINT);
     if (k == 1)
-      return launch(k_mp_residual_lds<D, NC, BS>, l.n, c.list(l), (const DBox *)c.d_boxes,
+      return with_bb(KC_RESID, [&](auto B) {
+      constexpr int bb = decltype(B)::value;
+      return items<bb>(KC_RESID, k_mp_residual_lds<D, NC, BS, bb>, l.n, c.list(l), (const DBox *)c.d_boxes,
                     (const double *)c.cc[iphi], (const double *)c.cc[AFGPU_CC_RHS], f32(op, 1), c.ops[op].lambda2,
                     (int)(c.coord == AFGPU_CYL), m, c.d_red + slot);
-    return launch(k_mp_residual<D, NC, BS>, l.n * k, c.list(l), (const DBox *)c.d_boxes, (const double *)c.cc[iphi],
+    });
+    return with_bb(KC_RESID, [&](auto B) {
+      constexpr int bb = decltype(B)::value;
+      return items<bb>(KC_RESID, k_mp_residual<D, NC, BS, bb>, l.n * k, c.list(l), (const DBox *)c.d_boxes, (const double *)c.cc[iphi],
                   (const double *)c.cc[AFGPU_CC_RHS], f32(op, 1), c.ops[op].lambda2, (int)(c.coord == AFGPU_CYL),
                   m, c.d_red + slot, k);
+    });
   }
 
   // Lowest level whose phi mp_close restricts to the level below (levels
@@ -981,8 +1139,11 @@ struct Impl : ImplBase {
     for (int lvl = hi; lvl >= lo; lvl--) {
       const List &l = c.l_ids[lvl];
       const int k = bpb(l.n, G::NINT / (1 << D));
-      RCHK(launch(k_restrict<D, NC, BS, double>, l.n * k, c.list(l), (const DBox *)c.d_boxes,
-                  (const double *)c.cc[iphi], c.cc[iphi], 0, (int)(c.coord == AFGPU_CYL), k));
+      RCHK(with_bb(KC_RESTRICT, [&](auto B) {
+      constexpr int bb = decltype(B)::value;
+      return items<bb>(KC_RESTRICT, k_restrict<D, NC, BS, double, bb>, l.n * k, c.list(l), (const DBox *)c.d_boxes,
+                  (const double *)c.cc[iphi], c.cc[iphi], 0, (int)(c.coord == AFGPU_CYL), k);
+    }));
     }
     return AFGPU_OK;
   }
@@ -1027,8 +1188,11 @@ struct Impl : ImplBase {
   int maxabs_runs(int iv, std::vector<double> &out) {
     RCHK(set_red(0, 0ull));
     const int k = bpb(c.l_all_leaves.n, G::NINT);
-    RCHK(launch(k_maxabs<D, NC, BS>, c.l_all_leaves.n * k, c.list(c.l_all_leaves), (const DBox *)c.d_boxes,
-                (const double *)c.cc[iv], c.d_red, k));
+    RCHK(with_bb(KC_BOXOP, [&](auto B) {
+      constexpr int bb = decltype(B)::value;
+      return items<bb>(KC_BOXOP, k_maxabs<D, NC, BS, bb>, c.l_all_leaves.n * k, c.list(c.l_all_leaves), (const DBox *)c.d_boxes,
+                (const double *)c.cc[iv], c.d_red, k);
+    }));
     RCHK(read_red(1));
     out.assign(c.n_runs, 0.0);
     for (int r = 0; r < c.n_runs; r++)
@@ -1045,8 +1209,11 @@ struct Impl : ImplBase {
   // --- field (src/m_field.f90) ----------------------------------------------
   int field_from_potential() override {
     return run_graph(c, {G_FFP, 0, 0, 0}, [&] {
-      RCHK(launch(k_gradient<D, NC, BS>, c.l_all_ids.n, c.list(c.l_all_ids), (const DBox *)c.d_boxes,
-                  (const double *)c.cc[AFGPU_CC_PHI], c.fc[AFGPU_FC_FIELD], c.cc[AFGPU_CC_EFIELD]));
+      RCHK(with_bb(KC_UPDATE, [&](auto B) {
+      constexpr int bb = decltype(B)::value;
+      return items<bb>(KC_UPDATE, k_gradient<D, NC, BS, bb>, c.l_all_ids.n, c.list(c.l_all_ids), (const DBox *)c.d_boxes,
+                  (const double *)c.cc[AFGPU_CC_PHI], c.fc[AFGPU_FC_FIELD], c.cc[AFGPU_CC_EFIELD]);
+    }));
       return gc_tree(AFGPU_CC_EFIELD);
     });
   }
@@ -1058,9 +1225,12 @@ struct Impl : ImplBase {
   int field_compute(int s_in, bool have_guess, int *n_out, double *res_out) override {
     const int R = c.n_runs;
     RCHK(set_active(nullptr));
-    RCHK(launch(k_set_rhs<G::NCELL, BS>, c.l_all_leaves.n, c.list(c.l_all_leaves),
+    RCHK(with_bb(KC_BOXOP, [&](auto B) {
+      constexpr int bb = decltype(B)::value;
+      return items<bb>(KC_BOXOP, k_set_rhs<G::NCELL, BS, bb>, c.l_all_leaves.n, c.list(c.l_all_leaves),
                 (const double *)c.cc[AFGPU_CC_ELECTRON + s_in], (const double *)c.cc[AFGPU_CC_POS_ION + s_in],
-                (const double *)c.cc[AFGPU_CC_NEG_ION + s_in], c.cc[AFGPU_CC_RHS]));
+                (const double *)c.cc[AFGPU_CC_NEG_ION + s_in], c.cc[AFGPU_CC_RHS]);
+    }));
     std::vector<double> max_rhs;
     RCHK(maxabs_runs(AFGPU_CC_RHS, max_rhs));
     const double conv_fac = 1e-10, min_residual = 1e-6, max_residual = 1e8;
@@ -1182,8 +1352,11 @@ struct Impl : ImplBase {
     const int cyl = c.coord == AFGPU_CYL;
     const int iv_e = AFGPU_CC_ELECTRON + s_deriv;
     // af_restrict_ref_boundary (afivo/src/m_af_restrict.f90:140-161)
-    RCHK(launch(k_restrict<D, NC, BS, double>, c.l_refb.n, c.list(c.l_refb), (const DBox *)c.d_boxes,
-                (const double *)c.cc[iv_e], c.cc[iv_e], 1, cyl, 1));
+    RCHK(with_bb(KC_RESTRICT, [&](auto B) {
+      constexpr int bb = decltype(B)::value;
+      return items<bb>(KC_RESTRICT, k_restrict<D, NC, BS, double, bb>, c.l_refb.n, c.list(c.l_refb), (const DBox *)c.d_boxes,
+                (const double *)c.cc[iv_e], c.cc[iv_e], 1, cyl, 1);
+    }));
     const double big = 1e100;
     unsigned long long bigbits;
     std::memcpy(&bigbits, &big, sizeof(big));
@@ -1196,18 +1369,27 @@ struct Impl : ImplBase {
       const List &l = c.l_leaves[lvl];
       const double fbytes = (2.0 + 2 * D) * 8.0 * l.n * G::NINT;
       if (c.precision == AFGPU_PREC_MIXED && c.flux32)
-        RCHK(launch_timed(1, fbytes, k_flux<D, NC, BS, float>, l.n, c.list(l), (const DBox *)c.d_boxes, c.cc[iv_e],
+        RCHK(with_bb(KC_FLUX, [&](auto B) {
+      constexpr int bb = decltype(B)::value;
+      return items_timed<bb>(1, fbytes, KC_FLUX, k_flux<D, NC, BS, float, bb>, l.n, c.list(l), (const DBox *)c.d_boxes, c.cc[iv_e],
                           (const double *)c.cc[AFGPU_CC_EFIELD], (const double *)c.fc[AFGPU_FC_FIELD],
-                          c.fc[AFGPU_FC_FLUX], msp, c.td, N_inv, c.d_red + 1, c.d_red + 2));
+                          c.fc[AFGPU_FC_FLUX], msp, c.td, N_inv, c.d_red + 1, c.d_red + 2);
+    }));
       else
-        RCHK(launch_timed(1, fbytes, k_flux<D, NC, BS, double>, l.n, c.list(l), (const DBox *)c.d_boxes, c.cc[iv_e],
+        RCHK(with_bb(KC_FLUX, [&](auto B) {
+      constexpr int bb = decltype(B)::value;
+      return items_timed<bb>(1, fbytes, KC_FLUX, k_flux<D, NC, BS, double, bb>, l.n, c.list(l), (const DBox *)c.d_boxes, c.cc[iv_e],
                           (const double *)c.cc[AFGPU_CC_EFIELD], (const double *)c.fc[AFGPU_FC_FIELD],
-                          c.fc[AFGPU_FC_FLUX], msp, c.td, N_inv, c.d_red + 1, c.d_red + 2));
+                          c.fc[AFGPU_FC_FLUX], msp, c.td, N_inv, c.d_red + 1, c.d_red + 2);
+    }));
     }
     for (int lvl = 1; lvl <= c.highest_lvl - 1; lvl++) {
       const List &l = c.l_parents[lvl];
-      RCHK(launch(k_consistent_fluxes<D, NC, BS>, l.n, c.list(l), (const DBox *)c.d_boxes, c.fc[AFGPU_FC_FLUX],
-                  cyl));
+      RCHK(with_bb(KC_UPDATE, [&](auto B) {
+      constexpr int bb = decltype(B)::value;
+      return items<bb>(KC_UPDATE, k_consistent_fluxes<D, NC, BS, bb>, l.n, c.list(l), (const DBox *)c.d_boxes, c.fc[AFGPU_FC_FLUX],
+                  cyl);
+    }));
     }
     UpdateArgs A{};
     for (int sp = 0; sp < 3; sp++)
@@ -1234,9 +1416,11 @@ struct Impl : ImplBase {
     A.photoi_enabled = c.p.photoi_enabled;
     A.photoi_species = c.p.photoi_species;
     A.cyl = cyl;
-    RCHK(launch_timed(2, (7.0 + D + (c.p.photoi_enabled ? 1 : 0)) * 8.0 * c.l_all_leaves.n * G::NINT,
-                      k_update<D, NC, BS>, c.l_all_leaves.n, c.list(c.l_all_leaves), (const DBox *)c.d_boxes, A,
-                      c.d_red + 3));
+    RCHK(with_bb(KC_UPDATE, [&](auto B) {
+      constexpr int bb = decltype(B)::value;
+      return items_timed<bb>(2, (7.0 + D + (c.p.photoi_enabled ? 1 : 0)) * 8.0 * c.l_all_leaves.n * G::NINT, KC_UPDATE, k_update<D, NC, BS, bb>, c.l_all_leaves.n, c.list(c.l_all_leaves), (const DBox *)c.d_boxes, A,
+                      c.d_red + 3);
+    }));
     RCHK(read_red(4));
     // per run (only the runs not masked out are meaningful)
     c.run_dt_lim.assign(c.n_runs, 1e100);
@@ -1312,9 +1496,12 @@ struct Impl : ImplBase {
   }
 
   int photoi() override {
-    RCHK(launch(k_photoi_src<D, NC, BS>, c.l_all_leaves.n, c.list(c.l_all_leaves),
+    RCHK(with_bb(KC_UPDATE, [&](auto B) {
+      constexpr int bb = decltype(B)::value;
+      return items<bb>(KC_UPDATE, k_photoi_src<D, NC, BS, bb>, c.l_all_leaves.n, c.list(c.l_all_leaves),
                 (const double *)c.cc[AFGPU_CC_EFIELD], (const double *)c.cc[AFGPU_CC_ELECTRON], c.cc[AFGPU_CC_RHS],
-                c.td, c.p.gas_number_density, c.p.photoi_eta_quench));
+                c.td, c.p.gas_number_density, c.p.photoi_eta_quench);
+    }));
     RCHK(boxop(c.l_all_ids, c.cc[AFGPU_CC_PHOTO], c.cc[AFGPU_CC_PHOTO], OP_CLEAR));
     double max_rhs;
     RCHK(maxabs(AFGPU_CC_RHS, &max_rhs));
@@ -1513,8 +1700,11 @@ struct Impl : ImplBase {
   int restrict_tree(int iv) override {
     for (int lvl = c.highest_lvl; lvl >= 2; lvl--) {
       const List &l = c.l_ids[lvl];
-      RCHK(launch(k_restrict<D, NC, BS, double>, l.n, c.list(l), (const DBox *)c.d_boxes, (const double *)c.cc[iv],
-                  c.cc[iv], 1, (int)(c.coord == AFGPU_CYL), 1));
+      RCHK(with_bb(KC_RESTRICT, [&](auto B) {
+      constexpr int bb = decltype(B)::value;
+      return items<bb>(KC_RESTRICT, k_restrict<D, NC, BS, double, bb>, l.n, c.list(l), (const DBox *)c.d_boxes, (const double *)c.cc[iv],
+                  c.cc[iv], 1, (int)(c.coord == AFGPU_CYL), 1);
+    }));
     }
     return AFGPU_OK;
   }
@@ -1588,6 +1778,17 @@ int afgpu_create(int ndim, int n_cell, int coord, int precision, afgpu_t **out) 
     set_err("no GPU device");
     return AFGPU_ERR_HIP;
   }
+  {
+    int dev = 0, ncu = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    if (ncu > 0) h->n_cu = ncu;
+  }
+  // Default shapes: a box of 8 x 8 cells is one wavefront, and streaming
+  // such boxes four to a workgroup from a persistent grid of 8 workgroups
+  // per CU moves ~1.6x the bytes per second of one box per workgroup; larger
+  // boxes fill a workgroup each.  afgpu_autotune measures the alternatives.
+  for (auto &sh : h->shape) sh = (ndim == 2 && n_cell == 8) ? Shape{4, 8} : Shape{1, 0};
   HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   h->cur = h->stream;
   for (int n = 0; n < AFGPU_MAX_MODES; n++) {
@@ -1792,38 +1993,12 @@ int afgpu_sync_tree(afgpu_t *h, int n_boxes, const int *lvl, const int *ix, cons
           return AFGPU_ERR_ARG;
         }
     }
-  // Lists are stored XCD-aware: the GPU hands consecutive workgroups to its
-  // 8 XCDs in turn, each XCD with its own L2, so workgroup j takes box
-  // v[start(j % 8) + j / 8] of the id-ordered list v.  Each XCD then works
-  // through a contiguous run of boxes (siblings and near neighbours), whose
-  // shared faces stay in its L2.
-  std::vector<int> all;
-  auto add = [&](const std::vector<int> &v) {
-    constexpr int X = 8;
-    List l;
-    l.off = (int)all.size();
-    l.n = (int)v.size();
-    const int n = l.n;
-    int start[X + 1];
-    start[0] = 0;
-    for (int x = 0; x < X; x++) start[x + 1] = start[x] + (n > x ? (n - x + X - 1) / X : 0);
-    for (int j = 0; j < n; j++) all.push_back(v[start[j % X] + j / X]);
-    return l;
-  };
-  h->l_ids.assign(hl + 1, List());
-  h->l_leaves.assign(hl + 1, List());
-  h->l_parents.assign(hl + 1, List());
-  std::vector<int> all_leaves, all_ids;
-  for (int l = 1; l <= hl; l++) {
-    h->l_ids[l] = add(ids[l]);
-    h->l_leaves[l] = add(leaves[l]);
-    h->l_parents[l] = add(parents[l]);
-    all_leaves.insert(all_leaves.end(), leaves[l].begin(), leaves[l].end());
-    all_ids.insert(all_ids.end(), ids[l].begin(), ids[l].end());
-  }
-  h->l_all_leaves = add(all_leaves);
-  h->l_all_ids = add(all_ids);
-  h->l_refb = add(refb);
+  h->src_ids = ids;
+  h->src_leaves = leaves;
+  h->src_parents = parents;
+  h->src_refb = refb;
+  build_lists(*h, h->shape[KC_GSRB].bb);
+  const std::vector<int> &all = h->h_lists;
   HIPCHK(hipStreamSynchronize(h->stream));
   // Grow the pools, keeping the data of existing box ids.
   if (n_boxes > h->cap_boxes) {
@@ -1928,7 +2103,6 @@ int afgpu_sync_tree(afgpu_t *h, int n_boxes, const int *lvl, const int *ix, cons
   h->masked = false;
   h->base_active.clear();
   h->hboxes = hb;
-  h->h_lists = all;
   h->n_boxes = n_boxes;
   h->highest_lvl = hl;
   for (auto &o : h->ops) o.ok = false;  // level-1 box may have changed
@@ -2297,6 +2471,181 @@ int afgpu_kernel_stats(afgpu_t *h, int kind, int reset, double *total_ms, long l
   if (launches) *launches = t.launches;
   if (algo_bytes) *algo_bytes = t.algo_bytes;
   if (reset) t = KTimer();
+  return AFGPU_OK;
+}
+
+// ---- work-decomposition shapes and their autotuner ---------------------------
+static int upload_lists(afgpu_t *h, int bb) {
+  RCHK(sync_stream(h));
+  build_lists(*h, bb);
+  if (!h->h_lists.empty())
+    HIPCHK(hipMemcpy(h->d_lists, h->h_lists.data(), h->h_lists.size() * sizeof(int), hipMemcpyHostToDevice));
+  h->masked = false;
+  h->base_active.clear();
+  drop_graphs(*h);
+  return AFGPU_OK;
+}
+
+static bool shape_ok(const afgpu_t *h, int bb, int per_cu) {
+  const bool wave_boxes = h->D == 2 && h->nc == 8;
+  return (bb == 1 || ((bb == 2 || bb == 4 || bb == 8) && wave_boxes)) && per_cu >= 0 && per_cu <= 64;
+}
+
+int afgpu_get_shapes(afgpu_t *h, int n_classes, afgpu_shape_t *shapes) {
+  if (!h || n_classes < 0 || n_classes > KC_N || (n_classes > 0 && !shapes)) return AFGPU_ERR_ARG;
+  for (int k = 0; k < n_classes; k++) {
+    shapes[k].boxes_per_workgroup = h->shape[k].bb;
+    shapes[k].workgroups_per_cu = h->shape[k].per_cu;
+    shapes[k].ms = 0;
+  }
+  return AFGPU_OK;
+}
+
+int afgpu_set_shapes(afgpu_t *h, int n_classes, const afgpu_shape_t *shapes) {
+  if (!h || n_classes < 0 || n_classes > KC_N || (n_classes > 0 && !shapes)) return AFGPU_ERR_ARG;
+  for (int k = 0; k < n_classes; k++)
+    if (!shape_ok(h, shapes[k].boxes_per_workgroup, shapes[k].workgroups_per_cu)) {
+      set_err("afgpu_set_shapes: unsupported shape");
+      return AFGPU_ERR_ARG;
+    }
+  RCHK(sync_stream(h));
+  for (int k = 0; k < n_classes; k++) h->shape[k] = Shape{shapes[k].boxes_per_workgroup, shapes[k].workgroups_per_cu};
+  if (h->highest_lvl >= 1 && h->shape[KC_GSRB].bb != h->lists_bb) RCHK(upload_lists(h, h->shape[KC_GSRB].bb));
+  drop_graphs(*h);
+  return AFGPU_OK;
+}
+
+// Everything a time step changes, for trial steps.
+struct Snapshot {
+  std::vector<std::pair<void *, void *>> bufs;  // (pool, copy)
+  std::vector<size_t> bytes;
+  int it;
+  double time, global_time, global_dt, dt, frac_rejected;
+  bool started;
+  std::vector<afgpu_ctx::RunTime> rt;
+};
+static int snap_add(Snapshot &sn, void *pool, size_t bytes) {
+  if (!pool || bytes == 0) return AFGPU_OK;
+  void *cp = nullptr;
+  HIPCHK(hipMalloc(&cp, bytes));
+  sn.bufs.push_back({pool, cp});
+  sn.bytes.push_back(bytes);
+  return AFGPU_OK;
+}
+static void snap_free(Snapshot &sn) {
+  for (auto &b : sn.bufs) (void)hipFree(b.second);
+  sn.bufs.clear();
+}
+static int snap_copy(afgpu_t *h, Snapshot &sn, bool save) {
+  for (size_t q = 0; q < sn.bufs.size(); q++) {
+    void *dst = save ? sn.bufs[q].second : sn.bufs[q].first;
+    const void *src = save ? sn.bufs[q].first : sn.bufs[q].second;
+    HIPCHK(hipMemcpyAsync(dst, src, sn.bytes[q], hipMemcpyDeviceToDevice, h->stream));
+  }
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (save) {
+    sn.it = h->it;
+    sn.time = h->time;
+    sn.global_time = h->global_time;
+    sn.global_dt = h->global_dt;
+    sn.dt = h->dt;
+    sn.frac_rejected = h->frac_rejected;
+    sn.started = h->started;
+    sn.rt = h->rt;
+  } else {
+    h->it = sn.it;
+    h->time = sn.time;
+    h->global_time = sn.global_time;
+    h->global_dt = sn.global_dt;
+    h->dt = sn.dt;
+    h->frac_rejected = sn.frac_rejected;
+    h->started = sn.started;
+    h->rt = sn.rt;
+  }
+  return AFGPU_OK;
+}
+
+static int trial_step(afgpu_t *h) {
+  if (h->n_runs > 1) {
+    RCHK(h->impl->step_runs(nullptr));
+    return sync_stream(h);
+  }
+  return afgpu_step(h, nullptr);
+}
+
+int afgpu_autotune(afgpu_t *h, int n_classes, afgpu_shape_t *shapes) {
+  RCHK(ready(h, true));
+  if (n_classes < 0 || n_classes > KC_N || (n_classes > 0 && !shapes)) return AFGPU_ERR_ARG;
+  RCHK(sync_stream(h));
+  Snapshot sn;
+  int rc = AFGPU_OK;
+  const size_t cells = (size_t)h->cap_boxes * h->ncell, faces = (size_t)h->cap_boxes * h->nface;
+  for (int i = 0; i < AFGPU_N_CC && rc == AFGPU_OK; i++) rc = snap_add(sn, h->cc[i], cells * sizeof(double));
+  for (int i = 0; i < AFGPU_N_FC && rc == AFGPU_OK; i++) rc = snap_add(sn, h->fc[i], faces * sizeof(double));
+  for (int i = 0; i < 4 && rc == AFGPU_OK; i++) rc = snap_add(sn, h->cc32[i], cells * sizeof(float));
+  for (auto &r : h->mres)
+    for (int i = 0; i < 4 && rc == AFGPU_OK; i++)
+      if (r.cap_boxes == h->cap_boxes) rc = snap_add(sn, r.cc32[i], cells * sizeof(float));
+  if (rc == AFGPU_OK) rc = snap_copy(h, sn, true);
+  if (rc != AFGPU_OK) {
+    snap_free(sn);
+    return rc;
+  }
+  const bool wave_boxes = h->D == 2 && h->nc == 8;
+  std::vector<Shape> cands;
+  if (wave_boxes) {
+    for (int bb : {1, 2, 4, 8})
+      for (int pc : {0, 4, 8, 16, 32}) cands.push_back(Shape{bb, pc});
+  } else {
+    for (int pc : {0, 2, 4, 8}) cands.push_back(Shape{1, pc});
+  }
+  const Shape saved[KC_N] = {h->shape[0], h->shape[1], h->shape[2], h->shape[3],
+                             h->shape[4], h->shape[5], h->shape[6], h->shape[7]};
+  const bool timing0 = h->timing;
+  double best[KC_N];
+  Shape pick[KC_N];
+  for (int k = 0; k < KC_N; k++) {
+    best[k] = std::numeric_limits<double>::max();
+    pick[k] = saved[k];
+  }
+  h->timing = true;
+  h->class_timing = true;
+  drop_graphs(*h);
+  // one unmeasured step first (set-up work of the first solve), then one
+  // step per candidate, each from the snapshot
+  for (int q = -1; q < (int)cands.size() && rc == AFGPU_OK; q++) {
+    const Shape sh = q < 0 ? saved[KC_GSRB] : cands[q];
+    for (int k = 0; k < KC_N; k++) h->shape[k] = q < 0 ? saved[k] : sh;
+    rc = upload_lists(h, h->shape[KC_GSRB].bb);
+    for (int k = 0; k < KC_N; k++) h->timers[3 + k] = KTimer();
+    if (rc == AFGPU_OK) rc = trial_step(h);
+    if (rc == AFGPU_OK) rc = sync_stream(h);
+    if (rc == AFGPU_OK && q >= 0)
+      for (int k = 0; k < KC_N; k++) {
+        const KTimer &t = h->timers[3 + k];
+        if (t.launches > 0 && t.total_ms < best[k]) {
+          best[k] = t.total_ms;
+          pick[k] = sh;
+        }
+      }
+    const int rc2 = snap_copy(h, sn, false);
+    if (rc == AFGPU_OK) rc = rc2;
+  }
+  h->timing = timing0;
+  h->class_timing = false;
+  for (int k = 0; k < KC_N; k++) {
+    h->timers[3 + k] = KTimer();
+    h->shape[k] = (rc == AFGPU_OK) ? pick[k] : saved[k];
+  }
+  const int rc3 = upload_lists(h, h->shape[KC_GSRB].bb);
+  snap_free(sn);
+  RCHK(rc);
+  RCHK(rc3);
+  for (int k = 0; k < n_classes; k++) {
+    shapes[k].boxes_per_workgroup = h->shape[k].bb;
+    shapes[k].workgroups_per_cu = h->shape[k].per_cu;
+    shapes[k].ms = best[k] < std::numeric_limits<double>::max() ? best[k] : 0.0;
+  }
   return AFGPU_OK;
 }
 

@@ -214,6 +214,47 @@ __global__ void k_mask_ids(int *ids, int n, const int *__restrict__ run_of, cons
   ids[q] = active[run_of[id - 1]] ? id : -id;
 }
 
+// ---- work decomposition of the list kernels ---------------------------------
+// A workgroup holds BB groups of BS threads; each group works on one work
+// item of the list (a box, or one of the `bpb` parts of a box) at a time
+// and moves on by the grid's item count, so the grid may be smaller than
+// the work (persistent workgroups).  BB > 1 needs BS = 64: a group is then
+// one wavefront and synchronises on its own (gsync), so groups never wait
+// for each other and may leave the loop at different times.  BS is a power
+// of two.
+template <int BS>
+__device__ inline int ltid() {
+  return threadIdx.x & (BS - 1);
+}
+// The group index is uniform over a wavefront (a group is one wavefront,
+// or the whole workgroup when BB = 1); readfirstlane tells the compiler so,
+// which keeps the item's box id and box record in scalar registers.
+template <int BS, int BB>
+__device__ inline int lgrp() {
+  if (BB == 1) return 0;
+  return __builtin_amdgcn_readfirstlane(threadIdx.x / BS);
+}
+template <int BS, int BB>
+__device__ inline void gsync() {
+  static_assert(BB == 1 || BS == 64, "several groups per workgroup need one wavefront per group");
+  if (BB == 1) {
+    __syncthreads();
+  } else {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+// first work item of this group and the stride between its items
+template <int BS, int BB>
+__device__ inline int item0() {
+  return blockIdx.x * BB + lgrp<BS, BB>();
+}
+template <int BB>
+__device__ inline int item_stride() {
+  return gridDim.x * BB;
+}
+
 template <int BS>
 __device__ inline double block_max(double v, double *sh) {
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
@@ -225,6 +266,27 @@ __device__ inline double block_max(double v, double *sh) {
   __syncthreads();
   return r;
 }
+// max / min over one group (a wavefront: shuffles only; else the
+// workgroup through LDS).  The result is in every thread of the group.
+template <int BS, int BB>
+__device__ inline double group_max(double v, double *sh) {
+  if (BS == 64) {
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    return v;
+  }
+  return block_max<BS>(v, sh);
+}
+template <int BS>
+__device__ inline double block_min(double v, double *sh);
+template <int BS, int BB>
+__device__ inline double group_min(double v, double *sh) {
+  if (BS == 64) {
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off));
+    return v;
+  }
+  return block_min<BS>(v, sh);
+}
+
 template <int BS>
 __device__ inline double block_min(double v, double *sh) {
   for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off));

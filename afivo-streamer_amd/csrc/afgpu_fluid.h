@@ -91,198 +91,204 @@ __device__ inline double gc2_value(const DBox *__restrict__ boxes, int id, const
 // float: the mixed-precision form's FP32 flux, accumulated in FP64 by
 // k_update).  The n_e halo is formed in FP64 and its first layer written
 // back at FP64 precision in either case.
-template <int D, int NC, int BS, typename TF>
-__global__ void __launch_bounds__(BS)
+template <int D, int NC, int BS, typename TF, int BB = 1>
+__global__ void __launch_bounds__(BS * BB)
     k_flux(const int *__restrict__ ids, const DBox *__restrict__ boxes, double *ne, const double *__restrict__ efield,
            const double *__restrict__ efc, double *flux, VarBc m, DTable td, double N_inv_d,
-           unsigned long long *cfl_max, unsigned long long *sig_max) {
+           unsigned long long *cfl_max, unsigned long long *sig_max, int nw) {
   using G = Geo<D, NC>;
   constexpr int NT = G::NTAN;
-  __shared__ TF halo[2 * D][2][NT];
+  __shared__ TF halo_all[BB][2 * D][2][NT];
+  TF(&halo)[2 * D][2][NT] = halo_all[lgrp<BS, BB>()];
   __shared__ double sh[BS / 64];
-  const int id = ids[blockIdx.x];
-  if (id <= 0) return;  // run masked out
-  const DBox &bx = boxes[id - 1];
-  const size_t base = (size_t)(id - 1) * G::NCELL;
-  const size_t fbase = (size_t)(id - 1) * D * G::NFACE;
-  const TF N_inv = (TF)N_inv_d;
+  for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
+    const int id = ids[w];
+    if (id <= 0) continue;  // run masked out
+    const DBox &bx = boxes[id - 1];
+    const size_t base = (size_t)(id - 1) * G::NCELL;
+    const size_t fbase = (size_t)(id - 1) * D * G::NFACE;
+    const TF N_inv = (TF)N_inv_d;
 
-  // the halo; its first layer also goes back into the box
-  // (m_af_ghostcell.f90:740-742): no thread of any box reads a ghost cell
-  // of this level during the launch
-  for (int q = threadIdx.x; q < 2 * D * 2 * NT; q += BS) {
-    const int nb = q / (2 * NT), layer = (q / NT) % 2, r = q % NT;
-    const int a = r % NC + 1, b = (D == 3) ? r / NC + 1 : 0;
-    const double hv = gc2_value<D, NC>(boxes, id, ne, m, nb, layer, a, b);
-    halo[nb][layer][r] = (TF)hv;
-    if (layer == 0) {
-      const int d = nb / 2;
+    // the halo; its first layer also goes back into the box
+    // (m_af_ghostcell.f90:740-742): no thread of any box reads a ghost cell
+    // of this level during the launch
+    for (int q = ltid<BS>(); q < 2 * D * 2 * NT; q += BS) {
+      const int nb = q / (2 * NT), layer = (q / NT) % 2, r = q % NT;
+      const int a = r % NC + 1, b = (D == 3) ? r / NC + 1 : 0;
+      const double hv = gc2_value<D, NC>(boxes, id, ne, m, nb, layer, a, b);
+      halo[nb][layer][r] = (TF)hv;
+      if (layer == 0) {
+        const int d = nb / 2;
+        int t0, t1;
+        tangential(D, d, t0, t1);
+        int p[3] = {0, 0, 0};
+        p[t0] = a;
+        if (D == 3) p[t1] = b;
+        p[d] = (nb % 2) ? NC + 1 : 0;
+        ne[base + G::idx(p[0], p[1], p[2])] = hv;
+      }
+    }
+    gsync<BS, BB>();
+
+    auto line = [&](int d, int r, const int *p, int q) -> TF {
+      if (q >= 1 && q <= NC) {
+        int pp[3] = {p[0], p[1], p[2]};
+        pp[d] = q;
+        return (TF)ne[base + G::idx(pp[0], pp[1], pp[2])];
+      }
+      if (q <= 0) return halo[2 * d][-q][r];
+      return halo[2 * d + 1][q - NC - 1][r];
+    };
+
+    TF smax = -std::numeric_limits<TF>::max();
+    // face fluxes (flux_upwind, src/m_fluid.f90:102-209)
+    for (int q = ltid<BS>(); q < D * (NC + 1) * NT; q += BS) {
+      const int d = q / ((NC + 1) * NT);
+      // x faces: consecutive threads take consecutive faces of one grid row
+      // (contiguous in memory); y/z faces: consecutive rows of one face plane
+      const int qd = q % ((NC + 1) * NT);
+      const int f = (d == 0) ? qd % (NC + 1) + 1 : qd / NT + 1;
+      const int r = (d == 0) ? qd / (NC + 1) : qd % NT;
       int t0, t1;
       tangential(D, d, t0, t1);
       int p[3] = {0, 0, 0};
-      p[t0] = a;
-      if (D == 3) p[t1] = b;
-      p[d] = (nb % 2) ? NC + 1 : 0;
-      ne[base + G::idx(p[0], p[1], p[2])] = hv;
-    }
-  }
-  __syncthreads();
-
-  auto line = [&](int d, int r, const int *p, int q) -> TF {
-    if (q >= 1 && q <= NC) {
-      int pp[3] = {p[0], p[1], p[2]};
-      pp[d] = q;
-      return (TF)ne[base + G::idx(pp[0], pp[1], pp[2])];
-    }
-    if (q <= 0) return halo[2 * d][-q][r];
-    return halo[2 * d + 1][q - NC - 1][r];
-  };
-
-  TF smax = -std::numeric_limits<TF>::max();
-  // face fluxes (flux_upwind, src/m_fluid.f90:102-209)
-  for (int q = threadIdx.x; q < D * (NC + 1) * NT; q += BS) {
-    const int d = q / ((NC + 1) * NT);
-    // x faces: consecutive threads take consecutive faces of one grid row
-    // (contiguous in memory); y/z faces: consecutive rows of one face plane
-    const int qd = q % ((NC + 1) * NT);
-    const int f = (d == 0) ? qd % (NC + 1) + 1 : qd / NT + 1;
-    const int r = (d == 0) ? qd / (NC + 1) : qd % NT;
-    int t0, t1;
-    tangential(D, d, t0, t1);
-    int p[3] = {0, 0, 0};
-    p[t0] = r % NC + 1;
-    if (D == 3) p[t1] = r / NC + 1;
-    const TF inv_dx = TF(1) / (TF)bx.dr[d];
-    int fp[3] = {p[0], p[1], (D == 3) ? p[2] : 1};
-    fp[d] = f;
-    const TF ex = (TF)efc[fbase + G::fidx(fp[0], fp[1], fp[2], d)];
-    const TF Lm2 = line(d, r, p, f - 2), Lm1 = line(d, r, p, f - 1), L0 = line(d, r, p, f),
-             Lp1 = line(d, r, p, f + 1);
-    TF u;
-    if (-1 * ex > 0) {
-      u = Lm1 + TF(0.5) * lim_koren<TF>(L0 - Lm1, Lm1 - Lm2);
-    } else {
-      u = L0 - TF(0.5) * lim_koren<TF>(L0 - Lm1, Lp1 - L0);
-    }
-    int e0[3] = {p[0], p[1], p[2]}, e1[3] = {p[0], p[1], p[2]};
-    e0[d] = f - 1;
-    e1[d] = f;
-    const TF tmp_fc = TF(0.5) * ((TF)efield[base + G::idx(e0[0], e0[1], e0[2])] +
-                                 (TF)efield[base + G::idx(e1[0], e1[1], e1[2])]) *
-                      (TF)SI_to_Townsend * N_inv;
-    int lix;
-    TF lw;
-    lt_loc_t<TF>(td, tmp_fc, lix, lw);
-    const TF mu = lt_at_t<TF>(td, 0, lix, lw) * N_inv;
-    const TF dc = lt_at_t<TF>(td, 1, lix, lw) * N_inv;
-    const TF v = -mu * ex;
-    flux[fbase + G::fidx(fp[0], fp[1], fp[2], d)] = (double)(v * u - dc * inv_dx * (L0 - Lm1));
-    smax = fmax(smax, mu * u);
-  }
-  double smax_d = block_max<BS>((double)smax, sh);
-  if (threadIdx.x == 0) atomic_max_nonneg(run_slot(sig_max, bx), fmax(smax_d, 1e-100));
-
-  // CFL sum per cell (m_fluid.f90:194-195 summed over dimensions,
-  // m_af_flux_schemes.f90:779-848)
-  TF cmax = 0;
-  for (int q = threadIdx.x; q < G::NINT; q += BS) {
-    int i, j, k;
-    interior_ijk<D, NC>(q, i, j, k);
-    TF cfl = 0;
-    for (int d = 0; d < D; d++) {
+      p[t0] = r % NC + 1;
+      if (D == 3) p[t1] = r / NC + 1;
       const TF inv_dx = TF(1) / (TF)bx.dr[d];
-      TF vv[2], dd[2];
-      for (int s = 0; s < 2; s++) {
-        int c[3] = {i, j, k};
-        int fp[3] = {i, j, (D == 3) ? k : 1};
-        fp[d] += s;  // faces c and c+1
-        const TF ex = (TF)efc[fbase + G::fidx(fp[0], fp[1], fp[2], d)];
-        int e0[3] = {c[0], c[1], c[2]};
-        e0[d] = c[d] - 1 + s;
-        int e1[3] = {c[0], c[1], c[2]};
-        e1[d] = c[d] + s;
-        const TF tmp_fc = TF(0.5) * ((TF)efield[base + G::idx(e0[0], e0[1], e0[2])] +
-                                     (TF)efield[base + G::idx(e1[0], e1[1], e1[2])]) *
-                          (TF)SI_to_Townsend * N_inv;
-        int lix;
-        TF lw;
-        lt_loc_t<TF>(td, tmp_fc, lix, lw);
-        const TF mu = lt_at_t<TF>(td, 0, lix, lw) * N_inv;
-        dd[s] = lt_at_t<TF>(td, 1, lix, lw) * N_inv;
-        vv[s] = -mu * ex;
+      int fp[3] = {p[0], p[1], (D == 3) ? p[2] : 1};
+      fp[d] = f;
+      const TF ex = (TF)efc[fbase + G::fidx(fp[0], fp[1], fp[2], d)];
+      const TF Lm2 = line(d, r, p, f - 2), Lm1 = line(d, r, p, f - 1), L0 = line(d, r, p, f),
+               Lp1 = line(d, r, p, f + 1);
+      TF u;
+      if (-1 * ex > 0) {
+        u = Lm1 + TF(0.5) * lim_koren<TF>(L0 - Lm1, Lm1 - Lm2);
+      } else {
+        u = L0 - TF(0.5) * lim_koren<TF>(L0 - Lm1, Lp1 - L0);
       }
-      const TF cl = TF(1) * fmax(fabs(vv[1]), fabs(vv[0])) * inv_dx + TF(2) * fmax(dd[1], dd[0]) * (inv_dx * inv_dx);
-      cfl = cfl + cl;
+      int e0[3] = {p[0], p[1], p[2]}, e1[3] = {p[0], p[1], p[2]};
+      e0[d] = f - 1;
+      e1[d] = f;
+      const TF tmp_fc = TF(0.5) * ((TF)efield[base + G::idx(e0[0], e0[1], e0[2])] +
+                                   (TF)efield[base + G::idx(e1[0], e1[1], e1[2])]) *
+                        (TF)SI_to_Townsend * N_inv;
+      int lix;
+      TF lw;
+      lt_loc_t<TF>(td, tmp_fc, lix, lw);
+      const TF mu = lt_at_t<TF>(td, 0, lix, lw) * N_inv;
+      const TF dc = lt_at_t<TF>(td, 1, lix, lw) * N_inv;
+      const TF v = -mu * ex;
+      flux[fbase + G::fidx(fp[0], fp[1], fp[2], d)] = (double)(v * u - dc * inv_dx * (L0 - Lm1));
+      smax = fmax(smax, mu * u);
     }
-    cmax = fmax(cmax, cfl);
+    double smax_d = group_max<BS, BB>((double)smax, sh);
+    if (ltid<BS>() == 0) atomic_max_nonneg(run_slot(sig_max, bx), fmax(smax_d, 1e-100));
+
+    // CFL sum per cell (m_fluid.f90:194-195 summed over dimensions,
+    // m_af_flux_schemes.f90:779-848)
+    TF cmax = 0;
+    for (int q = ltid<BS>(); q < G::NINT; q += BS) {
+      int i, j, k;
+      interior_ijk<D, NC>(q, i, j, k);
+      TF cfl = 0;
+      for (int d = 0; d < D; d++) {
+        const TF inv_dx = TF(1) / (TF)bx.dr[d];
+        TF vv[2], dd[2];
+        for (int s = 0; s < 2; s++) {
+          int c[3] = {i, j, k};
+          int fp[3] = {i, j, (D == 3) ? k : 1};
+          fp[d] += s;  // faces c and c+1
+          const TF ex = (TF)efc[fbase + G::fidx(fp[0], fp[1], fp[2], d)];
+          int e0[3] = {c[0], c[1], c[2]};
+          e0[d] = c[d] - 1 + s;
+          int e1[3] = {c[0], c[1], c[2]};
+          e1[d] = c[d] + s;
+          const TF tmp_fc = TF(0.5) * ((TF)efield[base + G::idx(e0[0], e0[1], e0[2])] +
+                                       (TF)efield[base + G::idx(e1[0], e1[1], e1[2])]) *
+                            (TF)SI_to_Townsend * N_inv;
+          int lix;
+          TF lw;
+          lt_loc_t<TF>(td, tmp_fc, lix, lw);
+          const TF mu = lt_at_t<TF>(td, 0, lix, lw) * N_inv;
+          dd[s] = lt_at_t<TF>(td, 1, lix, lw) * N_inv;
+          vv[s] = -mu * ex;
+        }
+        const TF cl = TF(1) * fmax(fabs(vv[1]), fabs(vv[0])) * inv_dx + TF(2) * fmax(dd[1], dd[0]) * (inv_dx * inv_dx);
+        cfl = cfl + cl;
+      }
+      cmax = fmax(cmax, cfl);
+    }
+    const double cmax_d = group_max<BS, BB>((double)cmax, sh);
+    if (ltid<BS>() == 0) atomic_max_nonneg(run_slot(cfl_max, bx), cmax_d);
+    gsync<BS, BB>();  // (the group's halo is reused by its next item)
   }
-  const double cmax_d = block_max<BS>((double)cmax, sh);
-  if (threadIdx.x == 0) atomic_max_nonneg(run_slot(cfl_max, bx), cmax_d);
 }
 
 // Coarse-leaf faces next to a refined neighbour get the (cylindrically
 // weighted) average of the fine fluxes: af_consistent_fluxes +
 // flux_from_children (afivo/src/m_af_core.f90:1257-1402).
-template <int D, int NC, int BS>
-__global__ void __launch_bounds__(BS) k_consistent_fluxes(const int *__restrict__ ids, const DBox *__restrict__ boxes,
-                                                          double *flux, int cyl) {
+template <int D, int NC, int BS, int BB = 1>
+__global__ void __launch_bounds__(BS * BB) k_consistent_fluxes(const int *__restrict__ ids, const DBox *__restrict__ boxes,
+                                                          double *flux, int cyl, int nw) {
   using G = Geo<D, NC>;
   constexpr int H = NC / 2;
-  const int id = ids[blockIdx.x];
-  if (id <= 0) return;  // run masked out
-  const DBox &bx = boxes[id - 1];
-  for (int q = threadIdx.x; q < 2 * D * G::NTAN; q += BS) {
-    const int nb = q / G::NTAN, r = q % G::NTAN;
-    const int nb_id = bx.nb[nb];
-    if (nb_id <= 0 || boxes[nb_id - 1].children[0] > 0) continue;
-    const int d = nb / 2;
-    const bool low = (nb % 2) == 0;
-    int t0, t1;
-    tangential(D, d, t0, t1);
-    const int a = r % NC + 1, b = (D == 3) ? r / NC + 1 : 0;
-    int cd[3] = {0, 0, 0};
-    cd[d] = low ? 0 : 1;
-    cd[t0] = (a > H) ? 1 : 0;
-    if (D == 3) cd[t1] = (b > H) ? 1 : 0;
-    const int i_ch = cd[0] + 2 * cd[1] + 4 * cd[2];
-    const int c_id = bx.children[i_ch];
-    const double *fc = flux + (size_t)(c_id - 1) * D * G::NFACE;
-    double *fn = flux + (size_t)(nb_id - 1) * D * G::NFACE;
-    const int fi = low ? 1 : NC + 1, fi_nb = low ? NC + 1 : 1;
-    const int pa = a - cd[t0] * H;  // index on the child's half
-    if (D == 2) {
-      int pn[2], s0[2], s1[2];
-      pn[d] = fi_nb;
-      pn[t0] = a;
-      s0[d] = s1[d] = fi;
-      s0[t0] = 2 * pa - 1;
-      s1[t0] = 2 * pa;
-      const double A = fc[G::fidx(s0[0], s0[1], 1, d)], B = fc[G::fidx(s1[0], s1[1], 1, d)];
-      double val;
-      if (d == 1 && cyl) {
-        const DBox &bn = boxes[nb_id - 1];
-        double rr = bn.r_min[0] + (a - 0.5) * bn.dr[0];
-        double tmp = 0.25 * bn.dr[0] / rr;
-        double w1 = 1 - tmp, w2 = 1 + tmp;
-        val = 0.5 * (w1 * A + w2 * B);
+  for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
+    const int id = ids[w];
+    if (id <= 0) continue;  // run masked out
+    const DBox &bx = boxes[id - 1];
+    for (int q = ltid<BS>(); q < 2 * D * G::NTAN; q += BS) {
+      const int nb = q / G::NTAN, r = q % G::NTAN;
+      const int nb_id = bx.nb[nb];
+      if (nb_id <= 0 || boxes[nb_id - 1].children[0] > 0) continue;
+      const int d = nb / 2;
+      const bool low = (nb % 2) == 0;
+      int t0, t1;
+      tangential(D, d, t0, t1);
+      const int a = r % NC + 1, b = (D == 3) ? r / NC + 1 : 0;
+      int cd[3] = {0, 0, 0};
+      cd[d] = low ? 0 : 1;
+      cd[t0] = (a > H) ? 1 : 0;
+      if (D == 3) cd[t1] = (b > H) ? 1 : 0;
+      const int i_ch = cd[0] + 2 * cd[1] + 4 * cd[2];
+      const int c_id = bx.children[i_ch];
+      const double *fc = flux + (size_t)(c_id - 1) * D * G::NFACE;
+      double *fn = flux + (size_t)(nb_id - 1) * D * G::NFACE;
+      const int fi = low ? 1 : NC + 1, fi_nb = low ? NC + 1 : 1;
+      const int pa = a - cd[t0] * H;  // index on the child's half
+      if (D == 2) {
+        int pn[2], s0[2], s1[2];
+        pn[d] = fi_nb;
+        pn[t0] = a;
+        s0[d] = s1[d] = fi;
+        s0[t0] = 2 * pa - 1;
+        s1[t0] = 2 * pa;
+        const double A = fc[G::fidx(s0[0], s0[1], 1, d)], B = fc[G::fidx(s1[0], s1[1], 1, d)];
+        double val;
+        if (d == 1 && cyl) {
+          const DBox &bn = boxes[nb_id - 1];
+          double rr = bn.r_min[0] + (a - 0.5) * bn.dr[0];
+          double tmp = 0.25 * bn.dr[0] / rr;
+          double w1 = 1 - tmp, w2 = 1 + tmp;
+          val = 0.5 * (w1 * A + w2 * B);
+        } else {
+          val = 0.5 * (A + B);
+        }
+        fn[G::fidx(pn[0], pn[1], 1, d)] = val;
       } else {
-        val = 0.5 * (A + B);
+        const int pb = b - cd[t1] * H;
+        int pn[3], s00[3], s10[3], s01[3], s11[3];
+        pn[d] = fi_nb;
+        pn[t0] = a;
+        pn[t1] = b;
+        s00[d] = s10[d] = s01[d] = s11[d] = fi;
+        s00[t0] = 2 * pa - 1; s00[t1] = 2 * pb - 1;
+        s10[t0] = 2 * pa;     s10[t1] = 2 * pb - 1;
+        s01[t0] = 2 * pa - 1; s01[t1] = 2 * pb;
+        s11[t0] = 2 * pa;     s11[t1] = 2 * pb;
+        fn[G::fidx(pn[0], pn[1], pn[2], d)] =
+            0.25 * (fc[G::fidx(s00[0], s00[1], s00[2], d)] + fc[G::fidx(s10[0], s10[1], s10[2], d)] +
+                    fc[G::fidx(s01[0], s01[1], s01[2], d)] + fc[G::fidx(s11[0], s11[1], s11[2], d)]);
       }
-      fn[G::fidx(pn[0], pn[1], 1, d)] = val;
-    } else {
-      const int pb = b - cd[t1] * H;
-      int pn[3], s00[3], s10[3], s01[3], s11[3];
-      pn[d] = fi_nb;
-      pn[t0] = a;
-      pn[t1] = b;
-      s00[d] = s10[d] = s01[d] = s11[d] = fi;
-      s00[t0] = 2 * pa - 1; s00[t1] = 2 * pb - 1;
-      s10[t0] = 2 * pa;     s10[t1] = 2 * pb - 1;
-      s01[t0] = 2 * pa - 1; s01[t1] = 2 * pb;
-      s11[t0] = 2 * pa;     s11[t1] = 2 * pb;
-      fn[G::fidx(pn[0], pn[1], pn[2], d)] =
-          0.25 * (fc[G::fidx(s00[0], s00[1], s00[2], d)] + fc[G::fidx(s10[0], s10[1], s10[2], d)] +
-                  fc[G::fidx(s01[0], s01[1], s01[2], d)] + fc[G::fidx(s11[0], s11[1], s11[2], d)]);
     }
   }
 }
@@ -311,183 +317,191 @@ struct UpdateArgs {
 // flux_update_densities (afivo/src/m_af_flux_schemes.f90:320-436) with the
 // standard-model add_source_terms (src/m_fluid.f90:298-466,
 // src/m_chemistry.f90:672-804) over all leaves.
-template <int D, int NC, int BS>
-__global__ void __launch_bounds__(BS) k_update(const int *__restrict__ ids, const DBox *__restrict__ boxes,
-                                               UpdateArgs A, unsigned long long *chem_min) {
+template <int D, int NC, int BS, int BB = 1>
+__global__ void __launch_bounds__(BS * BB) k_update(const int *__restrict__ ids, const DBox *__restrict__ boxes,
+                                               UpdateArgs A, unsigned long long *chem_min, int nw) {
   using G = Geo<D, NC>;
   __shared__ double sh[BS / 64];
-  const int id = ids[blockIdx.x];
-  if (id <= 0) return;  // run masked out
-  const DBox &bx = boxes[id - 1];
-  const size_t base = (size_t)(id - 1) * G::NCELL;
-  const size_t fbase = (size_t)(id - 1) * D * G::NFACE;
-  const double dt = A.dt_run ? A.dt_scale * A.dt_run[bx.run] : A.dt;
-  double dt_dr[3];
-  for (int d = 0; d < D; d++) dt_dr[d] = dt / bx.dr[d];
-  const double eps = 1e-100;
-  double bmin = 1e100;
-  for (int q = threadIdx.x; q < G::NINT; q += BS) {
-    int i, j, k;
-    interior_ijk<D, NC>(q, i, j, k);
-    const size_t x = base + G::idx(i, j, k);
-    double out[3];
-    for (int sp = 0; sp < 3; sp++) {
-      double tmp = 0.0;
-      for (int m = 0; m < A.n_prev; m++) tmp = tmp + A.w_prev[m] * A.sp[sp][A.s_prev[m]][x];
-      out[sp] = tmp;
-    }
-    double dens[3];
-    for (int sp = 0; sp < 3; sp++) {
-      const double cur = (A.s_deriv == A.s_out) ? out[sp] : A.sp[sp][A.s_deriv][x];
-      dens[sp] = fmax(cur, 0.0);
-    }
-    const double field = SI_to_Townsend * A.N_inv * A.efield[x];
-    int lix;
-    double lw;
-    lt_loc(A.chem, field, lix, lw);
-    const double k_ion = 1.0 * lt_at(A.chem, 0, lix, lw);
-    const double k_att = 1.0 * lt_at(A.chem, 1, lix, lw);
-    const double r1 = k_ion * dens[0];
-    const double r2 = k_att * dens[0];
-    double de = 0.0, dp = 0.0, dm = 0.0;
-    de = de - r1;
-    de = de + r1 * 2;
-    dp = dp + r1 * 1;
-    de = de - r2;
-    dm = dm + r2 * 1;
-    double dd[3] = {de, dp, dm};
-    if (A.last_step) {
+  for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
+    const int id = ids[w];
+    if (id <= 0) continue;  // run masked out
+    const DBox &bx = boxes[id - 1];
+    const size_t base = (size_t)(id - 1) * G::NCELL;
+    const size_t fbase = (size_t)(id - 1) * D * G::NFACE;
+    const double dt = A.dt_run ? A.dt_scale * A.dt_run[bx.run] : A.dt;
+    double dt_dr[3];
+    for (int d = 0; d < D; d++) dt_dr[d] = dt / bx.dr[d];
+    const double eps = 1e-100;
+    double bmin = 1e100;
+    for (int q = ltid<BS>(); q < G::NINT; q += BS) {
+      int i, j, k;
+      interior_ijk<D, NC>(q, i, j, k);
+      const size_t x = base + G::idx(i, j, k);
+      double out[3];
       for (int sp = 0; sp < 3; sp++) {
-        double t;
-        if (A.chem_nmin_on)
-          t = (dens[sp] + A.chem_nmin) / fmax(fabs(dd[sp]), eps);
-        else if (A.chem_limit_loss)
-          t = fmax(dens[sp], eps) / fmax(-dd[sp], eps);
-        else
-          t = 1e100;
-        bmin = fmin(bmin, t);
+        double tmp = 0.0;
+        for (int m = 0; m < A.n_prev; m++) tmp = tmp + A.w_prev[m] * A.sp[sp][A.s_prev[m]][x];
+        out[sp] = tmp;
       }
-    }
-    if (A.photoi_enabled) {
-      const double ph = A.photo[x];
-      dd[0] = dd[0] + ph;
-      dd[A.photoi_species] = dd[A.photoi_species] + ph;
-    }
-    for (int sp = 0; sp < 3; sp++) out[sp] = out[sp] + dt * dd[sp];
-    // conservative flux divergence of the electrons
-    const double *fl = A.flux + fbase;
-    double ne = out[0];
-    if (D == 2) {
-      if (A.cyl) {
-        const double dr = bx.dr[0];
-        const double r = bx.r_min[0] + (i - 0.5) * dr;
-        const double inv_r = 1 / r;
-        const double rf1 = (r - 0.5 * dr) * inv_r, rf2 = (r + 0.5 * dr) * inv_r;
-        ne = ne + dt_dr[0] * (rf1 * fl[G::fidx(i, j, 1, 0)] - rf2 * fl[G::fidx(i + 1, j, 1, 0)]) +
-             dt_dr[1] * (fl[G::fidx(i, j, 1, 1)] - fl[G::fidx(i, j + 1, 1, 1)]);
+      double dens[3];
+      for (int sp = 0; sp < 3; sp++) {
+        const double cur = (A.s_deriv == A.s_out) ? out[sp] : A.sp[sp][A.s_deriv][x];
+        dens[sp] = fmax(cur, 0.0);
+      }
+      const double field = SI_to_Townsend * A.N_inv * A.efield[x];
+      int lix;
+      double lw;
+      lt_loc(A.chem, field, lix, lw);
+      const double k_ion = 1.0 * lt_at(A.chem, 0, lix, lw);
+      const double k_att = 1.0 * lt_at(A.chem, 1, lix, lw);
+      const double r1 = k_ion * dens[0];
+      const double r2 = k_att * dens[0];
+      double de = 0.0, dp = 0.0, dm = 0.0;
+      de = de - r1;
+      de = de + r1 * 2;
+      dp = dp + r1 * 1;
+      de = de - r2;
+      dm = dm + r2 * 1;
+      double dd[3] = {de, dp, dm};
+      if (A.last_step) {
+        for (int sp = 0; sp < 3; sp++) {
+          double t;
+          if (A.chem_nmin_on)
+            t = (dens[sp] + A.chem_nmin) / fmax(fabs(dd[sp]), eps);
+          else if (A.chem_limit_loss)
+            t = fmax(dens[sp], eps) / fmax(-dd[sp], eps);
+          else
+            t = 1e100;
+          bmin = fmin(bmin, t);
+        }
+      }
+      if (A.photoi_enabled) {
+        const double ph = A.photo[x];
+        dd[0] = dd[0] + ph;
+        dd[A.photoi_species] = dd[A.photoi_species] + ph;
+      }
+      for (int sp = 0; sp < 3; sp++) out[sp] = out[sp] + dt * dd[sp];
+      // conservative flux divergence of the electrons
+      const double *fl = A.flux + fbase;
+      double ne = out[0];
+      if (D == 2) {
+        if (A.cyl) {
+          const double dr = bx.dr[0];
+          const double r = bx.r_min[0] + (i - 0.5) * dr;
+          const double inv_r = 1 / r;
+          const double rf1 = (r - 0.5 * dr) * inv_r, rf2 = (r + 0.5 * dr) * inv_r;
+          ne = ne + dt_dr[0] * (rf1 * fl[G::fidx(i, j, 1, 0)] - rf2 * fl[G::fidx(i + 1, j, 1, 0)]) +
+               dt_dr[1] * (fl[G::fidx(i, j, 1, 1)] - fl[G::fidx(i, j + 1, 1, 1)]);
+        } else {
+          ne = ne + dt_dr[0] * (fl[G::fidx(i, j, 1, 0)] - fl[G::fidx(i + 1, j, 1, 0)]) +
+               dt_dr[1] * (fl[G::fidx(i, j, 1, 1)] - fl[G::fidx(i, j + 1, 1, 1)]);
+        }
       } else {
-        ne = ne + dt_dr[0] * (fl[G::fidx(i, j, 1, 0)] - fl[G::fidx(i + 1, j, 1, 0)]) +
-             dt_dr[1] * (fl[G::fidx(i, j, 1, 1)] - fl[G::fidx(i, j + 1, 1, 1)]);
+        ne = ne + dt_dr[0] * (fl[G::fidx(i, j, k, 0)] - fl[G::fidx(i + 1, j, k, 0)]) +
+             dt_dr[1] * (fl[G::fidx(i, j, k, 1)] - fl[G::fidx(i, j + 1, k, 1)]) +
+             dt_dr[2] * (fl[G::fidx(i, j, k, 2)] - fl[G::fidx(i, j, k + 1, 2)]);
       }
-    } else {
-      ne = ne + dt_dr[0] * (fl[G::fidx(i, j, k, 0)] - fl[G::fidx(i + 1, j, k, 0)]) +
-           dt_dr[1] * (fl[G::fidx(i, j, k, 1)] - fl[G::fidx(i, j + 1, k, 1)]) +
-           dt_dr[2] * (fl[G::fidx(i, j, k, 2)] - fl[G::fidx(i, j, k + 1, 2)]);
+      out[0] = ne;
+      for (int sp = 0; sp < 3; sp++) A.sp[sp][A.s_out][x] = out[sp];
     }
-    out[0] = ne;
-    for (int sp = 0; sp < 3; sp++) A.sp[sp][A.s_out][x] = out[sp];
-  }
-  if (A.last_step) {
-    bmin = block_min<BS>(bmin, sh);
-    if (threadIdx.x == 0) atomic_min_nonneg(run_slot(chem_min, bx), bmin);
+    if (A.last_step) {
+      bmin = group_min<BS, BB>(bmin, sh);
+      if (ltid<BS>() == 0) atomic_min_nonneg(run_slot(chem_min, bx), bmin);
+    }
   }
 }
 
 // field_set_rhs (src/m_field.f90:406-444) on leaves, all cells.
-template <int NCELL, int BS>
-__global__ void __launch_bounds__(BS) k_set_rhs(const int *__restrict__ ids, const double *n0, const double *n1,
-                                                const double *n2, double *rhs) {
-  const int id = ids[blockIdx.x];
-  if (id <= 0) return;  // run masked out
-  const size_t base = (size_t)(id - 1) * NCELL;
-  const double fac = -UC_elem_charge / UC_eps0;
-  const double q0 = -1 * fac, q1 = 1 * fac, q2 = -1 * fac;
-  for (int x = threadIdx.x; x < NCELL; x += BS) {
-    double r = 0.0;
-    r = r + q0 * n0[base + x];
-    r = r + q1 * n1[base + x];
-    r = r + q2 * n2[base + x];
-    rhs[base + x] = r;
+template <int NCELL, int BS, int BB = 1>
+__global__ void __launch_bounds__(BS * BB) k_set_rhs(const int *__restrict__ ids, const double *n0, const double *n1,
+                                                const double *n2, double *rhs, int nw) {
+  for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
+    const int id = ids[w];
+    if (id <= 0) continue;  // run masked out
+    const size_t base = (size_t)(id - 1) * NCELL;
+    const double fac = -UC_elem_charge / UC_eps0;
+    const double q0 = -1 * fac, q1 = 1 * fac, q2 = -1 * fac;
+    for (int x = ltid<BS>(); x < NCELL; x += BS) {
+      double r = 0.0;
+      r = r + q0 * n0[base + x];
+      r = r + q1 * n1[base + x];
+      r = r + q2 * n2[base + x];
+      rhs[base + x] = r;
+    }
   }
 }
 
 // E = -grad phi on all faces and |E| at cell centres
 // (mg_compute_phi_gradient / mg_box_lpl_gradient / mg_box_field_norm,
 // afivo/src/m_af_multigrid.f90:1857-2045).
-template <int D, int NC, int BS>
-__global__ void __launch_bounds__(BS) k_gradient(const int *__restrict__ ids, const DBox *__restrict__ boxes,
-                                                 const double *__restrict__ phi, double *efc, double *enorm) {
+template <int D, int NC, int BS, int BB = 1>
+__global__ void __launch_bounds__(BS * BB) k_gradient(const int *__restrict__ ids, const DBox *__restrict__ boxes,
+                                                 const double *__restrict__ phi, double *efc, double *enorm, int nw) {
   using G = Geo<D, NC>;
-  const int id = ids[blockIdx.x];
-  if (id <= 0) return;  // run masked out
-  const DBox &bx = boxes[id - 1];
-  const double *u = phi + (size_t)(id - 1) * G::NCELL;
-  double *f = efc + (size_t)(id - 1) * D * G::NFACE;
-  double inv_dr[3];
-  for (int d = 0; d < D; d++) inv_dr[d] = -1.0 / bx.dr[d];
-  constexpr int NFD = (NC + 1) * G::NTAN;
-  constexpr int st[3] = {1, G::NG, G::NG * G::NG};
-  for (int q = threadIdx.x; q < D * NFD; q += BS) {
-    const int d = q / NFD, r = q % NFD;
-    int t0, t1;
-    tangential(D, d, t0, t1);
-    int p[3] = {0, 0, 1};
-    p[d] = r % (NC + 1) + 1;
-    p[t0] = (r / (NC + 1)) % NC + 1;
-    if (D == 3) p[t1] = r / ((NC + 1) * NC) + 1;
-    const int x = G::idx(p[0], p[1], D == 3 ? p[2] : 0);
-    f[G::fidx(p[0], p[1], p[2], d)] = inv_dr[d] * (u[x] - u[x - st[d]]);
-  }
-  double *e = enorm + (size_t)(id - 1) * G::NCELL;
-  for (int q = threadIdx.x; q < G::NINT; q += BS) {
-    int i, j, k;
-    interior_ijk<D, NC>(q, i, j, k);
-    const int x = G::idx(i, j, k);
-    double s = 0;
-    for (int d = 0; d < D; d++) {
-      const double lo = inv_dr[d] * (u[x] - u[x - st[d]]);
-      const double hi = inv_dr[d] * (u[x + st[d]] - u[x]);
-      const double a = lo + hi;
-      s = (d == 0) ? a * a : s + a * a;
+  for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
+    const int id = ids[w];
+    if (id <= 0) continue;  // run masked out
+    const DBox &bx = boxes[id - 1];
+    const double *u = phi + (size_t)(id - 1) * G::NCELL;
+    double *f = efc + (size_t)(id - 1) * D * G::NFACE;
+    double inv_dr[3];
+    for (int d = 0; d < D; d++) inv_dr[d] = -1.0 / bx.dr[d];
+    constexpr int NFD = (NC + 1) * G::NTAN;
+    constexpr int st[3] = {1, G::NG, G::NG * G::NG};
+    for (int q = ltid<BS>(); q < D * NFD; q += BS) {
+      const int d = q / NFD, r = q % NFD;
+      int t0, t1;
+      tangential(D, d, t0, t1);
+      int p[3] = {0, 0, 1};
+      p[d] = r % (NC + 1) + 1;
+      p[t0] = (r / (NC + 1)) % NC + 1;
+      if (D == 3) p[t1] = r / ((NC + 1) * NC) + 1;
+      const int x = G::idx(p[0], p[1], D == 3 ? p[2] : 0);
+      f[G::fidx(p[0], p[1], p[2], d)] = inv_dr[d] * (u[x] - u[x - st[d]]);
     }
-    e[x] = 0.5 * sqrt(s);
+    double *e = enorm + (size_t)(id - 1) * G::NCELL;
+    for (int q = ltid<BS>(); q < G::NINT; q += BS) {
+      int i, j, k;
+      interior_ijk<D, NC>(q, i, j, k);
+      const int x = G::idx(i, j, k);
+      double s = 0;
+      for (int d = 0; d < D; d++) {
+        const double lo = inv_dr[d] * (u[x] - u[x - st[d]]);
+        const double hi = inv_dr[d] * (u[x + st[d]] - u[x]);
+        const double a = lo + hi;
+        s = (d == 0) ? a * a : s + a * a;
+      }
+      e[x] = 0.5 * sqrt(s);
+    }
   }
 }
 
 // Photon production on leaves (photoionization_rate_from_alpha,
 // src/m_photoi.f90:259-291).
-template <int D, int NC, int BS>
-__global__ void __launch_bounds__(BS) k_photoi_src(const int *__restrict__ ids, const double *__restrict__ efield,
+template <int D, int NC, int BS, int BB = 1>
+__global__ void __launch_bounds__(BS * BB) k_photoi_src(const int *__restrict__ ids, const double *__restrict__ efield,
                                                    const double *__restrict__ ne, double *rhs, DTable td, double N,
-                                                   double coeff) {
+                                                   double coeff, int nw) {
   using G = Geo<D, NC>;
-  const int id = ids[blockIdx.x];
-  if (id <= 0) return;  // run masked out
-  const size_t base = (size_t)(id - 1) * G::NCELL;
-  for (int q = threadIdx.x; q < G::NINT; q += BS) {
-    int i, j, k;
-    interior_ijk<D, NC>(q, i, j, k);
-    const size_t x = base + G::idx(i, j, k);
-    const double fld = efield[x];
-    const double Td = fld * SI_to_Townsend / N;
-    int lix;
-    double lw;
-    lt_loc(td, Td, lix, lw);
-    const double alpha = lt_at(td, 2, lix, lw);
-    const double mob = lt_at(td, 0, lix, lw);
-    double tmp = fld * mob * alpha * ne[x] * coeff;
-    if (tmp < 0) tmp = 0;
-    rhs[x] = tmp;
+  for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
+    const int id = ids[w];
+    if (id <= 0) continue;  // run masked out
+    const size_t base = (size_t)(id - 1) * G::NCELL;
+    for (int q = ltid<BS>(); q < G::NINT; q += BS) {
+      int i, j, k;
+      interior_ijk<D, NC>(q, i, j, k);
+      const size_t x = base + G::idx(i, j, k);
+      const double fld = efield[x];
+      const double Td = fld * SI_to_Townsend / N;
+      int lix;
+      double lw;
+      lt_loc(td, Td, lix, lw);
+      const double alpha = lt_at(td, 2, lix, lw);
+      const double mob = lt_at(td, 0, lix, lw);
+      double tmp = fld * mob * alpha * ne[x] * coeff;
+      if (tmp < 0) tmp = 0;
+      rhs[x] = tmp;
+    }
   }
 }
 

@@ -14,18 +14,19 @@
 
 namespace afg {
 
-// Box of this workgroup and the first work item / stride of this thread.
+// Box of work item w (box w / bpb, part w % bpb) and the first cell index /
+// stride of this thread.
 struct Part {
   int id;
   int q0;
   int stride;
 };
 template <int BS>
-__device__ inline Part part_of(const int *__restrict__ ids, int bpb) {
+__device__ inline Part part_of(const int *__restrict__ ids, int bpb, int w) {
   Part p;
-  const int slot = blockIdx.x / bpb;
+  const int slot = w / bpb;
   p.id = ids[slot];
-  p.q0 = (blockIdx.x - slot * bpb) * BS + threadIdx.x;
+  p.q0 = (w - slot * bpb) * BS + ltid<BS>();
   p.stride = bpb * BS;
   return p;
 }
@@ -270,36 +271,38 @@ __device__ inline void gc_corner_cell(const DBox &bx, T *var, int id, int n) {
 // af_gc_box for all boxes of a list.  Corners need the face ghosts of the
 // same box, so with corners the box must be handled by one workgroup
 // (bpb = 1); face-only fills may be split over several workgroups.
-template <int D, int NC, int BS, typename T>
-__global__ void __launch_bounds__(BS) k_gc(const int *__restrict__ ids, const DBox *__restrict__ boxes, T *var,
-                                           VarBc m, int corners, int bpb) {
+template <int D, int NC, int BS, typename T, int BB = 1>
+__global__ void __launch_bounds__(BS * BB) k_gc(const int *__restrict__ ids, const DBox *__restrict__ boxes, T *var,
+                                           VarBc m, int corners, int bpb, int nw) {
   using G = Geo<D, NC>;
-  const Part pt = part_of<BS>(ids, bpb);
-  if (pt.id <= 0) return;  // run masked out
-  const int id = pt.id;
-  const DBox &bx = boxes[id - 1];
-  T *u = var + (size_t)(id - 1) * G::NCELL;
-  constexpr int NFC = 2 * D * G::NTAN;
-  for (int q = pt.q0; q < NFC; q += pt.stride) {
-    const int nb = q / G::NTAN, r = q % G::NTAN;
-    const int a = r % NC + 1, b = (D == 3) ? r / NC + 1 : 0;
-    const bool low = (nb % 2) == 0;
-    int p[3];
-    face_pos<D>(nb, a, b, low ? 1 : NC, p);
-    const T x1 = u[G::idx(p[0], p[1], p[2])];
-    face_pos<D>(nb, a, b, low ? 2 : NC - 1, p);
-    const T x2 = u[G::idx(p[0], p[1], p[2])];
-    const T g = ghost_value<D, NC, T>(boxes, bx, var, m, nb, a, b, x1, x2);
-    face_pos<D>(nb, a, b, low ? 0 : NC + 1, p);
-    u[G::idx(p[0], p[1], p[2])] = g;
+  for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
+    const Part pt = part_of<BS>(ids, bpb, w);
+    if (pt.id <= 0) continue;  // run masked out
+    const int id = pt.id;
+    const DBox &bx = boxes[id - 1];
+    T *u = var + (size_t)(id - 1) * G::NCELL;
+    constexpr int NFC = 2 * D * G::NTAN;
+    for (int q = pt.q0; q < NFC; q += pt.stride) {
+      const int nb = q / G::NTAN, r = q % G::NTAN;
+      const int a = r % NC + 1, b = (D == 3) ? r / NC + 1 : 0;
+      const bool low = (nb % 2) == 0;
+      int p[3];
+      face_pos<D>(nb, a, b, low ? 1 : NC, p);
+      const T x1 = u[G::idx(p[0], p[1], p[2])];
+      face_pos<D>(nb, a, b, low ? 2 : NC - 1, p);
+      const T x2 = u[G::idx(p[0], p[1], p[2])];
+      const T g = ghost_value<D, NC, T>(boxes, bx, var, m, nb, a, b, x1, x2);
+      face_pos<D>(nb, a, b, low ? 0 : NC + 1, p);
+      u[G::idx(p[0], p[1], p[2])] = g;
+    }
+    if (!corners) continue;
+    gsync<BS, BB>();
+    if (D == 3) {
+      for (int q = ltid<BS>(); q < 12 * NC; q += BS) gc_edge_cell<D, NC, T>(bx, var, id, q / NC, q % NC + 1);
+      gsync<BS, BB>();
+    }
+    for (int q = ltid<BS>(); q < (1 << D); q += BS) gc_corner_cell<D, NC, T>(bx, var, id, q);
   }
-  if (!corners) return;
-  __syncthreads();
-  if (D == 3) {
-    for (int q = threadIdx.x; q < 12 * NC; q += BS) gc_edge_cell<D, NC, T>(bx, var, id, q / NC, q % NC + 1);
-    __syncthreads();
-  }
-  for (int q = threadIdx.x; q < (1 << D); q += BS) gc_corner_cell<D, NC, T>(bx, var, id, q);
 }
 
 // Red-black Gauss-Seidel half-sweep (stencil_gsrb_357,
@@ -309,58 +312,60 @@ __global__ void __launch_bounds__(BS) k_gc(const int *__restrict__ ids, const DB
 // inputs af_gc_box would use, so the updated cells are bit-identical while
 // no separate ghost-fill pass is needed between half-sweeps.  (The stored
 // ghost layer is refreshed once after the last half-sweep.)
-template <int D, int NC, int BS, typename T>
-__global__ void __launch_bounds__(BS) k_gsrb(const int *__restrict__ ids, const DBox *__restrict__ boxes, T *phi,
+template <int D, int NC, int BS, typename T, int BB = 1>
+__global__ void __launch_bounds__(BS * BB) k_gsrb(const int *__restrict__ ids, const DBox *__restrict__ boxes, T *phi,
                                              const T *__restrict__ rhs, int redblack, T lambda2, int cyl, VarBc m,
-                                             int bpb) {
+                                             int bpb, int nw) {
   using G = Geo<D, NC>;
-  const Part pt = part_of<BS>(ids, bpb);
-  if (pt.id <= 0) return;  // run masked out
-  const int id = pt.id;
-  const DBox &bx = boxes[id - 1];
-  const Sten<D, T> s = lpl_stencil<D, T>(bx, lambda2);
-  T *u = phi + (size_t)(id - 1) * G::NCELL;
-  const T *r = rhs + (size_t)(id - 1) * G::NCELL;
-  constexpr int HALF = G::NINT / 2;
-  constexpr int st[3] = {1, G::NG, G::NG * G::NG};
-  for (int q = pt.q0; q < HALF; q += pt.stride) {
-    const int mm = q % (NC / 2);
-    const int j = (q / (NC / 2)) % NC + 1;
-    const int k = (D == 3) ? q / ((NC / 2) * NC) + 1 : 0;
-    const int i0 = 2 - ((redblack ^ (D == 3 ? k + j : j)) & 1);
-    const int i = i0 + 2 * mm;
-    const int x = G::idx(i, j, k);
-    const int pos[3] = {i, j, k};
-    const T uc = u[x];
-    T nbv[6];
-    for (int nb = 0; nb < 2 * D; nb++) {
-      const int d = nb / 2;
-      const bool low = (nb % 2) == 0;
-      const bool edge = low ? pos[d] == 1 : pos[d] == NC;
-      if (!edge) {
-        nbv[nb] = u[low ? x - st[d] : x + st[d]];
-      } else {
-        int t0, t1;
-        tangential(D, d, t0, t1);
-        const T x2 = u[low ? x + st[d] : x - st[d]];
-        nbv[nb] = ghost_value<D, NC, T>(boxes, bx, phi, m, nb, pos[t0], (D == 3) ? pos[t1] : 0, uc, x2);
+  for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
+    const Part pt = part_of<BS>(ids, bpb, w);
+    if (pt.id <= 0) continue;  // run masked out
+    const int id = pt.id;
+    const DBox &bx = boxes[id - 1];
+    const Sten<D, T> s = lpl_stencil<D, T>(bx, lambda2);
+    T *u = phi + (size_t)(id - 1) * G::NCELL;
+    const T *r = rhs + (size_t)(id - 1) * G::NCELL;
+    constexpr int HALF = G::NINT / 2;
+    constexpr int st[3] = {1, G::NG, G::NG * G::NG};
+    for (int q = pt.q0; q < HALF; q += pt.stride) {
+      const int mm = q % (NC / 2);
+      const int j = (q / (NC / 2)) % NC + 1;
+      const int k = (D == 3) ? q / ((NC / 2) * NC) + 1 : 0;
+      const int i0 = 2 - ((redblack ^ (D == 3 ? k + j : j)) & 1);
+      const int i = i0 + 2 * mm;
+      const int x = G::idx(i, j, k);
+      const int pos[3] = {i, j, k};
+      const T uc = u[x];
+      T nbv[6];
+      for (int nb = 0; nb < 2 * D; nb++) {
+        const int d = nb / 2;
+        const bool low = (nb % 2) == 0;
+        const bool edge = low ? pos[d] == 1 : pos[d] == NC;
+        if (!edge) {
+          nbv[nb] = u[low ? x - st[d] : x + st[d]];
+        } else {
+          int t0, t1;
+          tangential(D, d, t0, t1);
+          const T x2 = u[low ? x + st[d] : x - st[d]];
+          nbv[nb] = ghost_value<D, NC, T>(boxes, bx, phi, m, nb, pos[t0], (D == 3) ? pos[t1] : 0, uc, x2);
+        }
       }
-    }
-    if (D == 2) {
-      if (cyl) {
-        T c1, c2, c3;
-        cyl_coeffs<T>(bx, i, s.c, c1, c2, c3);
-        T inv1 = T(1) / c1;
-        u[x] = (r[x] - c2 * nbv[0] - c3 * nbv[1] - s.c[3] * nbv[2] - s.c[4] * nbv[3]) * inv1;
+      if (D == 2) {
+        if (cyl) {
+          T c1, c2, c3;
+          cyl_coeffs<T>(bx, i, s.c, c1, c2, c3);
+          T inv1 = T(1) / c1;
+          u[x] = (r[x] - c2 * nbv[0] - c3 * nbv[1] - s.c[3] * nbv[2] - s.c[4] * nbv[3]) * inv1;
+        } else {
+          T inv_c1 = T(1) / s.c[0];
+          u[x] = (r[x] - s.c[1] * nbv[0] - s.c[2] * nbv[1] - s.c[3] * nbv[2] - s.c[4] * nbv[3]) * inv_c1;
+        }
       } else {
         T inv_c1 = T(1) / s.c[0];
-        u[x] = (r[x] - s.c[1] * nbv[0] - s.c[2] * nbv[1] - s.c[3] * nbv[2] - s.c[4] * nbv[3]) * inv_c1;
+        u[x] = (r[x] - s.c[1] * nbv[0] - s.c[2] * nbv[1] - s.c[3] * nbv[2] - s.c[4] * nbv[3] - s.c[5] * nbv[4] -
+                s.c[6] * nbv[5]) *
+               inv_c1;
       }
-    } else {
-      T inv_c1 = T(1) / s.c[0];
-      u[x] = (r[x] - s.c[1] * nbv[0] - s.c[2] * nbv[1] - s.c[3] * nbv[2] - s.c[4] * nbv[3] - s.c[5] * nbv[4] -
-              s.c[6] * nbv[5]) *
-             inv_c1;
     }
   }
 }
@@ -368,31 +373,33 @@ __global__ void __launch_bounds__(BS) k_gsrb(const int *__restrict__ ids, const 
 // tmp = rhs - L phi on interior cells (residual_box, m_af_multigrid.f90:801-810);
 // for leaf boxes the max |tmp| is also reduced into *leafmax
 // (af_tree_maxabs_cc, afivo/src/m_af_utils.f90:773).
-template <int D, int NC, int BS, typename T>
-__global__ void __launch_bounds__(BS) k_residual(const int *__restrict__ ids, const DBox *__restrict__ boxes,
+template <int D, int NC, int BS, typename T, int BB = 1>
+__global__ void __launch_bounds__(BS * BB) k_residual(const int *__restrict__ ids, const DBox *__restrict__ boxes,
                                                  const T *__restrict__ phi, const T *__restrict__ rhs, T *tmp,
-                                                 T lambda2, int cyl, unsigned long long *leafmax, int bpb) {
+                                                 T lambda2, int cyl, unsigned long long *leafmax, int bpb, int nw) {
   using G = Geo<D, NC>;
   __shared__ double sh[BS / 64];
-  const Part pt = part_of<BS>(ids, bpb);
-  if (pt.id <= 0) return;  // run masked out
-  const int id = pt.id;
-  const DBox &bx = boxes[id - 1];
-  const Sten<D, T> s = lpl_stencil<D, T>(bx, lambda2);
-  const size_t base = (size_t)(id - 1) * G::NCELL;
-  double mx = 0;
-  for (int q = pt.q0; q < G::NINT; q += pt.stride) {
-    int i, j, k;
-    interior_ijk<D, NC>(q, i, j, k);
-    const int x = G::idx(i, j, k);
-    T lp = apply_cell<D, NC, T>(phi + base, s, cyl != 0, bx, i, j, k);
-    const T rv = rhs[base + x] - lp;
-    tmp[base + x] = rv;
-    mx = fmax(mx, (double)fabs(rv));
-  }
-  if (leafmax && bx.children[0] == 0) {
-    mx = block_max<BS>(mx, sh);
-    if (threadIdx.x == 0) atomic_max_nonneg(run_slot(leafmax, bx), mx);
+  for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
+    const Part pt = part_of<BS>(ids, bpb, w);
+    if (pt.id <= 0) continue;  // run masked out
+    const int id = pt.id;
+    const DBox &bx = boxes[id - 1];
+    const Sten<D, T> s = lpl_stencil<D, T>(bx, lambda2);
+    const size_t base = (size_t)(id - 1) * G::NCELL;
+    double mx = 0;
+    for (int q = pt.q0; q < G::NINT; q += pt.stride) {
+      int i, j, k;
+      interior_ijk<D, NC>(q, i, j, k);
+      const int x = G::idx(i, j, k);
+      T lp = apply_cell<D, NC, T>(phi + base, s, cyl != 0, bx, i, j, k);
+      const T rv = rhs[base + x] - lp;
+      tmp[base + x] = rv;
+      mx = fmax(mx, (double)fabs(rv));
+    }
+    if (leafmax && bx.children[0] == 0) {
+      mx = group_max<BS, BB>(mx, sh);
+      if (ltid<BS>() == 0) atomic_max_nonneg(run_slot(leafmax, bx), mx);
+    }
   }
 }
 
@@ -428,26 +435,28 @@ __device__ inline T restrict_cell(const T *f, int i_f, int jf, int kf, bool cyl_
 }
 
 // Restrict variable `src` of child boxes into `dst` of their parents.
-template <int D, int NC, int BS, typename T>
-__global__ void __launch_bounds__(BS) k_restrict(const int *__restrict__ ids, const DBox *__restrict__ boxes,
-                                                 const T *src, T *dst, int use_geom, int cyl, int bpb) {
+template <int D, int NC, int BS, typename T, int BB = 1>
+__global__ void __launch_bounds__(BS * BB) k_restrict(const int *__restrict__ ids, const DBox *__restrict__ boxes,
+                                                 const T *src, T *dst, int use_geom, int cyl, int bpb, int nw) {
   using G = Geo<D, NC>;
   constexpr int H = NC / 2;
   constexpr int NQ = (D == 3) ? H * H * H : H * H;
-  const Part pt = part_of<BS>(ids, bpb);
-  if (pt.id <= 0) return;  // run masked out
-  const int id = pt.id;
-  const DBox &bc = boxes[id - 1];
-  const DBox &bp = boxes[bc.parent - 1];
-  int off[3] = {0, 0, 0};
-  for (int d = 0; d < D; d++) off[d] = ((bc.ix[d] - 1) & 1) * H;
-  const T *f = src + (size_t)(id - 1) * G::NCELL;
-  T *u = dst + (size_t)(bc.parent - 1) * G::NCELL;
-  for (int q = pt.q0; q < NQ; q += pt.stride) {
-    const int i = q % H + 1, j = (q / H) % H + 1, k = (D == 3) ? q / (H * H) + 1 : 0;
-    const int ic = off[0] + i, jc = off[1] + j, kc = (D == 3) ? off[2] + k : 0;
-    u[G::idx(ic, jc, kc)] =
-        restrict_cell<D, NC, T>(f, 2 * i - 1, 2 * j - 1, (D == 3) ? 2 * k - 1 : 0, cyl && use_geom, bp, ic);
+  for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
+    const Part pt = part_of<BS>(ids, bpb, w);
+    if (pt.id <= 0) continue;  // run masked out
+    const int id = pt.id;
+    const DBox &bc = boxes[id - 1];
+    const DBox &bp = boxes[bc.parent - 1];
+    int off[3] = {0, 0, 0};
+    for (int d = 0; d < D; d++) off[d] = ((bc.ix[d] - 1) & 1) * H;
+    const T *f = src + (size_t)(id - 1) * G::NCELL;
+    T *u = dst + (size_t)(bc.parent - 1) * G::NCELL;
+    for (int q = pt.q0; q < NQ; q += pt.stride) {
+      const int i = q % H + 1, j = (q / H) % H + 1, k = (D == 3) ? q / (H * H) + 1 : 0;
+      const int ic = off[0] + i, jc = off[1] + j, kc = (D == 3) ? off[2] + k : 0;
+      u[G::idx(ic, jc, kc)] =
+          restrict_cell<D, NC, T>(f, 2 * i - 1, 2 * j - 1, (D == 3) ? 2 * k - 1 : 0, cyl && use_geom, bp, ic);
+    }
   }
 }
 
@@ -457,96 +466,102 @@ __global__ void __launch_bounds__(BS) k_restrict(const int *__restrict__ ids, co
 // on the fly, which leaves the child's tmp untouched exactly as the
 // reference's save/restore of tmp does.  With store_residual the residual is
 // also kept in the child's tmp (set_coarse_phi_rhs, :742-776).
-template <int D, int NC, int BS, typename T>
-__global__ void __launch_bounds__(BS) k_restrict_residual(const int *__restrict__ ids, const DBox *__restrict__ boxes,
+template <int D, int NC, int BS, typename T, int BB = 1>
+__global__ void __launch_bounds__(BS * BB) k_restrict_residual(const int *__restrict__ ids, const DBox *__restrict__ boxes,
                                                           T *phi, const T *__restrict__ rhs, T *tmp, T lambda2,
-                                                          int cyl, int store_residual, int bpb) {
+                                                          int cyl, int store_residual, int bpb, int nw) {
   using G = Geo<D, NC>;
   constexpr int H = NC / 2;
   constexpr int NQ = (D == 3) ? H * H * H : H * H;
-  const Part pt = part_of<BS>(ids, bpb);
-  if (pt.id <= 0) return;  // run masked out
-  const int id = pt.id;
-  const DBox &bc = boxes[id - 1];
-  const DBox &bp = boxes[bc.parent - 1];
-  const Sten<D, T> s = lpl_stencil<D, T>(bc, lambda2);
-  const size_t cb = (size_t)(id - 1) * G::NCELL, pb = (size_t)(bc.parent - 1) * G::NCELL;
-  int off[3] = {0, 0, 0};
-  for (int d = 0; d < D; d++) off[d] = ((bc.ix[d] - 1) & 1) * H;
-  for (int q = pt.q0; q < NQ; q += pt.stride) {
-    const int i = q % H + 1, j = (q / H) % H + 1, k = (D == 3) ? q / (H * H) + 1 : 0;
-    const int i_f = 2 * i - 1, jf = 2 * j - 1, kf = (D == 3) ? 2 * k - 1 : 0;
-    // residuals of the 2^D children cells, in the summation order of
-    // af_restrict_box (i fastest, then j, then k)
-    T rs[8];
-    const int nch = (D == 3) ? 8 : 4;
-    for (int c = 0; c < nch; c++) {
-      const int ii = i_f + (c & 1), jj = jf + ((c >> 1) & 1), kk = kf + ((c >> 2) & 1);
-      T lp = apply_cell<D, NC, T>(phi + cb, s, cyl != 0, bc, ii, jj, kk);
-      rs[c] = rhs[cb + G::idx(ii, jj, kk)] - lp;
-      if (store_residual) tmp[cb + G::idx(ii, jj, kk)] = rs[c];
-    }
-    const int ic = off[0] + i, jc = off[1] + j, kc = (D == 3) ? off[2] + k : 0;
-    T rv;
-    if (D == 2) {
-      if (cyl) {
-        T r = (T)bp.r_min[0] + (T(ic) - T(0.5)) * (T)bp.dr[0];
-        T t = T(0.25) * (T)bp.dr[0] / r;
-        T w1 = T(1) - t, w2 = T(1) + t;
-        rv = T(0.25) * (w1 * (rs[0] + rs[2]) + w2 * (rs[1] + rs[3]));
+  for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
+    const Part pt = part_of<BS>(ids, bpb, w);
+    if (pt.id <= 0) continue;  // run masked out
+    const int id = pt.id;
+    const DBox &bc = boxes[id - 1];
+    const DBox &bp = boxes[bc.parent - 1];
+    const Sten<D, T> s = lpl_stencil<D, T>(bc, lambda2);
+    const size_t cb = (size_t)(id - 1) * G::NCELL, pb = (size_t)(bc.parent - 1) * G::NCELL;
+    int off[3] = {0, 0, 0};
+    for (int d = 0; d < D; d++) off[d] = ((bc.ix[d] - 1) & 1) * H;
+    for (int q = pt.q0; q < NQ; q += pt.stride) {
+      const int i = q % H + 1, j = (q / H) % H + 1, k = (D == 3) ? q / (H * H) + 1 : 0;
+      const int i_f = 2 * i - 1, jf = 2 * j - 1, kf = (D == 3) ? 2 * k - 1 : 0;
+      // residuals of the 2^D children cells, in the summation order of
+      // af_restrict_box (i fastest, then j, then k)
+      T rs[8];
+      const int nch = (D == 3) ? 8 : 4;
+      for (int c = 0; c < nch; c++) {
+        const int ii = i_f + (c & 1), jj = jf + ((c >> 1) & 1), kk = kf + ((c >> 2) & 1);
+        T lp = apply_cell<D, NC, T>(phi + cb, s, cyl != 0, bc, ii, jj, kk);
+        rs[c] = rhs[cb + G::idx(ii, jj, kk)] - lp;
+        if (store_residual) tmp[cb + G::idx(ii, jj, kk)] = rs[c];
+      }
+      const int ic = off[0] + i, jc = off[1] + j, kc = (D == 3) ? off[2] + k : 0;
+      T rv;
+      if (D == 2) {
+        if (cyl) {
+          T r = (T)bp.r_min[0] + (T(ic) - T(0.5)) * (T)bp.dr[0];
+          T t = T(0.25) * (T)bp.dr[0] / r;
+          T w1 = T(1) - t, w2 = T(1) + t;
+          rv = T(0.25) * (w1 * (rs[0] + rs[2]) + w2 * (rs[1] + rs[3]));
+        } else {
+          T sm = rs[0];
+          sm = sm + rs[1];
+          sm = sm + rs[2];
+          sm = sm + rs[3];
+          rv = T(0.25) * sm;
+        }
       } else {
         T sm = rs[0];
-        sm = sm + rs[1];
-        sm = sm + rs[2];
-        sm = sm + rs[3];
-        rv = T(0.25) * sm;
+        for (int c = 1; c < 8; c++) sm = sm + rs[c];
+        rv = T(0.125) * sm;
       }
-    } else {
-      T sm = rs[0];
-      for (int c = 1; c < 8; c++) sm = sm + rs[c];
-      rv = T(0.125) * sm;
+      tmp[pb + G::idx(ic, jc, kc)] = rv;
+      phi[pb + G::idx(ic, jc, kc)] = restrict_cell<D, NC, T>(phi + cb, i_f, jf, kf, false, bp, ic);
     }
-    tmp[pb + G::idx(ic, jc, kc)] = rv;
-    phi[pb + G::idx(ic, jc, kc)] = restrict_cell<D, NC, T>(phi + cb, i_f, jf, kf, false, bp, ic);
   }
 }
 
 // update_coarse / set_coarse_phi_rhs, second half (m_af_multigrid.f90:726-736):
 // on parents, rhs = L phi (interior) + tmp (all cells); optionally tmp = phi.
-template <int D, int NC, int BS, typename T>
-__global__ void __launch_bounds__(BS) k_coarse_rhs(const int *__restrict__ ids, const DBox *__restrict__ boxes,
+template <int D, int NC, int BS, typename T, int BB = 1>
+__global__ void __launch_bounds__(BS * BB) k_coarse_rhs(const int *__restrict__ ids, const DBox *__restrict__ boxes,
                                                    const T *__restrict__ phi, T *rhs, T *tmp, T lambda2, int cyl,
-                                                   int copy_phi, int bpb) {
+                                                   int copy_phi, int bpb, int nw) {
   using G = Geo<D, NC>;
-  const Part pt = part_of<BS>(ids, bpb);
-  if (pt.id <= 0) return;  // run masked out
-  const int id = pt.id;
-  const DBox &bx = boxes[id - 1];
-  const Sten<D, T> s = lpl_stencil<D, T>(bx, lambda2);
-  const size_t base = (size_t)(id - 1) * G::NCELL;
-  for (int x = pt.q0; x < G::NCELL; x += pt.stride) {
-    const int i = x % G::NG, j = (x / G::NG) % G::NG, k = (D == 3) ? x / (G::NG * G::NG) : 0;
-    const bool inner = i >= 1 && i <= NC && j >= 1 && j <= NC && (D == 2 || (k >= 1 && k <= NC));
-    T rv = inner ? apply_cell<D, NC, T>(phi + base, s, cyl != 0, bx, i, j, k) : rhs[base + x];
-    rhs[base + x] = rv + tmp[base + x];
-    if (copy_phi) tmp[base + x] = phi[base + x];
+  for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
+    const Part pt = part_of<BS>(ids, bpb, w);
+    if (pt.id <= 0) continue;  // run masked out
+    const int id = pt.id;
+    const DBox &bx = boxes[id - 1];
+    const Sten<D, T> s = lpl_stencil<D, T>(bx, lambda2);
+    const size_t base = (size_t)(id - 1) * G::NCELL;
+    for (int x = pt.q0; x < G::NCELL; x += pt.stride) {
+      const int i = x % G::NG, j = (x / G::NG) % G::NG, k = (D == 3) ? x / (G::NG * G::NG) : 0;
+      const bool inner = i >= 1 && i <= NC && j >= 1 && j <= NC && (D == 2 || (k >= 1 && k <= NC));
+      T rv = inner ? apply_cell<D, NC, T>(phi + base, s, cyl != 0, bx, i, j, k) : rhs[base + x];
+      rhs[base + x] = rv + tmp[base + x];
+      if (copy_phi) tmp[base + x] = phi[base + x];
+    }
   }
 }
 
 // Whole-box elementwise operations (afivo/src/m_af_utils.f90:385-576).
 enum BoxOp { OP_COPY = 0, OP_CLEAR = 1, OP_SUB_FROM = 2, OP_AXPY_NEG = 3 };
 // OP_COPY: b = a; OP_CLEAR: b = 0; OP_SUB_FROM: b = a - b; OP_AXPY_NEG: b = b - s*a
-template <int NCELL, int BS, typename T>
-__global__ void __launch_bounds__(BS) k_boxop(const int *__restrict__ ids, const T *a, T *b, int op, T s, int bpb) {
-  const Part pt = part_of<BS>(ids, bpb);
-  if (pt.id <= 0) return;  // run masked out
-  const size_t base = (size_t)(pt.id - 1) * NCELL;
-  for (int x = pt.q0; x < NCELL; x += pt.stride) {
-    switch (op) {
-      case OP_COPY: b[base + x] = a[base + x]; break;
-      case OP_CLEAR: b[base + x] = T(0); break;
-      case OP_SUB_FROM: b[base + x] = a[base + x] - b[base + x]; break;
-      default: b[base + x] = b[base + x] - s * a[base + x]; break;
+template <int NCELL, int BS, typename T, int BB = 1>
+__global__ void __launch_bounds__(BS * BB) k_boxop(const int *__restrict__ ids, const T *a, T *b, int op, T s, int bpb, int nw) {
+  for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
+    const Part pt = part_of<BS>(ids, bpb, w);
+    if (pt.id <= 0) continue;  // run masked out
+    const size_t base = (size_t)(pt.id - 1) * NCELL;
+    for (int x = pt.q0; x < NCELL; x += pt.stride) {
+      switch (op) {
+        case OP_COPY: b[base + x] = a[base + x]; break;
+        case OP_CLEAR: b[base + x] = T(0); break;
+        case OP_SUB_FROM: b[base + x] = a[base + x] - b[base + x]; break;
+        default: b[base + x] = b[base + x] - s * a[base + x]; break;
+      }
     }
   }
 }
@@ -554,57 +569,61 @@ __global__ void __launch_bounds__(BS) k_boxop(const int *__restrict__ ids, const
 // phi(child) += P(tmp(parent)): stencil_prolong_248 with add
 // (afivo/src/m_af_stencil.f90:695-815), weights of
 // mg_box_prolong_linear_stencil (m_af_multigrid.f90:1267-1284).
-template <int D, int NC, int BS, typename T>
-__global__ void __launch_bounds__(BS) k_prolong_add(const int *__restrict__ ids, const DBox *__restrict__ boxes,
-                                                    const T *__restrict__ src, T *dst, int bpb) {
+template <int D, int NC, int BS, typename T, int BB = 1>
+__global__ void __launch_bounds__(BS * BB) k_prolong_add(const int *__restrict__ ids, const DBox *__restrict__ boxes,
+                                                    const T *__restrict__ src, T *dst, int bpb, int nw) {
   using G = Geo<D, NC>;
   constexpr int H = NC / 2;
-  const Part pt = part_of<BS>(ids, bpb);
-  if (pt.id <= 0) return;  // run masked out
-  const int id = pt.id;
-  const DBox &bc = boxes[id - 1];
-  const T *P = src + (size_t)(bc.parent - 1) * G::NCELL;
-  T *u = dst + (size_t)(id - 1) * G::NCELL;
-  int off[3] = {0, 0, 0};
-  for (int d = 0; d < D; d++) off[d] = ((bc.ix[d] - 1) & 1) * H;
-  for (int q = pt.q0; q < G::NINT; q += pt.stride) {
-    int i, j, k;
-    interior_ijk<D, NC>(q, i, j, k);
-    const int i1 = off[0] + ((i + 1) >> 1), i2 = i1 + 1 - 2 * (i & 1);
-    const int j1 = off[1] + ((j + 1) >> 1), j2 = j1 + 1 - 2 * (j & 1);
-    const int x = G::idx(i, j, k);
-    if (D == 2) {
-      const T w0 = T(9 / 16.0), w1 = T(3 / 16.0), w2 = T(3 / 16.0), w3 = T(1 / 16.0);
-      u[x] = u[x] + w0 * P[G::idx(i1, j1, 0)] + w1 * P[G::idx(i2, j1, 0)] + w2 * P[G::idx(i1, j2, 0)] +
-             w3 * P[G::idx(i2, j2, 0)];
-    } else {
-      const int k1 = off[2] + ((k + 1) >> 1), k2 = k1 + 1 - 2 * (k & 1);
-      const T w0 = T(27 / 64.0), w1 = T(9 / 64.0), w3 = T(3 / 64.0), w7 = T(1 / 64.0);
-      u[x] = u[x] + w0 * P[G::idx(i1, j1, k1)] + w1 * P[G::idx(i2, j1, k1)] + w1 * P[G::idx(i1, j2, k1)] +
-             w3 * P[G::idx(i2, j2, k1)] + w1 * P[G::idx(i1, j1, k2)] + w3 * P[G::idx(i2, j1, k2)] +
-             w3 * P[G::idx(i1, j2, k2)] + w7 * P[G::idx(i2, j2, k2)];
+  for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
+    const Part pt = part_of<BS>(ids, bpb, w);
+    if (pt.id <= 0) continue;  // run masked out
+    const int id = pt.id;
+    const DBox &bc = boxes[id - 1];
+    const T *P = src + (size_t)(bc.parent - 1) * G::NCELL;
+    T *u = dst + (size_t)(id - 1) * G::NCELL;
+    int off[3] = {0, 0, 0};
+    for (int d = 0; d < D; d++) off[d] = ((bc.ix[d] - 1) & 1) * H;
+    for (int q = pt.q0; q < G::NINT; q += pt.stride) {
+      int i, j, k;
+      interior_ijk<D, NC>(q, i, j, k);
+      const int i1 = off[0] + ((i + 1) >> 1), i2 = i1 + 1 - 2 * (i & 1);
+      const int j1 = off[1] + ((j + 1) >> 1), j2 = j1 + 1 - 2 * (j & 1);
+      const int x = G::idx(i, j, k);
+      if (D == 2) {
+        const T w0 = T(9 / 16.0), w1 = T(3 / 16.0), w2 = T(3 / 16.0), w3 = T(1 / 16.0);
+        u[x] = u[x] + w0 * P[G::idx(i1, j1, 0)] + w1 * P[G::idx(i2, j1, 0)] + w2 * P[G::idx(i1, j2, 0)] +
+               w3 * P[G::idx(i2, j2, 0)];
+      } else {
+        const int k1 = off[2] + ((k + 1) >> 1), k2 = k1 + 1 - 2 * (k & 1);
+        const T w0 = T(27 / 64.0), w1 = T(9 / 64.0), w3 = T(3 / 64.0), w7 = T(1 / 64.0);
+        u[x] = u[x] + w0 * P[G::idx(i1, j1, k1)] + w1 * P[G::idx(i2, j1, k1)] + w1 * P[G::idx(i1, j2, k1)] +
+               w3 * P[G::idx(i2, j2, k1)] + w1 * P[G::idx(i1, j1, k2)] + w3 * P[G::idx(i2, j1, k2)] +
+               w3 * P[G::idx(i1, j2, k2)] + w7 * P[G::idx(i2, j2, k2)];
+      }
     }
   }
 }
 
 // max |u| over interior cells of a box list (af_tree_maxabs_cc,
 // afivo/src/m_af_utils.f90:773).
-template <int D, int NC, int BS>
-__global__ void __launch_bounds__(BS) k_maxabs(const int *__restrict__ ids, const DBox *__restrict__ boxes,
-                                               const double *__restrict__ u, unsigned long long *out, int bpb) {
+template <int D, int NC, int BS, int BB = 1>
+__global__ void __launch_bounds__(BS * BB) k_maxabs(const int *__restrict__ ids, const DBox *__restrict__ boxes,
+                                               const double *__restrict__ u, unsigned long long *out, int bpb, int nw) {
   using G = Geo<D, NC>;
   __shared__ double sh[BS / 64];
-  const Part pt = part_of<BS>(ids, bpb);
-  if (pt.id <= 0) return;  // run masked out
-  const size_t base = (size_t)(pt.id - 1) * G::NCELL;
-  double m = 0;
-  for (int q = pt.q0; q < G::NINT; q += pt.stride) {
-    int i, j, k;
-    interior_ijk<D, NC>(q, i, j, k);
-    m = fmax(m, fabs(u[base + G::idx(i, j, k)]));
+  for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
+    const Part pt = part_of<BS>(ids, bpb, w);
+    if (pt.id <= 0) continue;  // run masked out
+    const size_t base = (size_t)(pt.id - 1) * G::NCELL;
+    double m = 0;
+    for (int q = pt.q0; q < G::NINT; q += pt.stride) {
+      int i, j, k;
+      interior_ijk<D, NC>(q, i, j, k);
+      m = fmax(m, fabs(u[base + G::idx(i, j, k)]));
+    }
+    m = group_max<BS, BB>(m, sh);
+    if (ltid<BS>() == 0) atomic_max_nonneg(run_slot(out, boxes[pt.id - 1]), m);
   }
-  m = block_max<BS>(m, sh);
-  if (threadIdx.x == 0) atomic_max_nonneg(run_slot(out, boxes[pt.id - 1]), m);
 }
 
 // Exact level-1 solve by fast diagonalisation (replaces hypre PFMG,
@@ -711,14 +730,16 @@ __global__ void __launch_bounds__(CBS) k_coarse_solve(const int *__restrict__ id
 
 // phi += e on whole boxes, then e = 0 so the next cycle starts from a zero
 // correction.
-template <int NCELL, int BS>
-__global__ void __launch_bounds__(BS) k_mp_correct(const int *__restrict__ ids, double *phi, float *e, int bpb) {
-  const Part pt = part_of<BS>(ids, bpb);
-  if (pt.id <= 0) return;  // run masked out
-  const size_t base = (size_t)(pt.id - 1) * NCELL;
-  for (int x = pt.q0; x < NCELL; x += pt.stride) {
-    phi[base + x] = phi[base + x] + (double)e[base + x];
-    e[base + x] = 0.0f;
+template <int NCELL, int BS, int BB = 1>
+__global__ void __launch_bounds__(BS * BB) k_mp_correct(const int *__restrict__ ids, double *phi, float *e, int bpb, int nw) {
+  for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
+    const Part pt = part_of<BS>(ids, bpb, w);
+    if (pt.id <= 0) continue;  // run masked out
+    const size_t base = (size_t)(pt.id - 1) * NCELL;
+    for (int x = pt.q0; x < NCELL; x += pt.stride) {
+      phi[base + x] = phi[base + x] + (double)e[base + x];
+      e[base + x] = 0.0f;
+    }
   }
 }
 
@@ -727,62 +748,64 @@ __global__ void __launch_bounds__(BS) k_mp_correct(const int *__restrict__ ids, 
 // layer of phi need not be current), stored in single precision as the rhs
 // of the next correction cycle; max |residual| over leaf boxes is reduced
 // into *leafmax.
-template <int D, int NC, int BS>
-__global__ void __launch_bounds__(BS) k_mp_residual(const int *__restrict__ ids, const DBox *__restrict__ boxes,
+template <int D, int NC, int BS, int BB = 1>
+__global__ void __launch_bounds__(BS * BB) k_mp_residual(const int *__restrict__ ids, const DBox *__restrict__ boxes,
                                                     const double *__restrict__ phi, const double *__restrict__ rhs,
                                                     float *r32, double lambda2, int cyl, VarBc m,
-                                                    unsigned long long *leafmax, int bpb) {
+                                                    unsigned long long *leafmax, int bpb, int nw) {
   using G = Geo<D, NC>;
   __shared__ double sh[BS / 64];
-  const Part pt = part_of<BS>(ids, bpb);
-  if (pt.id <= 0) return;  // run masked out
-  const int id = pt.id;
-  const DBox &bx = boxes[id - 1];
-  const Sten<D, double> s = lpl_stencil<D, double>(bx, lambda2);
-  const size_t base = (size_t)(id - 1) * G::NCELL;
-  const double *u = phi + base;
-  constexpr int st[3] = {1, G::NG, G::NG * G::NG};
-  double mx = 0;
-  for (int q = pt.q0; q < G::NINT; q += pt.stride) {
-    int i, j, k;
-    interior_ijk<D, NC>(q, i, j, k);
-    const int x = G::idx(i, j, k);
-    const int pos[3] = {i, j, k};
-    const double uc = u[x];
-    double nbv[6];
-    for (int nb = 0; nb < 2 * D; nb++) {
-      const int d = nb / 2;
-      const bool low = (nb % 2) == 0;
-      const bool edge = low ? pos[d] == 1 : pos[d] == NC;
-      if (!edge) {
-        nbv[nb] = u[low ? x - st[d] : x + st[d]];
-      } else {
-        int t0, t1;
-        tangential(D, d, t0, t1);
-        const double x2 = u[low ? x + st[d] : x - st[d]];
-        nbv[nb] = ghost_value<D, NC, double>(boxes, bx, phi, m, nb, pos[t0], (D == 3) ? pos[t1] : 0, uc, x2);
+  for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
+    const Part pt = part_of<BS>(ids, bpb, w);
+    if (pt.id <= 0) continue;  // run masked out
+    const int id = pt.id;
+    const DBox &bx = boxes[id - 1];
+    const Sten<D, double> s = lpl_stencil<D, double>(bx, lambda2);
+    const size_t base = (size_t)(id - 1) * G::NCELL;
+    const double *u = phi + base;
+    constexpr int st[3] = {1, G::NG, G::NG * G::NG};
+    double mx = 0;
+    for (int q = pt.q0; q < G::NINT; q += pt.stride) {
+      int i, j, k;
+      interior_ijk<D, NC>(q, i, j, k);
+      const int x = G::idx(i, j, k);
+      const int pos[3] = {i, j, k};
+      const double uc = u[x];
+      double nbv[6];
+      for (int nb = 0; nb < 2 * D; nb++) {
+        const int d = nb / 2;
+        const bool low = (nb % 2) == 0;
+        const bool edge = low ? pos[d] == 1 : pos[d] == NC;
+        if (!edge) {
+          nbv[nb] = u[low ? x - st[d] : x + st[d]];
+        } else {
+          int t0, t1;
+          tangential(D, d, t0, t1);
+          const double x2 = u[low ? x + st[d] : x - st[d]];
+          nbv[nb] = ghost_value<D, NC, double>(boxes, bx, phi, m, nb, pos[t0], (D == 3) ? pos[t1] : 0, uc, x2);
+        }
       }
-    }
-    double lp;
-    if (D == 2) {
-      if (cyl) {
-        double c1, c2, c3;
-        cyl_coeffs<double>(bx, i, s.c, c1, c2, c3);
-        lp = c1 * uc + c2 * nbv[0] + c3 * nbv[1] + s.c[3] * nbv[2] + s.c[4] * nbv[3];
+      double lp;
+      if (D == 2) {
+        if (cyl) {
+          double c1, c2, c3;
+          cyl_coeffs<double>(bx, i, s.c, c1, c2, c3);
+          lp = c1 * uc + c2 * nbv[0] + c3 * nbv[1] + s.c[3] * nbv[2] + s.c[4] * nbv[3];
+        } else {
+          lp = s.c[0] * uc + s.c[1] * nbv[0] + s.c[2] * nbv[1] + s.c[3] * nbv[2] + s.c[4] * nbv[3];
+        }
       } else {
-        lp = s.c[0] * uc + s.c[1] * nbv[0] + s.c[2] * nbv[1] + s.c[3] * nbv[2] + s.c[4] * nbv[3];
+        lp = s.c[0] * uc + s.c[1] * nbv[0] + s.c[2] * nbv[1] + s.c[3] * nbv[2] + s.c[4] * nbv[3] +
+             s.c[5] * nbv[4] + s.c[6] * nbv[5];
       }
-    } else {
-      lp = s.c[0] * uc + s.c[1] * nbv[0] + s.c[2] * nbv[1] + s.c[3] * nbv[2] + s.c[4] * nbv[3] +
-           s.c[5] * nbv[4] + s.c[6] * nbv[5];
+      const double rv = rhs[base + x] - lp;
+      r32[base + x] = (float)rv;
+      mx = fmax(mx, fabs(rv));
     }
-    const double rv = rhs[base + x] - lp;
-    r32[base + x] = (float)rv;
-    mx = fmax(mx, fabs(rv));
-  }
-  if (bx.children[0] == 0) {
-    mx = block_max<BS>(mx, sh);
-    if (threadIdx.x == 0) atomic_max_nonneg(run_slot(leafmax, bx), mx);
+    if (bx.children[0] == 0) {
+      mx = group_max<BS, BB>(mx, sh);
+      if (ltid<BS>() == 0) atomic_max_nonneg(run_slot(leafmax, bx), mx);
+    }
   }
 }
 
@@ -793,10 +816,10 @@ __global__ void __launch_bounds__(BS) k_mp_residual(const int *__restrict__ ids,
 // stencil then reads LDS only.  Same arithmetic, in the same order, as the
 // direct kernels above.
 
-template <int D, int NC, int BS, typename T>
+template <int D, int NC, int BS, typename T, int BB = 1>
 __device__ inline void lds_load_interior(T *s, const T *u) {
   using G = Geo<D, NC>;
-  for (int q = threadIdx.x; q < G::NINT; q += BS) {
+  for (int q = ltid<BS>(); q < G::NINT; q += BS) {
     int i, j, k;
     interior_ijk<D, NC>(q, i, j, k);
     const int x = G::idx(i, j, k);
@@ -807,12 +830,12 @@ __device__ inline void lds_load_interior(T *s, const T *u) {
 // Face ghosts into LDS.  colour >= 0: only the ghosts next to interior cells
 // of that GSRB colour ((i + j + k + colour) even), the only ones its
 // half-sweep reads; colour < 0: all face ghosts.
-template <int D, int NC, int BS, typename T>
+template <int D, int NC, int BS, typename T, int BB = 1>
 __device__ inline void lds_face_ghosts(T *s, const DBox *__restrict__ boxes, const DBox &bx, const T *var,
                                        const VarBc &m, int colour, const T *coarse = nullptr) {
   using G = Geo<D, NC>;
   constexpr int NFC = 2 * D * G::NTAN;
-  for (int q = threadIdx.x; q < NFC; q += BS) {
+  for (int q = ltid<BS>(); q < NFC; q += BS) {
     const int nb = q / G::NTAN, r = q % G::NTAN;
     const int a = r % NC + 1, b = (D == 3) ? r / NC + 1 : 0;
     const bool low = (nb % 2) == 0;
@@ -834,7 +857,7 @@ __device__ inline void lds_face_ghosts(T *s, const DBox *__restrict__ boxes, con
 // own cells (physical boundary, refinement boundary) are formed from LDS.
 // colour >= 0: only the ghosts next to interior cells of that GSRB colour.
 // Ends with a barrier.
-template <int D, int NC, int BS, typename T>
+template <int D, int NC, int BS, typename T, int BB = 1>
 __device__ inline void lds_stage(T *s, const DBox *__restrict__ boxes, const DBox &bx, int id, const T *var,
                                  const VarBc &m, int colour, const T *coarse = nullptr) {
   using G = Geo<D, NC>;
@@ -845,7 +868,7 @@ __device__ inline void lds_stage(T *s, const DBox *__restrict__ boxes, const DBo
   T vi[NI], vf[NF];
   int gi[NF];
   for (int t = 0; t < NI; t++) {
-    const int q = threadIdx.x + t * BS;
+    const int q = ltid<BS>() + t * BS;
     if (q < G::NINT) {
       int i, j, k;
       interior_ijk<D, NC>(q, i, j, k);
@@ -854,7 +877,7 @@ __device__ inline void lds_stage(T *s, const DBox *__restrict__ boxes, const DBo
   }
   for (int t = 0; t < NF; t++) {
     gi[t] = -1;
-    const int q = threadIdx.x + t * BS;
+    const int q = ltid<BS>() + t * BS;
     if (q >= NFC) continue;
     const int nb = q / G::NTAN, r = q % G::NTAN;
     const int nb_id = bx.nb[nb];
@@ -871,7 +894,7 @@ __device__ inline void lds_stage(T *s, const DBox *__restrict__ boxes, const DBo
     vf[t] = var[(size_t)(nb_id - 1) * G::NCELL + G::idx(p[0], p[1], p[2])];
   }
   for (int t = 0; t < NI; t++) {
-    const int q = threadIdx.x + t * BS;
+    const int q = ltid<BS>() + t * BS;
     if (q < G::NINT) {
       int i, j, k;
       interior_ijk<D, NC>(q, i, j, k);
@@ -880,11 +903,11 @@ __device__ inline void lds_stage(T *s, const DBox *__restrict__ boxes, const DBo
   }
   for (int t = 0; t < NF; t++)
     if (gi[t] >= 0) s[gi[t]] = vf[t];
-  __syncthreads();
+  gsync<BS, BB>();
   bool any = false;
   for (int nb = 0; nb < 2 * D; nb++) any = any || bx.nb[nb] <= 0;
   if (any) {
-    for (int q = threadIdx.x; q < NFC; q += BS) {
+    for (int q = ltid<BS>(); q < NFC; q += BS) {
       const int nb = q / G::NTAN, r = q % G::NTAN;
       if (bx.nb[nb] > 0) continue;
       const int a = r % NC + 1, b = (D == 3) ? r / NC + 1 : 0;
@@ -899,178 +922,190 @@ __device__ inline void lds_stage(T *s, const DBox *__restrict__ boxes, const DBo
       face_pos<D>(nb, a, b, low ? 0 : NC + 1, p);
       s[G::idx(p[0], p[1], p[2])] = g;
     }
-    __syncthreads();
+    gsync<BS, BB>();
   }
 }
 
 // k_gsrb with the box staged in LDS (levels with one workgroup per box).
-template <int D, int NC, int BS, typename T>
-__global__ void __launch_bounds__(BS) k_gsrb_lds(const int *__restrict__ ids, const DBox *__restrict__ boxes, T *phi,
-                                                 const T *__restrict__ rhs, int redblack, T lambda2, int cyl, VarBc m) {
+template <int D, int NC, int BS, typename T, int BB = 1>
+__global__ void __launch_bounds__(BS * BB) k_gsrb_lds(const int *__restrict__ ids, const DBox *__restrict__ boxes, T *phi,
+                                                 const T *__restrict__ rhs, int redblack, T lambda2, int cyl, VarBc m, int nw) {
   using G = Geo<D, NC>;
-  __shared__ T s[G::NCELL];
-  const int id = ids[blockIdx.x];
-  if (id <= 0) return;  // run masked out
-  const DBox &bx = boxes[id - 1];
-  const Sten<D, T> st = lpl_stencil<D, T>(bx, lambda2);
-  T *u = phi + (size_t)(id - 1) * G::NCELL;
-  const T *r = rhs + (size_t)(id - 1) * G::NCELL;
-  constexpr int HALF = G::NINT / 2;
-  constexpr int NH = (HALF + BS - 1) / BS;
-  constexpr int sx = 1, sy = G::NG, sz = G::NG * G::NG;
-  // updated cell t of this thread, and its rhs (loaded with the box)
-  auto cell = [&](int q, int &i, int &j, int &k) {
-    const int mm = q % (NC / 2);
-    j = (q / (NC / 2)) % NC + 1;
-    k = (D == 3) ? q / ((NC / 2) * NC) + 1 : 0;
-    const int i0 = 2 - ((redblack ^ (D == 3 ? k + j : j)) & 1);
-    i = i0 + 2 * mm;
-  };
-  T rr[NH];
-  for (int t = 0; t < NH; t++) {
-    const int q = threadIdx.x + t * BS;
-    if (q < HALF) {
+  __shared__ T s_all[BB][G::NCELL];
+  T *s = s_all[lgrp<BS, BB>()];
+  for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
+    const int id = ids[w];
+    if (id <= 0) continue;  // run masked out
+    const DBox &bx = boxes[id - 1];
+    const Sten<D, T> st = lpl_stencil<D, T>(bx, lambda2);
+    T *u = phi + (size_t)(id - 1) * G::NCELL;
+    const T *r = rhs + (size_t)(id - 1) * G::NCELL;
+    constexpr int HALF = G::NINT / 2;
+    constexpr int NH = (HALF + BS - 1) / BS;
+    constexpr int sx = 1, sy = G::NG, sz = G::NG * G::NG;
+    // updated cell t of this thread, and its rhs (loaded with the box)
+    auto cell = [&](int q, int &i, int &j, int &k) {
+      const int mm = q % (NC / 2);
+      j = (q / (NC / 2)) % NC + 1;
+      k = (D == 3) ? q / ((NC / 2) * NC) + 1 : 0;
+      const int i0 = 2 - ((redblack ^ (D == 3 ? k + j : j)) & 1);
+      i = i0 + 2 * mm;
+    };
+    T rr[NH];
+    for (int t = 0; t < NH; t++) {
+      const int q = ltid<BS>() + t * BS;
+      if (q < HALF) {
+        int i, j, k;
+        cell(q, i, j, k);
+        rr[t] = r[G::idx(i, j, k)];
+      }
+    }
+    // the updated cells satisfy (i + j + k + redblack) even
+    lds_stage<D, NC, BS, T, BB>(s, boxes, bx, id, phi, m, redblack & 1);
+    for (int t = 0; t < NH; t++) {
+      const int q = ltid<BS>() + t * BS;
+      if (q >= HALF) break;
       int i, j, k;
       cell(q, i, j, k);
-      rr[t] = r[G::idx(i, j, k)];
-    }
-  }
-  // the updated cells satisfy (i + j + k + redblack) even
-  lds_stage<D, NC, BS, T>(s, boxes, bx, id, phi, m, redblack & 1);
-  for (int t = 0; t < NH; t++) {
-    const int q = threadIdx.x + t * BS;
-    if (q >= HALF) break;
-    int i, j, k;
-    cell(q, i, j, k);
-    const int x = G::idx(i, j, k);
-    const T rx = rr[t];
-    if (D == 2) {
-      if (cyl) {
-        T c1, c2, c3;
-        cyl_coeffs<T>(bx, i, st.c, c1, c2, c3);
-        T inv1 = T(1) / c1;
-        s[x] = (rx - c2 * s[x - sx] - c3 * s[x + sx] - st.c[3] * s[x - sy] - st.c[4] * s[x + sy]) * inv1;
+      const int x = G::idx(i, j, k);
+      const T rx = rr[t];
+      if (D == 2) {
+        if (cyl) {
+          T c1, c2, c3;
+          cyl_coeffs<T>(bx, i, st.c, c1, c2, c3);
+          T inv1 = T(1) / c1;
+          s[x] = (rx - c2 * s[x - sx] - c3 * s[x + sx] - st.c[3] * s[x - sy] - st.c[4] * s[x + sy]) * inv1;
+        } else {
+          T inv_c1 = T(1) / st.c[0];
+          s[x] = (rx - st.c[1] * s[x - sx] - st.c[2] * s[x + sx] - st.c[3] * s[x - sy] - st.c[4] * s[x + sy]) *
+                 inv_c1;
+        }
       } else {
         T inv_c1 = T(1) / st.c[0];
-        s[x] = (rx - st.c[1] * s[x - sx] - st.c[2] * s[x + sx] - st.c[3] * s[x - sy] - st.c[4] * s[x + sy]) *
+        s[x] = (rx - st.c[1] * s[x - sx] - st.c[2] * s[x + sx] - st.c[3] * s[x - sy] - st.c[4] * s[x + sy] -
+                st.c[5] * s[x - sz] - st.c[6] * s[x + sz]) *
                inv_c1;
       }
-    } else {
-      T inv_c1 = T(1) / st.c[0];
-      s[x] = (rx - st.c[1] * s[x - sx] - st.c[2] * s[x + sx] - st.c[3] * s[x - sy] - st.c[4] * s[x + sy] -
-              st.c[5] * s[x - sz] - st.c[6] * s[x + sz]) *
-             inv_c1;
     }
-  }
-  // Whole interior rows back (the other colour unchanged): full-line
-  // stores instead of every other word, so no line has to be fetched to
-  // merge a partial write.  Neighbouring boxes only read this box's other
-  // colour during the launch, which keeps its value.
-  __syncthreads();
-  for (int q = threadIdx.x; q < G::NINT; q += BS) {
-    int i, j, k;
-    interior_ijk<D, NC>(q, i, j, k);
-    const int x = G::idx(i, j, k);
-    u[x] = s[x];
+    // Whole interior rows back (the other colour unchanged): full-line
+    // stores instead of every other word, so no line has to be fetched to
+    // merge a partial write.  Neighbouring boxes only read this box's other
+    // colour during the launch, which keeps its value.
+    gsync<BS, BB>();
+    for (int q = ltid<BS>(); q < G::NINT; q += BS) {
+      int i, j, k;
+      interior_ijk<D, NC>(q, i, j, k);
+      const int x = G::idx(i, j, k);
+      u[x] = s[x];
+    }
+    gsync<BS, BB>();  // (the group's LDS is reused by its next item)
   }
 }
 
 // k_mp_residual with the box staged in LDS.
-template <int D, int NC, int BS>
-__global__ void __launch_bounds__(BS) k_mp_residual_lds(const int *__restrict__ ids, const DBox *__restrict__ boxes,
+template <int D, int NC, int BS, int BB = 1>
+__global__ void __launch_bounds__(BS * BB) k_mp_residual_lds(const int *__restrict__ ids, const DBox *__restrict__ boxes,
                                                         const double *__restrict__ phi,
                                                         const double *__restrict__ rhs, float *r32, double lambda2,
-                                                        int cyl, VarBc m, unsigned long long *leafmax) {
+                                                        int cyl, VarBc m, unsigned long long *leafmax, int nw) {
   using G = Geo<D, NC>;
-  __shared__ double s[G::NCELL];
+  __shared__ double s_all[BB][G::NCELL];
+  double *s = s_all[lgrp<BS, BB>()];
   __shared__ double sh[BS / 64];
-  const int id = ids[blockIdx.x];
-  if (id <= 0) return;  // run masked out
-  const DBox &bx = boxes[id - 1];
-  const Sten<D, double> st = lpl_stencil<D, double>(bx, lambda2);
-  const size_t base = (size_t)(id - 1) * G::NCELL;
-  constexpr int NI = (G::NINT + BS - 1) / BS;
-  double rr[NI];
-  for (int t = 0; t < NI; t++) {
-    const int q = threadIdx.x + t * BS;
-    if (q < G::NINT) {
+  for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
+    const int id = ids[w];
+    if (id <= 0) continue;  // run masked out
+    const DBox &bx = boxes[id - 1];
+    const Sten<D, double> st = lpl_stencil<D, double>(bx, lambda2);
+    const size_t base = (size_t)(id - 1) * G::NCELL;
+    constexpr int NI = (G::NINT + BS - 1) / BS;
+    double rr[NI];
+    for (int t = 0; t < NI; t++) {
+      const int q = ltid<BS>() + t * BS;
+      if (q < G::NINT) {
+        int i, j, k;
+        interior_ijk<D, NC>(q, i, j, k);
+        rr[t] = rhs[base + G::idx(i, j, k)];
+      }
+    }
+    lds_stage<D, NC, BS, double, BB>(s, boxes, bx, id, phi, m, -1);
+    double mx = 0;
+    for (int t = 0; t < NI; t++) {
+      const int q = ltid<BS>() + t * BS;
+      if (q >= G::NINT) break;
       int i, j, k;
       interior_ijk<D, NC>(q, i, j, k);
-      rr[t] = rhs[base + G::idx(i, j, k)];
+      const int x = G::idx(i, j, k);
+      const double lp = apply_cell<D, NC, double>(s, st, cyl != 0, bx, i, j, k);
+      const double rv = rr[t] - lp;
+      r32[base + x] = (float)rv;
+      mx = fmax(mx, fabs(rv));
     }
-  }
-  lds_stage<D, NC, BS, double>(s, boxes, bx, id, phi, m, -1);
-  double mx = 0;
-  for (int t = 0; t < NI; t++) {
-    const int q = threadIdx.x + t * BS;
-    if (q >= G::NINT) break;
-    int i, j, k;
-    interior_ijk<D, NC>(q, i, j, k);
-    const int x = G::idx(i, j, k);
-    const double lp = apply_cell<D, NC, double>(s, st, cyl != 0, bx, i, j, k);
-    const double rv = rr[t] - lp;
-    r32[base + x] = (float)rv;
-    mx = fmax(mx, fabs(rv));
-  }
-  if (bx.children[0] == 0) {
-    mx = block_max<BS>(mx, sh);
-    if (threadIdx.x == 0) atomic_max_nonneg(run_slot(leafmax, bx), mx);
+    if (bx.children[0] == 0) {
+      mx = group_max<BS, BB>(mx, sh);
+      if (ltid<BS>() == 0) atomic_max_nonneg(run_slot(leafmax, bx), mx);
+    }
+    gsync<BS, BB>();  // (the group's LDS is reused by its next item)
   }
 }
 
 // k_restrict_residual with the child box staged in LDS and its face ghosts
 // formed in-kernel, so no ghost fill has to precede it.
-template <int D, int NC, int BS, typename T>
-__global__ void __launch_bounds__(BS) k_restrict_residual_lds(const int *__restrict__ ids,
+template <int D, int NC, int BS, typename T, int BB = 1>
+__global__ void __launch_bounds__(BS * BB) k_restrict_residual_lds(const int *__restrict__ ids,
                                                               const DBox *__restrict__ boxes, const T *child,
                                                               T *phi, const T *__restrict__ rhs, T *tmp, T lambda2,
-                                                              int cyl, int store_residual, VarBc m) {
+                                                              int cyl, int store_residual, VarBc m, int nw) {
   using G = Geo<D, NC>;
   constexpr int H = NC / 2;
   constexpr int NQ = (D == 3) ? H * H * H : H * H;
-  __shared__ T s[G::NCELL];
-  const int id = ids[blockIdx.x];
-  if (id <= 0) return;  // run masked out
-  const DBox &bc = boxes[id - 1];
-  const DBox &bp = boxes[bc.parent - 1];
-  const Sten<D, T> st = lpl_stencil<D, T>(bc, lambda2);
-  const size_t cb = (size_t)(id - 1) * G::NCELL, pb = (size_t)(bc.parent - 1) * G::NCELL;
-  lds_stage<D, NC, BS, T>(s, boxes, bc, id, child, m, -1, (const T *)phi);
-  int off[3] = {0, 0, 0};
-  for (int d = 0; d < D; d++) off[d] = ((bc.ix[d] - 1) & 1) * H;
-  for (int q = threadIdx.x; q < NQ; q += BS) {
-    const int i = q % H + 1, j = (q / H) % H + 1, k = (D == 3) ? q / (H * H) + 1 : 0;
-    const int i_f = 2 * i - 1, jf = 2 * j - 1, kf = (D == 3) ? 2 * k - 1 : 0;
-    T rs[8];
-    const int nch = (D == 3) ? 8 : 4;
-    for (int c = 0; c < nch; c++) {
-      const int ii = i_f + (c & 1), jj = jf + ((c >> 1) & 1), kk = kf + ((c >> 2) & 1);
-      T lp = apply_cell<D, NC, T>(s, st, cyl != 0, bc, ii, jj, kk);
-      rs[c] = rhs[cb + G::idx(ii, jj, kk)] - lp;
-      if (store_residual) tmp[cb + G::idx(ii, jj, kk)] = rs[c];
-    }
-    const int ic = off[0] + i, jc = off[1] + j, kc = (D == 3) ? off[2] + k : 0;
-    T rv;
-    if (D == 2) {
-      if (cyl) {
-        T r = (T)bp.r_min[0] + (T(ic) - T(0.5)) * (T)bp.dr[0];
-        T t = T(0.25) * (T)bp.dr[0] / r;
-        T w1 = T(1) - t, w2 = T(1) + t;
-        rv = T(0.25) * (w1 * (rs[0] + rs[2]) + w2 * (rs[1] + rs[3]));
+  __shared__ T s_all[BB][G::NCELL];
+  T *s = s_all[lgrp<BS, BB>()];
+  for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
+    const int id = ids[w];
+    if (id <= 0) continue;  // run masked out
+    const DBox &bc = boxes[id - 1];
+    const DBox &bp = boxes[bc.parent - 1];
+    const Sten<D, T> st = lpl_stencil<D, T>(bc, lambda2);
+    const size_t cb = (size_t)(id - 1) * G::NCELL, pb = (size_t)(bc.parent - 1) * G::NCELL;
+    lds_stage<D, NC, BS, T, BB>(s, boxes, bc, id, child, m, -1, (const T *)phi);
+    int off[3] = {0, 0, 0};
+    for (int d = 0; d < D; d++) off[d] = ((bc.ix[d] - 1) & 1) * H;
+    for (int q = ltid<BS>(); q < NQ; q += BS) {
+      const int i = q % H + 1, j = (q / H) % H + 1, k = (D == 3) ? q / (H * H) + 1 : 0;
+      const int i_f = 2 * i - 1, jf = 2 * j - 1, kf = (D == 3) ? 2 * k - 1 : 0;
+      T rs[8];
+      const int nch = (D == 3) ? 8 : 4;
+      for (int c = 0; c < nch; c++) {
+        const int ii = i_f + (c & 1), jj = jf + ((c >> 1) & 1), kk = kf + ((c >> 2) & 1);
+        T lp = apply_cell<D, NC, T>(s, st, cyl != 0, bc, ii, jj, kk);
+        rs[c] = rhs[cb + G::idx(ii, jj, kk)] - lp;
+        if (store_residual) tmp[cb + G::idx(ii, jj, kk)] = rs[c];
+      }
+      const int ic = off[0] + i, jc = off[1] + j, kc = (D == 3) ? off[2] + k : 0;
+      T rv;
+      if (D == 2) {
+        if (cyl) {
+          T r = (T)bp.r_min[0] + (T(ic) - T(0.5)) * (T)bp.dr[0];
+          T t = T(0.25) * (T)bp.dr[0] / r;
+          T w1 = T(1) - t, w2 = T(1) + t;
+          rv = T(0.25) * (w1 * (rs[0] + rs[2]) + w2 * (rs[1] + rs[3]));
+        } else {
+          T sm = rs[0];
+          sm = sm + rs[1];
+          sm = sm + rs[2];
+          sm = sm + rs[3];
+          rv = T(0.25) * sm;
+        }
       } else {
         T sm = rs[0];
-        sm = sm + rs[1];
-        sm = sm + rs[2];
-        sm = sm + rs[3];
-        rv = T(0.25) * sm;
+        for (int c = 1; c < 8; c++) sm = sm + rs[c];
+        rv = T(0.125) * sm;
       }
-    } else {
-      T sm = rs[0];
-      for (int c = 1; c < 8; c++) sm = sm + rs[c];
-      rv = T(0.125) * sm;
+      tmp[pb + G::idx(ic, jc, kc)] = rv;
+      phi[pb + G::idx(ic, jc, kc)] = restrict_cell<D, NC, T>(s, i_f, jf, kf, false, bp, ic);
     }
-    tmp[pb + G::idx(ic, jc, kc)] = rv;
-    phi[pb + G::idx(ic, jc, kc)] = restrict_cell<D, NC, T>(s, i_f, jf, kf, false, bp, ic);
+    gsync<BS, BB>();  // (the group's LDS is reused by its next item)
   }
 }
 
@@ -1168,17 +1203,19 @@ __global__ void __launch_bounds__(BS) k_smooth_block(const int *__restrict__ ids
 // a coarse leaf's ghost cells during the next cycle's first sweeps.  (Refined
 // cells get the restriction of the finer phi instead, and e there is
 // overwritten before the next cycle reads it.)
-template <int D, int NC, int BS>
-__global__ void __launch_bounds__(BS) k_mp_correct_leaves(const int *__restrict__ ids, double *phi, float *e) {
+template <int D, int NC, int BS, int BB = 1>
+__global__ void __launch_bounds__(BS * BB) k_mp_correct_leaves(const int *__restrict__ ids, double *phi, float *e, int nw) {
   using G = Geo<D, NC>;
-  const int id = ids[blockIdx.x];
-  if (id <= 0) return;  // run masked out
-  const size_t base = (size_t)(id - 1) * G::NCELL;
-  for (int x = threadIdx.x; x < G::NCELL; x += BS) {
-    const int i = x % G::NG, j = (x / G::NG) % G::NG, k = (D == 3) ? x / (G::NG * G::NG) : 0;
-    const bool inner = i >= 1 && i <= NC && j >= 1 && j <= NC && (D == 2 || (k >= 1 && k <= NC));
-    if (inner) phi[base + x] = phi[base + x] + (double)e[base + x];
-    e[base + x] = 0.0f;
+  for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
+    const int id = ids[w];
+    if (id <= 0) continue;  // run masked out
+    const size_t base = (size_t)(id - 1) * G::NCELL;
+    for (int x = ltid<BS>(); x < G::NCELL; x += BS) {
+      const int i = x % G::NG, j = (x / G::NG) % G::NG, k = (D == 3) ? x / (G::NG * G::NG) : 0;
+      const bool inner = i >= 1 && i <= NC && j >= 1 && j <= NC && (D == 2 || (k >= 1 && k <= NC));
+      if (inner) phi[base + x] = phi[base + x] + (double)e[base + x];
+      e[base + x] = 0.0f;
+    }
   }
 }
 
@@ -1186,72 +1223,76 @@ __global__ void __launch_bounds__(BS) k_mp_correct_leaves(const int *__restrict_
 // then (with corners) the edge and corner ghosts of gc_edge_cell /
 // gc_corner_cell from the LDS values, then only the ghost layer is written.
 // One workgroup per box.
-template <int D, int NC, int BS, typename T>
-__global__ void __launch_bounds__(BS) k_gc_lds(const int *__restrict__ ids, const DBox *__restrict__ boxes, T *var,
-                                               VarBc m, int corners) {
+template <int D, int NC, int BS, typename T, int BB = 1>
+__global__ void __launch_bounds__(BS * BB) k_gc_lds(const int *__restrict__ ids, const DBox *__restrict__ boxes, T *var,
+                                               VarBc m, int corners, int nw) {
   using G = Geo<D, NC>;
-  __shared__ T s[G::NCELL];
-  const int id = ids[blockIdx.x];
-  if (id <= 0) return;  // run masked out
-  const DBox &bx = boxes[id - 1];
-  T *u = var + (size_t)(id - 1) * G::NCELL;
-  lds_stage<D, NC, BS, T>(s, boxes, bx, id, var, m, -1);
-  if (corners) {
-    if (D == 3) {
-      for (int q = threadIdx.x; q < 12 * NC; q += BS) {
-        const int n = q / NC, qq = q % NC + 1;
-        const int dim = c_edge_dim3[n];
-        const int *dir = c_edge_dir3[n];
-        int lo[3];
-        for (int dd = 0; dd < 3; dd++) lo[dd] = (dir[dd] > 0) ? NC + 1 : 0;
-        lo[dim] = qq;
-        const int nb_id = bx.nbmat[(dir[0] + 1) + 3 * (dir[1] + 1) + 9 * (dir[2] + 1)];
+  __shared__ T s_all[BB][G::NCELL];
+  T *s = s_all[lgrp<BS, BB>()];
+  for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
+    const int id = ids[w];
+    if (id <= 0) continue;  // run masked out
+    const DBox &bx = boxes[id - 1];
+    T *u = var + (size_t)(id - 1) * G::NCELL;
+    lds_stage<D, NC, BS, T, BB>(s, boxes, bx, id, var, m, -1);
+    if (corners) {
+      if (D == 3) {
+        for (int q = ltid<BS>(); q < 12 * NC; q += BS) {
+          const int n = q / NC, qq = q % NC + 1;
+          const int dim = c_edge_dim3[n];
+          const int *dir = c_edge_dir3[n];
+          int lo[3];
+          for (int dd = 0; dd < 3; dd++) lo[dd] = (dir[dd] > 0) ? NC + 1 : 0;
+          lo[dim] = qq;
+          const int nb_id = bx.nbmat[(dir[0] + 1) + 3 * (dir[1] + 1) + 9 * (dir[2] + 1)];
+          T v;
+          if (nb_id > 0) {
+            const T *w = var + (size_t)(nb_id - 1) * G::NCELL;
+            v = w[G::idx(lo[0] - dir[0] * NC, lo[1] - dir[1] * NC, lo[2] - dir[2] * NC)];
+          } else {
+            const int o0 = (dim + 1) % 3, o1 = (dim + 2) % 3;
+            int di[3];
+            for (int dd = 0; dd < 3; dd++) di[dd] = 1 - 2 * (lo[dd] & 1);
+            di[dim] = 0;
+            int ia[3] = {lo[0], lo[1], lo[2]}, ib[3] = {lo[0], lo[1], lo[2]};
+            ia[o0] += di[o0];
+            ib[o1] += di[o1];
+            v = s[G::idx(ia[0], ia[1], ia[2])] + s[G::idx(ib[0], ib[1], ib[2])] -
+                s[G::idx(lo[0] + di[0], lo[1] + di[1], lo[2] + di[2])];
+          }
+          s[G::idx(lo[0], lo[1], lo[2])] = v;
+        }
+        gsync<BS, BB>();
+      }
+      for (int n = ltid<BS>(); n < (1 << D); n += BS) {
+        const int cx = n & 1, cy = (n >> 1) & 1, cz = (n >> 2) & 1;
+        const int dnb[3] = {2 * cx - 1, 2 * cy - 1, (D == 3) ? 2 * cz - 1 : 0};
+        const int lo[3] = {cx * (NC + 1), cy * (NC + 1), (D == 3) ? cz * (NC + 1) : 0};
+        const int nb_id = bx.nbmat[(dnb[0] + 1) + 3 * (dnb[1] + 1) + 9 * (dnb[2] + 1)];
         T v;
         if (nb_id > 0) {
           const T *w = var + (size_t)(nb_id - 1) * G::NCELL;
-          v = w[G::idx(lo[0] - dir[0] * NC, lo[1] - dir[1] * NC, lo[2] - dir[2] * NC)];
+          v = w[G::idx(lo[0] - dnb[0] * NC, lo[1] - dnb[1] * NC, lo[2] - dnb[2] * NC)];
+        } else if (D == 2) {
+          const int di = 1 - 2 * (lo[0] & 1), dj = 1 - 2 * (lo[1] & 1);
+          v = s[G::idx(lo[0] + di, lo[1], 0)] + s[G::idx(lo[0], lo[1] + dj, 0)] - s[G::idx(lo[0] + di, lo[1] + dj, 0)];
         } else {
-          const int o0 = (dim + 1) % 3, o1 = (dim + 2) % 3;
-          int di[3];
-          for (int dd = 0; dd < 3; dd++) di[dd] = 1 - 2 * (lo[dd] & 1);
-          di[dim] = 0;
-          int ia[3] = {lo[0], lo[1], lo[2]}, ib[3] = {lo[0], lo[1], lo[2]};
-          ia[o0] += di[o0];
-          ib[o1] += di[o1];
-          v = s[G::idx(ia[0], ia[1], ia[2])] + s[G::idx(ib[0], ib[1], ib[2])] -
-              s[G::idx(lo[0] + di[0], lo[1] + di[1], lo[2] + di[2])];
+          const int di = 1 - 2 * (lo[0] & 1), dj = 1 - 2 * (lo[1] & 1), dk = 1 - 2 * (lo[2] & 1);
+          const int x = lo[0], y = lo[1], z = lo[2];
+          v = s[G::idx(x, y + dj, z + dk)] + s[G::idx(x + di, y, z + dk)] + s[G::idx(x + di, y + dj, z)] -
+              2 * s[G::idx(x + di, y + dj, z + dk)];
         }
         s[G::idx(lo[0], lo[1], lo[2])] = v;
       }
-      __syncthreads();
+      gsync<BS, BB>();
     }
-    for (int n = threadIdx.x; n < (1 << D); n += BS) {
-      const int cx = n & 1, cy = (n >> 1) & 1, cz = (n >> 2) & 1;
-      const int dnb[3] = {2 * cx - 1, 2 * cy - 1, (D == 3) ? 2 * cz - 1 : 0};
-      const int lo[3] = {cx * (NC + 1), cy * (NC + 1), (D == 3) ? cz * (NC + 1) : 0};
-      const int nb_id = bx.nbmat[(dnb[0] + 1) + 3 * (dnb[1] + 1) + 9 * (dnb[2] + 1)];
-      T v;
-      if (nb_id > 0) {
-        const T *w = var + (size_t)(nb_id - 1) * G::NCELL;
-        v = w[G::idx(lo[0] - dnb[0] * NC, lo[1] - dnb[1] * NC, lo[2] - dnb[2] * NC)];
-      } else if (D == 2) {
-        const int di = 1 - 2 * (lo[0] & 1), dj = 1 - 2 * (lo[1] & 1);
-        v = s[G::idx(lo[0] + di, lo[1], 0)] + s[G::idx(lo[0], lo[1] + dj, 0)] - s[G::idx(lo[0] + di, lo[1] + dj, 0)];
-      } else {
-        const int di = 1 - 2 * (lo[0] & 1), dj = 1 - 2 * (lo[1] & 1), dk = 1 - 2 * (lo[2] & 1);
-        const int x = lo[0], y = lo[1], z = lo[2];
-        v = s[G::idx(x, y + dj, z + dk)] + s[G::idx(x + di, y, z + dk)] + s[G::idx(x + di, y + dj, z)] -
-            2 * s[G::idx(x + di, y + dj, z + dk)];
-      }
-      s[G::idx(lo[0], lo[1], lo[2])] = v;
+    // the ghost layer back: faces, and with corners the edges and corners
+    for (int x = ltid<BS>(); x < G::NCELL; x += BS) {
+      const int i = x % G::NG, j = (x / G::NG) % G::NG, k = (D == 3) ? x / (G::NG * G::NG) : 0;
+      const int out = (i == 0 || i == NC + 1) + (j == 0 || j == NC + 1) + (D == 3 && (k == 0 || k == NC + 1));
+      if (out == 1 || (corners && out > 1)) u[x] = s[x];
     }
-    __syncthreads();
-  }
-  // the ghost layer back: faces, and with corners the edges and corners
-  for (int x = threadIdx.x; x < G::NCELL; x += BS) {
-    const int i = x % G::NG, j = (x / G::NG) % G::NG, k = (D == 3) ? x / (G::NG * G::NG) : 0;
-    const int out = (i == 0 || i == NC + 1) + (j == 0 || j == NC + 1) + (D == 3 && (k == 0 || k == NC + 1));
-    if (out == 1 || (corners && out > 1)) u[x] = s[x];
+    gsync<BS, BB>();  // (the group's LDS is reused by its next item)
   }
 }
 

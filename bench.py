@@ -51,8 +51,25 @@ def make_case(args):
     return case
 
 
+FORCED_SHAPE = None
+
+
+def tune(p, enabled):
+    """Work-decomposition shapes for this workload (measured by the
+    library's autotuner from the current state, which it leaves unchanged)."""
+    if FORCED_SHAPE is not None:
+        p.set_shapes([FORCED_SHAPE] * A.N_KERNEL_CLASSES)
+        return {"all": list(FORCED_SHAPE)}
+    if not enabled:
+        return None
+    t0 = time.time()
+    shapes = p.autotune()
+    log("  autotune %.2f s: %s" % (time.time() - t0, {k: v[:2] for k, v in shapes.items()}))
+    return {k: list(v[:2]) for k, v in shapes.items()}
+
+
 def run_gpu(case, precision, warmup, steps, timing_steps, direct_n=None, smooth=None, block=None, batch=None,
-            photoi_vcycle=None):
+            photoi_vcycle=None, autotune=True):
     p = A.Path("gpu", case.D, case.nc, case.coord, precision)
     S.apply(p, case)
     if direct_n is not None:
@@ -71,6 +88,7 @@ def run_gpu(case, precision, warmup, steps, timing_steps, direct_n=None, smooth=
     for i in range(warmup):
         info = p.step()
         log("  warmup", i, info["n_vcycles"], "%.3e" % info["rel_residual"])
+    p.shapes = tune(p, autotune)
     # timed region: the production path (captured HIP graphs)
     infos = []
     t0 = time.perf_counter()
@@ -140,7 +158,7 @@ def ensemble_leaf_interior(ens, r, p, iv):
     return a[:, 1:c.nc + 1, 1:c.nc + 1]
 
 
-def run_ensemble(ens, precision, warmup, steps):
+def run_ensemble(ens, precision, warmup, steps, autotune=True):
     p = S.apply_ensemble(A.Path("gpu", 2, 8, A.CYL, precision), ens)
     t0 = time.time()
     p.field_compute_runs(0, have_guess=False)
@@ -148,6 +166,7 @@ def run_ensemble(ens, precision, warmup, steps):
     for i in range(warmup):
         infos = p.step_ensemble()
         log("  warmup", i, "vcycles", [f["n_vcycles"] for f in infos][:16], "...")
+    p.shapes = tune(p, autotune)
     all_infos = []
     t0 = time.perf_counter()
     for i in range(steps):
@@ -166,7 +185,7 @@ def bench_ensemble(args):
     log("workload", ens.name, "built in %.1f s; sizes" % (time.time() - t0),
         {int(np.sqrt(k)): sizes.count(k) for k in sorted(set(sizes))})
     prec = A.PREC_MIXED if args.precision == "mixed" else A.PREC_FP64
-    p, infos, secs = run_ensemble(ens, prec, args.warmup, args.steps)
+    p, infos, secs = run_ensemble(ens, prec, args.warmup, args.steps, bool(args.autotune))
     cell_steps = sum(f["leaf_cells"] for step in infos for f in step)
     value = cell_steps / secs
     nv = np.array([[f["n_vcycles"] for f in step] for step in infos])
@@ -175,7 +194,7 @@ def bench_ensemble(args):
     extra = {}
     if prec == A.PREC_MIXED and not args.no_fp64_ref:
         log("GPU fp64 ensemble")
-        ref, rinfos, rsecs = run_ensemble(ens, A.PREC_FP64, args.warmup, args.steps)
+        ref, rinfos, rsecs = run_ensemble(ens, A.PREC_FP64, args.warmup, args.steps, bool(args.autotune))
         fp64_value = sum(f["leaf_cells"] for step in rinfos for f in step) / rsecs
         extra["fp64_gpu_value"] = fp64_value
         extra["mixed_over_fp64"] = value / fp64_value
@@ -228,7 +247,7 @@ def bench_ensemble(args):
         "vs_baseline": None, "dtype": "f32+f64" if prec == A.PREC_MIXED else "f64", "data": "synthetic",
         "config": {"workload": ens.name, "runs": len(ens.cases), "cells": int(sum(sizes)),
                    "box_cells": "8^2", "precision": args.precision, "rel_residual": args.rel_residual,
-                   "mean_vcycles": float(nv.mean())},
+                   "mean_vcycles": float(nv.mean()), "shapes": p.shapes},
         "roofline": None, "cpu_baseline": cpu,
     }
     out.update(extra)
@@ -267,9 +286,16 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB a few times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes) and exit")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="pick the kernels' work-decomposition shapes by measurement before the timed region")
+    ap.add_argument("--shapes", default="", metavar="BB,PER_CU",
+                    help="one work-decomposition shape for every kernel class instead of the autotuner's")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64-ref", action="store_true")
     args = ap.parse_args()
+    if args.shapes:
+        global FORCED_SHAPE
+        FORCED_SHAPE = tuple(int(x) for x in args.shapes.split(","))
     if args.copy_probe > 0:
         return copy_probe(args.copy_probe)
     if args.gpus != 1:
@@ -288,7 +314,7 @@ def main():
     p, infos, secs, stats = run_gpu(case, prec, args.warmup, args.steps, args.timing_steps,
                                     args.direct_n if mixed else None, args.mixed_smooth if mixed else None,
                                     args.block_smooth if mixed else None, args.batch_modes if mixed else None,
-                                    args.photoi_vcycle if mixed else None)
+                                    args.photoi_vcycle if mixed else None, bool(args.autotune))
     cell_steps = sum(i["leaf_cells"] for i in infos)
     value = cell_steps / secs
     nv = [i["n_vcycles"] for i in infos]
@@ -317,7 +343,8 @@ def main():
     ref = None
     if prec == A.PREC_MIXED and not args.no_fp64_ref:
         log("GPU fp64 reference run")
-        ref, rinfos, rsecs, _ = run_gpu(case, A.PREC_FP64, args.warmup, args.steps, args.timing_steps)
+        ref, rinfos, rsecs, _ = run_gpu(case, A.PREC_FP64, args.warmup, args.steps, args.timing_steps,
+                                        autotune=bool(args.autotune))
         fp64_value = sum(i["leaf_cells"] for i in rinfos) / rsecs
         extra["fp64_gpu_value"] = fp64_value
         extra["mixed_over_fp64"] = value / fp64_value
@@ -379,7 +406,8 @@ def main():
                    "mixed_smooth": args.mixed_smooth if prec == A.PREC_MIXED else None,
                    "block_smooth": args.block_smooth if prec == A.PREC_MIXED else None,
                    "batch_modes": args.batch_modes if prec == A.PREC_MIXED else None,
-                   "photoi_vcycle": args.photoi_vcycle if prec == A.PREC_MIXED else None},
+                   "photoi_vcycle": args.photoi_vcycle if prec == A.PREC_MIXED else None,
+                   "shapes": p.shapes},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -307,6 +307,28 @@ int afgpu_step_ensemble(afgpu_t *h, int n_runs, afgpu_step_info_t *info);
 int afgpu_get_run_time_state(afgpu_t *h, int run, int *it, double *time, double *global_dt);
 int afgpu_set_run_time_state(afgpu_t *h, int run, int it, double time, double global_dt);
 
+/* ---- work decomposition and its autotuner --------------------------------
+ * The list kernels fall into AFGPU_N_KERNEL_CLASSES classes (0 half-sweeps,
+ * 1 residuals, 2 restriction, 3 prolongation and coarse rhs, 4 ghost layers,
+ * 5 box operations and reductions, 6 drift-diffusion flux, 7 update, field
+ * and sources).  Each class has a shape: how many boxes a workgroup takes at
+ * a time (1, 2, 4 or 8; above 1 only for boxes of 8 x 8 cells, one
+ * wavefront each) and how many workgroups per CU the persistent grid has
+ * (0: one workgroup per batch of boxes).  Shapes change no result. */
+#define AFGPU_N_KERNEL_CLASSES 8
+typedef struct {
+  int    boxes_per_workgroup;
+  int    workgroups_per_cu;
+  double ms;  /* autotune: the class's kernel time in one step with this shape */
+} afgpu_shape_t;
+/* Measure every candidate shape on the handle's own workload (one eager
+ * time step per candidate from a snapshot of the current state, restored
+ * after each, so the simulation is unchanged), keep the fastest shape of
+ * every class, and report them (n_classes entries; may be 0). */
+int afgpu_autotune(afgpu_t *h, int n_classes, afgpu_shape_t *shapes);
+int afgpu_get_shapes(afgpu_t *h, int n_classes, afgpu_shape_t *shapes);
+int afgpu_set_shapes(afgpu_t *h, int n_classes, const afgpu_shape_t *shapes);
+
 #ifdef __cplusplus
 }
 #endif
