@@ -44,6 +44,7 @@ OPT_BATCH_MODES = 5
 OPT_DIRECT_MFMA = 6
 OPT_PHOTOI_VCYCLE = 7
 OPT_FLUX32 = 8
+OPT_STEP_GRAPH = 9
 
 
 class ParamsT(C.Structure):

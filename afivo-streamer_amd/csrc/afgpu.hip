@@ -159,6 +159,11 @@ struct afgpu_ctx {
     bool started = false;
   };
   std::vector<RunTime> rt;         // time-loop state per run (ensemble stepping)
+  RunStep *d_step = nullptr, *h_step = nullptr;  // device-driven field solve records
+  int *d_run_ones = nullptr;       // every run active
+  double *d_run_min_dr = nullptr;
+  int step_k = 4;                  // cycles a captured ensemble step holds
+  bool step_graph = false;         // ensemble steps as one captured graph
   std::vector<DBox> hboxes;
   DBox *d_boxes = nullptr;
   std::vector<int> h_lists;
@@ -324,7 +329,8 @@ enum GraphKind {
   G_MP_GC = 6,
   G_MP_FMG = 7,
   G_MP_RES_MODES = 8,
-  G_MP_FMG_MODES = 9
+  G_MP_FMG_MODES = 9,
+  G_STEP_RUNS = 10
 };
 // reduction slots (per run): 0 residual max, 1-3 dt limits (3: a minimum),
 // 8 + n photoionization mode n
@@ -1152,11 +1158,12 @@ struct Impl : ImplBase {
   // unrestricted get the restricted solution too, and the stored ghost
   // layers are refreshed everywhere.
   int mp_finish(int op, int lc) {
+    return run_graph(c, {G_MP_GC, op, lc, 0}, [&] { return mp_finish_body(op, lc); });
+  }
+  int mp_finish_body(int op, int lc) {
     const int iphi = op_phi(op);
-    return run_graph(c, {G_MP_GC, op, lc, 0}, [&] {
-      RCHK(restrict_levels(iphi, mp_restrict_lo(lc) - 1, 2));
-      return gc_tree(iphi);
-    });
+    RCHK(restrict_levels(iphi, mp_restrict_lo(lc) - 1, 2));
+    return gc_tree(iphi);
   }
 
   // read slots 0..n-1 of every run
@@ -1208,14 +1215,15 @@ struct Impl : ImplBase {
 
   // --- field (src/m_field.f90) ----------------------------------------------
   int field_from_potential() override {
-    return run_graph(c, {G_FFP, 0, 0, 0}, [&] {
+    return run_graph(c, {G_FFP, 0, 0, 0}, [&] { return ffp_body(); });
+  }
+  int ffp_body() {
       RCHK(with_bb(KC_UPDATE, [&](auto B) {
       constexpr int bb = decltype(B)::value;
       return items<bb>(KC_UPDATE, k_gradient<D, NC, BS, bb>, c.l_all_ids.n, c.list(c.l_all_ids), (const DBox *)c.d_boxes,
                   (const double *)c.cc[AFGPU_CC_PHI], c.fc[AFGPU_FC_FIELD], c.cc[AFGPU_CC_EFIELD]);
     }));
       return gc_tree(AFGPU_CC_EFIELD);
-    });
   }
 
   // field_compute (src/m_field.f90:448-528).  Each run of a forest has its
@@ -1349,6 +1357,14 @@ struct Impl : ImplBase {
                     int i_step, int n_steps, double *dt_lim) override {
     const bool last_step = (i_step == n_steps);
     if (i_step > 1) RCHK(field_compute(s_deriv, true, nullptr, nullptr));
+    RCHK(fe_launch(dt, s_deriv, n_prev, s_prev, w_prev, s_out, last_step));
+    RCHK(read_red(4));
+    return fe_limits(last_step, dt_lim);
+  }
+  // The kernels of one forward-Euler stage (no host synchronisation: can
+  // be captured); the dt limits are left in reduction slots 1-3.
+  int fe_launch(double dt, int s_deriv, int n_prev, const int *s_prev, const double *w_prev, int s_out,
+                bool last_step) {
     const int cyl = c.coord == AFGPU_CYL;
     const int iv_e = AFGPU_CC_ELECTRON + s_deriv;
     // af_restrict_ref_boundary (afivo/src/m_af_restrict.f90:140-161)
@@ -1421,7 +1437,10 @@ struct Impl : ImplBase {
       return items_timed<bb>(2, (7.0 + D + (c.p.photoi_enabled ? 1 : 0)) * 8.0 * c.l_all_leaves.n * G::NINT, KC_UPDATE, k_update<D, NC, BS, bb>, c.l_all_leaves.n, c.list(c.l_all_leaves), (const DBox *)c.d_boxes, A,
                       c.d_red + 3);
     }));
-    RCHK(read_red(4));
+    return AFGPU_OK;
+  }
+  // dt limits of the stage from slots 1-3 as read back (src/m_fluid.f90:96-98)
+  int fe_limits(bool last_step, double *dt_lim) {
     // per run (only the runs not masked out are meaningful)
     c.run_dt_lim.assign(c.n_runs, 1e100);
     for (int r = 0; r < c.n_runs; r++) {
@@ -1579,6 +1598,118 @@ struct Impl : ImplBase {
     return AFGPU_OK;
   }
 
+  // --- ensembles: the field solve driven by the device ----------------------
+  // Host-side set-up of the Poisson solve (factors, the uniform level):
+  // allocations, so it runs before any capture.
+  int field_setup() {
+    RCHK(ensure_factors(AFGPU_OP_POISSON));
+    if (c.precision == AFGPU_PREC_MIXED) {
+      int lc;
+      RCHK(direct_level(&lc));
+      RCHK(ensure_direct_factors(AFGPU_OP_POISSON));
+    }
+    return AFGPU_OK;
+  }
+  // fold the stripes of every reduction slot (capturable)
+  int fold_red() {
+    const int nq = c.n_runs * N_RED_SLOTS;
+    hipLaunchKernelGGL(k_fold_stripes, dim3((nq + 255) / 256), dim3(256), 0, c.cur,
+                       (const unsigned long long *)c.d_red, c.d_red_out, c.n_runs, RED_MIN_SLOTS);
+    HIPCHK(hipGetLastError());
+    return AFGPU_OK;
+  }
+  // field_compute with a guess for every run of the forest as one
+  // capturable sequence: the thresholds are formed on the device, and after
+  // each of K cycles a kernel counts the cycle for every active run and
+  // drops the converged ones, whose boxes the list kernels then skip (so
+  // every run does the cycles it would do alone).  The records (RunStep)
+  // tell the host which runs need more than K cycles.  Ends with all runs
+  // active, the solve finished and the field computed.
+  int field_solve_dev(int K) {
+    const int op = AFGPU_OP_POISSON, R = c.n_runs, top = std::numeric_limits<int>::max();
+    const dim3 rg((R + 63) / 64), rb(64);
+    RCHK(with_bb(KC_BOXOP, [&](auto B) {
+      constexpr int bb = decltype(B)::value;
+      return items<bb>(KC_BOXOP, k_set_rhs<G::NCELL, BS, bb>, c.l_all_leaves.n, c.list(c.l_all_leaves),
+                       (const double *)c.cc[AFGPU_CC_ELECTRON], (const double *)c.cc[AFGPU_CC_POS_ION],
+                       (const double *)c.cc[AFGPU_CC_NEG_ION], c.cc[AFGPU_CC_RHS]);
+    }));
+    RCHK(set_red(0, 0ull));
+    const int k = bpb(c.l_all_leaves.n, G::NINT);
+    RCHK(with_bb(KC_BOXOP, [&](auto B) {
+      constexpr int bb = decltype(B)::value;
+      return items<bb>(KC_BOXOP, k_maxabs<D, NC, BS, bb>, c.l_all_leaves.n * k, c.list(c.l_all_leaves),
+                       (const DBox *)c.d_boxes, (const double *)c.cc[AFGPU_CC_RHS], c.d_red, k);
+    }));
+    RCHK(fold_red());
+    hipLaunchKernelGGL(k_run_thresholds, rg, rb, 0, c.cur, (const unsigned long long *)c.d_red_out,
+                       (const double *)c.d_run_volt, (const double *)c.d_run_min_dr, c.p.mg_max_rel_residual,
+                       c.p.domain_len_last, (const int *)c.d_run_ones, c.d_run_host, c.d_step, R);
+    HIPCHK(hipGetLastError());
+    RCHK(mask_levels(0, top));
+    const bool mixed = c.precision == AFGPU_PREC_MIXED;
+    const int lc = mixed ? c.dir.lc : 1;
+    if (mixed) RCHK(mp_close(op, false, lc));
+    for (int i = 0; i < K; i++) {
+      if (mixed) {
+        RCHK(vcycle_body(mg32(op), false, c.highest_lvl, lc));
+        RCHK(mp_close(op, true, lc));
+      } else {
+        RCHK(vcycle_body(mg64(op), true, c.highest_lvl));
+      }
+      RCHK(fold_red());
+      hipLaunchKernelGGL(k_run_converge, rg, rb, 0, c.cur, (const unsigned long long *)c.d_red_out, c.d_run_host,
+                         c.d_step, R);
+      HIPCHK(hipGetLastError());
+      RCHK(mask_levels(0, top));
+    }
+    hipLaunchKernelGGL(k_run_copy, rg, rb, 0, c.cur, (const int *)c.d_run_ones, c.d_run_host, R);
+    HIPCHK(hipGetLastError());
+    RCHK(mask_levels(0, top));
+    if (mixed) RCHK(mp_finish_body(op, lc));
+    return ffp_body();
+  }
+  // The runs in `more` go on with the cycles of field_compute's loop from
+  // where the device-driven solve left them (count and threshold in the
+  // records), each to its own stop, then their solve is finished again.
+  int continue_solve(const std::vector<int> &more) {
+    const int op = AFGPU_OP_POISSON, R = c.n_runs;
+    const bool mixed = c.precision == AFGPU_PREC_MIXED;
+    const int lc = mixed ? c.dir.lc : 1;
+    std::vector<int> active = more;
+    int n_active = 0, done_cycles = 0;
+    for (int r = 0; r < R; r++)
+      if (active[r]) {
+        n_active++;
+        done_cycles = c.h_step[r].ncyc;  // (equal for all of them: active throughout)
+      }
+    RCHK(set_active(&active));
+    for (int i = done_cycles + 1; i <= c.p.mg_num_vcycles && n_active > 0; i++) {
+      if (mixed)
+        RCHK(run_graph(c, {G_MP_VCYCLE, op, lc, c.highest_lvl}, [&] {
+          RCHK(vcycle_body(mg32(op), false, c.highest_lvl, lc));
+          return mp_close(op, true, lc);
+        }));
+      else
+        RCHK(vcycle(op, true, c.highest_lvl));
+      RCHK(read_red(1));
+      for (int r = 0; r < R; r++) {
+        if (!active[r]) continue;
+        c.run_res[r] = run_red(r, 0);
+        c.run_ncyc[r]++;
+        if (c.run_res[r] < c.h_step[r].thr) {
+          active[r] = 0;
+          n_active--;
+        }
+      }
+      RCHK(set_active(&active));
+    }
+    RCHK(set_active(&more));
+    if (mixed) RCHK(mp_finish(op, lc));
+    RCHK(field_from_potential());
+    return set_active(nullptr);
+  }
+
   // --- ensembles: one time step of every run of a forest -------------------
   // The time-loop body of src/streamer.f90:181-432 (as afgpu_step does it
   // for one run) for every run at once: each run has its own dt, its own
@@ -1599,7 +1730,83 @@ struct Impl : ImplBase {
     for (int r = 0; r < R; r++) dt[r] = c.rt[r].dt;
     const int max_attempts = 10;
     int n_todo = R;
-    for (int n = 1; n <= max_attempts && n_todo > 0; n++) {
+    // Fast path: the first attempt of every run and its field solve as one
+    // captured graph (backup, flux, update, K device-driven cycles, field),
+    // then one read-back.  The solve of a run whose step turns out rejected
+    // is undone by the restore; runs that need more than K cycles go on
+    // afterwards.
+    std::vector<int> solved(R, 0);
+    bool slow_field = false;
+    int n_first = 1;
+    if (c.p.time_integrator == AFGPU_FORWARD_EULER && c.step_graph && c.use_graphs && !c.timing && R > 1) {
+      RCHK(field_setup());
+      for (int r = 0; r < R; r++) c.h_dt_run[r] = dt[r];
+      HIPCHK(hipMemcpyAsync(c.d_dt_run, c.h_dt_run, R * sizeof(double), hipMemcpyHostToDevice, c.stream));
+      const int K = std::max(1, std::min(c.step_k, c.p.mg_num_vcycles));
+      c.use_run_dt = true;
+      const int rc = run_graph(c, {G_STEP_RUNS, K, c.precision, 0}, [&] {
+        RCHK(copy_states(true));
+        const int sp[1] = {0};
+        const double w[1] = {1.0};
+        RCHK(fe_launch(1.0, 0, 1, sp, w, 0, true));
+        return field_solve_dev(K);
+      });
+      c.use_run_dt = false;
+      RCHK(rc);
+      HIPCHK(hipMemcpyAsync(c.h_red, c.d_red_out, (size_t)R * N_RED_SLOTS * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, c.stream));
+      HIPCHK(hipMemcpyAsync(c.h_step, c.d_step, (size_t)R * sizeof(RunStep), hipMemcpyDeviceToHost, c.stream));
+      HIPCHK(hipStreamSynchronize(c.stream));
+      double dl;
+      RCHK(fe_limits(true, &dl));
+      c.run_ncyc.assign(R, 0);
+      c.run_res.assign(R, 0.0);
+      std::vector<int> rejected(R, 0), more(R, 0);
+      int n_rejected = 0, n_more = 0, kmax = 1;
+      for (int r = 0; r < R; r++) {
+        afgpu_ctx::RunTime &t = c.rt[r];
+        t.time = t.time + dt[r];
+        const double lim = c.run_dt_lim[r];
+        dt_lim[r] = std::min(dt_lim[r], lim);
+        n_att[r] = 1;
+        if (dt[r] <= lim) {
+          todo[r] = 0;
+          n_todo--;
+          solved[r] = 1;
+          c.run_ncyc[r] = c.h_step[r].ncyc;
+          c.run_res[r] = c.h_step[r].res;
+          kmax = std::max(kmax, c.h_step[r].ncyc);
+          if (!c.h_step[r].done) {
+            more[r] = 1;
+            n_more++;
+          }
+        } else {
+          n_rej[r]++;
+          dt[r] = c.p.dt_safety_factor * lim;
+          t.time = t.global_time;
+          rejected[r] = 1;
+          n_rejected++;
+        }
+      }
+      if (n_rejected > 0) {
+        c.base_active = rejected;
+        RCHK(set_active(nullptr));
+        RCHK(copy_states(false));
+        slow_field = true;
+      }
+      if (n_more > 0) {
+        c.base_active.clear();
+        RCHK(continue_solve(more));
+        for (int r = 0; r < R; r++)
+          if (more[r]) kmax = std::max(kmax, c.run_ncyc[r]);
+        slow_field = true;
+      }
+      // the next steps' graph holds as many cycles as any step has needed
+      // (non-decreasing, so that only a few graphs are ever captured)
+      c.step_k = std::max(c.step_k, kmax);
+      n_first = 2;
+    }
+    for (int n = n_first; n <= max_attempts && n_todo > 0; n++) {
       c.base_active = todo;
       RCHK(set_active(nullptr));
       RCHK(copy_states(true));
@@ -1659,9 +1866,36 @@ struct Impl : ImplBase {
       set_err("All time steps were rejected");
       return AFGPU_ERR_ALL_REJECTED;
     }
-    RCHK(field_compute(0, true, nullptr, nullptr));
-    std::vector<double> max_rhs;
-    if (info) RCHK(maxabs_runs(AFGPU_CC_RHS, max_rhs));
+    // the field solve of every run whose accepted step has not had it
+    std::vector<int> unsolved(R, 0);
+    int n_unsolved = 0;
+    for (int r = 0; r < R; r++)
+      if (!solved[r]) {
+        unsolved[r] = 1;
+        n_unsolved++;
+      }
+    if (n_unsolved == R) {
+      RCHK(field_compute(0, true, nullptr, nullptr));
+    } else if (n_unsolved > 0) {
+      const std::vector<int> ncyc = c.run_ncyc;
+      const std::vector<double> res = c.run_res;
+      c.base_active = unsolved;
+      RCHK(field_compute(0, true, nullptr, nullptr));
+      c.base_active.clear();
+      RCHK(set_active(nullptr));
+      for (int r = 0; r < R; r++)
+        if (solved[r]) {
+          c.run_ncyc[r] = ncyc[r];
+          c.run_res[r] = res[r];
+        }
+    }
+    std::vector<double> max_rhs(R, 0.0);
+    if (info) {
+      if (n_unsolved > 0 || slow_field || n_first == 1)
+        RCHK(maxabs_runs(AFGPU_CC_RHS, max_rhs));
+      else
+        for (int r = 0; r < R; r++) max_rhs[r] = c.h_step[r].max_rhs;
+    }
     bool too_small = false;
     for (int r = 0; r < R; r++) {
       afgpu_ctx::RunTime &t = c.rt[r];
@@ -1831,6 +2065,10 @@ int afgpu_destroy(afgpu_t *h) {
   (void)hipFree(h->d_run_host);
   (void)hipFree(h->d_run_maxlvl);
   (void)hipFree(h->d_dt_run);
+  (void)hipFree(h->d_step);
+  (void)hipFree(h->d_run_ones);
+  (void)hipFree(h->d_run_min_dr);
+  if (h->h_step) (void)hipHostFree(h->h_step);
   if (h->h_run_active) (void)hipHostFree(h->h_run_active);
   if (h->h_dt_run) (void)hipHostFree(h->h_dt_run);
   for (auto &o : h->ops) {
@@ -2058,6 +2296,13 @@ int afgpu_sync_tree(afgpu_t *h, int n_boxes, const int *lvl, const int *ix, cons
     (void)hipFree(h->d_run_host);
     (void)hipFree(h->d_run_maxlvl);
     (void)hipFree(h->d_dt_run);
+    (void)hipFree(h->d_step);
+    (void)hipFree(h->d_run_ones);
+    (void)hipFree(h->d_run_min_dr);
+    if (h->h_step) (void)hipHostFree(h->h_step);
+    h->d_step = h->h_step = nullptr;
+    h->d_run_ones = nullptr;
+    h->d_run_min_dr = nullptr;
     if (h->h_red) (void)hipHostFree(h->h_red);
     if (h->h_run_active) (void)hipHostFree(h->h_run_active);
     if (h->h_dt_run) (void)hipHostFree(h->h_dt_run);
@@ -2079,6 +2324,10 @@ int afgpu_sync_tree(afgpu_t *h, int n_boxes, const int *lvl, const int *ix, cons
     HIPCHK(hipHostMalloc(&h->h_run_active, nr * sizeof(int), hipHostMallocDefault));
     HIPCHK(hipMalloc(&h->d_dt_run, nr * sizeof(double)));
     HIPCHK(hipHostMalloc(&h->h_dt_run, nr * sizeof(double), hipHostMallocDefault));
+    HIPCHK(hipMalloc(&h->d_step, nr * sizeof(RunStep)));
+    HIPCHK(hipHostMalloc(&h->h_step, nr * sizeof(RunStep), hipHostMallocDefault));
+    HIPCHK(hipMalloc(&h->d_run_ones, nr * sizeof(int)));
+    HIPCHK(hipMalloc(&h->d_run_min_dr, nr * sizeof(double)));
     h->red_cap_runs = n_runs;
   }
   if ((int)h->run_volt.size() != n_runs) h->run_volt.assign(n_runs, h->voltage);
@@ -2088,7 +2337,10 @@ int afgpu_sync_tree(afgpu_t *h, int n_boxes, const int *lvl, const int *ix, cons
   {
     std::vector<int> ones(n_runs, 1);
     HIPCHK(hipMemcpy(h->d_run_host, ones.data(), (size_t)n_runs * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_run_ones, ones.data(), (size_t)n_runs * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_run_min_dr, min_dr.data(), (size_t)n_runs * sizeof(double), hipMemcpyHostToDevice));
   }
+  h->step_k = 4;
   h->uneven = *std::min_element(maxlvl.begin(), maxlvl.end()) != *std::max_element(maxlvl.begin(), maxlvl.end());
   h->n_runs = n_runs;
   h->lvl_rb = lvl_rb;
@@ -2667,6 +2919,11 @@ int afgpu_set_option(afgpu_t *h, int option, int value) {
       h->direct_max_n = value;
       h->dir.set = false;
       for (auto &f : h->dir.f) f.ok = false;
+      return AFGPU_OK;
+    case AFGPU_OPT_STEP_GRAPH:
+      RCHK(sync_stream(h));
+      drop_graphs(*h);
+      h->step_graph = value != 0;
       return AFGPU_OK;
     case AFGPU_OPT_FLUX32:
       RCHK(sync_stream(h));

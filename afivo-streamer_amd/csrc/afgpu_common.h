@@ -197,6 +197,52 @@ __global__ void k_fold_stripes(const unsigned long long *__restrict__ red, unsig
   }
   out[q] = v;
 }
+// ---- device-driven field solve of a forest ---------------------------------
+// Per-run record of one solve: max |rhs| over leaves, the stopping
+// threshold of field_compute (src/m_field.f90:478-480), the last max
+// |residual|, the cycles done, and whether the run has converged.
+struct RunStep {
+  double max_rhs, thr, res;
+  int ncyc, done;
+};
+__device__ inline double u64_as_double(unsigned long long u) { return __longlong_as_double((long long)u); }
+// From slot 0 (max |rhs|, folded): thresholds; every run of `base` active.
+__global__ void k_run_thresholds(const unsigned long long *__restrict__ red_out, const double *__restrict__ volt,
+                                 const double *__restrict__ min_dr, double rel, double len,
+                                 const int *__restrict__ base, int *run_host, RunStep *st, int n_runs) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_runs) return;
+  const double conv_fac = 1e-10, min_residual = 1e-6;
+  const double mr = u64_as_double(red_out[(size_t)r * RED_STRIDE]);
+  const double a = mr * rel;
+  const double t1 = (min_residual < a) ? a : min_residual;
+  const double t2 = conv_fac * fabs(volt[r]) / (len * min_dr[r]);
+  st[r].max_rhs = mr;
+  st[r].thr = (t1 < t2) ? t2 : t1;
+  st[r].res = 0;
+  st[r].ncyc = 0;
+  st[r].done = 0;
+  run_host[r] = base[r];
+}
+// After a cycle (slot 0: max |residual|, folded): every active run counts
+// the cycle and stops once its residual is below its threshold.
+__global__ void k_run_converge(const unsigned long long *__restrict__ red_out, int *run_host, RunStep *st,
+                               int n_runs) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_runs || !run_host[r]) return;
+  const double res = u64_as_double(red_out[(size_t)r * RED_STRIDE]);
+  st[r].ncyc = st[r].ncyc + 1;
+  st[r].res = res;
+  if (res < st[r].thr) {
+    st[r].done = 1;
+    run_host[r] = 0;
+  }
+}
+__global__ void k_run_copy(const int *__restrict__ from, int *to, int n_runs) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n_runs) to[r] = from[r];
+}
+
 // active[r] := host_active[r] and lo <= (highest level of run r) <= hi.
 __global__ void k_runs_in_levels(const int *__restrict__ host_active, const int *__restrict__ maxlvl, int *active,
                                  int lo, int hi, int n_runs) {

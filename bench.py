@@ -158,8 +158,9 @@ def ensemble_leaf_interior(ens, r, p, iv):
     return a[:, 1:c.nc + 1, 1:c.nc + 1]
 
 
-def run_ensemble(ens, precision, warmup, steps, autotune=True):
+def run_ensemble(ens, precision, warmup, steps, autotune=True, step_graph=1):
     p = S.apply_ensemble(A.Path("gpu", 2, 8, A.CYL, precision), ens)
+    p.set_option(A.OPT_STEP_GRAPH, step_graph)
     t0 = time.time()
     p.field_compute_runs(0, have_guess=False)
     log("  initial field solves %.2f s" % (time.time() - t0))
@@ -185,7 +186,7 @@ def bench_ensemble(args):
     log("workload", ens.name, "built in %.1f s; sizes" % (time.time() - t0),
         {int(np.sqrt(k)): sizes.count(k) for k in sorted(set(sizes))})
     prec = A.PREC_MIXED if args.precision == "mixed" else A.PREC_FP64
-    p, infos, secs = run_ensemble(ens, prec, args.warmup, args.steps, bool(args.autotune))
+    p, infos, secs = run_ensemble(ens, prec, args.warmup, args.steps, bool(args.autotune), args.step_graph)
     cell_steps = sum(f["leaf_cells"] for step in infos for f in step)
     value = cell_steps / secs
     nv = np.array([[f["n_vcycles"] for f in step] for step in infos])
@@ -194,7 +195,8 @@ def bench_ensemble(args):
     extra = {}
     if prec == A.PREC_MIXED and not args.no_fp64_ref:
         log("GPU fp64 ensemble")
-        ref, rinfos, rsecs = run_ensemble(ens, A.PREC_FP64, args.warmup, args.steps, bool(args.autotune))
+        ref, rinfos, rsecs = run_ensemble(ens, A.PREC_FP64, args.warmup, args.steps, bool(args.autotune),
+                                          args.step_graph)
         fp64_value = sum(f["leaf_cells"] for step in rinfos for f in step) / rsecs
         extra["fp64_gpu_value"] = fp64_value
         extra["mixed_over_fp64"] = value / fp64_value
@@ -247,7 +249,7 @@ def bench_ensemble(args):
         "vs_baseline": None, "dtype": "f32+f64" if prec == A.PREC_MIXED else "f64", "data": "synthetic",
         "config": {"workload": ens.name, "runs": len(ens.cases), "cells": int(sum(sizes)),
                    "box_cells": "8^2", "precision": args.precision, "rel_residual": args.rel_residual,
-                   "mean_vcycles": float(nv.mean()), "shapes": p.shapes},
+                   "mean_vcycles": float(nv.mean()), "step_graph": args.step_graph, "shapes": p.shapes},
         "roofline": None, "cpu_baseline": cpu,
     }
     out.update(extra)
@@ -259,6 +261,8 @@ def main():
     ap.add_argument("--workload", choices=["amr3d", "ensemble"], default="amr3d",
                     help="amr3d: the 3D AMR step (the headline metric); ensemble: independent 2D runs")
     ap.add_argument("--runs", type=int, default=64, help="ensemble: number of runs")
+    ap.add_argument("--step-graph", type=int, default=0,
+                    help="ensemble: each step's first attempt and field solve as one captured graph")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)

@@ -278,6 +278,10 @@ int afgpu_kernel_stats(afgpu_t *h, int kind, int reset, double *total_ms,
                                      (default 0) */
 #define AFGPU_OPT_FLUX32 8 /* mixed precision: the drift-diffusion flux stencil in FP32, the
                               density update accumulating in FP64 (default 1) */
+#define AFGPU_OPT_STEP_GRAPH 9 /* ensembles: each step's first attempt and its field solve as one
+                                  captured graph with the stopping tests on the device (1), or
+                                  one captured graph per cycle with the tests on the host (0);
+                                  the results are identical (default 0, measured faster) */
 int afgpu_set_option(afgpu_t *h, int option, int value);
 
 /* ---- ensembles of independent runs --------------------------------------

@@ -123,6 +123,24 @@ def test_forest_mixed_equals_single_runs(A, S):
     run_and_compare(A, ens, g, singles, 4, "mixed")
 
 
+@pytest.mark.parametrize("prec", ["PREC_FP64", "PREC_MIXED"])
+def test_forest_step_graph_equals_single_runs(A, S, prec):
+    """Each step's first attempt and field solve as one captured graph with
+    the stopping tests on the device: the first steps hold fewer cycles than
+    some runs need (they go on from the device's records), a run restarted
+    with a too large dt is rejected and redone, and every run still equals
+    its single-run form bit for bit.  (In mixed precision every run is as
+    deep as the forest's exact-solve level, 128^2, as alone; a shallower run
+    would give the whole forest a coarser exact-solve level, and so other
+    correction iterates than that run alone: a valid but different solve.)"""
+    levels = [5, 4, 6, 3] if prec == "PREC_FP64" else [5, 6, 5, 6]
+    ens = S.ensemble(level_list=levels, num_vcycles=20, rel_residual=1e-10, seed_density=1e21)
+    g, singles = forest_and_singles(A, S, ens, getattr(A, prec))
+    g.set_option(A.OPT_STEP_GRAPH, 1)
+    rejected = run_and_compare(A, ens, g, singles, 5, "step graph " + prec, dt_start={2: 5e-12})
+    assert rejected > 0
+
+
 def test_forest_mixed_vs_fp64_per_run_error(A, S):
     """Per-run error of the mixed form against the FP64 form (the oracle's
     arithmetic) after a few steps of an uneven ensemble."""
