@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # the same counter's reading for a plain copy of known size
 # (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000).
 MEASURED_TRAFFIC = {
-    ("xyz3d_amr_l5_r5.2mm", "mixed", "gsrb"): (37775.5 * 2.000 + 28416.0 * 1.000) * 1024,
+    ("xyz3d_amr_l5_r5.2mm", "mixed", "gsrb"): (36275.8 * 2.000 + 28420.8 * 1.000) * 1024,
 }
 
 
@@ -51,21 +51,29 @@ def make_case(args):
     return case
 
 
-FORCED_SHAPE = None
+FORCED_SHAPES = None   # class -> (boxes per workgroup, workgroups per CU)
+SAVE_SHAPES = None
 
 
 def tune(p, enabled):
     """Work-decomposition shapes for this workload (measured by the
-    library's autotuner from the current state, which it leaves unchanged)."""
-    if FORCED_SHAPE is not None:
-        p.set_shapes([FORCED_SHAPE] * A.N_KERNEL_CLASSES)
-        return {"all": list(FORCED_SHAPE)}
+    library's autotuner from the current state, which it leaves unchanged),
+    or the shapes given on the command line."""
+    if FORCED_SHAPES is not None:
+        p.set_shapes([tuple(FORCED_SHAPES[k]) for k in A.KERNEL_CLASSES])
+        return {k: list(FORCED_SHAPES[k]) for k in A.KERNEL_CLASSES}
     if not enabled:
         return None
     t0 = time.time()
     shapes = p.autotune()
     log("  autotune %.2f s: %s" % (time.time() - t0, {k: v[:2] for k, v in shapes.items()}))
-    return {k: list(v[:2]) for k, v in shapes.items()}
+    out = {k: list(v[:2]) for k, v in shapes.items()}
+    global SAVE_SHAPES
+    if SAVE_SHAPES:
+        with open(SAVE_SHAPES, "w") as f:
+            json.dump(out, f)
+        SAVE_SHAPES = None  # (the measured run's shapes, not the reference run's)
+    return out
 
 
 def run_gpu(case, precision, warmup, steps, timing_steps, direct_n=None, smooth=None, block=None, batch=None,
@@ -216,6 +224,8 @@ def bench_ensemble(args):
         log("  fp64 %.4e cell-steps/s; mixed/fp64 %.3f; worst errors %s" % (fp64_value, value / fp64_value, worst))
     cpu = None
     if not args.no_cpu:
+        if args.cpu_steps <= 0:
+            args.cpu_steps = 8
         # every run of the ensemble, one after the other as the reference's
         # separate processes would, continued by the oracle for cpu_steps
         # steps from the GPU state (the FP64 form's when it ran)
@@ -273,7 +283,9 @@ def main():
     ap.add_argument("--rel-residual", type=float, default=1e-8)
     ap.add_argument("--max-vcycles", type=int, default=100)
     ap.add_argument("--photoi-per-steps", type=int, default=1)
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=0,
+                    help="CPU oracle sample: steps of the workload (0: 25 for the 3D workload, about 10 s "
+                         "on 16 cores; 8 of every run for the ensemble)")
     ap.add_argument("--timing-steps", type=int, default=2,
                     help="eager steps after the timed region with per-kernel HIP event timing")
     ap.add_argument("--direct-n", type=int, default=128,
@@ -292,14 +304,23 @@ def main():
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes) and exit")
     ap.add_argument("--autotune", type=int, default=1,
                     help="pick the kernels' work-decomposition shapes by measurement before the timed region")
-    ap.add_argument("--shapes", default="", metavar="BB,PER_CU",
-                    help="one work-decomposition shape for every kernel class instead of the autotuner's")
+    ap.add_argument("--shapes", default="", metavar="BB,PER_CU|FILE",
+                    help="work-decomposition shapes instead of the autotuner's: one shape for every "
+                         "kernel class, or a JSON file of class -> [bb, per_cu] as --save-shapes writes")
+    ap.add_argument("--save-shapes", default="", metavar="FILE",
+                    help="write the autotuner's shapes (of the run measured first) to FILE")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64-ref", action="store_true")
     args = ap.parse_args()
+    global FORCED_SHAPES, SAVE_SHAPES
     if args.shapes:
-        global FORCED_SHAPE
-        FORCED_SHAPE = tuple(int(x) for x in args.shapes.split(","))
+        if os.path.exists(args.shapes):
+            with open(args.shapes) as f:
+                FORCED_SHAPES = json.load(f)
+        else:
+            one = [int(x) for x in args.shapes.split(",")]
+            FORCED_SHAPES = {k: one for k in A.KERNEL_CLASSES}
+    SAVE_SHAPES = args.save_shapes or None
     if args.copy_probe > 0:
         return copy_probe(args.copy_probe)
     if args.gpus != 1:
@@ -366,6 +387,8 @@ def main():
 
     cpu = None
     if not args.no_cpu:
+        if args.cpu_steps <= 0:
+            args.cpu_steps = 25
         src = ref if ref is not None else p
         threads = min(16, os.cpu_count() or 1)
         os.environ.setdefault("OMP_NUM_THREADS", str(threads))
