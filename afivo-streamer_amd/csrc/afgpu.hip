@@ -541,10 +541,12 @@ struct Impl : ImplBase {
   int gc_var(int lvl, T *var, const VarBc &m, bool corners) {
     const List &l = c.l_ids[lvl];
     const int k = corners ? 1 : bpb(l.n, 2 * D * G::NTAN);
-    if (k == 1) return with_bb(KC_GC, [&](auto B) {
-      constexpr int bb = decltype(B)::value;
-      return items<bb>(KC_GC, k_gc_lds<D, NC, BS, T, bb>, l.n, c.list(l), (const DBox *)c.d_boxes, var, m, (int)corners);
-    });
+    if (k == 1)
+      return with_bb(KC_GC, [&](auto B) {
+        constexpr int bb = decltype(B)::value;
+        return items<bb>(KC_GC, k_gc_lds<D, NC, BS, T, bb>, l.n, c.list(l), (const DBox *)c.d_boxes, var, m,
+                         (int)corners);
+      });
     return items<1>(KC_GC, k_gc<D, NC, BS, T, 1>, l.n * k, c.list(l), (const DBox *)c.d_boxes, var, m, (int)corners, k);
   }
   int gc_lvl(int lvl, int iv, bool corners) override { return gc_var(lvl, c.cc[iv], var_methods(c, iv), corners); }
@@ -615,16 +617,16 @@ struct Impl : ImplBase {
     for (int n = 1; n <= 2 * n_cycle; n++) {
       if (k == 1)
         RCHK(with_bb(KC_GSRB, [&](auto B) {
-      constexpr int bb = decltype(B)::value;
-      return items_timed<bb>(0, bytes, KC_GSRB, k_gsrb_lds<D, NC, BS, T, bb>, l.n, c.list(l), (const DBox *)c.d_boxes, v.phi,
-                          (const T *)v.rhs, n, v.lambda2, cyl, v.m);
-    }));
+          constexpr int bb = decltype(B)::value;
+          return items_timed<bb>(0, bytes, KC_GSRB, k_gsrb_lds<D, NC, BS, T, bb>, l.n, c.list(l),
+                                 (const DBox *)c.d_boxes, v.phi, (const T *)v.rhs, n, v.lambda2, cyl, v.m);
+        }));
       else
         RCHK(with_bb(KC_GSRB, [&](auto B) {
-      constexpr int bb = decltype(B)::value;
-      return items_timed<bb>(0, bytes, KC_GSRB, k_gsrb<D, NC, BS, T, bb>, l.n * k, c.list(l), (const DBox *)c.d_boxes, v.phi,
-                          (const T *)v.rhs, n, v.lambda2, cyl, v.m, k);
-    }));
+          constexpr int bb = decltype(B)::value;
+          return items_timed<bb>(0, bytes, KC_GSRB, k_gsrb<D, NC, BS, T, bb>, l.n * k, c.list(l),
+                                 (const DBox *)c.d_boxes, v.phi, (const T *)v.rhs, n, v.lambda2, cyl, v.m, k);
+        }));
     }
     // ghost layer after the last half-sweep (corners on the final up-sweep)
     return final_gc ? gc_var(lvl, v.phi, v.m, !down) : AFGPU_OK;
@@ -648,29 +650,29 @@ struct Impl : ImplBase {
     const int k = bpb(l.n, NQ);
     if (restrict_in_lds<T>(lvl)) {
       RCHK(with_bb(KC_RESTRICT, [&](auto B) {
-      constexpr int bb = decltype(B)::value;
-      return items<bb>(KC_RESTRICT, k_restrict_residual_lds<D, NC, BS, T, bb>, l.n, c.list(l), (const DBox *)c.d_boxes,
-                  (const T *)level_buf(v, lvl, v_cycle), v.phi, (const T *)v.rhs, v.tmp, v.lambda2, cyl,
-                  (int)!v_cycle, v.m);
-    }));
+        constexpr int bb = decltype(B)::value;
+        return items<bb>(KC_RESTRICT, k_restrict_residual_lds<D, NC, BS, T, bb>, l.n, c.list(l),
+                         (const DBox *)c.d_boxes, (const T *)level_buf(v, lvl, v_cycle), v.phi, (const T *)v.rhs, v.tmp,
+                         v.lambda2, cyl, (int)!v_cycle, v.m);
+      }));
     } else {
       // (a forest whose runs end on different levels fills every level: a
       // run's own finest level needs it, and for the other runs the fill
       // repeats the one the level above has just made, with the same values)
       if (!v_cycle && (lvl == c.highest_lvl || c.uneven)) RCHK(gc_var(lvl, v.phi, v.m, true));
       RCHK(with_bb(KC_RESTRICT, [&](auto B) {
-      constexpr int bb = decltype(B)::value;
-      return items<bb>(KC_RESTRICT, k_restrict_residual<D, NC, BS, T, bb>, l.n * k, c.list(l), (const DBox *)c.d_boxes, v.phi,
-                  (const T *)v.rhs, v.tmp, v.lambda2, cyl, (int)!v_cycle, k);
-    }));
+        constexpr int bb = decltype(B)::value;
+        return items<bb>(KC_RESTRICT, k_restrict_residual<D, NC, BS, T, bb>, l.n * k, c.list(l),
+                         (const DBox *)c.d_boxes, v.phi, (const T *)v.rhs, v.tmp, v.lambda2, cyl, (int)!v_cycle, k);
+      }));
     }
     RCHK(gc_var(lvl - 1, v.phi, v.m, true));
     const List &pl = c.l_parents[lvl - 1];
     const int k2 = bpb(pl.n, G::NCELL);
     return with_bb(KC_PROLONG, [&](auto B) {
       constexpr int bb = decltype(B)::value;
-      return items<bb>(KC_PROLONG, k_coarse_rhs<D, NC, BS, T, bb>, pl.n * k2, c.list(pl), (const DBox *)c.d_boxes, (const T *)v.phi,
-                  v.rhs, v.tmp, v.lambda2, cyl, (int)v_cycle, k2);
+      return items<bb>(KC_PROLONG, k_coarse_rhs<D, NC, BS, T, bb>, pl.n * k2, c.list(pl), (const DBox *)c.d_boxes,
+                       (const T *)v.phi, v.rhs, v.tmp, v.lambda2, cyl, (int)v_cycle, k2);
     });
   }
 
@@ -682,8 +684,8 @@ struct Impl : ImplBase {
     const int k = bpb(l.n, G::NINT);
     return with_bb(KC_PROLONG, [&](auto B) {
       constexpr int bb = decltype(B)::value;
-      return items<bb>(KC_PROLONG, k_prolong_add<D, NC, BS, T, bb>, l.n * k, c.list(l), (const DBox *)c.d_boxes, (const T *)v.tmp,
-                  target ? target : v.phi, k);
+      return items<bb>(KC_PROLONG, k_prolong_add<D, NC, BS, T, bb>, l.n * k, c.list(l), (const DBox *)c.d_boxes,
+                       (const T *)v.tmp, target ? target : v.phi, k);
     });
   }
 
@@ -766,10 +768,10 @@ struct Impl : ImplBase {
       const List &l = c.l_ids[lvl];
       const int k = bpb(l.n, G::NINT);
       RCHK(with_bb(KC_RESID, [&](auto B) {
-      constexpr int bb = decltype(B)::value;
-      return items<bb>(KC_RESID, k_residual<D, NC, BS, T, bb>, l.n * k, c.list(l), (const DBox *)c.d_boxes, (const T *)v.phi,
-                  (const T *)v.rhs, v.tmp, v.lambda2, cyl, c.d_red, k);
-    }));
+        constexpr int bb = decltype(B)::value;
+        return items<bb>(KC_RESID, k_residual<D, NC, BS, T, bb>, l.n * k, c.list(l), (const DBox *)c.d_boxes,
+                         (const T *)v.phi, (const T *)v.rhs, v.tmp, v.lambda2, cyl, c.d_red, k);
+      }));
     }
     return AFGPU_OK;
   }
@@ -845,10 +847,10 @@ struct Impl : ImplBase {
         RCHK(boxop<T>(l, v.phi, v.phi, OP_CLEAR));
         const int k = bpb(l.n, G::NINT / (1 << D));
         RCHK(with_bb(KC_RESTRICT, [&](auto B) {
-      constexpr int bb = decltype(B)::value;
-      return items<bb>(KC_RESTRICT, k_restrict<D, NC, BS, T, bb>, l.n * k, c.list(l), (const DBox *)c.d_boxes, (const T *)v.rhs, v.rhs,
-                    1, (int)(c.coord == AFGPU_CYL), k);
-    }));
+          constexpr int bb = decltype(B)::value;
+          return items<bb>(KC_RESTRICT, k_restrict<D, NC, BS, T, bb>, l.n * k, c.list(l), (const DBox *)c.d_boxes,
+                           (const T *)v.rhs, v.rhs, 1, (int)(c.coord == AFGPU_CYL), k);
+        }));
       }
     }
     // the single-precision correction starts from zero on its coarsest level
@@ -1095,9 +1097,9 @@ struct Impl : ImplBase {
     if (correct) {
       const List &l = c.l_all_leaves;
       RCHK(with_bb(KC_BOXOP, [&](auto B) {
-      constexpr int bb = decltype(B)::value;
-      return items<bb>(KC_BOXOP, k_mp_correct_leaves<D, NC, BS, bb>, l.n, c.list(l), c.cc[iphi], f32(op, 0));
-    }));
+        constexpr int bb = decltype(B)::value;
+        return items<bb>(KC_BOXOP, k_mp_correct_leaves<D, NC, BS, bb>, l.n, c.list(l), c.cc[iphi], f32(op, 0));
+      }));
     }
     // (levels below lc are covered everywhere and not part of the
     // iteration).  The covered values are read by the residual of coarse
@@ -1118,16 +1120,16 @@ struct Impl : ImplBase {
     const int k = bpb(l.n, G::NINT);
     if (k == 1)
       return with_bb(KC_RESID, [&](auto B) {
-      constexpr int bb = decltype(B)::value;
-      return items<bb>(KC_RESID, k_mp_residual_lds<D, NC, BS, bb>, l.n, c.list(l), (const DBox *)c.d_boxes,
-                    (const double *)c.cc[iphi], (const double *)c.cc[AFGPU_CC_RHS], f32(op, 1), c.ops[op].lambda2,
-                    (int)(c.coord == AFGPU_CYL), m, c.d_red + slot);
-    });
+        constexpr int bb = decltype(B)::value;
+        return items<bb>(KC_RESID, k_mp_residual_lds<D, NC, BS, bb>, l.n, c.list(l), (const DBox *)c.d_boxes,
+                         (const double *)c.cc[iphi], (const double *)c.cc[AFGPU_CC_RHS], f32(op, 1), c.ops[op].lambda2,
+                         (int)(c.coord == AFGPU_CYL), m, c.d_red + slot);
+      });
     return with_bb(KC_RESID, [&](auto B) {
       constexpr int bb = decltype(B)::value;
-      return items<bb>(KC_RESID, k_mp_residual<D, NC, BS, bb>, l.n * k, c.list(l), (const DBox *)c.d_boxes, (const double *)c.cc[iphi],
-                  (const double *)c.cc[AFGPU_CC_RHS], f32(op, 1), c.ops[op].lambda2, (int)(c.coord == AFGPU_CYL),
-                  m, c.d_red + slot, k);
+      return items<bb>(KC_RESID, k_mp_residual<D, NC, BS, bb>, l.n * k, c.list(l), (const DBox *)c.d_boxes,
+                       (const double *)c.cc[iphi], (const double *)c.cc[AFGPU_CC_RHS], f32(op, 1), c.ops[op].lambda2,
+                       (int)(c.coord == AFGPU_CYL), m, c.d_red + slot, k);
     });
   }
 
@@ -1146,10 +1148,10 @@ struct Impl : ImplBase {
       const List &l = c.l_ids[lvl];
       const int k = bpb(l.n, G::NINT / (1 << D));
       RCHK(with_bb(KC_RESTRICT, [&](auto B) {
-      constexpr int bb = decltype(B)::value;
-      return items<bb>(KC_RESTRICT, k_restrict<D, NC, BS, double, bb>, l.n * k, c.list(l), (const DBox *)c.d_boxes,
-                  (const double *)c.cc[iphi], c.cc[iphi], 0, (int)(c.coord == AFGPU_CYL), k);
-    }));
+        constexpr int bb = decltype(B)::value;
+        return items<bb>(KC_RESTRICT, k_restrict<D, NC, BS, double, bb>, l.n * k, c.list(l), (const DBox *)c.d_boxes,
+                         (const double *)c.cc[iphi], c.cc[iphi], 0, (int)(c.coord == AFGPU_CYL), k);
+      }));
     }
     return AFGPU_OK;
   }
@@ -1170,8 +1172,8 @@ struct Impl : ImplBase {
   int read_red(int n) {
     const size_t cnt = (size_t)(c.n_runs - 1) * N_RED_SLOTS + n;
     const int nq = c.n_runs * N_RED_SLOTS;
-    hipLaunchKernelGGL(k_fold_stripes, dim3((nq + 255) / 256), dim3(256), 0, c.stream, (const unsigned long long *)c.d_red,
-                       c.d_red_out, c.n_runs, RED_MIN_SLOTS);
+    hipLaunchKernelGGL(k_fold_stripes, dim3((nq + 255) / 256), dim3(256), 0, c.stream,
+                       (const unsigned long long *)c.d_red, c.d_red_out, c.n_runs, RED_MIN_SLOTS);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(c.h_red, c.d_red_out, cnt * sizeof(unsigned long long), hipMemcpyDeviceToHost, c.stream));
     HIPCHK(hipStreamSynchronize(c.stream));
@@ -1197,8 +1199,8 @@ struct Impl : ImplBase {
     const int k = bpb(c.l_all_leaves.n, G::NINT);
     RCHK(with_bb(KC_BOXOP, [&](auto B) {
       constexpr int bb = decltype(B)::value;
-      return items<bb>(KC_BOXOP, k_maxabs<D, NC, BS, bb>, c.l_all_leaves.n * k, c.list(c.l_all_leaves), (const DBox *)c.d_boxes,
-                (const double *)c.cc[iv], c.d_red, k);
+      return items<bb>(KC_BOXOP, k_maxabs<D, NC, BS, bb>, c.l_all_leaves.n * k, c.list(c.l_all_leaves),
+                       (const DBox *)c.d_boxes, (const double *)c.cc[iv], c.d_red, k);
     }));
     RCHK(read_red(1));
     out.assign(c.n_runs, 0.0);
@@ -1218,12 +1220,13 @@ struct Impl : ImplBase {
     return run_graph(c, {G_FFP, 0, 0, 0}, [&] { return ffp_body(); });
   }
   int ffp_body() {
-      RCHK(with_bb(KC_UPDATE, [&](auto B) {
+    RCHK(with_bb(KC_UPDATE, [&](auto B) {
       constexpr int bb = decltype(B)::value;
-      return items<bb>(KC_UPDATE, k_gradient<D, NC, BS, bb>, c.l_all_ids.n, c.list(c.l_all_ids), (const DBox *)c.d_boxes,
-                  (const double *)c.cc[AFGPU_CC_PHI], c.fc[AFGPU_FC_FIELD], c.cc[AFGPU_CC_EFIELD]);
+      return items<bb>(KC_UPDATE, k_gradient<D, NC, BS, bb>, c.l_all_ids.n, c.list(c.l_all_ids),
+                       (const DBox *)c.d_boxes, (const double *)c.cc[AFGPU_CC_PHI], c.fc[AFGPU_FC_FIELD],
+                       c.cc[AFGPU_CC_EFIELD]);
     }));
-      return gc_tree(AFGPU_CC_EFIELD);
+    return gc_tree(AFGPU_CC_EFIELD);
   }
 
   // field_compute (src/m_field.f90:448-528).  Each run of a forest has its
@@ -1236,8 +1239,8 @@ struct Impl : ImplBase {
     RCHK(with_bb(KC_BOXOP, [&](auto B) {
       constexpr int bb = decltype(B)::value;
       return items<bb>(KC_BOXOP, k_set_rhs<G::NCELL, BS, bb>, c.l_all_leaves.n, c.list(c.l_all_leaves),
-                (const double *)c.cc[AFGPU_CC_ELECTRON + s_in], (const double *)c.cc[AFGPU_CC_POS_ION + s_in],
-                (const double *)c.cc[AFGPU_CC_NEG_ION + s_in], c.cc[AFGPU_CC_RHS]);
+                       (const double *)c.cc[AFGPU_CC_ELECTRON + s_in], (const double *)c.cc[AFGPU_CC_POS_ION + s_in],
+                       (const double *)c.cc[AFGPU_CC_NEG_ION + s_in], c.cc[AFGPU_CC_RHS]);
     }));
     std::vector<double> max_rhs;
     RCHK(maxabs_runs(AFGPU_CC_RHS, max_rhs));
@@ -1370,8 +1373,8 @@ struct Impl : ImplBase {
     // af_restrict_ref_boundary (afivo/src/m_af_restrict.f90:140-161)
     RCHK(with_bb(KC_RESTRICT, [&](auto B) {
       constexpr int bb = decltype(B)::value;
-      return items<bb>(KC_RESTRICT, k_restrict<D, NC, BS, double, bb>, c.l_refb.n, c.list(c.l_refb), (const DBox *)c.d_boxes,
-                (const double *)c.cc[iv_e], c.cc[iv_e], 1, cyl, 1);
+      return items<bb>(KC_RESTRICT, k_restrict<D, NC, BS, double, bb>, c.l_refb.n, c.list(c.l_refb),
+                       (const DBox *)c.d_boxes, (const double *)c.cc[iv_e], c.cc[iv_e], 1, cyl, 1);
     }));
     const double big = 1e100;
     unsigned long long bigbits;
@@ -1386,26 +1389,28 @@ struct Impl : ImplBase {
       const double fbytes = (2.0 + 2 * D) * 8.0 * l.n * G::NINT;
       if (c.precision == AFGPU_PREC_MIXED && c.flux32)
         RCHK(with_bb(KC_FLUX, [&](auto B) {
-      constexpr int bb = decltype(B)::value;
-      return items_timed<bb>(1, fbytes, KC_FLUX, k_flux<D, NC, BS, float, bb>, l.n, c.list(l), (const DBox *)c.d_boxes, c.cc[iv_e],
-                          (const double *)c.cc[AFGPU_CC_EFIELD], (const double *)c.fc[AFGPU_FC_FIELD],
-                          c.fc[AFGPU_FC_FLUX], msp, c.td, N_inv, c.d_red + 1, c.d_red + 2);
-    }));
+          constexpr int bb = decltype(B)::value;
+          return items_timed<bb>(1, fbytes, KC_FLUX, k_flux<D, NC, BS, float, bb>, l.n, c.list(l),
+                                 (const DBox *)c.d_boxes, c.cc[iv_e], (const double *)c.cc[AFGPU_CC_EFIELD],
+                                 (const double *)c.fc[AFGPU_FC_FIELD], c.fc[AFGPU_FC_FLUX], msp, c.td, N_inv,
+                                 c.d_red + 1, c.d_red + 2);
+        }));
       else
         RCHK(with_bb(KC_FLUX, [&](auto B) {
-      constexpr int bb = decltype(B)::value;
-      return items_timed<bb>(1, fbytes, KC_FLUX, k_flux<D, NC, BS, double, bb>, l.n, c.list(l), (const DBox *)c.d_boxes, c.cc[iv_e],
-                          (const double *)c.cc[AFGPU_CC_EFIELD], (const double *)c.fc[AFGPU_FC_FIELD],
-                          c.fc[AFGPU_FC_FLUX], msp, c.td, N_inv, c.d_red + 1, c.d_red + 2);
-    }));
+          constexpr int bb = decltype(B)::value;
+          return items_timed<bb>(1, fbytes, KC_FLUX, k_flux<D, NC, BS, double, bb>, l.n, c.list(l),
+                                 (const DBox *)c.d_boxes, c.cc[iv_e], (const double *)c.cc[AFGPU_CC_EFIELD],
+                                 (const double *)c.fc[AFGPU_FC_FIELD], c.fc[AFGPU_FC_FLUX], msp, c.td, N_inv,
+                                 c.d_red + 1, c.d_red + 2);
+        }));
     }
     for (int lvl = 1; lvl <= c.highest_lvl - 1; lvl++) {
       const List &l = c.l_parents[lvl];
       RCHK(with_bb(KC_UPDATE, [&](auto B) {
-      constexpr int bb = decltype(B)::value;
-      return items<bb>(KC_UPDATE, k_consistent_fluxes<D, NC, BS, bb>, l.n, c.list(l), (const DBox *)c.d_boxes, c.fc[AFGPU_FC_FLUX],
-                  cyl);
-    }));
+        constexpr int bb = decltype(B)::value;
+        return items<bb>(KC_UPDATE, k_consistent_fluxes<D, NC, BS, bb>, l.n, c.list(l), (const DBox *)c.d_boxes,
+                         c.fc[AFGPU_FC_FLUX], cyl);
+      }));
     }
     UpdateArgs A{};
     for (int sp = 0; sp < 3; sp++)
@@ -1434,8 +1439,9 @@ struct Impl : ImplBase {
     A.cyl = cyl;
     RCHK(with_bb(KC_UPDATE, [&](auto B) {
       constexpr int bb = decltype(B)::value;
-      return items_timed<bb>(2, (7.0 + D + (c.p.photoi_enabled ? 1 : 0)) * 8.0 * c.l_all_leaves.n * G::NINT, KC_UPDATE, k_update<D, NC, BS, bb>, c.l_all_leaves.n, c.list(c.l_all_leaves), (const DBox *)c.d_boxes, A,
-                      c.d_red + 3);
+      return items_timed<bb>(2, (7.0 + D + (c.p.photoi_enabled ? 1 : 0)) * 8.0 * c.l_all_leaves.n * G::NINT, KC_UPDATE,
+                             k_update<D, NC, BS, bb>, c.l_all_leaves.n, c.list(c.l_all_leaves), (const DBox *)c.d_boxes,
+                             A, c.d_red + 3);
     }));
     return AFGPU_OK;
   }
@@ -1518,8 +1524,8 @@ struct Impl : ImplBase {
     RCHK(with_bb(KC_UPDATE, [&](auto B) {
       constexpr int bb = decltype(B)::value;
       return items<bb>(KC_UPDATE, k_photoi_src<D, NC, BS, bb>, c.l_all_leaves.n, c.list(c.l_all_leaves),
-                (const double *)c.cc[AFGPU_CC_EFIELD], (const double *)c.cc[AFGPU_CC_ELECTRON], c.cc[AFGPU_CC_RHS],
-                c.td, c.p.gas_number_density, c.p.photoi_eta_quench);
+                       (const double *)c.cc[AFGPU_CC_EFIELD], (const double *)c.cc[AFGPU_CC_ELECTRON],
+                       c.cc[AFGPU_CC_RHS], c.td, c.p.gas_number_density, c.p.photoi_eta_quench);
     }));
     RCHK(boxop(c.l_all_ids, c.cc[AFGPU_CC_PHOTO], c.cc[AFGPU_CC_PHOTO], OP_CLEAR));
     double max_rhs;
@@ -1935,10 +1941,10 @@ struct Impl : ImplBase {
     for (int lvl = c.highest_lvl; lvl >= 2; lvl--) {
       const List &l = c.l_ids[lvl];
       RCHK(with_bb(KC_RESTRICT, [&](auto B) {
-      constexpr int bb = decltype(B)::value;
-      return items<bb>(KC_RESTRICT, k_restrict<D, NC, BS, double, bb>, l.n, c.list(l), (const DBox *)c.d_boxes, (const double *)c.cc[iv],
-                  c.cc[iv], 1, (int)(c.coord == AFGPU_CYL), 1);
-    }));
+        constexpr int bb = decltype(B)::value;
+        return items<bb>(KC_RESTRICT, k_restrict<D, NC, BS, double, bb>, l.n, c.list(l), (const DBox *)c.d_boxes,
+                         (const double *)c.cc[iv], c.cc[iv], 1, (int)(c.coord == AFGPU_CYL), 1);
+      }));
     }
     return AFGPU_OK;
   }
@@ -2415,7 +2421,8 @@ int afgpu_set_params(afgpu_t *h, const afgpu_params_t *p) {
   h->p = *p;
   h->have_params = true;
   h->ops[AFGPU_OP_POISSON].lambda2 = 0.0;
-  for (int n = 0; n < p->photoi_n_modes; n++) h->ops[AFGPU_OP_HELMH + n].lambda2 = p->photoi_lambda[n] * p->photoi_lambda[n];
+  for (int n = 0; n < p->photoi_n_modes; n++)
+    h->ops[AFGPU_OP_HELMH + n].lambda2 = p->photoi_lambda[n] * p->photoi_lambda[n];
   return AFGPU_OK;
 }
 

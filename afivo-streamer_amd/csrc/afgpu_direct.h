@@ -79,7 +79,8 @@ __global__ void __launch_bounds__(256) k_lc_lines(const double *__restrict__ in,
   const int xc = blockIdx.x % nxc, o = blockIdx.x / nxc;  // o: the other coordinate
   const int x0 = xc * TC;
   const size_t sa = (axis == 1) ? (size_t)N : (size_t)N * N;            // stride of the axis
-  const size_t so = (D == 2) ? (size_t)N * N : ((axis == 1) ? (size_t)N * N : (size_t)N);  // stride of o (2D: the run of a forest)
+  // stride of o (2D: the run of a forest)
+  const size_t so = (D == 2) ? (size_t)N * N : ((axis == 1) ? (size_t)N * N : (size_t)N);
   const size_t base = (size_t)o * so + x0;
   for (int q = threadIdx.x; q < N * TC; q += 256) {
     const int m = q / TC, c = q % TC;

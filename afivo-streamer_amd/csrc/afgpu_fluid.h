@@ -228,8 +228,9 @@ __global__ void __launch_bounds__(BS * BB)
 // weighted) average of the fine fluxes: af_consistent_fluxes +
 // flux_from_children (afivo/src/m_af_core.f90:1257-1402).
 template <int D, int NC, int BS, int BB = 1>
-__global__ void __launch_bounds__(BS * BB) k_consistent_fluxes(const int *__restrict__ ids, const DBox *__restrict__ boxes,
-                                                          double *flux, int cyl, int nw) {
+__global__ void __launch_bounds__(BS * BB) k_consistent_fluxes(const int *__restrict__ ids,
+                                                               const DBox *__restrict__ boxes, double *flux, int cyl,
+                                                               int nw) {
   using G = Geo<D, NC>;
   constexpr int H = NC / 2;
   for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
@@ -319,7 +320,7 @@ struct UpdateArgs {
 // src/m_chemistry.f90:672-804) over all leaves.
 template <int D, int NC, int BS, int BB = 1>
 __global__ void __launch_bounds__(BS * BB) k_update(const int *__restrict__ ids, const DBox *__restrict__ boxes,
-                                               UpdateArgs A, unsigned long long *chem_min, int nw) {
+                                                    UpdateArgs A, unsigned long long *chem_min, int nw) {
   using G = Geo<D, NC>;
   __shared__ double sh[BS / 64];
   for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
@@ -414,7 +415,7 @@ __global__ void __launch_bounds__(BS * BB) k_update(const int *__restrict__ ids,
 // field_set_rhs (src/m_field.f90:406-444) on leaves, all cells.
 template <int NCELL, int BS, int BB = 1>
 __global__ void __launch_bounds__(BS * BB) k_set_rhs(const int *__restrict__ ids, const double *n0, const double *n1,
-                                                const double *n2, double *rhs, int nw) {
+                                                     const double *n2, double *rhs, int nw) {
   for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
     const int id = ids[w];
     if (id <= 0) continue;  // run masked out
@@ -436,7 +437,8 @@ __global__ void __launch_bounds__(BS * BB) k_set_rhs(const int *__restrict__ ids
 // afivo/src/m_af_multigrid.f90:1857-2045).
 template <int D, int NC, int BS, int BB = 1>
 __global__ void __launch_bounds__(BS * BB) k_gradient(const int *__restrict__ ids, const DBox *__restrict__ boxes,
-                                                 const double *__restrict__ phi, double *efc, double *enorm, int nw) {
+                                                      const double *__restrict__ phi, double *efc, double *enorm,
+                                                      int nw) {
   using G = Geo<D, NC>;
   for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
     const int id = ids[w];
@@ -480,8 +482,8 @@ __global__ void __launch_bounds__(BS * BB) k_gradient(const int *__restrict__ id
 // src/m_photoi.f90:259-291).
 template <int D, int NC, int BS, int BB = 1>
 __global__ void __launch_bounds__(BS * BB) k_photoi_src(const int *__restrict__ ids, const double *__restrict__ efield,
-                                                   const double *__restrict__ ne, double *rhs, DTable td, double N,
-                                                   double coeff, int nw) {
+                                                        const double *__restrict__ ne, double *rhs, DTable td, double N,
+                                                        double coeff, int nw) {
   using G = Geo<D, NC>;
   for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
     const int id = ids[w];

@@ -273,7 +273,7 @@ __device__ inline void gc_corner_cell(const DBox &bx, T *var, int id, int n) {
 // (bpb = 1); face-only fills may be split over several workgroups.
 template <int D, int NC, int BS, typename T, int BB = 1>
 __global__ void __launch_bounds__(BS * BB) k_gc(const int *__restrict__ ids, const DBox *__restrict__ boxes, T *var,
-                                           VarBc m, int corners, int bpb, int nw) {
+                                                VarBc m, int corners, int bpb, int nw) {
   using G = Geo<D, NC>;
   for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
     const Part pt = part_of<BS>(ids, bpb, w);
@@ -314,8 +314,8 @@ __global__ void __launch_bounds__(BS * BB) k_gc(const int *__restrict__ ids, con
 // ghost layer is refreshed once after the last half-sweep.)
 template <int D, int NC, int BS, typename T, int BB = 1>
 __global__ void __launch_bounds__(BS * BB) k_gsrb(const int *__restrict__ ids, const DBox *__restrict__ boxes, T *phi,
-                                             const T *__restrict__ rhs, int redblack, T lambda2, int cyl, VarBc m,
-                                             int bpb, int nw) {
+                                                  const T *__restrict__ rhs, int redblack, T lambda2, int cyl, VarBc m,
+                                                  int bpb, int nw) {
   using G = Geo<D, NC>;
   for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
     const Part pt = part_of<BS>(ids, bpb, w);
@@ -375,8 +375,9 @@ __global__ void __launch_bounds__(BS * BB) k_gsrb(const int *__restrict__ ids, c
 // (af_tree_maxabs_cc, afivo/src/m_af_utils.f90:773).
 template <int D, int NC, int BS, typename T, int BB = 1>
 __global__ void __launch_bounds__(BS * BB) k_residual(const int *__restrict__ ids, const DBox *__restrict__ boxes,
-                                                 const T *__restrict__ phi, const T *__restrict__ rhs, T *tmp,
-                                                 T lambda2, int cyl, unsigned long long *leafmax, int bpb, int nw) {
+                                                      const T *__restrict__ phi, const T *__restrict__ rhs, T *tmp,
+                                                      T lambda2, int cyl, unsigned long long *leafmax, int bpb,
+                                                      int nw) {
   using G = Geo<D, NC>;
   __shared__ double sh[BS / 64];
   for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
@@ -437,7 +438,7 @@ __device__ inline T restrict_cell(const T *f, int i_f, int jf, int kf, bool cyl_
 // Restrict variable `src` of child boxes into `dst` of their parents.
 template <int D, int NC, int BS, typename T, int BB = 1>
 __global__ void __launch_bounds__(BS * BB) k_restrict(const int *__restrict__ ids, const DBox *__restrict__ boxes,
-                                                 const T *src, T *dst, int use_geom, int cyl, int bpb, int nw) {
+                                                      const T *src, T *dst, int use_geom, int cyl, int bpb, int nw) {
   using G = Geo<D, NC>;
   constexpr int H = NC / 2;
   constexpr int NQ = (D == 3) ? H * H * H : H * H;
@@ -467,9 +468,10 @@ __global__ void __launch_bounds__(BS * BB) k_restrict(const int *__restrict__ id
 // reference's save/restore of tmp does.  With store_residual the residual is
 // also kept in the child's tmp (set_coarse_phi_rhs, :742-776).
 template <int D, int NC, int BS, typename T, int BB = 1>
-__global__ void __launch_bounds__(BS * BB) k_restrict_residual(const int *__restrict__ ids, const DBox *__restrict__ boxes,
-                                                          T *phi, const T *__restrict__ rhs, T *tmp, T lambda2,
-                                                          int cyl, int store_residual, int bpb, int nw) {
+__global__ void __launch_bounds__(BS * BB) k_restrict_residual(const int *__restrict__ ids,
+                                                               const DBox *__restrict__ boxes, T *phi,
+                                                               const T *__restrict__ rhs, T *tmp, T lambda2, int cyl,
+                                                               int store_residual, int bpb, int nw) {
   using G = Geo<D, NC>;
   constexpr int H = NC / 2;
   constexpr int NQ = (D == 3) ? H * H * H : H * H;
@@ -526,8 +528,8 @@ __global__ void __launch_bounds__(BS * BB) k_restrict_residual(const int *__rest
 // on parents, rhs = L phi (interior) + tmp (all cells); optionally tmp = phi.
 template <int D, int NC, int BS, typename T, int BB = 1>
 __global__ void __launch_bounds__(BS * BB) k_coarse_rhs(const int *__restrict__ ids, const DBox *__restrict__ boxes,
-                                                   const T *__restrict__ phi, T *rhs, T *tmp, T lambda2, int cyl,
-                                                   int copy_phi, int bpb, int nw) {
+                                                        const T *__restrict__ phi, T *rhs, T *tmp, T lambda2, int cyl,
+                                                        int copy_phi, int bpb, int nw) {
   using G = Geo<D, NC>;
   for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
     const Part pt = part_of<BS>(ids, bpb, w);
@@ -550,7 +552,8 @@ __global__ void __launch_bounds__(BS * BB) k_coarse_rhs(const int *__restrict__ 
 enum BoxOp { OP_COPY = 0, OP_CLEAR = 1, OP_SUB_FROM = 2, OP_AXPY_NEG = 3 };
 // OP_COPY: b = a; OP_CLEAR: b = 0; OP_SUB_FROM: b = a - b; OP_AXPY_NEG: b = b - s*a
 template <int NCELL, int BS, typename T, int BB = 1>
-__global__ void __launch_bounds__(BS * BB) k_boxop(const int *__restrict__ ids, const T *a, T *b, int op, T s, int bpb, int nw) {
+__global__ void __launch_bounds__(BS * BB) k_boxop(const int *__restrict__ ids, const T *a, T *b, int op, T s, int bpb,
+                                                   int nw) {
   for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
     const Part pt = part_of<BS>(ids, bpb, w);
     if (pt.id <= 0) continue;  // run masked out
@@ -571,7 +574,7 @@ __global__ void __launch_bounds__(BS * BB) k_boxop(const int *__restrict__ ids, 
 // mg_box_prolong_linear_stencil (m_af_multigrid.f90:1267-1284).
 template <int D, int NC, int BS, typename T, int BB = 1>
 __global__ void __launch_bounds__(BS * BB) k_prolong_add(const int *__restrict__ ids, const DBox *__restrict__ boxes,
-                                                    const T *__restrict__ src, T *dst, int bpb, int nw) {
+                                                         const T *__restrict__ src, T *dst, int bpb, int nw) {
   using G = Geo<D, NC>;
   constexpr int H = NC / 2;
   for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
@@ -608,7 +611,8 @@ __global__ void __launch_bounds__(BS * BB) k_prolong_add(const int *__restrict__
 // afivo/src/m_af_utils.f90:773).
 template <int D, int NC, int BS, int BB = 1>
 __global__ void __launch_bounds__(BS * BB) k_maxabs(const int *__restrict__ ids, const DBox *__restrict__ boxes,
-                                               const double *__restrict__ u, unsigned long long *out, int bpb, int nw) {
+                                                    const double *__restrict__ u, unsigned long long *out, int bpb,
+                                                    int nw) {
   using G = Geo<D, NC>;
   __shared__ double sh[BS / 64];
   for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
@@ -731,7 +735,8 @@ __global__ void __launch_bounds__(CBS) k_coarse_solve(const int *__restrict__ id
 // phi += e on whole boxes, then e = 0 so the next cycle starts from a zero
 // correction.
 template <int NCELL, int BS, int BB = 1>
-__global__ void __launch_bounds__(BS * BB) k_mp_correct(const int *__restrict__ ids, double *phi, float *e, int bpb, int nw) {
+__global__ void __launch_bounds__(BS * BB) k_mp_correct(const int *__restrict__ ids, double *phi, float *e, int bpb,
+                                                        int nw) {
   for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
     const Part pt = part_of<BS>(ids, bpb, w);
     if (pt.id <= 0) continue;  // run masked out
@@ -750,9 +755,9 @@ __global__ void __launch_bounds__(BS * BB) k_mp_correct(const int *__restrict__ 
 // into *leafmax.
 template <int D, int NC, int BS, int BB = 1>
 __global__ void __launch_bounds__(BS * BB) k_mp_residual(const int *__restrict__ ids, const DBox *__restrict__ boxes,
-                                                    const double *__restrict__ phi, const double *__restrict__ rhs,
-                                                    float *r32, double lambda2, int cyl, VarBc m,
-                                                    unsigned long long *leafmax, int bpb, int nw) {
+                                                         const double *__restrict__ phi, const double *__restrict__ rhs,
+                                                         float *r32, double lambda2, int cyl, VarBc m,
+                                                         unsigned long long *leafmax, int bpb, int nw) {
   using G = Geo<D, NC>;
   __shared__ double sh[BS / 64];
   for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
@@ -928,8 +933,9 @@ __device__ inline void lds_stage(T *s, const DBox *__restrict__ boxes, const DBo
 
 // k_gsrb with the box staged in LDS (levels with one workgroup per box).
 template <int D, int NC, int BS, typename T, int BB = 1>
-__global__ void __launch_bounds__(BS * BB) k_gsrb_lds(const int *__restrict__ ids, const DBox *__restrict__ boxes, T *phi,
-                                                 const T *__restrict__ rhs, int redblack, T lambda2, int cyl, VarBc m, int nw) {
+__global__ void __launch_bounds__(BS * BB) k_gsrb_lds(const int *__restrict__ ids, const DBox *__restrict__ boxes,
+                                                      T *phi, const T *__restrict__ rhs, int redblack, T lambda2,
+                                                      int cyl, VarBc m, int nw) {
   using G = Geo<D, NC>;
   __shared__ T s_all[BB][G::NCELL];
   T *s = s_all[lgrp<BS, BB>()];
@@ -1004,10 +1010,11 @@ __global__ void __launch_bounds__(BS * BB) k_gsrb_lds(const int *__restrict__ id
 
 // k_mp_residual with the box staged in LDS.
 template <int D, int NC, int BS, int BB = 1>
-__global__ void __launch_bounds__(BS * BB) k_mp_residual_lds(const int *__restrict__ ids, const DBox *__restrict__ boxes,
-                                                        const double *__restrict__ phi,
-                                                        const double *__restrict__ rhs, float *r32, double lambda2,
-                                                        int cyl, VarBc m, unsigned long long *leafmax, int nw) {
+__global__ void __launch_bounds__(BS * BB) k_mp_residual_lds(const int *__restrict__ ids,
+                                                             const DBox *__restrict__ boxes,
+                                                             const double *__restrict__ phi,
+                                                             const double *__restrict__ rhs, float *r32, double lambda2,
+                                                             int cyl, VarBc m, unsigned long long *leafmax, int nw) {
   using G = Geo<D, NC>;
   __shared__ double s_all[BB][G::NCELL];
   double *s = s_all[lgrp<BS, BB>()];
@@ -1053,9 +1060,9 @@ __global__ void __launch_bounds__(BS * BB) k_mp_residual_lds(const int *__restri
 // formed in-kernel, so no ghost fill has to precede it.
 template <int D, int NC, int BS, typename T, int BB = 1>
 __global__ void __launch_bounds__(BS * BB) k_restrict_residual_lds(const int *__restrict__ ids,
-                                                              const DBox *__restrict__ boxes, const T *child,
-                                                              T *phi, const T *__restrict__ rhs, T *tmp, T lambda2,
-                                                              int cyl, int store_residual, VarBc m, int nw) {
+                                                                   const DBox *__restrict__ boxes, const T *child,
+                                                                   T *phi, const T *__restrict__ rhs, T *tmp, T lambda2,
+                                                                   int cyl, int store_residual, VarBc m, int nw) {
   using G = Geo<D, NC>;
   constexpr int H = NC / 2;
   constexpr int NQ = (D == 3) ? H * H * H : H * H;
@@ -1204,7 +1211,8 @@ __global__ void __launch_bounds__(BS) k_smooth_block(const int *__restrict__ ids
 // cells get the restriction of the finer phi instead, and e there is
 // overwritten before the next cycle reads it.)
 template <int D, int NC, int BS, int BB = 1>
-__global__ void __launch_bounds__(BS * BB) k_mp_correct_leaves(const int *__restrict__ ids, double *phi, float *e, int nw) {
+__global__ void __launch_bounds__(BS * BB) k_mp_correct_leaves(const int *__restrict__ ids, double *phi, float *e,
+                                                               int nw) {
   using G = Geo<D, NC>;
   for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
     const int id = ids[w];
@@ -1225,7 +1233,7 @@ __global__ void __launch_bounds__(BS * BB) k_mp_correct_leaves(const int *__rest
 // One workgroup per box.
 template <int D, int NC, int BS, typename T, int BB = 1>
 __global__ void __launch_bounds__(BS * BB) k_gc_lds(const int *__restrict__ ids, const DBox *__restrict__ boxes, T *var,
-                                               VarBc m, int corners, int nw) {
+                                                    VarBc m, int corners, int nw) {
   using G = Geo<D, NC>;
   __shared__ T s_all[BB][G::NCELL];
   T *s = s_all[lgrp<BS, BB>()];
