@@ -45,6 +45,7 @@ OPT_DIRECT_MFMA = 6
 OPT_PHOTOI_VCYCLE = 7
 OPT_FLUX32 = 8
 OPT_STEP_GRAPH = 9
+OPT_FUSE_DIRECT = 10
 
 
 class ParamsT(C.Structure):

@@ -198,6 +198,7 @@ struct afgpu_ctx {
   bool direct_mfma = true;  // uniform-level transforms on the matrix cores
   bool photoi_inner_vcycle = false;  // photoionization inner cycle: V-cycle instead of FMG
   bool flux32 = true;  // mixed precision: FP32 flux stencil (FP64 accumulation)
+  bool fuse_direct = false;  // uniform-level solve: last-axis transforms fused into one pass
   struct Direct {
     bool set = false;  // lc and the map below are current for the tree
     int lc = 1, N = 0;
@@ -988,8 +989,16 @@ struct Impl : ImplBase {
     return AFGPU_OK;
   }
 
-  template <int N>
-  int direct_transforms(const afgpu_ctx::Direct::F &f, double lambda2, double *A0, double *A1) {
+  // The transforms of the uniform-level solve on the grids A0 / A1 (the
+  // level gathered into A0; *result: the grid holding the result).  On the matrix cores
+  // with a box pool given (rhs_pool, phi_pool), the first x transform reads
+  // the level's rhs from the pool and the last writes phi to it (gather and
+  // scatter fused), and the last forward transform, the division and the
+  // first backward one run as one pass when that pass has a workgroup per
+  // CU at least.
+  template <int N, typename T>
+  int direct_transforms(const afgpu_ctx::Direct::F &f, double lambda2, double *A0, double *A1,
+                        const double **result, const T *rhs_pool = nullptr, T *phi_pool = nullptr) {
     const size_t NN = (size_t)N * N;
     // the grids of all runs are transformed together: the x rows of every
     // run, and the y lines with the run as the remaining coordinate
@@ -1003,49 +1012,85 @@ struct Impl : ImplBase {
     // rows(): the x transform (MT: transposed matrix for the small-N
     // kernel, M itself for the tiled one); lines(): the y / z transforms
     constexpr bool TILED = (N == 64 || N == 128);
+    constexpr int NM = TILED ? N : 64;
     const int gemm_row_blocks = (N / 64) * (nrows / 64);
     const int gemm_line_blocks = (N / 64) * (N / 64) * batch;
     const bool MC = TILED && c.direct_mfma;
     const int mfma_row_blocks = (N / 64) * (nrows / 64);
     const int mfma_line_blocks = (N / 64) * (N / 64) * batch;
-    auto rows = [&](const double *in, double *out, const double *Mx, const double *MTx) {
-      if (MC)
-        hipLaunchKernelGGL((k_lc_mfma<D, (TILED ? N : 64), true>), dim3(mfma_row_blocks), dim3(256), 0, c.cur, in,
-                           out, Mx, 0, 0, (const double *)f.eig, lambda2);
+    const bool pool = MC && rhs_pool && phi_pool;
+    const int nb = N / NC;
+    const int *map = c.dir.d_map;
+    const int *nomap = nullptr;
+    const T *noin = nullptr;
+    T *noout = nullptr;
+    auto rows = [&](const double *in, double *out, const double *Mx, const double *MTx, int io) {
+      if (MC && io == 1)
+        hipLaunchKernelGGL((k_lc_mfma<D, NM, true, 1, NC, T>), dim3(mfma_row_blocks), dim3(256), 0, c.cur, in, out,
+                           Mx, 0, 0, (const double *)f.eig, lambda2, map, nb, rhs_pool, noout);
+      else if (MC && io == 2)
+        hipLaunchKernelGGL((k_lc_mfma<D, NM, true, 2, NC, T>), dim3(mfma_row_blocks), dim3(256), 0, c.cur, in, out,
+                           Mx, 0, 0, (const double *)f.eig, lambda2, map, nb, noin, phi_pool);
+      else if (MC)
+        hipLaunchKernelGGL((k_lc_mfma<D, NM, true, 0, NC, T>), dim3(mfma_row_blocks), dim3(256), 0, c.cur, in, out,
+                           Mx, 0, 0, (const double *)f.eig, lambda2, nomap, 0, noin, noout);
       else if (TILED)
-        hipLaunchKernelGGL((k_lc_gemm<D, (TILED ? N : 64), true>), dim3(gemm_row_blocks), dim3(256), 0, c.cur, in,
-                           out, Mx, 0, 0, (const double *)f.eig, lambda2);
+        hipLaunchKernelGGL((k_lc_gemm<D, NM, true>), dim3(gemm_row_blocks), dim3(256), 0, c.cur, in, out, Mx, 0, 0,
+                           (const double *)f.eig, lambda2);
       else
         hipLaunchKernelGGL((k_lc_rows<D, N>), dim3(row_blocks), dim3(256), 0, c.cur, in, out, MTx, nrows);
       return hipGetLastError();
     };
     auto lines = [&](const double *in, double *out, const double *M, int axis, int div) {
       if (MC)
-        hipLaunchKernelGGL((k_lc_mfma<D, (TILED ? N : 64), false>), dim3(mfma_line_blocks), dim3(256), 0, c.cur,
-                           in, out, M, axis, div, (const double *)f.eig, lambda2);
+        hipLaunchKernelGGL((k_lc_mfma<D, NM, false, 0, NC, T>), dim3(mfma_line_blocks), dim3(256), 0, c.cur, in,
+                           out, M, axis, div, (const double *)f.eig, lambda2, nomap, 0, noin, noout);
       else if (TILED)
-        hipLaunchKernelGGL((k_lc_gemm<D, (TILED ? N : 64), false>), dim3(gemm_line_blocks), dim3(256), 0, c.cur,
-                           in, out, M, axis, div, (const double *)f.eig, lambda2);
+        hipLaunchKernelGGL((k_lc_gemm<D, NM, false>), dim3(gemm_line_blocks), dim3(256), 0, c.cur, in, out, M, axis,
+                           div, (const double *)f.eig, lambda2);
       else
         hipLaunchKernelGGL((k_lc_lines<D, N>), dim3(line_blocks), dim3(256), 0, c.cur, in, out, M, axis, div,
                            (const double *)f.eig, lambda2);
       return hipGetLastError();
     };
-    if (D == 3) {
-      HIPCHK(rows(A0, A1, f.vinv, f.vinvT));
-      HIPCHK(lines(A1, A0, f.vinv + NN, 1, 0));
-      HIPCHK(lines(A0, A1, f.vinv + 2 * NN, 2, 1));
-      HIPCHK(lines(A1, A0, f.v + 2 * NN, 2, 0));
-      HIPCHK(lines(A0, A1, f.v + NN, 1, 0));
-      HIPCHK(rows(A1, A0, f.v, f.vT));
+    // the last forward, the division and the first backward transform
+    // along the last axis: one pass, or two
+    const int fused_blocks = (N / 64) * batch;
+    const bool fuse = MC && c.fuse_direct && fused_blocks >= c.n_cu;
+    const int last = D - 1;
+    const double *vinv_l = f.vinv + (size_t)last * NN, *v_l = f.v + (size_t)last * NN;
+    // ping-pong between the grids: `cur` holds the latest result
+    double *cur = A0, *nxt = A1;
+    auto step = [&](hipError_t e) {
+      std::swap(cur, nxt);
+      return e;
+    };
+    if (pool) {
+      nxt = A1;
+      HIPCHK(step(rows(nullptr, nxt, f.vinv, f.vinvT, 1)));
     } else {
-      HIPCHK(rows(A0, A1, f.vinv, f.vinvT));
-      HIPCHK(lines(A1, A0, f.vinv + NN, 1, 1));
-      HIPCHK(lines(A0, A1, f.v + NN, 1, 0));
-      HIPCHK(rows(A1, A0, f.v, f.vT));
+      HIPCHK(step(rows(cur, nxt, f.vinv, f.vinvT, 0)));
+    }
+    if (D == 3) HIPCHK(step(lines(cur, nxt, f.vinv + NN, 1, 0)));
+    if (fuse) {
+      hipLaunchKernelGGL((k_lc_mfma_fused<D, NM>), dim3(fused_blocks), dim3(4 * NM), 0, c.cur, cur, nxt, vinv_l,
+                         v_l, (const double *)f.eig, lambda2);
+      HIPCHK(step(hipGetLastError()));
+    } else {
+      HIPCHK(step(lines(cur, nxt, vinv_l, last, 1)));
+      HIPCHK(step(lines(cur, nxt, v_l, last, 0)));
+    }
+    if (D == 3) HIPCHK(step(lines(cur, nxt, f.v + NN, 1, 0)));
+    if (pool) {
+      HIPCHK(rows(cur, nullptr, f.v, f.vT, 2));
+      *result = nullptr;
+    } else {
+      HIPCHK(step(rows(cur, nxt, f.v, f.vT, 0)));
+      *result = cur;
     }
     return AFGPU_OK;
   }
+
 
   // phi(lc) := L^-1 rhs(lc) on the whole uniform level (homogeneous
   // boundary conditions: the correction equation), then its ghost layer.
@@ -1066,20 +1111,30 @@ struct Impl : ImplBase {
       set_err("uniform-level solve: grids not allocated");
       return AFGPU_ERR_STATE;
     }
-    hipLaunchKernelGGL((k_lc_gather<D, NC, BS, T>), dim3(nbox), dim3(BS), 0, c.cur, (const int *)d.d_map, nb,
-                       (const T *)v.rhs, A0);
-    HIPCHK(hipGetLastError());
+    // on the matrix cores (64 or 128 cells per axis) the transforms read
+    // the rhs from the pool and write phi to it themselves
+    const bool pool = c.direct_mfma && (d.N == 64 || d.N == 128);
+    if (!pool) {
+      hipLaunchKernelGGL((k_lc_gather<D, NC, BS, T>), dim3(nbox), dim3(BS), 0, c.cur, (const int *)d.d_map, nb,
+                         (const T *)v.rhs, A0);
+      HIPCHK(hipGetLastError());
+    }
+    const T *rp = pool ? (const T *)v.rhs : nullptr;
+    T *pp = pool ? v.phi : nullptr;
     const double l2 = c.ops[v.op].lambda2;
+    const double *res = nullptr;
     switch (d.N) {
-      case 16: RCHK(direct_transforms<16>(f, l2, A0, A1)); break;
-      case 32: RCHK(direct_transforms<32>(f, l2, A0, A1)); break;
-      case 64: RCHK(direct_transforms<64>(f, l2, A0, A1)); break;
-      case 128: RCHK(direct_transforms<128>(f, l2, A0, A1)); break;
+      case 16: RCHK((direct_transforms<16, T>(f, l2, A0, A1, &res))); break;
+      case 32: RCHK((direct_transforms<32, T>(f, l2, A0, A1, &res))); break;
+      case 64: RCHK((direct_transforms<64, T>(f, l2, A0, A1, &res, rp, pp))); break;
+      case 128: RCHK((direct_transforms<128, T>(f, l2, A0, A1, &res, rp, pp))); break;
       default: set_err("uniform-level solve: unsupported size"); return AFGPU_ERR_UNSUPPORTED;
     }
-    hipLaunchKernelGGL((k_lc_scatter<D, NC, BS, T>), dim3(nbox), dim3(BS), 0, c.cur, (const int *)d.d_map, nb,
-                       (const double *)A0, v.phi);
-    HIPCHK(hipGetLastError());
+    if (!pool) {
+      hipLaunchKernelGGL((k_lc_scatter<D, NC, BS, T>), dim3(nbox), dim3(BS), 0, c.cur, (const int *)d.d_map, nb,
+                         res, v.phi);
+      HIPCHK(hipGetLastError());
+    }
     return gc_var(lc, v.phi, v.m, true);
   }
 
@@ -2926,6 +2981,11 @@ int afgpu_set_option(afgpu_t *h, int option, int value) {
       h->direct_max_n = value;
       h->dir.set = false;
       for (auto &f : h->dir.f) f.ok = false;
+      return AFGPU_OK;
+    case AFGPU_OPT_FUSE_DIRECT:
+      RCHK(sync_stream(h));
+      drop_graphs(*h);
+      h->fuse_direct = value != 0;
       return AFGPU_OK;
     case AFGPU_OPT_STEP_GRAPH:
       RCHK(sync_stream(h));

@@ -277,10 +277,33 @@ __global__ void __launch_bounds__(256) k_lc_gemm(const double *__restrict__ in, 
 // B operand B[l / 16][l % 16], result register j of lane l D[4 j + l / 16][l % 16].
 typedef double lc_double4 __attribute__((ext_vector_type(4)));
 
-template <int D, int N, bool ROWS>
+// Offset in the box pool of point (x, y, r) of the uniform level, stored as
+// boxes of NC^D cells: r is z in 3D and the run of a forest in 2D; -1 when
+// the box is masked out.
+template <int D, int NC>
+__device__ inline long long lc_pool_offset(const int *__restrict__ map, int nb, int x, int y, int r) {
+  using G = Geo<D, NC>;
+  int b, ck = 0;
+  if (D == 3) {
+    b = x / NC + nb * (y / NC + nb * (r / NC));
+    ck = r % NC + 1;
+  } else {
+    b = x / NC + nb * (y / NC) + r * nb * nb;
+  }
+  const int id = map[b];
+  return id > 0 ? (long long)(id - 1) * G::NCELL + G::idx(x % NC + 1, y % NC + 1, ck) : -1;
+}
+
+// IO (x form only): 0 grid to grid; 1 the input rows read from the box
+// pool `pin` (the gather, fused); 2 the output rows written to the box pool
+// `pout` (the scatter, fused).  Box map and boxes per axis as k_lc_gather.
+template <int D, int N, bool ROWS, int IO = 0, int NC = 16, typename TS = float>
 __global__ void __launch_bounds__(256) k_lc_mfma(const double *__restrict__ in, double *__restrict__ out,
                                                  const double *__restrict__ M, int axis, int div,
-                                                 const double *__restrict__ eig, double lambda2) {
+                                                 const double *__restrict__ eig, double lambda2,
+                                                 const int *__restrict__ map, int nb, const TS *__restrict__ pin,
+                                                 TS *__restrict__ pout) {
+  static_assert(IO == 0 || ROWS, "box-pool input and output in the x form only");
   static_assert(N == 64 || N == 128, "matrix-core transform for N = 64, 128");
   constexpr int T = 64, KC = 32, AP = KC + 1, BP = T + 1, NT = N / T;
   constexpr int PER = T * KC / 256;  // values of each operand tile per thread
@@ -312,7 +335,13 @@ __global__ void __launch_bounds__(256) k_lc_mfma(const double *__restrict__ in, 
     for (int u = 0; u < PER; u++) {
       const int q = tid + 256 * u;
       const int pp = q / KC, kk = q % KC;
-      pa[u] = ROWS ? in[(size_t)(p0 + pp) * N + k0 + kk] : M[(size_t)(p0 + pp) * N + k0 + kk];
+      if (IO == 1) {
+        const int row = p0 + pp;
+        const long long off = lc_pool_offset<D, NC>(map, nb, k0 + kk, row % N, row / N);
+        pa[u] = off >= 0 ? (double)pin[off] : 0.0;
+      } else {
+        pa[u] = ROWS ? in[(size_t)(p0 + pp) * N + k0 + kk] : M[(size_t)(p0 + pp) * N + k0 + kk];
+      }
       if (ROWS) {
         pb[u] = M[(size_t)(q0 + pp) * N + k0 + kk];
       } else {
@@ -358,7 +387,10 @@ __global__ void __launch_bounds__(256) k_lc_mfma(const double *__restrict__ in, 
         const int pp = p0 + wp + 16 * ta + 4 * j + (lane >> 4);
         const int qq = q0 + wq + 16 * tb + (lane & 15);
         double v = acc[ta][tb][j];
-        if (ROWS) {
+        if (ROWS && IO == 2) {
+          const long long off = lc_pool_offset<D, NC>(map, nb, qq, pp % N, pp / N);
+          if (off >= 0) pout[off] = (TS)v;
+        } else if (ROWS) {
           out[(size_t)pp * N + qq] = v;
         } else {
           if (div) {
@@ -381,6 +413,71 @@ __global__ void __launch_bounds__(256) k_lc_mfma(const double *__restrict__ in, 
           }
           out[base + (size_t)pp * sm + qq] = v;
         }
+      }
+}
+
+// The last forward transform (z in 3D, y in 2D), the division by the
+// eigenvalue sums and the first backward transform along the same lines, in
+// one pass: a workgroup (N/16 waves) holds all N points of a 64-column tile
+// of lines in LDS, forms D1 = Mf B on the matrix cores, divides, keeps D1 in
+// LDS and forms D2 = Mb D1 (each product with the K order of k_lc_mfma).
+// One read and one write of the grid instead of two of each.
+template <int D, int N>
+__global__ void __launch_bounds__(4 * N) k_lc_mfma_fused(const double *__restrict__ in, double *__restrict__ out,
+                                                         const double *__restrict__ Mf, const double *__restrict__ Mb,
+                                                         const double *__restrict__ eig, double lambda2) {
+  static_assert(N == 64 || N == 128, "matrix-core transform for N = 64, 128");
+  constexpr int T = 64, KC = 32, AP = KC + 1, BP = T + 1, NT = N / T, NTH = 4 * N;
+  __shared__ double Bs[N * BP];   // the lines Bs[k][q], then D1
+  __shared__ double As[N * AP];   // one K chunk of the matrix, As[p][kk]
+  const int xt = blockIdx.x % NT, o = blockIdx.x / NT;  // o: y in 3D, the run in 2D
+  const int x0 = xt * T;
+  const size_t sm = (D == 3) ? (size_t)N * N : (size_t)N;  // stride along the lines
+  const size_t so = (D == 3) ? (size_t)N : (size_t)N * N;  // stride of o
+  const size_t base = (size_t)o * so + x0;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wp = (w >> 1) * 32, wq = (w & 1) * 32;
+  for (int q = tid; q < N * T; q += NTH) Bs[(q / T) * BP + q % T] = in[base + (size_t)(q / T) * sm + q % T];
+  lc_double4 acc[2][2];
+  auto product = [&](const double *__restrict__ M) {
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++) acc[a][b] = lc_double4{0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < N; k0 += KC) {
+      __syncthreads();  // (the chunk and, before the first, the lines in LDS)
+      for (int q = tid; q < N * KC; q += NTH) As[(q / KC) * AP + q % KC] = M[(size_t)(q / KC) * N + k0 + q % KC];
+      __syncthreads();
+      for (int ks = 0; ks < KC; ks += 4) {
+        const int kk = ks + (lane >> 4);
+        double av[2], bv[2];
+        for (int t = 0; t < 2; t++) {
+          av[t] = As[(wp + 16 * t + (lane & 15)) * AP + kk];
+          bv[t] = Bs[(k0 + kk) * BP + wq + 16 * t + (lane & 15)];
+        }
+        for (int ta = 0; ta < 2; ta++)
+          for (int tb = 0; tb < 2; tb++)
+            acc[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ta], bv[tb], acc[ta][tb], 0, 0, 0);
+      }
+    }
+  };
+  product(Mf);
+  __syncthreads();  // every wave is done with the lines
+  for (int ta = 0; ta < 2; ta++)
+    for (int tb = 0; tb < 2; tb++)
+      for (int j = 0; j < 4; j++) {
+        const int pp = wp + 16 * ta + 4 * j + (lane >> 4);  // position along the lines
+        const int qq = wq + 16 * tb + (lane & 15);
+        double den = eig[x0 + qq] + eig[N + ((D == 3) ? o : pp)];
+        if (D == 3) den = den + eig[2 * N + pp];
+        den = den - lambda2;
+        Bs[pp * BP + qq] = acc[ta][tb][j] / den;
+      }
+  product(Mb);
+  for (int ta = 0; ta < 2; ta++)
+    for (int tb = 0; tb < 2; tb++)
+      for (int j = 0; j < 4; j++) {
+        const int pp = wp + 16 * ta + 4 * j + (lane >> 4);
+        const int qq = wq + 16 * tb + (lane & 15);
+        out[base + (size_t)pp * sm + qq] = acc[ta][tb][j];
       }
 }
 

@@ -77,9 +77,11 @@ def tune(p, enabled):
 
 
 def run_gpu(case, precision, warmup, steps, timing_steps, direct_n=None, smooth=None, block=None, batch=None,
-            photoi_vcycle=None, autotune=True):
+            photoi_vcycle=None, autotune=True, fuse_direct=None):
     p = A.Path("gpu", case.D, case.nc, case.coord, precision)
     S.apply(p, case)
+    if fuse_direct is not None:
+        p.set_option(A.OPT_FUSE_DIRECT, fuse_direct)
     if direct_n is not None:
         p.set_option(A.OPT_DIRECT_N, direct_n)
     if smooth is not None:
@@ -297,6 +299,8 @@ def main():
                     help="mixed precision: block smoother in the inner cycle (1) or red-black sweeps (0)")
     ap.add_argument("--batch-modes", type=int, default=1,
                     help="mixed precision: photoionization modes as parallel graph branches (1) or in turn (0)")
+    ap.add_argument("--fuse-direct", type=int, default=0,
+                    help="mixed precision: uniform-level solve with the fused transform passes (1) or separate (0)")
     ap.add_argument("--photoi-vcycle", type=int, default=0,
                     help="mixed precision: photoionization inner cycle V-cycle (1) or FMG (0)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -339,7 +343,8 @@ def main():
     p, infos, secs, stats = run_gpu(case, prec, args.warmup, args.steps, args.timing_steps,
                                     args.direct_n if mixed else None, args.mixed_smooth if mixed else None,
                                     args.block_smooth if mixed else None, args.batch_modes if mixed else None,
-                                    args.photoi_vcycle if mixed else None, bool(args.autotune))
+                                    args.photoi_vcycle if mixed else None, bool(args.autotune),
+                                    args.fuse_direct if mixed else None)
     cell_steps = sum(i["leaf_cells"] for i in infos)
     value = cell_steps / secs
     nv = [i["n_vcycles"] for i in infos]
@@ -434,6 +439,7 @@ def main():
                    "block_smooth": args.block_smooth if prec == A.PREC_MIXED else None,
                    "batch_modes": args.batch_modes if prec == A.PREC_MIXED else None,
                    "photoi_vcycle": args.photoi_vcycle if prec == A.PREC_MIXED else None,
+                   "fuse_direct": args.fuse_direct if prec == A.PREC_MIXED else None,
                    "shapes": p.shapes},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -282,6 +282,10 @@ int afgpu_kernel_stats(afgpu_t *h, int kind, int reset, double *total_ms,
                                   captured graph with the stopping tests on the device (1), or
                                   one captured graph per cycle with the tests on the host (0);
                                   the results are identical (default 0, measured faster) */
+#define AFGPU_OPT_FUSE_DIRECT 10 /* mixed precision: the uniform-level solve's last forward
+                                    transform, division and first backward transform as one pass
+                                    (1) or two (0); identical results (default 0: the one pass,
+                                    at one 100 KB-LDS workgroup per CU, measured slower) */
 int afgpu_set_option(afgpu_t *h, int option, int value);
 
 /* ---- ensembles of independent runs --------------------------------------

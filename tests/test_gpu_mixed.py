@@ -226,6 +226,23 @@ def test_mixed_direct_solve_128_matrix_cores(A, S):
         assert e < 1e-11
 
 
+def test_mixed_direct_solve_fused_passes_bit_for_bit(A, S):
+    """The uniform-level solve with the gather and scatter folded into the
+    x transforms and the last axis's forward transform, division and
+    backward transform as one pass gives the same bits as the separate
+    passes: 3D with photoionization (level 4 = 128^3), a few full steps."""
+    case = setup(S.small_amr(3, nc=16, uniform_lvl=4, max_lvl=5, frac=0.03, seed=4), photoi=True)
+    runs = []
+    for fuse in (1, 0):
+        m = S.apply(A.Path("gpu", case.D, case.nc, case.coord, A.PREC_MIXED), case)
+        m.set_option(A.OPT_FUSE_DIRECT, fuse)
+        infos = [m.field_compute(0, False)] + [m.step() for _ in range(3)]
+        runs.append((infos, [m.get_cc(iv) for iv in (A.CC_PHI, A.CC_ELECTRON, A.CC_PHOTO)]))
+    assert runs[0][0] == runs[1][0]
+    for a, b in zip(runs[0][1], runs[1][1]):
+        assert np.array_equal(a, b)
+
+
 def test_mixed_graph_replays_match_eager(A, S):
     case = setup(CASES["xyz3d_amr"](S), photoi=True)
     case.params.photoi_max_rel_residual = 1e-3
