@@ -531,10 +531,11 @@ struct Impl : ImplBase {
 
   template <typename T>
   int boxop(const List &l, const T *a, T *b, int op, T s = T(0)) {
-    const int k = bpb(l.n, G::NCELL);
+    constexpr int STRIDE = Lay<D, NC, T>::STRIDE;
+    const int k = bpb(l.n, STRIDE);
     return with_bb(KC_BOXOP, [&](auto B) {
       constexpr int bb = decltype(B)::value;
-      return items<bb>(KC_BOXOP, k_boxop<G::NCELL, BS, T, bb>, l.n * k, c.list(l), a, b, op, s, k);
+      return items<bb>(KC_BOXOP, k_boxop<STRIDE, BS, T, bb>, l.n * k, c.list(l), a, b, op, s, k);
     });
   }
 
@@ -685,8 +686,12 @@ struct Impl : ImplBase {
     const int k = bpb(l.n, G::NINT);
     return with_bb(KC_PROLONG, [&](auto B) {
       constexpr int bb = decltype(B)::value;
-      return items<bb>(KC_PROLONG, k_prolong_add<D, NC, BS, T, bb>, l.n * k, c.list(l), (const DBox *)c.d_boxes,
-                       (const T *)v.tmp, target ? target : v.phi, k);
+      if constexpr (Lay<D, NC, T>::SPLIT)
+        return items<bb>(KC_PROLONG, k_prolong_add_split<D, NC, BS, bb>, l.n * k, c.list(l), (const DBox *)c.d_boxes,
+                         (const T *)v.tmp, target ? target : v.phi, k);
+      else
+        return items<bb>(KC_PROLONG, k_prolong_add<D, NC, BS, T, bb>, l.n * k, c.list(l), (const DBox *)c.d_boxes,
+                         (const T *)v.tmp, target ? target : v.phi, k);
     });
   }
 

@@ -82,6 +82,77 @@ struct Geo {
   }
 };
 
+// ---- storage of a box's cells in a pool -------------------------------------
+// Pools of FP64 values keep afivo's box layout (Geo::idx), so a box moves to
+// and from the Fortran arrays as it is.  Pools of FP32 values (the mixed
+// form's inner cycle, internal to the library) keep the interior cells of
+// each red-black colour ((i + j + k) & 1) together, in lexicographic order,
+// followed by the ghost cells in afivo's order: a half-sweep then reads and
+// writes the cache lines of one colour only.  Both take Geo::NCELL values
+// per box (STRIDE).  at(): position of any cell of the box; in_at(): of an
+// interior cell; nb_at(): of the neighbour (i, j, k) + o e_d of an interior
+// cell.
+//
+// afivo's layout, also that of every LDS copy of a box:
+template <int D, int NC>
+struct LayStd {
+  using G = Geo<D, NC>;
+  static constexpr bool SPLIT = false;
+  static constexpr int STRIDE = G::NCELL;
+  __host__ __device__ static constexpr int at(int i, int j, int k) { return G::idx(i, j, k); }
+  __host__ __device__ static constexpr int in_at(int i, int j, int k) { return G::idx(i, j, k); }
+  __host__ __device__ static int nb_at(int i, int j, int k, int d, int o) {
+    constexpr int st[3] = {1, G::NG, G::NG * G::NG};
+    return G::idx(i, j, k) + o * st[d];
+  }
+};
+template <int D, int NC, typename T>
+struct Lay : LayStd<D, NC> {};
+template <int D, int NC>
+struct Lay<D, NC, float> {
+  using G = Geo<D, NC>;
+  static constexpr bool SPLIT = true;
+  static constexpr int HALF = G::NINT / 2;  // interior cells of one colour
+  static constexpr int STRIDE = G::NCELL;
+  __host__ __device__ static int in_at(int i, int j, int k) {
+    const int q = (i - 1) + NC * ((j - 1) + NC * (D == 3 ? k - 1 : 0));
+    return ((i + j + k) & 1) * HALF + (q >> 1);
+  }
+  // a ghost cell: its afivo position less the interior cells before it
+  __host__ __device__ static int ghost_at(int i, int j, int k) {
+    auto before = [](int v) { return v < 1 ? 0 : (v > NC ? NC : v - 1); };
+    auto inside = [](int v) { return v >= 1 && v <= NC; };
+    int n = before(j) * NC + (inside(j) ? before(i) : 0);
+    if (D == 3) n = before(k) * NC * NC + (inside(k) ? n : 0);
+    return G::NINT + G::idx(i, j, k) - n;
+  }
+  __host__ __device__ static int at(int i, int j, int k) {
+    const bool in = i >= 1 && i <= NC && j >= 1 && j <= NC && (D == 2 || (k >= 1 && k <= NC));
+    return in ? in_at(i, j, k) : ghost_at(i, j, k);
+  }
+  __host__ __device__ static int nb_at(int i, int j, int k, int d, int o) {
+    constexpr int st[3] = {1, NC, NC * NC};
+    int p[3] = {i, j, k};
+    p[d] += o;
+    if (p[d] < 1 || p[d] > NC) return ghost_at(p[0], p[1], p[2]);
+    const int q = (i - 1) + NC * ((j - 1) + NC * (D == 3 ? k - 1 : 0));
+    return (1 - ((i + j + k) & 1)) * HALF + ((q + o * st[d]) >> 1);
+  }
+};
+
+// Interior cell number q of red-black colour c ((i + j + k) & 1 == c), in
+// lexicographic order: row by row, every other cell (cells 2q and 2q + 1 of
+// the box in lexicographic order are neighbours in one row, one of each
+// colour).  Lay<D, NC, float>::in_at of this cell is c * NINT / 2 + q.
+template <int D, int NC>
+__host__ __device__ inline void colour_cell(int c, int q, int &i, int &j, int &k) {
+  const int p = 2 * q;
+  i = p % NC + 1;
+  j = (p / NC) % NC + 1;
+  k = (D == 3) ? p / (NC * NC) + 1 : 0;
+  if (((i + j + k) & 1) != c) i += 1;
+}
+
 __host__ __device__ inline void tangential(int D, int d, int &t0, int &t1) {
   if (D == 2) {
     t0 = 1 - d;

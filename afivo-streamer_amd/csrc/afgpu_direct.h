@@ -32,11 +32,11 @@ __global__ void __launch_bounds__(BS) k_lc_gather(const int *__restrict__ map, i
   const int bx = b0 % nb, by = (b0 / nb) % nb, bz = (D == 3) ? b0 / (nb * nb) : 0;
   const int N = nb * NC;
   A += (size_t)run * ((D == 3) ? (size_t)N * N * N : (size_t)N * N);
-  const T *u = src + (size_t)(id - 1) * G::NCELL;
+  const T *u = src + (size_t)(id - 1) * Lay<D, NC, T>::STRIDE;
   for (int q = threadIdx.x; q < G::NINT; q += BS) {
     const int i = q % NC, j = (q / NC) % NC, k = (D == 3) ? q / (NC * NC) : 0;
     const size_t g = (size_t)(bx * NC + i) + (size_t)N * ((by * NC + j) + (size_t)N * (bz * NC + k));
-    A[g] = (double)u[G::idx(i + 1, j + 1, (D == 3) ? k + 1 : 0)];
+    A[g] = (double)u[Lay<D, NC, T>::in_at(i + 1, j + 1, (D == 3) ? k + 1 : 0)];
   }
 }
 
@@ -52,11 +52,11 @@ __global__ void __launch_bounds__(BS) k_lc_scatter(const int *__restrict__ map, 
   const int bx = b0 % nb, by = (b0 / nb) % nb, bz = (D == 3) ? b0 / (nb * nb) : 0;
   const int N = nb * NC;
   A += (size_t)run * ((D == 3) ? (size_t)N * N * N : (size_t)N * N);
-  T *u = dst + (size_t)(id - 1) * G::NCELL;
+  T *u = dst + (size_t)(id - 1) * Lay<D, NC, T>::STRIDE;
   for (int q = threadIdx.x; q < G::NINT; q += BS) {
     const int i = q % NC, j = (q / NC) % NC, k = (D == 3) ? q / (NC * NC) : 0;
     const size_t g = (size_t)(bx * NC + i) + (size_t)N * ((by * NC + j) + (size_t)N * (bz * NC + k));
-    u[G::idx(i + 1, j + 1, (D == 3) ? k + 1 : 0)] = (T)A[g];
+    u[Lay<D, NC, T>::in_at(i + 1, j + 1, (D == 3) ? k + 1 : 0)] = (T)A[g];
   }
 }
 
@@ -277,12 +277,12 @@ __global__ void __launch_bounds__(256) k_lc_gemm(const double *__restrict__ in, 
 // B operand B[l / 16][l % 16], result register j of lane l D[4 j + l / 16][l % 16].
 typedef double lc_double4 __attribute__((ext_vector_type(4)));
 
-// Offset in the box pool of point (x, y, r) of the uniform level, stored as
-// boxes of NC^D cells: r is z in 3D and the run of a forest in 2D; -1 when
-// the box is masked out.
-template <int D, int NC>
+// Offset in the box pool (of values of type TS) of point (x, y, r) of the
+// uniform level, stored as boxes of NC^D cells: r is z in 3D and the run of
+// a forest in 2D; -1 when the box is masked out.
+template <int D, int NC, typename TS>
 __device__ inline long long lc_pool_offset(const int *__restrict__ map, int nb, int x, int y, int r) {
-  using G = Geo<D, NC>;
+  using L = Lay<D, NC, TS>;
   int b, ck = 0;
   if (D == 3) {
     b = x / NC + nb * (y / NC + nb * (r / NC));
@@ -291,7 +291,7 @@ __device__ inline long long lc_pool_offset(const int *__restrict__ map, int nb, 
     b = x / NC + nb * (y / NC) + r * nb * nb;
   }
   const int id = map[b];
-  return id > 0 ? (long long)(id - 1) * G::NCELL + G::idx(x % NC + 1, y % NC + 1, ck) : -1;
+  return id > 0 ? (long long)(id - 1) * L::STRIDE + L::in_at(x % NC + 1, y % NC + 1, ck) : -1;
 }
 
 // IO (x form only): 0 grid to grid; 1 the input rows read from the box
@@ -337,7 +337,7 @@ __global__ void __launch_bounds__(256) k_lc_mfma(const double *__restrict__ in, 
       const int pp = q / KC, kk = q % KC;
       if (IO == 1) {
         const int row = p0 + pp;
-        const long long off = lc_pool_offset<D, NC>(map, nb, k0 + kk, row % N, row / N);
+        const long long off = lc_pool_offset<D, NC, TS>(map, nb, k0 + kk, row % N, row / N);
         pa[u] = off >= 0 ? (double)pin[off] : 0.0;
       } else {
         pa[u] = ROWS ? in[(size_t)(p0 + pp) * N + k0 + kk] : M[(size_t)(p0 + pp) * N + k0 + kk];
@@ -388,7 +388,7 @@ __global__ void __launch_bounds__(256) k_lc_mfma(const double *__restrict__ in, 
         const int qq = q0 + wq + 16 * tb + (lane & 15);
         double v = acc[ta][tb][j];
         if (ROWS && IO == 2) {
-          const long long off = lc_pool_offset<D, NC>(map, nb, qq, pp % N, pp / N);
+          const long long off = lc_pool_offset<D, NC, TS>(map, nb, qq, pp % N, pp / N);
           if (off >= 0) pout[off] = (TS)v;
         } else if (ROWS) {
           out[(size_t)pp * N + qq] = v;

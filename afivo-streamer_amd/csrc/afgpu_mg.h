@@ -66,10 +66,25 @@ __device__ inline void cyl_coeffs(const DBox &bx, int i, const T *c, T &c1, T &c
   c1 = c[0] - (c2 - c[1]) - (c3 - c[2]);
 }
 
-// L phi at interior cell (i,j,k) (stencil_apply_357, m_af_stencil.f90:367-496).
-template <int D, int NC, typename T>
+// L phi at interior cell (i,j,k) (stencil_apply_357, m_af_stencil.f90:367-496)
+// of a box stored with layout L (afivo's by default: LDS copies, FP64 pools).
+template <int D, int NC, typename T, typename L = LayStd<D, NC>>
 __device__ inline T apply_cell(const T *u, const Sten<D, T> &s, bool cyl, const DBox &bx, int i, int j, int k) {
   using G = Geo<D, NC>;
+  if constexpr (L::SPLIT) {
+    const T uc = u[L::in_at(i, j, k)], xm = u[L::nb_at(i, j, k, 0, -1)], xp = u[L::nb_at(i, j, k, 0, 1)];
+    const T ym = u[L::nb_at(i, j, k, 1, -1)], yp = u[L::nb_at(i, j, k, 1, 1)];
+    if (D == 2) {
+      if (cyl) {
+        T c1, c2, c3;
+        cyl_coeffs<T>(bx, i, s.c, c1, c2, c3);
+        return c1 * uc + c2 * xm + c3 * xp + s.c[3] * ym + s.c[4] * yp;
+      }
+      return s.c[0] * uc + s.c[1] * xm + s.c[2] * xp + s.c[3] * ym + s.c[4] * yp;
+    }
+    return s.c[0] * uc + s.c[1] * xm + s.c[2] * xp + s.c[3] * ym + s.c[4] * yp + s.c[5] * u[L::nb_at(i, j, k, 2, -1)] +
+           s.c[6] * u[L::nb_at(i, j, k, 2, 1)];
+  }
   const int x = G::idx(i, j, k);
   if (D == 2) {
     if (cyl) {
@@ -106,6 +121,7 @@ template <int D, int NC, typename T>
 __device__ inline T ghost_value(const DBox *__restrict__ boxes, const DBox &bx, const T *var, const VarBc &m, int nb,
                                 int a, int b, T x1, T x2, const T *coarse = nullptr) {
   using G = Geo<D, NC>;
+  using L = Lay<D, NC, T>;
   constexpr int H = NC / 2;
   const int d = nb / 2;
   const bool low = (nb % 2) == 0;
@@ -113,12 +129,12 @@ __device__ inline T ghost_value(const DBox *__restrict__ boxes, const DBox &bx, 
   tangential(D, d, t0, t1);
   const int nb_id = bx.nb[nb];
   if (nb_id > 0) {
-    const T *w = var + (size_t)(nb_id - 1) * G::NCELL;
+    const T *w = var + (size_t)(nb_id - 1) * L::STRIDE;
     int q[3] = {0, 0, 0};
     q[t0] = a;
     if (D == 3) q[t1] = b;
     q[d] = low ? NC : 1;
-    return w[G::idx(q[0], q[1], q[2])];
+    return w[L::at(q[0], q[1], q[2])];
   }
   if (nb_id < 0) {
     T c0, c1, c2;
@@ -132,7 +148,7 @@ __device__ inline T ghost_value(const DBox *__restrict__ boxes, const DBox &bx, 
     return c0 * (T)bc_value(m, nb, bx.run) + c1 * x1 + c2 * x2;
   }
   const int p_nb_id = boxes[bx.parent - 1].nb[nb];
-  const T *P = (coarse ? coarse : var) + (size_t)(p_nb_id - 1) * G::NCELL;
+  const T *P = (coarse ? coarse : var) + (size_t)(p_nb_id - 1) * L::STRIDE;
   int co[3] = {0, 0, 0};
   for (int dd = 0; dd < D; dd++) co[dd] = ((bx.ix[dd] - 1) & 1) * H;
   if (m.rb == RB_MG) {
@@ -143,11 +159,11 @@ __device__ inline T ghost_value(const DBox *__restrict__ boxes, const DBox &bx, 
       int q[3] = {0, 0, 0};
       q[d] = cn;
       q[t0] = co[t0] + i + 1;
-      const T tp = P[G::idx(q[0], q[1], q[2])];
+      const T tp = P[L::at(q[0], q[1], q[2])];
       q[t0] = co[t0] + i - 1;
-      const T tm = P[G::idx(q[0], q[1], q[2])];
+      const T tm = P[L::at(q[0], q[1], q[2])];
       q[t0] = co[t0] + i;
-      const T tc = P[G::idx(q[0], q[1], q[2])];
+      const T tc = P[L::at(q[0], q[1], q[2])];
       const T grad = T(0.125) * (tp - tm);
       gcv = (a & 1) ? tc - grad : tc + grad;
     } else {
@@ -157,7 +173,7 @@ __device__ inline T ghost_value(const DBox *__restrict__ boxes, const DBox &bx, 
         q[d] = cn;
         q[t0] = co[t0] + qa;
         q[t1] = co[t1] + qb;
-        return P[G::idx(q[0], q[1], q[2])];
+        return P[L::at(q[0], q[1], q[2])];
       };
       const T g1 = T(0.125) * (TT(i + 1, j) - TT(i - 1, j));
       const T g2 = T(0.125) * (TT(i, j + 1) - TT(i, j - 1));
@@ -176,9 +192,9 @@ __device__ inline T ghost_value(const DBox *__restrict__ boxes, const DBox &bx, 
     int q[3] = {0, 0, 0};
     q[d] = ix_c;
     q[t0] = a1;
-    const T c1 = P[G::idx(q[0], q[1], q[2])];
+    const T c1 = P[L::at(q[0], q[1], q[2])];
     q[t0] = a2;
-    const T c2 = P[G::idx(q[0], q[1], q[2])];
+    const T c2 = P[L::at(q[0], q[1], q[2])];
     v = T(0.5) * c1 + sixth * c2 + third * x1;
     if (limited && v > 2 * c1) v = 2 * c1;
   } else {
@@ -187,11 +203,11 @@ __device__ inline T ghost_value(const DBox *__restrict__ boxes, const DBox &bx, 
     int q[3];
     q[d] = ix_c;
     q[t0] = a1; q[t1] = b1;
-    const T c1 = P[G::idx(q[0], q[1], q[2])];
+    const T c1 = P[L::at(q[0], q[1], q[2])];
     q[t0] = a2; q[t1] = b1;
-    const T ca = P[G::idx(q[0], q[1], q[2])];
+    const T ca = P[L::at(q[0], q[1], q[2])];
     q[t0] = a1; q[t1] = b2;
-    const T cb = P[G::idx(q[0], q[1], q[2])];
+    const T cb = P[L::at(q[0], q[1], q[2])];
     // the z-face case sums the off-diagonal coarse values in the other order
     const T c2 = (d == 2) ? cb : ca, c3 = (d == 2) ? ca : cb;
     v = third * c1 + sixth * c2 + sixth * c3 + third * x1;
@@ -222,7 +238,8 @@ __constant__ int c_edge_dir3[12][3] = {{0, -1, -1}, {0, 1, -1}, {0, -1, 1}, {0, 
 template <int D, int NC, typename T>
 __device__ inline void gc_edge_cell(const DBox &bx, T *var, int id, int n, int qq) {
   using G = Geo<D, NC>;
-  T *u = var + (size_t)(id - 1) * G::NCELL;
+  using L = Lay<D, NC, T>;
+  T *u = var + (size_t)(id - 1) * L::STRIDE;
   const int dim = c_edge_dim3[n];
   const int *dir = c_edge_dir3[n];
   int lo[3];
@@ -230,8 +247,8 @@ __device__ inline void gc_edge_cell(const DBox &bx, T *var, int id, int n, int q
   lo[dim] = qq;
   const int nb_id = bx.nbmat[(dir[0] + 1) + 3 * (dir[1] + 1) + 9 * (dir[2] + 1)];
   if (nb_id > 0) {
-    const T *w = var + (size_t)(nb_id - 1) * G::NCELL;
-    u[G::idx(lo[0], lo[1], lo[2])] = w[G::idx(lo[0] - dir[0] * NC, lo[1] - dir[1] * NC, lo[2] - dir[2] * NC)];
+    const T *w = var + (size_t)(nb_id - 1) * L::STRIDE;
+    u[L::at(lo[0], lo[1], lo[2])] = w[L::at(lo[0] - dir[0] * NC, lo[1] - dir[1] * NC, lo[2] - dir[2] * NC)];
   } else {
     const int o0 = (dim + 1) % 3, o1 = (dim + 2) % 3;
     int di[3];
@@ -240,31 +257,32 @@ __device__ inline void gc_edge_cell(const DBox &bx, T *var, int id, int n, int q
     int ia[3] = {lo[0], lo[1], lo[2]}, ib[3] = {lo[0], lo[1], lo[2]};
     ia[o0] += di[o0];
     ib[o1] += di[o1];
-    u[G::idx(lo[0], lo[1], lo[2])] = u[G::idx(ia[0], ia[1], ia[2])] + u[G::idx(ib[0], ib[1], ib[2])] -
-                                     u[G::idx(lo[0] + di[0], lo[1] + di[1], lo[2] + di[2])];
+    u[L::at(lo[0], lo[1], lo[2])] = u[L::at(ia[0], ia[1], ia[2])] + u[L::at(ib[0], ib[1], ib[2])] -
+                                     u[L::at(lo[0] + di[0], lo[1] + di[1], lo[2] + di[2])];
   }
 }
 
 template <int D, int NC, typename T>
 __device__ inline void gc_corner_cell(const DBox &bx, T *var, int id, int n) {
   using G = Geo<D, NC>;
-  T *u = var + (size_t)(id - 1) * G::NCELL;
+  using L = Lay<D, NC, T>;
+  T *u = var + (size_t)(id - 1) * L::STRIDE;
   const int cx = n & 1, cy = (n >> 1) & 1, cz = (n >> 2) & 1;
   const int dnb[3] = {2 * cx - 1, 2 * cy - 1, (D == 3) ? 2 * cz - 1 : 0};
   const int lo[3] = {cx * (NC + 1), cy * (NC + 1), (D == 3) ? cz * (NC + 1) : 0};
   const int nb_id = bx.nbmat[(dnb[0] + 1) + 3 * (dnb[1] + 1) + 9 * (dnb[2] + 1)];
   if (nb_id > 0) {
-    const T *w = var + (size_t)(nb_id - 1) * G::NCELL;
-    u[G::idx(lo[0], lo[1], lo[2])] = w[G::idx(lo[0] - dnb[0] * NC, lo[1] - dnb[1] * NC, lo[2] - dnb[2] * NC)];
+    const T *w = var + (size_t)(nb_id - 1) * L::STRIDE;
+    u[L::at(lo[0], lo[1], lo[2])] = w[L::at(lo[0] - dnb[0] * NC, lo[1] - dnb[1] * NC, lo[2] - dnb[2] * NC)];
   } else if (D == 2) {
     const int di = 1 - 2 * (lo[0] & 1), dj = 1 - 2 * (lo[1] & 1);
-    u[G::idx(lo[0], lo[1], 0)] =
-        u[G::idx(lo[0] + di, lo[1], 0)] + u[G::idx(lo[0], lo[1] + dj, 0)] - u[G::idx(lo[0] + di, lo[1] + dj, 0)];
+    u[L::at(lo[0], lo[1], 0)] =
+        u[L::at(lo[0] + di, lo[1], 0)] + u[L::at(lo[0], lo[1] + dj, 0)] - u[L::at(lo[0] + di, lo[1] + dj, 0)];
   } else {
     const int di = 1 - 2 * (lo[0] & 1), dj = 1 - 2 * (lo[1] & 1), dk = 1 - 2 * (lo[2] & 1);
     const int x = lo[0], y = lo[1], z = lo[2];
-    u[G::idx(x, y, z)] = u[G::idx(x, y + dj, z + dk)] + u[G::idx(x + di, y, z + dk)] +
-                         u[G::idx(x + di, y + dj, z)] - 2 * u[G::idx(x + di, y + dj, z + dk)];
+    u[L::at(x, y, z)] = u[L::at(x, y + dj, z + dk)] + u[L::at(x + di, y, z + dk)] +
+                         u[L::at(x + di, y + dj, z)] - 2 * u[L::at(x + di, y + dj, z + dk)];
   }
 }
 
@@ -275,12 +293,13 @@ template <int D, int NC, int BS, typename T, int BB = 1>
 __global__ void __launch_bounds__(BS * BB) k_gc(const int *__restrict__ ids, const DBox *__restrict__ boxes, T *var,
                                                 VarBc m, int corners, int bpb, int nw) {
   using G = Geo<D, NC>;
+  using L = Lay<D, NC, T>;
   for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
     const Part pt = part_of<BS>(ids, bpb, w);
     if (pt.id <= 0) continue;  // run masked out
     const int id = pt.id;
     const DBox &bx = boxes[id - 1];
-    T *u = var + (size_t)(id - 1) * G::NCELL;
+    T *u = var + (size_t)(id - 1) * L::STRIDE;
     constexpr int NFC = 2 * D * G::NTAN;
     for (int q = pt.q0; q < NFC; q += pt.stride) {
       const int nb = q / G::NTAN, r = q % G::NTAN;
@@ -288,12 +307,12 @@ __global__ void __launch_bounds__(BS * BB) k_gc(const int *__restrict__ ids, con
       const bool low = (nb % 2) == 0;
       int p[3];
       face_pos<D>(nb, a, b, low ? 1 : NC, p);
-      const T x1 = u[G::idx(p[0], p[1], p[2])];
+      const T x1 = u[L::at(p[0], p[1], p[2])];
       face_pos<D>(nb, a, b, low ? 2 : NC - 1, p);
-      const T x2 = u[G::idx(p[0], p[1], p[2])];
+      const T x2 = u[L::at(p[0], p[1], p[2])];
       const T g = ghost_value<D, NC, T>(boxes, bx, var, m, nb, a, b, x1, x2);
       face_pos<D>(nb, a, b, low ? 0 : NC + 1, p);
-      u[G::idx(p[0], p[1], p[2])] = g;
+      u[L::at(p[0], p[1], p[2])] = g;
     }
     if (!corners) continue;
     gsync<BS, BB>();
@@ -317,37 +336,36 @@ __global__ void __launch_bounds__(BS * BB) k_gsrb(const int *__restrict__ ids, c
                                                   const T *__restrict__ rhs, int redblack, T lambda2, int cyl, VarBc m,
                                                   int bpb, int nw) {
   using G = Geo<D, NC>;
+  using L = Lay<D, NC, T>;
   for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
     const Part pt = part_of<BS>(ids, bpb, w);
     if (pt.id <= 0) continue;  // run masked out
     const int id = pt.id;
     const DBox &bx = boxes[id - 1];
     const Sten<D, T> s = lpl_stencil<D, T>(bx, lambda2);
-    T *u = phi + (size_t)(id - 1) * G::NCELL;
-    const T *r = rhs + (size_t)(id - 1) * G::NCELL;
+    T *u = phi + (size_t)(id - 1) * L::STRIDE;
+    const T *r = rhs + (size_t)(id - 1) * L::STRIDE;
     constexpr int HALF = G::NINT / 2;
-    constexpr int st[3] = {1, G::NG, G::NG * G::NG};
     for (int q = pt.q0; q < HALF; q += pt.stride) {
       const int mm = q % (NC / 2);
       const int j = (q / (NC / 2)) % NC + 1;
       const int k = (D == 3) ? q / ((NC / 2) * NC) + 1 : 0;
       const int i0 = 2 - ((redblack ^ (D == 3 ? k + j : j)) & 1);
       const int i = i0 + 2 * mm;
-      const int x = G::idx(i, j, k);
+      const int x = L::in_at(i, j, k);
       const int pos[3] = {i, j, k};
-      const T uc = u[x];
       T nbv[6];
       for (int nb = 0; nb < 2 * D; nb++) {
         const int d = nb / 2;
         const bool low = (nb % 2) == 0;
         const bool edge = low ? pos[d] == 1 : pos[d] == NC;
         if (!edge) {
-          nbv[nb] = u[low ? x - st[d] : x + st[d]];
+          nbv[nb] = u[L::nb_at(i, j, k, d, low ? -1 : 1)];
         } else {
           int t0, t1;
           tangential(D, d, t0, t1);
-          const T x2 = u[low ? x + st[d] : x - st[d]];
-          nbv[nb] = ghost_value<D, NC, T>(boxes, bx, phi, m, nb, pos[t0], (D == 3) ? pos[t1] : 0, uc, x2);
+          const T x2 = u[L::nb_at(i, j, k, d, low ? 1 : -1)];
+          nbv[nb] = ghost_value<D, NC, T>(boxes, bx, phi, m, nb, pos[t0], (D == 3) ? pos[t1] : 0, u[x], x2);
         }
       }
       if (D == 2) {
@@ -379,6 +397,7 @@ __global__ void __launch_bounds__(BS * BB) k_residual(const int *__restrict__ id
                                                       T lambda2, int cyl, unsigned long long *leafmax, int bpb,
                                                       int nw) {
   using G = Geo<D, NC>;
+  using L = Lay<D, NC, T>;
   __shared__ double sh[BS / 64];
   for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
     const Part pt = part_of<BS>(ids, bpb, w);
@@ -386,13 +405,13 @@ __global__ void __launch_bounds__(BS * BB) k_residual(const int *__restrict__ id
     const int id = pt.id;
     const DBox &bx = boxes[id - 1];
     const Sten<D, T> s = lpl_stencil<D, T>(bx, lambda2);
-    const size_t base = (size_t)(id - 1) * G::NCELL;
+    const size_t base = (size_t)(id - 1) * L::STRIDE;
     double mx = 0;
     for (int q = pt.q0; q < G::NINT; q += pt.stride) {
       int i, j, k;
       interior_ijk<D, NC>(q, i, j, k);
-      const int x = G::idx(i, j, k);
-      T lp = apply_cell<D, NC, T>(phi + base, s, cyl != 0, bx, i, j, k);
+      const int x = L::in_at(i, j, k);
+      T lp = apply_cell<D, NC, T, L>(phi + base, s, cyl != 0, bx, i, j, k);
       const T rv = rhs[base + x] - lp;
       tmp[base + x] = rv;
       mx = fmax(mx, (double)fabs(rv));
@@ -404,33 +423,47 @@ __global__ void __launch_bounds__(BS * BB) k_residual(const int *__restrict__ id
   }
 }
 
+// Positions in layout L of the 2^D children cells of a parent cell:
+// child c is (i_f + (c & 1), jf + ((c >> 1) & 1), kf + ((c >> 2) & 1)),
+// with i_f, jf (and kf) odd.  (In the colour-split layout the first cell
+// of each pair of the x direction is at an even lexicographic index, so both
+// share one position within their colour blocks.)
+template <int D, int NC, typename L>
+__host__ __device__ inline void child_pos(int i_f, int jf, int kf, int *pos) {
+  constexpr int nch = (D == 3) ? 8 : 4;
+  if constexpr (L::SPLIT) {
+    const int h0 = ((i_f - 1) + NC * ((jf - 1) + NC * (D == 3 ? kf - 1 : 0))) >> 1;
+    const int col0 = (i_f + jf + kf) & 1;
+    for (int c = 0; c < nch; c++) {
+      const int di = c & 1, dj = (c >> 1) & 1, dk = (c >> 2) & 1;
+      pos[c] = ((col0 + di + dj + dk) & 1) * L::HALF + h0 + dj * (NC / 2) + dk * (NC * NC / 2);
+    }
+  } else {
+    for (int c = 0; c < nch; c++) pos[c] = L::in_at(i_f + (c & 1), jf + ((c >> 1) & 1), kf + ((c >> 2) & 1));
+  }
+}
+
 // Restriction of one parent cell from its 2^D children values
 // (af_restrict_box, afivo/src/m_af_restrict.f90:62-136).
-template <int D, int NC, typename T>
+template <int D, int NC, typename T, typename L = LayStd<D, NC>>
 __device__ inline T restrict_cell(const T *f, int i_f, int jf, int kf, bool cyl_geom, const DBox &bp, int ic) {
-  using G = Geo<D, NC>;
+  int p[8];
+  child_pos<D, NC, L>(i_f, jf, kf, p);
   if (D == 2) {
     if (cyl_geom) {
       T r = (T)bp.r_min[0] + (T(ic) - T(0.5)) * (T)bp.dr[0];
       T tmp = T(0.25) * (T)bp.dr[0] / r;
       T w1 = T(1) - tmp, w2 = T(1) + tmp;
-      return T(0.25) * (w1 * (f[G::idx(i_f, jf, 0)] + f[G::idx(i_f, jf + 1, 0)]) +
-                        w2 * (f[G::idx(i_f + 1, jf, 0)] + f[G::idx(i_f + 1, jf + 1, 0)]));
+      return T(0.25) * (w1 * (f[p[0]] + f[p[2]]) + w2 * (f[p[1]] + f[p[3]]));
     }
-    T s = f[G::idx(i_f, jf, 0)];
-    s = s + f[G::idx(i_f + 1, jf, 0)];
-    s = s + f[G::idx(i_f, jf + 1, 0)];
-    s = s + f[G::idx(i_f + 1, jf + 1, 0)];
+    T s = f[p[0]];
+    s = s + f[p[1]];
+    s = s + f[p[2]];
+    s = s + f[p[3]];
     return T(0.25) * s;
   } else {
-    T s = f[G::idx(i_f, jf, kf)];
-    s = s + f[G::idx(i_f + 1, jf, kf)];
-    s = s + f[G::idx(i_f, jf + 1, kf)];
-    s = s + f[G::idx(i_f + 1, jf + 1, kf)];
-    s = s + f[G::idx(i_f, jf, kf + 1)];
-    s = s + f[G::idx(i_f + 1, jf, kf + 1)];
-    s = s + f[G::idx(i_f, jf + 1, kf + 1)];
-    s = s + f[G::idx(i_f + 1, jf + 1, kf + 1)];
+    T s = f[p[0]];
+    for (int c = 1; c < 8; c++) s = s + f[p[c]];
     return T(0.125) * s;
   }
 }
@@ -440,6 +473,7 @@ template <int D, int NC, int BS, typename T, int BB = 1>
 __global__ void __launch_bounds__(BS * BB) k_restrict(const int *__restrict__ ids, const DBox *__restrict__ boxes,
                                                       const T *src, T *dst, int use_geom, int cyl, int bpb, int nw) {
   using G = Geo<D, NC>;
+  using L = Lay<D, NC, T>;
   constexpr int H = NC / 2;
   constexpr int NQ = (D == 3) ? H * H * H : H * H;
   for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
@@ -450,13 +484,13 @@ __global__ void __launch_bounds__(BS * BB) k_restrict(const int *__restrict__ id
     const DBox &bp = boxes[bc.parent - 1];
     int off[3] = {0, 0, 0};
     for (int d = 0; d < D; d++) off[d] = ((bc.ix[d] - 1) & 1) * H;
-    const T *f = src + (size_t)(id - 1) * G::NCELL;
-    T *u = dst + (size_t)(bc.parent - 1) * G::NCELL;
+    const T *f = src + (size_t)(id - 1) * L::STRIDE;
+    T *u = dst + (size_t)(bc.parent - 1) * L::STRIDE;
     for (int q = pt.q0; q < NQ; q += pt.stride) {
       const int i = q % H + 1, j = (q / H) % H + 1, k = (D == 3) ? q / (H * H) + 1 : 0;
       const int ic = off[0] + i, jc = off[1] + j, kc = (D == 3) ? off[2] + k : 0;
-      u[G::idx(ic, jc, kc)] =
-          restrict_cell<D, NC, T>(f, 2 * i - 1, 2 * j - 1, (D == 3) ? 2 * k - 1 : 0, cyl && use_geom, bp, ic);
+      u[L::at(ic, jc, kc)] =
+          restrict_cell<D, NC, T, L>(f, 2 * i - 1, 2 * j - 1, (D == 3) ? 2 * k - 1 : 0, cyl && use_geom, bp, ic);
     }
   }
 }
@@ -472,7 +506,7 @@ __global__ void __launch_bounds__(BS * BB) k_restrict_residual(const int *__rest
                                                                const DBox *__restrict__ boxes, T *phi,
                                                                const T *__restrict__ rhs, T *tmp, T lambda2, int cyl,
                                                                int store_residual, int bpb, int nw) {
-  using G = Geo<D, NC>;
+  using L = Lay<D, NC, T>;
   constexpr int H = NC / 2;
   constexpr int NQ = (D == 3) ? H * H * H : H * H;
   for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
@@ -482,7 +516,7 @@ __global__ void __launch_bounds__(BS * BB) k_restrict_residual(const int *__rest
     const DBox &bc = boxes[id - 1];
     const DBox &bp = boxes[bc.parent - 1];
     const Sten<D, T> s = lpl_stencil<D, T>(bc, lambda2);
-    const size_t cb = (size_t)(id - 1) * G::NCELL, pb = (size_t)(bc.parent - 1) * G::NCELL;
+    const size_t cb = (size_t)(id - 1) * L::STRIDE, pb = (size_t)(bc.parent - 1) * L::STRIDE;
     int off[3] = {0, 0, 0};
     for (int d = 0; d < D; d++) off[d] = ((bc.ix[d] - 1) & 1) * H;
     for (int q = pt.q0; q < NQ; q += pt.stride) {
@@ -491,12 +525,14 @@ __global__ void __launch_bounds__(BS * BB) k_restrict_residual(const int *__rest
       // residuals of the 2^D children cells, in the summation order of
       // af_restrict_box (i fastest, then j, then k)
       T rs[8];
+      int cp[8];
+      child_pos<D, NC, L>(i_f, jf, kf, cp);
       const int nch = (D == 3) ? 8 : 4;
       for (int c = 0; c < nch; c++) {
         const int ii = i_f + (c & 1), jj = jf + ((c >> 1) & 1), kk = kf + ((c >> 2) & 1);
-        T lp = apply_cell<D, NC, T>(phi + cb, s, cyl != 0, bc, ii, jj, kk);
-        rs[c] = rhs[cb + G::idx(ii, jj, kk)] - lp;
-        if (store_residual) tmp[cb + G::idx(ii, jj, kk)] = rs[c];
+        T lp = apply_cell<D, NC, T, L>(phi + cb, s, cyl != 0, bc, ii, jj, kk);
+        rs[c] = rhs[cb + cp[c]] - lp;
+        if (store_residual) tmp[cb + cp[c]] = rs[c];
       }
       const int ic = off[0] + i, jc = off[1] + j, kc = (D == 3) ? off[2] + k : 0;
       T rv;
@@ -518,8 +554,8 @@ __global__ void __launch_bounds__(BS * BB) k_restrict_residual(const int *__rest
         for (int c = 1; c < 8; c++) sm = sm + rs[c];
         rv = T(0.125) * sm;
       }
-      tmp[pb + G::idx(ic, jc, kc)] = rv;
-      phi[pb + G::idx(ic, jc, kc)] = restrict_cell<D, NC, T>(phi + cb, i_f, jf, kf, false, bp, ic);
+      tmp[pb + L::in_at(ic, jc, kc)] = rv;
+      phi[pb + L::in_at(ic, jc, kc)] = restrict_cell<D, NC, T, L>(phi + cb, i_f, jf, kf, false, bp, ic);
     }
   }
 }
@@ -531,17 +567,25 @@ __global__ void __launch_bounds__(BS * BB) k_coarse_rhs(const int *__restrict__ 
                                                         const T *__restrict__ phi, T *rhs, T *tmp, T lambda2, int cyl,
                                                         int copy_phi, int bpb, int nw) {
   using G = Geo<D, NC>;
+  using L = Lay<D, NC, T>;
   for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
     const Part pt = part_of<BS>(ids, bpb, w);
     if (pt.id <= 0) continue;  // run masked out
     const int id = pt.id;
     const DBox &bx = boxes[id - 1];
     const Sten<D, T> s = lpl_stencil<D, T>(bx, lambda2);
-    const size_t base = (size_t)(id - 1) * G::NCELL;
+    const size_t base = (size_t)(id - 1) * L::STRIDE;
     for (int x = pt.q0; x < G::NCELL; x += pt.stride) {
-      const int i = x % G::NG, j = (x / G::NG) % G::NG, k = (D == 3) ? x / (G::NG * G::NG) : 0;
-      const bool inner = i >= 1 && i <= NC && j >= 1 && j <= NC && (D == 2 || (k >= 1 && k <= NC));
-      T rv = inner ? apply_cell<D, NC, T>(phi + base, s, cyl != 0, bx, i, j, k) : rhs[base + x];
+      int i, j, k;
+      bool inner;
+      if constexpr (L::SPLIT) {
+        inner = x < G::NINT;
+        if (inner) colour_cell<D, NC>(x >= L::HALF, x >= L::HALF ? x - L::HALF : x, i, j, k);
+      } else {
+        i = x % G::NG, j = (x / G::NG) % G::NG, k = (D == 3) ? x / (G::NG * G::NG) : 0;
+        inner = i >= 1 && i <= NC && j >= 1 && j <= NC && (D == 2 || (k >= 1 && k <= NC));
+      }
+      T rv = inner ? apply_cell<D, NC, T, L>(phi + base, s, cyl != 0, bx, i, j, k) : rhs[base + x];
       rhs[base + x] = rv + tmp[base + x];
       if (copy_phi) tmp[base + x] = phi[base + x];
     }
@@ -576,14 +620,15 @@ template <int D, int NC, int BS, typename T, int BB = 1>
 __global__ void __launch_bounds__(BS * BB) k_prolong_add(const int *__restrict__ ids, const DBox *__restrict__ boxes,
                                                          const T *__restrict__ src, T *dst, int bpb, int nw) {
   using G = Geo<D, NC>;
+  using L = Lay<D, NC, T>;
   constexpr int H = NC / 2;
   for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
     const Part pt = part_of<BS>(ids, bpb, w);
     if (pt.id <= 0) continue;  // run masked out
     const int id = pt.id;
     const DBox &bc = boxes[id - 1];
-    const T *P = src + (size_t)(bc.parent - 1) * G::NCELL;
-    T *u = dst + (size_t)(id - 1) * G::NCELL;
+    const T *P = src + (size_t)(bc.parent - 1) * L::STRIDE;
+    T *u = dst + (size_t)(id - 1) * L::STRIDE;
     int off[3] = {0, 0, 0};
     for (int d = 0; d < D; d++) off[d] = ((bc.ix[d] - 1) & 1) * H;
     for (int q = pt.q0; q < G::NINT; q += pt.stride) {
@@ -591,19 +636,68 @@ __global__ void __launch_bounds__(BS * BB) k_prolong_add(const int *__restrict__
       interior_ijk<D, NC>(q, i, j, k);
       const int i1 = off[0] + ((i + 1) >> 1), i2 = i1 + 1 - 2 * (i & 1);
       const int j1 = off[1] + ((j + 1) >> 1), j2 = j1 + 1 - 2 * (j & 1);
-      const int x = G::idx(i, j, k);
+      const int x = L::at(i, j, k);
       if (D == 2) {
         const T w0 = T(9 / 16.0), w1 = T(3 / 16.0), w2 = T(3 / 16.0), w3 = T(1 / 16.0);
-        u[x] = u[x] + w0 * P[G::idx(i1, j1, 0)] + w1 * P[G::idx(i2, j1, 0)] + w2 * P[G::idx(i1, j2, 0)] +
-               w3 * P[G::idx(i2, j2, 0)];
+        u[x] = u[x] + w0 * P[L::at(i1, j1, 0)] + w1 * P[L::at(i2, j1, 0)] + w2 * P[L::at(i1, j2, 0)] +
+               w3 * P[L::at(i2, j2, 0)];
       } else {
         const int k1 = off[2] + ((k + 1) >> 1), k2 = k1 + 1 - 2 * (k & 1);
         const T w0 = T(27 / 64.0), w1 = T(9 / 64.0), w3 = T(3 / 64.0), w7 = T(1 / 64.0);
-        u[x] = u[x] + w0 * P[G::idx(i1, j1, k1)] + w1 * P[G::idx(i2, j1, k1)] + w1 * P[G::idx(i1, j2, k1)] +
-               w3 * P[G::idx(i2, j2, k1)] + w1 * P[G::idx(i1, j1, k2)] + w3 * P[G::idx(i2, j1, k2)] +
-               w3 * P[G::idx(i1, j2, k2)] + w7 * P[G::idx(i2, j2, k2)];
+        u[x] = u[x] + w0 * P[L::at(i1, j1, k1)] + w1 * P[L::at(i2, j1, k1)] + w1 * P[L::at(i1, j2, k1)] +
+               w3 * P[L::at(i2, j2, k1)] + w1 * P[L::at(i1, j1, k2)] + w3 * P[L::at(i2, j1, k2)] +
+               w3 * P[L::at(i1, j2, k2)] + w7 * P[L::at(i2, j2, k2)];
       }
     }
+  }
+}
+
+// k_prolong_add for pools in the colour-split layout: the parent cells the
+// box's cells interpolate from (its octant and one cell around it) are
+// staged in LDS once, then every cell is formed from LDS, colour block by
+// colour block (same weights and order as k_prolong_add).
+template <int D, int NC, int BS, int BB = 1>
+__global__ void __launch_bounds__(BS * BB) k_prolong_add_split(const int *__restrict__ ids,
+                                                               const DBox *__restrict__ boxes,
+                                                               const float *__restrict__ src, float *dst, int bpb,
+                                                               int nw) {
+  using G = Geo<D, NC>;
+  using L = Lay<D, NC, float>;
+  constexpr int H = NC / 2, P = H + 2;
+  constexpr int NP = (D == 3) ? P * P * P : P * P;
+  __shared__ float s_all[BB][NP];
+  float *sp = s_all[lgrp<BS, BB>()];
+  for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
+    const Part pt = part_of<BS>(ids, bpb, w);
+    if (pt.id <= 0) continue;  // run masked out
+    const DBox &bc = boxes[pt.id - 1];
+    const float *Pb = src + (size_t)(bc.parent - 1) * L::STRIDE;
+    float *u = dst + (size_t)(pt.id - 1) * L::STRIDE;
+    int off[3] = {0, 0, 0};
+    for (int d = 0; d < D; d++) off[d] = ((bc.ix[d] - 1) & 1) * H;
+    // parent cells off[d] .. off[d] + H + 1 along each axis
+    for (int q = ltid<BS>(); q < NP; q += BS) {
+      const int a = q % P, b = (q / P) % P, c = (D == 3) ? q / (P * P) : 0;
+      sp[q] = Pb[L::at(off[0] + a, off[1] + b, (D == 3) ? off[2] + c : 0)];
+    }
+    gsync<BS, BB>();
+    auto S = [&](int a, int b, int c) { return sp[a + P * (b + P * c)]; };
+    for (int x = pt.q0; x < G::NINT; x += pt.stride) {
+      int i, j, k;
+      colour_cell<D, NC>(x >= L::HALF, x >= L::HALF ? x - L::HALF : x, i, j, k);
+      const int i1 = (i + 1) >> 1, i2 = i1 + 1 - 2 * (i & 1);
+      const int j1 = (j + 1) >> 1, j2 = j1 + 1 - 2 * (j & 1);
+      if (D == 2) {
+        const float w0 = 9 / 16.0f, w1 = 3 / 16.0f, w2 = 3 / 16.0f, w3 = 1 / 16.0f;
+        u[x] = u[x] + w0 * S(i1, j1, 0) + w1 * S(i2, j1, 0) + w2 * S(i1, j2, 0) + w3 * S(i2, j2, 0);
+      } else {
+        const int k1 = (k + 1) >> 1, k2 = k1 + 1 - 2 * (k & 1);
+        const float w0 = 27 / 64.0f, w1 = 9 / 64.0f, w3 = 3 / 64.0f, w7 = 1 / 64.0f;
+        u[x] = u[x] + w0 * S(i1, j1, k1) + w1 * S(i2, j1, k1) + w1 * S(i1, j2, k1) + w3 * S(i2, j2, k1) +
+               w1 * S(i1, j1, k2) + w3 * S(i2, j1, k2) + w3 * S(i1, j2, k2) + w7 * S(i2, j2, k2);
+      }
+    }
+    gsync<BS, BB>();  // (the group's LDS is reused by its next item)
   }
 }
 
@@ -645,6 +739,7 @@ __global__ void __launch_bounds__(CBS) k_coarse_solve(const int *__restrict__ id
                                                       const double *__restrict__ v,
                                                       const double *__restrict__ eig) {
   using G = Geo<D, NC>;
+  using L = Lay<D, NC, TS>;
   constexpr int N = G::NINT;
   constexpr int PER = (N + CBS - 1) / CBS;
   __shared__ double buf[N];
@@ -652,12 +747,12 @@ __global__ void __launch_bounds__(CBS) k_coarse_solve(const int *__restrict__ id
   if (id <= 0) return;  // run masked out
   const DBox &bx = boxes[id - 1];
   const Sten<D, double> s = lpl_stencil<D, double>(bx, lambda2);
-  const TS *r = rhs + (size_t)(id - 1) * G::NCELL;
+  const TS *r = rhs + (size_t)(id - 1) * L::STRIDE;
   auto lin = [](int i, int j, int k) { return (i - 1) + NC * ((j - 1) + NC * (D == 3 ? k - 1 : 0)); };
   for (int q = threadIdx.x; q < N; q += CBS) {
     int i, j, k;
     interior_ijk<D, NC>(q, i, j, k);
-    double bval = (double)r[G::idx(i, j, k)];
+    double bval = (double)r[L::in_at(i, j, k)];
     int p[3] = {i, j, (D == 3) ? k : 1};
     for (int nb = 0; nb < 2 * D; nb++) {
       if (bx.nb[nb] >= 0) continue;
@@ -719,11 +814,11 @@ __global__ void __launch_bounds__(CBS) k_coarse_solve(const int *__restrict__ id
     }
     __syncthreads();
   }
-  TS *u = phi + (size_t)(id - 1) * G::NCELL;
+  TS *u = phi + (size_t)(id - 1) * L::STRIDE;
   for (int q = threadIdx.x; q < N; q += CBS) {
     int i, j, k;
     interior_ijk<D, NC>(q, i, j, k);
-    u[G::idx(i, j, k)] = (TS)buf[q];
+    u[L::in_at(i, j, k)] = (TS)buf[q];
   }
 }
 
@@ -731,22 +826,6 @@ __global__ void __launch_bounds__(CBS) k_coarse_solve(const int *__restrict__ id
 // The inner multigrid cycle works on the correction e (single precision,
 // homogeneous boundary conditions) of the FP64 potential; these two kernels
 // close each cycle in FP64.
-
-// phi += e on whole boxes, then e = 0 so the next cycle starts from a zero
-// correction.
-template <int NCELL, int BS, int BB = 1>
-__global__ void __launch_bounds__(BS * BB) k_mp_correct(const int *__restrict__ ids, double *phi, float *e, int bpb,
-                                                        int nw) {
-  for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
-    const Part pt = part_of<BS>(ids, bpb, w);
-    if (pt.id <= 0) continue;  // run masked out
-    const size_t base = (size_t)(pt.id - 1) * NCELL;
-    for (int x = pt.q0; x < NCELL; x += pt.stride) {
-      phi[base + x] = phi[base + x] + (double)e[base + x];
-      e[base + x] = 0.0f;
-    }
-  }
-}
 
 // FP64 residual rhs - L phi on the interior cells of a box list, with every
 // neighbour outside the box taken from ghost_value() (so the stored ghost
@@ -759,6 +838,7 @@ __global__ void __launch_bounds__(BS * BB) k_mp_residual(const int *__restrict__
                                                          float *r32, double lambda2, int cyl, VarBc m,
                                                          unsigned long long *leafmax, int bpb, int nw) {
   using G = Geo<D, NC>;
+  using L = Lay<D, NC, float>;
   __shared__ double sh[BS / 64];
   for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
     const Part pt = part_of<BS>(ids, bpb, w);
@@ -768,6 +848,7 @@ __global__ void __launch_bounds__(BS * BB) k_mp_residual(const int *__restrict__
     const Sten<D, double> s = lpl_stencil<D, double>(bx, lambda2);
     const size_t base = (size_t)(id - 1) * G::NCELL;
     const double *u = phi + base;
+    float *r = r32 + (size_t)(id - 1) * L::STRIDE;
     constexpr int st[3] = {1, G::NG, G::NG * G::NG};
     double mx = 0;
     for (int q = pt.q0; q < G::NINT; q += pt.stride) {
@@ -804,7 +885,7 @@ __global__ void __launch_bounds__(BS * BB) k_mp_residual(const int *__restrict__
              s.c[5] * nbv[4] + s.c[6] * nbv[5];
       }
       const double rv = rhs[base + x] - lp;
-      r32[base + x] = (float)rv;
+      r[L::in_at(i, j, k)] = (float)rv;
       mx = fmax(mx, fabs(rv));
     }
     if (bx.children[0] == 0) {
@@ -824,35 +905,11 @@ __global__ void __launch_bounds__(BS * BB) k_mp_residual(const int *__restrict__
 template <int D, int NC, int BS, typename T, int BB = 1>
 __device__ inline void lds_load_interior(T *s, const T *u) {
   using G = Geo<D, NC>;
+  using L = Lay<D, NC, T>;
   for (int q = ltid<BS>(); q < G::NINT; q += BS) {
     int i, j, k;
     interior_ijk<D, NC>(q, i, j, k);
-    const int x = G::idx(i, j, k);
-    s[x] = u[x];
-  }
-}
-
-// Face ghosts into LDS.  colour >= 0: only the ghosts next to interior cells
-// of that GSRB colour ((i + j + k + colour) even), the only ones its
-// half-sweep reads; colour < 0: all face ghosts.
-template <int D, int NC, int BS, typename T, int BB = 1>
-__device__ inline void lds_face_ghosts(T *s, const DBox *__restrict__ boxes, const DBox &bx, const T *var,
-                                       const VarBc &m, int colour, const T *coarse = nullptr) {
-  using G = Geo<D, NC>;
-  constexpr int NFC = 2 * D * G::NTAN;
-  for (int q = ltid<BS>(); q < NFC; q += BS) {
-    const int nb = q / G::NTAN, r = q % G::NTAN;
-    const int a = r % NC + 1, b = (D == 3) ? r / NC + 1 : 0;
-    const bool low = (nb % 2) == 0;
-    int p[3];
-    face_pos<D>(nb, a, b, low ? 1 : NC, p);
-    if (colour >= 0 && ((p[0] + p[1] + p[2] + colour) & 1)) continue;
-    const T x1 = s[G::idx(p[0], p[1], p[2])];
-    face_pos<D>(nb, a, b, low ? 2 : NC - 1, p);
-    const T x2 = s[G::idx(p[0], p[1], p[2])];
-    const T g = ghost_value<D, NC, T>(boxes, bx, var, m, nb, a, b, x1, x2, coarse);
-    face_pos<D>(nb, a, b, low ? 0 : NC + 1, p);
-    s[G::idx(p[0], p[1], p[2])] = g;
+    s[G::idx(i, j, k)] = u[L::in_at(i, j, k)];
   }
 }
 
@@ -860,24 +917,42 @@ __device__ inline void lds_face_ghosts(T *s, const DBox *__restrict__ boxes, con
 // interior and the same-level neighbour copies are loaded together (into
 // registers, then LDS); after a barrier, the ghosts that depend on the box's
 // own cells (physical boundary, refinement boundary) are formed from LDS.
-// colour >= 0: only the ghosts next to interior cells of that GSRB colour.
+// colour >= 0: only the ghosts next to interior cells of that GSRB colour,
+// and in the colour-split FP32 layout only the interior cells of the other
+// colour, the only ones a half-sweep of that colour reads (the few cells of
+// the updated colour that those ghosts depend on are read from the pool).
 // Ends with a barrier.
 template <int D, int NC, int BS, typename T, int BB = 1>
 __device__ inline void lds_stage(T *s, const DBox *__restrict__ boxes, const DBox &bx, int id, const T *var,
                                  const VarBc &m, int colour, const T *coarse = nullptr) {
   using G = Geo<D, NC>;
+  using L = Lay<D, NC, T>;
   constexpr int NI = (G::NINT + BS - 1) / BS;
   constexpr int NFC = 2 * D * G::NTAN;
   constexpr int NF = (NFC + BS - 1) / BS;
-  const T *u = var + (size_t)(id - 1) * G::NCELL;
+  const T *u = var + (size_t)(id - 1) * L::STRIDE;
+  const bool one = L::SPLIT && colour >= 0;
+  const int n_load = one ? G::NINT / 2 : G::NINT;
+  // interior cell (i, j, k) of load q and its position in the pool: in the
+  // split layout the loads run over the colour blocks in order
+  auto cell = [&](int q, int &i, int &j, int &k) {
+    if constexpr (L::SPLIT) {
+      constexpr int HALF = G::NINT / 2;
+      const int x = q + (one ? (1 - colour) * HALF : 0);
+      colour_cell<D, NC>(x >= HALF, x >= HALF ? x - HALF : x, i, j, k);
+      return x;
+    } else {
+      interior_ijk<D, NC>(q, i, j, k);
+      return L::in_at(i, j, k);
+    }
+  };
   T vi[NI], vf[NF];
   int gi[NF];
   for (int t = 0; t < NI; t++) {
     const int q = ltid<BS>() + t * BS;
-    if (q < G::NINT) {
+    if (q < n_load) {
       int i, j, k;
-      interior_ijk<D, NC>(q, i, j, k);
-      vi[t] = u[G::idx(i, j, k)];
+      vi[t] = u[cell(q, i, j, k)];
     }
   }
   for (int t = 0; t < NF; t++) {
@@ -896,13 +971,13 @@ __device__ inline void lds_stage(T *s, const DBox *__restrict__ boxes, const DBo
     p[d] = low ? 0 : NC + 1;
     gi[t] = G::idx(p[0], p[1], p[2]);
     p[d] = low ? NC : 1;  // the neighbour's cell across the face
-    vf[t] = var[(size_t)(nb_id - 1) * G::NCELL + G::idx(p[0], p[1], p[2])];
+    vf[t] = var[(size_t)(nb_id - 1) * L::STRIDE + L::in_at(p[0], p[1], p[2])];
   }
   for (int t = 0; t < NI; t++) {
     const int q = ltid<BS>() + t * BS;
-    if (q < G::NINT) {
+    if (q < n_load) {
       int i, j, k;
-      interior_ijk<D, NC>(q, i, j, k);
+      cell(q, i, j, k);
       s[G::idx(i, j, k)] = vi[t];
     }
   }
@@ -920,7 +995,7 @@ __device__ inline void lds_stage(T *s, const DBox *__restrict__ boxes, const DBo
       int p[3];
       face_pos<D>(nb, a, b, low ? 1 : NC, p);
       if (colour >= 0 && ((p[0] + p[1] + p[2] + colour) & 1)) continue;
-      const T x1 = s[G::idx(p[0], p[1], p[2])];
+      const T x1 = one ? u[L::in_at(p[0], p[1], p[2])] : s[G::idx(p[0], p[1], p[2])];
       face_pos<D>(nb, a, b, low ? 2 : NC - 1, p);
       const T x2 = s[G::idx(p[0], p[1], p[2])];
       const T g = ghost_value<D, NC, T>(boxes, bx, var, m, nb, a, b, x1, x2, coarse);
@@ -937,6 +1012,7 @@ __global__ void __launch_bounds__(BS * BB) k_gsrb_lds(const int *__restrict__ id
                                                       T *phi, const T *__restrict__ rhs, int redblack, T lambda2,
                                                       int cyl, VarBc m, int nw) {
   using G = Geo<D, NC>;
+  using L = Lay<D, NC, T>;
   __shared__ T s_all[BB][G::NCELL];
   T *s = s_all[lgrp<BS, BB>()];
   for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
@@ -944,8 +1020,8 @@ __global__ void __launch_bounds__(BS * BB) k_gsrb_lds(const int *__restrict__ id
     if (id <= 0) continue;  // run masked out
     const DBox &bx = boxes[id - 1];
     const Sten<D, T> st = lpl_stencil<D, T>(bx, lambda2);
-    T *u = phi + (size_t)(id - 1) * G::NCELL;
-    const T *r = rhs + (size_t)(id - 1) * G::NCELL;
+    T *u = phi + (size_t)(id - 1) * L::STRIDE;
+    const T *r = rhs + (size_t)(id - 1) * L::STRIDE;
     constexpr int HALF = G::NINT / 2;
     constexpr int NH = (HALF + BS - 1) / BS;
     constexpr int sx = 1, sy = G::NG, sz = G::NG * G::NG;
@@ -963,7 +1039,7 @@ __global__ void __launch_bounds__(BS * BB) k_gsrb_lds(const int *__restrict__ id
       if (q < HALF) {
         int i, j, k;
         cell(q, i, j, k);
-        rr[t] = r[G::idx(i, j, k)];
+        rr[t] = r[L::SPLIT ? (redblack & 1) * HALF + q : L::in_at(i, j, k)];
       }
     }
     // the updated cells satisfy (i + j + k + redblack) even
@@ -993,16 +1069,25 @@ __global__ void __launch_bounds__(BS * BB) k_gsrb_lds(const int *__restrict__ id
                inv_c1;
       }
     }
-    // Whole interior rows back (the other colour unchanged): full-line
-    // stores instead of every other word, so no line has to be fetched to
-    // merge a partial write.  Neighbouring boxes only read this box's other
-    // colour during the launch, which keeps its value.
     gsync<BS, BB>();
-    for (int q = ltid<BS>(); q < G::NINT; q += BS) {
-      int i, j, k;
-      interior_ijk<D, NC>(q, i, j, k);
-      const int x = G::idx(i, j, k);
-      u[x] = s[x];
+    if (L::SPLIT) {
+      // the updated colour's block
+      for (int q = ltid<BS>(); q < HALF; q += BS) {
+        int i, j, k;
+        cell(q, i, j, k);
+        u[(redblack & 1) * HALF + q] = s[G::idx(i, j, k)];
+      }
+    } else {
+      // Whole interior rows back (the other colour unchanged): full-line
+      // stores instead of every other word, so no line has to be fetched to
+      // merge a partial write.  Neighbouring boxes only read this box's
+      // other colour during the launch, which keeps its value.
+      for (int q = ltid<BS>(); q < G::NINT; q += BS) {
+        int i, j, k;
+        interior_ijk<D, NC>(q, i, j, k);
+        const int x = G::idx(i, j, k);
+        u[x] = s[x];
+      }
     }
     gsync<BS, BB>();  // (the group's LDS is reused by its next item)
   }
@@ -1042,10 +1127,9 @@ __global__ void __launch_bounds__(BS * BB) k_mp_residual_lds(const int *__restri
       if (q >= G::NINT) break;
       int i, j, k;
       interior_ijk<D, NC>(q, i, j, k);
-      const int x = G::idx(i, j, k);
       const double lp = apply_cell<D, NC, double>(s, st, cyl != 0, bx, i, j, k);
       const double rv = rr[t] - lp;
-      r32[base + x] = (float)rv;
+      r32[(size_t)(id - 1) * Lay<D, NC, float>::STRIDE + Lay<D, NC, float>::in_at(i, j, k)] = (float)rv;
       mx = fmax(mx, fabs(rv));
     }
     if (bx.children[0] == 0) {
@@ -1064,6 +1148,7 @@ __global__ void __launch_bounds__(BS * BB) k_restrict_residual_lds(const int *__
                                                                    T *phi, const T *__restrict__ rhs, T *tmp, T lambda2,
                                                                    int cyl, int store_residual, VarBc m, int nw) {
   using G = Geo<D, NC>;
+  using L = Lay<D, NC, T>;
   constexpr int H = NC / 2;
   constexpr int NQ = (D == 3) ? H * H * H : H * H;
   __shared__ T s_all[BB][G::NCELL];
@@ -1074,7 +1159,7 @@ __global__ void __launch_bounds__(BS * BB) k_restrict_residual_lds(const int *__
     const DBox &bc = boxes[id - 1];
     const DBox &bp = boxes[bc.parent - 1];
     const Sten<D, T> st = lpl_stencil<D, T>(bc, lambda2);
-    const size_t cb = (size_t)(id - 1) * G::NCELL, pb = (size_t)(bc.parent - 1) * G::NCELL;
+    const size_t cb = (size_t)(id - 1) * L::STRIDE, pb = (size_t)(bc.parent - 1) * L::STRIDE;
     lds_stage<D, NC, BS, T, BB>(s, boxes, bc, id, child, m, -1, (const T *)phi);
     int off[3] = {0, 0, 0};
     for (int d = 0; d < D; d++) off[d] = ((bc.ix[d] - 1) & 1) * H;
@@ -1082,12 +1167,14 @@ __global__ void __launch_bounds__(BS * BB) k_restrict_residual_lds(const int *__
       const int i = q % H + 1, j = (q / H) % H + 1, k = (D == 3) ? q / (H * H) + 1 : 0;
       const int i_f = 2 * i - 1, jf = 2 * j - 1, kf = (D == 3) ? 2 * k - 1 : 0;
       T rs[8];
+      int cp[8];
+      child_pos<D, NC, L>(i_f, jf, kf, cp);
       const int nch = (D == 3) ? 8 : 4;
       for (int c = 0; c < nch; c++) {
         const int ii = i_f + (c & 1), jj = jf + ((c >> 1) & 1), kk = kf + ((c >> 2) & 1);
         T lp = apply_cell<D, NC, T>(s, st, cyl != 0, bc, ii, jj, kk);
-        rs[c] = rhs[cb + G::idx(ii, jj, kk)] - lp;
-        if (store_residual) tmp[cb + G::idx(ii, jj, kk)] = rs[c];
+        rs[c] = rhs[cb + cp[c]] - lp;
+        if (store_residual) tmp[cb + cp[c]] = rs[c];
       }
       const int ic = off[0] + i, jc = off[1] + j, kc = (D == 3) ? off[2] + k : 0;
       T rv;
@@ -1109,8 +1196,8 @@ __global__ void __launch_bounds__(BS * BB) k_restrict_residual_lds(const int *__
         for (int c = 1; c < 8; c++) sm = sm + rs[c];
         rv = T(0.125) * sm;
       }
-      tmp[pb + G::idx(ic, jc, kc)] = rv;
-      phi[pb + G::idx(ic, jc, kc)] = restrict_cell<D, NC, T>(s, i_f, jf, kf, false, bp, ic);
+      tmp[pb + L::in_at(ic, jc, kc)] = rv;
+      phi[pb + L::in_at(ic, jc, kc)] = restrict_cell<D, NC, T>(s, i_f, jf, kf, false, bp, ic);
     }
     gsync<BS, BB>();  // (the group's LDS is reused by its next item)
   }
@@ -1129,12 +1216,13 @@ __global__ void __launch_bounds__(BS) k_smooth_block(const int *__restrict__ ids
                                                      const T *in, T *out, const T *__restrict__ rhs,
                                                      const T *coarse, int n_half, T lambda2, int cyl, VarBc m) {
   using G = Geo<D, NC>;
+  using L = Lay<D, NC, T>;
   __shared__ T s[G::NCELL];
   const int id = ids[blockIdx.x];
   if (id <= 0) return;  // run masked out
   const DBox &bx = boxes[id - 1];
   const Sten<D, T> st = lpl_stencil<D, T>(bx, lambda2);
-  const size_t base = (size_t)(id - 1) * G::NCELL;
+  const size_t base = (size_t)(id - 1) * L::STRIDE;
   const T *r = rhs + base;
   lds_load_interior<D, NC, BS, T>(s, in + base);
   constexpr int NFC = 2 * D * G::NTAN;
@@ -1178,18 +1266,19 @@ __global__ void __launch_bounds__(BS) k_smooth_block(const int *__restrict__ ids
       const int i0 = 2 - ((n ^ (D == 3 ? k + j : j)) & 1);
       const int i = i0 + 2 * mm;
       const int x = G::idx(i, j, k);
+      const T rx = r[L::in_at(i, j, k)];
       T v;
       if (D == 2) {
         if (cyl) {
           T c1, c2, c3;
           cyl_coeffs<T>(bx, i, st.c, c1, c2, c3);
-          v = (r[x] - c2 * s[x - sx] - c3 * s[x + sx] - st.c[3] * s[x - sy] - st.c[4] * s[x + sy]) / c1;
+          v = (rx - c2 * s[x - sx] - c3 * s[x + sx] - st.c[3] * s[x - sy] - st.c[4] * s[x + sy]) / c1;
         } else {
-          v = (r[x] - st.c[1] * s[x - sx] - st.c[2] * s[x + sx] - st.c[3] * s[x - sy] - st.c[4] * s[x + sy]) /
+          v = (rx - st.c[1] * s[x - sx] - st.c[2] * s[x + sx] - st.c[3] * s[x - sy] - st.c[4] * s[x + sy]) /
               st.c[0];
         }
       } else {
-        v = (r[x] - st.c[1] * s[x - sx] - st.c[2] * s[x + sx] - st.c[3] * s[x - sy] - st.c[4] * s[x + sy] -
+        v = (rx - st.c[1] * s[x - sx] - st.c[2] * s[x + sx] - st.c[3] * s[x - sy] - st.c[4] * s[x + sy] -
              st.c[5] * s[x - sz] - st.c[6] * s[x + sz]) /
             st.c[0];
       }
@@ -1200,8 +1289,7 @@ __global__ void __launch_bounds__(BS) k_smooth_block(const int *__restrict__ ids
   for (int q = threadIdx.x; q < G::NINT; q += BS) {
     int i, j, k;
     interior_ijk<D, NC>(q, i, j, k);
-    const int x = G::idx(i, j, k);
-    out[base + x] = s[x];
+    out[base + L::in_at(i, j, k)] = s[G::idx(i, j, k)];
   }
 }
 
@@ -1214,16 +1302,25 @@ template <int D, int NC, int BS, int BB = 1>
 __global__ void __launch_bounds__(BS * BB) k_mp_correct_leaves(const int *__restrict__ ids, double *phi, float *e,
                                                                int nw) {
   using G = Geo<D, NC>;
+  using L = Lay<D, NC, float>;
   for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
     const int id = ids[w];
     if (id <= 0) continue;  // run masked out
-    const size_t base = (size_t)(id - 1) * G::NCELL;
-    for (int x = ltid<BS>(); x < G::NCELL; x += BS) {
-      const int i = x % G::NG, j = (x / G::NG) % G::NG, k = (D == 3) ? x / (G::NG * G::NG) : 0;
-      const bool inner = i >= 1 && i <= NC && j >= 1 && j <= NC && (D == 2 || (k >= 1 && k <= NC));
-      if (inner) phi[base + x] = phi[base + x] + (double)e[base + x];
-      e[base + x] = 0.0f;
+    double *u = phi + (size_t)(id - 1) * G::NCELL;
+    float *c = e + (size_t)(id - 1) * L::STRIDE;
+    // interior: the pair of cells 2s, 2s + 1 (lexicographic), one of each
+    // colour, both at s in their colour's block
+    for (int s = ltid<BS>(); s < L::HALF; s += BS) {
+      int i, j, k;
+      interior_ijk<D, NC>(2 * s, i, j, k);
+      const int y = G::idx(i, j, k), c0 = (i + j + k) & 1;
+      const int e0 = c0 * L::HALF + s, e1 = (1 - c0) * L::HALF + s;
+      u[y] = u[y] + (double)c[e0];
+      u[y + 1] = u[y + 1] + (double)c[e1];
+      c[e0] = 0.0f;
+      c[e1] = 0.0f;
     }
+    for (int x = G::NINT + ltid<BS>(); x < G::NCELL; x += BS) c[x] = 0.0f;  // the ghosts
   }
 }
 
@@ -1235,13 +1332,14 @@ template <int D, int NC, int BS, typename T, int BB = 1>
 __global__ void __launch_bounds__(BS * BB) k_gc_lds(const int *__restrict__ ids, const DBox *__restrict__ boxes, T *var,
                                                     VarBc m, int corners, int nw) {
   using G = Geo<D, NC>;
+  using L = Lay<D, NC, T>;
   __shared__ T s_all[BB][G::NCELL];
   T *s = s_all[lgrp<BS, BB>()];
   for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
     const int id = ids[w];
     if (id <= 0) continue;  // run masked out
     const DBox &bx = boxes[id - 1];
-    T *u = var + (size_t)(id - 1) * G::NCELL;
+    T *u = var + (size_t)(id - 1) * L::STRIDE;
     lds_stage<D, NC, BS, T, BB>(s, boxes, bx, id, var, m, -1);
     if (corners) {
       if (D == 3) {
@@ -1255,8 +1353,8 @@ __global__ void __launch_bounds__(BS * BB) k_gc_lds(const int *__restrict__ ids,
           const int nb_id = bx.nbmat[(dir[0] + 1) + 3 * (dir[1] + 1) + 9 * (dir[2] + 1)];
           T v;
           if (nb_id > 0) {
-            const T *w = var + (size_t)(nb_id - 1) * G::NCELL;
-            v = w[G::idx(lo[0] - dir[0] * NC, lo[1] - dir[1] * NC, lo[2] - dir[2] * NC)];
+            const T *w = var + (size_t)(nb_id - 1) * L::STRIDE;
+            v = w[L::in_at(lo[0] - dir[0] * NC, lo[1] - dir[1] * NC, lo[2] - dir[2] * NC)];
           } else {
             const int o0 = (dim + 1) % 3, o1 = (dim + 2) % 3;
             int di[3];
@@ -1279,8 +1377,8 @@ __global__ void __launch_bounds__(BS * BB) k_gc_lds(const int *__restrict__ ids,
         const int nb_id = bx.nbmat[(dnb[0] + 1) + 3 * (dnb[1] + 1) + 9 * (dnb[2] + 1)];
         T v;
         if (nb_id > 0) {
-          const T *w = var + (size_t)(nb_id - 1) * G::NCELL;
-          v = w[G::idx(lo[0] - dnb[0] * NC, lo[1] - dnb[1] * NC, lo[2] - dnb[2] * NC)];
+          const T *w = var + (size_t)(nb_id - 1) * L::STRIDE;
+          v = w[L::in_at(lo[0] - dnb[0] * NC, lo[1] - dnb[1] * NC, lo[2] - dnb[2] * NC)];
         } else if (D == 2) {
           const int di = 1 - 2 * (lo[0] & 1), dj = 1 - 2 * (lo[1] & 1);
           v = s[G::idx(lo[0] + di, lo[1], 0)] + s[G::idx(lo[0], lo[1] + dj, 0)] - s[G::idx(lo[0] + di, lo[1] + dj, 0)];
@@ -1298,7 +1396,7 @@ __global__ void __launch_bounds__(BS * BB) k_gc_lds(const int *__restrict__ ids,
     for (int x = ltid<BS>(); x < G::NCELL; x += BS) {
       const int i = x % G::NG, j = (x / G::NG) % G::NG, k = (D == 3) ? x / (G::NG * G::NG) : 0;
       const int out = (i == 0 || i == NC + 1) + (j == 0 || j == NC + 1) + (D == 3 && (k == 0 || k == NC + 1));
-      if (out == 1 || (corners && out > 1)) u[x] = s[x];
+      if (out == 1 || (corners && out > 1)) u[L::at(i, j, k)] = s[x];
     }
     gsync<BS, BB>();  // (the group's LDS is reused by its next item)
   }
