@@ -199,6 +199,7 @@ struct afgpu_ctx {
   bool photoi_inner_vcycle = false;  // photoionization inner cycle: V-cycle instead of FMG
   bool flux32 = true;  // mixed precision: FP32 flux stencil (FP64 accumulation)
   bool fuse_direct = false;  // uniform-level solve: last-axis transforms fused into one pass
+  bool direct_f32 = true;    // uniform-level solve of the FP32 inner cycle in FP32 (matrix cores)
   struct Direct {
     bool set = false;  // lc and the map below are current for the tree
     int lc = 1, N = 0;
@@ -208,6 +209,7 @@ struct afgpu_ctx {
     struct F {
       bool ok = false;
       double *vinv = nullptr, *v = nullptr, *vinvT = nullptr, *vT = nullptr, *eig = nullptr;
+      float *vinv32 = nullptr, *v32 = nullptr, *eig32 = nullptr;  // (rounded copies)
     } f[1 + AFGPU_MAX_MODES];
   } dir;
   // reductions
@@ -990,6 +992,15 @@ struct Impl : ImplBase {
       HIPCHK(hipMalloc(dst[q], srcs[q]->size() * sizeof(double)));
       HIPCHK(hipMemcpy(*dst[q], srcs[q]->data(), srcs[q]->size() * sizeof(double), hipMemcpyHostToDevice));
     }
+    float **dst32[3] = {&f.vinv32, &f.v32, &f.eig32};
+    const std::vector<double> *srcs32[3] = {&vinv, &v, &eig};
+    for (int q = 0; q < 3; q++) {
+      std::vector<float> r(srcs32[q]->begin(), srcs32[q]->end());
+      (void)hipFree(*dst32[q]);
+      *dst32[q] = nullptr;
+      HIPCHK(hipMalloc(dst32[q], r.size() * sizeof(float)));
+      HIPCHK(hipMemcpy(*dst32[q], r.data(), r.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
     f.ok = true;
     return AFGPU_OK;
   }
@@ -1097,6 +1108,42 @@ struct Impl : ImplBase {
   }
 
 
+  // The transforms of the uniform-level solve in FP32 on the matrix cores
+  // (64 or 128 cells per axis): the first x transform reads the rhs from the
+  // FP32 pool, the last writes phi to it; the grids A0 / A1 hold floats.
+  template <int N>
+  int direct_transforms32(const afgpu_ctx::Direct::F &f, float lambda2, float *A0, float *A1,
+                          const float *rhs_pool, float *phi_pool) {
+    static_assert(N == 64 || N == 128, "matrix-core transform for N = 64, 128");
+    const size_t NN = (size_t)N * N;
+    const int batch = (D == 3) ? N : c.n_runs;
+    const int nrows = (D == 3) ? N * N : N * c.n_runs;
+    const int row_blocks = (N / 64) * (nrows / 64);
+    const int line_blocks = (N / 64) * (N / 64) * batch;
+    const int nb = N / NC, last = D - 1;
+    const int *map = c.dir.d_map, *nomap = nullptr;
+    const float *noin = nullptr;
+    float *noout = nullptr;
+    float *cur = A0, *nxt = A1;
+    auto lines = [&](const float *M, int axis, int div) {
+      hipLaunchKernelGGL((k_lc_mfma32<D, N, false, 0, NC>), dim3(line_blocks), dim3(256), 0, c.cur, cur, nxt, M,
+                         axis, div, (const float *)f.eig32, lambda2, nomap, 0, noin, noout);
+      std::swap(cur, nxt);
+      return hipGetLastError();
+    };
+    hipLaunchKernelGGL((k_lc_mfma32<D, N, true, 1, NC>), dim3(row_blocks), dim3(256), 0, c.cur, noin, cur,
+                       (const float *)f.vinv32, 0, 0, (const float *)f.eig32, lambda2, map, nb, rhs_pool, noout);
+    HIPCHK(hipGetLastError());
+    if (D == 3) HIPCHK(lines(f.vinv32 + NN, 1, 0));
+    HIPCHK(lines(f.vinv32 + (size_t)last * NN, last, 1));
+    HIPCHK(lines(f.v32 + (size_t)last * NN, last, 0));
+    if (D == 3) HIPCHK(lines(f.v32 + NN, 1, 0));
+    hipLaunchKernelGGL((k_lc_mfma32<D, N, true, 2, NC>), dim3(row_blocks), dim3(256), 0, c.cur, (const float *)cur,
+                       noout, (const float *)f.v32, 0, 0, (const float *)f.eig32, lambda2, map, nb, noin, phi_pool);
+    HIPCHK(hipGetLastError());
+    return AFGPU_OK;
+  }
+
   // phi(lc) := L^-1 rhs(lc) on the whole uniform level (homogeneous
   // boundary conditions: the correction equation), then its ghost layer.
   template <typename T>
@@ -1127,6 +1174,16 @@ struct Impl : ImplBase {
     const T *rp = pool ? (const T *)v.rhs : nullptr;
     T *pp = pool ? v.phi : nullptr;
     const double l2 = c.ops[v.op].lambda2;
+    if constexpr (std::is_same<T, float>::value) {
+      if (pool && c.direct_f32 && !c.fuse_direct) {
+        float *F0 = (float *)A0, *F1 = (float *)A1;
+        if (d.N == 64)
+          RCHK(direct_transforms32<64>(f, (float)l2, F0, F1, rp, pp));
+        else
+          RCHK(direct_transforms32<128>(f, (float)l2, F0, F1, rp, pp));
+        return gc_var(lc, v.phi, v.m, true);
+      }
+    }
     const double *res = nullptr;
     switch (d.N) {
       case 16: RCHK((direct_transforms<16, T>(f, l2, A0, A1, &res))); break;
@@ -2117,6 +2174,9 @@ int afgpu_destroy(afgpu_t *h) {
     (void)hipFree(f.vinvT);
     (void)hipFree(f.vT);
     (void)hipFree(f.eig);
+    (void)hipFree(f.vinv32);
+    (void)hipFree(f.v32);
+    (void)hipFree(f.eig32);
   }
   (void)hipFree(h->d_boxes);
   (void)hipFree(h->d_lists);
@@ -2991,6 +3051,11 @@ int afgpu_set_option(afgpu_t *h, int option, int value) {
       RCHK(sync_stream(h));
       drop_graphs(*h);
       h->fuse_direct = value != 0;
+      return AFGPU_OK;
+    case AFGPU_OPT_DIRECT_F32:
+      RCHK(sync_stream(h));
+      drop_graphs(*h);
+      h->direct_f32 = value != 0;
       return AFGPU_OK;
     case AFGPU_OPT_STEP_GRAPH:
       RCHK(sync_stream(h));

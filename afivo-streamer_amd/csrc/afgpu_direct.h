@@ -276,6 +276,7 @@ __global__ void __launch_bounds__(256) k_lc_gemm(const double *__restrict__ in, 
 // staged in LDS.  Lane layout of the instruction: A operand A[l % 16][l / 16],
 // B operand B[l / 16][l % 16], result register j of lane l D[4 j + l / 16][l % 16].
 typedef double lc_double4 __attribute__((ext_vector_type(4)));
+typedef float lc_float4 __attribute__((ext_vector_type(4)));
 
 // Offset in the box pool (of values of type TS) of point (x, y, r) of the
 // uniform level, stored as boxes of NC^D cells: r is z in 3D and the run of
@@ -407,6 +408,130 @@ __global__ void __launch_bounds__(256) k_lc_mfma(const double *__restrict__ in, 
               z = pp;
             }
             double den = eig[x] + eig[N + y];
+            if (D == 3) den = den + eig[2 * N + z];
+            den = den - lambda2;
+            v = v / den;
+          }
+          out[base + (size_t)pp * sm + qq] = v;
+        }
+      }
+}
+
+// k_lc_mfma in single precision (the mixed form's inner cycle, option
+// AFGPU_OPT_DIRECT_F32): grids, matrices and eigenvalues in FP32, products
+// on the FP32 matrix cores (v_mfma_f32_16x16x4f32: the operands as in the
+// FP64 form, result register j of lane l holding row 4 (l / 16) + j,
+// column l % 16 of the 16 x 16 block).
+template <int D, int N, bool ROWS, int IO = 0, int NC = 16>
+__global__ void __launch_bounds__(256) k_lc_mfma32(const float *__restrict__ in, float *__restrict__ out,
+                                                   const float *__restrict__ M, int axis, int div,
+                                                   const float *__restrict__ eig, float lambda2,
+                                                   const int *__restrict__ map, int nb, const float *__restrict__ pin,
+                                                   float *__restrict__ pout) {
+  static_assert(IO == 0 || ROWS, "box-pool input and output in the x form only");
+  static_assert(N == 64 || N == 128, "matrix-core transform for N = 64, 128");
+  constexpr int T = 64, KC = 32, AP = KC + 1, BP = T + 1, NT = N / T;
+  constexpr int PER = T * KC / 256;  // values of each operand tile per thread
+  __shared__ float As[T * AP];   // As[p][kk]
+  __shared__ float Bs[KC * BP];  // Bs[kk][q]
+  // blocks: ROWS: (output tiles) x (row tiles); else (output tiles) x
+  // (column tiles) x (the remaining coordinate)
+  const int t0 = blockIdx.x % NT, rest = blockIdx.x / NT;
+  size_t sm = 1, base = 0;
+  int o = 0, x0 = 0, p0 = 0, q0 = 0;
+  if (ROWS) {
+    p0 = rest * T;  // grid rows
+    q0 = t0 * T;    // output index i
+  } else {
+    p0 = t0 * T;  // output index i
+    x0 = (rest % NT) * T;
+    o = rest / NT;
+    q0 = 0;
+    sm = (axis == 1) ? (size_t)N : (size_t)N * N;
+    const size_t so = (D == 2) ? (size_t)N * N : ((axis == 1) ? (size_t)N * N : (size_t)N);
+    base = (size_t)o * so + x0;
+  }
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wp = (w >> 1) * 32, wq = (w & 1) * 32;
+  // one K chunk of both operands, global -> registers (consecutive threads
+  // take consecutive addresses) and registers -> LDS
+  float pa[PER], pb[PER];
+  auto fetch = [&](int k0) {
+    for (int u = 0; u < PER; u++) {
+      const int q = tid + 256 * u;
+      const int pp = q / KC, kk = q % KC;
+      if (IO == 1) {
+        const int row = p0 + pp;
+        const long long off = lc_pool_offset<D, NC, float>(map, nb, k0 + kk, row % N, row / N);
+        pa[u] = off >= 0 ? pin[off] : 0.0f;
+      } else {
+        pa[u] = ROWS ? in[(size_t)(p0 + pp) * N + k0 + kk] : M[(size_t)(p0 + pp) * N + k0 + kk];
+      }
+      if (ROWS) {
+        pb[u] = M[(size_t)(q0 + pp) * N + k0 + kk];
+      } else {
+        const int kb = q / T, qq = q % T;
+        pb[u] = in[base + (size_t)(k0 + kb) * sm + qq];
+      }
+    }
+  };
+  auto stash = [&]() {
+    for (int u = 0; u < PER; u++) {
+      const int q = tid + 256 * u;
+      As[(q / KC) * AP + q % KC] = pa[u];
+      if (ROWS)
+        Bs[(q % KC) * BP + q / KC] = pb[u];
+      else
+        Bs[(q / T) * BP + q % T] = pb[u];
+    }
+  };
+  lc_float4 acc[2][2];
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++) acc[a][b] = lc_float4{0.0f, 0.0f, 0.0f, 0.0f};
+  fetch(0);
+  for (int k0 = 0; k0 < N; k0 += KC) {
+    stash();
+    __syncthreads();
+    if (k0 + KC < N) fetch(k0 + KC);
+    for (int ks = 0; ks < KC; ks += 4) {
+      const int kk = ks + (lane >> 4);
+      float a[2], b[2];
+      for (int t = 0; t < 2; t++) {
+        a[t] = As[(wp + 16 * t + (lane & 15)) * AP + kk];
+        b[t] = Bs[kk * BP + wq + 16 * t + (lane & 15)];
+      }
+      for (int ta = 0; ta < 2; ta++)
+        for (int tb = 0; tb < 2; tb++)
+          acc[ta][tb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ta], b[tb], acc[ta][tb], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  for (int ta = 0; ta < 2; ta++)
+    for (int tb = 0; tb < 2; tb++)
+      for (int j = 0; j < 4; j++) {
+        const int pp = p0 + wp + 16 * ta + 4 * (lane >> 4) + j;
+        const int qq = q0 + wq + 16 * tb + (lane & 15);
+        float v = acc[ta][tb][j];
+        if (ROWS && IO == 2) {
+          const long long off = lc_pool_offset<D, NC, float>(map, nb, qq, pp % N, pp / N);
+          if (off >= 0) pout[off] = v;
+        } else if (ROWS) {
+          out[(size_t)pp * N + qq] = v;
+        } else {
+          if (div) {
+            const int x = x0 + qq;
+            int y, z;
+            if (D == 2) {
+              y = pp;
+              z = 0;
+            } else if (axis == 1) {
+              y = pp;
+              z = o;
+            } else {
+              y = o;
+              z = pp;
+            }
+            float den = eig[x] + eig[N + y];
             if (D == 3) den = den + eig[2 * N + z];
             den = den - lambda2;
             v = v / den;

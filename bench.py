@@ -77,11 +77,13 @@ def tune(p, enabled):
 
 
 def run_gpu(case, precision, warmup, steps, timing_steps, direct_n=None, smooth=None, block=None, batch=None,
-            photoi_vcycle=None, autotune=True, fuse_direct=None):
+            photoi_vcycle=None, autotune=True, fuse_direct=None, direct_f32=None):
     p = A.Path("gpu", case.D, case.nc, case.coord, precision)
     S.apply(p, case)
     if fuse_direct is not None:
         p.set_option(A.OPT_FUSE_DIRECT, fuse_direct)
+    if direct_f32 is not None:
+        p.set_option(A.OPT_DIRECT_F32, direct_f32)
     if direct_n is not None:
         p.set_option(A.OPT_DIRECT_N, direct_n)
     if smooth is not None:
@@ -299,6 +301,8 @@ def main():
                     help="mixed precision: block smoother in the inner cycle (1) or red-black sweeps (0)")
     ap.add_argument("--batch-modes", type=int, default=1,
                     help="mixed precision: photoionization modes as parallel graph branches (1) or in turn (0)")
+    ap.add_argument("--direct-f32", type=int, default=1,
+                    help="mixed precision: uniform-level solve of the FP32 inner cycle in FP32 (1) or FP64 (0)")
     ap.add_argument("--fuse-direct", type=int, default=0,
                     help="mixed precision: uniform-level solve with the fused transform passes (1) or separate (0)")
     ap.add_argument("--photoi-vcycle", type=int, default=0,
@@ -344,7 +348,7 @@ def main():
                                     args.direct_n if mixed else None, args.mixed_smooth if mixed else None,
                                     args.block_smooth if mixed else None, args.batch_modes if mixed else None,
                                     args.photoi_vcycle if mixed else None, bool(args.autotune),
-                                    args.fuse_direct if mixed else None)
+                                    args.fuse_direct if mixed else None, args.direct_f32 if mixed else None)
     cell_steps = sum(i["leaf_cells"] for i in infos)
     value = cell_steps / secs
     nv = [i["n_vcycles"] for i in infos]
@@ -440,6 +444,7 @@ def main():
                    "batch_modes": args.batch_modes if prec == A.PREC_MIXED else None,
                    "photoi_vcycle": args.photoi_vcycle if prec == A.PREC_MIXED else None,
                    "fuse_direct": args.fuse_direct if prec == A.PREC_MIXED else None,
+                   "direct_f32": args.direct_f32 if prec == A.PREC_MIXED else None,
                    "shapes": p.shapes},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -286,6 +286,10 @@ int afgpu_kernel_stats(afgpu_t *h, int kind, int reset, double *total_ms,
                                     transform, division and first backward transform as one pass
                                     (1) or two (0); identical results (default 0: the one pass,
                                     at one 100 KB-LDS workgroup per CU, measured slower) */
+#define AFGPU_OPT_DIRECT_F32 11 /* mixed precision: the uniform-level solve of the FP32 inner
+                                   cycle on the matrix cores (64 or 128 cells per axis) in FP32
+                                   (1) or FP64 (0); the FP64 outer correction sets the accuracy
+                                   either way (default 1; 0 whenever AFGPU_OPT_FUSE_DIRECT is 1) */
 int afgpu_set_option(afgpu_t *h, int option, int value);
 
 /* ---- ensembles of independent runs --------------------------------------

@@ -207,22 +207,23 @@ def test_mixed_batched_modes_equal_sequential(A, S, name):
 
 def test_mixed_direct_solve_128_matrix_cores(A, S):
     """3D, 16^3 boxes, levels 1-4 uniform (level 4 = 128^3 cells, solved
-    exactly by the inner cycle): the matrix-core and the vector transforms
-    reach the FP64 fixed point."""
+    exactly by the inner cycle): the FP32 and FP64 matrix-core transforms
+    and the FP64 vector transforms all reach the FP64 fixed point."""
     case = setup(S.small_amr(3, nc=16, uniform_lvl=4, max_lvl=5, frac=0.03, seed=4))
     f = S.apply(A.Path("gpu", case.D, case.nc, case.coord, A.PREC_FP64), case)
     f.field_compute(0, False)
     for _ in range(6):
         f.mg_fas_vcycle(A.OP_POISSON, True, 0)
     ref = leaf_interior(case, f, A.CC_PHI)
-    for mfma in (1, 0):
+    for mfma, f32 in ((1, 1), (1, 0), (0, 0)):
         m = S.apply(A.Path("gpu", case.D, case.nc, case.coord, A.PREC_MIXED), case)
         m.set_option(A.OPT_DIRECT_MFMA, mfma)
+        m.set_option(A.OPT_DIRECT_F32, f32)
         n0, r0 = m.field_compute(0, False)
         for _ in range(4):
             m.field_compute(0, True)
         e = rel_max(leaf_interior(case, m, A.CC_PHI), ref)
-        print("mfma", mfma, "initial cycles", n0, "phi vs fp64 fixed point %.2e" % e, flush=True)
+        print("mfma", mfma, "f32", f32, "initial cycles", n0, "phi vs fp64 fixed point %.2e" % e, flush=True)
         assert e < 1e-11
 
 
@@ -230,17 +231,43 @@ def test_mixed_direct_solve_fused_passes_bit_for_bit(A, S):
     """The uniform-level solve with the gather and scatter folded into the
     x transforms and the last axis's forward transform, division and
     backward transform as one pass gives the same bits as the separate
-    passes: 3D with photoionization (level 4 = 128^3), a few full steps."""
+    passes: 3D with photoionization (level 4 = 128^3), a few full steps.
+    (The fused pass exists in FP64 only: both runs transform in FP64.)"""
     case = setup(S.small_amr(3, nc=16, uniform_lvl=4, max_lvl=5, frac=0.03, seed=4), photoi=True)
     runs = []
     for fuse in (1, 0):
         m = S.apply(A.Path("gpu", case.D, case.nc, case.coord, A.PREC_MIXED), case)
+        m.set_option(A.OPT_DIRECT_F32, 0)
         m.set_option(A.OPT_FUSE_DIRECT, fuse)
         infos = [m.field_compute(0, False)] + [m.step() for _ in range(3)]
         runs.append((infos, [m.get_cc(iv) for iv in (A.CC_PHI, A.CC_ELECTRON, A.CC_PHOTO)]))
     assert runs[0][0] == runs[1][0]
     for a, b in zip(runs[0][1], runs[1][1]):
         assert np.array_equal(a, b)
+
+
+def test_mixed_direct_solve_f32_transforms(A, S):
+    """The inner cycle's uniform-level solve in FP32 (the default) against
+    the same solve in FP64: 3D with photoionization (level 4 = 128^3), a few
+    steps.  The exact solve is one step of the FP32 correction cycle, so its
+    rounding changes the iterates but not what the FP64 outer correction
+    converges to: equal V-cycle counts (or one more), and potential, field
+    and densities within the threshold's accuracy of each other."""
+    case = setup(S.small_amr(3, nc=16, uniform_lvl=4, max_lvl=5, frac=0.03, seed=4), photoi=True)
+    runs = []
+    for f32 in (1, 0):
+        m = S.apply(A.Path("gpu", case.D, case.nc, case.coord, A.PREC_MIXED), case)
+        m.set_option(A.OPT_DIRECT_F32, f32)
+        n0, _ = m.field_compute(0, False)
+        infos = [m.step() for _ in range(4)]
+        runs.append((m, [n0] + [i["n_vcycles"] for i in infos]))
+    print("V-cycles f32", runs[0][1], "f64", runs[1][1], flush=True)
+    for a, b in zip(runs[0][1], runs[1][1]):
+        assert abs(a - b) <= 1
+    for iv, tol in ((A.CC_PHI, 1e-7), (A.CC_EFIELD, 1e-5), (A.CC_ELECTRON, 1e-6), (A.CC_PHOTO, 1e-6)):
+        e = rel_max(leaf_interior(case, runs[0][0], iv), leaf_interior(case, runs[1][0], iv))
+        print("var", iv, "f32 vs f64 transforms %.2e" % e, flush=True)
+        assert e < tol
 
 
 def test_mixed_graph_replays_match_eager(A, S):
