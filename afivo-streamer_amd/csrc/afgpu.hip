@@ -2357,6 +2357,10 @@ int afgpu_sync_tree(afgpu_t *h, int n_boxes, const int *lvl, const int *ix, cons
           return AFGPU_ERR_ARG;
         }
     }
+  for (int b = 0; b < n_boxes; b++) {
+    DBox &x = hb[b];
+    for (int q = 0; q < nnb; q++) x.pnb[q] = (x.nb[q] == 0 && x.parent > 0) ? hb[x.parent - 1].nb[q] : 0;
+  }
   h->src_ids = ids;
   h->src_leaves = leaves;
   h->src_parents = parents;

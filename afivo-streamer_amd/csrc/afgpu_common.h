@@ -21,6 +21,7 @@ struct DBox {
   int ix[3];
   int children[8];
   int nb[6];
+  int pnb[6];  // faces on a refinement boundary (nb 0): the parent's neighbour there; else 0
   int nbmat[27];
   double r_min[3];
   double dr[3];

@@ -50,8 +50,7 @@ __device__ inline double gc2_value(const DBox *__restrict__ boxes, int id, const
     return (layer == 0) ? c0 * bv + c1 * x : c2 * bv + c1 * x;
   }
   // gc2_prolong_rb: limited linear prolongation from the parent's neighbour
-  const int p_nb_id = boxes[bx.parent - 1].nb[nb];
-  const double *P = var + (size_t)(p_nb_id - 1) * G::NCELL;
+  const double *P = var + (size_t)(bx.pnb[nb] - 1) * G::NCELL;
   int off[3] = {0, 0, 0};
   for (int dd = 0; dd < D; dd++) off[dd] = ((bx.ix[dd] - 1) & 1) * H;
   // fine index along every dimension, and the coarse cell containing it

@@ -117,43 +117,42 @@ __device__ inline void interior_ijk(int q, int &i, int &j, int &k) {
 // are the box's first and second interior cells along the face normal.
 // `coarse` (default: var) holds the coarser level's values when the box's
 // own level is kept in a different buffer.
+// The part of a ghost value of a face without a same-level neighbour
+// (nb_id <= 0) that does not depend on the box's own cells: the boundary
+// value's term, or the coarse interpolation at a refinement boundary (and
+// the coarse cell the limiter compares with).  ghost_finish() adds the
+// terms of the box's cells x1, x2 in the order ghost_value() always used.
+template <typename T>
+struct GhostPart {
+  T p, c1;
+};
 template <int D, int NC, typename T>
-__device__ inline T ghost_value(const DBox *__restrict__ boxes, const DBox &bx, const T *var, const VarBc &m, int nb,
-                                int a, int b, T x1, T x2, const T *coarse = nullptr) {
-  using G = Geo<D, NC>;
+__device__ inline GhostPart<T> ghost_coarse(const DBox &bx, const T *var, const VarBc &m, int nb, int a, int b,
+                                            const T *coarse = nullptr) {
   using L = Lay<D, NC, T>;
   constexpr int H = NC / 2;
   const int d = nb / 2;
   const bool low = (nb % 2) == 0;
   int t0, t1;
   tangential(D, d, t0, t1);
-  const int nb_id = bx.nb[nb];
-  if (nb_id > 0) {
-    const T *w = var + (size_t)(nb_id - 1) * L::STRIDE;
-    int q[3] = {0, 0, 0};
-    q[t0] = a;
-    if (D == 3) q[t1] = b;
-    q[d] = low ? NC : 1;
-    return w[L::at(q[0], q[1], q[2])];
+  GhostPart<T> g;
+  g.c1 = T(0);
+  if (bx.nb[nb] < 0) {
+    T c0;
+    if (m.type[nb] == BC_DIRICHLET)
+      c0 = 2;
+    else if (m.type[nb] == BC_NEUMANN)
+      c0 = (T)bx.dr[d] * T(low ? -1 : 1);
+    else
+      c0 = 1;
+    g.p = c0 * (T)bc_value(m, nb, bx.run);
+    return g;
   }
-  if (nb_id < 0) {
-    T c0, c1, c2;
-    if (m.type[nb] == BC_DIRICHLET) {
-      c0 = 2; c1 = -1; c2 = 0;
-    } else if (m.type[nb] == BC_NEUMANN) {
-      c0 = (T)bx.dr[d] * T(low ? -1 : 1); c1 = 1; c2 = 0;
-    } else {
-      c0 = 1; c1 = 0; c2 = 0;
-    }
-    return c0 * (T)bc_value(m, nb, bx.run) + c1 * x1 + c2 * x2;
-  }
-  const int p_nb_id = boxes[bx.parent - 1].nb[nb];
-  const T *P = (coarse ? coarse : var) + (size_t)(p_nb_id - 1) * L::STRIDE;
+  const T *P = (coarse ? coarse : var) + (size_t)(bx.pnb[nb] - 1) * L::STRIDE;
   int co[3] = {0, 0, 0};
   for (int dd = 0; dd < D; dd++) co[dd] = ((bx.ix[dd] - 1) & 1) * H;
   if (m.rb == RB_MG) {
     const int cn = low ? NC : 1;
-    T gcv;
     if (D == 2) {
       const int i = (a + 1) >> 1;
       int q[3] = {0, 0, 0};
@@ -165,7 +164,7 @@ __device__ inline T ghost_value(const DBox *__restrict__ boxes, const DBox &bx, 
       q[t0] = co[t0] + i;
       const T tc = P[L::at(q[0], q[1], q[2])];
       const T grad = T(0.125) * (tp - tm);
-      gcv = (a & 1) ? tc - grad : tc + grad;
+      g.p = (a & 1) ? tc - grad : tc + grad;
     } else {
       const int i = (a + 1) >> 1, j = (b + 1) >> 1;
       auto TT = [&](int qa, int qb) {
@@ -179,14 +178,12 @@ __device__ inline T ghost_value(const DBox *__restrict__ boxes, const DBox &bx, 
       const T g2 = T(0.125) * (TT(i, j + 1) - TT(i, j - 1));
       const T tc = TT(i, j);
       const T h = (a & 1) ? tc - g1 : tc + g1;
-      gcv = (b & 1) ? h - g2 : h + g2;
+      g.p = (b & 1) ? h - g2 : h + g2;
     }
-    return T(0.5) * gcv + T(0.75) * x1 - T(0.25) * x2;
+    return g;
   }
   const T third = T(1 / 3.0), sixth = T(1 / 6.0);
   const int ix_c = low ? NC : 1;
-  const bool limited = (m.rb == RB_INTERP_LIM);
-  T v;
   if (D == 2) {
     const int a1 = co[t0] + ((a + 1) >> 1), a2 = a1 + 1 - 2 * (a & 1);
     int q[3] = {0, 0, 0};
@@ -195,8 +192,8 @@ __device__ inline T ghost_value(const DBox *__restrict__ boxes, const DBox &bx, 
     const T c1 = P[L::at(q[0], q[1], q[2])];
     q[t0] = a2;
     const T c2 = P[L::at(q[0], q[1], q[2])];
-    v = T(0.5) * c1 + sixth * c2 + third * x1;
-    if (limited && v > 2 * c1) v = 2 * c1;
+    g.p = T(0.5) * c1 + sixth * c2;
+    g.c1 = c1;
   } else {
     const int a1 = co[t0] + ((a + 1) >> 1), a2 = a1 + 1 - 2 * (a & 1);
     const int b1 = co[t1] + ((b + 1) >> 1), b2 = b1 + 1 - 2 * (b & 1);
@@ -210,10 +207,49 @@ __device__ inline T ghost_value(const DBox *__restrict__ boxes, const DBox &bx, 
     const T cb = P[L::at(q[0], q[1], q[2])];
     // the z-face case sums the off-diagonal coarse values in the other order
     const T c2 = (d == 2) ? cb : ca, c3 = (d == 2) ? ca : cb;
-    v = third * c1 + sixth * c2 + sixth * c3 + third * x1;
-    if (limited && v > 2 * c1) v = 2 * c1;
+    g.p = third * c1 + sixth * c2 + sixth * c3;
+    g.c1 = c1;
   }
+  return g;
+}
+
+template <int D, int NC, typename T>
+__device__ inline T ghost_finish(const DBox &bx, const VarBc &m, int nb, const GhostPart<T> &g, T x1, T x2) {
+  if (bx.nb[nb] < 0) {
+    T c1, c2;
+    if (m.type[nb] == BC_DIRICHLET) {
+      c1 = -1; c2 = 0;
+    } else if (m.type[nb] == BC_NEUMANN) {
+      c1 = 1; c2 = 0;
+    } else {
+      c1 = 0; c2 = 0;
+    }
+    return g.p + c1 * x1 + c2 * x2;
+  }
+  if (m.rb == RB_MG) return T(0.5) * g.p + T(0.75) * x1 - T(0.25) * x2;
+  T v = g.p + T(1 / 3.0) * x1;
+  if (m.rb == RB_INTERP_LIM && v > 2 * g.c1) v = 2 * g.c1;
   return v;
+}
+
+template <int D, int NC, typename T>
+__device__ inline T ghost_value(const DBox *__restrict__ boxes, const DBox &bx, const T *var, const VarBc &m, int nb,
+                                int a, int b, T x1, T x2, const T *coarse = nullptr) {
+  using L = Lay<D, NC, T>;
+  const int d = nb / 2;
+  const bool low = (nb % 2) == 0;
+  int t0, t1;
+  tangential(D, d, t0, t1);
+  const int nb_id = bx.nb[nb];
+  if (nb_id > 0) {
+    const T *w = var + (size_t)(nb_id - 1) * L::STRIDE;
+    int q[3] = {0, 0, 0};
+    q[t0] = a;
+    if (D == 3) q[t1] = b;
+    q[d] = low ? NC : 1;
+    return w[L::at(q[0], q[1], q[2])];
+  }
+  return ghost_finish<D, NC, T>(bx, m, nb, ghost_coarse<D, NC, T>(bx, var, m, nb, a, b, coarse), x1, x2);
 }
 
 // Position of face cell (nb, a, b) at normal index p.
@@ -946,7 +982,13 @@ __device__ inline void lds_stage(T *s, const DBox *__restrict__ boxes, const DBo
       return L::in_at(i, j, k);
     }
   };
-  T vi[NI], vf[NF];
+  // with a colour, the physical and refinement-boundary ghosts' parts that
+  // do not depend on the box (the coarse values) are loaded in the first
+  // round too, and so are the updated colour's cells they need in the split
+  // layout: the second round then works in LDS only
+  const bool pre = colour >= 0;
+  T vi[NI], vf[NF], x1g[NF];
+  GhostPart<T> gp[NF];
   int gi[NF];
   for (int t = 0; t < NI; t++) {
     const int q = ltid<BS>() + t * BS;
@@ -961,12 +1003,17 @@ __device__ inline void lds_stage(T *s, const DBox *__restrict__ boxes, const DBo
     if (q >= NFC) continue;
     const int nb = q / G::NTAN, r = q % G::NTAN;
     const int nb_id = bx.nb[nb];
-    if (nb_id <= 0) continue;
+    if (nb_id <= 0 && !pre) continue;
     const int a = r % NC + 1, b = (D == 3) ? r / NC + 1 : 0;
     const bool low = (nb % 2) == 0;
     int p[3];
     face_pos<D>(nb, a, b, low ? 1 : NC, p);
     if (colour >= 0 && ((p[0] + p[1] + p[2] + colour) & 1)) continue;
+    if (nb_id <= 0) {
+      gp[t] = ghost_coarse<D, NC, T>(bx, var, m, nb, a, b, coarse);
+      if (one) x1g[t] = u[L::in_at(p[0], p[1], p[2])];
+      continue;
+    }
     int d = nb / 2;
     p[d] = low ? 0 : NC + 1;
     gi[t] = G::idx(p[0], p[1], p[2]);
@@ -987,20 +1034,38 @@ __device__ inline void lds_stage(T *s, const DBox *__restrict__ boxes, const DBo
   bool any = false;
   for (int nb = 0; nb < 2 * D; nb++) any = any || bx.nb[nb] <= 0;
   if (any) {
-    for (int q = ltid<BS>(); q < NFC; q += BS) {
-      const int nb = q / G::NTAN, r = q % G::NTAN;
-      if (bx.nb[nb] > 0) continue;
-      const int a = r % NC + 1, b = (D == 3) ? r / NC + 1 : 0;
-      const bool low = (nb % 2) == 0;
-      int p[3];
-      face_pos<D>(nb, a, b, low ? 1 : NC, p);
-      if (colour >= 0 && ((p[0] + p[1] + p[2] + colour) & 1)) continue;
-      const T x1 = one ? u[L::in_at(p[0], p[1], p[2])] : s[G::idx(p[0], p[1], p[2])];
-      face_pos<D>(nb, a, b, low ? 2 : NC - 1, p);
-      const T x2 = s[G::idx(p[0], p[1], p[2])];
-      const T g = ghost_value<D, NC, T>(boxes, bx, var, m, nb, a, b, x1, x2, coarse);
-      face_pos<D>(nb, a, b, low ? 0 : NC + 1, p);
-      s[G::idx(p[0], p[1], p[2])] = g;
+    if (pre) {
+      for (int t = 0; t < NF; t++) {
+        const int q = ltid<BS>() + t * BS;
+        if (q >= NFC) continue;
+        const int nb = q / G::NTAN, r = q % G::NTAN;
+        if (bx.nb[nb] > 0) continue;
+        const int a = r % NC + 1, b = (D == 3) ? r / NC + 1 : 0;
+        const bool low = (nb % 2) == 0;
+        int p[3];
+        face_pos<D>(nb, a, b, low ? 1 : NC, p);
+        if ((p[0] + p[1] + p[2] + colour) & 1) continue;
+        const T x1 = one ? x1g[t] : s[G::idx(p[0], p[1], p[2])];
+        face_pos<D>(nb, a, b, low ? 2 : NC - 1, p);
+        const T x2 = s[G::idx(p[0], p[1], p[2])];
+        face_pos<D>(nb, a, b, low ? 0 : NC + 1, p);
+        s[G::idx(p[0], p[1], p[2])] = ghost_finish<D, NC, T>(bx, m, nb, gp[t], x1, x2);
+      }
+    } else {
+      for (int q = ltid<BS>(); q < NFC; q += BS) {
+        const int nb = q / G::NTAN, r = q % G::NTAN;
+        if (bx.nb[nb] > 0) continue;
+        const int a = r % NC + 1, b = (D == 3) ? r / NC + 1 : 0;
+        const bool low = (nb % 2) == 0;
+        int p[3];
+        face_pos<D>(nb, a, b, low ? 1 : NC, p);
+        const T x1 = s[G::idx(p[0], p[1], p[2])];
+        face_pos<D>(nb, a, b, low ? 2 : NC - 1, p);
+        const T x2 = s[G::idx(p[0], p[1], p[2])];
+        const T g = ghost_value<D, NC, T>(boxes, bx, var, m, nb, a, b, x1, x2, coarse);
+        face_pos<D>(nb, a, b, low ? 0 : NC + 1, p);
+        s[G::idx(p[0], p[1], p[2])] = g;
+      }
     }
     gsync<BS, BB>();
   }
