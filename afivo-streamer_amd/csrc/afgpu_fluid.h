@@ -99,6 +99,14 @@ __global__ void __launch_bounds__(BS * BB)
   constexpr int NT = G::NTAN;
   __shared__ TF halo_all[BB][2 * D][2][NT];
   TF(&halo)[2 * D][2][NT] = halo_all[lgrp<BS, BB>()];
+  __shared__ TF cfl_all[BB][G::NINT];  // CFL sum per interior cell
+  // the mobility and diffusion columns of the transport table (3D, single
+  // precision, up to TABMAX rows), converted as lt_at_t converts them
+  constexpr int TABMAX = (D == 3 && sizeof(TF) < sizeof(double)) ? 1024 : 1;
+  __shared__ TF tab_all[BB][2 * TABMAX];
+  const bool tab_lds = TABMAX > 1 && td.n <= TABMAX;
+  if (tab_lds)  // (read after the first item's halo barrier)
+    for (int q = ltid<BS>(); q < 2 * td.n; q += BS) tab_all[lgrp<BS, BB>()][(q / td.n) * TABMAX + q % td.n] = (TF)td.cols[q];
   __shared__ double sh[BS / 64];
   for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
     const int id = ids[w];
@@ -140,82 +148,164 @@ __global__ void __launch_bounds__(BS * BB)
     };
 
     TF smax = -std::numeric_limits<TF>::max();
-    // face fluxes (flux_upwind, src/m_fluid.f90:102-209)
-    for (int q = ltid<BS>(); q < D * (NC + 1) * NT; q += BS) {
-      const int d = q / ((NC + 1) * NT);
-      // x faces: consecutive threads take consecutive faces of one grid row
-      // (contiguous in memory); y/z faces: consecutive rows of one face plane
-      const int qd = q % ((NC + 1) * NT);
-      const int f = (d == 0) ? qd % (NC + 1) + 1 : qd / NT + 1;
-      const int r = (d == 0) ? qd / (NC + 1) : qd % NT;
-      int t0, t1;
-      tangential(D, d, t0, t1);
-      int p[3] = {0, 0, 0};
-      p[t0] = r % NC + 1;
-      if (D == 3) p[t1] = r / NC + 1;
-      const TF inv_dx = TF(1) / (TF)bx.dr[d];
-      int fp[3] = {p[0], p[1], (D == 3) ? p[2] : 1};
-      fp[d] = f;
-      const TF ex = (TF)efc[fbase + G::fidx(fp[0], fp[1], fp[2], d)];
-      const TF Lm2 = line(d, r, p, f - 2), Lm1 = line(d, r, p, f - 1), L0 = line(d, r, p, f),
-               Lp1 = line(d, r, p, f + 1);
-      TF u;
-      if (-1 * ex > 0) {
-        u = Lm1 + TF(0.5) * lim_koren<TF>(L0 - Lm1, Lm1 - Lm2);
-      } else {
-        u = L0 - TF(0.5) * lim_koren<TF>(L0 - Lm1, Lp1 - L0);
-      }
-      int e0[3] = {p[0], p[1], p[2]}, e1[3] = {p[0], p[1], p[2]};
-      e0[d] = f - 1;
-      e1[d] = f;
-      const TF tmp_fc = TF(0.5) * ((TF)efield[base + G::idx(e0[0], e0[1], e0[2])] +
-                                   (TF)efield[base + G::idx(e1[0], e1[1], e1[2])]) *
-                        (TF)SI_to_Townsend * N_inv;
-      int lix;
-      TF lw;
-      lt_loc_t<TF>(td, tmp_fc, lix, lw);
-      const TF mu = lt_at_t<TF>(td, 0, lix, lw) * N_inv;
-      const TF dc = lt_at_t<TF>(td, 1, lix, lw) * N_inv;
-      const TF v = -mu * ex;
-      flux[fbase + G::fidx(fp[0], fp[1], fp[2], d)] = (double)(v * u - dc * inv_dx * (L0 - Lm1));
-      smax = fmax(smax, mu * u);
-    }
-    double smax_d = group_max<BS, BB>((double)smax, sh);
-    if (ltid<BS>() == 0) atomic_max_nonneg(run_slot(sig_max, bx), fmax(smax_d, 1e-100));
-
-    // CFL sum per cell (m_fluid.f90:194-195 summed over dimensions,
-    // m_af_flux_schemes.f90:779-848)
     TF cmax = 0;
-    for (int q = ltid<BS>(); q < G::NINT; q += BS) {
-      int i, j, k;
-      interior_ijk<D, NC>(q, i, j, k);
-      TF cfl = 0;
+    if constexpr (D == 3) {
+      // Face fluxes (flux_upwind, src/m_fluid.f90:102-209) direction by
+      // direction, one grid line per thread: the line's n_e, cell field and
+      // face field are loaded into registers in one batch (so each thread
+      // waits for memory once per line, not twice per face), then its NC + 1
+      // faces are formed in order.  The velocity and diffusion coefficient
+      // of each face also give the CFL terms of the two cells it bounds
+      // (m_fluid.f90:194-195, m_af_flux_schemes.f90:779-848), summed over
+      // the directions in LDS in the order x, y, z.
+      TF *cfl = cfl_all[lgrp<BS, BB>()];
+      const TF *tab = tab_lds ? tab_all[lgrp<BS, BB>()] : nullptr;
+      auto coef = [&](int c, int ix, TF w) {
+        return tab_lds ? w * tab[c * TABMAX + ix - 1] + (1 - w) * tab[c * TABMAX + ix] : lt_at_t<TF>(td, c, ix, w);
+      };
       for (int d = 0; d < D; d++) {
         const TF inv_dx = TF(1) / (TF)bx.dr[d];
-        TF vv[2], dd[2];
-        for (int s = 0; s < 2; s++) {
-          int c[3] = {i, j, k};
-          int fp[3] = {i, j, (D == 3) ? k : 1};
-          fp[d] += s;  // faces c and c+1
-          const TF ex = (TF)efc[fbase + G::fidx(fp[0], fp[1], fp[2], d)];
-          int e0[3] = {c[0], c[1], c[2]};
-          e0[d] = c[d] - 1 + s;
-          int e1[3] = {c[0], c[1], c[2]};
-          e1[d] = c[d] + s;
-          const TF tmp_fc = TF(0.5) * ((TF)efield[base + G::idx(e0[0], e0[1], e0[2])] +
-                                       (TF)efield[base + G::idx(e1[0], e1[1], e1[2])]) *
-                            (TF)SI_to_Townsend * N_inv;
-          int lix;
-          TF lw;
-          lt_loc_t<TF>(td, tmp_fc, lix, lw);
-          const TF mu = lt_at_t<TF>(td, 0, lix, lw) * N_inv;
-          dd[s] = lt_at_t<TF>(td, 1, lix, lw) * N_inv;
-          vv[s] = -mu * ex;
+        int t0, t1;
+        tangential(D, d, t0, t1);
+        for (int r = ltid<BS>(); r < NT; r += BS) {
+          int p[3] = {0, 0, 0};
+          p[t0] = r % NC + 1;
+          p[t1] = r / NC + 1;
+          TF nl[NC + 4];  // n_e at q = -1 .. NC + 2 along the line (index q + 1)
+          TF el[NC + 2];  // cell field at q = 0 .. NC + 1
+          TF xl[NC + 1];  // face field at faces 1 .. NC + 1
+          for (int q = 0; q <= NC + 1; q++) {
+            int c[3] = {p[0], p[1], p[2]};
+            c[d] = q;
+            el[q] = (TF)efield[base + G::idx(c[0], c[1], c[2])];
+            if (q >= 1 && q <= NC) nl[q + 1] = (TF)ne[base + G::idx(c[0], c[1], c[2])];
+          }
+          for (int f = 1; f <= NC + 1; f++) {
+            int fp[3] = {p[0], p[1], p[2]};
+            fp[d] = f;
+            xl[f - 1] = (TF)efc[fbase + G::fidx(fp[0], fp[1], fp[2], d)];
+          }
+          nl[0] = halo[2 * d][1][r];
+          nl[1] = halo[2 * d][0][r];
+          nl[NC + 2] = halo[2 * d + 1][0][r];
+          nl[NC + 3] = halo[2 * d + 1][1][r];
+          TF v_lo = 0, dc_lo = 0;  // the face below the current cell
+          for (int f = 1; f <= NC + 1; f++) {
+            const TF Lm2 = nl[f - 1], Lm1 = nl[f], L0 = nl[f + 1], Lp1 = nl[f + 2];
+            const TF ex = xl[f - 1];
+            TF u;
+            if (-1 * ex > 0) {
+              u = Lm1 + TF(0.5) * lim_koren<TF>(L0 - Lm1, Lm1 - Lm2);
+            } else {
+              u = L0 - TF(0.5) * lim_koren<TF>(L0 - Lm1, Lp1 - L0);
+            }
+            const TF tmp_fc = TF(0.5) * (el[f - 1] + el[f]) * (TF)SI_to_Townsend * N_inv;
+            int lix;
+            TF lw;
+            lt_loc_t<TF>(td, tmp_fc, lix, lw);
+            const TF mu = coef(0, lix, lw) * N_inv;
+            const TF dc = coef(1, lix, lw) * N_inv;
+            const TF v = -mu * ex;
+            int fp[3] = {p[0], p[1], p[2]};
+            fp[d] = f;
+            flux[fbase + G::fidx(fp[0], fp[1], fp[2], d)] = (double)(v * u - dc * inv_dx * (L0 - Lm1));
+            smax = fmax(smax, mu * u);
+            if (f >= 2) {  // cell f - 1, between the faces f - 1 and f
+              const TF cl =
+                  TF(1) * fmax(fabs(v), fabs(v_lo)) * inv_dx + TF(2) * fmax(dc, dc_lo) * (inv_dx * inv_dx);
+              int c[3] = {p[0], p[1], p[2]};
+              c[d] = f - 1;
+              const int qc = (c[0] - 1) + NC * ((c[1] - 1) + NC * (c[2] - 1));
+              cfl[qc] = ((d == 0) ? TF(0) : cfl[qc]) + cl;
+            }
+            v_lo = v;
+            dc_lo = dc;
+          }
         }
-        const TF cl = TF(1) * fmax(fabs(vv[1]), fabs(vv[0])) * inv_dx + TF(2) * fmax(dd[1], dd[0]) * (inv_dx * inv_dx);
-        cfl = cfl + cl;
+        gsync<BS, BB>();  // (a cell's next direction is another thread's line)
       }
-      cmax = fmax(cmax, cfl);
+      double smax_d = group_max<BS, BB>((double)smax, sh);
+      if (ltid<BS>() == 0) atomic_max_nonneg(run_slot(sig_max, bx), fmax(smax_d, 1e-100));
+      for (int q = ltid<BS>(); q < G::NINT; q += BS) cmax = fmax(cmax, cfl[q]);
+    } else {
+      // 2D boxes have too few lines per direction for a thread each: one
+      // face per thread (flux_upwind, src/m_fluid.f90:102-209), then the
+      // CFL sum per cell from its faces
+      for (int q = ltid<BS>(); q < D * (NC + 1) * NT; q += BS) {
+        const int d = q / ((NC + 1) * NT);
+        // x faces: consecutive threads take consecutive faces of one grid row
+        // (contiguous in memory); y/z faces: consecutive rows of one face plane
+        const int qd = q % ((NC + 1) * NT);
+        const int f = (d == 0) ? qd % (NC + 1) + 1 : qd / NT + 1;
+        const int r = (d == 0) ? qd / (NC + 1) : qd % NT;
+        int t0, t1;
+        tangential(D, d, t0, t1);
+        int p[3] = {0, 0, 0};
+        p[t0] = r % NC + 1;
+        if (D == 3) p[t1] = r / NC + 1;
+        const TF inv_dx = TF(1) / (TF)bx.dr[d];
+        int fp[3] = {p[0], p[1], (D == 3) ? p[2] : 1};
+        fp[d] = f;
+        const TF ex = (TF)efc[fbase + G::fidx(fp[0], fp[1], fp[2], d)];
+        const TF Lm2 = line(d, r, p, f - 2), Lm1 = line(d, r, p, f - 1), L0 = line(d, r, p, f),
+                 Lp1 = line(d, r, p, f + 1);
+        TF u;
+        if (-1 * ex > 0) {
+          u = Lm1 + TF(0.5) * lim_koren<TF>(L0 - Lm1, Lm1 - Lm2);
+        } else {
+          u = L0 - TF(0.5) * lim_koren<TF>(L0 - Lm1, Lp1 - L0);
+        }
+        int e0[3] = {p[0], p[1], p[2]}, e1[3] = {p[0], p[1], p[2]};
+        e0[d] = f - 1;
+        e1[d] = f;
+        const TF tmp_fc = TF(0.5) * ((TF)efield[base + G::idx(e0[0], e0[1], e0[2])] +
+                                     (TF)efield[base + G::idx(e1[0], e1[1], e1[2])]) *
+                          (TF)SI_to_Townsend * N_inv;
+        int lix;
+        TF lw;
+        lt_loc_t<TF>(td, tmp_fc, lix, lw);
+        const TF mu = lt_at_t<TF>(td, 0, lix, lw) * N_inv;
+        const TF dc = lt_at_t<TF>(td, 1, lix, lw) * N_inv;
+        const TF v = -mu * ex;
+        flux[fbase + G::fidx(fp[0], fp[1], fp[2], d)] = (double)(v * u - dc * inv_dx * (L0 - Lm1));
+        smax = fmax(smax, mu * u);
+      }
+      double smax_d = group_max<BS, BB>((double)smax, sh);
+      if (ltid<BS>() == 0) atomic_max_nonneg(run_slot(sig_max, bx), fmax(smax_d, 1e-100));
+
+      // CFL sum per cell (m_fluid.f90:194-195 summed over dimensions,
+      // m_af_flux_schemes.f90:779-848)
+      for (int q = ltid<BS>(); q < G::NINT; q += BS) {
+        int i, j, k;
+        interior_ijk<D, NC>(q, i, j, k);
+        TF cfl = 0;
+        for (int d = 0; d < D; d++) {
+          const TF inv_dx = TF(1) / (TF)bx.dr[d];
+          TF vv[2], dd[2];
+          for (int s = 0; s < 2; s++) {
+            int c[3] = {i, j, k};
+            int fp[3] = {i, j, (D == 3) ? k : 1};
+            fp[d] += s;  // faces c and c+1
+            const TF ex = (TF)efc[fbase + G::fidx(fp[0], fp[1], fp[2], d)];
+            int e0[3] = {c[0], c[1], c[2]};
+            e0[d] = c[d] - 1 + s;
+            int e1[3] = {c[0], c[1], c[2]};
+            e1[d] = c[d] + s;
+            const TF tmp_fc = TF(0.5) * ((TF)efield[base + G::idx(e0[0], e0[1], e0[2])] +
+                                         (TF)efield[base + G::idx(e1[0], e1[1], e1[2])]) *
+                              (TF)SI_to_Townsend * N_inv;
+            int lix;
+            TF lw;
+            lt_loc_t<TF>(td, tmp_fc, lix, lw);
+            const TF mu = lt_at_t<TF>(td, 0, lix, lw) * N_inv;
+            dd[s] = lt_at_t<TF>(td, 1, lix, lw) * N_inv;
+            vv[s] = -mu * ex;
+          }
+          const TF cl = TF(1) * fmax(fabs(vv[1]), fabs(vv[0])) * inv_dx + TF(2) * fmax(dd[1], dd[0]) * (inv_dx * inv_dx);
+          cfl = cfl + cl;
+        }
+        cmax = fmax(cmax, cfl);
+      }
     }
     const double cmax_d = group_max<BS, BB>((double)cmax, sh);
     if (ltid<BS>() == 0) atomic_max_nonneg(run_slot(cfl_max, bx), cmax_d);
