@@ -1644,7 +1644,6 @@ struct Impl : ImplBase {
                        (const double *)c.cc[AFGPU_CC_EFIELD], (const double *)c.cc[AFGPU_CC_ELECTRON],
                        c.cc[AFGPU_CC_RHS], c.td, c.p.gas_number_density, c.p.photoi_eta_quench);
     }));
-    RCHK(boxop(c.l_all_ids, c.cc[AFGPU_CC_PHOTO], c.cc[AFGPU_CC_PHOTO], OP_CLEAR));
     double max_rhs;
     RCHK(maxabs(AFGPU_CC_RHS, &max_rhs));
     max_rhs = std::max(max_rhs, std::sqrt(std::numeric_limits<double>::epsilon()));
@@ -1685,10 +1684,7 @@ struct Impl : ImplBase {
       // No final ghost refresh of the modes: the next solve forms its
       // residual with in-kernel ghosts, and the photoionization source is
       // read on interior cells only (src/m_fluid.f90:437-439).
-      for (int n = 0; n < n_modes; n++)
-        RCHK(boxop(c.l_all_leaves, c.cc[op_phi(AFGPU_OP_HELMH + n)], c.cc[AFGPU_CC_PHOTO], OP_AXPY_NEG,
-                   c.p.photoi_coeff[n]));
-      return AFGPU_OK;
+      return photo_sum();
     }
     for (int n = 0; n < n_modes; n++) {
       const int op = AFGPU_OP_HELMH + n;
@@ -1698,9 +1694,25 @@ struct Impl : ImplBase {
         RCHK(leaf_residual_max(&residu));
         if (residu / max_rhs < c.p.photoi_max_rel_residual) break;
       }
-      RCHK(boxop(c.l_all_leaves, c.cc[op_phi(op)], c.cc[AFGPU_CC_PHOTO], OP_AXPY_NEG, c.p.photoi_coeff[n]));
     }
-    return AFGPU_OK;
+    return photo_sum();
+  }
+
+  // photo := 0 on all boxes, then -= coeff_n * phi_n on the leaves for
+  // every mode in turn, in one pass (no mode solve reads photo)
+  int photo_sum() {
+    PhotoModes pm{};
+    pm.n = c.p.photoi_n_modes;
+    static_assert(PHOTO_MAX_MODES == AFGPU_MAX_MODES, "mode count");
+    for (int n = 0; n < pm.n; n++) {
+      pm.phi[n] = c.cc[op_phi(AFGPU_OP_HELMH + n)];
+      pm.coeff[n] = c.p.photoi_coeff[n];
+    }
+    return with_bb(KC_BOXOP, [&](auto B) {
+      constexpr int bb = decltype(B)::value;
+      return items<bb>(KC_BOXOP, k_photo_sum<G::NCELL, BS, bb>, c.l_all_ids.n, c.list(c.l_all_ids),
+                       (const DBox *)c.d_boxes, pm, c.cc[AFGPU_CC_PHOTO]);
+    });
   }
 
   int copy_states(bool backup) override {

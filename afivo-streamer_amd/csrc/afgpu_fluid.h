@@ -539,14 +539,12 @@ __global__ void __launch_bounds__(BS * BB) k_gradient(const int *__restrict__ id
     for (int d = 0; d < D; d++) inv_dr[d] = -1.0 / bx.dr[d];
     constexpr int NFD = (NC + 1) * G::NTAN;
     constexpr int st[3] = {1, G::NG, G::NG * G::NG};
+    // consecutive threads take consecutive x positions (contiguous cells
+    // and faces) in every direction
     for (int q = ltid<BS>(); q < D * NFD; q += BS) {
       const int d = q / NFD, r = q % NFD;
-      int t0, t1;
-      tangential(D, d, t0, t1);
-      int p[3] = {0, 0, 1};
-      p[d] = r % (NC + 1) + 1;
-      p[t0] = (r / (NC + 1)) % NC + 1;
-      if (D == 3) p[t1] = r / ((NC + 1) * NC) + 1;
+      const int ni = (d == 0) ? NC + 1 : NC, nj = (d == 1) ? NC + 1 : NC;
+      const int p[3] = {r % ni + 1, (r / ni) % nj + 1, (D == 3) ? r / (ni * nj) + 1 : 1};
       const int x = G::idx(p[0], p[1], D == 3 ? p[2] : 0);
       f[G::fidx(p[0], p[1], p[2], d)] = inv_dr[d] * (u[x] - u[x - st[d]]);
     }
@@ -592,6 +590,34 @@ __global__ void __launch_bounds__(BS * BB) k_photoi_src(const int *__restrict__ 
       double tmp = fld * mob * alpha * ne[x] * coeff;
       if (tmp < 0) tmp = 0;
       rhs[x] = tmp;
+    }
+  }
+}
+
+// The photoionization source from the Helmholtz modes' solutions
+// (photoi_helmh_compute, src/m_photoi_helmh.f90:162-204): on whole boxes,
+// zero on refined boxes and on leaves the modes' terms -c_n phi_n added
+// mode by mode, in the order and with the operations of the reference's
+// clear followed by one update per mode.
+constexpr int PHOTO_MAX_MODES = 3;
+struct PhotoModes {
+  const double *phi[PHOTO_MAX_MODES];
+  double coeff[PHOTO_MAX_MODES];
+  int n;
+};
+template <int NCELL, int BS, int BB = 1>
+__global__ void __launch_bounds__(BS * BB) k_photo_sum(const int *__restrict__ ids, const DBox *__restrict__ boxes,
+                                                       PhotoModes pm, double *photo, int nw) {
+  for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
+    const int id = ids[w];
+    if (id <= 0) continue;  // run masked out
+    const bool leaf = boxes[id - 1].children[0] <= 0;
+    const size_t base = (size_t)(id - 1) * NCELL;
+    for (int x = ltid<BS>(); x < NCELL; x += BS) {
+      double v = 0.0;
+      if (leaf)
+        for (int n = 0; n < pm.n; n++) v = v - pm.coeff[n] * pm.phi[n][base + x];
+      photo[base + x] = v;
     }
   }
 }
