@@ -78,6 +78,77 @@ class ParamsT(C.Structure):
     ]
 
 
+REF_MAX_ITEMS = 8
+REF_DEREFINE, REF_REFINE = -2, 2
+
+
+class RefineT(C.Structure):
+    """afgpu_refine_t: the refinement settings of src/m_refine.f90:8-83."""
+    _fields_ = [
+        ("refine_adx", C.c_double),
+        ("refine_adx_fac", C.c_double),
+        ("derefine_dx", C.c_double),
+        ("refine_min_dx", C.c_double),
+        ("refine_max_dx", C.c_double),
+        ("refine_min_dens", C.c_double),
+        ("refine_init_time", C.c_double),
+        ("refine_init_fac", C.c_double),
+        ("global_time", C.c_double),
+        ("use_alpha_effective", C.c_int),
+        ("buffer_width", C.c_int),
+        ("max_lvl", C.c_int),
+        ("n_seeds", C.c_int),
+        ("seed_r0", (C.c_double * 3) * REF_MAX_ITEMS),
+        ("seed_r1", (C.c_double * 3) * REF_MAX_ITEMS),
+        ("seed_width", C.c_double * REF_MAX_ITEMS),
+        ("n_regions", C.c_int),
+        ("regions_dr", C.c_double * REF_MAX_ITEMS),
+        ("regions_tstop", C.c_double * REF_MAX_ITEMS),
+        ("regions_rmin", (C.c_double * 3) * REF_MAX_ITEMS),
+        ("regions_rmax", (C.c_double * 3) * REF_MAX_ITEMS),
+        ("n_limits", C.c_int),
+        ("limits_dr", C.c_double * REF_MAX_ITEMS),
+        ("limits_rmin", (C.c_double * 3) * REF_MAX_ITEMS),
+        ("limits_rmax", (C.c_double * 3) * REF_MAX_ITEMS),
+    ]
+
+
+def refine_settings(**kw):
+    """A RefineT with the reference's defaults (src/m_refine.f90:8-50:
+    buffer 4, min dx 1e-7, max dx 1e-3, adx 1, adx factor 1, derefine dx
+    1e-4, init time 10 ns, init factor 0.25, af_max_lvl 30) and no seeds,
+    regions or limits; keyword arguments override fields, and seeds /
+    regions / limits are given as lists of tuples (r0, r1, width),
+    (dr, tstop, rmin, rmax) and (dr, rmin, rmax)."""
+    r = RefineT()
+    r.refine_adx, r.refine_adx_fac, r.derefine_dx = 1.0, 1.0, 1e-4
+    r.refine_min_dx, r.refine_max_dx, r.refine_min_dens = 1e-7, 1e-3, -1e99
+    r.refine_init_time, r.refine_init_fac, r.global_time = 10e-9, 0.25, 0.0
+    r.use_alpha_effective, r.buffer_width, r.max_lvl = 0, 4, 30
+    for k, v in kw.items():
+        if k == "seeds":
+            r.n_seeds = len(v)
+            for n, (r0, r1, w) in enumerate(v):
+                for d in range(len(r0)):
+                    r.seed_r0[n][d], r.seed_r1[n][d] = r0[d], r1[d]
+                r.seed_width[n] = w
+        elif k == "regions":
+            r.n_regions = len(v)
+            for n, (dr, tstop, rmin, rmax) in enumerate(v):
+                r.regions_dr[n], r.regions_tstop[n] = dr, tstop
+                for d in range(len(rmin)):
+                    r.regions_rmin[n][d], r.regions_rmax[n][d] = rmin[d], rmax[d]
+        elif k == "limits":
+            r.n_limits = len(v)
+            for n, (dr, rmin, rmax) in enumerate(v):
+                r.limits_dr[n] = dr
+                for d in range(len(rmin)):
+                    r.limits_rmin[n][d], r.limits_rmax[n][d] = rmin[d], rmax[d]
+        else:
+            setattr(r, k, v)
+    return r
+
+
 class StepInfoT(C.Structure):
     _fields_ = [
         ("it", C.c_int),
@@ -464,6 +535,13 @@ class Path:
         info = StepInfoT()
         self._call("step", C.byref(info))
         return info.as_dict()
+
+    def refine_flags(self, settings, n_boxes):
+        """The reference's box refinement flags (af_adjust_refinement with
+        default_refinement) for box ids 1..n_boxes."""
+        out = np.zeros(n_boxes, np.int32)
+        self._call("refine_flags", C.byref(settings), n_boxes, _ip(out))
+        return out
 
     def get_time_state(self):
         it, t, dt = C.c_int(), C.c_double(), C.c_double()

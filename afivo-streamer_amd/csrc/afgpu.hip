@@ -26,6 +26,7 @@
 #include "afgpu.h"
 #include "afgpu_direct.h"
 #include "afgpu_fluid.h"
+#include "afgpu_refine.h"
 
 using namespace afg;
 
@@ -109,6 +110,7 @@ struct ImplBase {
                             int s_out, int i_step, int n_steps, double *dt_lim) = 0;
   virtual int photoi() = 0;
   virtual int copy_states(bool backup) = 0;
+  virtual int refine_cells(const afgpu_refine_t &rp, const int *ids, int n_ids, int *flags) = 0;
   virtual int restrict_tree(int iv) = 0;
   virtual int step_runs(afgpu_step_info_t *info) = 0;
 };
@@ -200,6 +202,9 @@ struct afgpu_ctx {
   bool flux32 = true;  // mixed precision: FP32 flux stencil (FP64 accumulation)
   bool fuse_direct = false;  // uniform-level solve: last-axis transforms fused into one pass
   bool direct_f32 = true;    // uniform-level solve of the FP32 inner cycle in FP32 (matrix cores)
+  // refinement flags: the boxes evaluated and their flags (device)
+  int *d_ref_ids = nullptr, *d_ref_flags = nullptr;
+  int ref_cap = 0;
   struct Direct {
     bool set = false;  // lc and the map below are current for the tree
     int lc = 1, N = 0;
@@ -1715,6 +1720,15 @@ struct Impl : ImplBase {
     });
   }
 
+  int refine_cells(const afgpu_refine_t &rp, const int *ids, int n_ids, int *flags) override {
+    if (n_ids <= 0) return AFGPU_OK;
+    hipLaunchKernelGGL((k_refine_cells<D, NC, BS>), dim3(std::min(n_ids, 8 * c.n_cu)), dim3(BS), 0, c.cur, ids,
+                       (const DBox *)c.d_boxes, (const double *)c.cc[AFGPU_CC_EFIELD],
+                       (const double *)c.cc[AFGPU_CC_ELECTRON], rp, c.td, c.p.gas_number_density, flags, n_ids);
+    HIPCHK(hipGetLastError());
+    return AFGPU_OK;
+  }
+
   int copy_states(bool backup) override {
     const int ns = c.n_states();
     for (int sp = 0; sp < 3; sp++) {
@@ -2178,6 +2192,8 @@ int afgpu_destroy(afgpu_t *h) {
   for (int i = 0; i < AFGPU_N_FC; i++) (void)hipFree(h->fc[i]);
   for (int i = 0; i < 4; i++) (void)hipFree(h->cc32[i]);
   (void)hipFree(h->dir.d_map);
+  (void)hipFree(h->d_ref_ids);
+  (void)hipFree(h->d_ref_flags);
   (void)hipFree(h->dir.A0);
   (void)hipFree(h->dir.A1);
   for (auto &f : h->dir.f) {
@@ -2741,6 +2757,97 @@ int afgpu_photoi_set_src(afgpu_t *h) {
   }
   RCHK(h->impl->photoi());
   return sync_stream(h);
+}
+
+// consistent_ref_flags (afivo/src/m_af_core.f90:929-1013): the cell flags
+// of every leaf and of every box with a leaf child (the reference evaluates
+// such a box once, from its first leaf child) on the device, the rest here
+// in the reference's order.
+int afgpu_refine_flags(afgpu_t *h, const afgpu_refine_t *rp, int n_boxes, int *ref_flags) {
+  RCHK(ready(h, true));
+  if (!rp || !ref_flags || n_boxes != (int)h->hboxes.size() || rp->buffer_width < 0 ||
+      rp->buffer_width > h->nc || rp->n_seeds < 0 || rp->n_seeds > AFGPU_REF_MAX_ITEMS || rp->n_regions < 0 ||
+      rp->n_regions > AFGPU_REF_MAX_ITEMS || rp->n_limits < 0 || rp->n_limits > AFGPU_REF_MAX_ITEMS) {
+    set_err("afgpu_refine_flags: invalid arguments");
+    return AFGPU_ERR_ARG;
+  }
+  const std::vector<DBox> &hb = h->hboxes;
+  auto has_children = [&](int id) { return hb[id - 1].children[0] > 0; };
+  const int nch = 1 << h->D, nnb = 2 * h->D;
+  std::vector<int> eval;
+  for (size_t l = 1; l < h->src_leaves.size(); l++) eval.insert(eval.end(), h->src_leaves[l].begin(), h->src_leaves[l].end());
+  for (size_t l = 1; l < h->src_parents.size(); l++)
+    for (int id : h->src_parents[l]) {
+      bool leaf_child = false;
+      for (int q = 0; q < nch; q++) leaf_child = leaf_child || !has_children(hb[id - 1].children[q]);
+      if (leaf_child) eval.push_back(id);
+    }
+  if (n_boxes > h->ref_cap) {
+    (void)hipFree(h->d_ref_ids);
+    (void)hipFree(h->d_ref_flags);
+    h->d_ref_ids = h->d_ref_flags = nullptr;
+    h->ref_cap = 0;
+    HIPCHK(hipMalloc(&h->d_ref_ids, (size_t)n_boxes * sizeof(int)));
+    HIPCHK(hipMalloc(&h->d_ref_flags, (size_t)n_boxes * sizeof(int)));
+    h->ref_cap = n_boxes;
+  }
+  const int unset = std::numeric_limits<int>::min();
+  h->cur = h->stream;
+  HIPCHK(hipMemcpyAsync(h->d_ref_ids, eval.data(), eval.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(k_fill_int, dim3((n_boxes + 255) / 256), dim3(256), 0, h->stream, h->d_ref_flags, n_boxes, unset);
+  HIPCHK(hipGetLastError());
+  RCHK(h->impl->refine_cells(*rp, h->d_ref_ids, (int)eval.size(), h->d_ref_flags));
+  std::vector<int> f(n_boxes);
+  HIPCHK(hipMemcpyAsync(f.data(), h->d_ref_flags, (size_t)n_boxes * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  RCHK(sync_stream(h));
+  // unset flags, the level cap (:987-1000)
+  for (int &x : f)
+    if (x == unset) x = REF_KEEP;
+  const int hl = (int)h->src_ids.size() - 1;
+  if (rp->max_lvl >= 1 && rp->max_lvl <= hl)
+    for (int id : h->src_ids[rp->max_lvl])
+      if (f[id - 1] == REF_DO) f[id - 1] = REF_KEEP;
+  // ensure_two_one_balance (:1016-1057)
+  for (int lvl = hl; lvl >= 1; lvl--)
+    for (int id : h->src_leaves[lvl]) {
+      int &x = f[id - 1];
+      const DBox &b = hb[id - 1];
+      if (x == REF_DO || x == AFGPU_REF_REFINE) {
+        x = AFGPU_REF_REFINE;
+        for (int nb = 0; nb < nnb; nb++)
+          if (b.nb[nb] == 0) f[hb[b.parent - 1].nb[nb] - 1] = AFGPU_REF_REFINE;
+      } else if (x == REF_RM) {
+        for (int nb = 0; nb < nnb; nb++) {
+          const int nb_id = b.nb[nb];
+          if (nb_id > 0 && (has_children(nb_id) || f[nb_id - 1] > REF_KEEP)) {
+            x = REF_KEEP;
+            break;
+          }
+        }
+      }
+    }
+  // handle_derefinement_flags (:1059-1088)
+  for (int lvl = hl - 1; lvl >= 1; lvl--)
+    for (int id : h->src_parents[lvl]) {
+      const DBox &b = hb[id - 1];
+      bool all_refined = true, all_rm = true;
+      for (int q = 0; q < nch; q++) {
+        all_refined = all_refined && has_children(b.children[q]);
+        all_rm = all_rm && f[b.children[q] - 1] == REF_RM;
+      }
+      if (all_refined) continue;
+      if (all_rm && f[id - 1] <= REF_KEEP) {
+        f[id - 1] = AFGPU_REF_DEREFINE;
+      } else {
+        f[id - 1] = REF_KEEP;
+        for (int q = 0; q < nch; q++) {
+          int &x = f[b.children[q] - 1];
+          if (x != AFGPU_REF_DEREFINE) x = std::max(x, REF_KEEP);
+        }
+      }
+    }
+  std::copy(f.begin(), f.end(), ref_flags);
+  return AFGPU_OK;
 }
 
 int afgpu_copy_current_state(afgpu_t *h) {

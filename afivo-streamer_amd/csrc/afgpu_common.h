@@ -154,6 +154,13 @@ __host__ __device__ inline void colour_cell(int c, int q, int &i, int &j, int &k
   if (((i + j + k) & 1) != c) i += 1;
 }
 
+// Position of the neighbour (di, dj, dk) in DBox::nbmat, the reference's
+// neighbor_mat(-1:1, -1:1[, -1:1]) in column-major order (3^D entries).
+template <int D>
+__host__ __device__ inline int nbmat_at(int di, int dj, int dk) {
+  return (di + 1) + 3 * (dj + 1) + (D == 3 ? 9 * (dk + 1) : 0);
+}
+
 __host__ __device__ inline void tangential(int D, int d, int &t0, int &t1) {
   if (D == 2) {
     t0 = 1 - d;

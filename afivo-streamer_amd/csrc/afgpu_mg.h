@@ -281,7 +281,7 @@ __device__ inline void gc_edge_cell(const DBox &bx, T *var, int id, int n, int q
   int lo[3];
   for (int dd = 0; dd < 3; dd++) lo[dd] = (dir[dd] > 0) ? NC + 1 : 0;
   lo[dim] = qq;
-  const int nb_id = bx.nbmat[(dir[0] + 1) + 3 * (dir[1] + 1) + 9 * (dir[2] + 1)];
+  const int nb_id = bx.nbmat[nbmat_at<D>(dir[0], dir[1], dir[2])];
   if (nb_id > 0) {
     const T *w = var + (size_t)(nb_id - 1) * L::STRIDE;
     u[L::at(lo[0], lo[1], lo[2])] = w[L::at(lo[0] - dir[0] * NC, lo[1] - dir[1] * NC, lo[2] - dir[2] * NC)];
@@ -306,7 +306,7 @@ __device__ inline void gc_corner_cell(const DBox &bx, T *var, int id, int n) {
   const int cx = n & 1, cy = (n >> 1) & 1, cz = (n >> 2) & 1;
   const int dnb[3] = {2 * cx - 1, 2 * cy - 1, (D == 3) ? 2 * cz - 1 : 0};
   const int lo[3] = {cx * (NC + 1), cy * (NC + 1), (D == 3) ? cz * (NC + 1) : 0};
-  const int nb_id = bx.nbmat[(dnb[0] + 1) + 3 * (dnb[1] + 1) + 9 * (dnb[2] + 1)];
+  const int nb_id = bx.nbmat[nbmat_at<D>(dnb[0], dnb[1], dnb[2])];
   if (nb_id > 0) {
     const T *w = var + (size_t)(nb_id - 1) * L::STRIDE;
     u[L::at(lo[0], lo[1], lo[2])] = w[L::at(lo[0] - dnb[0] * NC, lo[1] - dnb[1] * NC, lo[2] - dnb[2] * NC)];
@@ -1415,7 +1415,7 @@ __global__ void __launch_bounds__(BS * BB) k_gc_lds(const int *__restrict__ ids,
           int lo[3];
           for (int dd = 0; dd < 3; dd++) lo[dd] = (dir[dd] > 0) ? NC + 1 : 0;
           lo[dim] = qq;
-          const int nb_id = bx.nbmat[(dir[0] + 1) + 3 * (dir[1] + 1) + 9 * (dir[2] + 1)];
+          const int nb_id = bx.nbmat[nbmat_at<D>(dir[0], dir[1], dir[2])];
           T v;
           if (nb_id > 0) {
             const T *w = var + (size_t)(nb_id - 1) * L::STRIDE;
@@ -1439,7 +1439,7 @@ __global__ void __launch_bounds__(BS * BB) k_gc_lds(const int *__restrict__ ids,
         const int cx = n & 1, cy = (n >> 1) & 1, cz = (n >> 2) & 1;
         const int dnb[3] = {2 * cx - 1, 2 * cy - 1, (D == 3) ? 2 * cz - 1 : 0};
         const int lo[3] = {cx * (NC + 1), cy * (NC + 1), (D == 3) ? cz * (NC + 1) : 0};
-        const int nb_id = bx.nbmat[(dnb[0] + 1) + 3 * (dnb[1] + 1) + 9 * (dnb[2] + 1)];
+        const int nb_id = bx.nbmat[nbmat_at<D>(dnb[0], dnb[1], dnb[2])];
         T v;
         if (nb_id > 0) {
           const T *w = var + (size_t)(nb_id - 1) * L::STRIDE;

@@ -75,7 +75,7 @@ struct afhost_tree {
               v = (int)b + 1;
             else
               v = inside(x.lvl, q) ? find(x.lvl, q) : -1;
-            x.nbmat[(di + 1) + 3 * (dj + 1) + 9 * (dk + 1)] = v;
+            x.nbmat[(di + 1) + 3 * (dj + 1) + (D == 3 ? 9 * (dk + 1) : 0)] = v;
           }
     }
   }

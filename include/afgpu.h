@@ -215,6 +215,47 @@ int afgpu_forward_euler(afgpu_t *h, double dt, int s_deriv, int n_prev,
  * forward_euler) (afivo/src/m_af_advance.f90:121). */
 int afgpu_advance(afgpu_t *h, double dt, double *dt_lim);
 
+/* ---- refinement flags --------------------------------------------------
+ * The box refinement flags that af_adjust_refinement
+ * (afivo/src/m_af_core.f90:697-735) derives with the streamer's
+ * default_refinement (src/m_refine.f90:198-298), from the device's |E| and
+ * n_e, so that only the flags (one int per box) leave the device:
+ * consistent_ref_flags (m_af_core.f90:929-1013) evaluates every leaf and
+ * every box with a leaf child, cell_to_ref_flags (:1091-1145) turns the
+ * cell flags into box flags and, within buffer_width cells of a box side,
+ * into refinement of the neighbour there; then the level cap,
+ * ensure_two_one_balance (:1016-1057) and handle_derefinement_flags
+ * (:1059-1088).  On return ref_flags[id - 1] holds the reference's final
+ * flag of box id: AFGPU_REF_REFINE and AFGPU_REF_DEREFINE are acted on,
+ * the other values (-1, 0, 1) ignored, as there.  The densities of refined
+ * boxes are read as they are: restrict n_e first (afgpu_restrict_tree), as
+ * src/streamer.f90:399-400 does.  Boxes tagged for an electrode
+ * (m_refine.f90:267-270) do not occur on this path. */
+#define AFGPU_REF_DEREFINE (-2)
+#define AFGPU_REF_REFINE   2
+#define AFGPU_REF_MAX_ITEMS 8
+typedef struct {
+  double refine_adx;       /* refine if alpha * dx > refine_adx */
+  double refine_adx_fac;   /* alpha(f E) / f */
+  double derefine_dx;      /* derefine only below this spacing */
+  double refine_min_dx, refine_max_dx, refine_min_dens;
+  double refine_init_time, refine_init_fac;
+  double global_time;
+  int use_alpha_effective; /* alpha - eta */
+  int buffer_width;        /* refine_buffer_width, 0 .. n_cell */
+  int max_lvl;             /* af_max_lvl (30) */
+  int n_seeds;             /* seeds of the initial conditions (init_conds) */
+  double seed_r0[AFGPU_REF_MAX_ITEMS][3], seed_r1[AFGPU_REF_MAX_ITEMS][3];
+  double seed_width[AFGPU_REF_MAX_ITEMS];
+  int n_regions;           /* refine_regions_* */
+  double regions_dr[AFGPU_REF_MAX_ITEMS], regions_tstop[AFGPU_REF_MAX_ITEMS];
+  double regions_rmin[AFGPU_REF_MAX_ITEMS][3], regions_rmax[AFGPU_REF_MAX_ITEMS][3];
+  int n_limits;            /* refine_limits_* */
+  double limits_dr[AFGPU_REF_MAX_ITEMS];
+  double limits_rmin[AFGPU_REF_MAX_ITEMS][3], limits_rmax[AFGPU_REF_MAX_ITEMS][3];
+} afgpu_refine_t;
+int afgpu_refine_flags(afgpu_t *h, const afgpu_refine_t *rp, int n_boxes, int *ref_flags);
+
 /* photoi_set_src (src/m_photoi.f90:153) with the Helmholtz method
  * (src/m_photoi_helmh.f90:162). */
 int afgpu_photoi_set_src(afgpu_t *h);

@@ -40,7 +40,9 @@ static const int edge_min_ix3[12][3] = {{0, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 1, 
                                         {0, 0, 0}, {1, 0, 0}, {0, 0, 1}, {1, 0, 1},
                                         {0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0}};
 
+// neighbor_mat(-1:1, -1:1[, -1:1]) in column-major order (3^D entries)
 static inline int nbmat_index(int di, int dj, int dk) { return (di + 1) + 3 * (dj + 1) + 9 * (dk + 1); }
+static inline int nbmat_index2(int di, int dj) { return (di + 1) + 3 * (dj + 1); }
 
 static inline void tangential(int D, int d, int &t0, int &t1) {
   if (D == 2) {
@@ -342,7 +344,7 @@ static void gc_box_corner(Ctx &c, int id, int iv) {
   if (c.D == 2) {
     for (int n = 0; n < 4; n++) {
       int dnb[2] = {2 * child_dix2[n][0] - 1, 2 * child_dix2[n][1] - 1};
-      int nb_id = bx.nbmat[nbmat_index(dnb[0], dnb[1], 0)];
+      int nb_id = bx.nbmat[nbmat_index2(dnb[0], dnb[1])];
       int lo[2] = {child_dix2[n][0] * (nc + 1), child_dix2[n][1] * (nc + 1)};
       if (nb_id > 0) {
         u[c.idx(lo[0], lo[1], 0)] = c.C(iv, nb_id)[c.idx(lo[0] - dnb[0] * nc, lo[1] - dnb[1] * nc, 0)];
@@ -1745,6 +1747,222 @@ typedef struct afcpu_ctx afcpu_t;
     }                                       \
   } while (0)
 
+namespace afcpu {
+
+// ---- refinement flags -------------------------------------------------------
+// af_rm_ref, af_keep_ref, af_do_ref, af_derefine, af_refine
+// (afivo/src/m_af_types.f90:23-35)
+constexpr int REF_RM = -1, REF_KEEP = 0, REF_DO = 1, REF_DEREFINE = -2, REF_REFINE = 2;
+
+// GM_dist_vec_line / GM_dist_line (src/m_geometry.f90:23-51): distance of r
+// to the segment r0-r1; the length as the square root of the sum of squares
+static double dist_line(int D, const double *r, const double *r0, const double *r1) {
+  double len2 = 0, frac = 0;
+  for (int d = 0; d < D; d++) len2 = len2 + (r1[d] - r0[d]) * (r1[d] - r0[d]);
+  for (int d = 0; d < D; d++) frac = frac + (r[d] - r0[d]) * (r1[d] - r0[d]);
+  double v[3];
+  for (int d = 0; d < D; d++) {
+    if (frac <= 0)
+      v[d] = r[d] - r0[d];
+    else if (frac >= len2)
+      v[d] = r[d] - r1[d];
+    else
+      v[d] = r[d] - (r0[d] + frac / len2 * (r1[d] - r0[d]));
+  }
+  double s2 = 0;
+  for (int d = 0; d < D; d++) s2 = s2 + v[d] * v[d];
+  return std::sqrt(s2);
+}
+
+// default_refinement (src/m_refine.f90:198-298): the flags of the cells of
+// box id, stored i fastest.  The gas density is constant; |E| and n_e are
+// the box's own values.
+static void cell_flags(const Ctx &c, const afgpu_refine_t &rp, int id, std::vector<int> &cf) {
+  const Box &b = c.B(id);
+  const int D = c.D, nc = c.nc;
+  double min_dx = b.dr[0], max_dx = b.dr[0];
+  for (int d = 1; d < D; d++) {
+    min_dx = std::min(min_dx, b.dr[d]);
+    max_dx = std::max(max_dx, b.dr[d]);
+  }
+  const double gas = c.p.gas_number_density;
+  const double *E = &c.cc[AFGPU_CC_EFIELD][(size_t)(id - 1) * c.ncell];
+  const double *ne = &c.cc[AFGPU_CC_ELECTRON][(size_t)(id - 1) * c.ncell];
+  const int kmax = (D == 3) ? nc : 1;
+  int q = 0;
+  for (int k = 1; k <= kmax; k++)
+    for (int j = 1; j <= nc; j++)
+      for (int i = 1; i <= nc; i++, q++) {
+        const long x = c.idx(i, j, D == 3 ? k : 0);
+        const double fld = E[x] * SI_to_Townsend / gas;
+        double alpha;
+        if (rp.use_alpha_effective) {
+          alpha = (c.td.col(2, rp.refine_adx_fac * fld) - c.td.col(3, rp.refine_adx_fac * fld)) * gas /
+                  rp.refine_adx_fac;
+          alpha = std::max(0.0, alpha);
+        } else {
+          alpha = c.td.col(2, rp.refine_adx_fac * fld) * gas / rp.refine_adx_fac;
+        }
+        const double adx = max_dx * alpha;
+        int f;
+        if (adx > rp.refine_adx && ne[x] > rp.refine_min_dens)
+          f = REF_DO;
+        else if (adx < 0.125 * rp.refine_adx && max_dx < rp.derefine_dx)
+          f = REF_RM;
+        else
+          f = REF_KEEP;
+        if (rp.global_time < rp.refine_init_time) {
+          const int ijk[3] = {i, j, k};
+          double r[3];
+          for (int d = 0; d < D; d++) r[d] = b.r_min[d] + (ijk[d] - 0.5) * b.dr[d];
+          for (int n = 0; n < rp.n_seeds; n++) {
+            const double dist = dist_line(D, r, rp.seed_r0[n], rp.seed_r1[n]);
+            if (dist - rp.seed_width[n] < 2 * max_dx && max_dx > rp.refine_init_fac * rp.seed_width[n]) f = REF_DO;
+          }
+        }
+        cf[q] = f;
+      }
+  double rmin[3], rmax[3];
+  for (int d = 0; d < D; d++) {
+    rmin[d] = b.r_min[d];
+    rmax[d] = b.r_min[d] + b.dr[d] * nc;
+  }
+  for (int n = 0; n < rp.n_regions; n++) {
+    bool all = true;
+    for (int d = 0; d < D; d++) all = all && rmax[d] >= rp.regions_rmin[n][d] && rmin[d] <= rp.regions_rmax[n][d];
+    if (rp.global_time <= rp.regions_tstop[n] && max_dx > rp.regions_dr[n] && all) {
+      const int h = nc / 2 - 1;  // the centre cell (nc/2, nc/2[, nc/2])
+      cf[h + nc * (h + (D == 3 ? nc * h : 0))] = REF_DO;
+    }
+  }
+  for (int n = 0; n < rp.n_limits; n++) {
+    bool all = true;
+    for (int d = 0; d < D; d++) all = all && rmin[d] >= rp.limits_rmin[n][d] && rmax[d] <= rp.limits_rmax[n][d];
+    if (max_dx < 2 * rp.limits_dr[n] && all)
+      for (int &f : cf)
+        if (f == REF_DO) f = REF_KEEP;
+  }
+  if (max_dx > rp.refine_max_dx) {
+    for (int &f : cf) f = REF_DO;
+  } else if (min_dx < 2 * rp.refine_min_dx) {
+    for (int &f : cf)
+      if (f == REF_DO) f = REF_KEEP;
+  }
+}
+
+// cell_to_ref_flags (afivo/src/m_af_core.f90:1091-1145)
+static void cell_to_ref_flags(const Ctx &c, const std::vector<int> &cf, int buffer, int id, std::vector<int> &flags) {
+  const int D = c.D, nc = c.nc;
+  bool any_do = false, any_keep = false;
+  for (int f : cf) {
+    any_do = any_do || f == REF_DO;
+    any_keep = any_keep || f == REF_KEEP;
+  }
+  int &fl = flags[id - 1];
+  if (any_do)
+    fl = REF_DO;
+  else if (any_keep)
+    fl = std::max(fl, REF_KEEP);
+  else
+    fl = std::max(fl, REF_RM);
+  if (buffer <= 0) return;
+  const Box &b = c.B(id);
+  const int kmin = (D == 3) ? -1 : 0, kmax = (D == 3) ? 1 : 0;
+  for (int dk = kmin; dk <= kmax; dk++)
+    for (int dj = -1; dj <= 1; dj++)
+      for (int di = -1; di <= 1; di++) {
+        if (di == 0 && dj == 0 && dk == 0) continue;
+        const int nb_id = (D == 3) ? b.nbmat[nbmat_index(di, dj, dk)] : b.nbmat[nbmat_index2(di, dj)];
+        if (nb_id <= 0) continue;
+        const int dir[3] = {di, dj, dk};
+        int lo[3] = {1, 1, 1}, hi[3] = {nc, nc, (D == 3) ? nc : 1};
+        for (int d = 0; d < D; d++) {
+          if (dir[d] == 1) lo[d] = nc - buffer + 1;
+          if (dir[d] == -1) hi[d] = buffer;
+        }
+        bool hit = false;
+        for (int k = lo[2]; k <= hi[2] && !hit; k++)
+          for (int j = lo[1]; j <= hi[1] && !hit; j++)
+            for (int i = lo[0]; i <= hi[0] && !hit; i++)
+              hit = cf[(i - 1) + nc * ((j - 1) + nc * (k - 1))] == REF_DO;
+        if (hit) flags[nb_id - 1] = REF_DO;
+      }
+}
+
+// consistent_ref_flags (afivo/src/m_af_core.f90:929-1013) with
+// ensure_two_one_balance (:1016-1057) and handle_derefinement_flags
+// (:1059-1088)
+static void refine_flags(const Ctx &c, const afgpu_refine_t &rp, std::vector<int> &flags) {
+  const int unset = std::numeric_limits<int>::min();
+  flags.assign(c.n_boxes, unset);
+  std::vector<int> cf((size_t)c.nc * c.nc * (c.D == 3 ? c.nc : 1));
+  const int nch = 1 << c.D;
+  for (int lvl = 1; lvl <= c.highest_lvl; lvl++)
+    for (int id : c.leaves[lvl]) {
+      cell_flags(c, rp, id, cf);
+      cell_to_ref_flags(c, cf, rp.buffer_width, id, flags);
+      const int p = c.B(id).parent;
+      if (c.B(id).lvl > 1 && p > 0) {
+        // the parent, once, from its first child that is itself a leaf
+        const Box &bp = c.B(p);
+        int ich = 0;
+        while (ich < nch && bp.children[ich] != id) ich++;
+        int i_ch = 0;
+        while (i_ch < ich && c.has_children(bp.children[i_ch])) i_ch++;
+        if (i_ch == ich) {
+          cell_flags(c, rp, p, cf);
+          cell_to_ref_flags(c, cf, rp.buffer_width, p, flags);
+        }
+      }
+    }
+  for (int &f : flags)
+    if (f == unset) f = REF_KEEP;
+  if (rp.max_lvl >= 1 && rp.max_lvl <= c.highest_lvl)
+    for (int id : c.ids[rp.max_lvl])
+      if (flags[id - 1] == REF_DO) flags[id - 1] = REF_KEEP;
+  // ensure_two_one_balance
+  for (int lvl = c.highest_lvl; lvl >= 1; lvl--)
+    for (int id : c.leaves[lvl]) {
+      int &f = flags[id - 1];
+      const Box &b = c.B(id);
+      if (f == REF_DO || f == REF_REFINE) {
+        f = REF_REFINE;
+        for (int nb = 0; nb < 2 * c.D; nb++)
+          if (b.nb[nb] == 0) flags[c.B(b.parent).nb[nb] - 1] = REF_REFINE;
+      } else if (f == REF_RM) {
+        for (int nb = 0; nb < 2 * c.D; nb++) {
+          const int nb_id = b.nb[nb];
+          if (nb_id > 0 && (c.has_children(nb_id) || flags[nb_id - 1] > REF_KEEP)) {
+            f = REF_KEEP;
+            break;
+          }
+        }
+      }
+    }
+  // handle_derefinement_flags
+  for (int lvl = c.highest_lvl - 1; lvl >= 1; lvl--)
+    for (int id : c.parents[lvl]) {
+      const Box &b = c.B(id);
+      bool all_refined = true, all_rm = true;
+      for (int q = 0; q < nch; q++) {
+        all_refined = all_refined && c.has_children(b.children[q]);
+        all_rm = all_rm && flags[b.children[q] - 1] == REF_RM;
+      }
+      if (all_refined) continue;
+      if (all_rm && flags[id - 1] <= REF_KEEP) {
+        flags[id - 1] = REF_DEREFINE;
+      } else {
+        flags[id - 1] = REF_KEEP;
+        for (int q = 0; q < nch; q++) {
+          int &fc = flags[b.children[q] - 1];
+          if (fc != REF_DEREFINE) fc = std::max(fc, REF_KEEP);
+        }
+      }
+    }
+}
+
+}  // namespace afcpu
+
 extern "C" {
 
 const char *afcpu_last_error(void) { return g_err.c_str(); }
@@ -1978,6 +2196,25 @@ int afcpu_photoi_set_src(afcpu_t *h) {
       return AFGPU_ERR_STATE;
     }
   return photoi_set_src(h->c);
+}
+
+int afcpu_refine_flags(afcpu_t *h, const afgpu_refine_t *rp, int n_boxes, int *ref_flags) {
+  int rc = ready(h, -1);
+  if (rc) return rc;
+  if (!rp || !ref_flags || n_boxes != h->c.n_boxes || rp->buffer_width < 0 || rp->buffer_width > h->c.nc ||
+      rp->n_seeds < 0 || rp->n_seeds > AFGPU_REF_MAX_ITEMS || rp->n_regions < 0 ||
+      rp->n_regions > AFGPU_REF_MAX_ITEMS || rp->n_limits < 0 || rp->n_limits > AFGPU_REF_MAX_ITEMS) {
+    set_error("afcpu_refine_flags: invalid arguments");
+    return AFGPU_ERR_ARG;
+  }
+  if (h->c.td.n < 2) {
+    set_error("afcpu_refine_flags: tables not set");
+    return AFGPU_ERR_STATE;
+  }
+  std::vector<int> f;
+  refine_flags(h->c, *rp, f);
+  std::copy(f.begin(), f.end(), ref_flags);
+  return AFGPU_OK;
 }
 
 int afcpu_copy_current_state(afcpu_t *h) {

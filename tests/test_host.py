@@ -22,6 +22,42 @@ def test_tree_invariants(A, S):
     assert case.tree.info()["highest_lvl"] == 4
 
 
+import pytest
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_neighbour_matrix_layout(A, S, D):
+    """neighbor_mat(-1:1, -1:1[, -1:1]) is exported with 3^D entries per box
+    in column-major order, (di+1) + 3(dj+1) [+ 9(dk+1)]: the centre is the
+    box itself, every same-level box whose index differs by (di, dj[, dk])
+    appears at that position, and the face entries agree with the
+    neighbour list."""
+    case = S.small_amr(D, nc=8, uniform_lvl=2, max_lvl=4, frac=0.4, seed=13)
+    t = case.extra["export"]
+    n, nm, nnb = t["n_boxes"], 3 ** D, 2 * D
+    where = {(int(t["lvl"][b]), tuple(int(v) for v in t["ix"][b * D:(b + 1) * D])): b + 1 for b in range(n)}
+    dirs = [(di, dj, dk) for dk in ((-1, 0, 1) if D == 3 else (0,)) for dj in (-1, 0, 1) for di in (-1, 0, 1)]
+    n_diag = 0
+    for b in range(n):
+        mat = t["nbmat"][b * nm:(b + 1) * nm]
+        lvl, ix = int(t["lvl"][b]), [int(v) for v in t["ix"][b * D:(b + 1) * D]]
+        for d in dirs:
+            q = (d[0] + 1) + 3 * (d[1] + 1) + (9 * (d[2] + 1) if D == 3 else 0)
+            if d == (0, 0, 0):
+                assert mat[q] == b + 1
+                continue
+            other = where.get((lvl, tuple(ix[k] + d[k] for k in range(D))))
+            if other is not None:
+                assert mat[q] == other, (b + 1, d, mat[q], other)
+                n_diag += sum(1 for k in range(D) if d[k] != 0) > 1
+            else:
+                assert mat[q] <= 0, (b + 1, d)
+            if sum(1 for k in range(D) if d[k] != 0) == 1:  # a face
+                k = [k for k in range(D) if d[k] != 0][0]
+                assert mat[q] == t["neighbors"][b * nnb + 2 * k + (d[k] > 0)], (b + 1, d)
+    assert n_diag > 0
+
+
 def test_initial_conditions_independent_of_box_order(A, S):
     case = S.small_amr(2, nc=8, uniform_lvl=2, max_lvl=3, frac=0.5, seed=2)
     ids = np.arange(1, case.extra["export"]["n_boxes"] + 1, dtype=np.int32)

@@ -537,6 +537,38 @@ def test_field_of_linear_potential(A, S, D, coord):
                 assert np.abs(core[tuple(idx)] - V / L).max() <= 1e-9 * V / L, (bid, d, side)
 
 
+def test_corner_ghosts_copy_the_diagonal_neighbour_2d(A, S):
+    """af_gc_box_corner (afivo/src/m_af_ghostcell.f90:125-170): a 2D corner
+    ghost whose diagonal neighbour exists is that neighbour's corner cell;
+    otherwise it is the extrapolation a + b - c from the box's own cells."""
+    case = uniform_case(S, A, 2, A.XYZ, 3)
+    p = oracle(A, S, case)
+    t = case.extra["export"]
+    n, nc = t["n_boxes"], case.nc
+    rng = np.random.default_rng(3)
+    vals = rng.standard_normal((n, (nc + 2) ** 2))
+    p.put_cc(A.CC_PHI, vals)
+    p.gc_tree(A.CC_PHI)
+    u = p.get_cc(A.CC_PHI)
+    checked = 0
+    for b in range(n):
+        mat = t["nbmat"][b * 9:(b + 1) * 9]
+        v = box_view(u[b], nc, 2)  # (j, i)
+        for cx in (0, 1):
+            for cy in (0, 1):
+                di, dj = 2 * cx - 1, 2 * cy - 1
+                i, j = cx * (nc + 1), cy * (nc + 1)
+                nb = mat[(di + 1) + 3 * (dj + 1)]
+                if nb > 0:
+                    w = box_view(u[nb - 1], nc, 2)
+                    assert v[j, i] == w[j - dj * nc, i - di * nc], (b + 1, cx, cy)
+                    checked += 1
+                else:
+                    si, sj = 1 - 2 * (i & 1), 1 - 2 * (j & 1)
+                    assert v[j, i] == v[j, i + si] + v[j + sj, i] - v[j + sj, i + si], (b + 1, cx, cy)
+    assert checked > 0
+
+
 # ---- Heun's method and the time-step limits -------------------------------------
 def uniform_chemistry_case(A, S, integrator):
     """Neutral uniform plasma in a uniform field: the charge density stays
