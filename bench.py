@@ -131,6 +131,43 @@ def run_gpu(case, precision, warmup, steps, timing_steps, direct_n=None, smooth=
     return p, infos, t1 - t0, stats
 
 
+def measure_refine(case, p, c=None, reps=10):
+    """The refinement flags of the workload (afgpu_refine_flags, with the
+    reference's default settings) on the device state, against the download
+    of |E| and n_e that the flags replace and, given the oracle handle c,
+    against the oracle computing the same flags on the host."""
+    t = case.extra["export"]
+    n = t["n_boxes"]
+    D, nc = case.D, case.nc
+    s = A.refine_settings(global_time=1.0)
+    p.restrict_tree(A.CC_ELECTRON)
+    f = p.refine_flags(s, n)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        f = p.refine_flags(s, n)
+    gpu_ms = (time.perf_counter() - t0) * 1e3 / reps
+    t0 = time.perf_counter()
+    for _ in range(3):
+        p.get_cc(A.CC_EFIELD)
+        p.get_cc(A.CC_ELECTRON)
+    dl_ms = (time.perf_counter() - t0) * 1e3 / 3
+    ch = t["children"].reshape(n, 2 ** D)
+    leaf = ch[:, 0] == 0
+    evaluated = int(leaf.sum()) + int(sum(1 for b in range(n) if not leaf[b] and leaf[ch[b] - 1].any()))
+    out = {"gpu_ms": round(gpu_ms, 3), "download_E_ne_ms": round(dl_ms, 3), "boxes_evaluated": evaluated,
+           "cells_evaluated": evaluated * nc ** D, "algo_bytes": evaluated * nc ** D * 16,
+           "refine": int((f == A.REF_REFINE).sum()), "derefine": int((f == A.REF_DEREFINE).sum()),
+           "timing": "wall time of the entry point (list upload, device pass, flags back, host passes), "
+                     "mean of %d calls" % reps}
+    if c is not None:
+        t0 = time.perf_counter()
+        c.refine_flags(s, n)
+        out["cpu_oracle_ms"] = round((time.perf_counter() - t0) * 1e3, 3)
+        out["cpu_oracle_cores"] = 1
+    log("  refinement flags", json.dumps(out))
+    return out
+
+
 def rel_err(a, b):
     d = a - b
     return float(np.abs(d).max() / max(np.abs(b).max(), 1e-300)), \
@@ -420,6 +457,7 @@ def main():
                "sample": "%d step(s) of the same 3D AMR workload (%d leaf cells) from the GPU state "
                          "after warmup, %.1f s" % (args.cpu_steps, info["leaf_cells"], csecs)}
         log("  cpu %.4e cell-steps/s (%.1f s)" % (cpu["value"], csecs))
+    extra["refine_flags"] = measure_refine(case, p, None if args.no_cpu else c)
 
     out = {
         "metric": "cell-steps/s, 3D AMR step (DD+MG Poisson+photoi), FP64 vs mixed at 1e-8",
