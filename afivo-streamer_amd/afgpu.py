@@ -543,6 +543,17 @@ class Path:
         self._call("refine_flags", C.byref(settings), n_boxes, _ip(out))
         return out
 
+    def restrict_boxes(self, ids):
+        """auto_restrict: the children of the listed boxes into them."""
+        ids = np.ascontiguousarray(ids, np.int32)
+        self._call("restrict_boxes", len(ids), _ip(ids))
+
+    def prolong_boxes(self, ids):
+        """auto_prolong: the listed new boxes from their parents, then their
+        ghost cells."""
+        ids = np.ascontiguousarray(ids, np.int32)
+        self._call("prolong_boxes", len(ids), _ip(ids))
+
     def get_time_state(self):
         it, t, dt = C.c_int(), C.c_double(), C.c_double()
         self._call("get_time_state", C.byref(it), C.byref(t), C.byref(dt))

@@ -111,6 +111,11 @@ struct ImplBase {
   virtual int photoi() = 0;
   virtual int copy_states(bool backup) = 0;
   virtual int refine_cells(const afgpu_refine_t &rp, const int *ids, int n_ids, int *flags) = 0;
+  // boxes removed or added by refinement: restriction from the listed
+  // children, prolongation into the listed new boxes, their ghost cells
+  virtual int restrict_ids(const int *child_ids, int n, int iv) = 0;
+  virtual int prolong_new(const int *ids, int n, int iv, bool limited) = 0;
+  virtual int gc_ids(const int *ids, int n, int iv) = 0;
   virtual int restrict_tree(int iv) = 0;
   virtual int step_runs(afgpu_step_info_t *info) = 0;
 };
@@ -274,6 +279,8 @@ VarBc var_methods(const afgpu_ctx &c, int iv, bool homogeneous = false) {
     m.rb = RB_INTERP;
     m.type[2 * last] = BC_DIRICHLET;
     m.type[2 * last + 1] = BC_DIRICHLET;
+  } else if (iv == AFGPU_CC_RHS) {
+    m.rb = RB_INTERP;  // af_bc_neumann_zero + af_gc_interp (src/streamer.f90:103-108)
   } else if (iv >= 0 && iv < 9) {
     m.rb = RB_INTERP_LIM;
     if (c.p.species_bc == AFGPU_SPECIES_BC_DIRICHLET_ZERO) {
@@ -605,8 +612,11 @@ struct Impl : ImplBase {
     return (after_down && use_block<T>(lvl)) ? v.alt : v.phi;
   }
 
+  // stored_first: the first half-sweep reads the stored ghost layer (the
+  // finest level of a V-cycle called on its own, whose layer the reference
+  // smoother reads as the previous operation left it)
   template <typename T>
-  int gsrb_boxes(const Mg<T> &v, int lvl, bool down, bool final_gc = true) {
+  int gsrb_boxes(const Mg<T> &v, int lvl, bool down, bool final_gc = true, bool stored_first = false) {
     int n_cycle = down ? c.p.mg_n_cycle_down : c.p.mg_n_cycle_up;
     if (std::is_same<T, float>::value && c.mixed_smooth > 0) n_cycle = c.mixed_smooth;
     const List &l = c.l_ids[lvl];
@@ -624,17 +634,19 @@ struct Impl : ImplBase {
       return final_gc ? gc_var(lvl, out, v.m, !down) : AFGPU_OK;
     }
     for (int n = 1; n <= 2 * n_cycle; n++) {
+      const int stored = (stored_first && n == 1 && !Lay<D, NC, T>::SPLIT) ? 1 : 0;
       if (k == 1)
         RCHK(with_bb(KC_GSRB, [&](auto B) {
           constexpr int bb = decltype(B)::value;
           return items_timed<bb>(0, bytes, KC_GSRB, k_gsrb_lds<D, NC, BS, T, bb>, l.n, c.list(l),
-                                 (const DBox *)c.d_boxes, v.phi, (const T *)v.rhs, n, v.lambda2, cyl, v.m);
+                                 (const DBox *)c.d_boxes, v.phi, (const T *)v.rhs, n, v.lambda2, cyl, v.m, stored);
         }));
       else
         RCHK(with_bb(KC_GSRB, [&](auto B) {
           constexpr int bb = decltype(B)::value;
           return items_timed<bb>(0, bytes, KC_GSRB, k_gsrb<D, NC, BS, T, bb>, l.n * k, c.list(l),
-                                 (const DBox *)c.d_boxes, v.phi, (const T *)v.rhs, n, v.lambda2, cyl, v.m, k);
+                                 (const DBox *)c.d_boxes, v.phi, (const T *)v.rhs, n, v.lambda2, cyl, v.m, stored,
+                                 k);
         }));
     }
     // ghost layer after the last half-sweep (corners on the final up-sweep)
@@ -741,16 +753,18 @@ struct Impl : ImplBase {
   int vcycle(int op, bool set_residual, int max_lvl) override {
     RCHK(ensure_factors(op));  // allocation and upload happen outside any capture
     return run_graph(c, {G_VCYCLE, op, (int)set_residual, max_lvl},
-                     [&] { return vcycle_body(mg64(op), set_residual, max_lvl); });
+                     [&] { return vcycle_body(mg64(op), set_residual, max_lvl, 1, true); });
   }
 
   // FAS V-cycle from max_lvl down to the coarsest level lc (1: the level-1
   // box solve of the reference; > 1: the exact uniform-level solve of the
-  // mixed-precision inner cycle) and back.
+  // mixed-precision inner cycle) and back.  standalone: called on its own
+  // (not from FMG, which fills the ghost layer before each V-cycle), so the
+  // first half-sweep on max_lvl reads the stored ghost layer.
   template <typename T>
-  int vcycle_body(const Mg<T> &v, bool set_residual, int max_lvl, int lc = 1) {
+  int vcycle_body(const Mg<T> &v, bool set_residual, int max_lvl, int lc = 1, bool standalone = false) {
     for (int lvl = max_lvl; lvl >= lc + 1; lvl--) {
-      RCHK(gsrb_boxes(v, lvl, true, !restrict_in_lds<T>(lvl)));
+      RCHK(gsrb_boxes(v, lvl, true, !restrict_in_lds<T>(lvl), standalone && lvl == max_lvl));
       RCHK(update_coarse(v, lvl, true));
     }
     if (lc == 1)
@@ -1729,6 +1743,34 @@ struct Impl : ImplBase {
     return AFGPU_OK;
   }
 
+  int restrict_ids(const int *child_ids, int n, int iv) override {
+    if (n <= 0) return AFGPU_OK;
+    hipLaunchKernelGGL((k_restrict<D, NC, BS, double, 1>), dim3(std::min(n, 8 * c.n_cu)), dim3(BS), 0, c.cur,
+                       child_ids, (const DBox *)c.d_boxes, (const double *)c.cc[iv], c.cc[iv], 1,
+                       (int)(c.coord == AFGPU_CYL), 1, n);
+    HIPCHK(hipGetLastError());
+    return AFGPU_OK;
+  }
+  int prolong_new(const int *ids, int n, int iv, bool limited) override {
+    if (n <= 0) return AFGPU_OK;
+    const dim3 grid(std::min(n, 8 * c.n_cu));
+    if (limited)
+      hipLaunchKernelGGL((k_prolong_new<D, NC, BS, true>), grid, dim3(BS), 0, c.cur, ids, (const DBox *)c.d_boxes,
+                         c.cc[iv], var_methods(c, iv).limiter, n);
+    else
+      hipLaunchKernelGGL((k_prolong_new<D, NC, BS, false>), grid, dim3(BS), 0, c.cur, ids, (const DBox *)c.d_boxes,
+                         c.cc[iv], 0, n);
+    HIPCHK(hipGetLastError());
+    return AFGPU_OK;
+  }
+  int gc_ids(const int *ids, int n, int iv) override {
+    if (n <= 0) return AFGPU_OK;
+    hipLaunchKernelGGL((k_gc_lds<D, NC, BS, double, 1>), dim3(std::min(n, 8 * c.n_cu)), dim3(BS), 0, c.cur, ids,
+                       (const DBox *)c.d_boxes, c.cc[iv], var_methods(c, iv), 1, n);
+    HIPCHK(hipGetLastError());
+    return AFGPU_OK;
+  }
+
   int copy_states(bool backup) override {
     const int ns = c.n_states();
     for (int sp = 0; sp < 3; sp++) {
@@ -1804,7 +1846,7 @@ struct Impl : ImplBase {
         RCHK(vcycle_body(mg32(op), false, c.highest_lvl, lc));
         RCHK(mp_close(op, true, lc));
       } else {
-        RCHK(vcycle_body(mg64(op), true, c.highest_lvl));
+        RCHK(vcycle_body(mg64(op), true, c.highest_lvl, 1, true));
       }
       RCHK(fold_red());
       hipLaunchKernelGGL(k_run_converge, rg, rb, 0, c.cur, (const unsigned long long *)c.d_red_out, c.d_run_host,
@@ -2759,6 +2801,8 @@ int afgpu_photoi_set_src(afgpu_t *h) {
   return sync_stream(h);
 }
 
+static int ensure_ref_buffers(afgpu_t *h, int n);
+
 // consistent_ref_flags (afivo/src/m_af_core.f90:929-1013): the cell flags
 // of every leaf and of every box with a leaf child (the reference evaluates
 // such a box once, from its first leaf child) on the device, the rest here
@@ -2782,15 +2826,7 @@ int afgpu_refine_flags(afgpu_t *h, const afgpu_refine_t *rp, int n_boxes, int *r
       for (int q = 0; q < nch; q++) leaf_child = leaf_child || !has_children(hb[id - 1].children[q]);
       if (leaf_child) eval.push_back(id);
     }
-  if (n_boxes > h->ref_cap) {
-    (void)hipFree(h->d_ref_ids);
-    (void)hipFree(h->d_ref_flags);
-    h->d_ref_ids = h->d_ref_flags = nullptr;
-    h->ref_cap = 0;
-    HIPCHK(hipMalloc(&h->d_ref_ids, (size_t)n_boxes * sizeof(int)));
-    HIPCHK(hipMalloc(&h->d_ref_flags, (size_t)n_boxes * sizeof(int)));
-    h->ref_cap = n_boxes;
-  }
+  RCHK(ensure_ref_buffers(h, n_boxes));
   const int unset = std::numeric_limits<int>::min();
   h->cur = h->stream;
   HIPCHK(hipMemcpyAsync(h->d_ref_ids, eval.data(), eval.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
@@ -2848,6 +2884,91 @@ int afgpu_refine_flags(afgpu_t *h, const afgpu_refine_t *rp, int n_boxes, int *r
     }
   std::copy(f.begin(), f.end(), ref_flags);
   return AFGPU_OK;
+}
+
+// The variables with methods, in the order the streamer sets them
+// (photoi_initialize, field_initialize, mg_init, src/streamer.f90:84-86 and
+// 103-108), and whether they are prolonged with af_prolong_limit (the
+// densities and rhs: the streamer's default prolongation,
+// src/m_streamer.f90:402-417) or af_prolong_linear.
+static std::vector<std::pair<int, bool>> auto_vars(const afgpu_ctx &c) {
+  std::vector<std::pair<int, bool>> v;
+  if (c.p.photoi_enabled) v.push_back({AFGPU_CC_PHOTO, false});
+  v.push_back({AFGPU_CC_EFIELD, false});
+  v.push_back({AFGPU_CC_PHI, false});
+  for (int k = 0; k < 3; k++) v.push_back({AFGPU_CC_SPECIES(k), true});
+  v.push_back({AFGPU_CC_RHS, true});
+  return v;
+}
+static int ensure_ref_buffers(afgpu_t *h, int n) {
+  if (n <= h->ref_cap) return AFGPU_OK;
+  (void)hipFree(h->d_ref_ids);
+  (void)hipFree(h->d_ref_flags);
+  h->d_ref_ids = h->d_ref_flags = nullptr;
+  h->ref_cap = 0;
+  HIPCHK(hipMalloc(&h->d_ref_ids, (size_t)n * sizeof(int)));
+  HIPCHK(hipMalloc(&h->d_ref_flags, (size_t)n * sizeof(int)));
+  h->ref_cap = n;
+  return AFGPU_OK;
+}
+
+// auto_restrict (afivo/src/m_af_core.f90:825-840) for the listed boxes,
+// before their children are removed
+int afgpu_restrict_boxes(afgpu_t *h, int n_ids, const int *ids) {
+  RCHK(ready(h));
+  const int n_boxes = (int)h->hboxes.size(), nch = 1 << h->D;
+  if (n_ids < 0 || (n_ids > 0 && !ids)) return AFGPU_ERR_ARG;
+  std::vector<int> ch;
+  for (int q = 0; q < n_ids; q++) {
+    const int id = ids[q];
+    if (id < 1 || id > n_boxes || h->hboxes[id - 1].lvl < 1 || h->hboxes[id - 1].children[0] <= 0) {
+      set_err("afgpu_restrict_boxes: not a box with children");
+      return AFGPU_ERR_ARG;
+    }
+    for (int k = 0; k < nch; k++) ch.push_back(h->hboxes[id - 1].children[k]);
+  }
+  if (ch.empty()) return AFGPU_OK;
+  RCHK(ensure_ref_buffers(h, std::max((int)ch.size(), n_boxes)));
+  h->cur = h->stream;
+  HIPCHK(hipMemcpyAsync(h->d_ref_ids, ch.data(), ch.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
+  for (const auto &v : auto_vars(*h)) RCHK(h->impl->restrict_ids(h->d_ref_ids, (int)ch.size(), v.first));
+  return sync_stream(h);
+}
+
+// auto_prolong (afivo/src/m_af_core.f90:843-881) for the listed new boxes
+// of the synchronised tree: level by level, the values from the parents,
+// then the new boxes' ghost cells (af_gc_box with corners)
+int afgpu_prolong_boxes(afgpu_t *h, int n_ids, const int *ids) {
+  RCHK(ready(h));
+  const int n_boxes = (int)h->hboxes.size();
+  if (n_ids < 0 || (n_ids > 0 && !ids)) return AFGPU_ERR_ARG;
+  std::vector<int> sorted;
+  for (int q = 0; q < n_ids; q++) {
+    const int id = ids[q];
+    if (id < 1 || id > n_boxes || h->hboxes[id - 1].lvl < 2 || h->hboxes[id - 1].parent < 1) {
+      set_err("afgpu_prolong_boxes: not a box with a parent");
+      return AFGPU_ERR_ARG;
+    }
+    sorted.push_back(id);
+  }
+  if (sorted.empty()) return AFGPU_OK;
+  std::stable_sort(sorted.begin(), sorted.end(),
+                   [&](int a, int b) { return h->hboxes[a - 1].lvl < h->hboxes[b - 1].lvl; });
+  RCHK(ensure_ref_buffers(h, std::max((int)sorted.size(), n_boxes)));
+  h->cur = h->stream;
+  HIPCHK(hipMemcpyAsync(h->d_ref_ids, sorted.data(), sorted.size() * sizeof(int), hipMemcpyHostToDevice,
+                        h->stream));
+  const auto vars = auto_vars(*h);
+  for (size_t a = 0; a < sorted.size();) {
+    size_t b = a;
+    while (b < sorted.size() && h->hboxes[sorted[b] - 1].lvl == h->hboxes[sorted[a] - 1].lvl) b++;
+    const int *d = h->d_ref_ids + a;
+    const int n = (int)(b - a);
+    for (const auto &v : vars) RCHK(h->impl->prolong_new(d, n, v.first, v.second));
+    for (const auto &v : vars) RCHK(h->impl->gc_ids(d, n, v.first));
+    a = b;
+  }
+  return sync_stream(h);
 }
 
 int afgpu_copy_current_state(afgpu_t *h) {

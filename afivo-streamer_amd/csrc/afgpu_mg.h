@@ -366,11 +366,14 @@ __global__ void __launch_bounds__(BS * BB) k_gc(const int *__restrict__ ids, con
 // a neighbour outside the box is taken from ghost_value() with the same
 // inputs af_gc_box would use, so the updated cells are bit-identical while
 // no separate ghost-fill pass is needed between half-sweeps.  (The stored
-// ghost layer is refreshed once after the last half-sweep.)
+// ghost layer is refreshed once after the last half-sweep.)  stored: the
+// first half-sweep of a V-cycle's finest level reads the stored layer as
+// the reference does, also where it is out of date (the neighbours of boxes
+// just added or removed by refinement).
 template <int D, int NC, int BS, typename T, int BB = 1>
 __global__ void __launch_bounds__(BS * BB) k_gsrb(const int *__restrict__ ids, const DBox *__restrict__ boxes, T *phi,
                                                   const T *__restrict__ rhs, int redblack, T lambda2, int cyl, VarBc m,
-                                                  int bpb, int nw) {
+                                                  int stored, int bpb, int nw) {
   using G = Geo<D, NC>;
   using L = Lay<D, NC, T>;
   for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
@@ -395,7 +398,7 @@ __global__ void __launch_bounds__(BS * BB) k_gsrb(const int *__restrict__ ids, c
         const int d = nb / 2;
         const bool low = (nb % 2) == 0;
         const bool edge = low ? pos[d] == 1 : pos[d] == NC;
-        if (!edge) {
+        if (!edge || (stored && !L::SPLIT)) {
           nbv[nb] = u[L::nb_at(i, j, k, d, low ? -1 : 1)];
         } else {
           int t0, t1;
@@ -957,12 +960,16 @@ __device__ inline void lds_load_interior(T *s, const T *u) {
 // and in the colour-split FP32 layout only the interior cells of the other
 // colour, the only ones a half-sweep of that colour reads (the few cells of
 // the updated colour that those ghosts depend on are read from the pool).
+// stored: the face ghosts are the box's own stored ghost layer (the
+// reference's smoother reads that layer as it stands before its first
+// half-sweep; FP64 layout only).
 // Ends with a barrier.
 template <int D, int NC, int BS, typename T, int BB = 1>
 __device__ inline void lds_stage(T *s, const DBox *__restrict__ boxes, const DBox &bx, int id, const T *var,
-                                 const VarBc &m, int colour, const T *coarse = nullptr) {
+                                 const VarBc &m, int colour, const T *coarse = nullptr, bool stored = false) {
   using G = Geo<D, NC>;
   using L = Lay<D, NC, T>;
+  stored = stored && !L::SPLIT;
   constexpr int NI = (G::NINT + BS - 1) / BS;
   constexpr int NFC = 2 * D * G::NTAN;
   constexpr int NF = (NFC + BS - 1) / BS;
@@ -1003,12 +1010,18 @@ __device__ inline void lds_stage(T *s, const DBox *__restrict__ boxes, const DBo
     if (q >= NFC) continue;
     const int nb = q / G::NTAN, r = q % G::NTAN;
     const int nb_id = bx.nb[nb];
-    if (nb_id <= 0 && !pre) continue;
+    if (nb_id <= 0 && !pre && !stored) continue;
     const int a = r % NC + 1, b = (D == 3) ? r / NC + 1 : 0;
     const bool low = (nb % 2) == 0;
     int p[3];
     face_pos<D>(nb, a, b, low ? 1 : NC, p);
     if (colour >= 0 && ((p[0] + p[1] + p[2] + colour) & 1)) continue;
+    if (stored) {
+      p[nb / 2] = low ? 0 : NC + 1;
+      gi[t] = G::idx(p[0], p[1], p[2]);
+      vf[t] = u[L::at(p[0], p[1], p[2])];
+      continue;
+    }
     if (nb_id <= 0) {
       gp[t] = ghost_coarse<D, NC, T>(bx, var, m, nb, a, b, coarse);
       if (one) x1g[t] = u[L::in_at(p[0], p[1], p[2])];
@@ -1033,7 +1046,7 @@ __device__ inline void lds_stage(T *s, const DBox *__restrict__ boxes, const DBo
   gsync<BS, BB>();
   bool any = false;
   for (int nb = 0; nb < 2 * D; nb++) any = any || bx.nb[nb] <= 0;
-  if (any) {
+  if (any && !stored) {
     if (pre) {
       for (int t = 0; t < NF; t++) {
         const int q = ltid<BS>() + t * BS;
@@ -1075,7 +1088,7 @@ __device__ inline void lds_stage(T *s, const DBox *__restrict__ boxes, const DBo
 template <int D, int NC, int BS, typename T, int BB = 1>
 __global__ void __launch_bounds__(BS * BB) k_gsrb_lds(const int *__restrict__ ids, const DBox *__restrict__ boxes,
                                                       T *phi, const T *__restrict__ rhs, int redblack, T lambda2,
-                                                      int cyl, VarBc m, int nw) {
+                                                      int cyl, VarBc m, int stored, int nw) {
   using G = Geo<D, NC>;
   using L = Lay<D, NC, T>;
   __shared__ T s_all[BB][G::NCELL];
@@ -1108,7 +1121,7 @@ __global__ void __launch_bounds__(BS * BB) k_gsrb_lds(const int *__restrict__ id
       }
     }
     // the updated cells satisfy (i + j + k + redblack) even
-    lds_stage<D, NC, BS, T, BB>(s, boxes, bx, id, phi, m, redblack & 1);
+    lds_stage<D, NC, BS, T, BB>(s, boxes, bx, id, phi, m, redblack & 1, nullptr, stored != 0);
     for (int t = 0; t < NH; t++) {
       const int q = ltid<BS>() + t * BS;
       if (q >= HALF) break;

@@ -166,6 +166,80 @@ __global__ void __launch_bounds__(BS) k_refine_cells(const int *__restrict__ ids
   }
 }
 
+// A new box's values from its parent (auto_prolong, afivo/src/m_af_core.f90:
+// 843-881): af_prolong_limit (LIMITED: the slopes limited with the
+// variable's limiter, m_af_prolong.f90:311-420) or af_prolong_linear
+// (m_af_prolong.f90:531-679), not adding: every interior cell is written
+// once, as the reference's zero plus its sum.  One workgroup per new box.
+template <int D, int NC, int BS, bool LIMITED>
+__global__ void __launch_bounds__(BS) k_prolong_new(const int *__restrict__ ids, const DBox *__restrict__ boxes,
+                                                    double *var, int limiter, int nw) {
+  using G = Geo<D, NC>;
+  constexpr int H = NC / 2;
+  constexpr int NQ = (D == 3) ? H * H * H : H * H;
+  constexpr int st[3] = {1, G::NG, G::NG * G::NG};
+  for (int w = blockIdx.x; w < nw; w += gridDim.x) {
+    const int id = ids[w];
+    if (id <= 0) continue;
+    const DBox &bc = boxes[id - 1];
+    const double *P = var + (size_t)(bc.parent - 1) * G::NCELL;
+    double *u = var + (size_t)(id - 1) * G::NCELL;
+    int off[3] = {0, 0, 0};
+    for (int d = 0; d < D; d++) off[d] = ((bc.ix[d] - 1) & 1) * H;
+    for (int q = threadIdx.x; q < NQ; q += BS) {
+      const int i = q % H + 1, j = (q / H) % H + 1, k = (D == 3) ? q / (H * H) + 1 : 0;
+      const int x = G::idx(off[0] + i, off[1] + j, (D == 3) ? off[2] + k : 0);
+      const int y0 = G::idx(2 * i - 1, 2 * j - 1, (D == 3) ? 2 * k - 1 : 0);
+      if (LIMITED) {
+        const double f0 = P[x];
+        double f[3];
+        for (int d = 0; d < D; d++) f[d] = 0.25 * lim_apply<double>(P[x] - P[x - st[d]], P[x + st[d]] - P[x], limiter);
+        for (int c = 0; c < (1 << D); c++) {
+          const int di = c & 1, dj = (c >> 1) & 1, dk = (c >> 2) & 1;
+          double v = f0;
+          v = di ? v + f[0] : v - f[0];
+          v = dj ? v + f[1] : v - f[1];
+          if (D == 3) v = dk ? v + f[2] : v - f[2];
+          u[y0 + di * st[0] + dj * st[1] + dk * st[2]] = v + 0.0;
+        }
+      } else if (D == 2) {
+        const double f1 = 1 / 16.0, f3 = 3 / 16.0, f9 = 9 / 16.0;
+        auto p = [&](int a, int b) { return P[x + a + b * st[1]]; };
+        const double f0 = f9 * p(0, 0), flx = f3 * p(-1, 0), fhx = f3 * p(1, 0), fly = f3 * p(0, -1),
+                     fhy = f3 * p(0, 1), fll = f1 * p(-1, -1), fhl = f1 * p(1, -1), flh = f1 * p(-1, 1),
+                     fhh = f1 * p(1, 1);
+        u[y0] = f0 + flx + fly + fll + 0.0;
+        u[y0 + 1] = f0 + fhx + fly + fhl + 0.0;
+        u[y0 + st[1]] = f0 + flx + fhy + flh + 0.0;
+        u[y0 + st[1] + 1] = f0 + fhx + fhy + fhh + 0.0;
+      } else {
+        const double f1 = 1 / 64.0, f3 = 3 / 64.0, f9 = 9 / 64.0, f27 = 27 / 64.0;
+        auto p = [&](int a, int b, int e) { return P[x + a + b * st[1] + e * st[2]]; };
+        const double f000 = f27 * p(0, 0, 0);
+        const double f00l = f9 * p(0, 0, -1), f0l0 = f9 * p(0, -1, 0), f0ll = f3 * p(0, -1, -1),
+                     fl00 = f9 * p(-1, 0, 0), fl0l = f3 * p(-1, 0, -1), fll0 = f3 * p(-1, -1, 0),
+                     flll = f1 * p(-1, -1, -1);
+        const double f00h = f9 * p(0, 0, 1), f0h0 = f9 * p(0, 1, 0), f0hh = f3 * p(0, 1, 1),
+                     fh00 = f9 * p(1, 0, 0), fh0h = f3 * p(1, 0, 1), fhh0 = f3 * p(1, 1, 0),
+                     fhhh = f1 * p(1, 1, 1);
+        const double fl0h = f3 * p(-1, 0, 1), fh0l = f3 * p(1, 0, -1), flh0 = f3 * p(-1, 1, 0),
+                     fhl0 = f3 * p(1, -1, 0), f0lh = f3 * p(0, -1, 1), f0hl = f3 * p(0, 1, -1);
+        const double fllh = f1 * p(-1, -1, 1), flhl = f1 * p(-1, 1, -1), fhll = f1 * p(1, -1, -1);
+        const double fhhl = f1 * p(1, 1, -1), fhlh = f1 * p(1, -1, 1), flhh = f1 * p(-1, 1, 1);
+        auto Y = [&](int a, int b, int e) -> double & { return u[y0 + a + b * st[1] + e * st[2]]; };
+        Y(0, 0, 0) = f000 + fl00 + f0l0 + f00l + fll0 + fl0l + f0ll + flll + 0.0;
+        Y(1, 0, 0) = f000 + fh00 + f0l0 + f00l + fhl0 + fh0l + f0ll + fhll + 0.0;
+        Y(0, 1, 0) = f000 + fl00 + f0h0 + f00l + flh0 + fl0l + f0hl + flhl + 0.0;
+        Y(1, 1, 0) = f000 + fh00 + f0h0 + f00l + fhh0 + fh0l + f0hl + fhhl + 0.0;
+        Y(0, 0, 1) = f000 + fl00 + f0l0 + f00h + fll0 + fl0h + f0lh + fllh + 0.0;
+        Y(1, 0, 1) = f000 + fh00 + f0l0 + f00h + fhl0 + fh0h + f0lh + fhlh + 0.0;
+        Y(0, 1, 1) = f000 + fl00 + f0h0 + f00h + flh0 + fl0h + f0hh + flhh + 0.0;
+        Y(1, 1, 1) = f000 + fh00 + f0h0 + f00h + fhh0 + fh0h + f0hh + fhhh + 0.0;
+      }
+    }
+  }
+}
+
 __global__ void k_fill_int(int *p, int n, int v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;

@@ -168,6 +168,51 @@ def measure_refine(case, p, c=None, reps=10):
     return out
 
 
+def measure_new_boxes(case, p, c=None):
+    """Data of new boxes on the device (afgpu_prolong_boxes) after refining
+    a tenth of the finest leaves whose face neighbours all exist, against
+    the host doing it: the oracle's prolongation and the upload of the new
+    boxes' variables.  Changes the tree of p (and c): call last."""
+    t0 = case.extra["export"]
+    D, nc, n0 = case.D, case.nc, t0["n_boxes"]
+    nch, nnb = 2 ** D, 2 * D
+    top = t0["lvl"].max()
+    inner = [b + 1 for b in range(n0) if t0["lvl"][b] == top and t0["children"][b * nch] == 0
+             and all(v != 0 for v in t0["neighbors"][b * nnb:(b + 1) * nnb])]
+    pick = inner[::10]
+    case.tree.refine(np.asarray(pick, np.int32))
+    t1 = case.tree.export()
+    new = np.arange(n0 + 1, t1["n_boxes"] + 1, dtype=np.int32)
+    p.sync_tree(t1)
+    # the prolongation reads the parents only, so repeating it rewrites the
+    # same values: one untimed call, then the mean of five
+    p.prolong_boxes(new)
+    t_0 = time.perf_counter()
+    for _ in range(5):
+        p.prolong_boxes(new)
+    gpu_ms = (time.perf_counter() - t_0) * 1e3 / 5
+    n_vars = 7 if case.params.photoi_enabled else 6
+    out = {"new_boxes": int(len(new)), "refined": len(pick), "gpu_ms": round(gpu_ms, 3),
+           "algo_bytes": int(len(new)) * n_vars * (nc ** D + nc ** D // nch) * 8,
+           "timing": "wall time of afgpu_prolong_boxes (id upload, prolongation and ghost cells of %d "
+                     "variables, synchronised), mean of 5 calls after one" % n_vars}
+    data = [p.get_cc(iv, new) for iv in (A.CC_PHOTO, A.CC_EFIELD, A.CC_PHI, A.CC_ELECTRON, A.CC_POS_ION,
+                                         A.CC_NEG_ION, A.CC_RHS)[7 - n_vars:]]
+    t_0 = time.perf_counter()
+    for iv, d in zip((A.CC_PHOTO, A.CC_EFIELD, A.CC_PHI, A.CC_ELECTRON, A.CC_POS_ION, A.CC_NEG_ION,
+                      A.CC_RHS)[7 - n_vars:], data):
+        p.put_cc(iv, d, new)
+    out["upload_new_ms"] = round((time.perf_counter() - t_0) * 1e3, 3)
+    if c is not None:
+        c.sync_tree(t1)
+        t_0 = time.perf_counter()
+        c.prolong_boxes(new)
+        out["cpu_oracle_ms"] = round((time.perf_counter() - t_0) * 1e3, 3)
+        out["cpu_oracle_cores"] = 1
+    log("  new boxes", json.dumps(out))
+    return out
+
+
 def rel_err(a, b):
     d = a - b
     return float(np.abs(d).max() / max(np.abs(b).max(), 1e-300)), \
@@ -458,6 +503,7 @@ def main():
                          "after warmup, %.1f s" % (args.cpu_steps, info["leaf_cells"], csecs)}
         log("  cpu %.4e cell-steps/s (%.1f s)" % (cpu["value"], csecs))
     extra["refine_flags"] = measure_refine(case, p, None if args.no_cpu else c)
+    extra["new_boxes"] = measure_new_boxes(case, p, None if args.no_cpu else c)
 
     out = {
         "metric": "cell-steps/s, 3D AMR step (DD+MG Poisson+photoi), FP64 vs mixed at 1e-8",

@@ -256,6 +256,22 @@ typedef struct {
 } afgpu_refine_t;
 int afgpu_refine_flags(afgpu_t *h, const afgpu_refine_t *rp, int n_boxes, int *ref_flags);
 
+/* The data of boxes removed or added by af_adjust_refinement, on the
+ * device, for the variables with methods (photo with photoionization, |E|,
+ * phi, the electron and ion densities, rhs; src/streamer.f90:84-108):
+ * - afgpu_restrict_boxes: auto_restrict (afivo/src/m_af_core.f90:825-840),
+ *   the children of each listed box restricted into it (af_restrict_box,
+ *   m_af_restrict.f90:62-136), before the driver removes the children;
+ * - afgpu_prolong_boxes: auto_prolong (m_af_core.f90:843-881) for the
+ *   listed new boxes after afgpu_sync_tree with the new tree: level by
+ *   level, each from its parent (af_prolong_limit, m_af_prolong.f90:311-420,
+ *   for the densities and rhs; af_prolong_linear, :531-679, for the
+ *   others), then their ghost cells (af_gc_box with corners).
+ * Existing boxes keep their data across afgpu_sync_tree, so the new boxes'
+ * values never pass through the host. */
+int afgpu_restrict_boxes(afgpu_t *h, int n_ids, const int *ids);
+int afgpu_prolong_boxes(afgpu_t *h, int n_ids, const int *ids);
+
 /* photoi_set_src (src/m_photoi.f90:153) with the Helmholtz method
  * (src/m_photoi_helmh.f90:162). */
 int afgpu_photoi_set_src(afgpu_t *h);

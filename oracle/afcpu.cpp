@@ -106,6 +106,8 @@ static VarMethods methods(const Ctx &c, int iv) {
     m.rb = RB_INTERP;
     m.bc_type[2 * last] = BC_DIRICHLET;
     m.bc_type[2 * last + 1] = BC_DIRICHLET;
+  } else if (iv == AFGPU_CC_RHS) {
+    m.rb = RB_INTERP;  // af_bc_neumann_zero + af_gc_interp (src/streamer.f90:103-108)
   } else if (iv >= 0 && iv < 9) {
     m.rb = RB_INTERP_LIM;  // bc_species + af_gc_interp_lim
     if (c.p.species_bc == AFGPU_SPECIES_BC_DIRICHLET_ZERO) {
@@ -1961,6 +1963,148 @@ static void refine_flags(const Ctx &c, const afgpu_refine_t &rp, std::vector<int
     }
 }
 
+// ---- data of boxes removed or added by refinement ---------------------------
+// The variables with methods, in the order the streamer sets them
+// (photoi_initialize, field_initialize, mg_init, src/streamer.f90:84-86 and
+// 103-108): photo (with photoionization), |E|, phi, the densities, rhs.
+static std::vector<int> auto_vars(const Ctx &c) {
+  std::vector<int> v;
+  if (c.p.photoi_enabled) v.push_back(AFGPU_CC_PHOTO);
+  v.push_back(AFGPU_CC_EFIELD);
+  v.push_back(AFGPU_CC_PHI);
+  for (int k = 0; k < 3; k++) v.push_back(AFGPU_CC_SPECIES(k));
+  v.push_back(AFGPU_CC_RHS);
+  return v;
+}
+// the densities and rhs use af_prolong_limit (the streamer's default
+// prolongation, src/m_streamer.f90:402-417); the others af_prolong_linear
+static bool prolongs_limited(int iv) { return iv < 9 || iv == AFGPU_CC_RHS; }
+
+// af_prolong_limit (afivo/src/m_af_prolong.f90:311-420), not adding
+static void prolong_limit(Ctx &c, int id_c, int iv) {
+  const int nc = c.nc, hnc = nc / 2, D = c.D;
+  const Box &bc_ = c.B(id_c);
+  const double *P = c.C(iv, bc_.parent);
+  double *u = c.C(iv, id_c);
+  const int lim = methods(c, iv).limiter;
+  int off[3] = {0, 0, 0};
+  for (int d = 0; d < D; d++) off[d] = ((bc_.ix[d] - 1) & 1) * hnc;
+  const int kmax = (D == 3) ? nc : 1;
+  for (int k = (D == 3) ? 1 : 0; k <= (D == 3 ? kmax : 0); k++)
+    for (int j = 1; j <= nc; j++)
+      for (int i = 1; i <= nc; i++) u[c.idx(i, j, k)] = 0;
+  const int st[3] = {1, c.ng, c.ng * c.ng};
+  for (int k = 1; k <= (D == 3 ? hnc : 1); k++)
+    for (int j = 1; j <= hnc; j++)
+      for (int i = 1; i <= hnc; i++) {
+        const int ic = off[0] + i, jc = off[1] + j, kc = (D == 3) ? off[2] + k : 0;
+        const long x = c.idx(ic, jc, kc);
+        const double f0 = P[x];
+        double f[3];
+        for (int d = 0; d < D; d++) {
+          const double a = P[x] - P[x - st[d]], b = P[x + st[d]] - P[x];
+          f[d] = 0.25 * limiter_apply(a, b, lim);
+        }
+        const int i_f = 2 * i - 1, jf = 2 * j - 1, kf = (D == 3) ? 2 * k - 1 : 0;
+        for (int q = 0; q < (1 << D); q++) {
+          const int di = q & 1, dj = (q >> 1) & 1, dk = (q >> 2) & 1;
+          double v = f0;
+          v = di ? v + f[0] : v - f[0];
+          v = dj ? v + f[1] : v - f[1];
+          if (D == 3) v = dk ? v + f[2] : v - f[2];
+          double &y = u[c.idx(i_f + di, jf + dj, (D == 3) ? kf + dk : 0)];
+          y = v + y;
+        }
+      }
+}
+
+// af_prolong_linear (afivo/src/m_af_prolong.f90:531-679), not adding
+static void prolong_linear(Ctx &c, int id_c, int iv) {
+  const int nc = c.nc, hnc = nc / 2, D = c.D;
+  const Box &bc_ = c.B(id_c);
+  const double *P = c.C(iv, bc_.parent);
+  double *u = c.C(iv, id_c);
+  int off[3] = {0, 0, 0};
+  for (int d = 0; d < D; d++) off[d] = ((bc_.ix[d] - 1) & 1) * hnc;
+  for (int k = (D == 3) ? 1 : 0; k <= (D == 3 ? nc : 0); k++)
+    for (int j = 1; j <= nc; j++)
+      for (int i = 1; i <= nc; i++) u[c.idx(i, j, k)] = 0;
+  if (D == 2) {
+    const double f1 = 1 / 16.0, f3 = 3 / 16.0, f9 = 9 / 16.0;
+    for (int j = 1; j <= hnc; j++)
+      for (int i = 1; i <= hnc; i++) {
+        const int ic = off[0] + i, jc = off[1] + j, i_f = 2 * i - 1, jf = 2 * j - 1;
+        auto p = [&](int a, int b) { return P[c.idx(ic + a, jc + b, 0)]; };
+        const double f0 = f9 * p(0, 0), flx = f3 * p(-1, 0), fhx = f3 * p(1, 0), fly = f3 * p(0, -1),
+                     fhy = f3 * p(0, 1), fll = f1 * p(-1, -1), fhl = f1 * p(1, -1), flh = f1 * p(-1, 1),
+                     fhh = f1 * p(1, 1);
+        double *y = &u[c.idx(i_f, jf, 0)];
+        y[0] = f0 + flx + fly + fll + y[0];
+        y[1] = f0 + fhx + fly + fhl + y[1];
+        y[c.ng] = f0 + flx + fhy + flh + y[c.ng];
+        y[c.ng + 1] = f0 + fhx + fhy + fhh + y[c.ng + 1];
+      }
+    return;
+  }
+  const double f1 = 1 / 64.0, f3 = 3 / 64.0, f9 = 9 / 64.0, f27 = 27 / 64.0;
+  for (int k = 1; k <= hnc; k++)
+    for (int j = 1; j <= hnc; j++)
+      for (int i = 1; i <= hnc; i++) {
+        const int ic = off[0] + i, jc = off[1] + j, kc = off[2] + k;
+        const int i_f = 2 * i - 1, jf = 2 * j - 1, kf = 2 * k - 1;
+        auto p = [&](int a, int b, int e) { return P[c.idx(ic + a, jc + b, kc + e)]; };
+        const double f000 = f27 * p(0, 0, 0);
+        const double f00l = f9 * p(0, 0, -1), f0l0 = f9 * p(0, -1, 0), f0ll = f3 * p(0, -1, -1),
+                     fl00 = f9 * p(-1, 0, 0), fl0l = f3 * p(-1, 0, -1), fll0 = f3 * p(-1, -1, 0),
+                     flll = f1 * p(-1, -1, -1);
+        const double f00h = f9 * p(0, 0, 1), f0h0 = f9 * p(0, 1, 0), f0hh = f3 * p(0, 1, 1),
+                     fh00 = f9 * p(1, 0, 0), fh0h = f3 * p(1, 0, 1), fhh0 = f3 * p(1, 1, 0),
+                     fhhh = f1 * p(1, 1, 1);
+        const double fl0h = f3 * p(-1, 0, 1), fh0l = f3 * p(1, 0, -1), flh0 = f3 * p(-1, 1, 0),
+                     fhl0 = f3 * p(1, -1, 0), f0lh = f3 * p(0, -1, 1), f0hl = f3 * p(0, 1, -1);
+        const double fllh = f1 * p(-1, -1, 1), flhl = f1 * p(-1, 1, -1), fhll = f1 * p(1, -1, -1);
+        const double fhhl = f1 * p(1, 1, -1), fhlh = f1 * p(1, -1, 1), flhh = f1 * p(-1, 1, 1);
+        auto Y = [&](int a, int b, int e) -> double & { return u[c.idx(i_f + a, jf + b, kf + e)]; };
+        Y(0, 0, 0) = f000 + fl00 + f0l0 + f00l + fll0 + fl0l + f0ll + flll + Y(0, 0, 0);
+        Y(1, 0, 0) = f000 + fh00 + f0l0 + f00l + fhl0 + fh0l + f0ll + fhll + Y(1, 0, 0);
+        Y(0, 1, 0) = f000 + fl00 + f0h0 + f00l + flh0 + fl0l + f0hl + flhl + Y(0, 1, 0);
+        Y(1, 1, 0) = f000 + fh00 + f0h0 + f00l + fhh0 + fh0l + f0hl + fhhl + Y(1, 1, 0);
+        Y(0, 0, 1) = f000 + fl00 + f0l0 + f00h + fll0 + fl0h + f0lh + fllh + Y(0, 0, 1);
+        Y(1, 0, 1) = f000 + fh00 + f0l0 + f00h + fhl0 + fh0h + f0lh + fhlh + Y(1, 0, 1);
+        Y(0, 1, 1) = f000 + fl00 + f0h0 + f00h + flh0 + fl0h + f0hh + flhh + Y(0, 1, 1);
+        Y(1, 1, 1) = f000 + fh00 + f0h0 + f00h + fhh0 + fh0h + f0hh + fhhh + Y(1, 1, 1);
+      }
+}
+
+// auto_restrict (afivo/src/m_af_core.f90:825-840): every child into box id
+static void restrict_boxes(Ctx &c, const std::vector<int> &ids) {
+  const std::vector<int> vars = auto_vars(c);
+  const int nch = 1 << c.D;
+  for (int id : ids)
+    for (int q = 0; q < nch; q++)
+      for (int iv : vars) restrict_box(c, c.B(id).children[q], iv, iv, true);
+}
+
+// auto_prolong (afivo/src/m_af_core.f90:843-881): the new boxes level by
+// level, each from its parent, then their ghost cells (af_gc_box, corners)
+static void prolong_boxes(Ctx &c, const std::vector<int> &ids) {
+  const std::vector<int> vars = auto_vars(c);
+  for (int lvl = 1; lvl <= c.highest_lvl; lvl++) {
+    std::vector<int> l;
+    for (int id : ids)
+      if (c.B(id).lvl == lvl) l.push_back(id);
+    for (int id : l)
+      for (int iv : vars) {
+        if (prolongs_limited(iv))
+          prolong_limit(c, id, iv);
+        else
+          prolong_linear(c, id, iv);
+      }
+    for (int id : l)
+      for (int iv : vars) gc_box(c, id, iv, true);
+  }
+}
+
 }  // namespace afcpu
 
 extern "C" {
@@ -2214,6 +2358,36 @@ int afcpu_refine_flags(afcpu_t *h, const afgpu_refine_t *rp, int n_boxes, int *r
   std::vector<int> f;
   refine_flags(h->c, *rp, f);
   std::copy(f.begin(), f.end(), ref_flags);
+  return AFGPU_OK;
+}
+
+static int check_ids(afcpu_t *h, int n, const int *ids, bool parents) {
+  if (n < 0 || (n > 0 && !ids)) return AFGPU_ERR_ARG;
+  for (int q = 0; q < n; q++) {
+    const int id = ids[q];
+    if (id < 1 || id > h->c.n_boxes || h->c.B(id).lvl < 1) return AFGPU_ERR_ARG;
+    if (parents ? !h->c.has_children(id) : h->c.B(id).parent < 1) return AFGPU_ERR_ARG;
+  }
+  return AFGPU_OK;
+}
+int afcpu_restrict_boxes(afcpu_t *h, int n_ids, const int *ids) {
+  int rc = ready(h, -1);
+  if (rc) return rc;
+  if (check_ids(h, n_ids, ids, true)) {
+    set_error("afcpu_restrict_boxes: invalid ids");
+    return AFGPU_ERR_ARG;
+  }
+  restrict_boxes(h->c, std::vector<int>(ids, ids + n_ids));
+  return AFGPU_OK;
+}
+int afcpu_prolong_boxes(afcpu_t *h, int n_ids, const int *ids) {
+  int rc = ready(h, -1);
+  if (rc) return rc;
+  if (check_ids(h, n_ids, ids, false)) {
+    set_error("afcpu_prolong_boxes: invalid ids");
+    return AFGPU_ERR_ARG;
+  }
+  prolong_boxes(h->c, std::vector<int>(ids, ids + n_ids));
   return AFGPU_OK;
 }
 

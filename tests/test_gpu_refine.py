@@ -98,3 +98,58 @@ def test_refine_flags_forest_equals_runs(A, S):
         fs = single.refine_flags(s, case.extra["export"]["n_boxes"])
         o = ens.offsets[r]
         assert np.array_equal(fg[o:o + len(fs)], fs), r
+
+
+def inner_leaves(t, D):
+    """Leaves whose face neighbours all exist (refining them keeps 2:1)."""
+    n, nch, nnb = t["n_boxes"], 2 ** D, 2 * D
+    return [b + 1 for b in range(n) if t["children"][b * nch] == 0
+            and all(v != 0 for v in t["neighbors"][b * nnb:(b + 1) * nnb])]
+
+
+@pytest.mark.parametrize("name", ["cyl2d", "xyz3d_8", "xyz3d_16"])
+def test_restricted_and_prolonged_boxes_equal_oracle(A, S, name):
+    """auto_restrict into boxes about to lose their children, then new
+    boxes from refining leaves (the tree synchronised with its new boxes,
+    whose data come from auto_prolong on each side), then a time step on
+    the new tree: every cell-centred variable equal to the oracle's."""
+    case = cases(S)[name]
+    case.params.mg_num_vcycles = 20
+    case.params.mg_max_rel_residual = 1e-10
+    case.params.photoi_enabled = 1
+    case.params.photoi_per_steps = 1
+    g = S.apply(A.Path("gpu", case.D, case.nc, case.coord, A.PREC_FP64), case)
+    o = S.apply(A.Path("oracle", case.D, case.nc, case.coord), case)
+    for p in (g, o):
+        p.field_compute(0, have_guess=False)
+    assert g.step() == o.step()
+    for p in (g, o):
+        p.restrict_tree(A.CC_ELECTRON)
+        p.gc_tree(A.CC_ELECTRON)
+
+    def same(what):
+        for iv in range(A.N_CC):
+            a, b = g.get_cc(iv), o.get_cc(iv)
+            assert np.array_equal(a, b), (name, what, iv, np.abs(a - b).max())
+
+    t0 = case.extra["export"]
+    D, nch = case.D, 2 ** case.D
+    ch = t0["children"].reshape(t0["n_boxes"], nch)
+    derefine = [b + 1 for b in range(t0["n_boxes"]) if ch[b, 0] > 0 and (ch[ch[b] - 1, 0] == 0).all()]
+    assert derefine
+    for p in (g, o):
+        p.restrict_boxes(derefine)
+    same("restrict")
+    rng = np.random.default_rng(5)
+    leaves = inner_leaves(t0, D)
+    pick = sorted(rng.choice(leaves, size=max(1, len(leaves) // 4), replace=False))
+    case.tree.refine(np.asarray(pick, np.int32))
+    t1 = case.tree.export()
+    new = np.arange(t0["n_boxes"] + 1, t1["n_boxes"] + 1, dtype=np.int32)
+    print(name, "derefined", len(derefine), "refined", len(pick), "new boxes", len(new), flush=True)
+    for p in (g, o):
+        p.sync_tree(t1)
+        p.prolong_boxes(new)
+    same("prolong")
+    assert g.step() == o.step()
+    same("step on the new tree")

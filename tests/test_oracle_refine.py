@@ -192,3 +192,112 @@ def test_derefinement_without_field(A, S, D):
             expect[ch] = -1
     assert np.array_equal(f, expect), np.nonzero(f != expect)[0]
     assert (expect == DEREFINE).sum() > 0
+
+
+# ---- data of refined and derefined boxes (auto_prolong / auto_restrict) ------
+def centres(nc, D, rmin, dr, ghosts=True):
+    """Cell-centre coordinates of a box as arrays in (k, j, i) C order."""
+    rng = np.arange(0, nc + 2) if ghosts else np.arange(1, nc + 1)
+    axes = [rmin[d] + (rng - 0.5) * dr[d] for d in range(D)]
+    return np.meshgrid(*axes[::-1], indexing="ij")[::-1]
+
+
+def box_view(row, nc, D):
+    return row.reshape((nc + 2,) * D)
+
+
+def interior(v, D):
+    return v[(slice(1, -1),) * D]
+
+
+def refine_some(case, pick):
+    """Refine the leaves chosen by pick(export) with the host tree; returns
+    the new export and the ids of the new boxes (appended after the old)."""
+    t0 = case.extra["export"]
+    ids = pick(t0)
+    case.tree.refine(np.asarray(ids, np.int32))
+    t1 = case.tree.export()
+    return t1, np.arange(t0["n_boxes"] + 1, t1["n_boxes"] + 1, dtype=np.int32)
+
+
+def inner_leaves(t, D, lvl):
+    """Leaves of a level whose face neighbours all exist (refining them keeps
+    the 2:1 balance)."""
+    n, nch, nnb = t["n_boxes"], 2 ** D, 2 * D
+    return [b + 1 for b in range(n) if t["lvl"][b] == lvl and t["children"][b * nch] == 0
+            and all(v != 0 for v in t["neighbors"][b * nnb:(b + 1) * nnb])]
+
+
+AUTO = ("CC_PHOTO", "CC_EFIELD", "CC_PHI", "CC_ELECTRON", "CC_POS_ION", "CC_NEG_ION", "CC_RHS")
+
+
+def centres_all(t, b, D, nc):
+    rmin, dr = t["r_min"][b * D:(b + 1) * D], t["dr"][b * D:(b + 1) * D]
+    return centres(nc, D, rmin, dr)
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_prolongation_of_new_boxes_reproduces_linear_fields(A, S, D):
+    case = uniform_tree(S, D, 3 if D == 2 else 2)
+    case.params.photoi_enabled = 1
+    p = oracle(A, S, case)
+    t0 = case.extra["export"]
+    n0, nc = t0["n_boxes"], case.nc
+    g = [1.3e3, -0.7e3, 0.4e3][:D]
+    for k, name in enumerate(AUTO):  # a different linear field per variable, ghosts included
+        vals = np.stack([sum((k + 1) * g[d] * X[d] for d in range(D)) + 5.0 * (k + 1)
+                         for X in (centres_all(t0, b, D, nc) for b in range(n0))])
+        p.put_cc(getattr(A, name), vals.reshape(n0, -1))
+    t1, new = refine_some(case, lambda t: inner_leaves(t, D, t["lvl"].max())[:3])
+    p.sync_tree(t1)
+    p.prolong_boxes(new)
+    for k, name in enumerate(AUTO):
+        u = p.get_cc(getattr(A, name))
+        for b in new - 1:
+            X = centres(nc, D, t1["r_min"][b * D:(b + 1) * D], t1["dr"][b * D:(b + 1) * D], ghosts=False)
+            exact = sum((k + 1) * g[d] * X[d] for d in range(D)) + 5.0 * (k + 1)
+            got = interior(box_view(u[b], nc, D), D)
+            assert np.abs(got - exact).max() <= 1e-12 * np.abs(exact).max(), (name, b + 1)
+    # restriction back into the parents gives the parents' (linear) values
+    parents = sorted({int(t1["parent"][b]) for b in new - 1})
+    before = {name: p.get_cc(getattr(A, name))[np.array(parents) - 1] for name in AUTO}
+    p.restrict_boxes(parents)
+    for name in AUTO:
+        after = p.get_cc(getattr(A, name))[np.array(parents) - 1]
+        assert np.abs(after - before[name]).max() <= 1e-12 * np.abs(before[name]).max(), name
+
+
+def test_limited_prolongation_keeps_extrema_and_means(A, S):
+    """Densities (af_prolong_limit): a parent cell at a local maximum gives
+    its children its own value (the limited slopes vanish) where phi
+    (af_prolong_linear) interpolates; every group of 2^D children keeps the
+    parent cell's value as its mean."""
+    D = 2
+    case = uniform_tree(S, D, 3)
+    p = oracle(A, S, case)
+    t0 = case.extra["export"]
+    n0, nc = t0["n_boxes"], case.nc
+    rng = np.random.default_rng(4)
+    base = rng.uniform(1.0, 2.0, (n0, nc + 2, nc + 2))
+    leaf = inner_leaves(t0, D, 3)[0]
+    peak = (5, 5)  # (j, i) of a parent cell inside the refined quadrant region
+    base[leaf - 1][peak] = 10.0
+    for name in ("CC_ELECTRON", "CC_PHI"):
+        p.put_cc(getattr(A, name), base.reshape(n0, -1))
+    t1, new = refine_some(case, lambda t: [leaf])
+    p.sync_tree(t1)
+    p.prolong_boxes(new)
+    ne, phi = p.get_cc(A.CC_ELECTRON), p.get_cc(A.CC_PHI)
+    H = nc // 2
+    for b in new - 1:
+        off = [(int(t1["ix"][b * D + d]) - 1) % 2 * H for d in range(D)]
+        for j in range(1, H + 1):
+            for i in range(1, H + 1):
+                jc, ic = off[1] + j, off[0] + i
+                kids = box_view(ne[b], nc, D)[2 * j - 1:2 * j + 1, 2 * i - 1:2 * i + 1]
+                pv = base[leaf - 1][jc, ic]
+                assert abs(kids.mean() - pv) <= 1e-14 * pv
+                if (jc, ic) == peak:
+                    assert np.all(kids == pv)
+                    lin = box_view(phi[b], nc, D)[2 * j - 1:2 * j + 1, 2 * i - 1:2 * i + 1]
+                    assert np.all(lin < pv)
