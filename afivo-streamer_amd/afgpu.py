@@ -210,8 +210,13 @@ _COMMON = {
     "num_leaf_cells": (C.c_longlong, [_P]),
     "get_time_state": (_I, [_P, _IP, _DP, _DP]),
     "set_time_state": (_I, [_P, _I, _D, _D]),
+    "regression_values": (_I, [_P, _I, _IP, _DP]),
+    "get_boxes": (_I, [_P, _I, _IP, _I, _IP, _DP]),
+    "put_boxes": (_I, [_P, _I, _IP, _I, _IP, _DP]),
 }
 _GPU_ONLY = {
+    "format_regression_line": (_I, [_I, _I, _DP, C.c_char_p, _I]),
+    "write_regression_log": (_I, [_P, C.c_char_p, _I, _I, _IP, C.POINTER(C.c_char_p)]),
     "coarse_factors": (_I, [_I, _I, _I, _DP, _DP, _IP, _DP, _DP, _DP]),
     "kernel_timing": (_I, [_P, _I]),
     "kernel_stats": (_I, [_P, _I, _I, _DP, C.POINTER(C.c_longlong), _DP]),
@@ -380,6 +385,18 @@ def transport_tables(E, mu, dif, alpha, eta, N, table_size=1000, table_max_td=-1
         raise AfError("transport_tables failed")
     return {"n": table_size, "td_x_min": tx0.value, "td_inv_fac": tif.value, "td": td,
             "chem_x_min": cx0.value, "chem_inv_fac": cif.value, "chem": ch}
+
+
+def format_regression_line(out_cnt, vals):
+    """One regression-log line in the reference's "(I0, nE20.8e3)" format
+    (host code of the device library; no GPU use)."""
+    lib = load_gpu()
+    vals = np.ascontiguousarray(vals, np.float64)
+    buf = C.create_string_buffer(64 + 24 * len(vals))
+    n = lib.afgpu_format_regression_line(out_cnt, len(vals), _dp(vals), buf, len(buf))
+    if n < 0:
+        raise AfError("afgpu_format_regression_line failed (%d)" % n)
+    return buf.value.decode()
 
 
 def coarse_factors(ndim, nc, coord, r_min, dr, bc_dirichlet):
@@ -553,6 +570,38 @@ class Path:
         ghost cells."""
         ids = np.ascontiguousarray(ids, np.int32)
         self._call("prolong_boxes", len(ids), _ip(ids))
+
+    # diagnostics and box I/O ---------------------------------------------------
+    def regression_values(self, ivs):
+        """The statistics of output_regression_log for the variables ivs:
+        array (runs, len(ivs), 3) of sum/volume, sum of squares/volume and
+        maximum over the leaves (one run for a plain tree)."""
+        ivs = np.ascontiguousarray(ivs, np.int32)
+        runs = self.num_runs() if self.kind == "gpu" else 1
+        out = np.zeros((runs, len(ivs), 3))
+        self._call("regression_values", len(ivs), _ip(ivs), _dp(out))
+        return out
+
+    def write_regression_log(self, filename, out_cnt, ivs, names):
+        """output_regression_log: the header at out_cnt 0, then one line."""
+        ivs = np.ascontiguousarray(ivs, np.int32)
+        arr = (C.c_char_p * len(names))(*[n.encode() for n in names])
+        self._call("write_regression_log", os.fsencode(filename), out_cnt, len(ivs), _ip(ivs), arr)
+
+    def get_boxes(self, ivs, ids=None):
+        """Whole boxes of the variables ivs in af_write_tree's order:
+        array (len(ids), len(ivs), ncell)."""
+        ivs = np.ascontiguousarray(ivs, np.int32)
+        ids = self._ids(ids)
+        out = np.zeros((len(ids), len(ivs), self.ncell))
+        self._call("get_boxes", len(ivs), _ip(ivs), len(ids), _ip(ids), _dp(out))
+        return out
+
+    def put_boxes(self, ivs, data, ids=None):
+        ivs = np.ascontiguousarray(ivs, np.int32)
+        ids = self._ids(ids)
+        data = np.ascontiguousarray(data, np.float64).reshape(len(ids), len(ivs), self.ncell)
+        self._call("put_boxes", len(ivs), _ip(ivs), len(ids), _ip(ids), _dp(data))
 
     def get_time_state(self):
         it, t, dt = C.c_int(), C.c_double(), C.c_double()

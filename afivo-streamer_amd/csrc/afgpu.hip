@@ -26,6 +26,7 @@
 #include "afgpu.h"
 #include "afgpu_direct.h"
 #include "afgpu_fluid.h"
+#include "afgpu_diag.h"
 #include "afgpu_refine.h"
 
 using namespace afg;
@@ -118,6 +119,8 @@ struct ImplBase {
   virtual int gc_ids(const int *ids, int n, int iv) = 0;
   virtual int restrict_tree(int iv) = 0;
   virtual int step_runs(afgpu_step_info_t *info) = 0;
+  // per-leaf-box terms of the regression-log statistics (k_diag_box)
+  virtual int diag_boxes(const VarPtrs &vars, double *part) = 0;
 };
 
 struct afgpu_ctx {
@@ -210,6 +213,9 @@ struct afgpu_ctx {
   // refinement flags: the boxes evaluated and their flags (device)
   int *d_ref_ids = nullptr, *d_ref_flags = nullptr;
   int ref_cap = 0;
+  // diagnostics: per-box terms and per-run results; box I/O staging
+  double *d_diag = nullptr, *d_io = nullptr;
+  size_t diag_cap = 0, io_cap = 0;
   struct Direct {
     bool set = false;  // lc and the map below are current for the tree
     int lc = 1, N = 0;
@@ -1763,6 +1769,14 @@ struct Impl : ImplBase {
     HIPCHK(hipGetLastError());
     return AFGPU_OK;
   }
+  int diag_boxes(const VarPtrs &vars, double *part) override {
+    const List &l = c.l_all_leaves;
+    if (l.n <= 0) return AFGPU_OK;
+    hipLaunchKernelGGL((k_diag_box<D, NC, BS>), dim3(std::min(l.n, 8 * c.n_cu)), dim3(BS), 0, c.cur, c.list(l),
+                       (const DBox *)c.d_boxes, vars, (int)(c.coord == AFGPU_CYL), part, l.n);
+    HIPCHK(hipGetLastError());
+    return AFGPU_OK;
+  }
   int gc_ids(const int *ids, int n, int iv) override {
     if (n <= 0) return AFGPU_OK;
     hipLaunchKernelGGL((k_gc_lds<D, NC, BS, double, 1>), dim3(std::min(n, 8 * c.n_cu)), dim3(BS), 0, c.cur, ids,
@@ -2236,6 +2250,8 @@ int afgpu_destroy(afgpu_t *h) {
   (void)hipFree(h->dir.d_map);
   (void)hipFree(h->d_ref_ids);
   (void)hipFree(h->d_ref_flags);
+  (void)hipFree(h->d_diag);
+  (void)hipFree(h->d_io);
   (void)hipFree(h->dir.A0);
   (void)hipFree(h->dir.A1);
   for (auto &f : h->dir.f) {
@@ -2969,6 +2985,203 @@ int afgpu_prolong_boxes(afgpu_t *h, int n_ids, const int *ids) {
     a = b;
   }
   return sync_stream(h);
+}
+
+// af_total_volume (afivo/src/m_af_types.f90:805-825) of one run: its
+// level-1 box (coarse_grid_size = box_size)
+static double run_volume(const afgpu_t *h, int run) {
+  const DBox *b1 = nullptr;
+  for (int id : h->src_ids[1])
+    if (h->hboxes[id - 1].run == run) b1 = &h->hboxes[id - 1];
+  const int D = h->D;
+  double len[3];
+  for (int d = 0; d < D; d++) len[d] = h->nc * b1->dr[d];
+  if (D == 2 && h->coord == AFGPU_CYL) {
+    const double pi = 3.141592653589793;  // acos(-1)
+    const double r0 = b1->r_min[0], r1 = r0 + len[0];
+    return pi * (r1 * r1 - r0 * r0) * len[D - 1];
+  }
+  double p = 1;
+  for (int d = 0; d < D; d++) p *= len[d];
+  return p;
+}
+
+static int check_vars(const afgpu_t *h, int n_vars, const int *ivs, VarPtrs &vp, const char *who) {
+  if (n_vars < 1 || n_vars > AFGPU_DIAG_MAX_VARS || !ivs) {
+    set_err(std::string(who) + ": 1 to AFGPU_DIAG_MAX_VARS variables");
+    return AFGPU_ERR_ARG;
+  }
+  vp.n = n_vars;
+  for (int v = 0; v < n_vars; v++) {
+    if (ivs[v] < 0 || ivs[v] >= AFGPU_N_CC) {
+      set_err(std::string(who) + ": variable index out of range");
+      return AFGPU_ERR_ARG;
+    }
+    vp.v[v] = h->cc[ivs[v]];
+  }
+  return AFGPU_OK;
+}
+
+// The statistics of output_regression_log (src/m_output.f90:797-810) per
+// run: the per-box terms on the device, summed per run in a fixed order.
+int afgpu_regression_values(afgpu_t *h, int n_vars, const int *ivs, double *out) {
+  RCHK(ready(h));
+  VarPtrs vp;
+  RCHK(check_vars(h, n_vars, ivs, vp, "afgpu_regression_values"));
+  if (!out) return AFGPU_ERR_ARG;
+  const int R = h->n_runs, nw = h->l_all_leaves.n;
+  const size_t need = (size_t)std::max(nw, 1) * n_vars * 3 + (size_t)R * n_vars * 3;
+  if (need > h->diag_cap) {
+    (void)hipFree(h->d_diag);
+    h->d_diag = nullptr;
+    h->diag_cap = 0;
+    HIPCHK(hipMalloc(&h->d_diag, need * sizeof(double)));
+    h->diag_cap = need;
+  }
+  double *part = h->d_diag, *dout = h->d_diag + (size_t)std::max(nw, 1) * n_vars * 3;
+  h->cur = h->stream;
+  RCHK(h->impl->diag_boxes(vp, part));
+  hipLaunchKernelGGL(k_diag_fold<256>, dim3(R), dim3(256), 0, h->stream, h->list(h->l_all_leaves),
+                     (const DBox *)h->d_boxes, (const double *)part, nw, n_vars, (int)(R == 1), dout);
+  HIPCHK(hipGetLastError());
+  std::vector<double> res((size_t)R * n_vars * 3);
+  HIPCHK(hipMemcpyAsync(res.data(), dout, res.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  RCHK(sync_stream(h));
+  for (int r = 0; r < R; r++) {
+    const double vol = run_volume(h, r);
+    for (int v = 0; v < n_vars; v++) {
+      const double *x = &res[((size_t)r * n_vars + v) * 3];
+      double *o = out + ((size_t)r * n_vars + v) * 3;
+      o[0] = x[0] / vol;
+      o[1] = x[1] / vol;
+      o[2] = x[2];
+    }
+  }
+  return AFGPU_OK;
+}
+
+// Fortran's Ew.d with a three-digit exponent (E20.8e3): 0.dddddddd, the
+// decimal exponent as E+eee, right-aligned in w characters.
+static int fortran_e(char *dst, size_t cap, double x, int w, int d) {
+  char tmp[64];
+  if (!std::isfinite(x)) return std::snprintf(dst, cap, "%*s", w, std::isnan(x) ? "NaN" : (x > 0 ? "Infinity" : "-Infinity"));
+  if (x == 0) {
+    std::snprintf(tmp, sizeof tmp, "%s0.%0*dE+000", std::signbit(x) ? "-" : "", d, 0);
+  } else {
+    // d significant digits, correctly rounded: d.ddd...e+X, then 0.dddd E+(X+1)
+    char sci[64];
+    std::snprintf(sci, sizeof sci, "%.*e", d - 1, std::fabs(x));
+    const char *e = std::strchr(sci, 'e');
+    const int ex = std::atoi(e + 1) + 1;
+    std::string digits;
+    for (const char *p = sci; p < e; p++)
+      if (*p != '.') digits += *p;
+    std::snprintf(tmp, sizeof tmp, "%s0.%sE%c%03d", x < 0 ? "-" : "", digits.c_str(), ex < 0 ? '-' : '+',
+                  ex < 0 ? -ex : ex);
+  }
+  return std::snprintf(dst, cap, "%*s", w, tmp);
+}
+
+int afgpu_format_regression_line(int out_cnt, int n_vals, const double *vals, char *buf, int buf_len) {
+  if (n_vals < 0 || (n_vals > 0 && !vals) || !buf || buf_len < 1) return AFGPU_ERR_ARG;
+  std::string line = std::to_string(out_cnt);
+  for (int n = 0; n < n_vals; n++) {
+    char f[64];
+    fortran_e(f, sizeof f, vals[n], 20, 8);
+    line += f;
+  }
+  if ((int)line.size() + 1 > buf_len) {
+    set_err("afgpu_format_regression_line: buffer too short");
+    return AFGPU_ERR_ARG;
+  }
+  std::memcpy(buf, line.c_str(), line.size() + 1);
+  return (int)line.size();
+}
+
+// output_regression_log (src/m_output.f90:788-842) on a plain tree
+int afgpu_write_regression_log(afgpu_t *h, const char *filename, int out_cnt, int n_vars, const int *ivs,
+                               const char *const *names) {
+  RCHK(ready(h));
+  if (!filename || !names || out_cnt < 0) return AFGPU_ERR_ARG;
+  if (h->n_runs != 1) {
+    set_err("afgpu_write_regression_log: a forest of runs (use afgpu_regression_values)");
+    return AFGPU_ERR_STATE;
+  }
+  for (int v = 0; v < n_vars; v++)
+    if (!names[v]) return AFGPU_ERR_ARG;
+  std::vector<double> vals(3 + 3 * (size_t)std::max(n_vars, 0));
+  RCHK(afgpu_regression_values(h, n_vars, ivs, vals.data() + 3));
+  // (per variable: sum, sum of squares, maximum) -> the log's column order
+  std::vector<double> line(3 + 3 * (size_t)n_vars);
+  line[0] = 0;  // (out_cnt is written as an integer)
+  line[1] = h->time;
+  line[2] = h->started ? h->global_dt : h->p.dt_min;
+  for (int v = 0; v < n_vars; v++) {
+    line[3 + v] = vals[3 + 3 * v];
+    line[3 + n_vars + v] = vals[3 + 3 * v + 1];
+    line[3 + 2 * n_vars + v] = vals[3 + 3 * v + 2];
+  }
+  std::FILE *f = std::fopen(filename, out_cnt == 0 ? "w" : "a");
+  if (!f) {
+    set_err(std::string("afgpu_write_regression_log: cannot open ") + filename);
+    return AFGPU_ERR_ARG;
+  }
+  if (out_cnt == 0) {
+    std::fputs("it time dt", f);
+    for (int v = 0; v < n_vars; v++) std::fprintf(f, " sum(%s)", names[v]);
+    for (int v = 0; v < n_vars; v++) std::fprintf(f, " sum(%s^2)", names[v]);
+    for (int v = 0; v < n_vars; v++) std::fprintf(f, " max(%s)", names[v]);
+    std::fputs("\n", f);
+  }
+  std::vector<char> buf(64 + 24 * line.size());
+  const int len = afgpu_format_regression_line(out_cnt, (int)line.size() - 1, line.data() + 1, buf.data(),
+                                               (int)buf.size());
+  if (len < 0) {
+    std::fclose(f);
+    return len;
+  }
+  std::fputs(buf.data(), f);
+  std::fputs("\n", f);
+  return std::fclose(f) == 0 ? AFGPU_OK : AFGPU_ERR_ARG;
+}
+
+// Box data in af_write_tree's order (afivo/src/m_af_output.f90:90-114)
+static int box_io(afgpu_t *h, int n_vars, const int *ivs, int n_ids, const int *ids, double *data, bool put) {
+  RCHK(ready(h));
+  VarPtrs vp;
+  RCHK(check_vars(h, n_vars, ivs, vp, put ? "afgpu_put_boxes" : "afgpu_get_boxes"));
+  if (n_ids < 0 || (n_ids > 0 && (!ids || !data))) return AFGPU_ERR_ARG;
+  if (n_ids == 0) return AFGPU_OK;
+  for (int q = 0; q < n_ids; q++)
+    if (ids[q] < 1 || ids[q] > h->n_boxes) {
+      set_err("box I/O: box id out of range");
+      return AFGPU_ERR_ARG;
+    }
+  const size_t words = (size_t)n_ids * n_vars * h->ncell;
+  if (words > h->io_cap) {
+    (void)hipFree(h->d_io);
+    h->d_io = nullptr;
+    h->io_cap = 0;
+    HIPCHK(hipMalloc(&h->d_io, words * sizeof(double)));
+    h->io_cap = words;
+  }
+  RCHK(ensure_ref_buffers(h, std::max(n_ids, h->n_boxes)));
+  HIPCHK(hipMemcpyAsync(h->d_ref_ids, ids, (size_t)n_ids * sizeof(int), hipMemcpyHostToDevice, h->stream));
+  const int nw = n_ids * n_vars;
+  if (put) HIPCHK(hipMemcpyAsync(h->d_io, data, words * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(k_box_io, dim3(std::min(nw, 8 * h->n_cu)), dim3(256), 0, h->stream, (const int *)h->d_ref_ids,
+                     vp, (int)h->ncell, h->d_io, (int)put, nw);
+  HIPCHK(hipGetLastError());
+  if (!put) HIPCHK(hipMemcpyAsync(data, h->d_io, words * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  return sync_stream(h);
+}
+
+int afgpu_get_boxes(afgpu_t *h, int n_vars, const int *ivs, int n_ids, const int *ids, double *data) {
+  return box_io(h, n_vars, ivs, n_ids, ids, data, false);
+}
+
+int afgpu_put_boxes(afgpu_t *h, int n_vars, const int *ivs, int n_ids, const int *ids, const double *data) {
+  return box_io(h, n_vars, ivs, n_ids, ids, const_cast<double *>(data), true);
 }
 
 int afgpu_copy_current_state(afgpu_t *h) {

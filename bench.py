@@ -168,6 +168,49 @@ def measure_refine(case, p, c=None, reps=10):
     return out
 
 
+def measure_diagnostics(case, p, c=None, reps=10):
+    """The regression log's statistics of the three densities from the
+    device pool (afgpu_regression_values) against downloading the densities
+    for the host to reduce, and the oracle reducing them; whole boxes of
+    seven variables in af_write_tree's order (afgpu_get_boxes) against seven
+    per-variable downloads."""
+    t = case.extra["export"]
+    D, nc, n = case.D, case.nc, t["n_boxes"]
+    ivs = [A.CC_ELECTRON, A.CC_POS_ION, A.CC_NEG_ION]
+    p.regression_values(ivs)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        p.regression_values(ivs)
+    gpu_ms = (time.perf_counter() - t0) * 1e3 / reps
+    t0 = time.perf_counter()
+    for iv in ivs:
+        p.get_cc(iv)
+    dl_ms = (time.perf_counter() - t0) * 1e3
+    leaves = int((t["children"].reshape(n, 2 ** D)[:, 0] == 0).sum())
+    out = {"regression_values_ms": round(gpu_ms, 3), "download_densities_ms": round(dl_ms, 3),
+           "leaf_cells": leaves * nc ** D, "algo_bytes": 3 * leaves * nc ** D * 8,
+           "timing": "wall time of afgpu_regression_values (3 densities: sums, sums of squares, maxima; "
+                     "result back), mean of %d calls" % reps}
+    if c is not None:
+        t0 = time.perf_counter()
+        c.regression_values(ivs)
+        out["cpu_oracle_ms"] = round((time.perf_counter() - t0) * 1e3, 3)
+        out["cpu_oracle_cores"] = 1
+    io_vars = [A.CC_PHOTO, A.CC_EFIELD, A.CC_PHI, A.CC_ELECTRON, A.CC_POS_ION, A.CC_NEG_ION, A.CC_RHS]
+    p.get_boxes(io_vars[:1], np.arange(1, 3, dtype=np.int32))
+    t0 = time.perf_counter()
+    boxes = p.get_boxes(io_vars)
+    out["get_boxes_ms"] = round((time.perf_counter() - t0) * 1e3, 3)
+    out["get_boxes_bytes"] = int(boxes.nbytes)
+    t0 = time.perf_counter()
+    for iv in io_vars:
+        p.get_cc(iv)
+    out["get_cc_7_vars_ms"] = round((time.perf_counter() - t0) * 1e3, 3)
+    del boxes
+    log("  diagnostics", json.dumps(out))
+    return out
+
+
 def measure_new_boxes(case, p, c=None):
     """Data of new boxes on the device (afgpu_prolong_boxes) after refining
     a tenth of the finest leaves whose face neighbours all exist, against
@@ -503,6 +546,7 @@ def main():
                          "after warmup, %.1f s" % (args.cpu_steps, info["leaf_cells"], csecs)}
         log("  cpu %.4e cell-steps/s (%.1f s)" % (cpu["value"], csecs))
     extra["refine_flags"] = measure_refine(case, p, None if args.no_cpu else c)
+    extra["diagnostics"] = measure_diagnostics(case, p, None if args.no_cpu else c)
     extra["new_boxes"] = measure_new_boxes(case, p, None if args.no_cpu else c)
 
     out = {

@@ -272,6 +272,40 @@ int afgpu_refine_flags(afgpu_t *h, const afgpu_refine_t *rp, int n_boxes, int *r
 int afgpu_restrict_boxes(afgpu_t *h, int n_ids, const int *ids);
 int afgpu_prolong_boxes(afgpu_t *h, int n_ids, const int *ids);
 
+/* Diagnostics from the device pool.
+ * - afgpu_regression_values: the statistics of output_regression_log
+ *   (src/m_output.f90:788-842) for n_vars cell-centred variables (at most
+ *   AFGPU_DIAG_MAX_VARS), per run of the tree (one for a plain tree):
+ *   out[(r * n_vars + v) * 3 + 0] = af_tree_sum_cc / af_total_volume,
+ *   [.. + 1] the same with power 2, [.. + 2] af_tree_max_cc
+ *   (afivo/src/m_af_utils.f90:758-769, 966-1026; m_af_types.f90:805-825).
+ *   The sums are deterministic but in another order than the reference's
+ *   (whose own order depends on its OpenMP thread count); the maxima are
+ *   exact.
+ * - afgpu_format_regression_line: host only (no GPU use); one line of the
+ *   log in the reference's format "(I0, n E20.8e3)" without the newline;
+ *   returns the length, or a negative status if buf is too short.
+ * - afgpu_write_regression_log: output_regression_log itself on a plain
+ *   tree: at out_cnt == 0 the file is (re)created with the header
+ *   "it time dt sum(name) ... sum(name^2) ... max(name) ...", then one line
+ *   (out_cnt, time, dt and the values) is appended, time and dt being the
+ *   handle's time state. */
+#define AFGPU_DIAG_MAX_VARS 8
+int afgpu_regression_values(afgpu_t *h, int n_vars, const int *ivs, double *out);
+int afgpu_format_regression_line(int out_cnt, int n_vals, const double *vals, char *buf, int buf_len);
+int afgpu_write_regression_log(afgpu_t *h, const char *filename, int out_cnt, int n_vars, const int *ivs,
+                               const char *const *names);
+
+/* Box data for checkpoints and restarts, in the order af_write_tree writes
+ * it (afivo/src/m_af_output.f90:90-114: box by box, and within a box the
+ * variables in turn, each as box%cc(DTIMES(:), iv)); af_read_tree reads
+ * the same order (m_af_output.f90:269-295).  data holds
+ * n_ids * n_vars * (n_cell + 2)^D doubles; at most AFGPU_DIAG_MAX_VARS
+ * variables per call.  One gather (or scatter) on the device and one copy
+ * over the host link. */
+int afgpu_get_boxes(afgpu_t *h, int n_vars, const int *ivs, int n_ids, const int *ids, double *data);
+int afgpu_put_boxes(afgpu_t *h, int n_vars, const int *ivs, int n_ids, const int *ids, const double *data);
+
 /* photoi_set_src (src/m_photoi.f90:153) with the Helmholtz method
  * (src/m_photoi_helmh.f90:162). */
 int afgpu_photoi_set_src(afgpu_t *h);

@@ -2391,6 +2391,107 @@ int afcpu_prolong_boxes(afcpu_t *h, int n_ids, const int *ids) {
   return AFGPU_OK;
 }
 
+// output_regression_log's statistics (src/m_output.f90:797-810) as the
+// reference computes them with one thread: af_tree_sum_cc
+// (afivo/src/m_af_utils.f90:966-1026: levels in order, their leaves in
+// order, each box's interior summed with i fastest, sum_2pr_box in 2D
+// cylindrical coordinates), af_tree_max_cc (:758-769, box_max_cc and
+// reduce_max) and af_total_volume (m_af_types.f90:805-825).
+int afcpu_regression_values(afcpu_t *h, int n_vars, const int *ivs, double *out) {
+  int rc = ready(h, -1);
+  if (rc) return rc;
+  if (n_vars < 1 || n_vars > AFGPU_DIAG_MAX_VARS || !ivs || !out) return AFGPU_ERR_ARG;
+  for (int v = 0; v < n_vars; v++)
+    if (ivs[v] < 0 || ivs[v] >= AFGPU_N_CC) return AFGPU_ERR_ARG;
+  Ctx &c = h->c;
+  const int D = c.D, nc = c.nc;
+  const int kmin = (D == 3) ? 1 : 0, kmax = (D == 3) ? nc : 0;
+  const double pi = std::acos(-1.0), two_pi = 2 * std::acos(-1.0);
+  const Box &b1 = c.B(c.ids[1][0]);
+  double box_len[3];
+  for (int d = 0; d < D; d++) box_len[d] = nc * b1.dr[d];
+  double vol;
+  if (D == 2 && c.coord == AFGPU_CYL) {
+    vol = 0.0;
+    const double r0 = b1.r_min[0], r1 = r0 + box_len[0];
+    vol = vol + pi * (r1 * r1 - r0 * r0) * box_len[D - 1];
+  } else {
+    double p = 1;
+    for (int d = 0; d < D; d++) p = p * box_len[d];
+    vol = (double)c.ids[1].size() * p;
+  }
+  for (int v = 0; v < n_vars; v++) {
+    const int iv = ivs[v];
+    for (int pow = 1; pow <= 2; pow++) {
+      double my_sum = 0;
+      for (int lvl = 1; lvl <= c.highest_lvl; lvl++) {
+        if (c.ids[lvl].empty()) continue;
+        const Box &bl = c.B(c.ids[lvl][0]);
+        double fac = 1;
+        for (int d = 0; d < D; d++) fac = fac * bl.dr[d];
+        for (int id : c.leaves[lvl]) {
+          const double *u = c.C(iv, id);
+          const Box &bx = c.B(id);
+          double tmp = 0;
+          for (int k = kmin; k <= kmax; k++)
+            for (int j = 1; j <= nc; j++)
+              for (int i = 1; i <= nc; i++) {
+                const double x = u[c.idx(i, j, k)];
+                const double xp = (pow == 1) ? x : x * x;
+                if (D == 2 && c.coord == AFGPU_CYL)
+                  tmp = tmp + xp * (bx.r_min[0] + (i - 0.5) * bx.dr[0]);
+                else
+                  tmp = tmp + xp;
+              }
+          if (D == 2 && c.coord == AFGPU_CYL) tmp = tmp * two_pi;
+          my_sum = my_sum + fac * tmp;
+        }
+      }
+      out[3 * v + pow - 1] = my_sum / vol;
+    }
+    double mx = -std::numeric_limits<double>::max() / 10;
+    for (int lvl = 1; lvl <= c.highest_lvl; lvl++)
+      for (int id : c.leaves[lvl]) {
+        const double *u = c.C(iv, id);
+        double bm = -std::numeric_limits<double>::max();
+        for (int k = kmin; k <= kmax; k++)
+          for (int j = 1; j <= nc; j++)
+            for (int i = 1; i <= nc; i++) bm = std::max(bm, u[c.idx(i, j, k)]);
+        mx = std::max(bm, mx);
+      }
+    out[3 * v + 2] = mx;
+  }
+  return AFGPU_OK;
+}
+
+// Box data in af_write_tree's order (afivo/src/m_af_output.f90:90-114)
+static int box_io(afcpu_t *h, int n_vars, const int *ivs, int n_ids, const int *ids, double *data, bool put) {
+  CHECK_H(h);
+  if (n_vars < 1 || n_vars > AFGPU_DIAG_MAX_VARS || !ivs || n_ids < 0 || (n_ids > 0 && (!ids || !data)))
+    return AFGPU_ERR_ARG;
+  for (int v = 0; v < n_vars; v++)
+    if (ivs[v] < 0 || ivs[v] >= AFGPU_N_CC) return AFGPU_ERR_ARG;
+  for (int q = 0; q < n_ids; q++)
+    if (ids[q] < 1 || ids[q] > h->c.n_boxes) return AFGPU_ERR_ARG;
+  const size_t nc = h->c.ncell;
+  for (int q = 0; q < n_ids; q++)
+    for (int v = 0; v < n_vars; v++) {
+      double *b = data + ((size_t)q * n_vars + v) * nc;
+      double *u = h->c.C(ivs[v], ids[q]);
+      if (put)
+        std::memcpy(u, b, nc * sizeof(double));
+      else
+        std::memcpy(b, u, nc * sizeof(double));
+    }
+  return AFGPU_OK;
+}
+int afcpu_get_boxes(afcpu_t *h, int n_vars, const int *ivs, int n_ids, const int *ids, double *data) {
+  return box_io(h, n_vars, ivs, n_ids, ids, data, false);
+}
+int afcpu_put_boxes(afcpu_t *h, int n_vars, const int *ivs, int n_ids, const int *ids, const double *data) {
+  return box_io(h, n_vars, ivs, n_ids, ids, const_cast<double *>(data), true);
+}
+
 int afcpu_copy_current_state(afcpu_t *h) {
   int rc = ready(h, -1);
   if (rc) return rc;
