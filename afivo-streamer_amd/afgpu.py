@@ -236,6 +236,30 @@ _ORACLE_ONLY = {
     "num_threads": (_I, []),
 }
 
+# general chemistry (afgpu_chem_*): rate types with the reference's codes
+RATE_TABULATED_ENERGY, RATE_TABULATED_FIELD, RATE_CONSTANT, RATE_LINEAR = 0, 1, 2, 3
+RATE_EXP_V1, RATE_EXP_V2, RATE_K1 = 4, 5, 6
+(RATE_K3, RATE_K4, RATE_K5, RATE_K6, RATE_K7, RATE_K8, RATE_K9, RATE_K10, RATE_K11, RATE_K12, RATE_K13,
+ RATE_K14, RATE_K15, RATE_K16) = range(8, 22)
+CHEM_MAX_IN = CHEM_MAX_OUT = 8
+
+
+class ReactionT(C.Structure):
+    _fields_ = [("rate_type", C.c_int), ("lookup_col", C.c_int), ("n_in", C.c_int), ("n_out", C.c_int),
+                ("rate_factor", C.c_double), ("rate_data", C.c_double * 9),
+                ("ix_in", C.c_int * CHEM_MAX_IN), ("ix_out", C.c_int * CHEM_MAX_OUT),
+                ("multiplicity_out", C.c_int * CHEM_MAX_OUT)]
+
+
+_VP = C.c_void_p
+_CHEM = {
+    "chem_create": (_I, [_I, _I, C.POINTER(ReactionT), _I, _D, _D, _I, _DP, _DP, _D, C.POINTER(_VP)]),
+    "chem_destroy": (_I, [_VP]),
+    "chem_rates": (_I, [_VP, _I, _VP, _VP]),
+    "chem_derivatives": (_I, [_VP, _I, _VP, _VP, _VP]),
+    "chem_source": (_I, [_VP, _I, _VP, _VP, _VP]),
+}
+
 _libs = {}
 
 
@@ -258,6 +282,7 @@ def load_gpu():
         lib = C.CDLL(GPU_LIB)
         _bind(lib, "afgpu_", _COMMON)
         _bind(lib, "afgpu_", _GPU_ONLY)
+        _bind(lib, "afgpu_", _CHEM)
         _libs["gpu"] = lib
     return _libs["gpu"]
 
@@ -269,6 +294,7 @@ def load_oracle():
         lib = C.CDLL(ORACLE_LIB)
         _bind(lib, "afcpu_", _COMMON)
         _bind(lib, "afcpu_", _ORACLE_ONLY)
+        _bind(lib, "afcpu_", _CHEM)
         _libs["oracle"] = lib
     return _libs["oracle"]
 
@@ -385,6 +411,12 @@ def transport_tables(E, mu, dif, alpha, eta, N, table_size=1000, table_max_td=-1
         raise AfError("transport_tables failed")
     return {"n": table_size, "td_x_min": tx0.value, "td_inv_fac": tif.value, "td": td,
             "chem_x_min": cx0.value, "chem_inv_fac": cif.value, "chem": ch}
+
+
+def last_error(kind):
+    lib = load_gpu() if kind == "gpu" else load_oracle()
+    msg = (lib.afgpu_last_error if kind == "gpu" else lib.afcpu_last_error)()
+    return msg.decode() if msg else ""
 
 
 def format_regression_line(out_cnt, vals):

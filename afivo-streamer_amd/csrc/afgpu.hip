@@ -26,6 +26,7 @@
 #include "afgpu.h"
 #include "afgpu_direct.h"
 #include "afgpu_fluid.h"
+#include "afgpu_chem.h"
 #include "afgpu_diag.h"
 #include "afgpu_refine.h"
 
@@ -3560,3 +3561,120 @@ int afgpu_set_option(afgpu_t *h, int option, int value) {
 }
 
 }  // extern "C"
+
+// ---- general chemistry networks (src/m_chemistry.f90:672-804) --------------
+struct afgpu_chem {
+  int n_species = 0, n_reactions = 0;
+  afgpu_reaction_t *d_r = nullptr;
+  double *d_fld = nullptr, *d_energy = nullptr;
+  ChemArgs args{};
+  hipStream_t stream = nullptr;
+};
+
+int afgpu_chem_create(int n_species, int n_reactions, const afgpu_reaction_t *reactions, int table_n,
+                      double table_x_min, double table_x_max, int n_fld_cols, const double *fld_cols,
+                      const double *energy_col, double gas_temperature, afgpu_chem_t **out) {
+  if (!out || n_species < 1 || n_species > AFGPU_CHEM_MAX_SPECIES || n_reactions < 0 || (n_reactions > 0 && !reactions) ||
+      table_n < 2 || !(table_x_max > table_x_min) || n_fld_cols < 0 || (n_fld_cols > 0 && !fld_cols) || !(gas_temperature > 0)) {
+    set_err("afgpu_chem_create: invalid arguments");
+    return AFGPU_ERR_ARG;
+  }
+  bool need_te = false;
+  for (int r = 0; r < n_reactions; r++) {
+    const afgpu_reaction_t &R = reactions[r];
+    const int t = R.rate_type;
+    const bool known = (t >= AFGPU_RATE_TABULATED_FIELD && t <= AFGPU_RATE_K1) || (t >= AFGPU_RATE_K3 && t <= AFGPU_RATE_K16);
+    if (!known) {
+      set_err(t == AFGPU_RATE_TABULATED_ENERGY ? "afgpu_chem_create: energy tables need the energy equation"
+                                               : "afgpu_chem_create: unknown rate type");
+      return AFGPU_ERR_UNSUPPORTED;
+    }
+    if (R.n_in < 1 || R.n_in > AFGPU_CHEM_MAX_IN || R.n_out < 0 || R.n_out > AFGPU_CHEM_MAX_OUT ||
+        (t == AFGPU_RATE_TABULATED_FIELD && (R.lookup_col < 0 || R.lookup_col >= n_fld_cols))) {
+      set_err("afgpu_chem_create: invalid reaction");
+      return AFGPU_ERR_ARG;
+    }
+    for (int q = 0; q < R.n_in; q++)
+      if (R.ix_in[q] < 0 || R.ix_in[q] >= n_species) return set_err("afgpu_chem_create: species index"), AFGPU_ERR_ARG;
+    for (int q = 0; q < R.n_out; q++)
+      if (R.ix_out[q] < 0 || R.ix_out[q] >= n_species) return set_err("afgpu_chem_create: species index"), AFGPU_ERR_ARG;
+    need_te = need_te || t == AFGPU_RATE_K1 || t == AFGPU_RATE_K3 || t == AFGPU_RATE_K16;
+  }
+  if (need_te && !energy_col) {
+    set_err("afgpu_chem_create: the rate types with Te need the mean-energy column");
+    return AFGPU_ERR_ARG;
+  }
+  std::unique_ptr<afgpu_chem> c(new afgpu_chem);
+  c->n_species = n_species;
+  c->n_reactions = n_reactions;
+  if (n_reactions > 0) {
+    HIPCHK(hipMalloc(&c->d_r, (size_t)n_reactions * sizeof(afgpu_reaction_t)));
+    HIPCHK(hipMemcpy(c->d_r, reactions, (size_t)n_reactions * sizeof(afgpu_reaction_t), hipMemcpyHostToDevice));
+  }
+  if (n_fld_cols > 0) {
+    HIPCHK(hipMalloc(&c->d_fld, (size_t)n_fld_cols * table_n * sizeof(double)));
+    HIPCHK(hipMemcpy(c->d_fld, fld_cols, (size_t)n_fld_cols * table_n * sizeof(double), hipMemcpyHostToDevice));
+  }
+  if (energy_col) {
+    HIPCHK(hipMalloc(&c->d_energy, (size_t)table_n * sizeof(double)));
+    HIPCHK(hipMemcpy(c->d_energy, energy_col, (size_t)table_n * sizeof(double), hipMemcpyHostToDevice));
+  }
+  // LT_create (src/lookup_table_fortran/m_lookup_table.f90): linear spacing
+  const double inv_fac = (table_n - 1) / (table_x_max - table_x_min);
+  c->args.r = c->d_r;
+  c->args.n_reactions = n_reactions;
+  c->args.n_species = n_species;
+  c->args.fld = DTable{c->d_fld, table_n, table_x_min, inv_fac};
+  c->args.energy = DTable{c->d_energy, table_n, table_x_min, inv_fac};
+  c->args.tg = gas_temperature;
+  c->args.need_te = need_te ? 1 : 0;
+  HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  *out = c.release();
+  return AFGPU_OK;
+}
+
+int afgpu_chem_destroy(afgpu_chem_t *c) {
+  if (!c) return AFGPU_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(c->d_r);
+  (void)hipFree(c->d_fld);
+  (void)hipFree(c->d_energy);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return AFGPU_OK;
+}
+
+static int chem_launch(afgpu_chem_t *c, int n_cells, const double *fields, const double *dens, double *rates,
+                       double *derivs, int mode) {
+  if (!c || n_cells < 0) return AFGPU_ERR_ARG;
+  if (n_cells == 0) return AFGPU_OK;
+  if ((mode != 1 && !fields) || (mode != 0 && (!dens || !derivs)) || (mode != 2 && c->n_reactions > 0 && !rates)) {
+    set_err("afgpu_chem: missing array");
+    return AFGPU_ERR_ARG;
+  }
+  // LDS per workgroup within 48 KiB: 256 cells with their densities staged
+  // for small networks, else 64 cells (densities staged while they fit)
+  const size_t ns = (size_t)c->n_species, cap = 48 * 1024;
+  if (2 * ns * 256 * sizeof(double) <= cap)
+    hipLaunchKernelGGL((k_chem<256, true>), dim3((n_cells + 255) / 256), dim3(256), 2 * ns * 256 * sizeof(double),
+                       c->stream, c->args, n_cells, fields, dens, rates, derivs, mode);
+  else if (2 * ns * 64 * sizeof(double) <= cap)
+    hipLaunchKernelGGL((k_chem<64, true>), dim3((n_cells + 63) / 64), dim3(64), 2 * ns * 64 * sizeof(double),
+                       c->stream, c->args, n_cells, fields, dens, rates, derivs, mode);
+  else
+    hipLaunchKernelGGL((k_chem<64, false>), dim3((n_cells + 63) / 64), dim3(64), ns * 64 * sizeof(double),
+                       c->stream, c->args, n_cells, fields, dens, rates, derivs, mode);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return AFGPU_OK;
+}
+
+int afgpu_chem_rates(afgpu_chem_t *c, int n_cells, const double *fields, double *rates) {
+  return chem_launch(c, n_cells, fields, nullptr, rates, nullptr, 0);
+}
+int afgpu_chem_derivatives(afgpu_chem_t *c, int n_cells, const double *dens, double *rates, double *derivs) {
+  return chem_launch(c, n_cells, nullptr, dens, rates, derivs, 1);
+}
+int afgpu_chem_source(afgpu_chem_t *c, int n_cells, const double *fields, const double *dens, double *derivs) {
+  return chem_launch(c, n_cells, fields, dens, nullptr, derivs, 2);
+}

@@ -211,6 +211,52 @@ def measure_diagnostics(case, p, c=None, reps=10):
     return out
 
 
+def measure_chemistry(n_cells, with_cpu, reps=10):
+    """The source term of the reference's air network (8 species, 25
+    reactions; tests/golden/air_chemistry_v1.txt) on n_cells cells resident
+    in device memory (afgpu_chem_source), against the oracle on a sample."""
+    import torch
+    import chemistry as CH
+    path = os.path.join(ROOT, "tests", "golden", "air_chemistry_v1.txt")
+    net = CH.read_network(path, {"N2": 0.8, "O2": 0.2}, S.gas_number_density(), 0.0, 1000.0, 1000)
+    ns = len(net.species)
+    g = CH.Chem("gpu", ns, net.reactions, 0.0, 1000.0, net.fld_cols, None, 300.0)
+    rng = np.random.default_rng(3)
+    dev = torch.device("cuda:0")
+    # (host arrays uploaded whole: plain copies, no torch kernels)
+    fields_h = rng.uniform(0.0, 300.0, n_cells)
+    dens_h = 10.0 ** rng.uniform(12, 20, (ns, n_cells))
+    fields = torch.from_numpy(fields_h).to(dev)
+    dens = torch.from_numpy(dens_h).to(dev)
+    derivs = torch.empty((ns, n_cells), dtype=torch.float64, device=dev)
+    g.source(fields, dens, derivs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g.source(fields, dens, derivs)
+    gpu_s = (time.perf_counter() - t0) / reps
+    out = {"network": "air_chemistry_v1 (8 species, 25 reactions)", "cells": n_cells,
+           "source_ms": round(gpu_s * 1e3, 3), "cells_per_s": n_cells / gpu_s,
+           "algo_bytes": n_cells * (1 + 2 * ns) * 8,
+           "timing": "wall time of afgpu_chem_source (one launch, synchronised), mean of %d calls" % reps}
+    out["achieved_GBs"] = out["algo_bytes"] / gpu_s / 1e9
+    if with_cpu:
+        m = 200000
+        o = CH.Chem("oracle", ns, net.reactions, 0.0, 1000.0, net.fld_cols, None, 300.0)
+        f = fields_h[:m].copy()
+        d = np.ascontiguousarray(dens_h[:, :m])
+        dv = np.zeros((ns, m))
+        t0 = time.perf_counter()
+        o.source(f, d, dv)
+        cs = time.perf_counter() - t0
+        out["cpu_oracle_cells_per_s"] = m / cs
+        out["cpu_oracle_sample"] = "%d cells, one host core, %.2f s" % (m, cs)
+    del fields, dens, derivs, fields_h, dens_h
+    g.close()
+    log("  chemistry", json.dumps(out))
+    return out
+
+
 def measure_new_boxes(case, p, c=None):
     """Data of new boxes on the device (afgpu_prolong_boxes) after refining
     a tenth of the finest leaves whose face neighbours all exist, against
@@ -547,6 +593,7 @@ def main():
         log("  cpu %.4e cell-steps/s (%.1f s)" % (cpu["value"], csecs))
     extra["refine_flags"] = measure_refine(case, p, None if args.no_cpu else c)
     extra["diagnostics"] = measure_diagnostics(case, p, None if args.no_cpu else c)
+    extra["chemistry"] = measure_chemistry(int(info["leaf_cells"]), not args.no_cpu)
     extra["new_boxes"] = measure_new_boxes(case, p, None if args.no_cpu else c)
 
     out = {

@@ -432,6 +432,72 @@ int afgpu_autotune(afgpu_t *h, int n_classes, afgpu_shape_t *shapes);
 int afgpu_get_shapes(afgpu_t *h, int n_classes, afgpu_shape_t *shapes);
 int afgpu_set_shapes(afgpu_t *h, int n_classes, const afgpu_shape_t *shapes);
 
+/* ---- General chemistry networks (src/m_chemistry.f90) ----------------------
+ * get_rates (m_chemistry.f90:672-768) for every rate type the reference
+ * parses except the electron-energy table, and get_derivatives (:770-804)
+ * for an arbitrary reaction list, on arrays resident in device memory
+ * (the pointers of afgpu_chem_rates / _derivatives / _source are device
+ * addresses).  Arrays are column-major like the reference's: cell index
+ * fastest, fields[n_cells] in Td, dens[n_species][n_cells],
+ * rates[n_reactions][n_cells], derivs[n_species][n_cells].
+ * A reaction is the compiled form read_reactions / parse_reaction
+ * (m_chemistry.f90:857-1142, 1156-1278) build: gas species and "M" folded
+ * into rate_factor, the input species repeated by multiplicity, the output
+ * species with their multiplicities; indices are 0-based plasma species. */
+#define AFGPU_CHEM_MAX_SPECIES 100 /* max_num_species, m_chemistry.f90:129 */
+#define AFGPU_CHEM_MAX_IN      8
+#define AFGPU_CHEM_MAX_OUT     8
+#define AFGPU_CHEM_MAX_COEFF   9   /* rate_max_num_coeff */
+/* rate types, with the reference's codes (m_chemistry.f90:66-126) */
+#define AFGPU_RATE_TABULATED_ENERGY 0  /* not supported: needs the energy equation */
+#define AFGPU_RATE_TABULATED_FIELD  1
+#define AFGPU_RATE_CONSTANT         2
+#define AFGPU_RATE_LINEAR           3
+#define AFGPU_RATE_EXP_V1           4
+#define AFGPU_RATE_EXP_V2           5
+#define AFGPU_RATE_K1               6
+#define AFGPU_RATE_K3               8
+#define AFGPU_RATE_K4               9
+#define AFGPU_RATE_K5               10
+#define AFGPU_RATE_K6               11
+#define AFGPU_RATE_K7               12
+#define AFGPU_RATE_K8               13
+#define AFGPU_RATE_K9               14
+#define AFGPU_RATE_K10              15
+#define AFGPU_RATE_K11              16
+#define AFGPU_RATE_K12              17
+#define AFGPU_RATE_K13              18
+#define AFGPU_RATE_K14              19
+#define AFGPU_RATE_K15              20
+#define AFGPU_RATE_K16              21
+typedef struct {
+  int    rate_type;
+  int    lookup_col;                        /* column of the field table (0-based) */
+  int    n_in, n_out;
+  double rate_factor;                       /* c0 */
+  double rate_data[AFGPU_CHEM_MAX_COEFF];   /* c(1:n_coeff) */
+  int    ix_in[AFGPU_CHEM_MAX_IN];          /* repeated by multiplicity */
+  int    ix_out[AFGPU_CHEM_MAX_OUT];
+  int    multiplicity_out[AFGPU_CHEM_MAX_OUT];
+} afgpu_reaction_t;
+typedef struct afgpu_chem afgpu_chem_t;
+/* table_n rows on the uniform grid of LT_create(x_min, x_max, table_n)
+ * (chemtbl_fld and td_tbl share it, m_chemistry.f90:344-345): fld_cols holds n_fld_cols columns of
+ * table_n values; energy_col (td_tbl's mean energy in eV, for the types
+ * that use Te: K1, K3, K16) may be NULL when no reaction needs it;
+ * gas_temperature is m_gas's. */
+int afgpu_chem_create(int n_species, int n_reactions, const afgpu_reaction_t *reactions, int table_n,
+                      double table_x_min, double table_x_max, int n_fld_cols, const double *fld_cols,
+                      const double *energy_col, double gas_temperature, afgpu_chem_t **out);
+int afgpu_chem_destroy(afgpu_chem_t *c);
+/* get_rates: the rate coefficients of every reaction in every cell */
+int afgpu_chem_rates(afgpu_chem_t *c, int n_cells, const double *fields, double *rates);
+/* get_derivatives: rates become the reaction rates (times the input
+ * densities), derivs the species' time derivatives */
+int afgpu_chem_derivatives(afgpu_chem_t *c, int n_cells, const double *dens, double *rates, double *derivs);
+/* both in one pass, without the rates array: the same derivs */
+int afgpu_chem_source(afgpu_chem_t *c, int n_cells, const double *fields, const double *dens, double *derivs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -2542,4 +2542,150 @@ long long afcpu_num_leaf_cells(afcpu_t *h) {
 
 int afcpu_num_threads(void) { return omp_get_max_threads(); }
 
+// ---- general chemistry networks ---------------------------------------------
+// get_rates (src/m_chemistry.f90:672-768) and get_derivatives (:770-804):
+// reaction by reaction over all cells, as the reference's array statements.
+struct afcpu_chem {
+  int n_species = 0, n_reactions = 0;
+  std::vector<afgpu_reaction_t> r;
+  Table fld, energy;
+  double tg = 300;
+  bool need_te = false;
+};
+
+int afcpu_chem_create(int n_species, int n_reactions, const afgpu_reaction_t *reactions, int table_n,
+                      double table_x_min, double table_x_max, int n_fld_cols, const double *fld_cols,
+                      const double *energy_col, double gas_temperature, afcpu_chem **out) {
+  if (!out || n_species < 1 || n_species > AFGPU_CHEM_MAX_SPECIES || n_reactions < 0 || (n_reactions > 0 && !reactions) ||
+      table_n < 2 || !(table_x_max > table_x_min) || n_fld_cols < 0 || (n_fld_cols > 0 && !fld_cols))
+    return AFGPU_ERR_ARG;
+  afcpu_chem *c = new afcpu_chem;
+  c->n_species = n_species;
+  c->n_reactions = n_reactions;
+  c->r.assign(reactions, reactions + n_reactions);
+  const double inv_fac = (table_n - 1) / (table_x_max - table_x_min);  // LT_create, linear spacing
+  c->fld.n = c->energy.n = table_n;
+  c->fld.x_min = c->energy.x_min = table_x_min;
+  c->fld.inv_fac = c->energy.inv_fac = inv_fac;
+  c->fld.cols.assign(fld_cols, fld_cols + (size_t)n_fld_cols * table_n);
+  if (energy_col) c->energy.cols.assign(energy_col, energy_col + table_n);
+  for (const auto &R : c->r) {
+    const int t = R.rate_type;
+    c->need_te = c->need_te || t == AFGPU_RATE_K1 || t == AFGPU_RATE_K3 || t == AFGPU_RATE_K16;
+    if (t == AFGPU_RATE_TABULATED_ENERGY || t == 7 || t < 0 || t > AFGPU_RATE_K16) {
+      delete c;
+      return AFGPU_ERR_UNSUPPORTED;
+    }
+  }
+  if (c->need_te && !energy_col) {
+    delete c;
+    return AFGPU_ERR_ARG;
+  }
+  c->tg = gas_temperature;
+  *out = c;
+  return AFGPU_OK;
+}
+
+int afcpu_chem_destroy(afcpu_chem *c) {
+  delete c;
+  return AFGPU_OK;
+}
+
+int afcpu_chem_rates(afcpu_chem *c, int n_cells, const double *fields, double *rates) {
+  if (!c || n_cells < 0) return AFGPU_ERR_ARG;
+  const size_t n = (size_t)n_cells;
+  const double kB = 1.380649e-23, eV = 1.602176634e-19;  // src/m_units_constants.f90:15-39
+  const double electron_eV_to_K = 2 * eV / (3 * kB);
+  const double tg = c->tg;
+  std::vector<double> Te;
+  bool te_available = false;
+  for (int r = 0; r < c->n_reactions; r++) {
+    const afgpu_reaction_t &R = c->r[r];
+    const double c0 = R.rate_factor;
+    const double *k = R.rate_data;
+    double *out = rates + (size_t)r * n;
+    const int t = R.rate_type;
+    if ((t == AFGPU_RATE_K1 || t == AFGPU_RATE_K3 || t == AFGPU_RATE_K16) && !te_available) {
+      Te.resize(n);
+      for (size_t i = 0; i < n; i++) Te[i] = electron_eV_to_K * c->energy.col(0, fields[i]);
+      te_available = true;
+    }
+    for (size_t i = 0; i < n; i++) {
+      const double f = fields[i];
+      double v = 0;
+      switch (t) {
+        case AFGPU_RATE_TABULATED_FIELD: v = c0 * c->fld.col(R.lookup_col, f); break;
+        case AFGPU_RATE_CONSTANT: v = c0 * k[0]; break;
+        case AFGPU_RATE_LINEAR: v = c0 * k[0] * (f - k[1]); break;
+        case AFGPU_RATE_EXP_V1: {
+          const double q = k[1] / (k[2] + f);
+          v = c0 * k[0] * std::exp(-(q * q));
+          break;
+        }
+        case AFGPU_RATE_EXP_V2: {
+          const double q = f / k[1];
+          v = c0 * k[0] * std::exp(-(q * q));
+          break;
+        }
+        case AFGPU_RATE_K1: v = c0 * k[0] * std::pow(300 / Te[i], k[1]); break;
+        case AFGPU_RATE_K3: {
+          const double q = (kB / eV) * Te[i] + k[1];
+          v = c0 * (k[0] * (q * q) - k[2]) * k[3];
+          break;
+        }
+        case AFGPU_RATE_K4: v = c0 * k[0] * std::pow(tg / 300, k[1]) * std::exp(-k[2] / tg); break;
+        case AFGPU_RATE_K5: v = c0 * k[0] * std::exp(-k[1] / tg); break;
+        case AFGPU_RATE_K6: v = c0 * k[0] * std::pow(tg, k[1]); break;
+        case AFGPU_RATE_K7: v = c0 * k[0] * std::pow(tg / k[1], k[2]); break;
+        case AFGPU_RATE_K8: v = c0 * k[0] * std::pow(300 / tg, k[1]); break;
+        case AFGPU_RATE_K9: v = c0 * k[0] * std::exp(-k[1] * tg); break;
+        case AFGPU_RATE_K10: v = c0 * std::pow(10.0, k[0] + k[1] * (tg - 300)); break;
+        case AFGPU_RATE_K11: v = c0 * k[0] * std::pow(300 / tg, k[1]) * std::exp(-k[2] / tg); break;
+        case AFGPU_RATE_K12: v = c0 * k[0] * std::pow(tg, k[1]) * std::exp(-k[2] / tg); break;
+        case AFGPU_RATE_K13: v = c0 * k[0] * std::exp(-std::pow(k[1] / (k[2] + f), k[3])); break;
+        case AFGPU_RATE_K14: v = c0 * k[0] * std::exp(-std::pow(f / k[1], k[2])); break;
+        case AFGPU_RATE_K15: v = c0 * k[0] * std::exp(-std::pow(k[1] / (kB * (tg + f / k[2])), k[3])); break;
+        case AFGPU_RATE_K16:
+          v = c0 * k[0] * std::pow(300 / Te[i], k[1]) * std::exp(-k[2] / tg) *
+              std::exp(k[3] * (Te[i] - tg) / (Te[i] * tg));
+          break;
+      }
+      out[i] = v;
+    }
+  }
+  return AFGPU_OK;
+}
+
+int afcpu_chem_derivatives(afcpu_chem *c, int n_cells, const double *dens, double *rates, double *derivs) {
+  if (!c || n_cells < 0) return AFGPU_ERR_ARG;
+  const size_t n = (size_t)n_cells;
+  for (size_t x = 0; x < (size_t)c->n_species * n; x++) derivs[x] = 0.0;
+  for (int r = 0; r < c->n_reactions; r++) {
+    const afgpu_reaction_t &R = c->r[r];
+    double *k = rates + (size_t)r * n;
+    for (size_t i = 0; i < n; i++) {
+      double prod = 1.0;
+      for (int q = 0; q < R.n_in; q++) prod = prod * dens[(size_t)R.ix_in[q] * n + i];
+      k[i] = k[i] * prod;
+    }
+    for (int q = 0; q < R.n_in; q++) {
+      double *d = derivs + (size_t)R.ix_in[q] * n;
+      for (size_t i = 0; i < n; i++) d[i] = d[i] - k[i];
+    }
+    for (int q = 0; q < R.n_out; q++) {
+      double *d = derivs + (size_t)R.ix_out[q] * n;
+      for (size_t i = 0; i < n; i++) d[i] = d[i] + k[i] * R.multiplicity_out[q];
+    }
+  }
+  return AFGPU_OK;
+}
+
+int afcpu_chem_source(afcpu_chem *c, int n_cells, const double *fields, const double *dens, double *derivs) {
+  if (!c || n_cells < 0) return AFGPU_ERR_ARG;
+  std::vector<double> rates((size_t)c->n_reactions * n_cells);
+  int rc = afcpu_chem_rates(c, n_cells, fields, rates.data());
+  if (rc) return rc;
+  return afcpu_chem_derivatives(c, n_cells, dens, rates.data(), derivs);
+}
+
 }  // extern "C"

@@ -23,3 +23,9 @@ def A():
 def S():
     import setups
     return setups
+
+
+@pytest.fixture(scope="session")
+def CH():
+    import chemistry
+    return chemistry
