@@ -259,6 +259,11 @@ _CHEM = {
     "chem_derivatives": (_I, [_VP, _I, _VP, _VP, _VP]),
     "chem_source": (_I, [_VP, _I, _VP, _VP, _VP]),
 }
+_GPU_MEM = {
+    "device_alloc": (_I, [C.c_size_t, C.POINTER(_VP)]),
+    "device_free": (_I, [_VP]),
+    "device_copy": (_I, [_VP, _VP, C.c_size_t, _I]),
+}
 
 _libs = {}
 
@@ -283,6 +288,7 @@ def load_gpu():
         _bind(lib, "afgpu_", _COMMON)
         _bind(lib, "afgpu_", _GPU_ONLY)
         _bind(lib, "afgpu_", _CHEM)
+        _bind(lib, "afgpu_", _GPU_MEM)
         _libs["gpu"] = lib
     return _libs["gpu"]
 
@@ -411,6 +417,50 @@ def transport_tables(E, mu, dif, alpha, eta, N, table_size=1000, table_max_td=-1
         raise AfError("transport_tables failed")
     return {"n": table_size, "td_x_min": tx0.value, "td_inv_fac": tif.value, "td": td,
             "chem_x_min": cx0.value, "chem_inv_fac": cif.value, "chem": ch}
+
+
+class DeviceArray:
+    """A float64 array in device memory, allocated by the device library's
+    own HIP runtime (afgpu_device_alloc); data_ptr() is its address."""
+
+    def __init__(self, shape, host=None):
+        self.lib = load_gpu()
+        self.shape = tuple(np.atleast_1d(shape)) if host is None else np.shape(host)
+        self.nbytes = int(np.prod(self.shape)) * 8
+        self.p = C.c_void_p()
+        if self.lib.afgpu_device_alloc(self.nbytes, C.byref(self.p)) != OK:
+            raise AfError("afgpu_device_alloc failed: " + last_error("gpu"))
+        if host is not None:
+            self.upload(host)
+
+    def data_ptr(self):
+        return self.p.value
+
+    def upload(self, host):
+        h = np.ascontiguousarray(host, np.float64)
+        assert h.nbytes == self.nbytes
+        if self.lib.afgpu_device_copy(self.p, h.ctypes.data_as(C.c_void_p), self.nbytes, 1) != OK:
+            raise AfError("afgpu_device_copy failed: " + last_error("gpu"))
+
+    def numpy(self):
+        out = np.empty(self.shape)
+        if self.lib.afgpu_device_copy(out.ctypes.data_as(C.c_void_p), self.p, self.nbytes, 0) != OK:
+            raise AfError("afgpu_device_copy failed: " + last_error("gpu"))
+        return out
+
+    def __len__(self):
+        return self.shape[0]
+
+    def free(self):
+        if getattr(self, "p", None) and self.p.value:
+            self.lib.afgpu_device_free(self.p)
+            self.p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def last_error(kind):

@@ -243,8 +243,9 @@ def reaction_array(reactions):
 
 
 class Chem:
-    """The device chemistry (kind 'gpu', arrays are torch tensors on the
-    GPU) or the oracle's (kind 'oracle', numpy arrays)."""
+    """The device chemistry (kind 'gpu': arrays in device memory, such as
+    afgpu.DeviceArray, anything with data_ptr()) or the oracle's (kind
+    'oracle': numpy arrays)."""
 
     def __init__(self, kind, n_species, reactions, table_x_min, table_x_max, fld_cols, energy_col=None,
                  gas_temperature=300.0):
@@ -280,7 +281,7 @@ class Chem:
         self._call("rates", len(fields), self._ptr(fields), self._ptr(out))
 
     def derivatives(self, dens, rates, derivs):
-        self._call("derivatives", dens.shape[-1], self._ptr(dens), self._ptr(rates), self._ptr(derivs))
+        self._call("derivatives", derivs.shape[-1], self._ptr(dens), self._ptr(rates), self._ptr(derivs))
 
     def source(self, fields, dens, derivs):
         self._call("source", len(fields), self._ptr(fields), self._ptr(dens), self._ptr(derivs))

@@ -3678,3 +3678,20 @@ int afgpu_chem_derivatives(afgpu_chem_t *c, int n_cells, const double *dens, dou
 int afgpu_chem_source(afgpu_chem_t *c, int n_cells, const double *fields, const double *dens, double *derivs) {
   return chem_launch(c, n_cells, fields, dens, nullptr, derivs, 2);
 }
+
+int afgpu_device_alloc(size_t bytes, void **ptr) {
+  if (!ptr) return AFGPU_ERR_ARG;
+  *ptr = nullptr;
+  HIPCHK(hipMalloc(ptr, std::max<size_t>(bytes, 1)));
+  return AFGPU_OK;
+}
+int afgpu_device_free(void *ptr) {
+  if (ptr) HIPCHK(hipFree(ptr));
+  return AFGPU_OK;
+}
+int afgpu_device_copy(void *dst, const void *src, size_t bytes, int to_device) {
+  if ((!dst || !src) && bytes > 0) return AFGPU_ERR_ARG;
+  if (bytes == 0) return AFGPU_OK;
+  HIPCHK(hipMemcpy(dst, src, bytes, to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost));
+  return AFGPU_OK;
+}

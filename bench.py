@@ -215,22 +215,18 @@ def measure_chemistry(n_cells, with_cpu, reps=10):
     """The source term of the reference's air network (8 species, 25
     reactions; tests/golden/air_chemistry_v1.txt) on n_cells cells resident
     in device memory (afgpu_chem_source), against the oracle on a sample."""
-    import torch
     import chemistry as CH
     path = os.path.join(ROOT, "tests", "golden", "air_chemistry_v1.txt")
     net = CH.read_network(path, {"N2": 0.8, "O2": 0.2}, S.gas_number_density(), 0.0, 1000.0, 1000)
     ns = len(net.species)
     g = CH.Chem("gpu", ns, net.reactions, 0.0, 1000.0, net.fld_cols, None, 300.0)
     rng = np.random.default_rng(3)
-    dev = torch.device("cuda:0")
-    # (host arrays uploaded whole: plain copies, no torch kernels)
     fields_h = rng.uniform(0.0, 300.0, n_cells)
     dens_h = 10.0 ** rng.uniform(12, 20, (ns, n_cells))
-    fields = torch.from_numpy(fields_h).to(dev)
-    dens = torch.from_numpy(dens_h).to(dev)
-    derivs = torch.empty((ns, n_cells), dtype=torch.float64, device=dev)
+    fields = A.DeviceArray(None, fields_h)
+    dens = A.DeviceArray(None, dens_h)
+    derivs = A.DeviceArray((ns, n_cells))
     g.source(fields, dens, derivs)
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(reps):
         g.source(fields, dens, derivs)
@@ -251,7 +247,8 @@ def measure_chemistry(n_cells, with_cpu, reps=10):
         cs = time.perf_counter() - t0
         out["cpu_oracle_cells_per_s"] = m / cs
         out["cpu_oracle_sample"] = "%d cells, one host core, %.2f s" % (m, cs)
-    del fields, dens, derivs, fields_h, dens_h
+    for a in (fields, dens, derivs):
+        a.free()
     g.close()
     log("  chemistry", json.dumps(out))
     return out

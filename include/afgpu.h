@@ -32,6 +32,8 @@
 #ifndef AFGPU_H
 #define AFGPU_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -497,6 +499,12 @@ int afgpu_chem_rates(afgpu_chem_t *c, int n_cells, const double *fields, double 
 int afgpu_chem_derivatives(afgpu_chem_t *c, int n_cells, const double *dens, double *rates, double *derivs);
 /* both in one pass, without the rates array: the same derivs */
 int afgpu_chem_source(afgpu_chem_t *c, int n_cells, const double *fields, const double *dens, double *derivs);
+/* Device memory for such arrays, from the library's own HIP runtime, for
+ * callers without one: allocation, release, and a synchronous copy
+ * (to_device 1: host to device; 0: device to host). */
+int afgpu_device_alloc(size_t bytes, void **ptr);
+int afgpu_device_free(void *ptr);
+int afgpu_device_copy(void *dst, const void *src, size_t bytes, int to_device);
 
 #ifdef __cplusplus
 }
