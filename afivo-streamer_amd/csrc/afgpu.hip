@@ -3574,7 +3574,8 @@ struct afgpu_chem {
 int afgpu_chem_create(int n_species, int n_reactions, const afgpu_reaction_t *reactions, int table_n,
                       double table_x_min, double table_x_max, int n_fld_cols, const double *fld_cols,
                       const double *energy_col, double gas_temperature, afgpu_chem_t **out) {
-  if (!out || n_species < 1 || n_species > AFGPU_CHEM_MAX_SPECIES || n_reactions < 0 || (n_reactions > 0 && !reactions) ||
+  if (!out || n_species < 1 || n_species > AFGPU_CHEM_MAX_SPECIES || n_reactions < 0 ||
+      n_reactions > AFGPU_CHEM_MAX_REACTIONS || (n_reactions > 0 && !reactions) ||
       table_n < 2 || !(table_x_max > table_x_min) || n_fld_cols < 0 || (n_fld_cols > 0 && !fld_cols) || !(gas_temperature > 0)) {
     set_err("afgpu_chem_create: invalid arguments");
     return AFGPU_ERR_ARG;
@@ -3652,18 +3653,24 @@ static int chem_launch(afgpu_chem_t *c, int n_cells, const double *fields, const
     set_err("afgpu_chem: missing array");
     return AFGPU_ERR_ARG;
   }
-  // LDS per workgroup within 48 KiB: 256 cells with their densities staged
-  // for small networks, else 64 cells (densities staged while they fit)
-  const size_t ns = (size_t)c->n_species, cap = 48 * 1024;
-  if (2 * ns * 256 * sizeof(double) <= cap)
-    hipLaunchKernelGGL((k_chem<256, true>), dim3((n_cells + 255) / 256), dim3(256), 2 * ns * 256 * sizeof(double),
-                       c->stream, c->args, n_cells, fields, dens, rates, derivs, mode);
-  else if (2 * ns * 64 * sizeof(double) <= cap)
-    hipLaunchKernelGGL((k_chem<64, true>), dim3((n_cells + 63) / 64), dim3(64), 2 * ns * 64 * sizeof(double),
-                       c->stream, c->args, n_cells, fields, dens, rates, derivs, mode);
+  // LDS per workgroup within 64 KiB: the derivatives of 256 cells with
+  // their densities staged for small networks, else of 64 cells (densities
+  // staged while they fit), and the reaction records while they fit
+  const size_t ns = (size_t)c->n_species, cap = 64 * 1024, d = sizeof(double);
+  const size_t rb = (size_t)c->n_reactions * sizeof(afgpu_reaction_t);
+  const dim3 g256((n_cells + 255) / 256), g64((n_cells + 63) / 64);
+  if (rb + 2 * ns * 256 * d <= cap)
+    hipLaunchKernelGGL((k_chem<256, true, true>), g256, dim3(256), rb + 2 * ns * 256 * d, c->stream, c->args,
+                       n_cells, fields, dens, rates, derivs, mode);
+  else if (rb + 2 * ns * 64 * d <= cap)
+    hipLaunchKernelGGL((k_chem<64, true, true>), g64, dim3(64), rb + 2 * ns * 64 * d, c->stream, c->args, n_cells,
+                       fields, dens, rates, derivs, mode);
+  else if (rb + ns * 64 * d <= cap)
+    hipLaunchKernelGGL((k_chem<64, false, true>), g64, dim3(64), rb + ns * 64 * d, c->stream, c->args, n_cells,
+                       fields, dens, rates, derivs, mode);
   else
-    hipLaunchKernelGGL((k_chem<64, false>), dim3((n_cells + 63) / 64), dim3(64), ns * 64 * sizeof(double),
-                       c->stream, c->args, n_cells, fields, dens, rates, derivs, mode);
+    hipLaunchKernelGGL((k_chem<64, false, false>), g64, dim3(64), ns * 64 * d, c->stream, c->args, n_cells, fields,
+                       dens, rates, derivs, mode);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
   return AFGPU_OK;

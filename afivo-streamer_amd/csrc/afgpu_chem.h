@@ -84,14 +84,24 @@ __device__ inline double chem_rate(const afgpu_reaction_t &R, const ChemArgs &a,
 // become the reaction rates); 2: both, without the rates array.  Dynamic
 // LDS: the derivatives of the workgroup's BS cells ([n_species][BS]) and,
 // with STAGE, their densities (read from HBM once per cell and species).
-template <int BS, bool STAGE>
+// RLDS: the reaction records are copied to LDS first (after the species
+// blocks), else read from memory.
+template <int BS, bool STAGE, bool RLDS>
 __global__ void __launch_bounds__(BS) k_chem(ChemArgs a, int n_cells, const double *__restrict__ fields,
                                              const double *__restrict__ dens, double *rates, double *derivs,
                                              int mode) {
   extern __shared__ double lds[];
   double *sd = lds;
   double *sn = lds + (size_t)a.n_species * BS;
+  afgpu_reaction_t *sr = (afgpu_reaction_t *)(lds + (size_t)(STAGE ? 2 : 1) * a.n_species * BS);
   const int t = threadIdx.x;
+  if (RLDS) {
+    constexpr int W = sizeof(afgpu_reaction_t) / sizeof(int);
+    const int *src = (const int *)a.r;
+    int *dst = (int *)sr;
+    for (int q = t; q < a.n_reactions * W; q += BS) dst[q] = src[q];
+    __syncthreads();
+  }
   const int i = blockIdx.x * BS + t;
   const bool in = i < n_cells;
   const size_t n = (size_t)n_cells;
@@ -114,7 +124,7 @@ __global__ void __launch_bounds__(BS) k_chem(ChemArgs a, int n_cells, const doub
   }
   if (in)
     for (int r = 0; r < a.n_reactions; r++) {
-      const afgpu_reaction_t &R = a.r[r];
+      const afgpu_reaction_t &R = RLDS ? sr[r] : a.r[r];
       double k = (mode == 1) ? rates[r * n + i] : chem_rate(R, a, fld, te, ix, w);
       if (mode == 0) {
         rates[r * n + i] = k;

@@ -447,6 +447,7 @@ int afgpu_set_shapes(afgpu_t *h, int n_classes, const afgpu_shape_t *shapes);
  * into rate_factor, the input species repeated by multiplicity, the output
  * species with their multiplicities; indices are 0-based plasma species. */
 #define AFGPU_CHEM_MAX_SPECIES 100 /* max_num_species, m_chemistry.f90:129 */
+#define AFGPU_CHEM_MAX_REACTIONS 500 /* max_num_reactions, m_chemistry.f90:132 */
 #define AFGPU_CHEM_MAX_IN      8
 #define AFGPU_CHEM_MAX_OUT     8
 #define AFGPU_CHEM_MAX_COEFF   9   /* rate_max_num_coeff */

@@ -2556,8 +2556,9 @@ struct afcpu_chem {
 int afcpu_chem_create(int n_species, int n_reactions, const afgpu_reaction_t *reactions, int table_n,
                       double table_x_min, double table_x_max, int n_fld_cols, const double *fld_cols,
                       const double *energy_col, double gas_temperature, afcpu_chem **out) {
-  if (!out || n_species < 1 || n_species > AFGPU_CHEM_MAX_SPECIES || n_reactions < 0 || (n_reactions > 0 && !reactions) ||
-      table_n < 2 || !(table_x_max > table_x_min) || n_fld_cols < 0 || (n_fld_cols > 0 && !fld_cols))
+  if (!out || n_species < 1 || n_species > AFGPU_CHEM_MAX_SPECIES || n_reactions < 0 ||
+      n_reactions > AFGPU_CHEM_MAX_REACTIONS || (n_reactions > 0 && !reactions) || table_n < 2 ||
+      !(table_x_max > table_x_min) || n_fld_cols < 0 || (n_fld_cols > 0 && !fld_cols))
     return AFGPU_ERR_ARG;
   afcpu_chem *c = new afcpu_chem;
   c->n_species = n_species;

@@ -123,14 +123,14 @@ def test_chemistry_equals_oracle(A, CH, tmp_path, which, n):
     assert np.array_equal(ts.numpy(), dg), which
 
 
-@pytest.mark.parametrize("ns", [5, 20, 60, 100])
-def test_chemistry_many_species(A, CH, ns):
-    """Random networks of constant rates over ns species (the workgroup
-    shapes for small and large networks): bit for bit equal to the oracle
-    (no exp or pow)."""
-    rng = np.random.default_rng(ns)
+@pytest.mark.parametrize("ns, nr", [(5, 15), (20, 60), (60, 60), (60, 180), (100, 300)])
+def test_chemistry_many_species(A, CH, ns, nr):
+    """Random networks of nr constant-rate reactions over ns species (every
+    workgroup shape: 256 or 64 cells, densities and reaction records in LDS
+    or not): bit for bit equal to the oracle (no exp or pow)."""
+    rng = np.random.default_rng(ns + nr)
     reactions = []
-    for _ in range(3 * ns):
+    for _ in range(nr):
         r = A.ReactionT()
         r.rate_type = A.RATE_CONSTANT
         r.rate_factor = 10.0 ** rng.uniform(-20, -14)
