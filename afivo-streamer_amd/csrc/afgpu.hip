@@ -116,7 +116,7 @@ struct ImplBase {
   // boxes removed or added by refinement: restriction from the listed
   // children, prolongation into the listed new boxes, their ghost cells
   virtual int restrict_ids(const int *child_ids, int n, int iv) = 0;
-  virtual int prolong_new(const int *ids, int n, int iv, bool limited) = 0;
+  virtual int prolong_new(const int *ids, int n, const int *ivs, const bool *limited, int n_vars) = 0;
   virtual int gc_ids(const int *ids, int n, int iv) = 0;
   virtual int restrict_tree(int iv) = 0;
   virtual int step_runs(afgpu_step_info_t *info) = 0;
@@ -1758,15 +1758,21 @@ struct Impl : ImplBase {
     HIPCHK(hipGetLastError());
     return AFGPU_OK;
   }
-  int prolong_new(const int *ids, int n, int iv, bool limited) override {
-    if (n <= 0) return AFGPU_OK;
-    const dim3 grid(std::min(n, 8 * c.n_cu));
-    if (limited)
-      hipLaunchKernelGGL((k_prolong_new<D, NC, BS, true>), grid, dim3(BS), 0, c.cur, ids, (const DBox *)c.d_boxes,
-                         c.cc[iv], var_methods(c, iv).limiter, n);
-    else
-      hipLaunchKernelGGL((k_prolong_new<D, NC, BS, false>), grid, dim3(BS), 0, c.cur, ids, (const DBox *)c.d_boxes,
-                         c.cc[iv], 0, n);
+  int prolong_new(const int *ids, int n, const int *ivs, const bool *limited, int n_vars) override {
+    if (n <= 0 || n_vars <= 0) return AFGPU_OK;
+    if (n_vars > NEW_MAX_VARS) {
+      set_err("prolongation of new boxes: too many variables");
+      return AFGPU_ERR_ARG;
+    }
+    NewVars nv;
+    nv.n = n_vars;
+    for (int k = 0; k < n_vars; k++) {
+      nv.v[k] = c.cc[ivs[k]];
+      nv.limiter[k] = limited[k] ? var_methods(c, ivs[k]).limiter : -1;
+    }
+    const int nw = n * n_vars;
+    hipLaunchKernelGGL((k_prolong_new<D, NC, BS>), dim3(std::min(nw, 8 * c.n_cu)), dim3(BS), 0, c.cur, ids,
+                       (const DBox *)c.d_boxes, nv, nw);
     HIPCHK(hipGetLastError());
     return AFGPU_OK;
   }
@@ -2981,7 +2987,15 @@ int afgpu_prolong_boxes(afgpu_t *h, int n_ids, const int *ids) {
     while (b < sorted.size() && h->hboxes[sorted[b] - 1].lvl == h->hboxes[sorted[a] - 1].lvl) b++;
     const int *d = h->d_ref_ids + a;
     const int n = (int)(b - a);
-    for (const auto &v : vars) RCHK(h->impl->prolong_new(d, n, v.first, v.second));
+    {
+      int ivs[NEW_MAX_VARS];
+      bool lim[NEW_MAX_VARS];
+      for (size_t k = 0; k < vars.size(); k++) {
+        ivs[k] = vars[k].first;
+        lim[k] = vars[k].second;
+      }
+      RCHK(h->impl->prolong_new(d, n, ivs, lim, (int)vars.size()));
+    }
     for (const auto &v : vars) RCHK(h->impl->gc_ids(d, n, v.first));
     a = b;
   }

@@ -167,20 +167,32 @@ __global__ void __launch_bounds__(BS) k_refine_cells(const int *__restrict__ ids
 }
 
 // A new box's values from its parent (auto_prolong, afivo/src/m_af_core.f90:
-// 843-881): af_prolong_limit (LIMITED: the slopes limited with the
+// 843-881): af_prolong_limit (limiter >= 0: the slopes limited with the
 // variable's limiter, m_af_prolong.f90:311-420) or af_prolong_linear
 // (m_af_prolong.f90:531-679), not adding: every interior cell is written
 // once, as the reference's zero plus its sum.  One workgroup per new box.
-template <int D, int NC, int BS, bool LIMITED>
+// The variables of one call: pool pointer and limiter (-1: linear).
+constexpr int NEW_MAX_VARS = 8;
+struct NewVars {
+  double *v[NEW_MAX_VARS];
+  int limiter[NEW_MAX_VARS];
+  int n;
+};
+
+// Work item w is box w / vars.n, variable w % vars.n.
+template <int D, int NC, int BS>
 __global__ void __launch_bounds__(BS) k_prolong_new(const int *__restrict__ ids, const DBox *__restrict__ boxes,
-                                                    double *var, int limiter, int nw) {
+                                                    NewVars vars, int nw) {
   using G = Geo<D, NC>;
   constexpr int H = NC / 2;
   constexpr int NQ = (D == 3) ? H * H * H : H * H;
   constexpr int st[3] = {1, G::NG, G::NG * G::NG};
   for (int w = blockIdx.x; w < nw; w += gridDim.x) {
-    const int id = ids[w];
+    const int id = ids[w / vars.n];
     if (id <= 0) continue;
+    double *var = vars.v[w % vars.n];
+    const int limiter = vars.limiter[w % vars.n];
+    const bool LIMITED = limiter >= 0;
     const DBox &bc = boxes[id - 1];
     const double *P = var + (size_t)(bc.parent - 1) * G::NCELL;
     double *u = var + (size_t)(id - 1) * G::NCELL;
