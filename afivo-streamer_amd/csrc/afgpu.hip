@@ -117,7 +117,7 @@ struct ImplBase {
   // children, prolongation into the listed new boxes, their ghost cells
   virtual int restrict_ids(const int *child_ids, int n, int iv) = 0;
   virtual int prolong_new(const int *ids, int n, const int *ivs, const bool *limited, int n_vars) = 0;
-  virtual int gc_ids(const int *ids, int n, int iv) = 0;
+  virtual int gc_ids(const int *ids, int n, const int *ivs, int n_vars) = 0;
   virtual int restrict_tree(int iv) = 0;
   virtual int step_runs(afgpu_step_info_t *info) = 0;
   // per-leaf-box terms of the regression-log statistics (k_diag_box)
@@ -1784,10 +1784,21 @@ struct Impl : ImplBase {
     HIPCHK(hipGetLastError());
     return AFGPU_OK;
   }
-  int gc_ids(const int *ids, int n, int iv) override {
-    if (n <= 0) return AFGPU_OK;
-    hipLaunchKernelGGL((k_gc_lds<D, NC, BS, double, 1>), dim3(std::min(n, 8 * c.n_cu)), dim3(BS), 0, c.cur, ids,
-                       (const DBox *)c.d_boxes, c.cc[iv], var_methods(c, iv), 1, n);
+  int gc_ids(const int *ids, int n, const int *ivs, int n_vars) override {
+    if (n <= 0 || n_vars <= 0) return AFGPU_OK;
+    if (n_vars > GC_MAX_VARS) {
+      set_err("ghost cells of new boxes: too many variables");
+      return AFGPU_ERR_ARG;
+    }
+    GcVars gv;
+    gv.n = n_vars;
+    for (int k = 0; k < n_vars; k++) {
+      gv.v[k] = c.cc[ivs[k]];
+      gv.m[k] = var_methods(c, ivs[k]);
+    }
+    const int nw = n * n_vars;
+    hipLaunchKernelGGL((k_gc_lds_vars<D, NC, BS>), dim3(std::min(nw, 8 * c.n_cu)), dim3(BS), 0, c.cur, ids,
+                       (const DBox *)c.d_boxes, gv, 1, nw);
     HIPCHK(hipGetLastError());
     return AFGPU_OK;
   }
@@ -2995,8 +3006,8 @@ int afgpu_prolong_boxes(afgpu_t *h, int n_ids, const int *ids) {
         lim[k] = vars[k].second;
       }
       RCHK(h->impl->prolong_new(d, n, ivs, lim, (int)vars.size()));
+      RCHK(h->impl->gc_ids(d, n, ivs, (int)vars.size()));
     }
-    for (const auto &v : vars) RCHK(h->impl->gc_ids(d, n, v.first));
     a = b;
   }
   return sync_stream(h);

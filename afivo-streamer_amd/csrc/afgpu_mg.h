@@ -1406,77 +1406,101 @@ __global__ void __launch_bounds__(BS * BB) k_mp_correct_leaves(const int *__rest
 // then (with corners) the edge and corner ghosts of gc_edge_cell /
 // gc_corner_cell from the LDS values, then only the ghost layer is written.
 // One workgroup per box.
-template <int D, int NC, int BS, typename T, int BB = 1>
-__global__ void __launch_bounds__(BS * BB) k_gc_lds(const int *__restrict__ ids, const DBox *__restrict__ boxes, T *var,
-                                                    VarBc m, int corners, int nw) {
+template <int D, int NC, int BS, typename T, int BB>
+__device__ inline void gc_lds_box(T *s, const DBox *__restrict__ boxes, int id, T *var, const VarBc &m,
+                                  int corners) {
   using G = Geo<D, NC>;
   using L = Lay<D, NC, T>;
-  __shared__ T s_all[BB][G::NCELL];
-  T *s = s_all[lgrp<BS, BB>()];
-  for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
-    const int id = ids[w];
-    if (id <= 0) continue;  // run masked out
-    const DBox &bx = boxes[id - 1];
-    T *u = var + (size_t)(id - 1) * L::STRIDE;
-    lds_stage<D, NC, BS, T, BB>(s, boxes, bx, id, var, m, -1);
-    if (corners) {
-      if (D == 3) {
-        for (int q = ltid<BS>(); q < 12 * NC; q += BS) {
-          const int n = q / NC, qq = q % NC + 1;
-          const int dim = c_edge_dim3[n];
-          const int *dir = c_edge_dir3[n];
-          int lo[3];
-          for (int dd = 0; dd < 3; dd++) lo[dd] = (dir[dd] > 0) ? NC + 1 : 0;
-          lo[dim] = qq;
-          const int nb_id = bx.nbmat[nbmat_at<D>(dir[0], dir[1], dir[2])];
-          T v;
-          if (nb_id > 0) {
-            const T *w = var + (size_t)(nb_id - 1) * L::STRIDE;
-            v = w[L::in_at(lo[0] - dir[0] * NC, lo[1] - dir[1] * NC, lo[2] - dir[2] * NC)];
-          } else {
-            const int o0 = (dim + 1) % 3, o1 = (dim + 2) % 3;
-            int di[3];
-            for (int dd = 0; dd < 3; dd++) di[dd] = 1 - 2 * (lo[dd] & 1);
-            di[dim] = 0;
-            int ia[3] = {lo[0], lo[1], lo[2]}, ib[3] = {lo[0], lo[1], lo[2]};
-            ia[o0] += di[o0];
-            ib[o1] += di[o1];
-            v = s[G::idx(ia[0], ia[1], ia[2])] + s[G::idx(ib[0], ib[1], ib[2])] -
-                s[G::idx(lo[0] + di[0], lo[1] + di[1], lo[2] + di[2])];
-          }
-          s[G::idx(lo[0], lo[1], lo[2])] = v;
-        }
-        gsync<BS, BB>();
-      }
-      for (int n = ltid<BS>(); n < (1 << D); n += BS) {
-        const int cx = n & 1, cy = (n >> 1) & 1, cz = (n >> 2) & 1;
-        const int dnb[3] = {2 * cx - 1, 2 * cy - 1, (D == 3) ? 2 * cz - 1 : 0};
-        const int lo[3] = {cx * (NC + 1), cy * (NC + 1), (D == 3) ? cz * (NC + 1) : 0};
-        const int nb_id = bx.nbmat[nbmat_at<D>(dnb[0], dnb[1], dnb[2])];
+  const DBox &bx = boxes[id - 1];
+  T *u = var + (size_t)(id - 1) * L::STRIDE;
+  lds_stage<D, NC, BS, T, BB>(s, boxes, bx, id, var, m, -1);
+  if (corners) {
+    if (D == 3) {
+      for (int q = ltid<BS>(); q < 12 * NC; q += BS) {
+        const int n = q / NC, qq = q % NC + 1;
+        const int dim = c_edge_dim3[n];
+        const int *dir = c_edge_dir3[n];
+        int lo[3];
+        for (int dd = 0; dd < 3; dd++) lo[dd] = (dir[dd] > 0) ? NC + 1 : 0;
+        lo[dim] = qq;
+        const int nb_id = bx.nbmat[nbmat_at<D>(dir[0], dir[1], dir[2])];
         T v;
         if (nb_id > 0) {
           const T *w = var + (size_t)(nb_id - 1) * L::STRIDE;
-          v = w[L::in_at(lo[0] - dnb[0] * NC, lo[1] - dnb[1] * NC, lo[2] - dnb[2] * NC)];
-        } else if (D == 2) {
-          const int di = 1 - 2 * (lo[0] & 1), dj = 1 - 2 * (lo[1] & 1);
-          v = s[G::idx(lo[0] + di, lo[1], 0)] + s[G::idx(lo[0], lo[1] + dj, 0)] - s[G::idx(lo[0] + di, lo[1] + dj, 0)];
+          v = w[L::in_at(lo[0] - dir[0] * NC, lo[1] - dir[1] * NC, lo[2] - dir[2] * NC)];
         } else {
-          const int di = 1 - 2 * (lo[0] & 1), dj = 1 - 2 * (lo[1] & 1), dk = 1 - 2 * (lo[2] & 1);
-          const int x = lo[0], y = lo[1], z = lo[2];
-          v = s[G::idx(x, y + dj, z + dk)] + s[G::idx(x + di, y, z + dk)] + s[G::idx(x + di, y + dj, z)] -
-              2 * s[G::idx(x + di, y + dj, z + dk)];
+          const int o0 = (dim + 1) % 3, o1 = (dim + 2) % 3;
+          int di[3];
+          for (int dd = 0; dd < 3; dd++) di[dd] = 1 - 2 * (lo[dd] & 1);
+          di[dim] = 0;
+          int ia[3] = {lo[0], lo[1], lo[2]}, ib[3] = {lo[0], lo[1], lo[2]};
+          ia[o0] += di[o0];
+          ib[o1] += di[o1];
+          v = s[G::idx(ia[0], ia[1], ia[2])] + s[G::idx(ib[0], ib[1], ib[2])] -
+              s[G::idx(lo[0] + di[0], lo[1] + di[1], lo[2] + di[2])];
         }
         s[G::idx(lo[0], lo[1], lo[2])] = v;
       }
       gsync<BS, BB>();
     }
-    // the ghost layer back: faces, and with corners the edges and corners
-    for (int x = ltid<BS>(); x < G::NCELL; x += BS) {
-      const int i = x % G::NG, j = (x / G::NG) % G::NG, k = (D == 3) ? x / (G::NG * G::NG) : 0;
-      const int out = (i == 0 || i == NC + 1) + (j == 0 || j == NC + 1) + (D == 3 && (k == 0 || k == NC + 1));
-      if (out == 1 || (corners && out > 1)) u[L::at(i, j, k)] = s[x];
+    for (int n = ltid<BS>(); n < (1 << D); n += BS) {
+      const int cx = n & 1, cy = (n >> 1) & 1, cz = (n >> 2) & 1;
+      const int dnb[3] = {2 * cx - 1, 2 * cy - 1, (D == 3) ? 2 * cz - 1 : 0};
+      const int lo[3] = {cx * (NC + 1), cy * (NC + 1), (D == 3) ? cz * (NC + 1) : 0};
+      const int nb_id = bx.nbmat[nbmat_at<D>(dnb[0], dnb[1], dnb[2])];
+      T v;
+      if (nb_id > 0) {
+        const T *w = var + (size_t)(nb_id - 1) * L::STRIDE;
+        v = w[L::in_at(lo[0] - dnb[0] * NC, lo[1] - dnb[1] * NC, lo[2] - dnb[2] * NC)];
+      } else if (D == 2) {
+        const int di = 1 - 2 * (lo[0] & 1), dj = 1 - 2 * (lo[1] & 1);
+        v = s[G::idx(lo[0] + di, lo[1], 0)] + s[G::idx(lo[0], lo[1] + dj, 0)] - s[G::idx(lo[0] + di, lo[1] + dj, 0)];
+      } else {
+        const int di = 1 - 2 * (lo[0] & 1), dj = 1 - 2 * (lo[1] & 1), dk = 1 - 2 * (lo[2] & 1);
+        const int x = lo[0], y = lo[1], z = lo[2];
+        v = s[G::idx(x, y + dj, z + dk)] + s[G::idx(x + di, y, z + dk)] + s[G::idx(x + di, y + dj, z)] -
+            2 * s[G::idx(x + di, y + dj, z + dk)];
+      }
+      s[G::idx(lo[0], lo[1], lo[2])] = v;
     }
-    gsync<BS, BB>();  // (the group's LDS is reused by its next item)
+    gsync<BS, BB>();
+  }
+  // the ghost layer back: faces, and with corners the edges and corners
+  for (int x = ltid<BS>(); x < G::NCELL; x += BS) {
+    const int i = x % G::NG, j = (x / G::NG) % G::NG, k = (D == 3) ? x / (G::NG * G::NG) : 0;
+    const int out = (i == 0 || i == NC + 1) + (j == 0 || j == NC + 1) + (D == 3 && (k == 0 || k == NC + 1));
+    if (out == 1 || (corners && out > 1)) u[L::at(i, j, k)] = s[x];
+  }
+  gsync<BS, BB>();  // (the group's LDS is reused by its next item)
+}
+template <int D, int NC, int BS, typename T, int BB = 1>
+__global__ void __launch_bounds__(BS * BB) k_gc_lds(const int *__restrict__ ids, const DBox *__restrict__ boxes, T *var,
+                                                    VarBc m, int corners, int nw) {
+  __shared__ T s_all[BB][Geo<D, NC>::NCELL];
+  T *s = s_all[lgrp<BS, BB>()];
+  for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
+    const int id = ids[w];
+    if (id <= 0) continue;  // run masked out
+    gc_lds_box<D, NC, BS, T, BB>(s, boxes, id, var, m, corners);
+  }
+}
+
+// The same for several FP64 variables in one launch: work item w is box
+// w / vars.n, variable w % vars.n.
+constexpr int GC_MAX_VARS = 8;
+struct GcVars {
+  double *v[GC_MAX_VARS];
+  VarBc m[GC_MAX_VARS];
+  int n;
+};
+template <int D, int NC, int BS>
+__global__ void __launch_bounds__(BS) k_gc_lds_vars(const int *__restrict__ ids, const DBox *__restrict__ boxes,
+                                                    GcVars vars, int corners, int nw) {
+  __shared__ double s[Geo<D, NC>::NCELL];
+  for (int w = blockIdx.x; w < nw; w += gridDim.x) {
+    const int id = ids[w / vars.n];
+    if (id <= 0) continue;
+    gc_lds_box<D, NC, BS, double, 1>(s, boxes, id, vars.v[w % vars.n], vars.m[w % vars.n], corners);
   }
 }
 
