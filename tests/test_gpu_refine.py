@@ -153,3 +153,51 @@ def test_restricted_and_prolonged_boxes_equal_oracle(A, S, name):
     same("prolong")
     assert g.step() == o.step()
     same("step on the new tree")
+
+
+@pytest.mark.parametrize("name", ["cyl2d", "xyz3d_8"])
+def test_refinement_cycles_equal_oracle(A, S, name):
+    """The reference's loop with refinement (src/streamer.f90:397-427) for
+    three cycles: two steps; restrict and fill the electron ghosts; the
+    flags (afgpu_refine_flags); the boxes flagged for refinement refined
+    (the tree change stays with the caller; derefinement is not applied
+    here); restriction and prolongation of the changed boxes on each side;
+    the field on the new mesh and the photoionization source.  Every
+    cell-centred variable stays equal to the oracle's bit for bit."""
+    case = cases(S)[name]
+    case.params.mg_num_vcycles = 20
+    case.params.mg_max_rel_residual = 1e-10
+    case.params.photoi_enabled = 1
+    case.params.photoi_per_steps = 1
+    g = S.apply(A.Path("gpu", case.D, case.nc, case.coord, A.PREC_FP64), case)
+    o = S.apply(A.Path("oracle", case.D, case.nc, case.coord), case)
+    for p in (g, o):
+        p.field_compute(0, have_guess=False)
+    t = case.extra["export"]
+    s = A.refine_settings(refine_adx=0.01, derefine_dx=1.0, global_time=1.0, max_lvl=int(t["lvl"].max()) + 1)
+    added = 0
+    for cycle in range(3):
+        for _ in range(2):
+            assert g.step() == o.step(), (name, cycle)
+        for p in (g, o):
+            p.restrict_tree(A.CC_ELECTRON)
+            p.gc_tree(A.CC_ELECTRON)
+        n = t["n_boxes"]
+        fg, fo = g.refine_flags(s, n), o.refine_flags(s, n)
+        assert np.array_equal(fg, fo)
+        ref = np.nonzero(fo == REFINE)[0] + 1
+        print(name, "cycle", cycle, "boxes", n, "refine", len(ref), flush=True)
+        if len(ref) == 0:
+            continue
+        case.tree.refine(ref.astype(np.int32))
+        t = case.tree.export()
+        new = np.arange(n + 1, t["n_boxes"] + 1, dtype=np.int32)
+        added += len(new)
+        for p in (g, o):
+            p.sync_tree(t)
+            p.prolong_boxes(new)
+            p.field_compute(0, have_guess=True)
+            p.photoi_set_src()
+        for iv in range(A.N_CC):
+            assert np.array_equal(g.get_cc(iv), o.get_cc(iv)), (name, cycle, iv)
+    assert added > 0
