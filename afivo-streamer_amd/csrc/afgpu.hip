@@ -3594,6 +3594,13 @@ struct afgpu_chem {
   double *d_fld = nullptr, *d_energy = nullptr;
   ChemArgs args{};
   hipStream_t stream = nullptr;
+  ~afgpu_chem() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    (void)hipFree(d_r);
+    (void)hipFree(d_fld);
+    (void)hipFree(d_energy);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
 };
 
 int afgpu_chem_create(int n_species, int n_reactions, const afgpu_reaction_t *reactions, int table_n,
@@ -3660,13 +3667,7 @@ int afgpu_chem_create(int n_species, int n_reactions, const afgpu_reaction_t *re
 }
 
 int afgpu_chem_destroy(afgpu_chem_t *c) {
-  if (!c) return AFGPU_OK;
-  if (c->stream) (void)hipStreamSynchronize(c->stream);
-  (void)hipFree(c->d_r);
-  (void)hipFree(c->d_fld);
-  (void)hipFree(c->d_energy);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
-  delete c;
+  delete c;  // (the destructor releases the device arrays and the stream)
   return AFGPU_OK;
 }
 
