@@ -211,6 +211,7 @@ _COMMON = {
     "get_time_state": (_I, [_P, _IP, _DP, _DP]),
     "set_time_state": (_I, [_P, _I, _D, _D]),
     "regression_values": (_I, [_P, _I, _IP, _DP]),
+    "set_chemistry": (_I, [_P, _P]),
     "get_boxes": (_I, [_P, _I, _IP, _I, _IP, _DP]),
     "put_boxes": (_I, [_P, _I, _IP, _I, _IP, _DP]),
 }
@@ -669,6 +670,13 @@ class Path:
         ivs = np.ascontiguousarray(ivs, np.int32)
         arr = (C.c_char_p * len(names))(*[n.encode() for n in names])
         self._call("write_regression_log", os.fsencode(filename), out_cnt, len(ivs), _ip(ivs), arr)
+
+    def set_chemistry(self, chem):
+        """The step's source terms from a reaction list over the three
+        species (a chemistry.Chem of the same kind), or the standard model
+        (None).  Keep chem alive while it is set."""
+        self._chem = chem
+        self._call("set_chemistry", chem.h if chem is not None else None)
 
     def get_boxes(self, ivs, ids=None):
         """Whole boxes of the variables ivs in af_write_tree's order:

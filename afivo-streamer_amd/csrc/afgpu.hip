@@ -98,6 +98,22 @@ struct GraphRec {
 
 }  // namespace
 
+// A reaction list on the device (afgpu_chem_create).
+struct afgpu_chem {
+  int n_species = 0, n_reactions = 0;
+  afgpu_reaction_t *d_r = nullptr;
+  double *d_fld = nullptr, *d_energy = nullptr;
+  ChemArgs args{};
+  hipStream_t stream = nullptr;
+  ~afgpu_chem() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    (void)hipFree(d_r);
+    (void)hipFree(d_fld);
+    (void)hipFree(d_energy);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
 struct afgpu_ctx;
 
 struct ImplBase {
@@ -212,6 +228,8 @@ struct afgpu_ctx {
   bool fuse_direct = false;  // uniform-level solve: last-axis transforms fused into one pass
   bool direct_f32 = true;    // uniform-level solve of the FP32 inner cycle in FP32 (matrix cores)
   // refinement flags: the boxes evaluated and their flags (device)
+  // a reaction list over the three species in place of the standard model
+  const afgpu_chem *net = nullptr;
   int *d_ref_ids = nullptr, *d_ref_flags = nullptr;
   int ref_cap = 0;
   // diagnostics: per-box terms and per-run results; box I/O staging
@@ -1580,6 +1598,8 @@ struct Impl : ImplBase {
     A.photoi_enabled = c.p.photoi_enabled;
     A.photoi_species = c.p.photoi_species;
     A.cyl = cyl;
+    A.use_net = c.net ? 1 : 0;
+    if (c.net) A.net = c.net->args;
     RCHK(with_bb(KC_UPDATE, [&](auto B) {
       constexpr int bb = decltype(B)::value;
       return items_timed<bb>(2, (7.0 + D + (c.p.photoi_enabled ? 1 : 0)) * 8.0 * c.l_all_leaves.n * G::NINT, KC_UPDATE,
@@ -3588,20 +3608,6 @@ int afgpu_set_option(afgpu_t *h, int option, int value) {
 }  // extern "C"
 
 // ---- general chemistry networks (src/m_chemistry.f90:672-804) --------------
-struct afgpu_chem {
-  int n_species = 0, n_reactions = 0;
-  afgpu_reaction_t *d_r = nullptr;
-  double *d_fld = nullptr, *d_energy = nullptr;
-  ChemArgs args{};
-  hipStream_t stream = nullptr;
-  ~afgpu_chem() {
-    if (stream) (void)hipStreamSynchronize(stream);
-    (void)hipFree(d_r);
-    (void)hipFree(d_fld);
-    (void)hipFree(d_energy);
-    if (stream) (void)hipStreamDestroy(stream);
-  }
-};
 
 int afgpu_chem_create(int n_species, int n_reactions, const afgpu_reaction_t *reactions, int table_n,
                       double table_x_min, double table_x_max, int n_fld_cols, const double *fld_cols,
@@ -3726,5 +3732,21 @@ int afgpu_device_copy(void *dst, const void *src, size_t bytes, int to_device) {
   if ((!dst || !src) && bytes > 0) return AFGPU_ERR_ARG;
   if (bytes == 0) return AFGPU_OK;
   HIPCHK(hipMemcpy(dst, src, bytes, to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost));
+  return AFGPU_OK;
+}
+
+// The time step's source terms from a reaction list over the pools' three
+// species (0: electrons, 1: the positive ion, 2: the negative ion), as the
+// reference's add_source_terms computes them for any list
+// (src/m_fluid.f90:298-466); NULL returns to the standard model.  The
+// chemistry handle must outlive its use by the step.
+int afgpu_set_chemistry(afgpu_t *h, const afgpu_chem_t *chem) {
+  if (!h) return AFGPU_ERR_ARG;
+  if (chem && chem->n_species != 3) {
+    set_err("afgpu_set_chemistry: the pools hold three species (electrons, positive and negative ions)");
+    return AFGPU_ERR_UNSUPPORTED;
+  }
+  if (chem != h->net) drop_graphs(*h);  // (the list is an argument of captured updates)
+  h->net = chem;
   return AFGPU_OK;
 }

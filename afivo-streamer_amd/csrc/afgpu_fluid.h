@@ -7,6 +7,7 @@
 
 #include <limits>
 
+#include "afgpu_chem.h"
 #include "afgpu_mg.h"
 
 namespace afg {
@@ -402,11 +403,53 @@ struct UpdateArgs {
   int chem_limit_loss;
   int photoi_enabled, photoi_species;
   int cyl;
+  // a reaction list over the three species (afgpu_set_chemistry), else the
+  // standard model from the chem table
+  int use_net;
+  ChemArgs net;
 };
 
-// flux_update_densities (afivo/src/m_af_flux_schemes.f90:320-436) with the
-// standard-model add_source_terms (src/m_fluid.f90:298-466,
-// src/m_chemistry.f90:672-804) over all leaves.
+// get_rates and get_derivatives of a reaction list at one cell, for the
+// three species of the pools (m_chemistry.f90:672-804)
+__device__ inline void net_derivs(const ChemArgs &a, double field, const double *dens, double *dd) {
+  int ix;
+  double w, te = 0;
+  lt_loc(a.fld, field, ix, w);
+  if (a.need_te) {
+    int ie;
+    double we;
+    lt_loc(a.energy, field, ie, we);
+    te = CHEM_EV_TO_K * lt_at(a.energy, 0, ie, we);
+  }
+  for (int r = 0; r < a.n_reactions; r++) {
+    const afgpu_reaction_t &R = a.r[r];
+    double k = chem_rate(R, a, field, te, ix, w);
+    double prod = 1.0;
+    for (int q = 0; q < R.n_in; q++) {
+      const int s = R.ix_in[q];
+      prod = prod * (s == 0 ? dens[0] : s == 1 ? dens[1] : dens[2]);
+    }
+    k = k * prod;
+    for (int q = 0; q < R.n_in; q++) {
+      const int s = R.ix_in[q];
+      if (s == 0) dd[0] = dd[0] - k;
+      else if (s == 1) dd[1] = dd[1] - k;
+      else dd[2] = dd[2] - k;
+    }
+    for (int q = 0; q < R.n_out; q++) {
+      const int s = R.ix_out[q];
+      const double v = k * R.multiplicity_out[q];
+      if (s == 0) dd[0] = dd[0] + v;
+      else if (s == 1) dd[1] = dd[1] + v;
+      else dd[2] = dd[2] + v;
+    }
+  }
+}
+
+// flux_update_densities (afivo/src/m_af_flux_schemes.f90:320-436) with
+// add_source_terms (src/m_fluid.f90:298-466, src/m_chemistry.f90:672-804)
+// over all leaves: the standard model, or a reaction list over the three
+// species (A.use_net).
 template <int D, int NC, int BS, int BB = 1>
 __global__ void __launch_bounds__(BS * BB) k_update(const int *__restrict__ ids, const DBox *__restrict__ boxes,
                                                     UpdateArgs A, unsigned long long *chem_min, int nw) {
@@ -439,20 +482,27 @@ __global__ void __launch_bounds__(BS * BB) k_update(const int *__restrict__ ids,
         dens[sp] = fmax(cur, 0.0);
       }
       const double field = SI_to_Townsend * A.N_inv * A.efield[x];
-      int lix;
-      double lw;
-      lt_loc(A.chem, field, lix, lw);
-      const double k_ion = 1.0 * lt_at(A.chem, 0, lix, lw);
-      const double k_att = 1.0 * lt_at(A.chem, 1, lix, lw);
-      const double r1 = k_ion * dens[0];
-      const double r2 = k_att * dens[0];
-      double de = 0.0, dp = 0.0, dm = 0.0;
-      de = de - r1;
-      de = de + r1 * 2;
-      dp = dp + r1 * 1;
-      de = de - r2;
-      dm = dm + r2 * 1;
-      double dd[3] = {de, dp, dm};
+      double dd[3] = {0.0, 0.0, 0.0};
+      if (A.use_net) {
+        net_derivs(A.net, field, dens, dd);
+      } else {
+        int lix;
+        double lw;
+        lt_loc(A.chem, field, lix, lw);
+        const double k_ion = 1.0 * lt_at(A.chem, 0, lix, lw);
+        const double k_att = 1.0 * lt_at(A.chem, 1, lix, lw);
+        const double r1 = k_ion * dens[0];
+        const double r2 = k_att * dens[0];
+        double de = 0.0, dp = 0.0, dm = 0.0;
+        de = de - r1;
+        de = de + r1 * 2;
+        dp = dp + r1 * 1;
+        de = de - r2;
+        dm = dm + r2 * 1;
+        dd[0] = de;
+        dd[1] = dp;
+        dd[2] = dm;
+      }
       if (A.last_step) {
         for (int sp = 0; sp < 3; sp++) {
           double t;

@@ -500,6 +500,14 @@ int afgpu_chem_rates(afgpu_chem_t *c, int n_cells, const double *fields, double 
 int afgpu_chem_derivatives(afgpu_chem_t *c, int n_cells, const double *dens, double *rates, double *derivs);
 /* both in one pass, without the rates array: the same derivs */
 int afgpu_chem_source(afgpu_chem_t *c, int n_cells, const double *fields, const double *dens, double *derivs);
+/* The time step with a reaction list in place of the standard model
+ * (add_source_terms, src/m_fluid.f90:298-466, for the list given to
+ * chemistry_initialize, src/m_chemistry.f90:196-404): lists over the three
+ * species of the pools, in the order electrons, positive ion, negative ion
+ * (n_species 3; charges -1, +1, -1, as the rhs and the photoionization
+ * species assume).  NULL returns to the standard model.  The chemistry
+ * handle must stay alive while the step uses it. */
+int afgpu_set_chemistry(afgpu_t *h, const afgpu_chem_t *chem);
 /* Device memory for such arrays, from the library's own HIP runtime, for
  * callers without one: allocation, release, and a synchronous copy
  * (to_device 1: host to device; 0: device to host). */

@@ -20,6 +20,15 @@
 
 #include "afcpu_impl.h"
 
+struct afcpu_chem {
+  int n_species = 0, n_reactions = 0;
+  std::vector<afgpu_reaction_t> r;
+  afcpu::Table fld, energy;
+  double tg = 300;
+  bool need_te = false;
+};
+
+
 namespace afcpu {
 
 static thread_local std::string g_err;
@@ -1384,6 +1393,53 @@ static void flux_upwind_tree(Ctx &c, int s_deriv, double *dt_lim) {
   dt_lim[1] = l1;
 }
 
+// The rate coefficient of reaction R at field f (Td) and electron
+// temperature te, get_rates (src/m_chemistry.f90:699-767)
+static double chem_rate_cell(const afcpu_chem &c, const afgpu_reaction_t &R, double f, double te) {
+  const double kB = 1.380649e-23, eV = 1.602176634e-19;  // src/m_units_constants.f90:15-39
+  const double c0 = R.rate_factor;
+  const double *k = R.rate_data;
+  const double tg = c.tg;
+  switch (R.rate_type) {
+    case AFGPU_RATE_TABULATED_FIELD: return c0 * c.fld.col(R.lookup_col, f);
+    case AFGPU_RATE_CONSTANT: return c0 * k[0];
+    case AFGPU_RATE_LINEAR: return c0 * k[0] * (f - k[1]);
+    case AFGPU_RATE_EXP_V1: {
+      const double q = k[1] / (k[2] + f);
+      return c0 * k[0] * std::exp(-(q * q));
+    }
+    case AFGPU_RATE_EXP_V2: {
+      const double q = f / k[1];
+      return c0 * k[0] * std::exp(-(q * q));
+    }
+    case AFGPU_RATE_K1: return c0 * k[0] * std::pow(300 / te, k[1]);
+    case AFGPU_RATE_K3: {
+      const double q = (kB / eV) * te + k[1];
+      return c0 * (k[0] * (q * q) - k[2]) * k[3];
+    }
+    case AFGPU_RATE_K4: return c0 * k[0] * std::pow(tg / 300, k[1]) * std::exp(-k[2] / tg);
+    case AFGPU_RATE_K5: return c0 * k[0] * std::exp(-k[1] / tg);
+    case AFGPU_RATE_K6: return c0 * k[0] * std::pow(tg, k[1]);
+    case AFGPU_RATE_K7: return c0 * k[0] * std::pow(tg / k[1], k[2]);
+    case AFGPU_RATE_K8: return c0 * k[0] * std::pow(300 / tg, k[1]);
+    case AFGPU_RATE_K9: return c0 * k[0] * std::exp(-k[1] * tg);
+    case AFGPU_RATE_K10: return c0 * std::pow(10.0, k[0] + k[1] * (tg - 300));
+    case AFGPU_RATE_K11: return c0 * k[0] * std::pow(300 / tg, k[1]) * std::exp(-k[2] / tg);
+    case AFGPU_RATE_K12: return c0 * k[0] * std::pow(tg, k[1]) * std::exp(-k[2] / tg);
+    case AFGPU_RATE_K13: return c0 * k[0] * std::exp(-std::pow(k[1] / (k[2] + f), k[3]));
+    case AFGPU_RATE_K14: return c0 * k[0] * std::exp(-std::pow(f / k[1], k[2]));
+    case AFGPU_RATE_K15: return c0 * k[0] * std::exp(-std::pow(k[1] / (kB * (tg + f / k[2])), k[3]));
+    case AFGPU_RATE_K16:
+      return c0 * k[0] * std::pow(300 / te, k[1]) * std::exp(-k[2] / tg) * std::exp(k[3] * (te - tg) / (te * tg));
+  }
+  return 0;
+}
+static double chem_te(const afcpu_chem &c, double f) {
+  const double kB = 1.380649e-23, eV = 1.602176634e-19;
+  const double electron_eV_to_K = 2 * eV / (3 * kB);
+  return c.need_te ? electron_eV_to_K * c.energy.col(0, f) : 0.0;
+}
+
 // flux_update_densities (afivo/src/m_af_flux_schemes.f90:320-436) with
 // add_source_terms (src/m_fluid.f90:298-466), get_rates / get_derivatives
 // for the standard model (src/m_chemistry.f90:220-258, 672-804).
@@ -1427,17 +1483,33 @@ static void flux_update_densities(Ctx &c, double dt, int s_deriv, int n_prev, co
             double field = SI_to_Townsend * Ninv * E[x];
             double dens[3];
             for (int sp = 0; sp < 3; sp++) dens[sp] = std::max(c.C(AFGPU_CC_SPECIES(sp) + s_deriv, id)[x], 0.0);
-            double k_ion = 1.0 * c.chem.col(0, field);
-            double k_att = 1.0 * c.chem.col(1, field);
-            double r1 = k_ion * dens[0];
-            double r2 = k_att * dens[0];
-            double de = 0.0, dp = 0.0, dm = 0.0;
-            de = de - r1;
-            de = de + r1 * 2;
-            dp = dp + r1 * 1;
-            de = de - r2;
-            dm = dm + r2 * 1;
-            double dd[3] = {de, dp, dm};
+            double dd[3] = {0.0, 0.0, 0.0};
+            if (c.net) {
+              // get_rates and get_derivatives of the list (m_chemistry.f90:672-804)
+              const double te = chem_te(*c.net, field);
+              for (const afgpu_reaction_t &R : c.net->r) {
+                double kr = chem_rate_cell(*c.net, R, field, te);
+                double prod = 1.0;
+                for (int q = 0; q < R.n_in; q++) prod = prod * dens[R.ix_in[q]];
+                kr = kr * prod;
+                for (int q = 0; q < R.n_in; q++) dd[R.ix_in[q]] = dd[R.ix_in[q]] - kr;
+                for (int q = 0; q < R.n_out; q++) dd[R.ix_out[q]] = dd[R.ix_out[q]] + kr * R.multiplicity_out[q];
+              }
+            } else {
+              double k_ion = 1.0 * c.chem.col(0, field);
+              double k_att = 1.0 * c.chem.col(1, field);
+              double r1 = k_ion * dens[0];
+              double r2 = k_att * dens[0];
+              double de = 0.0, dp = 0.0, dm = 0.0;
+              de = de - r1;
+              de = de + r1 * 2;
+              dp = dp + r1 * 1;
+              de = de - r2;
+              dm = dm + r2 * 1;
+              dd[0] = de;
+              dd[1] = dp;
+              dd[2] = dm;
+            }
             if (last_step) {
               for (int sp = 0; sp < 3; sp++) {
                 double t;
@@ -2545,13 +2617,6 @@ int afcpu_num_threads(void) { return omp_get_max_threads(); }
 // ---- general chemistry networks ---------------------------------------------
 // get_rates (src/m_chemistry.f90:672-768) and get_derivatives (:770-804):
 // reaction by reaction over all cells, as the reference's array statements.
-struct afcpu_chem {
-  int n_species = 0, n_reactions = 0;
-  std::vector<afgpu_reaction_t> r;
-  Table fld, energy;
-  double tg = 300;
-  bool need_te = false;
-};
 
 int afcpu_chem_create(int n_species, int n_reactions, const afgpu_reaction_t *reactions, int table_n,
                       double table_x_min, double table_x_max, int n_fld_cols, const double *fld_cols,
@@ -2595,65 +2660,10 @@ int afcpu_chem_destroy(afcpu_chem *c) {
 int afcpu_chem_rates(afcpu_chem *c, int n_cells, const double *fields, double *rates) {
   if (!c || n_cells < 0) return AFGPU_ERR_ARG;
   const size_t n = (size_t)n_cells;
-  const double kB = 1.380649e-23, eV = 1.602176634e-19;  // src/m_units_constants.f90:15-39
-  const double electron_eV_to_K = 2 * eV / (3 * kB);
-  const double tg = c->tg;
-  std::vector<double> Te;
-  bool te_available = false;
-  for (int r = 0; r < c->n_reactions; r++) {
-    const afgpu_reaction_t &R = c->r[r];
-    const double c0 = R.rate_factor;
-    const double *k = R.rate_data;
-    double *out = rates + (size_t)r * n;
-    const int t = R.rate_type;
-    if ((t == AFGPU_RATE_K1 || t == AFGPU_RATE_K3 || t == AFGPU_RATE_K16) && !te_available) {
-      Te.resize(n);
-      for (size_t i = 0; i < n; i++) Te[i] = electron_eV_to_K * c->energy.col(0, fields[i]);
-      te_available = true;
-    }
-    for (size_t i = 0; i < n; i++) {
-      const double f = fields[i];
-      double v = 0;
-      switch (t) {
-        case AFGPU_RATE_TABULATED_FIELD: v = c0 * c->fld.col(R.lookup_col, f); break;
-        case AFGPU_RATE_CONSTANT: v = c0 * k[0]; break;
-        case AFGPU_RATE_LINEAR: v = c0 * k[0] * (f - k[1]); break;
-        case AFGPU_RATE_EXP_V1: {
-          const double q = k[1] / (k[2] + f);
-          v = c0 * k[0] * std::exp(-(q * q));
-          break;
-        }
-        case AFGPU_RATE_EXP_V2: {
-          const double q = f / k[1];
-          v = c0 * k[0] * std::exp(-(q * q));
-          break;
-        }
-        case AFGPU_RATE_K1: v = c0 * k[0] * std::pow(300 / Te[i], k[1]); break;
-        case AFGPU_RATE_K3: {
-          const double q = (kB / eV) * Te[i] + k[1];
-          v = c0 * (k[0] * (q * q) - k[2]) * k[3];
-          break;
-        }
-        case AFGPU_RATE_K4: v = c0 * k[0] * std::pow(tg / 300, k[1]) * std::exp(-k[2] / tg); break;
-        case AFGPU_RATE_K5: v = c0 * k[0] * std::exp(-k[1] / tg); break;
-        case AFGPU_RATE_K6: v = c0 * k[0] * std::pow(tg, k[1]); break;
-        case AFGPU_RATE_K7: v = c0 * k[0] * std::pow(tg / k[1], k[2]); break;
-        case AFGPU_RATE_K8: v = c0 * k[0] * std::pow(300 / tg, k[1]); break;
-        case AFGPU_RATE_K9: v = c0 * k[0] * std::exp(-k[1] * tg); break;
-        case AFGPU_RATE_K10: v = c0 * std::pow(10.0, k[0] + k[1] * (tg - 300)); break;
-        case AFGPU_RATE_K11: v = c0 * k[0] * std::pow(300 / tg, k[1]) * std::exp(-k[2] / tg); break;
-        case AFGPU_RATE_K12: v = c0 * k[0] * std::pow(tg, k[1]) * std::exp(-k[2] / tg); break;
-        case AFGPU_RATE_K13: v = c0 * k[0] * std::exp(-std::pow(k[1] / (k[2] + f), k[3])); break;
-        case AFGPU_RATE_K14: v = c0 * k[0] * std::exp(-std::pow(f / k[1], k[2])); break;
-        case AFGPU_RATE_K15: v = c0 * k[0] * std::exp(-std::pow(k[1] / (kB * (tg + f / k[2])), k[3])); break;
-        case AFGPU_RATE_K16:
-          v = c0 * k[0] * std::pow(300 / Te[i], k[1]) * std::exp(-k[2] / tg) *
-              std::exp(k[3] * (Te[i] - tg) / (Te[i] * tg));
-          break;
-      }
-      out[i] = v;
-    }
-  }
+  // (reaction by reaction over all cells, as the reference's array statements)
+  for (int r = 0; r < c->n_reactions; r++)
+    for (size_t i = 0; i < n; i++)
+      rates[(size_t)r * n + i] = chem_rate_cell(*c, c->r[r], fields[i], chem_te(*c, fields[i]));
   return AFGPU_OK;
 }
 
@@ -2690,3 +2700,13 @@ int afcpu_chem_source(afcpu_chem *c, int n_cells, const double *fields, const do
 }
 
 }  // extern "C"
+
+extern "C" int afcpu_set_chemistry(afcpu_t *h, const afcpu_chem *chem) {
+  CHECK_H(h);
+  if (chem && chem->n_species != 3) {
+    set_error("afcpu_set_chemistry: the pools hold three species");
+    return AFGPU_ERR_UNSUPPORTED;
+  }
+  h->c.net = chem;
+  return AFGPU_OK;
+}

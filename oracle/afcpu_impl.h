@@ -13,6 +13,8 @@
 
 #include "afgpu.h"
 
+struct afcpu_chem;  // a reaction list (afcpu_chem_create)
+
 namespace afcpu {
 
 constexpr double UC_eps0 = 8.8541878128e-12;       // src/m_units_constants.f90:12
@@ -69,6 +71,7 @@ struct Ctx {
   afgpu_params_t p{};
   bool have_params = false;
   Table td, chem;
+  const ::afcpu_chem *net = nullptr;  // a reaction list in place of the standard model
   double voltage = 0;
   Operator ops[1 + AFGPU_MAX_MODES];
   // time-loop state

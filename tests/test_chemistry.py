@@ -180,3 +180,16 @@ def test_tabulated_rates_interpolate_the_table(CH, tmp_path):
     f = np.array([-5.0, 0.0, 250.0, 500.0, 730.0, 1000.0, 2000.0])
     r, _ = run(CH, net, f, [np.ones(len(f)), np.zeros(len(f))])
     assert np.allclose(r[0], [1.0, 1.0, 6.0, 11.0, 15.6, 21.0, 21.0], rtol=1e-13)
+
+
+@pytest.mark.parametrize("name", ["cyl2d", "xyz3d_8"])
+def test_oracle_standard_model_as_a_list_equals_standard_steps(A, S, CH, name):
+    """The oracle's time step with the standard model written as a reaction
+    list (afcpu_set_chemistry) is the standard step, bit for bit."""
+    import test_gpu_chem_step as T
+    case = T.cases(S)[name]
+    ref, ri = T.run(A, S, case, "oracle", None)
+    p, pi = T.run(A, S, case, "oracle", T.chem_for(S, CH, case, "oracle", T.standard_list(A)))
+    assert pi == ri
+    for iv in range(A.N_CC):
+        assert np.array_equal(p.get_cc(iv), ref.get_cc(iv)), iv

@@ -1,0 +1,116 @@
+"""The time step with a reaction list (afgpu_set_chemistry).
+
+- The standard model written as a reaction list (the two tabulated
+  reactions chemistry_initialize builds without a list,
+  src/m_chemistry.f90:226-258, on the same table) gives the standard
+  model's steps bit for bit, on the device and in the oracle.
+- A richer list over the same three species (detachment with an exp-type
+  rate, electron-ion and ion-ion recombination, a gas-temperature rate):
+  device against oracle.  Its exp and pow may differ in the last bit between
+  device and host, so the states are compared to 1e-11 of each variable's
+  largest magnitude after three steps (they agree to about 1e-15 per rate).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def standard_list(A):
+    r1, r2 = A.ReactionT(), A.ReactionT()
+    for r, col in ((r1, 0), (r2, 1)):
+        r.rate_type, r.lookup_col, r.rate_factor, r.n_in = A.RATE_TABULATED_FIELD, col, 1.0, 1
+        r.ix_in[0] = 0
+    r1.n_out, r2.n_out = 2, 1
+    r1.ix_out[0], r1.multiplicity_out[0], r1.ix_out[1], r1.multiplicity_out[1] = 0, 2, 1, 1
+    r2.ix_out[0], r2.multiplicity_out[0] = 2, 1
+    return [r1, r2]
+
+
+def rich_list(A, N):
+    out = standard_list(A)
+
+    def add(t, c0, coef, ins, outs):
+        r = A.ReactionT()
+        r.rate_type, r.rate_factor, r.n_in, r.n_out = t, c0, len(ins), len(outs)
+        for q, v in enumerate(coef):
+            r.rate_data[q] = v
+        for q, s in enumerate(ins):
+            r.ix_in[q] = s
+        for q, (s, m) in enumerate(outs):
+            r.ix_out[q], r.multiplicity_out[q] = s, m
+        out.append(r)
+    add(A.RATE_EXP_V1, N, [1.24e-17, 179.0, 8.8], [2], [(0, 1)])      # M- + M -> e
+    add(A.RATE_CONSTANT, 1.0, [1e-13], [0, 1], [])                    # e + M+ ->
+    add(A.RATE_CONSTANT, 1.0, [2e-13], [1, 2], [])                    # M+ + M- ->
+    add(A.RATE_K8, 1.0, [3e-14, 0.7], [0, 1], [])                     # e + M+ -> (Tg)
+    return out
+
+
+def chem_for(S, CH, case, kind, reactions):
+    td = case.tdata
+    n, x0 = td["n"], td["chem_x_min"]
+    # the last grid point as the tables' set-up forms it (the field range of
+    # the transport data, in Td), so that LT_create's inv_fac is the table's
+    E = S.analytic_transport()[0]
+    x_max_td = E[-1] * 1e21 / case.params.gas_number_density
+    tmp = 1.0 / (n - 1)
+    xl = (n - 1) * tmp
+    x_last = 0.0 + xl * (x_max_td - 0.0)
+    assert (n - 1) / (x_last - x0) == td["chem_inv_fac"]
+    cols = np.asarray(td["chem"]).reshape(2, n)
+    return CH.Chem(kind, 3, reactions, x0, x_last, cols, None, 300.0)
+
+
+def cases(S):
+    return {
+        "cyl2d": S.small_amr(2, coord=2, uniform_lvl=3, max_lvl=5, frac=0.3, seed=3),
+        "xyz3d_8": S.small_amr(3, nc=8, uniform_lvl=2, max_lvl=4, frac=0.3, seed=11),
+    }
+
+
+def run(A, S, case, kind, chem, steps=3, seed_neg=False):
+    if kind == "gpu":
+        p = A.Path("gpu", case.D, case.nc, case.coord, A.PREC_FP64)
+    else:
+        p = A.Path("oracle", case.D, case.nc, case.coord)
+    S.apply(p, case)
+    if seed_neg:  # negative ions (a tenth of the electrons), neutral overall
+        p.put_cc(A.CC_NEG_ION, 0.1 * np.asarray(case.ne0))
+        p.put_cc(A.CC_POS_ION, 1.1 * np.asarray(case.ne0))
+    if chem is not None:
+        p.set_chemistry(chem)
+    p.field_compute(0, have_guess=False)
+    infos = [p.step() for _ in range(steps)]
+    return p, infos
+
+
+@pytest.mark.parametrize("name", ["cyl2d", "xyz3d_8"])
+def test_standard_model_as_a_list_equals_standard_steps(A, S, CH, name):
+    case = cases(S)[name]
+    for kind in ("gpu", "oracle"):
+        ref, ri = run(A, S, case, kind, None)
+        chem = chem_for(S, CH, case, kind, standard_list(A))
+        p, pi = run(A, S, case, kind, chem)
+        assert pi == ri, (name, kind)
+        for iv in range(A.N_CC):
+            assert np.array_equal(p.get_cc(iv), ref.get_cc(iv)), (name, kind, iv)
+
+
+@pytest.mark.parametrize("name", ["cyl2d", "xyz3d_8"])
+def test_reaction_list_steps_equal_oracle(A, S, CH, name):
+    case = cases(S)[name]
+    N = case.params.gas_number_density
+    g, gi = run(A, S, case, "gpu", chem_for(S, CH, case, "gpu", rich_list(A, N)), seed_neg=True)
+    o, oi = run(A, S, case, "oracle", chem_for(S, CH, case, "oracle", rich_list(A, N)), seed_neg=True)
+    for a, b in zip(gi, oi):
+        assert a["n_rejected"] == b["n_rejected"] and a["n_vcycles"] == b["n_vcycles"]
+        assert a["dt_next"] == pytest.approx(b["dt_next"], rel=1e-11)
+    for iv in (A.CC_ELECTRON, A.CC_POS_ION, A.CC_NEG_ION, A.CC_PHI, A.CC_EFIELD):
+        x, y = g.get_cc(iv), o.get_cc(iv)
+        assert np.max(np.abs(x - y)) <= 1e-11 * np.max(np.abs(y)), (name, iv)
+    # the list changed the state: electrons and ions recombine, negative
+    # ions detach and recombine
+    std, _ = run(A, S, case, "oracle", None, seed_neg=True)
+    for iv in (A.CC_ELECTRON, A.CC_POS_ION, A.CC_NEG_ION):
+        assert not np.array_equal(std.get_cc(iv), o.get_cc(iv)), (name, iv)
