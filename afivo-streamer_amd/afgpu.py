@@ -211,7 +211,9 @@ _COMMON = {
     "get_time_state": (_I, [_P, _IP, _DP, _DP]),
     "set_time_state": (_I, [_P, _I, _D, _D]),
     "regression_values": (_I, [_P, _I, _IP, _DP]),
-    "set_chemistry": (_I, [_P, _P]),
+    "set_chemistry": (_I, [_P, _P, _IP]),
+    "get_species": (_I, [_P, _I, _I, _I, _IP, _DP]),
+    "put_species": (_I, [_P, _I, _I, _I, _IP, _DP]),
     "get_boxes": (_I, [_P, _I, _IP, _I, _IP, _DP]),
     "put_boxes": (_I, [_P, _I, _IP, _I, _IP, _DP]),
 }
@@ -671,12 +673,24 @@ class Path:
         arr = (C.c_char_p * len(names))(*[n.encode() for n in names])
         self._call("write_regression_log", os.fsencode(filename), out_cnt, len(ivs), _ip(ivs), arr)
 
-    def set_chemistry(self, chem):
-        """The step's source terms from a reaction list over the three
-        species (a chemistry.Chem of the same kind), or the standard model
-        (None).  Keep chem alive while it is set."""
+    def set_chemistry(self, chem, charges=None):
+        """The step's source terms from a reaction list (a chemistry.Chem of
+        the same kind; species 0 the electrons, charges per species), or
+        the standard model (None).  Keep chem alive while it is set."""
         self._chem = chem
-        self._call("set_chemistry", chem.h if chem is not None else None)
+        q = None if charges is None else np.ascontiguousarray(charges, np.int32)
+        self._call("set_chemistry", chem.h if chem is not None else None, _ip(q) if q is not None else None)
+
+    def put_species(self, k, data, state=0, ids=None):
+        ids = self._ids(ids)
+        data = np.ascontiguousarray(data, np.float64).reshape(len(ids), self.ncell)
+        self._call("put_species", k, state, len(ids), _ip(ids), _dp(data))
+
+    def get_species(self, k, state=0, ids=None):
+        ids = self._ids(ids)
+        out = np.zeros((len(ids), self.ncell))
+        self._call("get_species", k, state, len(ids), _ip(ids), _dp(out))
+        return out
 
     def get_boxes(self, ivs, ids=None):
         """Whole boxes of the variables ivs in af_write_tree's order:

@@ -228,8 +228,23 @@ struct afgpu_ctx {
   bool fuse_direct = false;  // uniform-level solve: last-axis transforms fused into one pass
   bool direct_f32 = true;    // uniform-level solve of the FP32 inner cycle in FP32 (matrix cores)
   // refinement flags: the boxes evaluated and their flags (device)
-  // a reaction list over the three species in place of the standard model
+  // a reaction list in place of the standard model; with more species than
+  // the three pools (or other charges), the species beyond the pools live
+  // in xsp[(k - 3) * 3 + state] and the update is k_update_n
   const afgpu_chem *net = nullptr;
+  int n_sp = 3;
+  bool gen_sp = false;
+  std::vector<int> sp_charge{-1, 1, -1};
+  std::vector<double *> xsp;
+  SpeciesArgs species_args() const {
+    SpeciesArgs S{};
+    S.n = n_sp;
+    for (int k = 0; k < n_sp; k++) {
+      S.charge[k] = sp_charge[k];
+      for (int st = 0; st < 3; st++) S.sp[k][st] = k < 3 ? cc[AFGPU_CC_SPECIES(k) + st] : xsp[(k - 3) * 3 + st];
+    }
+    return S;
+  }
   int *d_ref_ids = nullptr, *d_ref_flags = nullptr;
   int ref_cap = 0;
   // diagnostics: per-box terms and per-run results; box I/O staging
@@ -1397,12 +1412,21 @@ struct Impl : ImplBase {
   int field_compute(int s_in, bool have_guess, int *n_out, double *res_out) override {
     const int R = c.n_runs;
     RCHK(set_active(nullptr));
-    RCHK(with_bb(KC_BOXOP, [&](auto B) {
-      constexpr int bb = decltype(B)::value;
-      return items<bb>(KC_BOXOP, k_set_rhs<G::NCELL, BS, bb>, c.l_all_leaves.n, c.list(c.l_all_leaves),
-                       (const double *)c.cc[AFGPU_CC_ELECTRON + s_in], (const double *)c.cc[AFGPU_CC_POS_ION + s_in],
-                       (const double *)c.cc[AFGPU_CC_NEG_ION + s_in], c.cc[AFGPU_CC_RHS]);
-    }));
+    if (c.gen_sp) {
+      const List &l = c.l_all_leaves;
+      if (l.n > 0) {
+        hipLaunchKernelGGL((k_set_rhs_n<G::NCELL, BS>), dim3(std::min(l.n, 8 * c.n_cu)), dim3(BS), 0, c.cur,
+                           c.list(l), c.species_args(), s_in, c.cc[AFGPU_CC_RHS], l.n);
+        HIPCHK(hipGetLastError());
+      }
+    } else {
+      RCHK(with_bb(KC_BOXOP, [&](auto B) {
+        constexpr int bb = decltype(B)::value;
+        return items<bb>(KC_BOXOP, k_set_rhs<G::NCELL, BS, bb>, c.l_all_leaves.n, c.list(c.l_all_leaves),
+                         (const double *)c.cc[AFGPU_CC_ELECTRON + s_in], (const double *)c.cc[AFGPU_CC_POS_ION + s_in],
+                         (const double *)c.cc[AFGPU_CC_NEG_ION + s_in], c.cc[AFGPU_CC_RHS]);
+      }));
+    }
     std::vector<double> max_rhs;
     RCHK(maxabs_runs(AFGPU_CC_RHS, max_rhs));
     const double conv_fac = 1e-10, min_residual = 1e-6, max_residual = 1e8;
@@ -1600,6 +1624,16 @@ struct Impl : ImplBase {
     A.cyl = cyl;
     A.use_net = c.net ? 1 : 0;
     if (c.net) A.net = c.net->args;
+    if (c.gen_sp) {
+      const List &l = c.l_all_leaves;
+      if (l.n > 0) {
+        const size_t lds = 2 * (size_t)c.n_sp * BS * sizeof(double);
+        hipLaunchKernelGGL((k_update_n<D, NC, BS>), dim3(std::min(l.n, 8 * c.n_cu)), dim3(BS), lds, c.cur, c.list(l),
+                           (const DBox *)c.d_boxes, A, c.species_args(), c.d_red + 3, l.n);
+        HIPCHK(hipGetLastError());
+      }
+      return AFGPU_OK;
+    }
     RCHK(with_bb(KC_UPDATE, [&](auto B) {
       constexpr int bb = decltype(B)::value;
       return items_timed<bb>(2, (7.0 + D + (c.p.photoi_enabled ? 1 : 0)) * 8.0 * c.l_all_leaves.n * G::NINT, KC_UPDATE,
@@ -1825,12 +1859,12 @@ struct Impl : ImplBase {
 
   int copy_states(bool backup) override {
     const int ns = c.n_states();
-    for (int sp = 0; sp < 3; sp++) {
-      const int iv = AFGPU_CC_SPECIES(sp);
+    const SpeciesArgs S = c.species_args();
+    for (int sp = 0; sp < S.n; sp++) {
       if (backup)
-        RCHK(boxop(c.l_all_ids, c.cc[iv], c.cc[iv + ns], OP_COPY));
+        RCHK(boxop(c.l_all_ids, S.sp[sp][0], S.sp[sp][ns], OP_COPY));
       else
-        RCHK(boxop(c.l_all_ids, c.cc[iv + ns], c.cc[iv], OP_COPY));
+        RCHK(boxop(c.l_all_ids, S.sp[sp][ns], S.sp[sp][0], OP_COPY));
     }
     if (backup)
       RCHK(boxop(c.l_all_ids, c.cc[AFGPU_CC_PHI], c.cc[AFGPU_CC_PHI_2], OP_COPY));
@@ -2290,6 +2324,7 @@ int afgpu_destroy(afgpu_t *h) {
   (void)hipFree(h->d_ref_flags);
   (void)hipFree(h->d_diag);
   (void)hipFree(h->d_io);
+  for (double *x : h->xsp) (void)hipFree(x);
   (void)hipFree(h->dir.A0);
   (void)hipFree(h->dir.A1);
   for (auto &f : h->dir.f) {
@@ -2510,6 +2545,7 @@ int afgpu_sync_tree(afgpu_t *h, int n_boxes, const int *lvl, const int *ix, cons
     };
     for (int i = 0; i < AFGPU_N_CC; i++) RCHK(grow(h->cc[i], h->ncell));
     for (int i = 0; i < AFGPU_N_FC; i++) RCHK(grow(h->fc[i], h->nface));
+    for (double *&x : h->xsp) RCHK(grow(x, h->ncell));
     if (h->precision == AFGPU_PREC_MIXED) {
       for (int i = 0; i < 4; i++) {
         (void)hipFree(h->cc32[i]);
@@ -2618,15 +2654,20 @@ int afgpu_sync_tree(afgpu_t *h, int n_boxes, const int *lvl, const int *ix, cons
   return AFGPU_OK;
 }
 
+static int xfer_pool(afgpu_t *h, double *pool, size_t blk, int n_ids, const int *ids, double *data, bool put);
 static int xfer(afgpu_t *h, bool cc, int iv, int n_ids, const int *ids, double *data, bool put) {
-  if (!h || (cc && (iv < 0 || iv >= AFGPU_N_CC)) || (!cc && (iv < 0 || iv >= AFGPU_N_FC)) || n_ids < 0 ||
-      (n_ids > 0 && (!ids || !data))) {
+  if (!h || (cc && (iv < 0 || iv >= AFGPU_N_CC)) || (!cc && (iv < 0 || iv >= AFGPU_N_FC))) {
+    set_err("transfer: invalid arguments");
+    return AFGPU_ERR_ARG;
+  }
+  return xfer_pool(h, cc ? h->cc[iv] : h->fc[iv], cc ? h->ncell : h->nface, n_ids, ids, data, put);
+}
+static int xfer_pool(afgpu_t *h, double *pool, size_t blk, int n_ids, const int *ids, double *data, bool put) {
+  if (n_ids < 0 || (n_ids > 0 && (!ids || !data))) {
     set_err("transfer: invalid arguments");
     return AFGPU_ERR_ARG;
   }
   if (n_ids == 0) return AFGPU_OK;
-  const size_t blk = cc ? h->ncell : h->nface;
-  double *pool = cc ? h->cc[iv] : h->fc[iv];
   for (int n = 0; n < n_ids; n++)
     if (ids[n] < 1 || ids[n] > h->n_boxes) {
       set_err("transfer: box id out of range");
@@ -2970,6 +3011,10 @@ static int ensure_ref_buffers(afgpu_t *h, int n) {
 // before their children are removed
 int afgpu_restrict_boxes(afgpu_t *h, int n_ids, const int *ids) {
   RCHK(ready(h));
+  if (h->gen_sp) {
+    set_err("afgpu_restrict_boxes: not with a reaction list beyond the three standard species");
+    return AFGPU_ERR_UNSUPPORTED;
+  }
   const int n_boxes = (int)h->hboxes.size(), nch = 1 << h->D;
   if (n_ids < 0 || (n_ids > 0 && !ids)) return AFGPU_ERR_ARG;
   std::vector<int> ch;
@@ -2994,6 +3039,10 @@ int afgpu_restrict_boxes(afgpu_t *h, int n_ids, const int *ids) {
 // then the new boxes' ghost cells (af_gc_box with corners)
 int afgpu_prolong_boxes(afgpu_t *h, int n_ids, const int *ids) {
   RCHK(ready(h));
+  if (h->gen_sp) {
+    set_err("afgpu_prolong_boxes: not with a reaction list beyond the three standard species");
+    return AFGPU_ERR_UNSUPPORTED;
+  }
   const int n_boxes = (int)h->hboxes.size();
   if (n_ids < 0 || (n_ids > 0 && !ids)) return AFGPU_ERR_ARG;
   std::vector<int> sorted;
@@ -3735,18 +3784,62 @@ int afgpu_device_copy(void *dst, const void *src, size_t bytes, int to_device) {
   return AFGPU_OK;
 }
 
-// The time step's source terms from a reaction list over the pools' three
-// species (0: electrons, 1: the positive ion, 2: the negative ion), as the
-// reference's add_source_terms computes them for any list
-// (src/m_fluid.f90:298-466); NULL returns to the standard model.  The
+// The time step's source terms from a reaction list (add_source_terms,
+// src/m_fluid.f90:298-466).  Species 0 are the electrons; species 1 and 2
+// live in the pools of the positive and negative ions, further species in
+// pools of their own; charges[k] is species k's (NULL: -1, +1, -1 for a
+// list over three species).  NULL returns to the standard model.  The
 // chemistry handle must outlive its use by the step.
-int afgpu_set_chemistry(afgpu_t *h, const afgpu_chem_t *chem) {
+int afgpu_set_chemistry(afgpu_t *h, const afgpu_chem_t *chem, const int *charges) {
   if (!h) return AFGPU_ERR_ARG;
-  if (chem && chem->n_species != 3) {
-    set_err("afgpu_set_chemistry: the pools hold three species (electrons, positive and negative ions)");
+  const int n = chem ? chem->n_species : 3;
+  if (chem && (n < 3 || n > SP_MAX || (n > 3 && !charges) || (charges && charges[0] != -1))) {
+    set_err("afgpu_set_chemistry: 3 to 16 species, the electrons first, charges for more than three");
+    return AFGPU_ERR_ARG;
+  }
+  std::vector<int> q{-1, 1, -1};
+  if (chem && charges) q.assign(charges, charges + n);
+  const bool gen = chem && (n > 3 || q != std::vector<int>{-1, 1, -1});
+  if (gen && h->n_runs > 1) {
+    set_err("afgpu_set_chemistry: lists beyond the three standard species in a forest of runs");
     return AFGPU_ERR_UNSUPPORTED;
   }
-  if (chem != h->net) drop_graphs(*h);  // (the list is an argument of captured updates)
+  if (gen && h->p.photoi_enabled && (h->p.photoi_species < 0 || h->p.photoi_species >= n)) {
+    set_err("afgpu_set_chemistry: photoionization species out of range");
+    return AFGPU_ERR_ARG;
+  }
+  HIPCHK(hipStreamSynchronize(h->stream));
+  drop_graphs(*h);  // (the list and the pools are arguments of captured updates)
+  // pools of the species beyond the three, zero-filled
+  const size_t want = gen ? (size_t)(n - 3) * 3 : 0;
+  if (h->xsp.size() != want) {
+    for (double *x : h->xsp) (void)hipFree(x);
+    h->xsp.assign(want, nullptr);
+    for (double *&x : h->xsp) {
+      HIPCHK(hipMalloc(&x, std::max<size_t>(1, (size_t)h->cap_boxes * h->ncell) * sizeof(double)));
+      HIPCHK(hipMemset(x, 0, std::max<size_t>(1, (size_t)h->cap_boxes * h->ncell) * sizeof(double)));
+    }
+  }
   h->net = chem;
+  h->n_sp = gen ? n : 3;
+  h->gen_sp = gen;
+  h->sp_charge = gen ? q : std::vector<int>{-1, 1, -1};
   return AFGPU_OK;
+}
+
+// A species' cells (box by box, as afgpu_get_cc): species 0-2 are the pools
+// of the electrons and the two ions, further species those of a reaction
+// list (afgpu_set_chemistry).
+static int species_xfer(afgpu_t *h, int k, int state, int n_ids, const int *ids, double *data, bool put) {
+  if (!h || k < 0 || k >= h->n_sp || state < 0 || state > 2) {
+    set_err("species transfer: invalid species or state");
+    return AFGPU_ERR_ARG;
+  }
+  return xfer_pool(h, h->species_args().sp[k][state], h->ncell, n_ids, ids, data, put);
+}
+int afgpu_get_species(afgpu_t *h, int k, int state, int n_ids, const int *ids, double *data) {
+  return species_xfer(h, k, state, n_ids, ids, data, false);
+}
+int afgpu_put_species(afgpu_t *h, int k, int state, int n_ids, const int *ids, const double *data) {
+  return species_xfer(h, k, state, n_ids, ids, const_cast<double *>(data), true);
 }

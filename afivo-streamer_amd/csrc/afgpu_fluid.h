@@ -571,6 +571,142 @@ __global__ void __launch_bounds__(BS * BB) k_set_rhs(const int *__restrict__ ids
   }
 }
 
+// More species than the standard three (a reaction list, afgpu_set_chemistry):
+// pool pointers and charges of every species.
+constexpr int SP_MAX = 16;
+struct SpeciesArgs {
+  double *sp[SP_MAX][3];  // species k, state s
+  int charge[SP_MAX];
+  int n;
+};
+
+// field_set_rhs (src/m_field.f90:406-444) with every charged species, in
+// species order.
+template <int NCELL, int BS>
+__global__ void __launch_bounds__(BS) k_set_rhs_n(const int *__restrict__ ids, SpeciesArgs S, int s_in, double *rhs,
+                                                  int nw) {
+  for (int w = blockIdx.x; w < nw; w += gridDim.x) {
+    const int id = ids[w];
+    if (id <= 0) continue;
+    const size_t base = (size_t)(id - 1) * NCELL;
+    const double fac = -UC_elem_charge / UC_eps0;
+    for (int x = threadIdx.x; x < NCELL; x += BS) {
+      double r = 0.0;
+      for (int k = 0; k < S.n; k++)
+        if (S.charge[k] != 0) r = r + (S.charge[k] * fac) * S.sp[k][s_in][base + x];
+      rhs[base + x] = r;
+    }
+  }
+}
+
+// k_update for a reaction list over S.n species: the densities and
+// derivatives of the workgroup's cells in LDS ([S.n][BS] each); the
+// electrons (species 0) also take the flux divergence.  One workgroup per
+// leaf box.
+template <int D, int NC, int BS>
+__global__ void __launch_bounds__(BS) k_update_n(const int *__restrict__ ids, const DBox *__restrict__ boxes,
+                                                 UpdateArgs A, SpeciesArgs S, unsigned long long *chem_min, int nw) {
+  using G = Geo<D, NC>;
+  extern __shared__ double lds[];
+  double *sdn = lds, *sdd = lds + (size_t)S.n * BS;
+  __shared__ double sh[BS / 64];
+  const int t = threadIdx.x;
+  for (int w = blockIdx.x; w < nw; w += gridDim.x) {
+    const int id = ids[w];
+    if (id <= 0) continue;
+    const DBox &bx = boxes[id - 1];
+    const size_t base = (size_t)(id - 1) * G::NCELL;
+    const size_t fbase = (size_t)(id - 1) * D * G::NFACE;
+    const double dt = A.dt;
+    double dt_dr[3];
+    for (int d = 0; d < D; d++) dt_dr[d] = dt / bx.dr[d];
+    const double eps = 1e-100;
+    double bmin = 1e100;
+    for (int q = t; q < G::NINT; q += BS) {
+      int i, j, k;
+      interior_ijk<D, NC>(q, i, j, k);
+      const size_t x = base + G::idx(i, j, k);
+      for (int s = 0; s < S.n; s++) {
+        double tmp = 0.0;
+        for (int m = 0; m < A.n_prev; m++) tmp = tmp + A.w_prev[m] * S.sp[s][A.s_prev[m]][x];
+        const double cur = (A.s_deriv == A.s_out) ? tmp : S.sp[s][A.s_deriv][x];
+        sdn[s * BS + t] = fmax(cur, 0.0);
+        sdd[s * BS + t] = 0.0;
+        S.sp[s][A.s_out][x] = tmp;  // (the output state's old value is no longer read)
+      }
+      const double field = SI_to_Townsend * A.N_inv * A.efield[x];
+      const ChemArgs &a = A.net;
+      int ix;
+      double wl, te = 0;
+      lt_loc(a.fld, field, ix, wl);
+      if (a.need_te) {
+        int ie;
+        double we;
+        lt_loc(a.energy, field, ie, we);
+        te = CHEM_EV_TO_K * lt_at(a.energy, 0, ie, we);
+      }
+      for (int r = 0; r < a.n_reactions; r++) {
+        const afgpu_reaction_t &R = a.r[r];
+        double kr = chem_rate(R, a, field, te, ix, wl);
+        double prod = 1.0;
+        for (int qq = 0; qq < R.n_in; qq++) prod = prod * sdn[R.ix_in[qq] * BS + t];
+        kr = kr * prod;
+        for (int qq = 0; qq < R.n_in; qq++) {
+          double &d = sdd[R.ix_in[qq] * BS + t];
+          d = d - kr;
+        }
+        for (int qq = 0; qq < R.n_out; qq++) {
+          double &d = sdd[R.ix_out[qq] * BS + t];
+          d = d + kr * R.multiplicity_out[qq];
+        }
+      }
+      if (A.last_step)
+        for (int s = 0; s < S.n; s++) {
+          const double dn = sdn[s * BS + t], dd = sdd[s * BS + t];
+          double tl;
+          if (A.chem_nmin_on)
+            tl = (dn + A.chem_nmin) / fmax(fabs(dd), eps);
+          else if (A.chem_limit_loss)
+            tl = fmax(dn, eps) / fmax(-dd, eps);
+          else
+            tl = 1e100;
+          bmin = fmin(bmin, tl);
+        }
+      if (A.photoi_enabled) {
+        const double ph = A.photo[x];
+        sdd[0 * BS + t] = sdd[0 * BS + t] + ph;
+        sdd[A.photoi_species * BS + t] = sdd[A.photoi_species * BS + t] + ph;
+      }
+      for (int s = 0; s < S.n; s++) S.sp[s][A.s_out][x] = S.sp[s][A.s_out][x] + dt * sdd[s * BS + t];
+      // conservative flux divergence of the electrons
+      const double *fl = A.flux + fbase;
+      double ne = S.sp[0][A.s_out][x];
+      if (D == 2) {
+        if (A.cyl) {
+          const double dr = bx.dr[0];
+          const double r = bx.r_min[0] + (i - 0.5) * dr;
+          const double inv_r = 1 / r;
+          const double rf1 = (r - 0.5 * dr) * inv_r, rf2 = (r + 0.5 * dr) * inv_r;
+          ne = ne + dt_dr[0] * (rf1 * fl[G::fidx(i, j, 1, 0)] - rf2 * fl[G::fidx(i + 1, j, 1, 0)]) +
+               dt_dr[1] * (fl[G::fidx(i, j, 1, 1)] - fl[G::fidx(i, j + 1, 1, 1)]);
+        } else {
+          ne = ne + dt_dr[0] * (fl[G::fidx(i, j, 1, 0)] - fl[G::fidx(i + 1, j, 1, 0)]) +
+               dt_dr[1] * (fl[G::fidx(i, j, 1, 1)] - fl[G::fidx(i, j + 1, 1, 1)]);
+        }
+      } else {
+        ne = ne + dt_dr[0] * (fl[G::fidx(i, j, k, 0)] - fl[G::fidx(i + 1, j, k, 0)]) +
+             dt_dr[1] * (fl[G::fidx(i, j, k, 1)] - fl[G::fidx(i, j + 1, k, 1)]) +
+             dt_dr[2] * (fl[G::fidx(i, j, k, 2)] - fl[G::fidx(i, j, k + 1, 2)]);
+      }
+      S.sp[0][A.s_out][x] = ne;
+    }
+    if (A.last_step) {
+      bmin = block_min<BS>(bmin, sh);
+      if (t == 0) atomic_min_nonneg(run_slot(chem_min, bx), bmin);
+    }
+  }
+}
+
 // E = -grad phi on all faces and |E| at cell centres
 // (mg_compute_phi_gradient / mg_box_lpl_gradient / mg_box_field_norm,
 // afivo/src/m_af_multigrid.f90:1857-2045).

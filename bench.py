@@ -254,6 +254,37 @@ def measure_chemistry(n_cells, with_cpu, reps=10):
     return out
 
 
+def measure_air_step(case, steps=5, warmup=2):
+    """The time step with the reference's air network (8 species, 25
+    reactions) in place of the standard model (afgpu_set_chemistry), mixed
+    precision, on the same workload: cell-steps/s."""
+    import chemistry as CH
+    N = case.params.gas_number_density
+    E = S.analytic_transport()[0]
+    x_max = E[-1] * 1e21 / N
+    net = CH.read_network(os.path.join(ROOT, "tests", "golden", "air_chemistry_v1.txt"), {"N2": 0.8, "O2": 0.2}, N,
+                          0.0, x_max, 1000)
+    p = A.Path("gpu", case.D, case.nc, case.coord, A.PREC_MIXED)
+    S.apply(p, case)
+    chem = CH.Chem("gpu", len(net.species), net.reactions, 0.0, x_max, net.fld_cols, None, 300.0)
+    p.set_chemistry(chem, net.charges)
+    ne0 = np.asarray(case.ne0)
+    p.put_species(1, 0.8 * ne0)
+    p.put_species(2, 0.2 * ne0)
+    p.field_compute(0, have_guess=False)
+    for _ in range(warmup):
+        p.step()
+    t0 = time.perf_counter()
+    cells = sum(p.step()["leaf_cells"] for _ in range(steps))
+    secs = time.perf_counter() - t0
+    out = {"network": "air_chemistry_v1 (8 species, 25 reactions)", "precision": "mixed", "steps": steps,
+           "value": cells / secs, "unit": "cell-steps/s", "ms_per_step": secs * 1e3 / steps}
+    p.close()
+    chem.close()
+    log("  air network step", json.dumps(out))
+    return out
+
+
 def measure_new_boxes(case, p, c=None):
     """Data of new boxes on the device (afgpu_prolong_boxes) after refining
     a tenth of the finest leaves whose face neighbours all exist, against
@@ -591,6 +622,7 @@ def main():
     extra["refine_flags"] = measure_refine(case, p, None if args.no_cpu else c)
     extra["diagnostics"] = measure_diagnostics(case, p, None if args.no_cpu else c)
     extra["chemistry"] = measure_chemistry(int(info["leaf_cells"]), not args.no_cpu)
+    extra["air_network_step"] = measure_air_step(case)
     extra["new_boxes"] = measure_new_boxes(case, p, None if args.no_cpu else c)
 
     out = {

@@ -502,12 +502,20 @@ int afgpu_chem_derivatives(afgpu_chem_t *c, int n_cells, const double *dens, dou
 int afgpu_chem_source(afgpu_chem_t *c, int n_cells, const double *fields, const double *dens, double *derivs);
 /* The time step with a reaction list in place of the standard model
  * (add_source_terms, src/m_fluid.f90:298-466, for the list given to
- * chemistry_initialize, src/m_chemistry.f90:196-404): lists over the three
- * species of the pools, in the order electrons, positive ion, negative ion
- * (n_species 3; charges -1, +1, -1, as the rhs and the photoionization
- * species assume).  NULL returns to the standard model.  The chemistry
- * handle must stay alive while the step uses it. */
-int afgpu_set_chemistry(afgpu_t *h, const afgpu_chem_t *chem);
+ * chemistry_initialize, src/m_chemistry.f90:196-404).  Species 0 are the
+ * electrons (charge -1, the transported species); species 1 and 2 live in
+ * the pools of the positive and negative ions (AFGPU_CC_SPECIES(1), (2)),
+ * further species (at most 16 in all) in pools of their own; charges[k] is
+ * species k's charge (NULL with three species: -1, +1, -1).  The rhs sums
+ * the charged species in species order (field_set_rhs); photoi_species
+ * indexes the list.  NULL returns to the standard model.  With more than
+ * the three species the handle must hold one run, and afgpu_restrict_boxes
+ * / afgpu_prolong_boxes are refused.  The chemistry handle must stay alive
+ * while the step uses it. */
+int afgpu_set_chemistry(afgpu_t *h, const afgpu_chem_t *chem, const int *charges);
+/* A species' cells in a state (box by box, as afgpu_get_cc / _put_cc). */
+int afgpu_get_species(afgpu_t *h, int k, int state, int n_ids, const int *ids, double *data);
+int afgpu_put_species(afgpu_t *h, int k, int state, int n_ids, const int *ids, const double *data);
 /* Device memory for such arrays, from the library's own HIP runtime, for
  * callers without one: allocation, release, and a synchronous copy
  * (to_device 1: host to device; 0: device to host). */

@@ -941,7 +941,7 @@ static double tree_maxabs(Ctx &c, int iv) {
 // ---- field (src/m_field.f90) ----------------------------------------------
 static void field_set_rhs(Ctx &c, int s_in) {  // :406-444
   const double fac = -UC_elem_charge / UC_eps0;
-  const int charge[3] = {-1, 1, -1};
+  const std::vector<int> &charge = c.sp_charge;
   for (int lvl = 1; lvl <= c.highest_lvl; lvl++) {
     const std::vector<int> &lv = c.leaves[lvl];
 #pragma omp parallel for schedule(static)
@@ -949,8 +949,9 @@ static void field_set_rhs(Ctx &c, int s_in) {  // :406-444
       int id = lv[i];
       double *rh = c.C(AFGPU_CC_RHS, id);
       for (size_t q = 0; q < c.ncell; q++) rh[q] = 0.0;
-      for (int n = 0; n < 3; n++) {
-        const double *u = c.C(AFGPU_CC_SPECIES(n) + s_in, id);
+      for (int n = 0; n < c.n_sp; n++) {
+        if (charge[n] == 0) continue;
+        const double *u = c.SP(n, s_in, id);
         double qf = charge[n] * fac;
         for (size_t q = 0; q < c.ncell; q++) rh[q] = rh[q] + qf * u[q];
       }
@@ -1440,6 +1441,113 @@ static double chem_te(const afcpu_chem &c, double f) {
   return c.need_te ? electron_eV_to_K * c.energy.col(0, f) : 0.0;
 }
 
+// add_source_terms (src/m_fluid.f90:298-466) for a reaction list over
+// c.n_sp species on one box, after the states' combination of
+// flux_update_densities; returns the box's chemistry dt limit.
+static double update_species_box(Ctx &c, int id, double dt, int s_deriv, int n_prev, const int *s_prev,
+                                 const double *w_prev, int s_out, bool last_step) {
+  const int nc = c.nc, D = c.D, ns = c.n_sp;
+  const int kmin = (D == 3) ? 1 : 0, kmax = (D == 3) ? nc : 0;
+  const double eps = 1e-100;
+  for (int sp = 0; sp < ns; sp++) {
+    double *out = c.SP(sp, s_out, id);
+    for (int k = kmin; k <= kmax; k++)
+      for (int j = 1; j <= nc; j++)
+        for (int i = 1; i <= nc; i++) {
+          long x = c.idx(i, j, k);
+          double tmp = 0.0;
+          for (int m = 0; m < n_prev; m++) tmp = tmp + w_prev[m] * c.SP(sp, s_prev[m], id)[x];
+          out[x] = tmp;
+        }
+  }
+  const double Ninv = 1 / c.p.gas_number_density;
+  const double *E = c.C(AFGPU_CC_EFIELD, id);
+  const double *photo = c.C(AFGPU_CC_PHOTO, id);
+  const afcpu_chem &net = *c.net;
+  double box_min = 1e100;
+  std::vector<double> der((size_t)ns * nc * nc * (D == 3 ? nc : 1)), dens(ns), dd(ns);
+  size_t cell = 0;
+  for (int k = kmin; k <= kmax; k++)
+    for (int j = 1; j <= nc; j++)
+      for (int i = 1; i <= nc; i++, cell++) {
+        long x = c.idx(i, j, k);
+        const double field = SI_to_Townsend * Ninv * E[x];
+        for (int sp = 0; sp < ns; sp++) {
+          dens[sp] = std::max(c.SP(sp, s_deriv, id)[x], 0.0);
+          dd[sp] = 0.0;
+        }
+        const double te = chem_te(net, field);
+        for (const afgpu_reaction_t &R : net.r) {
+          double kr = chem_rate_cell(net, R, field, te);
+          double prod = 1.0;
+          for (int q = 0; q < R.n_in; q++) prod = prod * dens[R.ix_in[q]];
+          kr = kr * prod;
+          for (int q = 0; q < R.n_in; q++) dd[R.ix_in[q]] = dd[R.ix_in[q]] - kr;
+          for (int q = 0; q < R.n_out; q++) dd[R.ix_out[q]] = dd[R.ix_out[q]] + kr * R.multiplicity_out[q];
+        }
+        if (last_step)
+          for (int sp = 0; sp < ns; sp++) {
+            double t;
+            if (c.p.dt_chemistry_nmin > 0)
+              t = (dens[sp] + c.p.dt_chemistry_nmin) / std::max(std::fabs(dd[sp]), eps);
+            else if (c.p.dt_chemistry_limit_loss)
+              t = std::max(dens[sp], eps) / std::max(-dd[sp], eps);
+            else
+              t = 1e100;
+            box_min = std::min(box_min, t);
+          }
+        if (c.p.photoi_enabled) {
+          dd[0] = dd[0] + photo[x];
+          dd[c.p.photoi_species] = dd[c.p.photoi_species] + photo[x];
+        }
+        for (int sp = 0; sp < ns; sp++) der[(size_t)ns * cell + sp] = dd[sp];
+      }
+  for (int sp = 0; sp < ns; sp++) {
+    double *out = c.SP(sp, s_out, id);
+    cell = 0;
+    for (int k = kmin; k <= kmax; k++)
+      for (int j = 1; j <= nc; j++)
+        for (int i = 1; i <= nc; i++, cell++) {
+          long x = c.idx(i, j, k);
+          out[x] = out[x] + dt * der[(size_t)ns * cell + sp];
+        }
+  }
+  return box_min;
+}
+
+// The conservative flux divergence of the electrons on one box
+// (flux_update_densities, afivo/src/m_af_flux_schemes.f90:320-436).
+static void electron_flux_divergence(Ctx &c, int id, const double *dt_dr, int s_out) {
+  const int nc = c.nc, D = c.D;
+  const Box &bx = c.B(id);
+  double *ne = c.C(AFGPU_CC_ELECTRON + s_out, id);
+  const double *fl = c.F(AFGPU_FC_FLUX, id);
+  if (D == 2) {
+    double rf1[33], rf2[33];
+    const bool cyl = (c.coord == AFGPU_CYL);
+    if (cyl) cyl_factors(c, bx, rf1, rf2);
+    for (int j = 1; j <= nc; j++)
+      for (int i = 1; i <= nc; i++) {
+        long x = c.idx(i, j, 0);
+        if (cyl)
+          ne[x] = ne[x] + dt_dr[0] * (rf1[i] * fl[c.fidx(i, j, 1, 0)] - rf2[i] * fl[c.fidx(i + 1, j, 1, 0)]) +
+                  dt_dr[1] * (fl[c.fidx(i, j, 1, 1)] - fl[c.fidx(i, j + 1, 1, 1)]);
+        else
+          ne[x] = ne[x] + dt_dr[0] * (fl[c.fidx(i, j, 1, 0)] - fl[c.fidx(i + 1, j, 1, 0)]) +
+                  dt_dr[1] * (fl[c.fidx(i, j, 1, 1)] - fl[c.fidx(i, j + 1, 1, 1)]);
+      }
+  } else {
+    for (int k = 1; k <= nc; k++)
+      for (int j = 1; j <= nc; j++)
+        for (int i = 1; i <= nc; i++) {
+          long x = c.idx(i, j, k);
+          ne[x] = ne[x] + dt_dr[0] * (fl[c.fidx(i, j, k, 0)] - fl[c.fidx(i + 1, j, k, 0)]) +
+                  dt_dr[1] * (fl[c.fidx(i, j, k, 1)] - fl[c.fidx(i, j + 1, k, 1)]) +
+                  dt_dr[2] * (fl[c.fidx(i, j, k, 2)] - fl[c.fidx(i, j, k + 1, 2)]);
+        }
+  }
+}
+
 // flux_update_densities (afivo/src/m_af_flux_schemes.f90:320-436) with
 // add_source_terms (src/m_fluid.f90:298-466), get_rates / get_derivatives
 // for the standard model (src/m_chemistry.f90:220-258, 672-804).
@@ -1457,6 +1565,11 @@ static void flux_update_densities(Ctx &c, double dt, int s_deriv, int n_prev, co
       const Box &bx = c.B(id);
       double dt_dr[3];
       for (int d = 0; d < D; d++) dt_dr[d] = dt / bx.dr[d];
+      if (c.gen_sp) {
+        chem_min = std::min(chem_min, update_species_box(c, id, dt, s_deriv, n_prev, s_prev, w_prev, s_out, last_step));
+        electron_flux_divergence(c, id, dt_dr, s_out);
+        continue;
+      }
       for (int sp = 0; sp < 3; sp++) {
         const int iv = AFGPU_CC_SPECIES(sp);
         double *out = c.C(iv + s_out, id);
@@ -1672,12 +1785,12 @@ static void copy_states(Ctx &c, bool backup) {  // src/streamer.f90:659-688
     const std::vector<int> &ids = c.ids[lvl];
 #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < ids.size(); i++) {
-      for (int sp = 0; sp < 3; sp++) {
-        int iv = AFGPU_CC_SPECIES(sp);
+      for (int sp = 0; sp < c.n_sp; sp++) {
+        double *a = c.SP(sp, 0, ids[i]), *b = c.SP(sp, ns, ids[i]);
         if (backup)
-          box_copy(c, ids[i], iv, iv + ns);
+          std::copy(a, a + c.ncell, b);
         else
-          box_copy(c, ids[i], iv + ns, iv);
+          std::copy(b, b + c.ncell, a);
       }
       if (backup)
         box_copy(c, ids[i], AFGPU_CC_PHI, AFGPU_CC_PHI_2);
@@ -1797,6 +1910,7 @@ static int sync_tree(Ctx &c, int n_boxes, const int *lvl, const int *ix, const i
   }
   for (int iv = 0; iv < AFGPU_N_CC; iv++) c.cc[iv].resize((size_t)n_boxes * c.ncell, 0.0);
   for (int f = 0; f < AFGPU_N_FC; f++) c.fc[f].resize((size_t)n_boxes * D * c.nface, 0.0);
+  for (auto &x : c.xs) x.resize((size_t)n_boxes * c.ncell, 0.0);
   if (hl < 1 || c.ids[1].size() != 1) {
     set_error("level 1 must consist of exactly one box");
     return AFGPU_ERR_UNSUPPORTED;
@@ -2445,6 +2559,7 @@ static int check_ids(afcpu_t *h, int n, const int *ids, bool parents) {
 int afcpu_restrict_boxes(afcpu_t *h, int n_ids, const int *ids) {
   int rc = ready(h, -1);
   if (rc) return rc;
+  if (h->c.gen_sp) return AFGPU_ERR_UNSUPPORTED;
   if (check_ids(h, n_ids, ids, true)) {
     set_error("afcpu_restrict_boxes: invalid ids");
     return AFGPU_ERR_ARG;
@@ -2455,6 +2570,7 @@ int afcpu_restrict_boxes(afcpu_t *h, int n_ids, const int *ids) {
 int afcpu_prolong_boxes(afcpu_t *h, int n_ids, const int *ids) {
   int rc = ready(h, -1);
   if (rc) return rc;
+  if (h->c.gen_sp) return AFGPU_ERR_UNSUPPORTED;
   if (check_ids(h, n_ids, ids, false)) {
     set_error("afcpu_prolong_boxes: invalid ids");
     return AFGPU_ERR_ARG;
@@ -2701,12 +2817,47 @@ int afcpu_chem_source(afcpu_chem *c, int n_cells, const double *fields, const do
 
 }  // extern "C"
 
-extern "C" int afcpu_set_chemistry(afcpu_t *h, const afcpu_chem *chem) {
+extern "C" int afcpu_set_chemistry(afcpu_t *h, const afcpu_chem *chem, const int *charges) {
   CHECK_H(h);
-  if (chem && chem->n_species != 3) {
-    set_error("afcpu_set_chemistry: the pools hold three species");
-    return AFGPU_ERR_UNSUPPORTED;
+  Ctx &c = h->c;
+  const int n = chem ? chem->n_species : 3;
+  if (chem && (n < 3 || n > 16 || (n > 3 && !charges) || (charges && charges[0] != -1))) {
+    set_error("afcpu_set_chemistry: 3 to 16 species, the electrons first, charges for more than three");
+    return AFGPU_ERR_ARG;
   }
-  h->c.net = chem;
+  std::vector<int> q{-1, 1, -1};
+  if (chem && charges) q.assign(charges, charges + n);
+  const bool gen = chem && (n > 3 || q != std::vector<int>{-1, 1, -1});
+  if (gen && c.p.photoi_enabled && (c.p.photoi_species < 0 || c.p.photoi_species >= n)) return AFGPU_ERR_ARG;
+  const size_t want = gen ? (size_t)(n - 3) * 3 : 0;
+  if (c.xs.size() != want) c.xs.assign(want, std::vector<double>((size_t)c.n_boxes * c.ncell, 0.0));
+  c.net = chem;
+  c.n_sp = gen ? n : 3;
+  c.gen_sp = gen;
+  c.sp_charge = gen ? q : std::vector<int>{-1, 1, -1};
   return AFGPU_OK;
+}
+
+static int species_xfer(afcpu_t *h, int k, int state, int n_ids, const int *ids, double *data, bool put) {
+  CHECK_H(h);
+  Ctx &c = h->c;
+  if (k < 0 || k >= c.n_sp || state < 0 || state > 2 || n_ids < 0 || (n_ids > 0 && (!ids || !data)))
+    return AFGPU_ERR_ARG;
+  for (int q = 0; q < n_ids; q++)
+    if (ids[q] < 1 || ids[q] > c.n_boxes) return AFGPU_ERR_ARG;
+  for (int q = 0; q < n_ids; q++) {
+    double *u = c.SP(k, state, ids[q]);
+    double *b = data + (size_t)q * c.ncell;
+    if (put)
+      std::copy(b, b + c.ncell, u);
+    else
+      std::copy(u, u + c.ncell, b);
+  }
+  return AFGPU_OK;
+}
+extern "C" int afcpu_get_species(afcpu_t *h, int k, int state, int n_ids, const int *ids, double *data) {
+  return species_xfer(h, k, state, n_ids, ids, data, false);
+}
+extern "C" int afcpu_put_species(afcpu_t *h, int k, int state, int n_ids, const int *ids, const double *data) {
+  return species_xfer(h, k, state, n_ids, ids, const_cast<double *>(data), true);
 }

@@ -72,6 +72,15 @@ struct Ctx {
   bool have_params = false;
   Table td, chem;
   const ::afcpu_chem *net = nullptr;  // a reaction list in place of the standard model
+  // species: 0-2 in the pools of the electrons and ions, further ones of a
+  // reaction list in xs[(k - 3) * 3 + state] (gen_sp: the general update)
+  int n_sp = 3;
+  bool gen_sp = false;
+  std::vector<int> sp_charge{-1, 1, -1};
+  std::vector<std::vector<double>> xs;
+  double *SP(int k, int state, int id) {
+    return k < 3 ? C(3 * k + state, id) : &xs[(size_t)(k - 3) * 3 + state][(size_t)(id - 1) * ncell];
+  }
   double voltage = 0;
   Operator ops[1 + AFGPU_MAX_MODES];
   // time-loop state

@@ -47,7 +47,7 @@ def rich_list(A, N):
     return out
 
 
-def chem_for(S, CH, case, kind, reactions):
+def chem_for(S, CH, case, kind, reactions, n_species=3):
     td = case.tdata
     n, x0 = td["n"], td["chem_x_min"]
     # the last grid point as the tables' set-up forms it (the field range of
@@ -59,7 +59,7 @@ def chem_for(S, CH, case, kind, reactions):
     x_last = 0.0 + xl * (x_max_td - 0.0)
     assert (n - 1) / (x_last - x0) == td["chem_inv_fac"]
     cols = np.asarray(td["chem"]).reshape(2, n)
-    return CH.Chem(kind, 3, reactions, x0, x_last, cols, None, 300.0)
+    return CH.Chem(kind, n_species, reactions, x0, x_last, cols, None, 300.0)
 
 
 def cases(S):
@@ -69,7 +69,7 @@ def cases(S):
     }
 
 
-def run(A, S, case, kind, chem, steps=3, seed_neg=False):
+def run(A, S, case, kind, chem, steps=3, seed_neg=False, charges=None, species=None):
     if kind == "gpu":
         p = A.Path("gpu", case.D, case.nc, case.coord, A.PREC_FP64)
     else:
@@ -79,7 +79,9 @@ def run(A, S, case, kind, chem, steps=3, seed_neg=False):
         p.put_cc(A.CC_NEG_ION, 0.1 * np.asarray(case.ne0))
         p.put_cc(A.CC_POS_ION, 1.1 * np.asarray(case.ne0))
     if chem is not None:
-        p.set_chemistry(chem)
+        p.set_chemistry(chem, charges)
+    for k, dens in (species or {}).items():
+        p.put_species(k, dens)
     p.field_compute(0, have_guess=False)
     infos = [p.step() for _ in range(steps)]
     return p, infos
@@ -114,3 +116,51 @@ def test_reaction_list_steps_equal_oracle(A, S, CH, name):
     std, _ = run(A, S, case, "oracle", None, seed_neg=True)
     for iv in (A.CC_ELECTRON, A.CC_POS_ION, A.CC_NEG_ION):
         assert not np.array_equal(std.get_cc(iv), o.get_cc(iv)), (name, iv)
+
+
+@pytest.mark.parametrize("name", ["cyl2d", "xyz3d_8"])
+def test_general_species_path_reproduces_standard_steps(A, S, CH, name):
+    """The standard model as a list over four species, the fourth neutral
+    and inert, takes the general path (pools beyond the three, the rhs over
+    the charged species, k_update_n) and still gives the standard steps bit
+    for bit, on the device and in the oracle."""
+    case = cases(S)[name]
+    for kind in ("gpu", "oracle"):
+        ref, ri = run(A, S, case, kind, None)
+        chem = chem_for(S, CH, case, kind, standard_list(A), n_species=4)
+        p, pi = run(A, S, case, kind, chem, charges=[-1, 1, -1, 0])
+        assert pi == ri, (name, kind)
+        for iv in range(A.N_CC):
+            assert np.array_equal(p.get_cc(iv), ref.get_cc(iv)), (name, kind, iv)
+        assert not p.get_species(3).any()
+
+
+@pytest.mark.parametrize("name", ["cyl2d", "xyz3d_8"])
+def test_air_network_steps_equal_oracle(A, S, CH, name):
+    """The reference's air network (8 species, 25 reactions, read from
+    tests/golden/air_chemistry_v1.txt) in the time step: device against
+    oracle after three steps, every species to 1e-11 of its largest
+    magnitude (exp-type rates differ in the last bit)."""
+    import os
+    from conftest import ROOT
+    case = cases(S)[name]
+    N = case.params.gas_number_density
+    E = S.analytic_transport()[0]
+    x_max = E[-1] * 1e21 / N
+    net = CH.read_network(os.path.join(ROOT, "tests", "golden", "air_chemistry_v1.txt"), {"N2": 0.8, "O2": 0.2}, N,
+                          0.0, x_max, 1000)
+    ns = len(net.species)
+    ne0 = np.asarray(case.ne0)
+    seed = {1: 0.8 * ne0, 2: 0.2 * ne0}  # N2+ and O2+ neutralise the electrons
+    out = {}
+    for kind in ("gpu", "oracle"):
+        chem = CH.Chem(kind, ns, net.reactions, 0.0, x_max, net.fld_cols, None, 300.0)
+        p, infos = run(A, S, case, kind, chem, charges=net.charges, species=seed)
+        out[kind] = (p, infos, chem)
+    (g, gi, _), (o, oi, _) = out["gpu"], out["oracle"]
+    for a, b in zip(gi, oi):
+        assert a["n_rejected"] == b["n_rejected"] and a["n_vcycles"] == b["n_vcycles"]
+    for k in range(ns):
+        x, y = g.get_species(k), o.get_species(k)
+        assert np.max(np.abs(x - y)) <= 1e-11 * max(np.max(np.abs(y)), 1e-300), (name, k, net.names[k])
+    assert o.get_species(ns - 1).any()  # O4+ formed
