@@ -33,6 +33,11 @@ FORWARD_EULER, HEUNS_METHOD = 1, 2
 CC_ELECTRON, CC_POS_ION, CC_NEG_ION = 0, 3, 6
 CC_PHI, CC_PHI_2, CC_EFIELD, CC_RHS, CC_TMP, CC_PHOTO, CC_HELMH = 9, 10, 11, 12, 13, 14, 15
 N_CC = 18
+
+
+def VAR_SPECIES(k):
+    """Species k of a reaction list in variable lists (AFGPU_VAR_SPECIES)."""
+    return 100 + k
 FC_FLUX, FC_FIELD = 0, 1
 OP_POISSON, OP_HELMH = 0, 1
 MAX_MODES = 3

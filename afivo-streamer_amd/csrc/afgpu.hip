@@ -3107,7 +3107,13 @@ static int check_vars(const afgpu_t *h, int n_vars, const int *ivs, VarPtrs &vp,
     return AFGPU_ERR_ARG;
   }
   vp.n = n_vars;
+  const SpeciesArgs S = h->species_args();
   for (int v = 0; v < n_vars; v++) {
+    const int k = ivs[v] - AFGPU_VAR_SPECIES(0);
+    if (k >= 0 && k < S.n) {
+      vp.v[v] = S.sp[k][0];
+      continue;
+    }
     if (ivs[v] < 0 || ivs[v] >= AFGPU_N_CC) {
       set_err(std::string(who) + ": variable index out of range");
       return AFGPU_ERR_ARG;

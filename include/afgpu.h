@@ -293,6 +293,10 @@ int afgpu_prolong_boxes(afgpu_t *h, int n_ids, const int *ids);
  *   (out_cnt, time, dt and the values) is appended, time and dt being the
  *   handle's time state. */
 #define AFGPU_DIAG_MAX_VARS 8
+/* In the variable lists of afgpu_regression_values, afgpu_write_regression_log
+ * and afgpu_get_boxes / afgpu_put_boxes: species k of a reaction list
+ * (state 0), including the species beyond the three standard pools. */
+#define AFGPU_VAR_SPECIES(k) (100 + (k))
 int afgpu_regression_values(afgpu_t *h, int n_vars, const int *ivs, double *out);
 int afgpu_format_regression_line(int out_cnt, int n_vals, const double *vals, char *buf, int buf_len);
 int afgpu_write_regression_log(afgpu_t *h, const char *filename, int out_cnt, int n_vars, const int *ivs,

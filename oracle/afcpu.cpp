@@ -2589,9 +2589,15 @@ int afcpu_regression_values(afcpu_t *h, int n_vars, const int *ivs, double *out)
   int rc = ready(h, -1);
   if (rc) return rc;
   if (n_vars < 1 || n_vars > AFGPU_DIAG_MAX_VARS || !ivs || !out) return AFGPU_ERR_ARG;
-  for (int v = 0; v < n_vars; v++)
-    if (ivs[v] < 0 || ivs[v] >= AFGPU_N_CC) return AFGPU_ERR_ARG;
   Ctx &c = h->c;
+  for (int v = 0; v < n_vars; v++) {
+    const int k = ivs[v] - AFGPU_VAR_SPECIES(0);
+    if (!(k >= 0 && k < c.n_sp) && (ivs[v] < 0 || ivs[v] >= AFGPU_N_CC)) return AFGPU_ERR_ARG;
+  }
+  auto var = [&](int iv, int id) {
+    const int k = iv - AFGPU_VAR_SPECIES(0);
+    return (k >= 0 && k < c.n_sp) ? c.SP(k, 0, id) : c.C(iv, id);
+  };
   const int D = c.D, nc = c.nc;
   const int kmin = (D == 3) ? 1 : 0, kmax = (D == 3) ? nc : 0;
   const double pi = std::acos(-1.0), two_pi = 2 * std::acos(-1.0);
@@ -2618,7 +2624,7 @@ int afcpu_regression_values(afcpu_t *h, int n_vars, const int *ivs, double *out)
         double fac = 1;
         for (int d = 0; d < D; d++) fac = fac * bl.dr[d];
         for (int id : c.leaves[lvl]) {
-          const double *u = c.C(iv, id);
+          const double *u = var(iv, id);
           const Box &bx = c.B(id);
           double tmp = 0;
           for (int k = kmin; k <= kmax; k++)
@@ -2640,7 +2646,7 @@ int afcpu_regression_values(afcpu_t *h, int n_vars, const int *ivs, double *out)
     double mx = -std::numeric_limits<double>::max() / 10;
     for (int lvl = 1; lvl <= c.highest_lvl; lvl++)
       for (int id : c.leaves[lvl]) {
-        const double *u = c.C(iv, id);
+        const double *u = var(iv, id);
         double bm = -std::numeric_limits<double>::max();
         for (int k = kmin; k <= kmax; k++)
           for (int j = 1; j <= nc; j++)

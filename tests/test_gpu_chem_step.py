@@ -164,3 +164,36 @@ def test_air_network_steps_equal_oracle(A, S, CH, name):
         x, y = g.get_species(k), o.get_species(k)
         assert np.max(np.abs(x - y)) <= 1e-11 * max(np.max(np.abs(y)), 1e-300), (name, k, net.names[k])
     assert o.get_species(ns - 1).any()  # O4+ formed
+
+
+def test_air_network_regression_log(A, S, CH, tmp_path):
+    """The regression log of a run with the air network: its header is the
+    reference's own test_cyl_chem_rtest.log header (8 species), and its
+    statistics equal the oracle's (maxima exactly, sums to 1e-12 plus the
+    run's 1e-11 state difference)."""
+    import json
+    import os
+    from conftest import ROOT
+    case = cases(S)["cyl2d"]
+    N = case.params.gas_number_density
+    E = S.analytic_transport()[0]
+    x_max = E[-1] * 1e21 / N
+    net = CH.read_network(os.path.join(ROOT, "tests", "golden", "air_chemistry_v1.txt"), {"N2": 0.8, "O2": 0.2}, N,
+                          0.0, x_max, 1000)
+    ns = len(net.species)
+    ne0 = np.asarray(case.ne0)
+    ivs = [A.VAR_SPECIES(k) for k in range(ns)]
+    vals, keep = {}, []
+    fn = str(tmp_path / "chem_rtest.log")
+    for kind in ("gpu", "oracle"):
+        chem = CH.Chem(kind, ns, net.reactions, 0.0, x_max, net.fld_cols, None, 300.0)
+        keep.append(chem)
+        p, _ = run(A, S, case, kind, chem, charges=net.charges, species={1: 0.8 * ne0, 2: 0.2 * ne0})
+        vals[kind] = np.concatenate([p.regression_values(ivs[:4]), p.regression_values(ivs[4:])], axis=1)
+        if kind == "gpu":
+            p.write_regression_log(fn, 0, ivs[:8], net.names)
+    ref = json.load(open(os.path.join(ROOT, "tests", "golden", "regression_logs.json")))
+    assert open(fn).read().splitlines()[0] == ref["programs/standard_2d/tests/test_cyl_chem_rtest.log"]["header"]
+    g, o = vals["gpu"], vals["oracle"]
+    assert np.allclose(g[..., :2], o[..., :2], rtol=1e-10, atol=0)
+    assert np.allclose(g[..., 2], o[..., 2], rtol=1e-11, atol=0)
