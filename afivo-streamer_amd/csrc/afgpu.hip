@@ -116,6 +116,14 @@ struct afgpu_chem {
 
 struct afgpu_ctx;
 
+// A variable of the refinement's data transfers: its pool, the index whose
+// methods it has, and whether it prolongs with a limiter.
+struct VarRef {
+  double *p;
+  int methods;
+  bool limited;
+};
+
 struct ImplBase {
   virtual ~ImplBase() {}
   virtual int gc_lvl(int lvl, int iv, bool corners) = 0;
@@ -131,9 +139,9 @@ struct ImplBase {
   virtual int refine_cells(const afgpu_refine_t &rp, const int *ids, int n_ids, int *flags) = 0;
   // boxes removed or added by refinement: restriction from the listed
   // children, prolongation into the listed new boxes, their ghost cells
-  virtual int restrict_ids(const int *child_ids, int n, int iv) = 0;
-  virtual int prolong_new(const int *ids, int n, const int *ivs, const bool *limited, int n_vars) = 0;
-  virtual int gc_ids(const int *ids, int n, const int *ivs, int n_vars) = 0;
+  virtual int restrict_ids(const int *child_ids, int n, double *pool) = 0;
+  virtual int prolong_new(const int *ids, int n, const VarRef *vars, int n_vars) = 0;
+  virtual int gc_ids(const int *ids, int n, const VarRef *vars, int n_vars) = 0;
   virtual int restrict_tree(int iv) = 0;
   virtual int step_runs(afgpu_step_info_t *info) = 0;
   // per-leaf-box terms of the regression-log statistics (k_diag_box)
@@ -1804,15 +1812,15 @@ struct Impl : ImplBase {
     return AFGPU_OK;
   }
 
-  int restrict_ids(const int *child_ids, int n, int iv) override {
+  int restrict_ids(const int *child_ids, int n, double *pool) override {
     if (n <= 0) return AFGPU_OK;
     hipLaunchKernelGGL((k_restrict<D, NC, BS, double, 1>), dim3(std::min(n, 8 * c.n_cu)), dim3(BS), 0, c.cur,
-                       child_ids, (const DBox *)c.d_boxes, (const double *)c.cc[iv], c.cc[iv], 1,
+                       child_ids, (const DBox *)c.d_boxes, (const double *)pool, pool, 1,
                        (int)(c.coord == AFGPU_CYL), 1, n);
     HIPCHK(hipGetLastError());
     return AFGPU_OK;
   }
-  int prolong_new(const int *ids, int n, const int *ivs, const bool *limited, int n_vars) override {
+  int prolong_new(const int *ids, int n, const VarRef *vars, int n_vars) override {
     if (n <= 0 || n_vars <= 0) return AFGPU_OK;
     if (n_vars > NEW_MAX_VARS) {
       set_err("prolongation of new boxes: too many variables");
@@ -1821,8 +1829,8 @@ struct Impl : ImplBase {
     NewVars nv;
     nv.n = n_vars;
     for (int k = 0; k < n_vars; k++) {
-      nv.v[k] = c.cc[ivs[k]];
-      nv.limiter[k] = limited[k] ? var_methods(c, ivs[k]).limiter : -1;
+      nv.v[k] = vars[k].p;
+      nv.limiter[k] = vars[k].limited ? var_methods(c, vars[k].methods).limiter : -1;
     }
     const int nw = n * n_vars;
     hipLaunchKernelGGL((k_prolong_new<D, NC, BS>), dim3(std::min(nw, 8 * c.n_cu)), dim3(BS), 0, c.cur, ids,
@@ -1838,7 +1846,7 @@ struct Impl : ImplBase {
     HIPCHK(hipGetLastError());
     return AFGPU_OK;
   }
-  int gc_ids(const int *ids, int n, const int *ivs, int n_vars) override {
+  int gc_ids(const int *ids, int n, const VarRef *vars, int n_vars) override {
     if (n <= 0 || n_vars <= 0) return AFGPU_OK;
     if (n_vars > GC_MAX_VARS) {
       set_err("ghost cells of new boxes: too many variables");
@@ -1847,8 +1855,8 @@ struct Impl : ImplBase {
     GcVars gv;
     gv.n = n_vars;
     for (int k = 0; k < n_vars; k++) {
-      gv.v[k] = c.cc[ivs[k]];
-      gv.m[k] = var_methods(c, ivs[k]);
+      gv.v[k] = vars[k].p;
+      gv.m[k] = var_methods(c, vars[k].methods);
     }
     const int nw = n * n_vars;
     hipLaunchKernelGGL((k_gc_lds_vars<D, NC, BS>), dim3(std::min(nw, 8 * c.n_cu)), dim3(BS), 0, c.cur, ids,
@@ -2986,13 +2994,15 @@ int afgpu_refine_flags(afgpu_t *h, const afgpu_refine_t *rp, int n_boxes, int *r
 // 103-108), and whether they are prolonged with af_prolong_limit (the
 // densities and rhs: the streamer's default prolongation,
 // src/m_streamer.f90:402-417) or af_prolong_linear.
-static std::vector<std::pair<int, bool>> auto_vars(const afgpu_ctx &c) {
-  std::vector<std::pair<int, bool>> v;
-  if (c.p.photoi_enabled) v.push_back({AFGPU_CC_PHOTO, false});
-  v.push_back({AFGPU_CC_EFIELD, false});
-  v.push_back({AFGPU_CC_PHI, false});
-  for (int k = 0; k < 3; k++) v.push_back({AFGPU_CC_SPECIES(k), true});
-  v.push_back({AFGPU_CC_RHS, true});
+// (every species has the densities' methods, src/m_streamer.f90:402-417)
+static std::vector<VarRef> auto_vars(const afgpu_ctx &c) {
+  std::vector<VarRef> v;
+  if (c.p.photoi_enabled) v.push_back({c.cc[AFGPU_CC_PHOTO], AFGPU_CC_PHOTO, false});
+  v.push_back({c.cc[AFGPU_CC_EFIELD], AFGPU_CC_EFIELD, false});
+  v.push_back({c.cc[AFGPU_CC_PHI], AFGPU_CC_PHI, false});
+  const SpeciesArgs S = c.species_args();
+  for (int k = 0; k < S.n; k++) v.push_back({S.sp[k][0], AFGPU_CC_SPECIES(k < 3 ? k : 0), true});
+  v.push_back({c.cc[AFGPU_CC_RHS], AFGPU_CC_RHS, true});
   return v;
 }
 static int ensure_ref_buffers(afgpu_t *h, int n) {
@@ -3011,10 +3021,7 @@ static int ensure_ref_buffers(afgpu_t *h, int n) {
 // before their children are removed
 int afgpu_restrict_boxes(afgpu_t *h, int n_ids, const int *ids) {
   RCHK(ready(h));
-  if (h->gen_sp) {
-    set_err("afgpu_restrict_boxes: not with a reaction list beyond the three standard species");
-    return AFGPU_ERR_UNSUPPORTED;
-  }
+
   const int n_boxes = (int)h->hboxes.size(), nch = 1 << h->D;
   if (n_ids < 0 || (n_ids > 0 && !ids)) return AFGPU_ERR_ARG;
   std::vector<int> ch;
@@ -3030,7 +3037,7 @@ int afgpu_restrict_boxes(afgpu_t *h, int n_ids, const int *ids) {
   RCHK(ensure_ref_buffers(h, std::max((int)ch.size(), n_boxes)));
   h->cur = h->stream;
   HIPCHK(hipMemcpyAsync(h->d_ref_ids, ch.data(), ch.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
-  for (const auto &v : auto_vars(*h)) RCHK(h->impl->restrict_ids(h->d_ref_ids, (int)ch.size(), v.first));
+  for (const auto &v : auto_vars(*h)) RCHK(h->impl->restrict_ids(h->d_ref_ids, (int)ch.size(), v.p));
   return sync_stream(h);
 }
 
@@ -3039,10 +3046,7 @@ int afgpu_restrict_boxes(afgpu_t *h, int n_ids, const int *ids) {
 // then the new boxes' ghost cells (af_gc_box with corners)
 int afgpu_prolong_boxes(afgpu_t *h, int n_ids, const int *ids) {
   RCHK(ready(h));
-  if (h->gen_sp) {
-    set_err("afgpu_prolong_boxes: not with a reaction list beyond the three standard species");
-    return AFGPU_ERR_UNSUPPORTED;
-  }
+
   const int n_boxes = (int)h->hboxes.size();
   if (n_ids < 0 || (n_ids > 0 && !ids)) return AFGPU_ERR_ARG;
   std::vector<int> sorted;
@@ -3067,15 +3071,11 @@ int afgpu_prolong_boxes(afgpu_t *h, int n_ids, const int *ids) {
     while (b < sorted.size() && h->hboxes[sorted[b] - 1].lvl == h->hboxes[sorted[a] - 1].lvl) b++;
     const int *d = h->d_ref_ids + a;
     const int n = (int)(b - a);
-    {
-      int ivs[NEW_MAX_VARS];
-      bool lim[NEW_MAX_VARS];
-      for (size_t k = 0; k < vars.size(); k++) {
-        ivs[k] = vars[k].first;
-        lim[k] = vars[k].second;
-      }
-      RCHK(h->impl->prolong_new(d, n, ivs, lim, (int)vars.size()));
-      RCHK(h->impl->gc_ids(d, n, ivs, (int)vars.size()));
+    // (in groups of at most NEW_MAX_VARS variables per launch)
+    for (size_t k0 = 0; k0 < vars.size(); k0 += NEW_MAX_VARS) {
+      const int nv = (int)std::min<size_t>(NEW_MAX_VARS, vars.size() - k0);
+      RCHK(h->impl->prolong_new(d, n, vars.data() + k0, nv));
+      RCHK(h->impl->gc_ids(d, n, vars.data() + k0, nv));
     }
     a = b;
   }

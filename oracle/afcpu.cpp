@@ -2263,12 +2263,28 @@ static void prolong_linear(Ctx &c, int id_c, int iv) {
 }
 
 // auto_restrict (afivo/src/m_af_core.f90:825-840): every child into box id
+// Species of a reaction list beyond the three pools: the routines address
+// pools by index, so each such species is handled in the electron pool's
+// slot (same methods), its storage swapped in and out around the call.
+template <typename F>
+static void for_extra_species(Ctx &c, F f) {
+  for (int k = 3; k < c.n_sp; k++) {
+    std::vector<double> &x = c.xs[(size_t)(k - 3) * 3];
+    std::swap(c.cc[AFGPU_CC_ELECTRON], x);
+    f(AFGPU_CC_ELECTRON);
+    std::swap(c.cc[AFGPU_CC_ELECTRON], x);
+  }
+}
+
 static void restrict_boxes(Ctx &c, const std::vector<int> &ids) {
   const std::vector<int> vars = auto_vars(c);
   const int nch = 1 << c.D;
-  for (int id : ids)
-    for (int q = 0; q < nch; q++)
-      for (int iv : vars) restrict_box(c, c.B(id).children[q], iv, iv, true);
+  auto one = [&](int iv) {
+    for (int id : ids)
+      for (int q = 0; q < nch; q++) restrict_box(c, c.B(id).children[q], iv, iv, true);
+  };
+  for (int iv : vars) one(iv);
+  for_extra_species(c, one);
 }
 
 // auto_prolong (afivo/src/m_af_core.f90:843-881): the new boxes level by
@@ -2288,6 +2304,10 @@ static void prolong_boxes(Ctx &c, const std::vector<int> &ids) {
       }
     for (int id : l)
       for (int iv : vars) gc_box(c, id, iv, true);
+    for_extra_species(c, [&](int iv) {
+      for (int id : l) prolong_limit(c, id, iv);
+      for (int id : l) gc_box(c, id, iv, true);
+    });
   }
 }
 
@@ -2559,7 +2579,6 @@ static int check_ids(afcpu_t *h, int n, const int *ids, bool parents) {
 int afcpu_restrict_boxes(afcpu_t *h, int n_ids, const int *ids) {
   int rc = ready(h, -1);
   if (rc) return rc;
-  if (h->c.gen_sp) return AFGPU_ERR_UNSUPPORTED;
   if (check_ids(h, n_ids, ids, true)) {
     set_error("afcpu_restrict_boxes: invalid ids");
     return AFGPU_ERR_ARG;
@@ -2570,7 +2589,6 @@ int afcpu_restrict_boxes(afcpu_t *h, int n_ids, const int *ids) {
 int afcpu_prolong_boxes(afcpu_t *h, int n_ids, const int *ids) {
   int rc = ready(h, -1);
   if (rc) return rc;
-  if (h->c.gen_sp) return AFGPU_ERR_UNSUPPORTED;
   if (check_ids(h, n_ids, ids, false)) {
     set_error("afcpu_prolong_boxes: invalid ids");
     return AFGPU_ERR_ARG;

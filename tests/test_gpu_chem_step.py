@@ -197,3 +197,63 @@ def test_air_network_regression_log(A, S, CH, tmp_path):
     g, o = vals["gpu"], vals["oracle"]
     assert np.allclose(g[..., :2], o[..., :2], rtol=1e-10, atol=0)
     assert np.allclose(g[..., 2], o[..., 2], rtol=1e-11, atol=0)
+
+
+@pytest.mark.parametrize("name", ["cyl2d", "xyz3d_8"])
+def test_air_network_new_boxes_equal_oracle(A, S, CH, name):
+    """Restriction into coarsened boxes and prolongation into new boxes
+    (auto_restrict / auto_prolong) cover every species of the list: from
+    the same state, every species equals the oracle's bit for bit; a step on
+    the new tree then matches to 1e-11."""
+    import os
+    from conftest import ROOT
+    case = cases(S)[name]
+    N = case.params.gas_number_density
+    E = S.analytic_transport()[0]
+    x_max = E[-1] * 1e21 / N
+    net = CH.read_network(os.path.join(ROOT, "tests", "golden", "air_chemistry_v1.txt"), {"N2": 0.8, "O2": 0.2}, N,
+                          0.0, x_max, 1000)
+    ns = len(net.species)
+    t0 = case.extra["export"]
+    n0 = t0["n_boxes"]
+    rng = np.random.default_rng(1)
+    dens = {k: 10.0 ** rng.uniform(12, 16, (n0, (case.nc + 2) ** case.D)) for k in range(1, ns)}
+    paths, keep = {}, []
+    for kind in ("gpu", "oracle"):
+        chem = CH.Chem(kind, ns, net.reactions, 0.0, x_max, net.fld_cols, None, 300.0)
+        keep.append(chem)
+        if kind == "gpu":
+            p = A.Path("gpu", case.D, case.nc, case.coord, A.PREC_FP64)
+        else:
+            p = A.Path("oracle", case.D, case.nc, case.coord)
+        S.apply(p, case)
+        p.set_chemistry(chem, net.charges)
+        for k, d in dens.items():
+            p.put_species(k, d)
+        p.field_compute(0, have_guess=False)
+        paths[kind] = p
+    g, o = paths["gpu"], paths["oracle"]
+    nch = 2 ** case.D
+    ch = t0["children"].reshape(n0, nch)
+    derefine = [b + 1 for b in range(n0) if ch[b, 0] > 0 and (ch[ch[b] - 1, 0] == 0).all()]
+    leaves = [b + 1 for b in range(n0) if ch[b, 0] == 0
+              and all(v != 0 for v in t0["neighbors"][b * 2 * case.D:(b + 1) * 2 * case.D])]
+    pick = sorted(rng.choice(leaves, size=max(1, len(leaves) // 4), replace=False))
+    for p in (g, o):
+        p.restrict_boxes(derefine)
+    for k in range(ns):
+        assert np.array_equal(g.get_species(k), o.get_species(k)), (name, "restrict", k)
+    case.tree.refine(np.asarray(pick, np.int32))
+    t1 = case.tree.export()
+    new = np.arange(n0 + 1, t1["n_boxes"] + 1, dtype=np.int32)
+    for p in (g, o):
+        p.sync_tree(t1)
+        p.prolong_boxes(new)
+    for k in range(ns):
+        assert np.array_equal(g.get_species(k), o.get_species(k)), (name, "prolong", k)
+    for p in (g, o):
+        p.field_compute(0, have_guess=True)
+        p.step()
+    for k in range(ns):
+        x, y = g.get_species(k), o.get_species(k)
+        assert np.max(np.abs(x - y)) <= 1e-11 * np.max(np.abs(y)), (name, "step", k)
