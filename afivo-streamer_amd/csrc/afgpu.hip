@@ -1635,7 +1635,11 @@ struct Impl : ImplBase {
     if (c.gen_sp) {
       const List &l = c.l_all_leaves;
       if (l.n > 0) {
-        const size_t lds = 2 * (size_t)c.n_sp * BS * sizeof(double);
+        const size_t lds = 2 * (size_t)c.n_sp * BS * sizeof(double) + (size_t)c.net->n_reactions * sizeof(afgpu_reaction_t);
+        if (lds > 64 * 1024) {
+          set_err("reaction list too large for the update's LDS (species and reaction records)");
+          return AFGPU_ERR_UNSUPPORTED;
+        }
         hipLaunchKernelGGL((k_update_n<D, NC, BS>), dim3(std::min(l.n, 8 * c.n_cu)), dim3(BS), lds, c.cur, c.list(l),
                            (const DBox *)c.d_boxes, A, c.species_args(), c.d_red + 3, l.n);
         HIPCHK(hipGetLastError());

@@ -600,7 +600,8 @@ __global__ void __launch_bounds__(BS) k_set_rhs_n(const int *__restrict__ ids, S
 }
 
 // k_update for a reaction list over S.n species: the densities and
-// derivatives of the workgroup's cells in LDS ([S.n][BS] each); the
+// derivatives of the workgroup's cells in LDS ([S.n][BS] each, then the
+// reaction records); the
 // electrons (species 0) also take the flux divergence.  One workgroup per
 // leaf box.
 template <int D, int NC, int BS>
@@ -609,8 +610,16 @@ __global__ void __launch_bounds__(BS) k_update_n(const int *__restrict__ ids, co
   using G = Geo<D, NC>;
   extern __shared__ double lds[];
   double *sdn = lds, *sdd = lds + (size_t)S.n * BS;
+  afgpu_reaction_t *sr = (afgpu_reaction_t *)(lds + 2 * (size_t)S.n * BS);  // the records, copied once
   __shared__ double sh[BS / 64];
   const int t = threadIdx.x;
+  {
+    constexpr int W = sizeof(afgpu_reaction_t) / sizeof(int);
+    const int *src = (const int *)A.net.r;
+    int *dst = (int *)sr;
+    for (int q = t; q < A.net.n_reactions * W; q += BS) dst[q] = src[q];
+    __syncthreads();
+  }
   for (int w = blockIdx.x; w < nw; w += gridDim.x) {
     const int id = ids[w];
     if (id <= 0) continue;
@@ -646,7 +655,7 @@ __global__ void __launch_bounds__(BS) k_update_n(const int *__restrict__ ids, co
         te = CHEM_EV_TO_K * lt_at(a.energy, 0, ie, we);
       }
       for (int r = 0; r < a.n_reactions; r++) {
-        const afgpu_reaction_t &R = a.r[r];
+        const afgpu_reaction_t &R = sr[r];
         double kr = chem_rate(R, a, field, te, ix, wl);
         double prod = 1.0;
         for (int qq = 0; qq < R.n_in; qq++) prod = prod * sdn[R.ix_in[qq] * BS + t];
