@@ -1917,12 +1917,21 @@ struct Impl : ImplBase {
   int field_solve_dev(int K) {
     const int op = AFGPU_OP_POISSON, R = c.n_runs, top = std::numeric_limits<int>::max();
     const dim3 rg((R + 63) / 64), rb(64);
-    RCHK(with_bb(KC_BOXOP, [&](auto B) {
-      constexpr int bb = decltype(B)::value;
-      return items<bb>(KC_BOXOP, k_set_rhs<G::NCELL, BS, bb>, c.l_all_leaves.n, c.list(c.l_all_leaves),
-                       (const double *)c.cc[AFGPU_CC_ELECTRON], (const double *)c.cc[AFGPU_CC_POS_ION],
-                       (const double *)c.cc[AFGPU_CC_NEG_ION], c.cc[AFGPU_CC_RHS]);
-    }));
+    if (c.gen_sp) {
+      const List &l = c.l_all_leaves;
+      if (l.n > 0) {
+        hipLaunchKernelGGL((k_set_rhs_n<G::NCELL, BS>), dim3(std::min(l.n, 8 * c.n_cu)), dim3(BS), 0, c.cur,
+                           c.list(l), c.species_args(), 0, c.cc[AFGPU_CC_RHS], l.n);
+        HIPCHK(hipGetLastError());
+      }
+    } else {
+      RCHK(with_bb(KC_BOXOP, [&](auto B) {
+        constexpr int bb = decltype(B)::value;
+        return items<bb>(KC_BOXOP, k_set_rhs<G::NCELL, BS, bb>, c.l_all_leaves.n, c.list(c.l_all_leaves),
+                         (const double *)c.cc[AFGPU_CC_ELECTRON], (const double *)c.cc[AFGPU_CC_POS_ION],
+                         (const double *)c.cc[AFGPU_CC_NEG_ION], c.cc[AFGPU_CC_RHS]);
+      }));
+    }
     RCHK(set_red(0, 0ull));
     const int k = bpb(c.l_all_leaves.n, G::NINT);
     RCHK(with_bb(KC_BOXOP, [&](auto B) {
@@ -3810,10 +3819,6 @@ int afgpu_set_chemistry(afgpu_t *h, const afgpu_chem_t *chem, const int *charges
   std::vector<int> q{-1, 1, -1};
   if (chem && charges) q.assign(charges, charges + n);
   const bool gen = chem && (n > 3 || q != std::vector<int>{-1, 1, -1});
-  if (gen && h->n_runs > 1) {
-    set_err("afgpu_set_chemistry: lists beyond the three standard species in a forest of runs");
-    return AFGPU_ERR_UNSUPPORTED;
-  }
   if (gen && h->p.photoi_enabled && (h->p.photoi_species < 0 || h->p.photoi_species >= n)) {
     set_err("afgpu_set_chemistry: photoionization species out of range");
     return AFGPU_ERR_ARG;

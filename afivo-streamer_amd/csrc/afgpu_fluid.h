@@ -626,7 +626,7 @@ __global__ void __launch_bounds__(BS) k_update_n(const int *__restrict__ ids, co
     const DBox &bx = boxes[id - 1];
     const size_t base = (size_t)(id - 1) * G::NCELL;
     const size_t fbase = (size_t)(id - 1) * D * G::NFACE;
-    const double dt = A.dt;
+    const double dt = A.dt_run ? A.dt_scale * A.dt_run[bx.run] : A.dt;
     double dt_dr[3];
     for (int d = 0; d < D; d++) dt_dr[d] = dt / bx.dr[d];
     const double eps = 1e-100;

@@ -513,9 +513,9 @@ int afgpu_chem_source(afgpu_chem_t *c, int n_cells, const double *fields, const 
  * species k's charge (NULL with three species: -1, +1, -1).  The rhs sums
  * the charged species in species order (field_set_rhs); photoi_species
  * indexes the list; afgpu_restrict_boxes / afgpu_prolong_boxes cover every
- * species (the densities' methods).  NULL returns to the standard model.
- * With more than the three species the handle must hold one run.  The
- * chemistry handle must stay alive while the step uses it. */
+ * species (the densities' methods); in a forest of runs every run takes
+ * the list.  NULL returns to the standard model.  The chemistry handle must
+ * stay alive while the step uses it. */
 int afgpu_set_chemistry(afgpu_t *h, const afgpu_chem_t *chem, const int *charges);
 /* A species' cells in a state (box by box, as afgpu_get_cc / _put_cc). */
 int afgpu_get_species(afgpu_t *h, int k, int state, int n_ids, const int *ids, double *data);

@@ -257,3 +257,46 @@ def test_air_network_new_boxes_equal_oracle(A, S, CH, name):
     for k in range(ns):
         x, y = g.get_species(k), o.get_species(k)
         assert np.max(np.abs(x - y)) <= 1e-11 * np.max(np.abs(y)), (name, "step", k)
+
+
+def test_air_network_forest_equals_single_runs(A, S, CH):
+    """A forest of runs with the air network: every run's steps and all
+    eight species equal the same run alone, bit for bit."""
+    import os
+    from conftest import ROOT
+    from test_gpu_ensemble import STEP_KEYS
+    ens = S.ensemble(level_list=[4, 3, 5], num_vcycles=20, rel_residual=1e-10)
+    case0 = ens.cases[0]
+    N = case0.params.gas_number_density
+    E = S.analytic_transport()[0]
+    x_max = E[-1] * 1e21 / N
+    net = CH.read_network(os.path.join(ROOT, "tests", "golden", "air_chemistry_v1.txt"), {"N2": 0.8, "O2": 0.2}, N,
+                          0.0, x_max, 1000)
+    ns = len(net.species)
+    chem = CH.Chem("gpu", ns, net.reactions, 0.0, x_max, net.fld_cols, None, 300.0)
+    g = A.Path("gpu", 2, 8, A.CYL, A.PREC_FP64)
+    S.apply_ensemble(g, ens)
+    g.set_chemistry(chem, net.charges)
+    ne = g.get_cc(A.CC_ELECTRON)
+    g.put_species(1, 0.8 * ne)
+    g.put_species(2, 0.2 * ne)
+    singles = []
+    for case in ens.cases:
+        s = A.Path("gpu", 2, 8, A.CYL, A.PREC_FP64)
+        S.apply(s, case)
+        s.set_chemistry(chem, net.charges)
+        ne = s.get_cc(A.CC_ELECTRON)
+        s.put_species(1, 0.8 * ne)
+        s.put_species(2, 0.2 * ne)
+        singles.append(s)
+    g.field_compute_runs(0, have_guess=False)
+    for s in singles:
+        s.field_compute(0, have_guess=False)
+    for it in range(3):
+        infos = g.step_ensemble()
+        for r, s in enumerate(singles):
+            ref = s.step()
+            assert {k: infos[r][k] for k in STEP_KEYS} == {k: ref[k] for k in STEP_KEYS}, (it, r)
+            ids = ens.run_ids(r)
+            for k in range(ns):
+                assert np.array_equal(g.get_species(k, ids=ids), s.get_species(k)), (it, r, net.names[k])
