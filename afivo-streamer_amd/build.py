@@ -23,7 +23,8 @@ ARCH = os.environ.get("AFGPU_ARCH", "gfx950")
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=" + ARCH,
     "-ffp-contract=off",
-    # no packed FP32 instructions (see the build notes in DESIGN.md)
+    # no packed FP32 instructions: some of their forms have given wrong
+    # results on this hardware now and then, so the compiler must not emit them
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
 ]

@@ -531,8 +531,8 @@ def main():
     if args.copy_probe > 0:
         return copy_probe(args.copy_probe)
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box in this setup has a single MI355X "
-              "(the multi-GPU sharding of the path is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: the path runs on a single MI355X here "
+              "(sharding across GPUs is a design, not built)", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "ensemble":
