@@ -2782,6 +2782,14 @@ int afcpu_chem_create(int n_species, int n_reactions, const afgpu_reaction_t *re
       delete c;
       return AFGPU_ERR_UNSUPPORTED;
     }
+    bool ok = R.n_in >= 1 && R.n_in <= AFGPU_CHEM_MAX_IN && R.n_out >= 0 && R.n_out <= AFGPU_CHEM_MAX_OUT &&
+              (t != AFGPU_RATE_TABULATED_FIELD || (R.lookup_col >= 0 && R.lookup_col < n_fld_cols));
+    for (int q = 0; ok && q < R.n_in; q++) ok = R.ix_in[q] >= 0 && R.ix_in[q] < n_species;
+    for (int q = 0; ok && q < R.n_out; q++) ok = R.ix_out[q] >= 0 && R.ix_out[q] < n_species;
+    if (!ok) {
+      delete c;
+      return AFGPU_ERR_ARG;
+    }
   }
   if (c->need_te && !energy_col) {
     delete c;

@@ -193,3 +193,20 @@ def test_oracle_standard_model_as_a_list_equals_standard_steps(A, S, CH, name):
     assert pi == ri
     for iv in range(A.N_CC):
         assert np.array_equal(p.get_cc(iv), ref.get_cc(iv)), iv
+
+
+@pytest.mark.parametrize("kind", ["gpu", "oracle"])
+def test_invalid_networks_are_refused(A, CH, kind):
+    """Rate type 7 (absent from the reference), the electron-energy table
+    (needs the energy equation) and a species index out of range are
+    refused before anything is allocated (no GPU use)."""
+    def net(t, ix_in=0):
+        r = A.ReactionT()
+        r.rate_type, r.rate_factor, r.n_in = t, 1.0, 1
+        r.ix_in[0] = ix_in
+        r.rate_data[0] = 1.0
+        return [r]
+    cols = np.zeros((0, 16))
+    for bad in (net(7), net(A.RATE_TABULATED_ENERGY), net(A.RATE_CONSTANT, ix_in=3)):
+        with pytest.raises(A.AfError):
+            CH.Chem(kind, 3, bad, 0.0, 1000.0, cols, None, 300.0)
