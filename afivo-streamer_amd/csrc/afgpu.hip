@@ -3532,6 +3532,8 @@ int afgpu_autotune(afgpu_t *h, int n_classes, afgpu_shape_t *shapes) {
   const size_t cells = (size_t)h->cap_boxes * h->ncell, faces = (size_t)h->cap_boxes * h->nface;
   for (int i = 0; i < AFGPU_N_CC && rc == AFGPU_OK; i++) rc = snap_add(sn, h->cc[i], cells * sizeof(double));
   for (int i = 0; i < AFGPU_N_FC && rc == AFGPU_OK; i++) rc = snap_add(sn, h->fc[i], faces * sizeof(double));
+  // the species of a reaction list beyond the three standard pools
+  for (size_t i = 0; i < h->xsp.size() && rc == AFGPU_OK; i++) rc = snap_add(sn, h->xsp[i], cells * sizeof(double));
   for (int i = 0; i < 4 && rc == AFGPU_OK; i++) rc = snap_add(sn, h->cc32[i], cells * sizeof(float));
   for (auto &r : h->mres)
     for (int i = 0; i < 4 && rc == AFGPU_OK; i++)

@@ -300,3 +300,44 @@ def test_air_network_forest_equals_single_runs(A, S, CH):
             ids = ens.run_ids(r)
             for k in range(ns):
                 assert np.array_equal(g.get_species(k, ids=ids), s.get_species(k)), (it, r, net.names[k])
+
+
+@pytest.mark.parametrize("name", ["cyl2d", "xyz3d_8"])
+def test_autotune_with_air_network_leaves_every_species(A, S, CH, name):
+    """Tuning the work decomposition with a reaction list set runs trial
+    steps from a snapshot: every species pool (those beyond the three
+    standard pools included) is restored, so the following steps equal an
+    untuned handle's bit for bit, and the tuned steps equal the oracle's to
+    1e-11 (exp-type rates differ in the last bit)."""
+    import os
+    from conftest import ROOT
+    case = cases(S)[name]
+    N = case.params.gas_number_density
+    E = S.analytic_transport()[0]
+    x_max = E[-1] * 1e21 / N
+    net = CH.read_network(os.path.join(ROOT, "tests", "golden", "air_chemistry_v1.txt"), {"N2": 0.8, "O2": 0.2}, N,
+                          0.0, x_max, 1000)
+    ns = len(net.species)
+    ne0 = np.asarray(case.ne0)
+    seed = {1: 0.8 * ne0, 2: 0.2 * ne0}
+    paths, keep = {}, []
+    for kind in ("tuned", "plain", "oracle"):
+        chem = CH.Chem("oracle" if kind == "oracle" else "gpu", ns, net.reactions, 0.0, x_max, net.fld_cols, None,
+                       300.0)
+        keep.append(chem)
+        p, _ = run(A, S, case, "oracle" if kind == "oracle" else "gpu", chem, steps=1, charges=net.charges,
+                   species=seed)
+        paths[kind] = p
+    t, u, o = paths["tuned"], paths["plain"], paths["oracle"]
+    before = [t.get_species(k) for k in range(ns)]
+    t.autotune()
+    for k in range(ns):
+        assert np.array_equal(t.get_species(k), before[k]), (name, "restored", net.names[k])
+    for it in range(2):
+        a, b, c = t.step(), u.step(), o.step()
+        assert a == b, (name, it)
+        assert a["n_vcycles"] == c["n_vcycles"] and a["n_rejected"] == c["n_rejected"], (name, it)
+    for k in range(ns):
+        x, y, z = t.get_species(k), u.get_species(k), o.get_species(k)
+        assert np.array_equal(x, y), (name, net.names[k])
+        assert np.max(np.abs(x - z)) <= 1e-11 * max(np.max(np.abs(z)), 1e-300), (name, net.names[k])
