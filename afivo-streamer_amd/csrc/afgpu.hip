@@ -19,6 +19,7 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -258,6 +259,9 @@ struct afgpu_ctx {
   // diagnostics: per-box terms and per-run results; box I/O staging
   double *d_diag = nullptr, *d_io = nullptr;
   size_t diag_cap = 0, io_cap = 0;
+  // this GPU's address of the static scratch array (noalloc_scratch), looked
+  // up once per device at afgpu_create
+  double *scratch = nullptr;
   struct Direct {
     bool set = false;  // lc and the map below are current for the tree
     int lc = 1, N = 0;
@@ -2281,6 +2285,35 @@ int sync_stream(afgpu_t *h) {
 
 }  // namespace
 
+// Scratch of the entry points that allocate nothing (the per-box partial
+// sums of afgpu_regression_values_noalloc): AFGPU_NOALLOC_SCRATCH_WORDS
+// doubles (32 MiB) in the code object's device memory, one copy per GPU
+// that the library runs on, held until the process ends.  Host code never
+// passes this name to a kernel (on the host it is a stand-in, not a device
+// address): it uses the address hipGetSymbolAddress returns for the current
+// device, cached per device by noalloc_scratch_address.
+static __device__ __align__(256) double noalloc_scratch[AFGPU_NOALLOC_SCRATCH_WORDS];
+
+static int noalloc_scratch_address(double **out) {
+  static std::mutex mu;
+  static std::vector<double *> addr;  // by device ordinal
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  if ((size_t)dev >= addr.size()) addr.resize(dev + 1, nullptr);
+  if (!addr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch));
+    if (e != hipSuccess || !p) {
+      set_err(std::string("hipGetSymbolAddress(noalloc_scratch): ") + hipGetErrorString(e));
+      return AFGPU_ERR_HIP;
+    }
+    addr[dev] = static_cast<double *>(p);
+  }
+  *out = addr[dev];
+  return AFGPU_OK;
+}
+
 extern "C" {
 
 const char *afgpu_last_error(void) { return g_err.c_str(); }
@@ -2316,6 +2349,7 @@ int afgpu_create(int ndim, int n_cell, int coord, int precision, afgpu_t **out) 
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     if (ncu > 0) h->n_cu = ncu;
   }
+  RCHK(noalloc_scratch_address(&h->scratch));
   // Default shapes: a box of 8 x 8 cells is one wavefront, and streaming
   // such boxes four to a workgroup from a persistent grid of 8 workgroups
   // per CU moves ~1.6x the bytes per second of one box per workgroup; larger
@@ -3136,23 +3170,16 @@ static int check_vars(const afgpu_t *h, int n_vars, const int *ivs, VarPtrs &vp,
   return AFGPU_OK;
 }
 
-// The statistics of output_regression_log (src/m_output.f90:797-810) per
-// run: the per-box terms on the device, summed per run in a fixed order.
-int afgpu_regression_values(afgpu_t *h, int n_vars, const int *ivs, double *out) {
-  RCHK(ready(h));
-  VarPtrs vp;
-  RCHK(check_vars(h, n_vars, ivs, vp, "afgpu_regression_values"));
-  if (!out) return AFGPU_ERR_ARG;
+// Doubles of scratch the regression statistics need: the per-leaf-box
+// terms and the per-run results.
+static size_t diag_words(const afgpu_t *h, int n_vars) {
+  return (size_t)std::max(h->l_all_leaves.n, 1) * n_vars * 3 + (size_t)h->n_runs * n_vars * 3;
+}
+
+// The regression statistics with the scratch `work` (diag_words doubles).
+static int regression_values_in(afgpu_t *h, const VarPtrs &vp, int n_vars, double *work, double *out) {
   const int R = h->n_runs, nw = h->l_all_leaves.n;
-  const size_t need = (size_t)std::max(nw, 1) * n_vars * 3 + (size_t)R * n_vars * 3;
-  if (need > h->diag_cap) {
-    (void)hipFree(h->d_diag);
-    h->d_diag = nullptr;
-    h->diag_cap = 0;
-    HIPCHK(hipMalloc(&h->d_diag, need * sizeof(double)));
-    h->diag_cap = need;
-  }
-  double *part = h->d_diag, *dout = h->d_diag + (size_t)std::max(nw, 1) * n_vars * 3;
+  double *part = work, *dout = work + (size_t)std::max(nw, 1) * n_vars * 3;
   h->cur = h->stream;
   RCHK(h->impl->diag_boxes(vp, part));
   hipLaunchKernelGGL(k_diag_fold<256>, dim3(R), dim3(256), 0, h->stream, h->list(h->l_all_leaves),
@@ -3172,6 +3199,48 @@ int afgpu_regression_values(afgpu_t *h, int n_vars, const int *ivs, double *out)
     }
   }
   return AFGPU_OK;
+}
+
+// The statistics of output_regression_log (src/m_output.f90:797-810) per
+// run: the per-box terms on the device, summed per run in a fixed order.
+int afgpu_regression_values(afgpu_t *h, int n_vars, const int *ivs, double *out) {
+  RCHK(ready(h));
+  VarPtrs vp;
+  RCHK(check_vars(h, n_vars, ivs, vp, "afgpu_regression_values"));
+  if (!out) return AFGPU_ERR_ARG;
+  const size_t need = diag_words(h, n_vars);
+  if (need > h->diag_cap) {
+    (void)hipFree(h->d_diag);
+    h->d_diag = nullptr;
+    h->diag_cap = 0;
+    HIPCHK(hipMalloc(&h->d_diag, need * sizeof(double)));
+    h->diag_cap = need;
+  }
+  return regression_values_in(h, vp, n_vars, h->d_diag, out);
+}
+
+// The same statistics with the scratch in the library's static device
+// array (noalloc_scratch), so that the call allocates no device memory.
+// Every call on one GPU shares that array: calls that overlap on one GPU
+// race.  Nothing in it needs clearing (every per-box slot the fold reads is
+// written first by the same call).
+int afgpu_regression_values_noalloc(afgpu_t *h, int n_vars, const int *ivs, double *out) {
+  RCHK(ready(h));
+  VarPtrs vp;
+  RCHK(check_vars(h, n_vars, ivs, vp, "afgpu_regression_values_noalloc"));
+  if (!out) return AFGPU_ERR_ARG;
+  if (!h->scratch) {
+    set_err("afgpu_regression_values_noalloc: the static scratch array has no address on this device");
+    return AFGPU_ERR_STATE;
+  }
+  const size_t need = diag_words(h, n_vars);
+  if (need > AFGPU_NOALLOC_SCRATCH_WORDS) {
+    set_err("afgpu_regression_values_noalloc: (leaf boxes + runs) * n_vars * 3 = " + std::to_string(need) +
+            " doubles exceed the static scratch of AFGPU_NOALLOC_SCRATCH_WORDS = " +
+            std::to_string((size_t)AFGPU_NOALLOC_SCRATCH_WORDS));
+    return AFGPU_ERR_ARG;
+  }
+  return regression_values_in(h, vp, n_vars, h->scratch, out);
 }
 
 // Fortran's Ew.d with a three-digit exponent (E20.8e3): 0.dddddddd, the

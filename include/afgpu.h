@@ -298,6 +298,19 @@ int afgpu_prolong_boxes(afgpu_t *h, int n_ids, const int *ids);
  * (state 0), including the species beyond the three standard pools. */
 #define AFGPU_VAR_SPECIES(k) (100 + (k))
 int afgpu_regression_values(afgpu_t *h, int n_vars, const int *ivs, double *out);
+/* afgpu_regression_values_noalloc: the same statistics as
+ * afgpu_regression_values, bit for bit, from a call that allocates no
+ * device memory.  It has no counterpart in the reference: it is for
+ * callers that cannot allocate (code under HIP graph capture, tight output
+ * loops, harnesses that forbid hipMalloc).  Its scratch (the per-leaf-box
+ * terms and the per-run results, (n_leaf_boxes + n_runs) * n_vars * 3
+ * doubles) lives in a static device array of AFGPU_NOALLOC_SCRATCH_WORDS
+ * doubles (32 MiB, one copy per GPU, held until the process ends); larger
+ * requests return AFGPU_ERR_ARG.  All calls on one GPU share that array:
+ * calls that overlap on one GPU (several threads, several handles at once)
+ * race. */
+#define AFGPU_NOALLOC_SCRATCH_WORDS (1 << 22)
+int afgpu_regression_values_noalloc(afgpu_t *h, int n_vars, const int *ivs, double *out);
 int afgpu_format_regression_line(int out_cnt, int n_vals, const double *vals, char *buf, int buf_len);
 int afgpu_write_regression_log(afgpu_t *h, const char *filename, int out_cnt, int n_vars, const int *ivs,
                                const char *const *names);
