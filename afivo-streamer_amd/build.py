@@ -3,6 +3,8 @@
 * libafgpu.so  -- HIP for gfx950 (hipcc), the product.
 * libafhost.so -- host-side tree / initial-condition / table stand-in (g++).
 * oracle/liboracle_afcpu.so -- the CPU oracle (test infrastructure).
+* oracle/_ref/ -- the reference's own routines and the fixture drivers
+  (test infrastructure), where the reference checkout is present.
 
 All floating-point code is built with -ffp-contract=off so that no fused
 multiply-adds are formed: the FP64 device path then evaluates every cell
@@ -59,13 +61,30 @@ def build_oracle():
     return os.path.join(REPO_DIR, "oracle", "liboracle_afcpu.so")
 
 
+REFERENCE = os.environ.get("AFGPU_REFERENCE", "/root/reference")
+FLANG = "/opt/rocm/bin/amdflang"
+
+
+def build_ref():
+    """The reference's own afivo and lookup-table routines and the fixture
+    drivers that call them (oracle/ref/Makefile, outputs in oracle/_ref),
+    where the reference checkout and amdflang are present; the tests that
+    use them are skipped elsewhere."""
+    if not (os.path.isdir(REFERENCE) and os.path.exists(FLANG)):
+        print("reference checkout or amdflang not present: oracle/_ref not built", flush=True)
+        return None
+    _run(["make", "-C", os.path.join(REPO_DIR, "oracle", "ref"), "REF=" + REFERENCE, "all"])
+    return os.path.join(REPO_DIR, "oracle", "_ref")
+
+
 def build_all():
     build_host()
     build_oracle()
+    build_ref()
     build_gpu()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "gpu": build_gpu, "host": build_host, "oracle": build_oracle}[w]()
+        {"all": build_all, "gpu": build_gpu, "host": build_host, "oracle": build_oracle, "ref": build_ref}[w]()

@@ -7,7 +7,14 @@
 // floating-point operation order (compiled with -ffp-contract=off like
 // gfortran -O2 on x86-64, which does not contract).  The only deliberate
 // departure is the level-1 solve: hypre PFMG (tolerance 1e-6) is replaced by
-// an exact solve whose factors are supplied by afcpu_set_coarse_factors.
+// an exact solve whose factors are supplied by afcpu_set_coarse_factors
+// (parity unpinned against hypre).
+//
+// Pins: the afivo and lookup-table routines restated here are checked bit
+// for bit against the reference's own routines, built from its sources
+// (oracle/ref, tests/test_oracle_ref.py, through the afcpu_test_* entry
+// points at the end of this file); the rest against closed forms and
+// invariants (tests/test_oracle_pins.py and the other CPU tests).
 #include <omp.h>
 
 #include <algorithm>
@@ -2893,3 +2900,139 @@ extern "C" int afcpu_get_species(afcpu_t *h, int k, int state, int n_ids, const 
 extern "C" int afcpu_put_species(afcpu_t *h, int k, int state, int n_ids, const int *ids, const double *data) {
   return species_xfer(h, k, state, n_ids, ids, const_cast<double *>(data), true);
 }
+
+// ---- single routines, for the pins against the reference's own routines -
+// (the fixture drivers under oracle/ref call the same afivo routines, built
+// from the reference's sources, on the same inputs)
+namespace afcpu {
+static int test_ids(const Ctx &c, int n, const int *ids) {
+  if (n < 0 || (n > 0 && !ids)) return AFGPU_ERR_ARG;
+  for (int q = 0; q < n; q++)
+    if (ids[q] < 1 || ids[q] > c.n_boxes || c.B(ids[q]).lvl < 1) return AFGPU_ERR_ARG;
+  return AFGPU_OK;
+}
+}  // namespace afcpu
+
+extern "C" {
+
+// kind 0: af_limiter_koren, 1: af_limiter_mc, 2: af_limiter_gminmod43
+// (afivo/src/m_af_limiters.f90:72-151)
+int afcpu_test_limiter(int kind, int n, const double *a, const double *b, double *out) {
+  if (kind < 0 || kind > 2 || n < 0 || (n > 0 && (!a || !b || !out))) return AFGPU_ERR_ARG;
+  for (int q = 0; q < n; q++)
+    out[q] = kind == 0 ? limiter_koren(a[q], b[q]) : limiter_apply(a[q], b[q], kind == 1 ? LIM_MC : LIM_GMINMOD43);
+  return AFGPU_OK;
+}
+
+// LT_get_loc / LT_get_col (src/lookup_table_fortran/m_lookup_table.f90:330-406)
+// on one column of n_rows values; low_ix is the reference's 1-based row
+int afcpu_test_table(int n_rows, double x_min, double inv_fac, const double *col, int n, const double *x,
+                     double *y, int *low_ix, double *low_frac) {
+  if (n_rows < 2 || !col || n < 0 || (n > 0 && (!x || !y))) return AFGPU_ERR_ARG;
+  Table t;
+  t.n = n_rows;
+  t.x_min = x_min;
+  t.inv_fac = inv_fac;
+  t.cols.assign(col, col + n_rows);
+  for (int q = 0; q < n; q++) {
+    int ix;
+    double w;
+    t.loc(x[q], ix, w);
+    y[q] = t.at(0, ix, w);
+    if (low_ix) low_ix[q] = ix;
+    if (low_frac) low_frac[q] = w;
+  }
+  return AFGPU_OK;
+}
+
+// af_stencil_gsrb_box with the multigrid's constant Laplacian stencil
+// (redblack 1 or 2), for each listed box in turn
+int afcpu_test_gsrb_boxes(afcpu_t *h, int n, const int *ids, int redblack, int iv, int i_rhs, double lambda2) {
+  CHECK_H(h);
+  Ctx &c = h->c;
+  if (test_ids(c, n, ids) || iv < 0 || iv >= AFGPU_N_CC || i_rhs < 0 || i_rhs >= AFGPU_N_CC) return AFGPU_ERR_ARG;
+  for (int q = 0; q < n; q++) gsrb_box(c, ids[q], redblack, iv, i_rhs, lambda2);
+  return AFGPU_OK;
+}
+
+// af_stencil_apply_box with the multigrid's constant Laplacian stencil
+int afcpu_test_apply_boxes(afcpu_t *h, int n, const int *ids, int iv, int i_out, double lambda2) {
+  CHECK_H(h);
+  Ctx &c = h->c;
+  if (test_ids(c, n, ids) || iv < 0 || iv >= AFGPU_N_CC || i_out < 0 || i_out >= AFGPU_N_CC) return AFGPU_ERR_ARG;
+  for (int q = 0; q < n; q++) apply_box(c, ids[q], iv, i_out, lambda2);
+  return AFGPU_OK;
+}
+
+// mode 0: stencil_prolong_248 adding iv of the parent to iv_to of the box;
+// 1: af_prolong_limit of iv (the variable's limiter); 2: af_prolong_linear
+// of iv; for each listed box (with a parent) in turn
+int afcpu_test_prolong_boxes(afcpu_t *h, int n, const int *ids, int mode, int iv, int iv_to) {
+  CHECK_H(h);
+  Ctx &c = h->c;
+  if (test_ids(c, n, ids) || mode < 0 || mode > 2 || iv < 0 || iv >= AFGPU_N_CC || iv_to < 0 || iv_to >= AFGPU_N_CC)
+    return AFGPU_ERR_ARG;
+  for (int q = 0; q < n; q++)
+    if (c.B(ids[q]).parent < 1) return AFGPU_ERR_ARG;
+  for (int q = 0; q < n; q++) {
+    const int id = ids[q];
+    if (mode == 0)
+      prolong_add(c, c.B(id).parent, id, iv, iv_to);
+    else if (mode == 1)
+      prolong_limit(c, id, iv);
+    else
+      prolong_linear(c, id, iv);
+  }
+  return AFGPU_OK;
+}
+
+// af_gc2_box of iv for each listed box in turn: its (n_cell+4)^D halo into
+// out (box after box); each call writes its first layer back into the box
+int afcpu_test_gc2_boxes(afcpu_t *h, int n, const int *ids, int iv, double *out) {
+  CHECK_H(h);
+  Ctx &c = h->c;
+  if (test_ids(c, n, ids) || iv < 0 || iv >= AFGPU_N_CC || !methods(c, iv).ok || (n > 0 && !out))
+    return AFGPU_ERR_ARG;
+  const size_t ng4 = (size_t)c.nc + 4, m = c.D == 3 ? ng4 * ng4 * ng4 : ng4 * ng4;
+  std::vector<double> g;
+  for (int q = 0; q < n; q++) {
+    g.assign(m, 0.0);
+    gc2_box(c, ids[q], iv, g);
+    std::copy(g.begin(), g.end(), out + q * m);
+  }
+  return AFGPU_OK;
+}
+
+// af_consistent_fluxes of the electron flux
+int afcpu_test_consistent_fluxes(afcpu_t *h) {
+  CHECK_H(h);
+  consistent_fluxes(h->c);
+  return AFGPU_OK;
+}
+
+// flux_upwind_tree of the electrons (state s_deriv) with dt_lim[0] the CFL
+// limit and dt_lim[1] the dielectric relaxation time
+int afcpu_test_flux_upwind_tree(afcpu_t *h, int s_deriv, double *dt_lim) {
+  int rc = ready(h, -1);
+  if (rc) return rc;
+  if (!dt_lim || s_deriv < 0 || s_deriv > 2 || h->c.td.n < 2) return AFGPU_ERR_ARG;
+  flux_upwind_tree(h->c, s_deriv, dt_lim);
+  return AFGPU_OK;
+}
+
+// The flux divergence of flux_update_densities on the electrons (state 0,
+// in place: one previous state of weight 1, no sources), every leaf
+int afcpu_test_flux_divergence(afcpu_t *h, double dt) {
+  int rc = ready(h, -1);
+  if (rc) return rc;
+  Ctx &c = h->c;
+  for (int lvl = 1; lvl <= c.highest_lvl; lvl++)
+    for (int id : c.leaves[lvl]) {
+      double dt_dr[3];
+      for (int d = 0; d < c.D; d++) dt_dr[d] = dt / c.B(id).dr[d];
+      electron_flux_divergence(c, id, dt_dr, 0);
+    }
+  return AFGPU_OK;
+}
+
+}  // extern "C"
