@@ -167,6 +167,7 @@ class StepInfoT(C.Structure):
         ("rel_residual", C.c_double),
         ("photoi_done", C.c_int),
         ("leaf_cells", C.c_longlong),
+        ("photoi_cycles", C.c_int),
     ]
 
     def as_dict(self):

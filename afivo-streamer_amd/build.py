@@ -5,6 +5,8 @@
 * oracle/liboracle_afcpu.so -- the CPU oracle (test infrastructure).
 * oracle/_ref/ -- the reference's own routines and the fixture drivers
   (test infrastructure), where the reference checkout is present.
+* tests/fortran/_build/drive_step -- a Fortran driver of libafgpu through
+  include/m_afgpu.f90 (test infrastructure), where amdflang is present.
 
 All floating-point code is built with -ffp-contract=off so that no fused
 multiply-adds are formed: the FP64 device path then evaluates every cell
@@ -77,14 +79,33 @@ def build_ref():
     return os.path.join(REPO_DIR, "oracle", "_ref")
 
 
+def build_fortran():
+    """The Fortran driver of the C-ABI (tests/fortran/drive_step.f90 over the
+    interface module include/m_afgpu.f90), linked against libafgpu.so; the
+    GPU test that runs it is skipped where amdflang is absent."""
+    if not os.path.exists(FLANG):
+        print("amdflang not present: the Fortran driver is not built", flush=True)
+        return None
+    out_dir = os.path.join(REPO_DIR, "tests", "fortran", "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "drive_step")
+    _run([FLANG, "-O2", "-module-dir", out_dir, os.path.join(INC, "m_afgpu.f90"),
+          os.path.join(REPO_DIR, "tests", "fortran", "drive_step.f90"), "-L" + PKG_DIR, "-lafgpu",
+          "-Wl,-rpath,$ORIGIN/../../../afivo-streamer_amd", "-o", out + ".tmp"], cwd=out_dir)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all():
     build_host()
     build_oracle()
     build_ref()
     build_gpu()
+    build_fortran()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "gpu": build_gpu, "host": build_host, "oracle": build_oracle, "ref": build_ref}[w]()
+        {"all": build_all, "gpu": build_gpu, "host": build_host, "oracle": build_oracle, "ref": build_ref,
+         "fortran": build_fortran}[w]()

@@ -7,7 +7,8 @@ and to_simple_ascii (src/m_chemistry.f90:1359-1400) for gases of constant
 density: gas species and "M" fold into the rate factor, species are
 numbered in order of first appearance, field tables are interpolated onto
 the uniform lookup grid as LT_get_spaced_data does (linear, clamped).
-Reaction groups ("@" lines) are not supported.
+Reaction groups ("@" lines) are expanded as read_reactions expands them
+(src/m_chemistry.f90:918-975).
 """
 import ctypes as C
 import re
@@ -138,6 +139,80 @@ class Network:
         self.reactions, self.descriptions, self.tables = [], [], []
 
 
+def lt_grid(x_min, x_max, n):
+    """The rows of a linearly spaced lookup table, as LT_create forms them
+    (get_x, src/lookup_table_fortran/m_lookup_table.f90:303-327):
+    x_min + ((i - 1) * (1 / (n - 1))) * (x_max - x_min)."""
+    tmp = 1.0 / (n - 1)
+    return x_min + (np.arange(n) * tmp) * (x_max - x_min)
+
+
+def electrons_first(net):
+    """The device path takes species 0 as the electrons (the reference finds
+    them by name, "e"): move "e" to the front, keeping the other species in
+    their order of first appearance, and renumber the reactions."""
+    if "e" not in net.species or net.species[0] == "e":
+        return
+    order = ["e"] + [sp for sp in net.species if sp != "e"]
+    new_ix = {old: order.index(sp) for old, sp in enumerate(net.species)}
+    for r in net.reactions:
+        for q in range(r.n_in):
+            r.ix_in[q] = new_ix[r.ix_in[q]]
+        for q in range(r.n_out):
+            r.ix_out[q] = new_ix[r.ix_out[q]]
+    net.species = order
+
+
+MAX_GROUPS = 10  # max_groups, src/m_chemistry.f90:881
+
+
+def parse_group(line, groups):
+    """An "@name = a, b, ..." line (src/m_chemistry.f90:920-940): the fields
+    between "=" and "," delimiters, the first the name, the others the
+    members; every group of one reaction has the same number of members."""
+    fields = [f.strip() for f in re.split(r"[=,]", line) if f.strip()]
+    if len(groups) + 1 > MAX_GROUPS:
+        raise ValueError("Too many groups")
+    if groups and len(fields) - 1 != len(groups[0][1]):
+        raise ValueError("Groups for a reaction should have the same size: %s" % line)
+    return fields[0], fields[1:]
+
+
+def string_replace(text, old, new):
+    """Every occurrence of old replaced, scanning again from the start after
+    each replacement (string_replace, src/m_chemistry.f90:1334-1356)."""
+    while True:
+        i = text.find(old)
+        if i < 0:
+            return text
+        text = text[:i] + new + text[i + len(old):]
+
+
+def expand_groups(entries, groups):
+    """The reaction read before the "@" lines becomes one reaction per group
+    member: member k of every group replaces the group's name in the
+    reaction, its rate type and its data, groups in the order given
+    (src/m_chemistry.f90:942-975).  The reference does this when it reads
+    the next reaction line, so groups after the list's last reaction are not
+    applied.  Reactions that come out the same in text and data are
+    refused, as there."""
+    if not entries:
+        raise ValueError("reaction groups before any reaction")
+    template = entries.pop()
+    new = []
+    for k in range(len(groups[0][1])):
+        reaction, how, data, unit = template
+        for name, members in groups:
+            reaction = string_replace(reaction, name, members[k])
+            how = string_replace(how, name, members[k])
+            data = string_replace(data, name, members[k])
+        new.append((reaction, how, data, unit))
+    for e in new:
+        if sum(1 for x in new if x[0] == e[0]) > 1 and sum(1 for x in new if x[2] == e[2]) > 1:
+            raise ValueError("Groups lead to duplicate reactions: %s" % e[0])
+    entries.extend(new)
+
+
 def read_network(path, gas_fractions, gas_number_density, table_x_min, table_x_max, table_size):
     """The reaction list of an input file for a gas of constant density
     with the given component fractions (e.g. {"N2": 0.8, "O2": 0.2})."""
@@ -150,17 +225,22 @@ def read_network(path, gas_fractions, gas_number_density, table_x_min, table_x_m
     if not starts:
         raise ValueError("no reaction_list in %s" % path)
     entries = []
+    groups = []  # (name, members) of the "@" lines after a reaction
     for s in _block(lines, starts[0]):
         if s.startswith("#") or s == "":
             continue
         if s.startswith("@"):
-            raise NotImplementedError("reaction groups are not supported")
+            groups.append(parse_group(s, groups))
+            continue
+        if groups:
+            expand_groups(entries, groups)
+            groups = []
         f = [t.strip() for t in s.split(",")]
         if not 3 <= len(f) <= 4:
             raise ValueError("invalid chemistry syntax: %s" % s)
         entries.append((f[0], f[1], f[2], f[3] if len(f) > 3 else "m"))
     net = Network()
-    grid = table_x_min + np.arange(table_size) * ((table_x_max - table_x_min) / (table_size - 1))
+    grid = lt_grid(table_x_min, table_x_max, table_size)
     for text, how, data, unit in entries:
         r = A.ReactionT()
         rfactor, n_species_in, left = 1.0, 0, True
@@ -226,6 +306,7 @@ def read_network(path, gas_fractions, gas_number_density, table_x_min, table_x_m
             r.multiplicity_out[q] = m
         net.reactions.append(r)
         net.descriptions.append(text)
+    electrons_first(net)
     for name in net.species:
         simple, charge = simple_ascii(name)
         net.names.append(simple)

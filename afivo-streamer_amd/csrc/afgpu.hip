@@ -125,6 +125,13 @@ struct VarRef {
   bool limited;
 };
 
+// LDS of the general update (k_update_n): each thread's densities and
+// derivatives of every species, and the reaction records.
+constexpr size_t UPDATE_N_LDS_MAX = 64 * 1024;
+static size_t update_n_lds(int bs, int n_sp, int n_reactions) {
+  return 2 * (size_t)n_sp * bs * sizeof(double) + (size_t)n_reactions * sizeof(afgpu_reaction_t);
+}
+
 struct ImplBase {
   virtual ~ImplBase() {}
   virtual int gc_lvl(int lvl, int iv, bool corners) = 0;
@@ -235,7 +242,7 @@ struct afgpu_ctx {
   bool photoi_inner_vcycle = false;  // photoionization inner cycle: V-cycle instead of FMG
   bool flux32 = true;  // mixed precision: FP32 flux stencil (FP64 accumulation)
   bool fuse_direct = false;  // uniform-level solve: last-axis transforms fused into one pass
-  bool direct_f32 = true;    // uniform-level solve of the FP32 inner cycle in FP32 (matrix cores)
+  bool direct_f32 = false;   // uniform-level solve of the FP32 inner cycle in FP32 (else FP64; matrix cores)
   // refinement flags: the boxes evaluated and their flags (device)
   // a reaction list in place of the standard model; with more species than
   // the three pools (or other charges), the species beyond the pools live
@@ -262,6 +269,7 @@ struct afgpu_ctx {
   // this GPU's address of the static scratch array (noalloc_scratch), looked
   // up once per device at afgpu_create
   double *scratch = nullptr;
+  int photoi_cycles = 0;  // FMG cycles of the Helmholtz modes in the last photoionization solve
   struct Direct {
     bool set = false;  // lc and the map below are current for the tree
     int lc = 1, N = 0;
@@ -1639,8 +1647,8 @@ struct Impl : ImplBase {
     if (c.gen_sp) {
       const List &l = c.l_all_leaves;
       if (l.n > 0) {
-        const size_t lds = 2 * (size_t)c.n_sp * BS * sizeof(double) + (size_t)c.net->n_reactions * sizeof(afgpu_reaction_t);
-        if (lds > 64 * 1024) {
+        const size_t lds = update_n_lds(BS, c.n_sp, c.net->n_reactions);
+        if (lds > UPDATE_N_LDS_MAX) {
           set_err("reaction list too large for the update's LDS (species and reaction records)");
           return AFGPU_ERR_UNSUPPORTED;
         }
@@ -1758,9 +1766,11 @@ struct Impl : ImplBase {
       }
       RCHK(ensure_mode_pools());
       int active = (1 << n_modes) - 1;
+      c.photoi_cycles = 0;
       RCHK(run_graph(c, {G_MP_RES_MODES, active, lc, 0},
                      [&] { return fork_modes(active, [&](int op) { return mp_close(op, false, lc); }); }));
       for (int i = 1; i <= c.p.photoi_max_fmg_cycles && active; i++) {
+        c.photoi_cycles += __builtin_popcount(active);
         RCHK(run_graph(c, {G_MP_FMG_MODES, active, lc, 0}, [&] {
           return fork_modes(active, [&](int op) {
             if (c.photoi_inner_vcycle)
@@ -1782,9 +1792,11 @@ struct Impl : ImplBase {
       // read on interior cells only (src/m_fluid.f90:437-439).
       return photo_sum();
     }
+    c.photoi_cycles = 0;
     for (int n = 0; n < n_modes; n++) {
       const int op = AFGPU_OP_HELMH + n;
       for (int i = 1; i <= c.p.photoi_max_fmg_cycles; i++) {
+        c.photoi_cycles++;
         RCHK(fmg(op, true, true));
         double residu;
         RCHK(leaf_residual_max(&residu));
@@ -2223,6 +2235,7 @@ struct Impl : ImplBase {
         f.residual = c.run_res[r];
         f.rel_residual = max_rhs[r] > 0 ? c.run_res[r] / max_rhs[r] : 0;
         f.photoi_done = 0;
+        f.photoi_cycles = 0;
         f.leaf_cells = c.run_leaf_cells[r];
       }
     }
@@ -2245,6 +2258,9 @@ struct Impl : ImplBase {
     return AFGPU_OK;
   }
 };
+
+// Workgroup size of the box kernels of a handle's (dimension, box size).
+static int box_threads(int D, int nc) { return (D == 2 && nc == 8) ? 64 : 256; }
 
 int make_impl(afgpu_ctx &c) {
   if (c.D == 2 && c.nc == 8)
@@ -2757,7 +2773,7 @@ int afgpu_get_fc(afgpu_t *h, int iv, int n, const int *ids, double *d) { return 
 int afgpu_set_params(afgpu_t *h, const afgpu_params_t *p) {
   if (!h || !p || p->photoi_n_modes < 0 || p->photoi_n_modes > AFGPU_MAX_MODES ||
       (p->time_integrator != AFGPU_FORWARD_EULER && p->time_integrator != AFGPU_HEUNS_METHOD) ||
-      p->photoi_species < 0 || p->photoi_species > 2 || p->mg_n_cycle_down < 1 || p->mg_n_cycle_up < 1) {
+      p->photoi_species < 0 || p->photoi_species >= SP_MAX || p->mg_n_cycle_down < 1 || p->mg_n_cycle_up < 1) {
     set_err("afgpu_set_params: invalid arguments");
     return AFGPU_ERR_ARG;
   }
@@ -2906,6 +2922,17 @@ int afgpu_field_from_potential(afgpu_t *h) {
   return sync_stream(h);
 }
 
+// The species receiving the photoionization source exists in the model
+// (the three standard species, or those of the reaction list).
+static int check_photoi_species(const afgpu_t *h) {
+  if (h->p.photoi_enabled && h->p.photoi_species >= h->n_sp) {
+    set_err("photoionization species " + std::to_string(h->p.photoi_species) + " is not among the " +
+            std::to_string(h->n_sp) + " species of the model");
+    return AFGPU_ERR_ARG;
+  }
+  return AFGPU_OK;
+}
+
 int afgpu_forward_euler(afgpu_t *h, double dt, int s_deriv, int n_prev, const int *s_prev, const double *w_prev,
                         int s_out, int i_step, int n_steps, double *dt_lim) {
   RCHK(ready(h, true));
@@ -2914,6 +2941,7 @@ int afgpu_forward_euler(afgpu_t *h, double dt, int s_deriv, int n_prev, const in
     return AFGPU_ERR_ARG;
   for (int m = 0; m < n_prev; m++)
     if (s_prev[m] < 0 || s_prev[m] > 2) return AFGPU_ERR_ARG;
+  RCHK(check_photoi_species(h));
   return h->impl->forward_euler(dt, s_deriv, n_prev, s_prev, w_prev, s_out, i_step, n_steps, dt_lim);
 }
 
@@ -2938,6 +2966,7 @@ static int advance(afgpu_t *h, double dt, double *dt_lim) {
 int afgpu_advance(afgpu_t *h, double dt, double *dt_lim) {
   RCHK(ready(h, true));
   if (!dt_lim) return AFGPU_ERR_ARG;
+  RCHK(check_photoi_species(h));
   return advance(h, dt, dt_lim);
 }
 
@@ -3381,6 +3410,7 @@ int afgpu_restore_previous_state(afgpu_t *h) {
 
 int afgpu_step(afgpu_t *h, afgpu_step_info_t *info) {
   RCHK(ready(h, true));
+  RCHK(check_photoi_species(h));
   if (h->n_runs > 1) {
     set_err("afgpu_step: the tree is a forest of runs (use afgpu_step_ensemble)");
     return AFGPU_ERR_STATE;
@@ -3441,6 +3471,7 @@ int afgpu_step(afgpu_t *h, afgpu_step_info_t *info) {
     info->rel_residual = mr > 0 ? res / mr : 0;
     info->photoi_done = photoi_done;
     info->leaf_cells = afgpu_num_leaf_cells(h);
+    info->photoi_cycles = photoi_done ? h->photoi_cycles : 0;
   }
   RCHK(sync_stream(h));
   if (h->global_dt < h->p.dt_min) {
@@ -3893,6 +3924,12 @@ int afgpu_set_chemistry(afgpu_t *h, const afgpu_chem_t *chem, const int *charges
   if (gen && h->p.photoi_enabled && (h->p.photoi_species < 0 || h->p.photoi_species >= n)) {
     set_err("afgpu_set_chemistry: photoionization species out of range");
     return AFGPU_ERR_ARG;
+  }
+  if (gen && update_n_lds(box_threads(h->D, h->nc), n, chem->n_reactions) > UPDATE_N_LDS_MAX) {
+    set_err("afgpu_set_chemistry: " + std::to_string(n) + " species and " + std::to_string(chem->n_reactions) +
+            " reactions need more LDS than the update of boxes of this size has (64 KiB: 2 x species x " +
+            std::to_string(box_threads(h->D, h->nc)) + " doubles plus the reaction records)");
+    return AFGPU_ERR_UNSUPPORTED;
   }
   HIPCHK(hipStreamSynchronize(h->stream));
   drop_graphs(*h);  // (the list and the pools are arguments of captured updates)

@@ -100,7 +100,6 @@ __global__ void __launch_bounds__(BS * BB)
   constexpr int NT = G::NTAN;
   __shared__ TF halo_all[BB][2 * D][2][NT];
   TF(&halo)[2 * D][2][NT] = halo_all[lgrp<BS, BB>()];
-  __shared__ TF cfl_all[BB][G::NINT];  // CFL sum per interior cell
   // the mobility and diffusion columns of the transport table (3D, single
   // precision, up to TABMAX rows), converted as lt_at_t converts them
   constexpr int TABMAX = (D == 3 && sizeof(TF) < sizeof(double)) ? 1024 : 1;
@@ -151,83 +150,73 @@ __global__ void __launch_bounds__(BS * BB)
     TF smax = -std::numeric_limits<TF>::max();
     TF cmax = 0;
     if constexpr (D == 3) {
-      // Face fluxes (flux_upwind, src/m_fluid.f90:102-209) direction by
-      // direction, one grid line per thread: the line's n_e, cell field and
-      // face field are loaded into registers in one batch (so each thread
-      // waits for memory once per line, not twice per face), then its NC + 1
-      // faces are formed in order.  The velocity and diffusion coefficient
-      // of each face also give the CFL terms of the two cells it bounds
-      // (m_fluid.f90:194-195, m_af_flux_schemes.f90:779-848), summed over
-      // the directions in LDS in the order x, y, z.
-      TF *cfl = cfl_all[lgrp<BS, BB>()];
+      // Face fluxes (flux_upwind, src/m_fluid.f90:102-209) cell by cell,
+      // x fastest over the threads, so that the loads of every direction
+      // are coalesced and a cell's n_e and field are read while still in
+      // cache by all three directions.  For each direction a cell forms its
+      // lower and its upper face: an interior face is formed by both of its
+      // cells, from the same values, and written by the cell above it; the
+      // box's last face by the cell below it.  The two faces' velocities and
+      // diffusion coefficients give the cell's CFL term (m_fluid.f90:194-195,
+      // m_af_flux_schemes.f90:779-848), summed over the directions in the
+      // order x, y, z.
       const TF *tab = tab_lds ? tab_all[lgrp<BS, BB>()] : nullptr;
       auto coef = [&](int c, int ix, TF w) {
         return tab_lds ? w * tab[c * TABMAX + ix - 1] + (1 - w) * tab[c * TABMAX + ix] : lt_at_t<TF>(td, c, ix, w);
       };
-      for (int d = 0; d < D; d++) {
-        const TF inv_dx = TF(1) / (TF)bx.dr[d];
-        int t0, t1;
-        tangential(D, d, t0, t1);
-        for (int r = ltid<BS>(); r < NT; r += BS) {
-          int p[3] = {0, 0, 0};
-          p[t0] = r % NC + 1;
-          p[t1] = r / NC + 1;
-          TF nl[NC + 4];  // n_e at q = -1 .. NC + 2 along the line (index q + 1)
-          TF el[NC + 2];  // cell field at q = 0 .. NC + 1
-          TF xl[NC + 1];  // face field at faces 1 .. NC + 1
-          for (int q = 0; q <= NC + 1; q++) {
-            int c[3] = {p[0], p[1], p[2]};
-            c[d] = q;
-            el[q] = (TF)efield[base + G::idx(c[0], c[1], c[2])];
-            if (q >= 1 && q <= NC) nl[q + 1] = (TF)ne[base + G::idx(c[0], c[1], c[2])];
-          }
-          for (int f = 1; f <= NC + 1; f++) {
-            int fp[3] = {p[0], p[1], p[2]};
-            fp[d] = f;
-            xl[f - 1] = (TF)efc[fbase + G::fidx(fp[0], fp[1], fp[2], d)];
-          }
-          nl[0] = halo[2 * d][1][r];
-          nl[1] = halo[2 * d][0][r];
-          nl[NC + 2] = halo[2 * d + 1][0][r];
-          nl[NC + 3] = halo[2 * d + 1][1][r];
-          TF v_lo = 0, dc_lo = 0;  // the face below the current cell
-          for (int f = 1; f <= NC + 1; f++) {
-            const TF Lm2 = nl[f - 1], Lm1 = nl[f], L0 = nl[f + 1], Lp1 = nl[f + 2];
-            const TF ex = xl[f - 1];
-            TF u;
-            if (-1 * ex > 0) {
-              u = Lm1 + TF(0.5) * lim_koren<TF>(L0 - Lm1, Lm1 - Lm2);
-            } else {
-              u = L0 - TF(0.5) * lim_koren<TF>(L0 - Lm1, Lp1 - L0);
-            }
-            const TF tmp_fc = TF(0.5) * (el[f - 1] + el[f]) * (TF)SI_to_Townsend * N_inv;
-            int lix;
-            TF lw;
-            lt_loc_t<TF>(td, tmp_fc, lix, lw);
-            const TF mu = coef(0, lix, lw) * N_inv;
-            const TF dc = coef(1, lix, lw) * N_inv;
-            const TF v = -mu * ex;
-            int fp[3] = {p[0], p[1], p[2]};
-            fp[d] = f;
-            flux[fbase + G::fidx(fp[0], fp[1], fp[2], d)] = (double)(v * u - dc * inv_dx * (L0 - Lm1));
-            smax = fmax(smax, mu * u);
-            if (f >= 2) {  // cell f - 1, between the faces f - 1 and f
-              const TF cl =
-                  TF(1) * fmax(fabs(v), fabs(v_lo)) * inv_dx + TF(2) * fmax(dc, dc_lo) * (inv_dx * inv_dx);
-              int c[3] = {p[0], p[1], p[2]};
-              c[d] = f - 1;
-              const int qc = (c[0] - 1) + NC * ((c[1] - 1) + NC * (c[2] - 1));
-              cfl[qc] = ((d == 0) ? TF(0) : cfl[qc]) + cl;
-            }
-            v_lo = v;
-            dc_lo = dc;
-          }
+      // one face: the upwind value, velocity, diffusion coefficient, flux
+      auto face = [&](TF Lm2, TF Lm1, TF L0, TF Lp1, TF ex, TF e_lo, TF e_hi, TF inv_dx, TF &v, TF &dc) -> TF {
+        TF u;
+        if (-1 * ex > 0) {
+          u = Lm1 + TF(0.5) * lim_koren<TF>(L0 - Lm1, Lm1 - Lm2);
+        } else {
+          u = L0 - TF(0.5) * lim_koren<TF>(L0 - Lm1, Lp1 - L0);
         }
-        gsync<BS, BB>();  // (a cell's next direction is another thread's line)
+        const TF tmp_fc = TF(0.5) * (e_lo + e_hi) * (TF)SI_to_Townsend * N_inv;
+        int lix;
+        TF lw;
+        lt_loc_t<TF>(td, tmp_fc, lix, lw);
+        const TF mu = coef(0, lix, lw) * N_inv;
+        dc = coef(1, lix, lw) * N_inv;
+        v = -mu * ex;
+        smax = fmax(smax, mu * u);
+        return v * u - dc * inv_dx * (L0 - Lm1);
+      };
+      for (int q = ltid<BS>(); q < G::NINT; q += BS) {
+        const int p[3] = {q % NC + 1, (q / NC) % NC + 1, q / (NC * NC) + 1};
+        TF cfl = 0;
+        for (int d = 0; d < D; d++) {
+          const TF inv_dx = TF(1) / (TF)bx.dr[d];
+          int t0, t1;
+          tangential(D, d, t0, t1);
+          const int r = (p[t0] - 1) + NC * (p[t1] - 1);  // the cell's position on a face of direction d
+          constexpr int st[3] = {1, G::NG, G::NG * G::NG};
+          const int cd = p[d];
+          const size_t c0 = base + G::idx(p[0], p[1], p[2]);
+          TF nl[5];  // n_e at cd - 2 .. cd + 2 along d
+          for (int s2 = -2; s2 <= 2; s2++) {
+            const int pos = cd + s2;
+            nl[s2 + 2] = (pos >= 1 && pos <= NC) ? (TF)ne[c0 + s2 * st[d]]
+                         : (pos <= 0)              ? halo[2 * d][-pos][r]
+                                                   : halo[2 * d + 1][pos - NC - 1][r];
+          }
+          const TF e_m = (TF)efield[c0 - st[d]], e_0 = (TF)efield[c0], e_p = (TF)efield[c0 + st[d]];
+          int fp[3] = {p[0], p[1], p[2]};
+          const size_t f_lo = fbase + G::fidx(fp[0], fp[1], fp[2], d);
+          fp[d] = cd + 1;
+          const size_t f_hi = fbase + G::fidx(fp[0], fp[1], fp[2], d);
+          TF v_lo, dc_lo, v_hi, dc_hi;
+          const TF fl_lo = face(nl[0], nl[1], nl[2], nl[3], (TF)efc[f_lo], e_m, e_0, inv_dx, v_lo, dc_lo);
+          const TF fl_hi = face(nl[1], nl[2], nl[3], nl[4], (TF)efc[f_hi], e_0, e_p, inv_dx, v_hi, dc_hi);
+          flux[f_lo] = (double)fl_lo;
+          if (cd == NC) flux[f_hi] = (double)fl_hi;
+          const TF cl = TF(1) * fmax(fabs(v_hi), fabs(v_lo)) * inv_dx + TF(2) * fmax(dc_hi, dc_lo) * (inv_dx * inv_dx);
+          cfl = cfl + cl;
+        }
+        cmax = fmax(cmax, cfl);
       }
       double smax_d = group_max<BS, BB>((double)smax, sh);
       if (ltid<BS>() == 0) atomic_max_nonneg(run_slot(sig_max, bx), fmax(smax_d, 1e-100));
-      for (int q = ltid<BS>(); q < G::NINT; q += BS) cmax = fmax(cmax, cfl[q]);
     } else {
       // 2D boxes have too few lines per direction for a thread each: one
       // face per thread (flux_upwind, src/m_fluid.f90:102-209), then the

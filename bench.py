@@ -346,6 +346,45 @@ def leaf_interior(case, p, iv):
     return a[(slice(None),) + (slice(1, case.nc + 1),) * case.D]
 
 
+# Geometric factor of the coarse levels of a V-cycle (1 + 1/2^D + ...)
+GEO = {2: 4.0 / 3.0, 3: 8.0 / 7.0}
+
+
+def step_bytes(info, D, mixed, photoi):
+    """Algorithmic bytes of one time step, counted as words per leaf cell
+    touched once per kernel pass over interior cells (ghost layers, halos
+    and the rejection backup excluded):
+    - per step: flux 2 + 2D words, update and source 7 + D (+1 reading the
+      photoionization source), rhs 4, gradient and norm 2 + D (27 in 3D);
+    - per V-cycle (outer iteration of the mixed form): 22 words times the
+      coarse levels' factor (4/3 in 2D, 8/7 in 3D): four GSRB sweeps of 3
+      words, residual 3, restriction 2, correction 2, final residual 3;
+    - photoionization: 4 words for the source and the mode sum, and
+      (4/3) 22 words (times the same factor) per FMG cycle of a mode.
+    FP64 words are 8 bytes.  In the mixed form the inner cycle's words
+    (sweeps, restriction, correction: 16 of the 22) are FP32, 4 bytes; its
+    FP64 residual and correction (6 of 22) and the per-step words stay 8."""
+    g = GEO[D]
+    cycle = (16 * 4 + 6 * 8) if mixed else 22 * 8
+    fixed = (2 + 2 * D) + (7 + D + (1 if photoi else 0)) + 4 + (2 + D)
+    b = fixed * 8 + info["n_vcycles"] * g * cycle
+    if photoi and info["photoi_done"]:
+        b += 4 * 8 + info["photoi_cycles"] * (4.0 / 3.0) * g * cycle
+    return b * info["leaf_cells"]
+
+
+def host_cpu():
+    """The host's CPU model (as /proc/cpuinfo names it)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def copy_probe(mib):
     """A plain device-to-device copy of known size, for counter calibration:
     each launch reads and writes exactly `mib` MiB."""
@@ -435,6 +474,7 @@ def bench_ensemble(args):
         src = ref if "fp64_gpu_value" in extra else p
         threads = min(16, os.cpu_count() or 1)
         os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+        os.environ.setdefault("OMP_PROC_BIND", "close")
         csecs, ccells, nthreads = 0.0, 0, 0
         for r, case in enumerate(ens.cases):
             c = A.Path("oracle", 2, 8, A.CYL)
@@ -453,7 +493,10 @@ def bench_ensemble(args):
             c.close()
         cpu = {"value": ccells / csecs, "unit": "cell-steps/s", "cores": nthreads, "kind": "port",
                "sample": "%d step(s) of every run (%d cells) from the GPU FP64 state, %.1f s"
-                         % (args.cpu_steps, int(sum(sizes)), csecs)}
+                         % (args.cpu_steps, int(sum(sizes)), csecs),
+               "cpu_model": host_cpu(), "host_cpus": os.cpu_count(),
+               "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+               "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
         log("  cpu %.4e cell-steps/s (%.1f s)" % (cpu["value"], csecs))
     out = {
         "metric": "cell-steps/s, ensemble of %d uneven 2D runs (mixed at %g)" % (len(ens.cases), args.rel_residual),
@@ -500,7 +543,7 @@ def main():
                     help="mixed precision: block smoother in the inner cycle (1) or red-black sweeps (0)")
     ap.add_argument("--batch-modes", type=int, default=1,
                     help="mixed precision: photoionization modes as parallel graph branches (1) or in turn (0)")
-    ap.add_argument("--direct-f32", type=int, default=1,
+    ap.add_argument("--direct-f32", type=int, default=0,
                     help="mixed precision: uniform-level solve of the FP32 inner cycle in FP32 (1) or FP64 (0)")
     ap.add_argument("--fuse-direct", type=int, default=0,
                     help="mixed precision: uniform-level solve with the fused transform passes (1) or separate (0)")
@@ -518,6 +561,8 @@ def main():
                     help="write the autotuner's shapes (of the run measured first) to FILE")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64-ref", action="store_true")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the mixed form's variants (FP32 coarsest solve, inner cycle to level 1)")
     args = ap.parse_args()
     global FORCED_SHAPES, SAVE_SHAPES
     if args.shapes:
@@ -572,8 +617,37 @@ def main():
                           % args.timing_steps}
         log("  kernel stats", json.dumps(stats))
 
+    photoi = bool(case.params.photoi_enabled)
+    sb = sum(step_bytes(i, case.D, mixed, photoi) for i in infos)
+    step_roof = {"bound": "hbm", "algo_bytes_per_step": sb / len(infos), "achieved": round(sb / secs / 1e9, 1),
+                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(sb / secs / 1e9 / HBM_PEAK_GBS, 4),
+                 "mean_photoi_cycles": float(np.mean([i["photoi_cycles"] for i in infos])),
+                 "rule": "words per leaf cell once per kernel pass: 27 per step (+1 with photoionization), "
+                         "22 x 8/7 per V-cycle, 4 + (4/3) 22 x 8/7 per mode FMG cycle; FP64 words 8 B, "
+                         "the mixed inner cycle's 16 of 22 words 4 B; timed steps' wall time"}
+    log("  whole step: %.3e algorithmic bytes per step, %.1f GB/s (%.3f of peak)"
+        % (step_roof["algo_bytes_per_step"], step_roof["achieved"], step_roof["frac"]))
+
     extra = {}
     ref = None
+    if mixed and not args.no_variants:
+        # variants of the mixed form beside the measured one: the inner
+        # cycle's uniform-level solve in FP32, and the inner cycle descending
+        # to level 1 as the plain V-cycle does (separates the precision from
+        # the inner cycle's structure)
+        for key, kw in (("mixed_f32_coarsest_value", {"direct_f32": 1 - args.direct_f32}),
+                        ("mixed_vcycle_to_level1_value", {"direct_n": 0})):
+            log("GPU mixed variant %s" % key)
+            opts = dict(direct_n=args.direct_n, direct_f32=args.direct_f32)
+            opts.update(kw)
+            q, qinfos, qsecs, _ = run_gpu(case, A.PREC_MIXED, args.warmup, args.steps, 0, opts["direct_n"],
+                                          args.mixed_smooth, args.block_smooth, args.batch_modes,
+                                          args.photoi_vcycle, bool(args.autotune), args.fuse_direct,
+                                          opts["direct_f32"])
+            extra[key] = sum(i["leaf_cells"] for i in qinfos) / qsecs
+            extra[key.replace("_value", "_config")] = opts
+            log("  %s %.4e cell-steps/s" % (key, extra[key]))
+            q.close()
     if prec == A.PREC_MIXED and not args.no_fp64_ref:
         log("GPU fp64 reference run")
         ref, rinfos, rsecs, _ = run_gpu(case, A.PREC_FP64, args.warmup, args.steps, args.timing_steps,
@@ -600,6 +674,7 @@ def main():
         src = ref if ref is not None else p
         threads = min(16, os.cpu_count() or 1)
         os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+        os.environ.setdefault("OMP_PROC_BIND", "close")
         c = A.Path("oracle", case.D, case.nc, case.coord)
         S.apply(c, case)
         for iv in list(range(9)) + [A.CC_PHI, A.CC_EFIELD, A.CC_PHOTO] + [A.CC_HELMH + k for k in range(3)]:
@@ -617,7 +692,10 @@ def main():
         csecs = time.perf_counter() - t0
         cpu = {"value": ccells / csecs, "unit": "cell-steps/s", "cores": nthreads, "kind": "port",
                "sample": "%d step(s) of the same 3D AMR workload (%d leaf cells) from the GPU state "
-                         "after warmup, %.1f s" % (args.cpu_steps, info["leaf_cells"], csecs)}
+                         "after warmup, %.1f s" % (args.cpu_steps, info["leaf_cells"], csecs),
+               "cpu_model": host_cpu(), "host_cpus": os.cpu_count(),
+               "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+               "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
         log("  cpu %.4e cell-steps/s (%.1f s)" % (cpu["value"], csecs))
     extra["refine_flags"] = measure_refine(case, p, None if args.no_cpu else c)
     extra["diagnostics"] = measure_diagnostics(case, p, None if args.no_cpu else c)
@@ -651,6 +729,7 @@ def main():
                    "direct_f32": args.direct_f32 if prec == A.PREC_MIXED else None,
                    "shapes": p.shapes},
         "roofline": roof,
+        "step_roofline": step_roof,
         "cpu_baseline": cpu,
     }
     out.update(extra)

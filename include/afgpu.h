@@ -115,7 +115,10 @@ typedef struct {
   int    species_bc;           /* AFGPU_SPECIES_BC_*                          */
   int    photoi_enabled;
   int    photoi_per_steps;     /* photoi%per_steps                            */
-  int    photoi_species;       /* 0 = e, 1 = M+, 2 = M- (photoi%species)      */
+  int    photoi_species;       /* species index that gains the source too
+                                  (photoi%species): 0 = e, 1 = M+, 2 = M- of the
+                                  standard model, or a reaction list's index;
+                                  checked against the model at step time */
   double photoi_eta_quench;    /* photoi_eta * quench_fac (m_photoi.f90:175-184) */
   int    photoi_n_modes;       /* <= AFGPU_MAX_MODES                          */
   double photoi_lambda[AFGPU_MAX_MODES]; /* 1/m, after pressure scaling       */
@@ -137,6 +140,8 @@ typedef struct {
   double rel_residual;  /* residual / max |rhs|                           */
   int    photoi_done;   /* photoionization recomputed in this step        */
   long long leaf_cells; /* leaf cells advanced (af_num_cells_used)        */
+  int    photoi_cycles; /* FMG cycles of the Helmholtz modes in this step,
+                           summed over the modes (0 without photoionization) */
 } afgpu_step_info_t;
 
 /* ---- lifetime ---------------------------------------------------------
@@ -399,7 +404,8 @@ int afgpu_kernel_stats(afgpu_t *h, int kind, int reset, double *total_ms,
 #define AFGPU_OPT_DIRECT_F32 11 /* mixed precision: the uniform-level solve of the FP32 inner
                                    cycle on the matrix cores (64 or 128 cells per axis) in FP32
                                    (1) or FP64 (0); the FP64 outer correction sets the accuracy
-                                   either way (default 1; 0 whenever AFGPU_OPT_FUSE_DIRECT is 1) */
+                                   either way (default 0: the coarsest-grid solve stays FP64;
+                                   0 whenever AFGPU_OPT_FUSE_DIRECT is 1) */
 int afgpu_set_option(afgpu_t *h, int option, int value);
 
 /* ---- ensembles of independent runs --------------------------------------

@@ -1765,9 +1765,11 @@ static int photoi_set_src(Ctx &c) {
     for (int id : c.ids[lvl]) box_clear(c, id, AFGPU_CC_PHOTO);
   double max_rhs = tree_maxabs(c, AFGPU_CC_RHS);
   max_rhs = std::max(max_rhs, std::sqrt(std::numeric_limits<double>::epsilon()));
+  c.photoi_cycles = 0;
   for (int n = 0; n < c.p.photoi_n_modes; n++) {
     MG mg = mg_of(c, AFGPU_OP_HELMH + n);
     for (int i = 1; i <= c.p.photoi_max_fmg_cycles; i++) {
+      c.photoi_cycles++;
       fmg(c, mg, true, true);
       double residu = tree_maxabs(c, AFGPU_CC_TMP);
       if (residu / max_rhs < c.p.photoi_max_rel_residual) break;
@@ -1868,6 +1870,7 @@ static int step(Ctx &c, afgpu_step_info_t *info) {
     double mr = tree_maxabs(c, AFGPU_CC_RHS);
     info->rel_residual = mr > 0 ? res / mr : 0;
     info->photoi_done = photoi_done;
+    info->photoi_cycles = photoi_done ? c.photoi_cycles : 0;
     long long cells = 0;
     for (int lvl = 1; lvl <= c.highest_lvl; lvl++) cells += (long long)c.leaves[lvl].size();
     for (int d = 0; d < c.D; d++) cells *= c.nc;
@@ -2402,7 +2405,8 @@ int afcpu_get_fc(afcpu_t *h, int iv, int n, const int *ids, double *d) { return 
 
 int afcpu_set_params(afcpu_t *h, const afgpu_params_t *p) {
   CHECK_H(h);
-  if (!p || p->photoi_n_modes < 0 || p->photoi_n_modes > AFGPU_MAX_MODES) {
+  if (!p || p->photoi_n_modes < 0 || p->photoi_n_modes > AFGPU_MAX_MODES || p->photoi_species < 0 ||
+      p->photoi_species >= 16) {
     set_error("afcpu_set_params: invalid arguments");
     return AFGPU_ERR_ARG;
   }
@@ -2463,6 +2467,15 @@ static int ready(afcpu_t *h, int op) {
   if (op >= 0 && !h->c.ops[op].have_factors) {
     set_error("coarse factors not set for operator");
     return AFGPU_ERR_STATE;
+  }
+  return AFGPU_OK;
+}
+
+// The species receiving the photoionization source exists in the model.
+static int check_photoi_species(const Ctx &c) {
+  if (c.p.photoi_enabled && c.p.photoi_species >= c.n_sp) {
+    set_error("photoionization species not among the species of the model");
+    return AFGPU_ERR_ARG;
   }
   return AFGPU_OK;
 }
@@ -2534,6 +2547,7 @@ int afcpu_forward_euler(afcpu_t *h, double dt, int s_deriv, int n_prev, const in
   int rc = ready(h, -1);
   if (rc) return rc;
   if (!dt_lim || n_prev < 1 || !s_prev || !w_prev) return AFGPU_ERR_ARG;
+  if ((rc = check_photoi_species(h->c))) return rc;
   return forward_euler(h->c, dt, s_deriv, n_prev, s_prev, w_prev, s_out, i_step, n_steps, dt_lim);
 }
 
@@ -2541,6 +2555,7 @@ int afcpu_advance(afcpu_t *h, double dt, double *dt_lim) {
   int rc = ready(h, -1);
   if (rc) return rc;
   if (!dt_lim) return AFGPU_ERR_ARG;
+  if ((rc = check_photoi_species(h->c))) return rc;
   return advance(h->c, dt, dt_lim);
 }
 
@@ -2728,6 +2743,7 @@ int afcpu_restore_previous_state(afcpu_t *h) {
 int afcpu_step(afcpu_t *h, afgpu_step_info_t *info) {
   int rc = ready(h, AFGPU_OP_POISSON);
   if (rc) return rc;
+  if ((rc = check_photoi_species(h->c))) return rc;
   return step(h->c, info);
 }
 

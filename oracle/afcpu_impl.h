@@ -87,6 +87,7 @@ struct Ctx {
   int it = 0;
   double time = 0, global_time = 0, global_dt = 0, frac_rejected = 0, dt = 0;
   bool started = false;
+  int photoi_cycles = 0;  // FMG cycles of the Helmholtz modes in the last photoionization solve
 
   double *C(int iv, int id) { return &cc[iv][(size_t)(id - 1) * ncell]; }
   double *F(int ifc, int id) { return &fc[ifc][(size_t)(id - 1) * D * nface]; }

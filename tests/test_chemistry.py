@@ -210,3 +210,54 @@ def test_invalid_networks_are_refused(A, CH, kind):
     for bad in (net(7), net(A.RATE_TABULATED_ENERGY), net(A.RATE_CONSTANT, ix_in=3)):
         with pytest.raises(A.AfError):
             CH.Chem(kind, 3, bad, 0.0, 1000.0, cols, None, 300.0)
+
+
+def test_lookup_grid_equals_reference_lt_create(CH):
+    """The rows of the chemistry's field table are LT_create's (get_x,
+    src/lookup_table_fortran/m_lookup_table.f90:303-327), bit for bit
+    against the rows the reference's own LT_create produced
+    (tests/golden/ref/pointwise.npz, oracle/ref)."""
+    import refvec
+    d = refvec.load("pointwise")
+    x = d["td.x"]
+    g = CH.lt_grid(0.0, float(x[-1]), len(x))
+    assert np.array_equal(g.view(np.uint64), x.view(np.uint64))
+    # (the other grouping of the same product differs in the last bit)
+    alt = 0.0 + np.arange(len(x)) * ((x[-1] - 0.0) / (len(x) - 1))
+    assert not np.array_equal(alt, x)
+
+
+def test_electrons_are_moved_first(CH, tmp_path):
+    """A list whose first species is not the electron (the reference finds
+    the electrons by name): "e" becomes species 0, the others keep their
+    order, and every reaction's species indices follow."""
+    net = single(CH, tmp_path, "O2- + M -> e + O2 + M,c1,1e-18\ne + O2 -> O2-,c1,2e-16", 2)
+    assert net.species == ["e", "O2-"] and net.charges == [-1, -1]
+    r0, r1 = net.reactions
+    assert list(r0.ix_in[:r0.n_in]) == [1] and list(r0.ix_out[:r0.n_out]) == [0]
+    assert list(r1.ix_in[:r1.n_in]) == [0] and list(r1.ix_out[:r1.n_out]) == [1]
+
+
+def test_reaction_groups_expand_as_read_reactions(CH, tmp_path):
+    """"@" groups (src/m_chemistry.f90:918-975): the reaction before the
+    group lines becomes one reaction per member, member k of every group
+    replacing the group's name everywhere in the reaction and its data
+    ("@x" inside "@x+" too), when the next reaction line is read."""
+    text = ("e + @x -> e + e + @x+,c1,@k\n@x = N2, O2\n@k = 1e-16, 2e-16\n"
+            "e + O2 -> O2-,c1,3e-17")
+    net = single(CH, tmp_path, text)
+    assert net.species == ["e", "N2+", "O2+", "O2-"]
+    assert [r.rate_data[0] for r in net.reactions] == [1e-16, 2e-16, 3e-17]
+    assert net.reactions[0].rate_factor == 0.8 * N_GAS and net.reactions[1].rate_factor == 0.2 * N_GAS
+    assert CH.string_replace("@x + @x+ -> @xy", "@x", "O2") == "O2 + O2+ -> O2y"
+    # the same member twice with different data is allowed (98air_2water's H2O)
+    net = single(CH, tmp_path, "e + @x -> e + e + @x+,c1,@k\n@x = N2, N2\n@k = 1e-16, 2e-16\ne -> e,c1,0")
+    assert len(net.reactions) == 3
+    with pytest.raises(ValueError, match="duplicate"):
+        single(CH, tmp_path, "e + @x -> e + e + @x+,c1,1e-16\n@x = N2, N2\ne -> e,c1,0")
+    with pytest.raises(ValueError, match="same size"):
+        single(CH, tmp_path, "e + @x -> e + e + @x+,c1,@k\n@x = N2, O2\n@k = 1e-16\ne -> e,c1,0")
+    # groups after the list's last reaction are not applied (the reference
+    # expands on reading the next reaction line)
+    net = single(CH, tmp_path, "e + O2 -> O2-,c1,3e-17\n@x = N2, O2")
+    assert net.species == ["e", "O2-"] and len(net.reactions) == 1

@@ -1,0 +1,161 @@
+"""The five workloads at their full sizes and lengths on the device, against
+the CPU oracle run beside them on the same seeded inputs.
+
+- 2D axisymmetric, 256^2 in 1024 boxes of 8^2, Poisson to 1e-10 of
+  max |rhs|: 100 FP64 steps, every step's results and the states at every
+  25th step bit for bit.
+- 3D AMR, 2121 boxes of 16^3 (7,602,176 leaf cells), Poisson to 1e-8: 50
+  FP64 steps bit for bit (step results every step, the states every 10th
+  step and at the end); beside it the mixed-precision form (FP32
+  smoothing, restriction and prolongation inside FP64 defect correction,
+  FP32 flux stencil, FP64 coarsest-grid solve and update): its max and L2
+  errors of potential, field and electron density after 50 steps against
+  the oracle.
+- The same tree with photoionization every step (three Helmholtz modes,
+  lambda p_O2 = {0.0553, 0.146, 0.89}/(m Torr) at 150 Torr): FP64 against
+  the oracle bit for bit over 20 steps, photoionization source included;
+  the mixed form with the three modes batched in one graph against the
+  oracle within error bounds.
+- 64 axisymmetric runs of 128^2 to 1024^2 (21,250,048 cells) in one forest
+  in mixed precision: every run equals its single-run handle bit for bit
+  over 3 steps, and four runs (one of 1024^2) are within error bounds of
+  the oracle.
+
+Error bounds (max over leaf interiors of |x - x_oracle| / max |x_oracle|,
+and the L2 ratio) are ten times what the mixed form reaches in the
+measured runs: a lost term or a wrong precision would exceed them."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+BIT_STATE = ("CC_ELECTRON", "CC_POS_ION", "CC_NEG_ION", "CC_PHI", "CC_EFIELD")
+
+
+def leaves_of(ex, D):
+    ch = ex["children"].reshape(ex["n_boxes"], 2 ** D)
+    return (np.nonzero(ch[:, 0] == 0)[0] + 1).astype(np.int32)
+
+
+def interior(p, iv, ids, D, nc):
+    a = p.get_cc(iv, ids=ids).reshape((len(ids),) + (nc + 2,) * D)
+    return a[(slice(None),) + (slice(1, nc + 1),) * D]
+
+
+def errors(p, ref, iv, ids, D, nc):
+    a, b = interior(p, iv, ids, D, nc), interior(ref, iv, ids, D, nc)
+    mx = float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+    l2 = float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+    return mx, l2
+
+
+def bit_equal(A, g, c, what, names=BIT_STATE, ids=None):
+    for name in names:
+        iv = getattr(A, name)
+        a, b = g.get_cc(iv, ids), c.get_cc(iv, ids)
+        assert np.array_equal(a, b), "%s: %s differs (max %.3e)" % (what, name, np.abs(a - b).max())
+
+
+def check_errors(A, p, ref, ids, D, nc, bounds, what):
+    out = {}
+    for name, (bmax, bl2) in bounds.items():
+        mx, l2 = errors(p, ref, getattr(A, name), ids, D, nc)
+        out[name] = (mx, l2)
+        print("%s %s max %.3e l2 %.3e" % (what, name, mx, l2), flush=True)
+    for name, (bmax, bl2) in bounds.items():
+        mx, l2 = out[name]
+        assert mx <= bmax and l2 <= bl2, (what, name, mx, l2, bmax, bl2)
+    return out
+
+
+def test_config0_100_steps_bit_exact(A, S):
+    case = S.config0(uniform_lvl=6, num_vcycles=100, rel_residual=1e-10)
+    g = S.apply(A.Path("gpu", 2, 8, A.CYL), case)
+    c = S.apply(A.Path("oracle", 2, 8, A.CYL), case)
+    assert g.field_compute(0, False) == c.field_compute(0, False)
+    for it in range(100):
+        ig, ic = g.step(), c.step()
+        assert ig == ic, (it, ig, ic)
+        if (it + 1) % 25 == 0:
+            print("config0 step", it + 1, "time %.4e" % ig["time"], "vcycles", ig["n_vcycles"], flush=True)
+            bit_equal(A, g, c, "config0 step %d" % (it + 1))
+
+
+def test_config1_50_steps_and_mixed_errors(A, S):
+    case = S.config1(num_vcycles=100, rel_residual=1e-8)
+    D, nc = 3, 16
+    ids = leaves_of(case.extra["export"], D)
+    c = S.apply(A.Path("oracle", D, nc, A.XYZ), case)
+    g = S.apply(A.Path("gpu", D, nc, A.XYZ, A.PREC_FP64), case)
+    m = S.apply(A.Path("gpu", D, nc, A.XYZ, A.PREC_MIXED), case)
+    assert g.field_compute(0, False) == c.field_compute(0, False)
+    m.field_compute(0, False)
+    bit_equal(A, g, c, "config1 field", ("CC_PHI", "CC_EFIELD"))
+    for it in range(50):
+        ic, ig, im = c.step(), g.step(), m.step()
+        assert ig == ic, (it, ig, ic)
+        assert im["n_rejected"] == ic["n_rejected"] and abs(im["time"] - ic["time"]) <= 1e-9 * ic["time"]
+        if (it + 1) % 10 == 0:
+            print("config1 step", it + 1, "vcycles fp64", ic["n_vcycles"], "mixed", im["n_vcycles"], flush=True)
+            bit_equal(A, g, c, "config1 step %d" % (it + 1))
+    check_errors(A, m, c, ids, D, nc, {"CC_PHI": (3e-9, 3e-10), "CC_EFIELD": (2e-7, 3e-9),
+                                       "CC_ELECTRON": (1e-8, 5e-9)}, "config2 mixed after 50 steps")
+
+
+def test_config3_photoi_every_step(A, S):
+    case = S.config1(num_vcycles=100, rel_residual=1e-8, photoi=True)
+    assert case.params.photoi_enabled == 1 and case.params.photoi_per_steps == 1
+    D, nc = 3, 16
+    ids = leaves_of(case.extra["export"], D)
+    c = S.apply(A.Path("oracle", D, nc, A.XYZ), case)
+    g = S.apply(A.Path("gpu", D, nc, A.XYZ, A.PREC_FP64), case)
+    m = S.apply(A.Path("gpu", D, nc, A.XYZ, A.PREC_MIXED), case)
+    m.set_option(A.OPT_BATCH_MODES, 1)
+    assert g.field_compute(0, False) == c.field_compute(0, False)
+    m.field_compute(0, False)
+    for it in range(20):
+        ic, ig, im = c.step(), g.step(), m.step()
+        assert ig == ic, (it, ig, ic)
+        assert ic["photoi_done"] == 1 and im["photoi_done"] == 1
+        if (it + 1) % 5 == 0:
+            print("config3 step", it + 1, "vcycles", ic["n_vcycles"], "mixed", im["n_vcycles"], flush=True)
+            bit_equal(A, g, c, "config3 step %d" % (it + 1), BIT_STATE + ("CC_PHOTO",))
+    check_errors(A, m, c, ids, D, nc, {"CC_PHI": (3e-9, 3e-10), "CC_EFIELD": (2e-7, 3e-9),
+                                       "CC_ELECTRON": (1e-8, 5e-9), "CC_PHOTO": (5e-3, 5e-3)},
+                 "config3 mixed batched after 20 steps")
+
+
+def test_config4_ensemble_full_size(A, S):
+    ens = S.ensemble()
+    assert len(ens.cases) == 64
+    cells = [c.tree.info()["leaf_cells"] for c in ens.cases]
+    assert sum(cells) == 21250048 and max(cells) == 1024 ** 2 and min(cells) == 128 ** 2
+    print(ens.name, flush=True)
+    g = S.apply_ensemble(A.Path("gpu", 2, 8, A.CYL, A.PREC_MIXED), ens)
+    singles = [S.apply(A.Path("gpu", 2, 8, A.CYL, A.PREC_MIXED), case) for case in ens.cases]
+    # four runs against the oracle: the first of each size
+    pick = [cells.index(n) for n in (128 ** 2, 256 ** 2, 512 ** 2, 1024 ** 2)]
+    oracles = {r: S.apply(A.Path("oracle", 2, 8, A.CYL), ens.cases[r]) for r in pick}
+    n_g, _ = g.field_compute_runs(0, have_guess=False)
+    for r, s in enumerate(singles):
+        assert s.field_compute(0, False)[0] == int(n_g[r]), r
+    for o in oracles.values():
+        o.field_compute(0, False)
+    from test_gpu_ensemble import STEP_KEYS
+    for it in range(3):
+        infos = g.step_ensemble()
+        for r, s in enumerate(singles):
+            ref = s.step()
+            assert {k: infos[r][k] for k in STEP_KEYS} == {k: ref[k] for k in STEP_KEYS}, (it, r)
+        for r, o in oracles.items():
+            io = o.step()
+            assert abs(infos[r]["time"] - io["time"]) <= 1e-9 * io["time"], (it, r)
+        print("config4 step", it + 1, "vcycles", sorted(set(int(i["n_vcycles"]) for i in infos)), flush=True)
+    for r, s in enumerate(singles):
+        bit_equal(A, g, s, "config4 run %d" % r, ids=ens.run_ids(r))
+    for r, o in oracles.items():
+        ex = ens.cases[r].extra["export"]
+        lv = leaves_of(ex, 2)
+        check_errors(A, singles[r], o, lv, 2, 8, {"CC_PHI": (3e-9, 3e-10), "CC_EFIELD": (2e-7, 3e-9),
+                                                  "CC_ELECTRON": (1e-8, 5e-9)},
+                     "config4 run %d (%d cells) mixed" % (r, cells[r]))

@@ -171,3 +171,17 @@ def test_forest_refuses_single_run_calls(A, S):
         g.step()
     with pytest.raises(A.AfError):
         g.set_run_voltages([1.0, 2.0, 3.0])
+
+
+def test_forest_rejected_steps_equal_oracle(A, S):
+    """Rejected steps (the backup restored, copy_current_state /
+    restore_previous_state, src/streamer.f90:254-294, 659-688) against the
+    oracle: run 1 restarts with a dt far above its limits, so its first step
+    is rejected and redone; every run's steps and state equal the oracle's
+    bit for bit."""
+    ens = S.ensemble(level_list=[4, 5, 4], num_vcycles=20, rel_residual=1e-10, seed_density=1e21)
+    p = params_for(A, ens.cases[0].params, dt_safety=1.0)
+    g, singles = forest_and_singles(A, S, ens, A.PREC_FP64, params=p, oracle=True)
+    rejected = run_and_compare(A, ens, g, singles, 5, "fp64 rejections vs oracle", dt_start={1: 5e-12})
+    print("rejected attempts", rejected, flush=True)
+    assert rejected > 0

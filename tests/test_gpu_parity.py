@@ -147,3 +147,28 @@ def test_config1_full_size_parity(A, S):
         print(it, ig["n_vcycles"], ig["dt_used"], flush=True)
         assert ig == ic, (ig, ic)
     compare(A, g, c, [A.CC_ELECTRON, A.CC_POS_ION, A.CC_NEG_ION, A.CC_PHI, A.CC_EFIELD], what="config1 steps")
+
+
+@pytest.mark.parametrize("name", ["cyl2d_amr", "xyz3d_16"])
+def test_rejected_steps_parity(A, S, name):
+    """A restart with a dt far above the limits: the first step is rejected,
+    the backup restored (copy_current_state / restore_previous_state and the
+    field recomputed from the restored potential, src/streamer.f90:254-294,
+    659-688) and the step redone with a smaller dt; safety factor 1 lets
+    later steps reject too.  Step results and states bit for bit."""
+    case = CASES[name](S)
+    case.params.dt_safety_factor = 1.0
+    case.params.mg_num_vcycles = 20
+    case.params.mg_max_rel_residual = 1e-10
+    g, c = make_pair(A, S, case)
+    assert g.field_compute(0, False) == c.field_compute(0, False)
+    for p in (g, c):
+        p.set_time_state(0, 0.0, 5e-11)
+    rejected = 0
+    for it in range(5):
+        ig, ic = g.step(), c.step()
+        assert ig == ic, (it, ig, ic)
+        rejected += ic["n_rejected"]
+        compare(A, g, c, [A.CC_ELECTRON, A.CC_POS_ION, A.CC_PHI, A.CC_EFIELD], what="%s reject %d" % (name, it))
+    print(name, "rejected attempts", rejected, flush=True)
+    assert rejected > 0

@@ -66,3 +66,23 @@ def test_autotune_single_3d_run(A, S):
         assert np.array_equal(a, b)
     with pytest.raises(A.AfError):
         g.set_shapes([(4, 8)] * A.N_KERNEL_CLASSES)  # several boxes per workgroup: 8 x 8 boxes only
+
+
+def test_non_default_shapes_equal_oracle(A, S):
+    """A non-default work decomposition against the oracle itself: a forest
+    with several 8 x 8 boxes per workgroup on a persistent grid, and a 3D
+    16^3 run on a persistent grid, step by step bit for bit."""
+    from test_gpu_ensemble import forest_and_singles, run_and_compare
+    ens = S.ensemble(level_list=[4, 5, 3], num_vcycles=20, rel_residual=1e-10)
+    g, singles = forest_and_singles(A, S, ens, A.PREC_FP64, oracle=True)
+    g.set_shapes([(2, 16)] * A.N_KERNEL_CLASSES)
+    run_and_compare(A, ens, g, singles, 3, "shape (2, 16) vs oracle")
+    case = S.small_amr(3, nc=16, uniform_lvl=2, max_lvl=3, frac=0.3, seed=7)
+    g = S.apply(A.Path("gpu", 3, 16, A.XYZ, A.PREC_FP64), case)
+    c = S.apply(A.Path("oracle", 3, 16, A.XYZ), case)
+    g.set_shapes([(1, 2)] * A.N_KERNEL_CLASSES)
+    assert g.field_compute(0, False) == c.field_compute(0, False)
+    for it in range(3):
+        assert g.step() == c.step(), it
+    for a, b in zip(snapshot(A, g), snapshot(A, c)):
+        assert np.array_equal(a, b)
