@@ -216,6 +216,7 @@ _COMMON = {
     "num_leaf_cells": (C.c_longlong, [_P]),
     "get_time_state": (_I, [_P, _IP, _DP, _DP]),
     "set_time_state": (_I, [_P, _I, _D, _D]),
+    "limit_step_dt": (_I, [_P, _D]),
     "regression_values": (_I, [_P, _I, _IP, _DP]),
     "set_chemistry": (_I, [_P, _P, _IP]),
     "get_species": (_I, [_P, _I, _I, _I, _IP, _DP]),
@@ -326,6 +327,8 @@ def load_host():
         lib.afhost_refine.argtypes = [_P, _I, _IP]
         lib.afhost_refine_uniform.restype = _I
         lib.afhost_refine_uniform.argtypes = [_P, _I]
+        lib.afhost_derefine.restype = _I
+        lib.afhost_derefine.argtypes = [_P, _I, _IP]
         lib.afhost_tree_info.restype = _I
         lib.afhost_tree_info.argtypes = [_P, _IP, _IP, C.POINTER(C.c_longlong), _IP]
         lib.afhost_tree_export.restype = _I
@@ -374,6 +377,15 @@ class Tree:
         n = self.lib.afhost_refine(self.h, len(ids), _ip(ids))
         if n < 0:
             raise AfError("refine failed")
+        return n
+
+    def derefine(self, ids):
+        """Remove the children (all leaves) of the given boxes; their slots
+        become unused (level 0)."""
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        n = self.lib.afhost_derefine(self.h, len(ids), _ip(ids))
+        if n < 0:
+            raise AfError("derefine failed")
         return n
 
     def info(self):
@@ -720,6 +732,11 @@ class Path:
 
     def set_time_state(self, it, time, global_dt):
         self._call("set_time_state", it, time, global_dt)
+
+    def limit_step_dt(self, dt_cap):
+        """The next step's dt := min(dt, dt_cap); global_dt unchanged (the
+        time loop's output-time clipping)."""
+        self._call("limit_step_dt", dt_cap)
 
     def set_option(self, option, value):
         """Device library only (e.g. OPT_GRAPHS)."""

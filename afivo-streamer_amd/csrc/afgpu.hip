@@ -3498,6 +3498,21 @@ int afgpu_set_time_state(afgpu_t *h, int it, double time, double global_dt) {
   return AFGPU_OK;
 }
 
+int afgpu_limit_step_dt(afgpu_t *h, double dt_cap) {
+  if (!h || !(dt_cap >= 0)) return AFGPU_ERR_ARG;
+  if (!h->have_params) {
+    set_err("afgpu_limit_step_dt: parameters not set");
+    return AFGPU_ERR_STATE;
+  }
+  if (!h->started) {  // (as afgpu_step starts: dt = global_dt = dt_min)
+    h->global_dt = h->p.dt_min;
+    h->dt = h->global_dt;
+    h->started = true;
+  }
+  h->dt = std::min(h->dt, dt_cap);
+  return AFGPU_OK;
+}
+
 long long afgpu_num_leaf_cells(afgpu_t *h) {
   if (!h) return 0;
   long long n = h->l_all_leaves.n;

@@ -179,6 +179,37 @@ int afhost_refine(afhost_tree_t *t, int n, const int *ids) {
   return added;
 }
 
+// Removal of the children of the given boxes, whose children must all be
+// leaves (remove_children, m_af_core.f90:1159-1192, for the boxes that
+// af_adjust_refinement derefines): the children's slots become unused
+// (level 0) and are not reused; the boxes become leaves.  Returns the
+// number of boxes removed, or -1 on error.
+int afhost_derefine(afhost_tree_t *t, int n, const int *ids) {
+  if (!t || n < 0 || (n > 0 && !ids)) return -1;
+  const int nch = 1 << t->D;
+  for (int i = 0; i < n; i++) {
+    if (ids[i] < 1 || ids[i] > (int)t->boxes.size() || t->boxes[ids[i] - 1].lvl < 1) return -1;
+    const HBox &p = t->boxes[ids[i] - 1];
+    if (p.children[0] == 0) return -1;
+    for (int q = 0; q < nch; q++)
+      if (!t->is_leaf(p.children[q])) return -1;
+  }
+  int removed = 0;
+  for (int i = 0; i < n; i++) {
+    HBox &p = t->boxes[ids[i] - 1];
+    for (int q = 0; q < nch; q++) {
+      const int c = p.children[q];
+      t->lookup.erase(t->key(t->boxes[c - 1].lvl, t->boxes[c - 1].ix));
+      t->boxes[c - 1] = HBox();
+      t->boxes[c - 1].lvl = 0;
+      p.children[q] = 0;
+      removed++;
+    }
+  }
+  t->set_neighbors();
+  return removed;
+}
+
 // af_refine_up_to_lvl: refine every leaf until all leaves are at `lvl`.
 int afhost_refine_uniform(afhost_tree_t *t, int lvl) {
   if (!t) return -1;

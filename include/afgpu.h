@@ -361,6 +361,11 @@ int afgpu_get_time_state(afgpu_t *h, int *it, double *time, double *global_dt);
 int afgpu_set_time_state(afgpu_t *h, int it, double time, double global_dt);
 
 /* Leaf-cell count of the current tree (af_num_cells_used). */
+/* The output-time clipping of the time loop (src/streamer.f90:222-229):
+ * the next step's dt becomes min(dt, dt_cap), while global_dt, from which
+ * the following step's dt grows, stays as it is.  dt_cap >= 0; a step that
+ * has not started yet starts at dt_min, as the first step does. */
+int afgpu_limit_step_dt(afgpu_t *h, double dt_cap);
 long long afgpu_num_leaf_cells(afgpu_t *h);
 
 /* Kernel-level timing: when enabled, every launch of kernel class `kind`

@@ -308,6 +308,12 @@ module m_afgpu
        real(c_double), value :: time, global_dt
      end function afgpu_set_time_state
 
+     integer(c_int) function afgpu_limit_step_dt(h, dt_cap) bind(c, name="afgpu_limit_step_dt")
+       import
+       type(c_ptr), value :: h
+       real(c_double), value :: dt_cap
+     end function afgpu_limit_step_dt
+
      integer(c_long_long) function afgpu_num_leaf_cells(h) bind(c, name="afgpu_num_leaf_cells")
        import
        type(c_ptr), value :: h
@@ -581,7 +587,7 @@ module m_afgpu
        afgpu_field_compute, afgpu_field_from_potential, afgpu_forward_euler, &
        afgpu_advance, afgpu_photoi_set_src, afgpu_copy_current_state, &
        afgpu_restore_previous_state, afgpu_step, afgpu_get_time_state, &
-       afgpu_set_time_state, afgpu_num_leaf_cells, afgpu_coarse_factors, &
+       afgpu_set_time_state, afgpu_limit_step_dt, afgpu_num_leaf_cells, afgpu_coarse_factors, &
        afgpu_refine_flags, afgpu_restrict_boxes, afgpu_prolong_boxes, &
        afgpu_regression_values, afgpu_regression_values_noalloc, &
        afgpu_format_regression_line, afgpu_write_regression_log, afgpu_get_boxes, &

@@ -2767,6 +2767,19 @@ int afcpu_set_time_state(afcpu_t *h, int it, double time, double global_dt) {
   return AFGPU_OK;
 }
 
+int afcpu_limit_step_dt(afcpu_t *h, double dt_cap) {  // src/streamer.f90:222-229
+  CHECK_H(h);
+  Ctx &c = h->c;
+  if (!(dt_cap >= 0) || !c.have_params) return AFGPU_ERR_ARG;
+  if (!c.started) {
+    c.global_dt = c.p.dt_min;
+    c.dt = c.global_dt;
+    c.started = true;
+  }
+  c.dt = std::min(c.dt, dt_cap);
+  return AFGPU_OK;
+}
+
 long long afcpu_num_leaf_cells(afcpu_t *h) {
   if (!h) return 0;
   long long n = 0;

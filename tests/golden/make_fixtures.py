@@ -7,7 +7,10 @@ holds the reference at /root/reference):
   line again and compare the text.
 - air_chemistry_v1.txt: the input data file of the reference's chemistry
   rtests (programs/standard_2d/tests/air_chemistry_v1.txt: reaction list,
-  transport and rate tables), copied unchanged."""
+  transport and rate tables), copied unchanged.
+- td_air_siglo_swarm.txt: the old-style transport data of the reference's
+  standard rtests (programs/standard_2d/tests/td_air_siglo_swarm.txt; the
+  3D tests hold the same file), copied unchanged."""
 import glob
 import json
 import os
@@ -26,3 +29,5 @@ with open(os.path.join(HERE, "regression_logs.json"), "w") as f:
 print(len(logs), "logs,", sum(len(v["lines"]) for v in logs.values()), "lines")
 shutil.copyfile(os.path.join(REF, "programs", "standard_2d", "tests", "air_chemistry_v1.txt"),
                 os.path.join(HERE, "air_chemistry_v1.txt"))
+shutil.copyfile(os.path.join(REF, "programs", "standard_2d", "tests", "td_air_siglo_swarm.txt"),
+                os.path.join(HERE, "td_air_siglo_swarm.txt"))
