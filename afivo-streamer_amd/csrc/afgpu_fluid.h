@@ -150,16 +150,6 @@ __global__ void __launch_bounds__(BS * BB)
     TF smax = -std::numeric_limits<TF>::max();
     TF cmax = 0;
     if constexpr (D == 3) {
-      // Face fluxes (flux_upwind, src/m_fluid.f90:102-209) cell by cell,
-      // x fastest over the threads, so that the loads of every direction
-      // are coalesced and a cell's n_e and field are read while still in
-      // cache by all three directions.  For each direction a cell forms its
-      // lower and its upper face: an interior face is formed by both of its
-      // cells, from the same values, and written by the cell above it; the
-      // box's last face by the cell below it.  The two faces' velocities and
-      // diffusion coefficients give the cell's CFL term (m_fluid.f90:194-195,
-      // m_af_flux_schemes.f90:779-848), summed over the directions in the
-      // order x, y, z.
       const TF *tab = tab_lds ? tab_all[lgrp<BS, BB>()] : nullptr;
       auto coef = [&](int c, int ix, TF w) {
         return tab_lds ? w * tab[c * TABMAX + ix - 1] + (1 - w) * tab[c * TABMAX + ix] : lt_at_t<TF>(td, c, ix, w);
@@ -182,38 +172,161 @@ __global__ void __launch_bounds__(BS * BB)
         smax = fmax(smax, mu * u);
         return v * u - dc * inv_dx * (L0 - Lm1);
       };
-      for (int q = ltid<BS>(); q < G::NINT; q += BS) {
-        const int p[3] = {q % NC + 1, (q / NC) % NC + 1, q / (NC * NC) + 1};
-        TF cfl = 0;
-        for (int d = 0; d < D; d++) {
-          const TF inv_dx = TF(1) / (TF)bx.dr[d];
-          int t0, t1;
-          tangential(D, d, t0, t1);
-          const int r = (p[t0] - 1) + NC * (p[t1] - 1);  // the cell's position on a face of direction d
-          constexpr int st[3] = {1, G::NG, G::NG * G::NG};
-          const int cd = p[d];
-          const size_t c0 = base + G::idx(p[0], p[1], p[2]);
-          TF nl[5];  // n_e at cd - 2 .. cd + 2 along d
-          for (int s2 = -2; s2 <= 2; s2++) {
-            const int pos = cd + s2;
-            nl[s2 + 2] = (pos >= 1 && pos <= NC) ? (TF)ne[c0 + s2 * st[d]]
-                         : (pos <= 0)              ? halo[2 * d][-pos][r]
-                                                   : halo[2 * d + 1][pos - NC - 1][r];
+      // The cell's CFL term of one direction from its lower and upper faces
+      // (m_fluid.f90:194-195, m_af_flux_schemes.f90:779-848); the terms are
+      // summed over the directions in the order x, y, z.
+      auto cfl_term = [](TF av_hi, TF av_lo, TF dc_hi, TF dc_lo, TF inv_dx) -> TF {
+        return TF(1) * fmax(av_hi, av_lo) * inv_dx + TF(2) * fmax(dc_hi, dc_lo) * (inv_dx * inv_dx);
+      };
+      if constexpr (BS == NC * NC) {
+        // Face fluxes (flux_upwind, src/m_fluid.f90:102-209), one thread per
+        // grid column (i, j), walking up the box in z.  Plane k is staged in
+        // LDS (n_e with its two halo layers in x and y, |E| with its ghost
+        // ring), so the four upwind values and two field values of an x or
+        // y face come from LDS; the column's own z neighbours stay in
+        // registers.  Every face is formed once: the thread forms the lower
+        // x, y and z faces of its cell, the first wavefront the box's upper
+        // x and y faces of the plane, and every thread its column's upper z
+        // face at the end.  A cell takes its upper faces' |v| and D for the
+        // CFL term from the threads that formed them (LDS), or, in z, from
+        // the next plane.  Global loads and stores are rows of cells or
+        // faces in x, the contiguous direction.
+        static_assert(BB == 1 && BS >= 12 * NC, "one box of NC x NC columns per workgroup");
+        constexpr int NP = NC + 4, NE = NC + 2;
+        __shared__ TF pn[NP * NP];            // n_e of the plane: (i, j) at (j + 1) NP + i + 1, i, j = -1 .. NC + 2
+        __shared__ TF pe[NE * NE];            // |E| of the plane: (i, j) at j NE + i, i, j = 0 .. NC + 1
+        __shared__ TF fv[2][BS + NC], fd[2][BS + NC];  // |v| and D of the plane's x and y faces (lower, upper boundary)
+        constexpr int st2 = G::NG * G::NG;
+        const int t = ltid<BS>();
+        const int i = t % NC + 1, j = t / NC + 1;
+        const TF inv_dx[3] = {TF(1) / (TF)bx.dr[0], TF(1) / (TF)bx.dr[1], TF(1) / (TF)bx.dr[2]};
+        const int rz = (i - 1) + NC * (j - 1);
+        const size_t col = base + G::idx(i, j, 0);
+        // the column's n_e at k - 2 .. k + 1 and |E| at k - 1, k
+        TF n0 = halo[4][1][rz], n1 = halo[4][0][rz], n2 = (TF)ne[col + st2], n3 = (TF)ne[col + 2 * st2];
+        TF em = (TF)efield[col], e0 = (TF)efield[col + st2];
+        TF pend = 0, avz_prev = 0, dz_prev = 0;  // cell k - 1: its x + y CFL sum, its lower z face
+#pragma unroll 1
+        for (int k = 1; k <= NC; k++) {
+          const size_t c0 = col + (size_t)k * st2;
+          const TF ex = (TF)efc[fbase + G::fidx(i, j, k, 0)];
+          const TF ey = (TF)efc[fbase + G::fidx(i, j, k, 1)];
+          const TF ez = (TF)efc[fbase + G::fidx(i, j, k, 2)];
+          const TF n4 = (k + 2 <= NC) ? (TF)ne[c0 + 2 * st2] : halo[5][k + 1 - NC][rz];
+          const TF ep = (TF)efield[c0 + st2];
+          pn[(j + 1) * NP + i + 1] = n2;
+          pe[j * NE + i] = e0;
+          if (t < 8 * NC) {
+            // halo of n_e: s = direction (x, y) x side (low, high) x layer
+            const int s = t / NC, a = t % NC + 1;
+            const int layer = s & 1, side = (s >> 1) & 1, dd = s >> 2;
+            const TF hv = halo[2 * dd + side][layer][(a - 1) + NC * (k - 1)];
+            const int q = side ? NC + 1 + layer : -layer;
+            if (dd == 0)
+              pn[(a + 1) * NP + q + 1] = hv;
+            else
+              pn[(q + 1) * NP + a + 1] = hv;
+          } else if (t < 12 * NC) {
+            // ghost ring of |E|: x low, x high, y low, y high
+            const int s = (t - 8 * NC) / NC, a = (t - 8 * NC) % NC + 1;
+            const int q = (s & 1) ? NC + 1 : 0;
+            const int ii = (s < 2) ? q : a, jj = (s < 2) ? a : q;
+            pe[jj * NE + ii] = (TF)efield[base + G::idx(ii, jj, k)];
           }
-          const TF e_m = (TF)efield[c0 - st[d]], e_0 = (TF)efield[c0], e_p = (TF)efield[c0 + st[d]];
-          int fp[3] = {p[0], p[1], p[2]};
-          const size_t f_lo = fbase + G::fidx(fp[0], fp[1], fp[2], d);
-          fp[d] = cd + 1;
-          const size_t f_hi = fbase + G::fidx(fp[0], fp[1], fp[2], d);
-          TF v_lo, dc_lo, v_hi, dc_hi;
-          const TF fl_lo = face(nl[0], nl[1], nl[2], nl[3], (TF)efc[f_lo], e_m, e_0, inv_dx, v_lo, dc_lo);
-          const TF fl_hi = face(nl[1], nl[2], nl[3], nl[4], (TF)efc[f_hi], e_0, e_p, inv_dx, v_hi, dc_hi);
-          flux[f_lo] = (double)fl_lo;
-          if (cd == NC) flux[f_hi] = (double)fl_hi;
-          const TF cl = TF(1) * fmax(fabs(v_hi), fabs(v_lo)) * inv_dx + TF(2) * fmax(dc_hi, dc_lo) * (inv_dx * inv_dx);
-          cfl = cfl + cl;
+          gsync<BS, BB>();
+          TF vx, dx, vy, dy;
+          {
+            const int o = (j + 1) * NP + i + 1;
+            flux[fbase + G::fidx(i, j, k, 0)] = (double)face(pn[o - 2], pn[o - 1], pn[o], pn[o + 1], ex,
+                                                              pe[j * NE + i - 1], pe[j * NE + i], inv_dx[0], vx, dx);
+            flux[fbase + G::fidx(i, j, k, 1)] =
+                (double)face(pn[o - 2 * NP], pn[o - NP], pn[o], pn[o + NP], ey, pe[(j - 1) * NE + i], pe[j * NE + i],
+                             inv_dx[1], vy, dy);
+          }
+          fv[0][t] = fabs(vx);
+          fd[0][t] = dx;
+          fv[1][t] = fabs(vy);
+          fd[1][t] = dy;
+          if (t < 2 * NC) {
+            // the box's upper x face of row j = a (dd = 0) or y face of column i = a (dd = 1)
+            const int dd = t / NC, a = t % NC + 1;
+            const int ii = dd ? a : NC + 1, jj = dd ? NC + 1 : a;
+            const int o = (jj + 1) * NP + ii + 1, sp = dd ? NP : 1, se = dd ? NE : 1;
+            TF vb, db;
+            flux[fbase + G::fidx(ii, jj, k, dd)] =
+                (double)face(pn[o - 2 * sp], pn[o - sp], pn[o], pn[o + sp], (TF)efc[fbase + G::fidx(ii, jj, k, dd)],
+                             pe[jj * NE + ii - se], pe[jj * NE + ii], inv_dx[dd], vb, db);
+            fv[dd][BS + a - 1] = fabs(vb);
+            fd[dd][BS + a - 1] = db;
+          }
+          gsync<BS, BB>();
+          const int hx = (i < NC) ? t + 1 : BS + j - 1, hy = (j < NC) ? t + NC : BS + i - 1;
+          TF c = 0;
+          c = c + cfl_term(fv[0][hx], fabs(vx), fd[0][hx], dx, inv_dx[0]);
+          c = c + cfl_term(fv[1][hy], fabs(vy), fd[1][hy], dy, inv_dx[1]);
+          // z face k: the lower face of cell k, the upper one of cell k - 1
+          TF vz, dz;
+          flux[fbase + G::fidx(i, j, k, 2)] = (double)face(n0, n1, n2, n3, ez, em, e0, inv_dx[2], vz, dz);
+          if (k > 1) cmax = fmax(cmax, pend + cfl_term(fabs(vz), avz_prev, dz, dz_prev, inv_dx[2]));
+          pend = c;
+          avz_prev = fabs(vz);
+          dz_prev = dz;
+          n0 = n1;
+          n1 = n2;
+          n2 = n3;
+          n3 = n4;
+          em = e0;
+          e0 = ep;
         }
-        cmax = fmax(cmax, cfl);
+        // the box's upper z face and the top cell's CFL sum
+        TF vz, dz;
+        flux[fbase + G::fidx(i, j, NC + 1, 2)] =
+            (double)face(n0, n1, n2, n3, (TF)efc[fbase + G::fidx(i, j, NC + 1, 2)], em, e0, inv_dx[2], vz, dz);
+        cmax = fmax(cmax, pend + cfl_term(fabs(vz), avz_prev, dz, dz_prev, inv_dx[2]));
+      } else {
+        // Boxes with fewer columns than threads (8^3): face fluxes
+        // (flux_upwind, src/m_fluid.f90:102-209) cell by cell, x fastest
+        // over the threads, so that the loads of every direction
+        // are coalesced and a cell's n_e and field are read while still in
+        // cache by all three directions.  For each direction a cell forms its
+        // lower and its upper face: an interior face is formed by both of its
+        // cells, from the same values, and written by the cell above it; the
+        // box's last face by the cell below it.  The two faces' velocities and
+        // diffusion coefficients give the cell's CFL term (m_fluid.f90:194-195,
+        // m_af_flux_schemes.f90:779-848), summed over the directions in the
+        // order x, y, z.
+        for (int q = ltid<BS>(); q < G::NINT; q += BS) {
+          const int p[3] = {q % NC + 1, (q / NC) % NC + 1, q / (NC * NC) + 1};
+          TF cfl = 0;
+          for (int d = 0; d < D; d++) {
+            const TF inv_dx = TF(1) / (TF)bx.dr[d];
+            int t0, t1;
+            tangential(D, d, t0, t1);
+            const int r = (p[t0] - 1) + NC * (p[t1] - 1);  // the cell's position on a face of direction d
+            constexpr int st[3] = {1, G::NG, G::NG * G::NG};
+            const int cd = p[d];
+            const size_t c0 = base + G::idx(p[0], p[1], p[2]);
+            TF nl[5];  // n_e at cd - 2 .. cd + 2 along d
+            for (int s2 = -2; s2 <= 2; s2++) {
+              const int pos = cd + s2;
+              nl[s2 + 2] = (pos >= 1 && pos <= NC) ? (TF)ne[c0 + s2 * st[d]]
+                           : (pos <= 0)              ? halo[2 * d][-pos][r]
+                                                     : halo[2 * d + 1][pos - NC - 1][r];
+            }
+            const TF e_m = (TF)efield[c0 - st[d]], e_0 = (TF)efield[c0], e_p = (TF)efield[c0 + st[d]];
+            int fp[3] = {p[0], p[1], p[2]};
+            const size_t f_lo = fbase + G::fidx(fp[0], fp[1], fp[2], d);
+            fp[d] = cd + 1;
+            const size_t f_hi = fbase + G::fidx(fp[0], fp[1], fp[2], d);
+            TF v_lo, dc_lo, v_hi, dc_hi;
+            const TF fl_lo = face(nl[0], nl[1], nl[2], nl[3], (TF)efc[f_lo], e_m, e_0, inv_dx, v_lo, dc_lo);
+            const TF fl_hi = face(nl[1], nl[2], nl[3], nl[4], (TF)efc[f_hi], e_0, e_p, inv_dx, v_hi, dc_hi);
+            flux[f_lo] = (double)fl_lo;
+            if (cd == NC) flux[f_hi] = (double)fl_hi;
+            cfl = cfl + cfl_term(fabs(v_hi), fabs(v_lo), dc_hi, dc_lo, inv_dx);
+          }
+          cmax = fmax(cmax, cfl);
+        }
       }
       double smax_d = group_max<BS, BB>((double)smax, sh);
       if (ltid<BS>() == 0) atomic_max_nonneg(run_slot(sig_max, bx), fmax(smax_d, 1e-100));

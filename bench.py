@@ -563,6 +563,9 @@ def main():
     ap.add_argument("--no-fp64-ref", action="store_true")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the mixed form's variants (FP32 coarsest solve, inner cycle to level 1)")
+    ap.add_argument("--no-side", action="store_true",
+                    help="skip the measurements beyond the time step (refinement flags, diagnostics, chemistry, "
+                         "new boxes), e.g. to profile the time step alone")
     args = ap.parse_args()
     global FORCED_SHAPES, SAVE_SHAPES
     if args.shapes:
@@ -697,11 +700,12 @@ def main():
                "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
                "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
         log("  cpu %.4e cell-steps/s (%.1f s)" % (cpu["value"], csecs))
-    extra["refine_flags"] = measure_refine(case, p, None if args.no_cpu else c)
-    extra["diagnostics"] = measure_diagnostics(case, p, None if args.no_cpu else c)
-    extra["chemistry"] = measure_chemistry(int(info["leaf_cells"]), not args.no_cpu)
-    extra["air_network_step"] = measure_air_step(case)
-    extra["new_boxes"] = measure_new_boxes(case, p, None if args.no_cpu else c)
+    if not args.no_side:
+        extra["refine_flags"] = measure_refine(case, p, None if args.no_cpu else c)
+        extra["diagnostics"] = measure_diagnostics(case, p, None if args.no_cpu else c)
+        extra["chemistry"] = measure_chemistry(int(info["leaf_cells"]), not args.no_cpu)
+        extra["air_network_step"] = measure_air_step(case)
+        extra["new_boxes"] = measure_new_boxes(case, p, None if args.no_cpu else c)
 
     out = {
         "metric": "cell-steps/s, 3D AMR step (DD+MG Poisson+photoi), FP64 vs mixed at 1e-8",

@@ -49,10 +49,13 @@ def errors(p, ref, iv, ids, D, nc):
     return mx, l2
 
 
-def bit_equal(A, g, c, what, names=BIT_STATE, ids=None):
+def bit_equal(A, g, c, what, names=BIT_STATE, ids=None, c_ids=None):
+    """g's boxes ``ids`` against c's boxes ``c_ids`` (default: the same ids)."""
+    if c_ids is None:
+        c_ids = ids
     for name in names:
         iv = getattr(A, name)
-        a, b = g.get_cc(iv, ids), c.get_cc(iv, ids)
+        a, b = g.get_cc(iv, ids), c.get_cc(iv, c_ids)
         assert np.array_equal(a, b), "%s: %s differs (max %.3e)" % (what, name, np.abs(a - b).max())
 
 
@@ -152,7 +155,8 @@ def test_config4_ensemble_full_size(A, S):
             assert abs(infos[r]["time"] - io["time"]) <= 1e-9 * io["time"], (it, r)
         print("config4 step", it + 1, "vcycles", sorted(set(int(i["n_vcycles"]) for i in infos)), flush=True)
     for r, s in enumerate(singles):
-        bit_equal(A, g, s, "config4 run %d" % r, ids=ens.run_ids(r))
+        run = ens.run_ids(r)
+        bit_equal(A, g, s, "config4 run %d" % r, ids=run, c_ids=np.arange(1, len(run) + 1, dtype=np.int32))
     for r, o in oracles.items():
         ex = ens.cases[r].extra["export"]
         lv = leaves_of(ex, 2)
