@@ -52,6 +52,7 @@ OPT_FLUX32 = 8
 OPT_STEP_GRAPH = 9
 OPT_FUSE_DIRECT = 10
 OPT_DIRECT_F32 = 11
+OPT_BOX_LDS = 12
 
 
 class ParamsT(C.Structure):

@@ -241,6 +241,7 @@ struct afgpu_ctx {
   bool direct_mfma = true;  // uniform-level transforms on the matrix cores
   bool photoi_inner_vcycle = false;  // photoionization inner cycle: V-cycle instead of FMG
   bool flux32 = true;  // mixed precision: FP32 flux stencil (FP64 accumulation)
+  bool box_lds = false;  // mixed precision, 3D 16^3 boxes: whole boxes staged in LDS by the half-sweeps
   bool fuse_direct = false;  // uniform-level solve: last-axis transforms fused into one pass
   bool direct_f32 = false;   // uniform-level solve of the FP32 inner cycle in FP32 (else FP64; matrix cores)
   // refinement flags: the boxes evaluated and their flags (device)
@@ -606,7 +607,7 @@ struct Impl : ImplBase {
   template <typename T>
   int boxop(const List &l, const T *a, T *b, int op, T s = T(0)) {
     constexpr int STRIDE = Lay<D, NC, T>::STRIDE;
-    const int k = bpb(l.n, STRIDE);
+    const int k = bpb(l.n, STRIDE * (int)sizeof(T) / 16);  // (16-byte words)
     return with_bb(KC_BOXOP, [&](auto B) {
       constexpr int bb = decltype(B)::value;
       return items<bb>(KC_BOXOP, k_boxop<STRIDE, BS, T, bb>, l.n * k, c.list(l), a, b, op, s, k);
@@ -695,6 +696,13 @@ struct Impl : ImplBase {
     }
     for (int n = 1; n <= 2 * n_cycle; n++) {
       const int stored = (stored_first && n == 1 && !Lay<D, NC, T>::SPLIT) ? 1 : 0;
+      if constexpr (D == 3 && Lay<D, NC, T>::SPLIT && BS == NC * NC) {
+        if (k == 1 && !c.box_lds) {
+          RCHK(items_timed<1>(0, bytes, KC_GSRB, k_gsrb_split<D, NC, BS>, l.n, c.list(l), (const DBox *)c.d_boxes,
+                              v.phi, (const T *)v.rhs, n, v.lambda2, v.m));
+          continue;
+        }
+      }
       if (k == 1)
         RCHK(with_bb(KC_GSRB, [&](auto B) {
           constexpr int bb = decltype(B)::value;
@@ -1660,9 +1668,12 @@ struct Impl : ImplBase {
     }
     RCHK(with_bb(KC_UPDATE, [&](auto B) {
       constexpr int bb = decltype(B)::value;
-      return items_timed<bb>(2, (7.0 + D + (c.p.photoi_enabled ? 1 : 0)) * 8.0 * c.l_all_leaves.n * G::NINT, KC_UPDATE,
-                             k_update<D, NC, BS, bb>, c.l_all_leaves.n, c.list(c.l_all_leaves), (const DBox *)c.d_boxes,
-                             A, c.d_red + 3);
+      const double bytes = (7.0 + D + (c.p.photoi_enabled ? 1 : 0)) * 8.0 * c.l_all_leaves.n * G::NINT;
+      if (A.use_net)
+        return items_timed<bb>(2, bytes, KC_UPDATE, k_update<D, NC, BS, bb, true>, c.l_all_leaves.n,
+                               c.list(c.l_all_leaves), (const DBox *)c.d_boxes, A, c.d_red + 3);
+      return items_timed<bb>(2, bytes, KC_UPDATE, k_update<D, NC, BS, bb, false>, c.l_all_leaves.n,
+                             c.list(c.l_all_leaves), (const DBox *)c.d_boxes, A, c.d_red + 3);
     }));
     return AFGPU_OK;
   }
@@ -3754,6 +3765,11 @@ int afgpu_set_option(afgpu_t *h, int option, int value) {
       RCHK(sync_stream(h));
       drop_graphs(*h);
       h->flux32 = value != 0;
+      return AFGPU_OK;
+    case AFGPU_OPT_BOX_LDS:
+      RCHK(sync_stream(h));
+      drop_graphs(*h);
+      h->box_lds = value != 0;
       return AFGPU_OK;
     case AFGPU_OPT_PHOTOI_VCYCLE:
       RCHK(sync_stream(h));

@@ -206,12 +206,25 @@ __global__ void __launch_bounds__(BS * BB)
         TF n0 = halo[4][1][rz], n1 = halo[4][0][rz], n2 = (TF)ne[col + st2], n3 = (TF)ne[col + 2 * st2];
         TF em = (TF)efield[col], e0 = (TF)efield[col + st2];
         TF pend = 0, avz_prev = 0, dz_prev = 0;  // cell k - 1: its x + y CFL sum, its lower z face
+        // loads of plane k + 1 are made on plane k (the faces' field and the
+        // |E| ghost ring), so no plane waits for memory
+        const int rs = (t - 8 * NC) / NC, ra = (t - 8 * NC) % NC + 1;  // |E| ring: side, position
+        const int rq = (rs & 1) ? NC + 1 : 0;
+        const int ri = (rs < 2) ? rq : ra, rj = (rs < 2) ? ra : rq;
+        const bool ring = t >= 8 * NC && t < 12 * NC;
+        TF exn = (TF)efc[fbase + G::fidx(i, j, 1, 0)], eyn = (TF)efc[fbase + G::fidx(i, j, 1, 1)],
+           ezn = (TF)efc[fbase + G::fidx(i, j, 1, 2)];
+        TF ern = ring ? (TF)efield[base + G::idx(ri, rj, 1)] : TF(0);
 #pragma unroll 1
         for (int k = 1; k <= NC; k++) {
           const size_t c0 = col + (size_t)k * st2;
-          const TF ex = (TF)efc[fbase + G::fidx(i, j, k, 0)];
-          const TF ey = (TF)efc[fbase + G::fidx(i, j, k, 1)];
-          const TF ez = (TF)efc[fbase + G::fidx(i, j, k, 2)];
+          const TF ex = exn, ey = eyn, ez = ezn, er = ern;
+          if (k < NC) {
+            exn = (TF)efc[fbase + G::fidx(i, j, k + 1, 0)];
+            eyn = (TF)efc[fbase + G::fidx(i, j, k + 1, 1)];
+            ezn = (TF)efc[fbase + G::fidx(i, j, k + 1, 2)];
+            if (ring) ern = (TF)efield[base + G::idx(ri, rj, k + 1)];
+          }
           const TF n4 = (k + 2 <= NC) ? (TF)ne[c0 + 2 * st2] : halo[5][k + 1 - NC][rz];
           const TF ep = (TF)efield[c0 + st2];
           pn[(j + 1) * NP + i + 1] = n2;
@@ -226,12 +239,9 @@ __global__ void __launch_bounds__(BS * BB)
               pn[(a + 1) * NP + q + 1] = hv;
             else
               pn[(q + 1) * NP + a + 1] = hv;
-          } else if (t < 12 * NC) {
+          } else if (ring) {
             // ghost ring of |E|: x low, x high, y low, y high
-            const int s = (t - 8 * NC) / NC, a = (t - 8 * NC) % NC + 1;
-            const int q = (s & 1) ? NC + 1 : 0;
-            const int ii = (s < 2) ? q : a, jj = (s < 2) ? a : q;
-            pe[jj * NE + ii] = (TF)efield[base + G::idx(ii, jj, k)];
+            pe[rj * NE + ri] = er;
           }
           gsync<BS, BB>();
           TF vx, dx, vy, dy;
@@ -552,7 +562,9 @@ __device__ inline void net_derivs(const ChemArgs &a, double field, const double 
 // add_source_terms (src/m_fluid.f90:298-466, src/m_chemistry.f90:672-804)
 // over all leaves: the standard model, or a reaction list over the three
 // species (A.use_net).
-template <int D, int NC, int BS, int BB = 1>
+// NET: the reaction list (A.use_net) compiled in; the standard model's
+// instantiation holds none of its code or registers.
+template <int D, int NC, int BS, int BB = 1, bool NET = false>
 __global__ void __launch_bounds__(BS * BB) k_update(const int *__restrict__ ids, const DBox *__restrict__ boxes,
                                                     UpdateArgs A, unsigned long long *chem_min, int nw) {
   using G = Geo<D, NC>;
@@ -585,7 +597,7 @@ __global__ void __launch_bounds__(BS * BB) k_update(const int *__restrict__ ids,
       }
       const double field = SI_to_Townsend * A.N_inv * A.efield[x];
       double dd[3] = {0.0, 0.0, 0.0};
-      if (A.use_net) {
+      if constexpr (NET) {
         net_derivs(A.net, field, dens, dd);
       } else {
         int lix;

@@ -126,15 +126,68 @@ template <typename T>
 struct GhostPart {
   T p, c1;
 };
+// ghost_coarse in two phases: the coarse values it reads (none on a
+// physical boundary), then the part formed from them, so that a kernel can
+// have the loads of several ghosts in flight before it forms any.
+template <typename T>
+struct CoarseVals {
+  T v[5];
+};
 template <int D, int NC, typename T>
-__device__ inline GhostPart<T> ghost_coarse(const DBox &bx, const T *var, const VarBc &m, int nb, int a, int b,
-                                            const T *coarse = nullptr) {
+__device__ inline CoarseVals<T> ghost_coarse_load(const DBox &bx, const T *var, const VarBc &m, int nb, int a, int b,
+                                                  const T *coarse = nullptr) {
   using L = Lay<D, NC, T>;
   constexpr int H = NC / 2;
   const int d = nb / 2;
   const bool low = (nb % 2) == 0;
   int t0, t1;
   tangential(D, d, t0, t1);
+  CoarseVals<T> cv = {{T(0), T(0), T(0), T(0), T(0)}};
+  if (bx.nb[nb] < 0) return cv;
+  const T *P = (coarse ? coarse : var) + (size_t)(bx.pnb[nb] - 1) * L::STRIDE;
+  int co[3] = {0, 0, 0};
+  for (int dd = 0; dd < D; dd++) co[dd] = ((bx.ix[dd] - 1) & 1) * H;
+  // coarse cell at tangential (qa, qb) on the coarse neighbour's face cn
+  auto at = [&](int cn, int qa, int qb) {
+    int q[3] = {0, 0, 0};
+    q[d] = cn;
+    q[t0] = qa;
+    if (D == 3) q[t1] = qb;
+    return P[L::at(q[0], q[1], q[2])];
+  };
+  const int cn = low ? NC : 1;
+  if (m.rb == RB_MG) {
+    const int i = (a + 1) >> 1, j = (b + 1) >> 1;
+    if (D == 2) {
+      cv.v[0] = at(cn, co[t0] + i + 1, 0);
+      cv.v[1] = at(cn, co[t0] + i - 1, 0);
+      cv.v[2] = at(cn, co[t0] + i, 0);
+    } else {
+      cv.v[0] = at(cn, co[t0] + i + 1, co[t1] + j);
+      cv.v[1] = at(cn, co[t0] + i - 1, co[t1] + j);
+      cv.v[2] = at(cn, co[t0] + i, co[t1] + j + 1);
+      cv.v[3] = at(cn, co[t0] + i, co[t1] + j - 1);
+      cv.v[4] = at(cn, co[t0] + i, co[t1] + j);
+    }
+    return cv;
+  }
+  const int a1 = co[t0] + ((a + 1) >> 1), a2 = a1 + 1 - 2 * (a & 1);
+  if (D == 2) {
+    cv.v[0] = at(cn, a1, 0);
+    cv.v[1] = at(cn, a2, 0);
+  } else {
+    const int b1 = co[t1] + ((b + 1) >> 1), b2 = b1 + 1 - 2 * (b & 1);
+    cv.v[0] = at(cn, a1, b1);
+    cv.v[1] = at(cn, a2, b1);
+    cv.v[2] = at(cn, a1, b2);
+  }
+  return cv;
+}
+template <int D, int NC, typename T>
+__device__ inline GhostPart<T> ghost_coarse_form(const DBox &bx, const VarBc &m, int nb, int a, int b,
+                                                 const CoarseVals<T> &cv) {
+  const int d = nb / 2;
+  const bool low = (nb % 2) == 0;
   GhostPart<T> g;
   g.c1 = T(0);
   if (bx.nb[nb] < 0) {
@@ -148,69 +201,34 @@ __device__ inline GhostPart<T> ghost_coarse(const DBox &bx, const T *var, const 
     g.p = c0 * (T)bc_value(m, nb, bx.run);
     return g;
   }
-  const T *P = (coarse ? coarse : var) + (size_t)(bx.pnb[nb] - 1) * L::STRIDE;
-  int co[3] = {0, 0, 0};
-  for (int dd = 0; dd < D; dd++) co[dd] = ((bx.ix[dd] - 1) & 1) * H;
   if (m.rb == RB_MG) {
-    const int cn = low ? NC : 1;
     if (D == 2) {
-      const int i = (a + 1) >> 1;
-      int q[3] = {0, 0, 0};
-      q[d] = cn;
-      q[t0] = co[t0] + i + 1;
-      const T tp = P[L::at(q[0], q[1], q[2])];
-      q[t0] = co[t0] + i - 1;
-      const T tm = P[L::at(q[0], q[1], q[2])];
-      q[t0] = co[t0] + i;
-      const T tc = P[L::at(q[0], q[1], q[2])];
-      const T grad = T(0.125) * (tp - tm);
-      g.p = (a & 1) ? tc - grad : tc + grad;
+      const T grad = T(0.125) * (cv.v[0] - cv.v[1]);
+      g.p = (a & 1) ? cv.v[2] - grad : cv.v[2] + grad;
     } else {
-      const int i = (a + 1) >> 1, j = (b + 1) >> 1;
-      auto TT = [&](int qa, int qb) {
-        int q[3];
-        q[d] = cn;
-        q[t0] = co[t0] + qa;
-        q[t1] = co[t1] + qb;
-        return P[L::at(q[0], q[1], q[2])];
-      };
-      const T g1 = T(0.125) * (TT(i + 1, j) - TT(i - 1, j));
-      const T g2 = T(0.125) * (TT(i, j + 1) - TT(i, j - 1));
-      const T tc = TT(i, j);
+      const T g1 = T(0.125) * (cv.v[0] - cv.v[1]);
+      const T g2 = T(0.125) * (cv.v[2] - cv.v[3]);
+      const T tc = cv.v[4];
       const T h = (a & 1) ? tc - g1 : tc + g1;
       g.p = (b & 1) ? h - g2 : h + g2;
     }
     return g;
   }
   const T third = T(1 / 3.0), sixth = T(1 / 6.0);
-  const int ix_c = low ? NC : 1;
   if (D == 2) {
-    const int a1 = co[t0] + ((a + 1) >> 1), a2 = a1 + 1 - 2 * (a & 1);
-    int q[3] = {0, 0, 0};
-    q[d] = ix_c;
-    q[t0] = a1;
-    const T c1 = P[L::at(q[0], q[1], q[2])];
-    q[t0] = a2;
-    const T c2 = P[L::at(q[0], q[1], q[2])];
-    g.p = T(0.5) * c1 + sixth * c2;
-    g.c1 = c1;
+    g.p = T(0.5) * cv.v[0] + sixth * cv.v[1];
   } else {
-    const int a1 = co[t0] + ((a + 1) >> 1), a2 = a1 + 1 - 2 * (a & 1);
-    const int b1 = co[t1] + ((b + 1) >> 1), b2 = b1 + 1 - 2 * (b & 1);
-    int q[3];
-    q[d] = ix_c;
-    q[t0] = a1; q[t1] = b1;
-    const T c1 = P[L::at(q[0], q[1], q[2])];
-    q[t0] = a2; q[t1] = b1;
-    const T ca = P[L::at(q[0], q[1], q[2])];
-    q[t0] = a1; q[t1] = b2;
-    const T cb = P[L::at(q[0], q[1], q[2])];
     // the z-face case sums the off-diagonal coarse values in the other order
-    const T c2 = (d == 2) ? cb : ca, c3 = (d == 2) ? ca : cb;
-    g.p = third * c1 + sixth * c2 + sixth * c3;
-    g.c1 = c1;
+    const T c2 = (d == 2) ? cv.v[2] : cv.v[1], c3 = (d == 2) ? cv.v[1] : cv.v[2];
+    g.p = third * cv.v[0] + sixth * c2 + sixth * c3;
   }
+  g.c1 = cv.v[0];
   return g;
+}
+template <int D, int NC, typename T>
+__device__ inline GhostPart<T> ghost_coarse(const DBox &bx, const T *var, const VarBc &m, int nb, int a, int b,
+                                            const T *coarse = nullptr) {
+  return ghost_coarse_form<D, NC, T>(bx, m, nb, a, b, ghost_coarse_load<D, NC, T>(bx, var, m, nb, a, b, coarse));
 }
 
 template <int D, int NC, typename T>
@@ -634,20 +652,44 @@ __global__ void __launch_bounds__(BS * BB) k_coarse_rhs(const int *__restrict__ 
 // Whole-box elementwise operations (afivo/src/m_af_utils.f90:385-576).
 enum BoxOp { OP_COPY = 0, OP_CLEAR = 1, OP_SUB_FROM = 2, OP_AXPY_NEG = 3 };
 // OP_COPY: b = a; OP_CLEAR: b = 0; OP_SUB_FROM: b = a - b; OP_AXPY_NEG: b = b - s*a
+// A box's NCELL words are a whole number of 16-byte words (every box size
+// here), so the operations run on 16-byte words: each lane moves 16 bytes
+// per access.
+template <typename T>
+struct alignas(16) Word16 {
+  T v[16 / sizeof(T)];
+};
 template <int NCELL, int BS, typename T, int BB = 1>
 __global__ void __launch_bounds__(BS * BB) k_boxop(const int *__restrict__ ids, const T *a, T *b, int op, T s, int bpb,
                                                    int nw) {
+  using V = Word16<T>;
+  constexpr int NL = 16 / sizeof(T);
+  static_assert((NCELL * sizeof(T)) % 16 == 0, "boxes of whole 16-byte words");
+  constexpr int NV = NCELL / NL;
   for (int w = item0<BS, BB>(); w < nw; w += item_stride<BB>()) {
     const Part pt = part_of<BS>(ids, bpb, w);
     if (pt.id <= 0) continue;  // run masked out
-    const size_t base = (size_t)(pt.id - 1) * NCELL;
-    for (int x = pt.q0; x < NCELL; x += pt.stride) {
+    const V *av = reinterpret_cast<const V *>(a + (size_t)(pt.id - 1) * NCELL);
+    V *bv = reinterpret_cast<V *>(b + (size_t)(pt.id - 1) * NCELL);
+    for (int x = pt.q0; x < NV; x += pt.stride) {
+      V r;
       switch (op) {
-        case OP_COPY: b[base + x] = a[base + x]; break;
-        case OP_CLEAR: b[base + x] = T(0); break;
-        case OP_SUB_FROM: b[base + x] = a[base + x] - b[base + x]; break;
-        default: b[base + x] = b[base + x] - s * a[base + x]; break;
+        case OP_COPY: r = av[x]; break;
+        case OP_CLEAR:
+          for (int l = 0; l < NL; l++) r.v[l] = T(0);
+          break;
+        case OP_SUB_FROM: {
+          const V p = av[x], q = bv[x];
+          for (int l = 0; l < NL; l++) r.v[l] = p.v[l] - q.v[l];
+          break;
+        }
+        default: {
+          const V p = av[x], q = bv[x];
+          for (int l = 0; l < NL; l++) r.v[l] = q.v[l] - s * p.v[l];
+          break;
+        }
       }
+      bv[x] = r;
     }
   }
 }
@@ -1168,6 +1210,149 @@ __global__ void __launch_bounds__(BS * BB) k_gsrb_lds(const int *__restrict__ id
       }
     }
     gsync<BS, BB>();  // (the group's LDS is reused by its next item)
+  }
+}
+
+// k_gsrb_lds for 3D boxes in the colour-split single-precision layout.  A
+// half-sweep reads only the other colour's interior, the face ghosts next to
+// the updated cells and the updated colour's rhs; the updated colour's old
+// values are never read.  The other colour's block is staged in LDS as it
+// lies in the pool (16-byte loads, no reordering), the ghosts in a face
+// array, and each thread updates four consecutive cells of the colour block
+// per pass and stores them as one 16-byte word.  The stencil expression and
+// its operands are those of k_gsrb_lds, so the results are identical; the
+// box's registers and LDS (14 KB) are a fraction of k_gsrb_lds's, so all
+// boxes of a finest level run at once.
+template <int D, int NC, int BS>
+__global__ void __launch_bounds__(BS) k_gsrb_split(const int *__restrict__ ids, const DBox *__restrict__ boxes,
+                                                   float *phi, const float *__restrict__ rhs, int redblack,
+                                                   float lambda2, VarBc m, int nw) {
+  static_assert(D == 3, "the colour blocks of 3D boxes");
+  using G = Geo<D, NC>;
+  using L = Lay<D, NC, float>;
+  constexpr int HALF = L::HALF;
+  constexpr int NV = HALF / 4;               // 16-byte words of a colour block
+  constexpr int NW = NV / BS;                // per thread
+  constexpr int NGH = D * G::NTAN;           // ghosts next to updated cells: half of each face
+  constexpr int NGT = (NGH + BS - 1) / BS;   // per thread
+  static_assert(NV % BS == 0 && (L::STRIDE * sizeof(float)) % 16 == 0, "16-byte words of whole boxes");
+  __shared__ float4 s_other4[NV];
+  __shared__ float4 s_rhs4[NV];
+  __shared__ float s_gh[2 * D][G::NTAN];
+  const float *s_other = reinterpret_cast<const float *>(s_other4);
+  const int t = threadIdx.x, wv = __builtin_amdgcn_readfirstlane(t / 64);  // (the wavefront)
+  const int uc = redblack & 1, oc = 1 - uc;  // updated cells: (i + j + k + redblack) even
+  for (int w = blockIdx.x; w < nw; w += gridDim.x) {
+    const int id = ids[w];
+    if (id <= 0) continue;  // run masked out
+    const DBox &bx = boxes[id - 1];
+    const Sten<D, float> st = lpl_stencil<D, float>(bx, lambda2);
+    float *u = phi + (size_t)(id - 1) * L::STRIDE;
+    const float4 *ob = reinterpret_cast<const float4 *>(u + oc * HALF);
+    const float4 *rb = reinterpret_cast<const float4 *>(rhs + (size_t)(id - 1) * L::STRIDE + uc * HALF);
+    // the other colour's block and the rhs straight into LDS (each lane
+    // 16 bytes, a wavefront 1 KB; no registers held)
+    for (int q = 0; q < NW; q++) {
+      __builtin_amdgcn_global_load_lds((const void *)(ob + t + q * BS),
+                                       (__attribute__((address_space(3))) void *)(s_other4 + 64 * wv + q * BS), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void *)(rb + t + q * BS),
+                                       (__attribute__((address_space(3))) void *)(s_rhs4 + 64 * wv + q * BS), 16, 0, 0);
+    }
+    int nbid[2 * D];
+    for (int nb = 0; nb < 2 * D; nb++) nbid[nb] = bx.nb[nb];
+    // the ghosts next to updated cells, item g: face nb (a face's NTAN / 2
+    // ghosts are whole wavefronts, so nb is uniform), tangential (a, b)
+    auto ghost_pos = [&](int g, int &nb, int &a, int &b) {
+      nb = __builtin_amdgcn_readfirstlane(g / (G::NTAN / 2));
+      const int e = g % (G::NTAN / 2);
+      const int pd = (nb % 2) == 0 ? 1 : NC;
+      b = e / (NC / 2) + 1;
+      a = 2 * (e % (NC / 2)) + 1 + ((uc + pd + b + 1) & 1);  // (pd + a + b + uc) even
+    };
+    // every item's loads first: the same-level neighbour's cell across the
+    // face, or the coarse values and the box's two cells next to the face
+    float x0[NGT], x1[NGT], x2[NGT];
+    CoarseVals<float> cv[NGT];
+    for (int q = 0; q < NGT; q++) {
+      const int g = t + q * BS;
+      int nb, a, b;
+      ghost_pos(g, nb, a, b);
+      if (g >= NGH) continue;
+      const bool low = (nb % 2) == 0;
+      int p[3];
+      if (nbid[nb] > 0) {
+        face_pos<D>(nb, a, b, low ? NC : 1, p);
+        x0[q] = phi[(size_t)(nbid[nb] - 1) * L::STRIDE + L::in_at(p[0], p[1], p[2])];
+      } else {
+        cv[q] = ghost_coarse_load<D, NC, float>(bx, phi, m, nb, a, b);
+        face_pos<D>(nb, a, b, low ? 1 : NC, p);
+        x1[q] = u[L::in_at(p[0], p[1], p[2])];
+        face_pos<D>(nb, a, b, low ? 2 : NC - 1, p);
+        x2[q] = u[L::in_at(p[0], p[1], p[2])];
+      }
+    }
+    for (int q = 0; q < NGT; q++) {
+      const int g = t + q * BS;
+      int nb, a, b;
+      ghost_pos(g, nb, a, b);
+      if (g >= NGH) continue;
+      s_gh[nb][(a - 1) + NC * (b - 1)] =
+          nbid[nb] > 0 ? x0[q]
+                       : ghost_finish<D, NC, float>(bx, m, nb, ghost_coarse_form<D, NC, float>(bx, m, nb, a, b, cv[q]),
+                                                    x1[q], x2[q]);
+    }
+    __syncthreads();
+    const float inv_c1 = 1.0f / st.c[0];
+    float4 *ub = reinterpret_cast<float4 *>(u + uc * HALF);
+    for (int q = 0; q < NW; q++) {
+      // four cells: half of a row (j, k) of the colour block, at
+      // i = i0 + 2 (m0 + c); their other-colour neighbours in the row are
+      // the five values from B on, in the rows and planes beside it four
+      // consecutive values (one 16-byte word), unless they are ghosts
+      const int h0 = 4 * (t + q * BS);
+      const int row = h0 / (NC / 2), m0 = h0 % (NC / 2);
+      const int j = row % NC + 1, k = row / NC + 1;
+      const int i0 = 1 + ((1 + j + k + uc) & 1);
+      const int R = row * (NC / 2);
+      const int B = R + m0 + i0 - 2;
+      // (one 16-byte word at R + m0 and one value before or after it:
+      // consecutive lanes read consecutive words, free of bank conflicts)
+      const bool lo_gh = i0 == 1 && m0 == 0, hi_gh = i0 == 2 && m0 == NC / 2 - 4;
+      const float4 xc = s_other4[(R + m0) / 4];
+      const float xe = (i0 == 1) ? (lo_gh ? s_gh[0][row] : s_other[max(B, 0)])
+                                 : (hi_gh ? s_gh[1][row] : s_other[min(B + 4, HALF - 1)]);
+      float xw[5];
+      if (i0 == 1) {
+        xw[0] = xe; xw[1] = xc.x; xw[2] = xc.y; xw[3] = xc.z; xw[4] = xc.w;
+      } else {
+        xw[0] = xc.x; xw[1] = xc.y; xw[2] = xc.z; xw[3] = xc.w; xw[4] = xe;
+      }
+      float ym[4], yp[4], zm[4], zp[4];
+      auto four = [&](float *v, bool ghost, int nb, int r0, int off) {
+        if (ghost) {
+          for (int c = 0; c < 4; c++) v[c] = s_gh[nb][r0 + i0 - 1 + 2 * (m0 + c)];
+        } else {
+          const float4 w = s_other4[min(max(R + off + m0, 0), HALF - 4) / 4];
+          v[0] = w.x;
+          v[1] = w.y;
+          v[2] = w.z;
+          v[3] = w.w;
+        }
+      };
+      four(ym, j == 1, 2, NC * (k - 1), -NC / 2);
+      four(yp, j == NC, 3, NC * (k - 1), NC / 2);
+      four(zm, k == 1, 4, NC * (j - 1), -NC * NC / 2);
+      four(zp, k == NC, 5, NC * (j - 1), NC * NC / 2);
+      const float4 rv = s_rhs4[t + q * BS];
+      const float rr[4] = {rv.x, rv.y, rv.z, rv.w};
+      float res[4];
+      for (int c = 0; c < 4; c++)
+        res[c] = (rr[c] - st.c[1] * xw[c] - st.c[2] * xw[c + 1] - st.c[3] * ym[c] - st.c[4] * yp[c] -
+                  st.c[5] * zm[c] - st.c[6] * zp[c]) *
+                 inv_c1;
+      ub[t + q * BS] = make_float4(res[0], res[1], res[2], res[3]);
+    }
+    __syncthreads();  // (the LDS is reused by the next item)
   }
 }
 

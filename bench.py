@@ -53,6 +53,7 @@ def make_case(args):
 
 FORCED_SHAPES = None   # class -> (boxes per workgroup, workgroups per CU)
 SAVE_SHAPES = None
+BOX_LDS = 0            # AFGPU_OPT_BOX_LDS of every handle
 
 
 def tune(p, enabled):
@@ -80,6 +81,7 @@ def run_gpu(case, precision, warmup, steps, timing_steps, direct_n=None, smooth=
             photoi_vcycle=None, autotune=True, fuse_direct=None, direct_f32=None):
     p = A.Path("gpu", case.D, case.nc, case.coord, precision)
     S.apply(p, case)
+    p.set_option(A.OPT_BOX_LDS, BOX_LDS)
     if fuse_direct is not None:
         p.set_option(A.OPT_FUSE_DIRECT, fuse_direct)
     if direct_f32 is not None:
@@ -559,6 +561,9 @@ def main():
                          "kernel class, or a JSON file of class -> [bb, per_cu] as --save-shapes writes")
     ap.add_argument("--save-shapes", default="", metavar="FILE",
                     help="write the autotuner's shapes (of the run measured first) to FILE")
+    ap.add_argument("--box-lds", type=int, default=0,
+                    help="mixed precision, 3D 16^3 boxes: stage whole boxes in LDS in the half-sweeps and the FP64 "
+                         "residual (1) instead of only what they read (0); identical results")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64-ref", action="store_true")
     ap.add_argument("--no-variants", action="store_true",
@@ -567,7 +572,8 @@ def main():
                     help="skip the measurements beyond the time step (refinement flags, diagnostics, chemistry, "
                          "new boxes), e.g. to profile the time step alone")
     args = ap.parse_args()
-    global FORCED_SHAPES, SAVE_SHAPES
+    global FORCED_SHAPES, SAVE_SHAPES, BOX_LDS
+    BOX_LDS = args.box_lds
     if args.shapes:
         if os.path.exists(args.shapes):
             with open(args.shapes) as f:
@@ -729,6 +735,7 @@ def main():
                    "block_smooth": args.block_smooth if prec == A.PREC_MIXED else None,
                    "batch_modes": args.batch_modes if prec == A.PREC_MIXED else None,
                    "photoi_vcycle": args.photoi_vcycle if prec == A.PREC_MIXED else None,
+                   "box_lds": args.box_lds if prec == A.PREC_MIXED else None,
                    "fuse_direct": args.fuse_direct if prec == A.PREC_MIXED else None,
                    "direct_f32": args.direct_f32 if prec == A.PREC_MIXED else None,
                    "shapes": p.shapes},

@@ -411,6 +411,10 @@ int afgpu_kernel_stats(afgpu_t *h, int kind, int reset, double *total_ms,
                                    (1) or FP64 (0); the FP64 outer correction sets the accuracy
                                    either way (default 0: the coarsest-grid solve stays FP64;
                                    0 whenever AFGPU_OPT_FUSE_DIRECT is 1) */
+#define AFGPU_OPT_BOX_LDS 12 /* mixed precision, 3D boxes of 16^3 cells: the inner cycle's
+                               half-sweeps stage the whole box in LDS (1), or only the other
+                               colour's block as it is stored and the rhs (0); identical
+                               results (default 0) */
 int afgpu_set_option(afgpu_t *h, int option, int value);
 
 /* ---- ensembles of independent runs --------------------------------------

@@ -246,6 +246,33 @@ def test_mixed_direct_solve_fused_passes_bit_for_bit(A, S):
         assert np.array_equal(a, b)
 
 
+def test_split_half_sweeps_bit_for_bit(A, S):
+    """The inner cycle's half-sweeps staging only the other colour's block
+    and the rhs (k_gsrb_split, the default for 3D 16^3 boxes) give the same
+    bits as the half-sweeps staging the whole box in LDS (k_gsrb_lds).
+    They run on levels of at least 256 boxes: the channel tree's level 5
+    (1536 boxes, refinement boundaries and the domain's faces), and a
+    uniform level 4 (512 boxes) smoothed above a 64^3 direct solve; Poisson
+    and the photoionization modes (Helmholtz) over a few steps."""
+    cases = ((setup(S.config1(photoi=True), photoi=True), {}),
+             (setup(S.small_amr(3, nc=16, uniform_lvl=4, max_lvl=5, frac=0.05, seed=6), photoi=True),
+              {A.OPT_DIRECT_N: 64}))
+    for case, opts in cases:
+        runs = []
+        for box_lds in (0, 1):
+            print(case.name, "box_lds", box_lds, flush=True)
+            m = S.apply(A.Path("gpu", case.D, case.nc, case.coord, A.PREC_MIXED), case)
+            for k, v in opts.items():
+                m.set_option(k, v)
+            m.set_option(A.OPT_BOX_LDS, box_lds)
+            infos = [m.field_compute(0, False)] + [m.step() for _ in range(2)]
+            runs.append((infos, [m.get_cc(iv) for iv in (A.CC_PHI, A.CC_EFIELD, A.CC_ELECTRON, A.CC_PHOTO)]))
+            m.close()
+        assert runs[0][0] == runs[1][0], case.name
+        for a, b in zip(runs[0][1], runs[1][1]):
+            assert np.array_equal(a, b), case.name
+
+
 def test_mixed_direct_solve_f32_transforms(A, S):
     """The inner cycle's uniform-level solve in FP32 (the default) against
     the same solve in FP64: 3D with photoionization (level 4 = 128^3), a few
