@@ -1099,6 +1099,8 @@ struct Impl : ImplBase {
   // scatter fused), and the last forward transform, the division and the
   // first backward one run as one pass when that pass has a workgroup per
   // CU at least.
+  // wavefronts per 64 x 64 tile of the matrix-core transforms
+  static constexpr int LC_WAVES = 8;
   template <int N, typename T>
   int direct_transforms(const afgpu_ctx::Direct::F &f, double lambda2, double *A0, double *A1,
                         const double **result, const T *rhs_pool = nullptr, T *phi_pool = nullptr) {
@@ -1129,13 +1131,13 @@ struct Impl : ImplBase {
     T *noout = nullptr;
     auto rows = [&](const double *in, double *out, const double *Mx, const double *MTx, int io) {
       if (MC && io == 1)
-        hipLaunchKernelGGL((k_lc_mfma<D, NM, true, 1, NC, T>), dim3(mfma_row_blocks), dim3(256), 0, c.cur, in, out,
+        hipLaunchKernelGGL((k_lc_mfma<D, NM, true, 1, NC, T, LC_WAVES>), dim3(mfma_row_blocks), dim3(64 * LC_WAVES), 0, c.cur, in, out,
                            Mx, 0, 0, (const double *)f.eig, lambda2, map, nb, rhs_pool, noout);
       else if (MC && io == 2)
-        hipLaunchKernelGGL((k_lc_mfma<D, NM, true, 2, NC, T>), dim3(mfma_row_blocks), dim3(256), 0, c.cur, in, out,
+        hipLaunchKernelGGL((k_lc_mfma<D, NM, true, 2, NC, T, LC_WAVES>), dim3(mfma_row_blocks), dim3(64 * LC_WAVES), 0, c.cur, in, out,
                            Mx, 0, 0, (const double *)f.eig, lambda2, map, nb, noin, phi_pool);
       else if (MC)
-        hipLaunchKernelGGL((k_lc_mfma<D, NM, true, 0, NC, T>), dim3(mfma_row_blocks), dim3(256), 0, c.cur, in, out,
+        hipLaunchKernelGGL((k_lc_mfma<D, NM, true, 0, NC, T, LC_WAVES>), dim3(mfma_row_blocks), dim3(64 * LC_WAVES), 0, c.cur, in, out,
                            Mx, 0, 0, (const double *)f.eig, lambda2, nomap, 0, noin, noout);
       else if (TILED)
         hipLaunchKernelGGL((k_lc_gemm<D, NM, true>), dim3(gemm_row_blocks), dim3(256), 0, c.cur, in, out, Mx, 0, 0,
@@ -1146,7 +1148,7 @@ struct Impl : ImplBase {
     };
     auto lines = [&](const double *in, double *out, const double *M, int axis, int div) {
       if (MC)
-        hipLaunchKernelGGL((k_lc_mfma<D, NM, false, 0, NC, T>), dim3(mfma_line_blocks), dim3(256), 0, c.cur, in,
+        hipLaunchKernelGGL((k_lc_mfma<D, NM, false, 0, NC, T, LC_WAVES>), dim3(mfma_line_blocks), dim3(64 * LC_WAVES), 0, c.cur, in,
                            out, M, axis, div, (const double *)f.eig, lambda2, nomap, 0, noin, noout);
       else if (TILED)
         hipLaunchKernelGGL((k_lc_gemm<D, NM, false>), dim3(gemm_line_blocks), dim3(256), 0, c.cur, in, out, M, axis,

@@ -298,16 +298,23 @@ __device__ inline long long lc_pool_offset(const int *__restrict__ map, int nb, 
 // IO (x form only): 0 grid to grid; 1 the input rows read from the box
 // pool `pin` (the gather, fused); 2 the output rows written to the box pool
 // `pout` (the scatter, fused).  Box map and boxes per axis as k_lc_gather.
-template <int D, int N, bool ROWS, int IO = 0, int NC = 16, typename TS = float>
-__global__ void __launch_bounds__(256) k_lc_mfma(const double *__restrict__ in, double *__restrict__ out,
-                                                 const double *__restrict__ M, int axis, int div,
-                                                 const double *__restrict__ eig, double lambda2,
-                                                 const int *__restrict__ map, int nb, const TS *__restrict__ pin,
-                                                 TS *__restrict__ pout) {
+// WAVES wavefronts share a 64 x 64 output tile: two rows of WAVES / 2, each
+// wave 32 x (128 / WAVES) (at 8, two waves per SIMD in every workgroup, for
+// the few workgroups a pass has: (N / 64)^2 N).  Every output element is
+// one wave's sum over K in the same order whatever WAVES is.
+template <int D, int N, bool ROWS, int IO = 0, int NC = 16, typename TS = float, int WAVES = 8>
+__global__ void __launch_bounds__(64 * WAVES) k_lc_mfma(const double *__restrict__ in, double *__restrict__ out,
+                                                        const double *__restrict__ M, int axis, int div,
+                                                        const double *__restrict__ eig, double lambda2,
+                                                        const int *__restrict__ map, int nb,
+                                                        const TS *__restrict__ pin, TS *__restrict__ pout) {
   static_assert(IO == 0 || ROWS, "box-pool input and output in the x form only");
   static_assert(N == 64 || N == 128, "matrix-core transform for N = 64, 128");
+  static_assert(WAVES == 4 || WAVES == 8, "4 or 8 wavefronts per 64 x 64 tile");
+  constexpr int BT = 64 * WAVES;
   constexpr int T = 64, KC = 32, AP = KC + 1, BP = T + 1, NT = N / T;
-  constexpr int PER = T * KC / 256;  // values of each operand tile per thread
+  constexpr int PER = T * KC / BT;  // values of each operand tile per thread
+  constexpr int WQ = WAVES / 2, QW = T / WQ, NTB = QW / 16;  // waves along q, their width, 16-wide tiles
   __shared__ double As[T * AP];   // As[p][kk]
   __shared__ double Bs[KC * BP];  // Bs[kk][q]
   // blocks: ROWS: (output tiles) x (row tiles); else (output tiles) x
@@ -328,13 +335,13 @@ __global__ void __launch_bounds__(256) k_lc_mfma(const double *__restrict__ in, 
     base = (size_t)o * so + x0;
   }
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int wp = (w >> 1) * 32, wq = (w & 1) * 32;
+  const int wp = (w / WQ) * 32, wq = (w % WQ) * QW;
   // one K chunk of both operands, global -> registers (consecutive threads
   // take consecutive addresses) and registers -> LDS
   double pa[PER], pb[PER];
   auto fetch = [&](int k0) {
     for (int u = 0; u < PER; u++) {
-      const int q = tid + 256 * u;
+      const int q = tid + BT * u;
       const int pp = q / KC, kk = q % KC;
       if (IO == 1) {
         const int row = p0 + pp;
@@ -353,7 +360,7 @@ __global__ void __launch_bounds__(256) k_lc_mfma(const double *__restrict__ in, 
   };
   auto stash = [&]() {
     for (int u = 0; u < PER; u++) {
-      const int q = tid + 256 * u;
+      const int q = tid + BT * u;
       As[(q / KC) * AP + q % KC] = pa[u];
       if (ROWS)
         Bs[(q % KC) * BP + q / KC] = pb[u];
@@ -361,9 +368,9 @@ __global__ void __launch_bounds__(256) k_lc_mfma(const double *__restrict__ in, 
         Bs[(q / T) * BP + q % T] = pb[u];
     }
   };
-  lc_double4 acc[2][2];
+  lc_double4 acc[2][NTB];
   for (int a = 0; a < 2; a++)
-    for (int b = 0; b < 2; b++) acc[a][b] = lc_double4{0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < NTB; b++) acc[a][b] = lc_double4{0.0, 0.0, 0.0, 0.0};
   fetch(0);
   for (int k0 = 0; k0 < N; k0 += KC) {
     stash();
@@ -371,19 +378,17 @@ __global__ void __launch_bounds__(256) k_lc_mfma(const double *__restrict__ in, 
     if (k0 + KC < N) fetch(k0 + KC);
     for (int ks = 0; ks < KC; ks += 4) {
       const int kk = ks + (lane >> 4);
-      double a[2], b[2];
-      for (int t = 0; t < 2; t++) {
-        a[t] = As[(wp + 16 * t + (lane & 15)) * AP + kk];
-        b[t] = Bs[kk * BP + wq + 16 * t + (lane & 15)];
-      }
+      double a[2], b[NTB];
+      for (int t = 0; t < 2; t++) a[t] = As[(wp + 16 * t + (lane & 15)) * AP + kk];
+      for (int t = 0; t < NTB; t++) b[t] = Bs[kk * BP + wq + 16 * t + (lane & 15)];
       for (int ta = 0; ta < 2; ta++)
-        for (int tb = 0; tb < 2; tb++)
+        for (int tb = 0; tb < NTB; tb++)
           acc[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta], b[tb], acc[ta][tb], 0, 0, 0);
     }
     __syncthreads();
   }
   for (int ta = 0; ta < 2; ta++)
-    for (int tb = 0; tb < 2; tb++)
+    for (int tb = 0; tb < NTB; tb++)
       for (int j = 0; j < 4; j++) {
         const int pp = p0 + wp + 16 * ta + 4 * j + (lane >> 4);
         const int qq = q0 + wq + 16 * tb + (lane & 15);
