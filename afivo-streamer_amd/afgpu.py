@@ -53,6 +53,7 @@ OPT_STEP_GRAPH = 9
 OPT_FUSE_DIRECT = 10
 OPT_DIRECT_F32 = 11
 OPT_BOX_LDS = 12
+OPT_DIRECT_FOLD = 13
 
 
 class ParamsT(C.Structure):

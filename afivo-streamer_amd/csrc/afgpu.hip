@@ -244,6 +244,7 @@ struct afgpu_ctx {
   bool box_lds = false;  // mixed precision, 3D 16^3 boxes: whole boxes staged in LDS by the half-sweeps
   bool fuse_direct = false;  // uniform-level solve: last-axis transforms fused into one pass
   bool direct_f32 = false;   // uniform-level solve of the FP32 inner cycle in FP32 (else FP64; matrix cores)
+  bool direct_fold = true;   // uniform-level solve: reflection-symmetric axes transformed by parity (half the products)
   // refinement flags: the boxes evaluated and their flags (device)
   // a reaction list in place of the standard model; with more species than
   // the three pools (or other charges), the species beyond the pools live
@@ -281,6 +282,10 @@ struct afgpu_ctx {
       bool ok = false;
       double *vinv = nullptr, *v = nullptr, *vinvT = nullptr, *vT = nullptr, *eig = nullptr;
       float *vinv32 = nullptr, *v32 = nullptr, *eig32 = nullptr;  // (rounded copies)
+      // parity-folded factors (k_lc_mfma FOLD): per axis N x N/2 forward,
+      // N/2 x N backward, the eigenvalues in the folded modes' order
+      bool fold[3] = {false, false, false};
+      double *mf = nullptr, *bf = nullptr, *eigf = nullptr;
     } f[1 + AFGPU_MAX_MODES];
   } dir;
   // reductions
@@ -1071,9 +1076,60 @@ struct Impl : ImplBase {
           vinvT[o + (size_t)m * N + i] = vinv[o + (size_t)i * N + m];
           vT[o + (size_t)m * N + i] = v[o + (size_t)i * N + m];
         }
-    double **dst[5] = {&f.vinv, &f.v, &f.vinvT, &f.vT, &f.eig};
-    const std::vector<double> *srcs[5] = {&vinv, &v, &vinvT, &vT, &eig};
-    for (int q = 0; q < 5; q++) {
+    // Axes whose 1-D operator is symmetric under reversal (Cartesian, the
+    // same boundary condition at both ends) have even and odd eigenvectors:
+    // the folded factors list the even modes first, then the odd ones.
+    const int H = N / 2;
+    std::vector<double> mf((size_t)D * N * H, 0.0), bf((size_t)D * H * N, 0.0), eigf(eig);
+    for (int dd = 0; dd < D; dd++) {
+      f.fold[dd] = false;
+      if (N != 128 || (c.coord == AFGPU_CYL && dd == 0) || bcd[2 * dd] != bcd[2 * dd + 1]) continue;
+      const double *Vd = v.data() + (size_t)dd * N * N;     // V[m][i]: component m of eigenvector i
+      const double *Wd = vinv.data() + (size_t)dd * N * N;  // V^-1[i][m]
+      std::vector<int> even, odd;
+      bool ok = true;
+      for (int i = 0; i < N && ok; i++) {
+        double se = 0, so = 0, nv = 0, we = 0, wo = 0, nw = 0;
+        for (int m = 0; m < N; m++) {
+          const double a = Vd[(size_t)m * N + i], b = Vd[(size_t)(N - 1 - m) * N + i];
+          const double x = Wd[(size_t)i * N + m], y = Wd[(size_t)i * N + N - 1 - m];
+          se += (a - b) * (a - b);
+          so += (a + b) * (a + b);
+          nv += a * a;
+          we += (x - y) * (x - y);
+          wo += (x + y) * (x + y);
+          nw += x * x;
+        }
+        // (the eigenvectors come out symmetric or antisymmetric to ~1e-10:
+        // the other measure is then ~4 times the norm)
+        if (se <= 1e-12 * nv && we <= 1e-12 * nw)
+          even.push_back(i);
+        else if (so <= 1e-12 * nv && wo <= 1e-12 * nw)
+          odd.push_back(i);
+        else
+          ok = false;
+      }
+      if (!ok || (int)even.size() != H) continue;
+      std::vector<int> mode(even);
+      mode.insert(mode.end(), odd.begin(), odd.end());
+      // exactly (anti)symmetric halves: the mean of each value and its mirror
+      for (int r = 0; r < N; r++) {
+        const double sg = (r < H) ? 1.0 : -1.0;
+        const double *w = Wd + (size_t)mode[r] * N;
+        for (int m = 0; m < H; m++) mf[(size_t)dd * N * H + (size_t)r * H + m] = 0.5 * (w[m] + sg * w[N - 1 - m]);
+        eigf[(size_t)dd * N + r] = eig[(size_t)dd * N + mode[r]];
+      }
+      for (int m = 0; m < H; m++)
+        for (int r = 0; r < N; r++) {
+          const double sg = (r < H) ? 1.0 : -1.0;
+          bf[(size_t)dd * H * N + (size_t)m * N + r] =
+              0.5 * (Vd[(size_t)m * N + mode[r]] + sg * Vd[(size_t)(N - 1 - m) * N + mode[r]]);
+        }
+      f.fold[dd] = true;
+    }
+    double **dst[8] = {&f.vinv, &f.v, &f.vinvT, &f.vT, &f.eig, &f.mf, &f.bf, &f.eigf};
+    const std::vector<double> *srcs[8] = {&vinv, &v, &vinvT, &vT, &eig, &mf, &bf, &eigf};
+    for (int q = 0; q < 8; q++) {
       (void)hipFree(*dst[q]);
       *dst[q] = nullptr;
       HIPCHK(hipMalloc(dst[q], srcs[q]->size() * sizeof(double)));
@@ -1129,39 +1185,77 @@ struct Impl : ImplBase {
     const int *nomap = nullptr;
     const T *noin = nullptr;
     T *noout = nullptr;
+    // parity-folded transforms (FOLD 1 forward, 2 backward) of the axes
+    // that allow them; the eigenvalue sums then follow the folded order
+    const bool folding = MC && NM == 128 && c.direct_fold && (f.fold[0] || f.fold[1] || (D == 3 && f.fold[2]));
+    const double *eigs = folding ? (const double *)f.eigf : (const double *)f.eig;
+    auto rows_fold = [&](const double *in, double *out, int io, int fold) {
+      if constexpr (NM == 128) {
+        const double *Mx = (fold == 1) ? f.mf : f.bf;
+        const int blocks = (fold == 2) ? nrows / 64 : mfma_row_blocks;
+        if (io == 1)
+          hipLaunchKernelGGL((k_lc_mfma<D, NM, true, 1, NC, T, LC_WAVES, 1>), dim3(blocks), dim3(64 * LC_WAVES), 0,
+                             c.cur, in, out, Mx, 0, 0, eigs, lambda2, map, nb, rhs_pool, noout);
+        else if (io == 2)
+          hipLaunchKernelGGL((k_lc_mfma<D, NM, true, 2, NC, T, LC_WAVES, 2>), dim3(blocks), dim3(64 * LC_WAVES), 0,
+                             c.cur, in, out, Mx, 0, 0, eigs, lambda2, map, nb, noin, phi_pool);
+        else if (fold == 1)
+          hipLaunchKernelGGL((k_lc_mfma<D, NM, true, 0, NC, T, LC_WAVES, 1>), dim3(blocks), dim3(64 * LC_WAVES), 0,
+                             c.cur, in, out, Mx, 0, 0, eigs, lambda2, nomap, 0, noin, noout);
+        else
+          hipLaunchKernelGGL((k_lc_mfma<D, NM, true, 0, NC, T, LC_WAVES, 2>), dim3(blocks), dim3(64 * LC_WAVES), 0,
+                             c.cur, in, out, Mx, 0, 0, eigs, lambda2, nomap, 0, noin, noout);
+      }
+      return hipGetLastError();
+    };
+    auto lines_fold = [&](const double *in, double *out, int axis, int div, int fold) {
+      if constexpr (NM == 128) {
+        const double *Mx = ((fold == 1) ? f.mf : f.bf) + (size_t)axis * N * N / 2;
+        const int blocks = (fold == 2) ? mfma_line_blocks / (N / 64) : mfma_line_blocks;
+        if (fold == 1)
+          hipLaunchKernelGGL((k_lc_mfma<D, NM, false, 0, NC, T, LC_WAVES, 1>), dim3(blocks), dim3(64 * LC_WAVES),
+                             0, c.cur, in, out, Mx, axis, div, eigs, lambda2, nomap, 0, noin, noout);
+        else
+          hipLaunchKernelGGL((k_lc_mfma<D, NM, false, 0, NC, T, LC_WAVES, 2>), dim3(blocks), dim3(64 * LC_WAVES),
+                             0, c.cur, in, out, Mx, axis, div, eigs, lambda2, nomap, 0, noin, noout);
+      }
+      return hipGetLastError();
+    };
     auto rows = [&](const double *in, double *out, const double *Mx, const double *MTx, int io) {
+      if (folding && f.fold[0]) return rows_fold(in, out, io, Mx == f.vinv ? 1 : 2);
       if (MC && io == 1)
         hipLaunchKernelGGL((k_lc_mfma<D, NM, true, 1, NC, T, LC_WAVES>), dim3(mfma_row_blocks), dim3(64 * LC_WAVES), 0, c.cur, in, out,
-                           Mx, 0, 0, (const double *)f.eig, lambda2, map, nb, rhs_pool, noout);
+                           Mx, 0, 0, eigs, lambda2, map, nb, rhs_pool, noout);
       else if (MC && io == 2)
         hipLaunchKernelGGL((k_lc_mfma<D, NM, true, 2, NC, T, LC_WAVES>), dim3(mfma_row_blocks), dim3(64 * LC_WAVES), 0, c.cur, in, out,
-                           Mx, 0, 0, (const double *)f.eig, lambda2, map, nb, noin, phi_pool);
+                           Mx, 0, 0, eigs, lambda2, map, nb, noin, phi_pool);
       else if (MC)
         hipLaunchKernelGGL((k_lc_mfma<D, NM, true, 0, NC, T, LC_WAVES>), dim3(mfma_row_blocks), dim3(64 * LC_WAVES), 0, c.cur, in, out,
-                           Mx, 0, 0, (const double *)f.eig, lambda2, nomap, 0, noin, noout);
+                           Mx, 0, 0, eigs, lambda2, nomap, 0, noin, noout);
       else if (TILED)
         hipLaunchKernelGGL((k_lc_gemm<D, NM, true>), dim3(gemm_row_blocks), dim3(256), 0, c.cur, in, out, Mx, 0, 0,
-                           (const double *)f.eig, lambda2);
+                           eigs, lambda2);
       else
         hipLaunchKernelGGL((k_lc_rows<D, N>), dim3(row_blocks), dim3(256), 0, c.cur, in, out, MTx, nrows);
       return hipGetLastError();
     };
     auto lines = [&](const double *in, double *out, const double *M, int axis, int div) {
+      if (folding && f.fold[axis]) return lines_fold(in, out, axis, div, M == f.vinv + (size_t)axis * NN ? 1 : 2);
       if (MC)
         hipLaunchKernelGGL((k_lc_mfma<D, NM, false, 0, NC, T, LC_WAVES>), dim3(mfma_line_blocks), dim3(64 * LC_WAVES), 0, c.cur, in,
-                           out, M, axis, div, (const double *)f.eig, lambda2, nomap, 0, noin, noout);
+                           out, M, axis, div, eigs, lambda2, nomap, 0, noin, noout);
       else if (TILED)
         hipLaunchKernelGGL((k_lc_gemm<D, NM, false>), dim3(gemm_line_blocks), dim3(256), 0, c.cur, in, out, M, axis,
-                           div, (const double *)f.eig, lambda2);
+                           div, eigs, lambda2);
       else
         hipLaunchKernelGGL((k_lc_lines<D, N>), dim3(line_blocks), dim3(256), 0, c.cur, in, out, M, axis, div,
-                           (const double *)f.eig, lambda2);
+                           eigs, lambda2);
       return hipGetLastError();
     };
     // the last forward, the division and the first backward transform
     // along the last axis: one pass, or two
     const int fused_blocks = (N / 64) * batch;
-    const bool fuse = MC && c.fuse_direct && fused_blocks >= c.n_cu;
+    const bool fuse = MC && c.fuse_direct && !folding && fused_blocks >= c.n_cu;
     const int last = D - 1;
     const double *vinv_l = f.vinv + (size_t)last * NN, *v_l = f.v + (size_t)last * NN;
     // ping-pong between the grids: `cur` holds the latest result
@@ -2420,6 +2514,9 @@ int afgpu_destroy(afgpu_t *h) {
     (void)hipFree(f.vinv32);
     (void)hipFree(f.v32);
     (void)hipFree(f.eig32);
+    (void)hipFree(f.mf);
+    (void)hipFree(f.bf);
+    (void)hipFree(f.eigf);
   }
   (void)hipFree(h->d_boxes);
   (void)hipFree(h->d_lists);
@@ -3757,6 +3854,11 @@ int afgpu_set_option(afgpu_t *h, int option, int value) {
       RCHK(sync_stream(h));
       drop_graphs(*h);
       h->direct_f32 = value != 0;
+      return AFGPU_OK;
+    case AFGPU_OPT_DIRECT_FOLD:
+      RCHK(sync_stream(h));
+      drop_graphs(*h);
+      h->direct_fold = value != 0;
       return AFGPU_OK;
     case AFGPU_OPT_STEP_GRAPH:
       RCHK(sync_stream(h));

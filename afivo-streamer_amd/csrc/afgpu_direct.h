@@ -302,7 +302,17 @@ __device__ inline long long lc_pool_offset(const int *__restrict__ map, int nb, 
 // wave 32 x (128 / WAVES) (at 8, two waves per SIMD in every workgroup, for
 // the few workgroups a pass has: (N / 64)^2 N).  Every output element is
 // one wave's sum over K in the same order whatever WAVES is.
-template <int D, int N, bool ROWS, int IO = 0, int NC = 16, typename TS = float, int WAVES = 8>
+//
+// FOLD (N = 128, an axis whose 1-D operator is symmetric under reversal,
+// m -> N - 1 - m): its eigenvectors are even or odd, so a transform needs
+// half the products.  1, forward: M is N x N/2, the even modes' rows first
+// (M[i][m], m < N/2, of the even eigenvectors), then the odd ones'; the
+// even half of the output (the first 64-row tile) takes u_m + u_{N-1-m},
+// the odd half u_m - u_{N-1-m}, over K = N/2.  2, backward: M is N/2 x N,
+// M[m][i] for m < N/2 over the modes in the same order; the sums over the
+// even modes (E) and the odd ones (O) give u_m = E + O and
+// u_{N-1-m} = E - O, one 64-row output tile writing both halves.
+template <int D, int N, bool ROWS, int IO = 0, int NC = 16, typename TS = float, int WAVES = 8, int FOLD = 0>
 __global__ void __launch_bounds__(64 * WAVES) k_lc_mfma(const double *__restrict__ in, double *__restrict__ out,
                                                         const double *__restrict__ M, int axis, int div,
                                                         const double *__restrict__ eig, double lambda2,
@@ -311,15 +321,21 @@ __global__ void __launch_bounds__(64 * WAVES) k_lc_mfma(const double *__restrict
   static_assert(IO == 0 || ROWS, "box-pool input and output in the x form only");
   static_assert(N == 64 || N == 128, "matrix-core transform for N = 64, 128");
   static_assert(WAVES == 4 || WAVES == 8, "4 or 8 wavefronts per 64 x 64 tile");
+  static_assert(FOLD == 0 || N == 128, "the folded form has one 64-row tile per parity");
   constexpr int BT = 64 * WAVES;
   constexpr int T = 64, KC = 32, AP = KC + 1, BP = T + 1, NT = N / T;
+  constexpr int NTO = (FOLD == 2) ? 1 : NT;    // output tiles
+  constexpr int KN = (FOLD == 1) ? N / 2 : N;  // K extent
+  constexpr int MS = (FOLD == 1) ? N / 2 : N;  // row stride of M
+  constexpr int NS = (FOLD == 2) ? 2 : 1;      // accumulator sets (E, O)
   constexpr int PER = T * KC / BT;  // values of each operand tile per thread
   constexpr int WQ = WAVES / 2, QW = T / WQ, NTB = QW / 16;  // waves along q, their width, 16-wide tiles
   __shared__ double As[T * AP];   // As[p][kk]
   __shared__ double Bs[KC * BP];  // Bs[kk][q]
   // blocks: ROWS: (output tiles) x (row tiles); else (output tiles) x
   // (column tiles) x (the remaining coordinate)
-  const int t0 = blockIdx.x % NT, rest = blockIdx.x / NT;
+  const int t0 = blockIdx.x % NTO, rest = blockIdx.x / NTO;
+  const double sg = (t0 == 0) ? 1.0 : -1.0;  // the fold of the forward form: even, odd half
   size_t sm = 1, base = 0;
   int o = 0, x0 = 0, p0 = 0, q0 = 0;
   if (ROWS) {
@@ -343,18 +359,29 @@ __global__ void __launch_bounds__(64 * WAVES) k_lc_mfma(const double *__restrict
     for (int u = 0; u < PER; u++) {
       const int q = tid + BT * u;
       const int pp = q / KC, kk = q % KC;
+      const int k = k0 + kk;
       if (IO == 1) {
         const int row = p0 + pp;
-        const long long off = lc_pool_offset<D, NC, TS>(map, nb, k0 + kk, row % N, row / N);
-        pa[u] = off >= 0 ? (double)pin[off] : 0.0;
+        const long long off = lc_pool_offset<D, NC, TS>(map, nb, k, row % N, row / N);
+        double a = off >= 0 ? (double)pin[off] : 0.0;
+        if (FOLD == 1) {
+          const long long off2 = lc_pool_offset<D, NC, TS>(map, nb, N - 1 - k, row % N, row / N);
+          a = a + sg * (off2 >= 0 ? (double)pin[off2] : 0.0);
+        }
+        pa[u] = a;
+      } else if (ROWS) {
+        const double *r = in + (size_t)(p0 + pp) * N;
+        pa[u] = (FOLD == 1) ? r[k] + sg * r[N - 1 - k] : r[k];
       } else {
-        pa[u] = ROWS ? in[(size_t)(p0 + pp) * N + k0 + kk] : M[(size_t)(p0 + pp) * N + k0 + kk];
+        pa[u] = M[(size_t)(p0 + pp) * MS + k];
       }
       if (ROWS) {
-        pb[u] = M[(size_t)(q0 + pp) * N + k0 + kk];
+        pb[u] = M[(size_t)(q0 + pp) * MS + k];
       } else {
         const int kb = q / T, qq = q % T;
-        pb[u] = in[base + (size_t)(k0 + kb) * sm + qq];
+        const double *c = in + base + qq;
+        pb[u] = (FOLD == 1) ? c[(size_t)(k0 + kb) * sm] + sg * c[(size_t)(N - 1 - k0 - kb) * sm]
+                            : c[(size_t)(k0 + kb) * sm];
       }
     }
   };
@@ -368,22 +395,28 @@ __global__ void __launch_bounds__(64 * WAVES) k_lc_mfma(const double *__restrict
         Bs[(q / T) * BP + q % T] = pb[u];
     }
   };
-  lc_double4 acc[2][NTB];
-  for (int a = 0; a < 2; a++)
-    for (int b = 0; b < NTB; b++) acc[a][b] = lc_double4{0.0, 0.0, 0.0, 0.0};
+  lc_double4 acc[NS][2][NTB];
+  for (int e = 0; e < NS; e++)
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < NTB; b++) acc[e][a][b] = lc_double4{0.0, 0.0, 0.0, 0.0};
   fetch(0);
-  for (int k0 = 0; k0 < N; k0 += KC) {
+  for (int k0 = 0; k0 < KN; k0 += KC) {
     stash();
     __syncthreads();
-    if (k0 + KC < N) fetch(k0 + KC);
+    if (k0 + KC < KN) fetch(k0 + KC);
+    const int e = (NS == 2 && k0 >= N / 2) ? 1 : 0;  // (the backward form: the odd modes' sum)
     for (int ks = 0; ks < KC; ks += 4) {
       const int kk = ks + (lane >> 4);
       double a[2], b[NTB];
       for (int t = 0; t < 2; t++) a[t] = As[(wp + 16 * t + (lane & 15)) * AP + kk];
       for (int t = 0; t < NTB; t++) b[t] = Bs[kk * BP + wq + 16 * t + (lane & 15)];
       for (int ta = 0; ta < 2; ta++)
-        for (int tb = 0; tb < NTB; tb++)
-          acc[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta], b[tb], acc[ta][tb], 0, 0, 0);
+        for (int tb = 0; tb < NTB; tb++) {
+          if (NS == 2 && e == 1)
+            acc[NS - 1][ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta], b[tb], acc[NS - 1][ta][tb], 0, 0, 0);
+          else
+            acc[0][ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta], b[tb], acc[0][ta][tb], 0, 0, 0);
+        }
     }
     __syncthreads();
   }
@@ -392,7 +425,25 @@ __global__ void __launch_bounds__(64 * WAVES) k_lc_mfma(const double *__restrict
       for (int j = 0; j < 4; j++) {
         const int pp = p0 + wp + 16 * ta + 4 * j + (lane >> 4);
         const int qq = q0 + wq + 16 * tb + (lane & 15);
-        double v = acc[ta][tb][j];
+        if (FOLD == 2) {
+          // u_m = E + O, u_{N-1-m} = E - O; m is qq (ROWS) or pp (lines)
+          const double vE = acc[0][ta][tb][j], vO = acc[NS - 1][ta][tb][j];
+          const double lo = vE + vO, hi = vE - vO;
+          if (ROWS && IO == 2) {
+            long long off = lc_pool_offset<D, NC, TS>(map, nb, qq, pp % N, pp / N);
+            if (off >= 0) pout[off] = (TS)lo;
+            off = lc_pool_offset<D, NC, TS>(map, nb, N - 1 - qq, pp % N, pp / N);
+            if (off >= 0) pout[off] = (TS)hi;
+          } else if (ROWS) {
+            out[(size_t)pp * N + qq] = lo;
+            out[(size_t)pp * N + N - 1 - qq] = hi;
+          } else {
+            out[base + (size_t)pp * sm + qq] = lo;
+            out[base + (size_t)(N - 1 - pp) * sm + qq] = hi;
+          }
+          continue;
+        }
+        double v = acc[0][ta][tb][j];
         if (ROWS && IO == 2) {
           const long long off = lc_pool_offset<D, NC, TS>(map, nb, qq, pp % N, pp / N);
           if (off >= 0) pout[off] = (TS)v;

@@ -415,6 +415,11 @@ int afgpu_kernel_stats(afgpu_t *h, int kind, int reset, double *total_ms,
                                half-sweeps stage the whole box in LDS (1), or only the other
                                colour's block as it is stored and the rhs (0); identical
                                results (default 0) */
+#define AFGPU_OPT_DIRECT_FOLD 13 /* mixed precision: the FP64 transforms (128 cells per axis) of
+                                   axes whose 1-D operator is symmetric under reversal (the same
+                                   boundary condition at both ends, Cartesian) split the modes by
+                                   parity: half the products; results differ in rounding only
+                                   (default 1; AFGPU_OPT_FUSE_DIRECT applies with it off) */
 int afgpu_set_option(afgpu_t *h, int option, int value);
 
 /* ---- ensembles of independent runs --------------------------------------

@@ -49,7 +49,7 @@ module m_afgpu
        afgpu_opt_batch_modes = 5, afgpu_opt_direct_mfma = 6, &
        afgpu_opt_photoi_vcycle = 7, afgpu_opt_flux32 = 8, &
        afgpu_opt_step_graph = 9, afgpu_opt_fuse_direct = 10, &
-       afgpu_opt_direct_f32 = 11, afgpu_opt_box_lds = 12
+       afgpu_opt_direct_f32 = 11, afgpu_opt_box_lds = 12, afgpu_opt_direct_fold = 13
 
   !> afgpu_params_t
   type, bind(c), public :: afgpu_params_t

@@ -208,22 +208,25 @@ def test_mixed_batched_modes_equal_sequential(A, S, name):
 def test_mixed_direct_solve_128_matrix_cores(A, S):
     """3D, 16^3 boxes, levels 1-4 uniform (level 4 = 128^3 cells, solved
     exactly by the inner cycle): the FP32 and FP64 matrix-core transforms
-    and the FP64 vector transforms all reach the FP64 fixed point."""
+    (the FP64 ones split by parity, the default, and unsplit) and the FP64
+    vector transforms all reach the FP64 fixed point."""
     case = setup(S.small_amr(3, nc=16, uniform_lvl=4, max_lvl=5, frac=0.03, seed=4))
     f = S.apply(A.Path("gpu", case.D, case.nc, case.coord, A.PREC_FP64), case)
     f.field_compute(0, False)
     for _ in range(6):
         f.mg_fas_vcycle(A.OP_POISSON, True, 0)
     ref = leaf_interior(case, f, A.CC_PHI)
-    for mfma, f32 in ((1, 1), (1, 0), (0, 0)):
+    for mfma, f32, fold in ((1, 1, 1), (1, 0, 1), (1, 0, 0), (0, 0, 0)):
         m = S.apply(A.Path("gpu", case.D, case.nc, case.coord, A.PREC_MIXED), case)
         m.set_option(A.OPT_DIRECT_MFMA, mfma)
         m.set_option(A.OPT_DIRECT_F32, f32)
+        m.set_option(A.OPT_DIRECT_FOLD, fold)
         n0, r0 = m.field_compute(0, False)
         for _ in range(4):
             m.field_compute(0, True)
         e = rel_max(leaf_interior(case, m, A.CC_PHI), ref)
-        print("mfma", mfma, "f32", f32, "initial cycles", n0, "phi vs fp64 fixed point %.2e" % e, flush=True)
+        print("mfma", mfma, "f32", f32, "fold", fold, "initial cycles", n0, "phi vs fp64 fixed point %.2e" % e,
+              flush=True)
         assert e < 1e-11
 
 
@@ -238,6 +241,7 @@ def test_mixed_direct_solve_fused_passes_bit_for_bit(A, S):
     for fuse in (1, 0):
         m = S.apply(A.Path("gpu", case.D, case.nc, case.coord, A.PREC_MIXED), case)
         m.set_option(A.OPT_DIRECT_F32, 0)
+        m.set_option(A.OPT_DIRECT_FOLD, 0)  # (the fused pass transforms unsplit)
         m.set_option(A.OPT_FUSE_DIRECT, fuse)
         infos = [m.field_compute(0, False)] + [m.step() for _ in range(3)]
         runs.append((infos, [m.get_cc(iv) for iv in (A.CC_PHI, A.CC_ELECTRON, A.CC_PHOTO)]))
@@ -271,6 +275,31 @@ def test_split_half_sweeps_bit_for_bit(A, S):
         assert runs[0][0] == runs[1][0], case.name
         for a, b in zip(runs[0][1], runs[1][1]):
             assert np.array_equal(a, b), case.name
+
+
+def test_mixed_direct_solve_parity_split(A, S):
+    """The FP64 transforms split by parity (every axis of a 3D Cartesian
+    level is symmetric under reversal) against the unsplit ones: 3D with
+    photoionization (level 4 = 128^3), a few steps.  The split transform
+    is the same solve in another summation order, so the runs agree to
+    round-off: equal V-cycle counts (or one more) and fields within a few
+    1e-13 of each other."""
+    case = setup(S.small_amr(3, nc=16, uniform_lvl=4, max_lvl=5, frac=0.03, seed=4), photoi=True)
+    runs = []
+    for fold in (1, 0):
+        m = S.apply(A.Path("gpu", case.D, case.nc, case.coord, A.PREC_MIXED), case)
+        m.set_option(A.OPT_DIRECT_F32, 0)
+        m.set_option(A.OPT_DIRECT_FOLD, fold)
+        n0, _ = m.field_compute(0, False)
+        infos = [m.step() for _ in range(3)]
+        runs.append((m, [n0] + [i["n_vcycles"] for i in infos]))
+    print("V-cycles split", runs[0][1], "unsplit", runs[1][1], flush=True)
+    for a, b in zip(runs[0][1], runs[1][1]):
+        assert abs(a - b) <= 1
+    for iv, tol in ((A.CC_PHI, 1e-10), (A.CC_EFIELD, 1e-8), (A.CC_ELECTRON, 1e-9), (A.CC_PHOTO, 1e-6)):
+        e = rel_max(leaf_interior(case, runs[0][0], iv), leaf_interior(case, runs[1][0], iv))
+        print("var", iv, "split vs unsplit %.2e" % e, flush=True)
+        assert e < tol
 
 
 def test_mixed_direct_solve_f32_transforms(A, S):
