@@ -9,7 +9,7 @@ path, the tree changed on the host, the new boxes prolonged on the path,
 the field and photoionization recomputed).
 
 The settings are those of the reference's rtest configuration files
-(programs/standard_2d/tests/test_cyl.cfg, test_2d_photoi.cfg,
+(programs/standard_2d/tests/test_cyl.cfg, test_2d.cfg, test_2d_photoi.cfg,
 programs/standard_3d/tests/test_3d.cfg) with the reference's defaults for
 everything they leave out; the transport data is their
 td_air_siglo_swarm.txt (tests/golden)."""
@@ -38,6 +38,11 @@ CASES = {
         field=-2e6, background=1e14, refine_adx=1.0, photoi=None,
         seeds=[seed((0.0, 0.52), (0.0, 0.48), (16e-3, 16e-3), 5e18, 2e-4, "smoothstep", 1)],
         log="programs/standard_2d/tests/test_cyl_rtest.log"),
+    "test_2d": dict(   # programs/standard_2d/tests/test_2d.cfg
+        D=2, coord=A.XYZ, nc=8, L=(32e-3, 32e-3), end_time=7e-9, output_dt=1e-9, dt_max=1e-10,
+        field=2.5e6, background=1e14, refine_adx=1.0, photoi=None,
+        seeds=[seed((0.5, 0.32), (0.5, 0.28), (32e-3, 32e-3), 1e18, 2e-4, "smoothstep", 1)],
+        log="programs/standard_2d/tests/test_2d_rtest.log"),
     "test_2d_photoi": dict(   # programs/standard_2d/tests/test_2d_photoi.cfg
         D=2, coord=A.XYZ, nc=8, L=(32e-3, 32e-3), end_time=7e-9, output_dt=1e-9, dt_max=1e-10,
         field=2.5e6, background=1e11, refine_adx=1.0, photoi="Bourdon-3",

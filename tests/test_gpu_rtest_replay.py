@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 SUM_COLS = [3, 4, 5, 6, 7, 8]
 
 
-@pytest.mark.parametrize("name", ["test_cyl", "test_3d", "test_2d_photoi"])
+@pytest.mark.parametrize("name", ["test_cyl", "test_2d", "test_3d", "test_2d_photoi"])
 def test_device_replays_reference_log(A, S, name):
     print("replay", name, flush=True)
     g = R.Run(name, "gpu")

@@ -6,9 +6,11 @@ derefinement every two steps, output-time dt clipping, the rtest
 configuration files' settings and their td_air_siglo_swarm.txt table),
 and the rows of the regression log it writes are compared with the logs
 the reference holds (programs/standard_2d/tests/test_cyl_rtest.log,
-test_2d_photoi_rtest.log, programs/standard_3d/tests/test_3d_rtest.log;
+test_2d_rtest.log, test_2d_photoi_rtest.log,
+programs/standard_3d/tests/test_3d_rtest.log;
 tests/golden/regression_logs.json) at the tolerance of the reference's own
-comparison, tools/compare_logs.py:14-28 (np.isclose, rtol 1e-5, atol 1e-8).
+comparison, tools/compare_logs.py:14-28 (np.isclose, rtol 1e-5,
+atol 1e-8): test_cyl and test_2d on every row and column.
 
 Two departures are bounded instead, both where the level-1 solve (exact
 here, hypre PFMG to 1e-6 in the reference) moves the potential within the
@@ -40,7 +42,7 @@ def check_rows(name, rows):
     assert rows.shape == ref.shape, (name, rows.shape, ref.shape)
     err = rel(rows, ref)
     print(name, "max relative error per column", np.array2string(err.max(axis=0), precision=2), flush=True)
-    if name == "test_cyl":
+    if name in ("test_cyl", "test_2d"):
         assert np.isclose(rows, ref, **TOL).all()
     elif name == "test_3d":
         other = [c for c in range(rows.shape[1]) if c not in M_MIN_COLS]
@@ -52,7 +54,7 @@ def check_rows(name, rows):
     return err
 
 
-@pytest.mark.parametrize("name", ["test_cyl", "test_3d", "test_2d_photoi"])
+@pytest.mark.parametrize("name", ["test_cyl", "test_2d", "test_3d", "test_2d_photoi"])
 def test_oracle_replays_reference_log(A, S, name):
     rows = R.Run(name, "oracle").run()
     check_rows(name, rows)
