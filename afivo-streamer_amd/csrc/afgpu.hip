@@ -151,6 +151,9 @@ struct ImplBase {
   virtual int prolong_new(const int *ids, int n, const VarRef *vars, int n_vars) = 0;
   virtual int gc_ids(const int *ids, int n, const VarRef *vars, int n_vars) = 0;
   virtual int restrict_tree(int iv) = 0;
+  // the same on any FP64 pool of cell values (a reaction list's species)
+  virtual int restrict_pool(double *var) = 0;
+  virtual int gc_pool(int lvl, double *var, const VarBc &m, bool corners) = 0;
   virtual int step_runs(afgpu_step_info_t *info) = 0;
   // per-leaf-box terms of the regression-log statistics (k_diag_box)
   virtual int diag_boxes(const VarPtrs &vars, double *part) = 0;
@@ -2353,17 +2356,19 @@ struct Impl : ImplBase {
     return AFGPU_OK;
   }
 
-  int restrict_tree(int iv) override {
+  int restrict_tree(int iv) override { return restrict_pool(c.cc[iv]); }
+  int restrict_pool(double *var) override {
     for (int lvl = c.highest_lvl; lvl >= 2; lvl--) {
       const List &l = c.l_ids[lvl];
       RCHK(with_bb(KC_RESTRICT, [&](auto B) {
         constexpr int bb = decltype(B)::value;
         return items<bb>(KC_RESTRICT, k_restrict<D, NC, BS, double, bb>, l.n, c.list(l), (const DBox *)c.d_boxes,
-                         (const double *)c.cc[iv], c.cc[iv], 1, (int)(c.coord == AFGPU_CYL), 1);
+                         (const double *)var, var, 1, (int)(c.coord == AFGPU_CYL), 1);
       }));
     }
     return AFGPU_OK;
   }
+  int gc_pool(int lvl, double *var, const VarBc &m, bool corners) override { return gc_var(lvl, var, m, corners); }
 };
 
 // Workgroup size of the box kernels of a handle's (dimension, box size).
@@ -3005,8 +3010,21 @@ int afgpu_tree_maxabs_cc(afgpu_t *h, int iv, double *out) {
   return h->impl->maxabs(iv, out);
 }
 
+// Species k of a reaction list (iv = AFGPU_VAR_SPECIES(k)): its state-0
+// pool, with the densities' boundary and refinement-boundary methods.
+static double *species_pool(const afgpu_t *h, int iv) {
+  const int k = iv - AFGPU_VAR_SPECIES(0);
+  const SpeciesArgs S = h->species_args();
+  return (k >= 0 && k < S.n) ? S.sp[k][0] : nullptr;
+}
+
 int afgpu_gc_tree(afgpu_t *h, int iv) {
   RCHK(ready(h));
+  if (double *sp = species_pool(h, iv)) {
+    const VarBc m = var_methods(*h, AFGPU_CC_ELECTRON);
+    for (int lvl = 1; lvl <= h->highest_lvl; lvl++) RCHK(h->impl->gc_pool(lvl, sp, m, true));
+    return sync_stream(h);
+  }
   if (iv < 0 || iv >= AFGPU_N_CC || var_methods(*h, iv).rb == RB_NONE) return AFGPU_ERR_ARG;
   for (int lvl = 1; lvl <= h->highest_lvl; lvl++) RCHK(h->impl->gc_lvl(lvl, iv, true));
   return sync_stream(h);
@@ -3014,6 +3032,10 @@ int afgpu_gc_tree(afgpu_t *h, int iv) {
 
 int afgpu_restrict_tree(afgpu_t *h, int iv) {
   RCHK(ready(h));
+  if (double *sp = species_pool(h, iv)) {
+    RCHK(h->impl->restrict_pool(sp));
+    return sync_stream(h);
+  }
   if (iv < 0 || iv >= AFGPU_N_CC) return AFGPU_ERR_ARG;
   RCHK(h->impl->restrict_tree(iv));
   return sync_stream(h);

@@ -197,7 +197,10 @@ int afgpu_mg_fas_fmg(afgpu_t *h, int op, int set_residual, int have_guess);
 int afgpu_tree_maxabs_cc(afgpu_t *h, int iv, double *out);
 /* af_gc_tree (afivo/src/m_af_ghostcell.f90:24) and af_restrict_tree
  * (afivo/src/m_af_restrict.f90:50) for one variable, with the variable's
- * own methods; used before refinement (src/streamer.f90:399-400). */
+ * own methods; used before refinement on all_densities
+ * (src/streamer.f90:399-400).  iv may also be AFGPU_VAR_SPECIES(k), a
+ * species of the reaction list set with afgpu_set_chemistry (state 0),
+ * with the densities' methods. */
 int afgpu_gc_tree(afgpu_t *h, int iv);
 int afgpu_restrict_tree(afgpu_t *h, int iv);
 

@@ -2504,19 +2504,32 @@ int afcpu_tree_maxabs_cc(afcpu_t *h, int iv, double *out) {
   return AFGPU_OK;
 }
 
-int afcpu_gc_tree(afcpu_t *h, int iv) {
-  int rc = ready(h, -1);
-  if (rc) return rc;
-  if (iv < 0 || iv >= AFGPU_N_CC || !methods(h->c, iv).ok) return AFGPU_ERR_ARG;
-  gc_tree(h->c, iv);
-  return AFGPU_OK;
+}  // extern "C"
+
+// Species k of a reaction list (iv = AFGPU_VAR_SPECIES(k)): the pool index
+// of species 0-2, or -1 - (its pool in xs) beyond them; n_sp when none.
+static int species_var(const Ctx &c, int iv) {
+  const int k = iv - AFGPU_VAR_SPECIES(0);
+  if (k < 0 || k >= c.n_sp) return AFGPU_N_CC;
+  return k < 3 ? AFGPU_CC_SPECIES(k) : -1 - (k - 3) * 3;
+}
+// Runs f(iv') with the variable of iv at a density index: a species beyond
+// the three pools is swapped into the electrons' third state for the call
+// (the densities' boundary methods) and back.
+template <typename F>
+static int with_density_var(Ctx &c, int iv, F f) {
+  const int sv = species_var(c, iv);
+  if (sv == AFGPU_N_CC) return f(iv);
+  if (sv >= 0) return f(sv);
+  const int spare = AFGPU_CC_ELECTRON + 2;
+  std::swap(c.cc[spare], c.xs[(size_t)(-1 - sv)]);
+  const int rc = f(spare);
+  std::swap(c.cc[spare], c.xs[(size_t)(-1 - sv)]);
+  return rc;
 }
 
-int afcpu_restrict_tree(afcpu_t *h, int iv) {
-  int rc = ready(h, -1);
-  if (rc) return rc;
+static int restrict_tree_var(Ctx &c, int iv) {
   if (iv < 0 || iv >= AFGPU_N_CC) return AFGPU_ERR_ARG;
-  Ctx &c = h->c;
   for (int lvl = c.highest_lvl - 1; lvl >= 1; lvl--) {
     const std::vector<int> &par = c.parents[lvl];
 #pragma omp parallel for schedule(static)
@@ -2528,6 +2541,25 @@ int afcpu_restrict_tree(afcpu_t *h, int iv) {
   }
   return AFGPU_OK;
 }
+
+extern "C" {
+
+int afcpu_gc_tree(afcpu_t *h, int iv) {
+  int rc = ready(h, -1);
+  if (rc) return rc;
+  return with_density_var(h->c, iv, [&](int v) {
+    if (v < 0 || v >= AFGPU_N_CC || !methods(h->c, v).ok) return (int)AFGPU_ERR_ARG;
+    gc_tree(h->c, v);
+    return (int)AFGPU_OK;
+  });
+}
+
+int afcpu_restrict_tree(afcpu_t *h, int iv) {
+  int rc = ready(h, -1);
+  if (rc) return rc;
+  return with_density_var(h->c, iv, [&](int v) { return restrict_tree_var(h->c, v); });
+}
+
 
 int afcpu_field_compute(afcpu_t *h, int s_in, int have_guess, int *n_cycles, double *residual) {
   int rc = ready(h, AFGPU_OP_POISSON);

@@ -10,7 +10,10 @@ holds the reference at /root/reference):
   transport and rate tables), copied unchanged.
 - td_air_siglo_swarm.txt: the old-style transport data of the reference's
   standard rtests (programs/standard_2d/tests/td_air_siglo_swarm.txt; the
-  3D tests hold the same file), copied unchanged."""
+  3D tests hold the same file), copied unchanged.
+- air_chemistry_small_v1.txt: the input data file of the reference's 3D
+  chemistry rtests (programs/standard_3d/tests/air_chemistry_small_v1.txt),
+  copied unchanged."""
 import glob
 import json
 import os
@@ -31,3 +34,5 @@ shutil.copyfile(os.path.join(REF, "programs", "standard_2d", "tests", "air_chemi
                 os.path.join(HERE, "air_chemistry_v1.txt"))
 shutil.copyfile(os.path.join(REF, "programs", "standard_2d", "tests", "td_air_siglo_swarm.txt"),
                 os.path.join(HERE, "td_air_siglo_swarm.txt"))
+shutil.copyfile(os.path.join(REF, "programs", "standard_3d", "tests", "air_chemistry_small_v1.txt"),
+                os.path.join(HERE, "air_chemistry_small_v1.txt"))

@@ -10,9 +10,12 @@ the field and photoionization recomputed).
 
 The settings are those of the reference's rtest configuration files
 (programs/standard_2d/tests/test_cyl.cfg, test_2d.cfg, test_2d_photoi.cfg,
-programs/standard_3d/tests/test_3d.cfg) with the reference's defaults for
-everything they leave out; the transport data is their
-td_air_siglo_swarm.txt (tests/golden)."""
+test_cyl_chem.cfg, test_cyl_photoi_chem.cfg, test_2d_photoi_chem.cfg,
+programs/standard_3d/tests/test_3d.cfg, test_3d_chem.cfg,
+test_3d_photoi_chem.cfg) with the reference's defaults for everything they
+leave out; the input data are their td_air_siglo_swarm.txt (old-style
+transport data), air_chemistry_v1.txt and air_chemistry_small_v1.txt
+(reaction lists with their transport data; tests/golden)."""
 import os
 
 import numpy as np
@@ -43,6 +46,21 @@ CASES = {
         field=2.5e6, background=1e14, refine_adx=1.0, photoi=None,
         seeds=[seed((0.5, 0.32), (0.5, 0.28), (32e-3, 32e-3), 1e18, 2e-4, "smoothstep", 1)],
         log="programs/standard_2d/tests/test_2d_rtest.log"),
+    "test_cyl_chem": dict(   # programs/standard_2d/tests/test_cyl_chem.cfg
+        D=2, coord=A.CYL, nc=8, L=(16e-3, 16e-3), end_time=1.5e-9, output_dt=0.25e-9, dt_max=1e-10,
+        field=-2e6, background=1e14, refine_adx=1.0, photoi=None, chem="air_chemistry_v1.txt",
+        seeds=[seed((0.0, 0.52), (0.0, 0.48), (16e-3, 16e-3), 5e18, 2e-4, "smoothstep", 1)],
+        log="programs/standard_2d/tests/test_cyl_chem_rtest.log"),
+    "test_cyl_photoi_chem": dict(   # programs/standard_2d/tests/test_cyl_photoi_chem.cfg
+        D=2, coord=A.CYL, nc=8, L=(16e-3, 16e-3), end_time=1.5e-9, output_dt=0.25e-9, dt_max=1e-10,
+        field=-2e6, background=1e14, refine_adx=1.0, photoi="Bourdon-3", chem="air_chemistry_v1.txt",
+        seeds=[seed((0.0, 0.52), (0.0, 0.48), (16e-3, 16e-3), 5e18, 2e-4, "smoothstep", 1)],
+        log="programs/standard_2d/tests/test_cyl_photoi_chem_rtest.log"),
+    "test_2d_photoi_chem": dict(   # programs/standard_2d/tests/test_2d_photoi_chem.cfg
+        D=2, coord=A.XYZ, nc=8, L=(32e-3, 32e-3), end_time=7e-9, output_dt=1e-9, dt_max=1e-10,
+        field=2.5e6, background=1e11, refine_adx=1.0, photoi="Bourdon-3", chem="air_chemistry_v1.txt",
+        seeds=[seed((0.5, 0.32), (0.5, 0.28), (32e-3, 32e-3), 1e18, 2e-4, "smoothstep", 1)],
+        log="programs/standard_2d/tests/test_2d_photoi_chem_rtest.log"),
     "test_2d_photoi": dict(   # programs/standard_2d/tests/test_2d_photoi.cfg
         D=2, coord=A.XYZ, nc=8, L=(32e-3, 32e-3), end_time=7e-9, output_dt=1e-9, dt_max=1e-10,
         field=2.5e6, background=1e11, refine_adx=1.0, photoi="Bourdon-3",
@@ -53,6 +71,16 @@ CASES = {
         field=-2.5e6, background=1e14, refine_adx=1.5, photoi=None,
         seeds=[seed((0.5, 0.5, 0.52), (0.5, 0.5, 0.48), (16e-3, 16e-3, 16e-3), 5e18, 5e-4, "smoothstep", 0)],
         log="programs/standard_3d/tests/test_3d_rtest.log"),
+    "test_3d_chem": dict(   # programs/standard_3d/tests/test_3d_chem.cfg
+        D=3, coord=A.XYZ, nc=8, L=(16e-3, 16e-3, 16e-3), end_time=3e-9, output_dt=0.2e-9, dt_max=5e-11,
+        field=-2.5e6, background=1e14, refine_adx=1.5, photoi=None, chem="air_chemistry_small_v1.txt",
+        seeds=[seed((0.5, 0.5, 0.52), (0.5, 0.5, 0.48), (16e-3, 16e-3, 16e-3), 5e18, 5e-4, "smoothstep", 0)],
+        log="programs/standard_3d/tests/test_3d_chem_rtest.log"),
+    "test_3d_photoi_chem": dict(   # programs/standard_3d/tests/test_3d_photoi_chem.cfg
+        D=3, coord=A.XYZ, nc=8, L=(16e-3, 16e-3, 16e-3), end_time=3e-9, output_dt=0.2e-9, dt_max=5e-11,
+        field=-2.5e6, background=1e14, refine_adx=1.5, photoi="Bourdon-3", chem="air_chemistry_small_v1.txt",
+        seeds=[seed((0.5, 0.5, 0.52), (0.5, 0.5, 0.48), (16e-3, 16e-3, 16e-3), 5e18, 5e-4, "smoothstep", 0)],
+        log="programs/standard_3d/tests/test_3d_photoi_chem_rtest.log"),
 }
 
 REFINE_MAX_DX = 1e-3     # src/m_refine.f90:19
@@ -74,6 +102,27 @@ def transport_tables(N):
         ys.append(y)
     assert all(np.array_equal(x, xs[0]) for x in xs)
     return A.transport_tables(xs[0], *ys, N)
+
+
+def transport_tables_new(path, N, table_size=1000):
+    """td_tbl from a new-style input file (src/m_transport_data.f90:128-162):
+    reduced field in Td, mobility and diffusion times N, alpha and eta over
+    N, on LT_create's grid from 0 to the mobility data's last field.  The
+    standard model's rate columns (unused with a reaction list) are formed
+    as for old-style data.  Returns the tables and the grid's last field."""
+    lines = open(path).read().splitlines()
+    cols = [CH.table_from_file(lines, name) for name in
+            ("Mobility *N (1/m/V/s)", "Diffusion coefficient *N (1/m/s)", "Townsend ioniz. coef. alpha/N (m2)",
+             "Townsend attach. coef. eta/N (m2)")]
+    x_max = cols[0][0][-1]
+    grid = CH.lt_grid(0.0, x_max, table_size)
+    td = [CH.spaced_data(np.asarray(x), np.asarray(y), grid) for x, y in cols]
+    ion = td[2] * td[0] * grid * 1e-21 * N
+    att = td[3] * td[0] * grid * 1e-21 * N
+    tables = {"n": table_size, "td_x_min": 0.0, "td_inv_fac": (table_size - 1) / (x_max - 0.0),
+              "td": np.concatenate(td), "chem_x_min": grid[0],
+              "chem_inv_fac": (table_size - 1) / (grid[-1] - grid[0]), "chem": np.concatenate([ion, att])}
+    return tables, x_max
 
 
 def falloff_value(dist, width, kind):
@@ -168,7 +217,33 @@ class Run:
         self.p = A.Path(kind, D, nc, cfg["coord"], precision if kind == "gpu" else A.PREC_FP64)
         self.p.sync_tree(self.t)
         self.p.set_params(params_for(cfg, self.N))
-        self.p.set_tables(transport_tables(self.N))
+        self.net = None
+        if cfg.get("chem"):
+            # a reaction list with its transport data (input_data%old_style = f)
+            path = os.path.join(HERE, "golden", cfg["chem"])
+            tables, x_max = transport_tables_new(path, self.N)
+            self.p.set_tables(tables)
+            self.net = CH.read_network(path, {"N2": 0.8, "O2": 0.2}, self.N, 0.0, x_max, 1000)
+            # td_tbl's mean energy column (eV, src/m_transport_data.f90:158-161),
+            # read by the rate types that use the electron temperature
+            lines = open(path).read().splitlines()
+            xe, ye = CH.table_from_file(lines, "Mean energy (eV)")
+            energy = CH.spaced_data(np.asarray(xe), np.asarray(ye), CH.lt_grid(0.0, x_max, 1000))
+            self.chem = CH.Chem(kind, len(self.net.species), self.net.reactions, 0.0, x_max, self.net.fld_cols,
+                                energy, 300.0)
+            self.p.set_chemistry(self.chem, self.net.charges)
+            if cfg["photoi"] is not None:
+                # photoi%species, default O2_plus (src/m_photoi.f90:27-28)
+                prm = params_for(cfg, self.N)
+                prm.photoi_species = list(self.net.names).index("O2_plus")
+                self.p.set_params(prm)
+            self.dens = [A.VAR_SPECIES(k) for k in range(len(self.net.species))]
+            # the first positive ion takes the ions of the initial conditions
+            # (src/m_streamer.f90:228-238)
+            self.i_pos = list(self.net.charges).index(1)
+        else:
+            self.p.set_tables(transport_tables(self.N))
+            self.dens = list(self.DENS)
         # field_given_by field / field_amplitude: V = -L * E (src/m_field.f90:135-161)
         self.p.set_voltage(-cfg["L"][-1] * cfg["field"])
         if kind != "gpu":
@@ -186,6 +261,10 @@ class Run:
 
     def set_initial(self, ids):
         e, ion = initial_densities(self.cfg, self.t, ids)
+        if self.net is not None:
+            for k in range(len(self.net.species)):
+                self.p.put_species(k, e if k == 0 else ion if k == self.i_pos else np.zeros_like(e), ids=ids)
+            return
         self.p.put_cc(A.CC_ELECTRON, e, ids)
         self.p.put_cc(A.CC_POS_ION, ion, ids)
         self.p.put_cc(A.CC_NEG_ION, np.zeros_like(e), ids)
@@ -219,7 +298,8 @@ class Run:
 
     def log_row(self, out_cnt):
         it, time, dt = self.p.get_time_state()
-        v = self.p.regression_values(list(self.DENS))[0]  # (3 species, (mean, mean of squares, max))
+        # (species, (mean, mean of squares, max)), four variables per call
+        v = np.concatenate([self.p.regression_values(self.dens[q:q + 4])[0] for q in range(0, len(self.dens), 4)])
         self.rows.append([out_cnt, time, dt] + list(v[:, 0]) + list(v[:, 1]) + list(v[:, 2]))
 
     def initial_conditions(self):
@@ -262,7 +342,7 @@ class Run:
                 if max_rows is not None and out_cnt >= max_rows:
                     break
             if it % REFINE_PER_STEPS == 0:
-                for iv in self.DENS:
+                for iv in self.dens:
                     self.p.restrict_tree(iv)
                     self.p.gc_tree(iv)
                 _, gtime, _ = self.p.get_time_state()

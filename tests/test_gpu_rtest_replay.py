@@ -3,13 +3,13 @@
 tests/test_rtest_replay.py): the plain FP64 form against the reference's
 logs and against the oracle's replay of the same run (times, time steps
 and maxima equal, sums to 1e-12: they add the same terms in another
-order), and the mixed-precision form against the reference's
-axisymmetric log."""
+order; with a reaction list to 1e-9), and the mixed-precision form
+against the reference's axisymmetric log."""
 import numpy as np
 import pytest
 
 import rtest as R
-from test_rtest_replay import check_rows, rel
+from test_rtest_replay import CPU_CASES, check_rows, rel
 
 pytestmark = pytest.mark.gpu
 
@@ -27,6 +27,23 @@ def test_device_replays_reference_log(A, S, name):
     assert np.array_equal(rows[:, exact], o[:, exact]), name
     assert rel(rows[:, SUM_COLS], o[:, SUM_COLS]).max() <= 1e-12, name
     assert g.steps > 50
+
+
+@pytest.mark.parametrize("name", ["test_cyl_chem", "test_cyl_photoi_chem", "test_3d_chem", "test_3d_photoi_chem",
+                                  "test_2d_photoi_chem"])
+def test_device_replays_reference_chemistry_log(A, S, name):
+    """The reaction-list rtests on the device: against the reference's log
+    (the bounds of tests/test_rtest_replay.py) and, where the CPU suite runs
+    the oracle, against the oracle's replay: the device's and the host's
+    exp-type rates may differ in the last bit, so the rows agree to 1e-9."""
+    print("replay", name, flush=True)
+    g = R.Run(name, "gpu")
+    rows = g.run()
+    check_rows(name, rows)
+    assert g.steps > 50
+    if name in CPU_CASES:
+        o = R.Run(name, "oracle").run()
+        assert rel(rows, o).max() <= 1e-9, name
 
 
 def test_mixed_form_replays_cylindrical_log(A, S):

@@ -6,15 +6,22 @@ derefinement every two steps, output-time dt clipping, the rtest
 configuration files' settings and their td_air_siglo_swarm.txt table),
 and the rows of the regression log it writes are compared with the logs
 the reference holds (programs/standard_2d/tests/test_cyl_rtest.log,
-test_2d_rtest.log, test_2d_photoi_rtest.log,
-programs/standard_3d/tests/test_3d_rtest.log;
+test_2d_rtest.log, test_2d_photoi_rtest.log, test_cyl_chem_rtest.log,
+test_cyl_photoi_chem_rtest.log, test_2d_photoi_chem_rtest.log,
+programs/standard_3d/tests/test_3d_rtest.log, test_3d_chem_rtest.log,
+test_3d_photoi_chem_rtest.log;
 tests/golden/regression_logs.json) at the tolerance of the reference's own
 comparison, tools/compare_logs.py:14-28 (np.isclose, rtol 1e-5,
 atol 1e-8): test_cyl and test_2d on every row and column.
 
-Two departures are bounded instead, both where the level-1 solve (exact
-here, hypre PFMG to 1e-6 in the reference) moves the potential within the
-field solve's stopping tolerance:
+The reaction-list cases read their network and transport data from the
+input files (air_chemistry_v1.txt, air_chemistry_small_v1.txt; eight
+species each): test_cyl_chem and test_cyl_photoi_chem are also within
+compare_logs' tolerance everywhere.
+
+The departures are bounded instead, all where the level-1 solve (exact
+here, hypre PFMG to 1e-6 in the reference; in 3D its box holds 512 cells,
+in 2D 64) moves the potential within the field solve's stopping tolerance:
 - test_3d, the negative ions (columns sum, sum of squares and max of
   M_min): the attachment happens at low fields, where the relative field
   error of the stopping tolerance is largest (tightening the field solve
@@ -23,7 +30,14 @@ field solve's stopping tolerance:
 - test_2d_photoi, rows 5-7 of the growing streamer: the Helmholtz modes
   are solved to 1e-2 of max |rhs| (photoi_helmh%max_rel_residual), which
   leaves the photoionization source most dependent on the level-1 solve;
-  the rows drift to within 1.9e-5."""
+  the rows drift to within 1.9e-5;
+- test_3d_chem and test_3d_photoi_chem, the species whose density follows
+  the local ionization rate (N2+, converted to N4+ within ~30 ps, so
+  N2+ ~ alpha(E): an error of 1e-6 in E moves it by ~1e-5), the negative
+  ions formed at low fields (O-, O3-) and dt: within 4e-5, every other
+  column within compare_logs' tolerance;
+- test_2d_photoi_chem (device only), rows 5-7 as in test_2d_photoi, and
+  N2+: within 4e-5."""
 import numpy as np
 import pytest
 
@@ -31,6 +45,20 @@ import rtest as R
 
 TOL = dict(rtol=1e-5, atol=1e-8)  # tools/compare_logs.py:14-28
 M_MIN_COLS = [5, 8, 11]           # sum(M_min), sum(M_min^2), max(M_min)
+# with the 8 species of the air networks: N2+, O-, O3-, N4+ (each a mean,
+# a mean of squares and a maximum) and dt
+CHEM_NS = 8
+CHEM_FIELD_COLS = [2] + [3 + q * CHEM_NS + k for q in range(3) for k in (1, 4, 5, 6)]
+# rtest logs within compare_logs' tolerance on every row and column
+EXACT = ("test_cyl", "test_2d", "test_cyl_chem", "test_cyl_photoi_chem")
+# every case and the largest relative departure allowed where not exact
+BOUND = {"test_cyl": 1e-5, "test_2d": 1e-5, "test_cyl_chem": 1e-5, "test_cyl_photoi_chem": 1e-5, "test_3d": 2e-5,
+         "test_2d_photoi": 3e-5, "test_3d_chem": 5e-5, "test_3d_photoi_chem": 5e-5, "test_2d_photoi_chem": 5e-5}
+# the cases the oracle replays in the CPU suite (test_2d_photoi_chem, eight
+# species on a growing 2D streamer with photoionization, takes minutes on
+# the host: it runs on the device only)
+CPU_CASES = ["test_cyl", "test_2d", "test_3d", "test_2d_photoi", "test_cyl_chem", "test_cyl_photoi_chem",
+             "test_3d_chem", "test_3d_photoi_chem"]
 
 
 def rel(a, b):
@@ -42,19 +70,21 @@ def check_rows(name, rows):
     assert rows.shape == ref.shape, (name, rows.shape, ref.shape)
     err = rel(rows, ref)
     print(name, "max relative error per column", np.array2string(err.max(axis=0), precision=2), flush=True)
-    if name in ("test_cyl", "test_2d"):
+    if name in EXACT:
         assert np.isclose(rows, ref, **TOL).all()
     elif name == "test_3d":
         other = [c for c in range(rows.shape[1]) if c not in M_MIN_COLS]
         assert np.isclose(rows[:, other], ref[:, other], **TOL).all()
-        assert err[:, M_MIN_COLS].max() < 2e-5
+    elif name in ("test_3d_chem", "test_3d_photoi_chem"):
+        other = [c for c in range(rows.shape[1]) if c not in CHEM_FIELD_COLS]
+        assert np.isclose(rows[:, other], ref[:, other], **TOL).all()
     else:
         assert np.isclose(rows[:5], ref[:5], **TOL).all()
-        assert err.max() < 3e-5
+    assert err.max() < BOUND[name]
     return err
 
 
-@pytest.mark.parametrize("name", ["test_cyl", "test_2d", "test_3d", "test_2d_photoi"])
+@pytest.mark.parametrize("name", CPU_CASES)
 def test_oracle_replays_reference_log(A, S, name):
     rows = R.Run(name, "oracle").run()
     check_rows(name, rows)
