@@ -588,9 +588,14 @@ struct Impl : ImplBase {
   // classes are timed)
   template <int BB, typename K, typename... Args>
   int items_timed(int kind, double bytes, int cls, K kernel, int nw, Args... args) {
+    return items_nt<BS * BB, BB>(kind, bytes, cls, kernel, nw, args...);
+  }
+  // items_timed() of a kernel with NT threads per workgroup
+  template <int NT, int BB, typename K, typename... Args>
+  int items_nt(int kind, double bytes, int cls, K kernel, int nw, Args... args) {
     if (!c.timing || c.capture_pairs) {
       if (nw <= 0) return AFGPU_OK;
-      hipLaunchKernelGGL(kernel, dim3(grid_for(cls, nw, BB)), dim3(BS * BB), 0, c.cur, args..., nw);
+      hipLaunchKernelGGL(kernel, dim3(grid_for(cls, nw, BB)), dim3(NT), 0, c.cur, args..., nw);
       HIPCHK(hipGetLastError());
       return AFGPU_OK;
     }
@@ -598,7 +603,7 @@ struct Impl : ImplBase {
     if (c.class_timing) kind = 3 + cls;
     hipEvent_t a, b;
     RCHK(timer_events(c, kind, bytes, &a, &b));
-    hipExtLaunchKernelGGL(kernel, dim3(grid_for(cls, nw, BB)), dim3(BS * BB), 0, c.cur, a, b, 0, args..., nw);
+    hipExtLaunchKernelGGL(kernel, dim3(grid_for(cls, nw, BB)), dim3(NT), 0, c.cur, a, b, 0, args..., nw);
     HIPCHK(hipGetLastError());
     return AFGPU_OK;
   }
@@ -706,8 +711,8 @@ struct Impl : ImplBase {
       const int stored = (stored_first && n == 1 && !Lay<D, NC, T>::SPLIT) ? 1 : 0;
       if constexpr (D == 3 && Lay<D, NC, T>::SPLIT && BS == NC * NC) {
         if (k == 1 && !c.box_lds) {
-          RCHK(items_timed<1>(0, bytes, KC_GSRB, k_gsrb_split<D, NC, BS>, l.n, c.list(l), (const DBox *)c.d_boxes,
-                              v.phi, (const T *)v.rhs, n, v.lambda2, v.m));
+          RCHK((items_nt<GSRB_SPLIT_NT, 1>(0, bytes, KC_GSRB, k_gsrb_split<D, NC, GSRB_SPLIT_NT>, l.n, c.list(l),
+                                           (const DBox *)c.d_boxes, v.phi, (const T *)v.rhs, n, v.lambda2, v.m)));
           continue;
         }
       }

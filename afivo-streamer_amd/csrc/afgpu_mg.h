@@ -1213,18 +1213,53 @@ __global__ void __launch_bounds__(BS * BB) k_gsrb_lds(const int *__restrict__ id
   }
 }
 
+// The coarse values ghost_coarse_load reads for ghost (a, b) of a 3D face at
+// a refinement boundary (mg_sides_rb, afivo/src/m_af_multigrid.f90:294-461;
+// af_gc_interp(_lim), afivo/src/m_af_ghostcell.f90:394-605), in its order,
+// from a PP x PP patch of the coarse neighbour's face layer: patch cell
+// pa + PP pb holds the coarse cell at tangential (co[t0] + pa, co[t1] + pb),
+// co the box's offset within its parent.
+template <int NC>
+__device__ inline CoarseVals<float> coarse_patch_vals(const float *cp, const VarBc &m, int a, int b) {
+  constexpr int PP = NC / 2 + 2;
+  auto at = [&](int qa, int qb) { return cp[qa + PP * qb]; };
+  CoarseVals<float> cv = {{0.0f, 0.0f, 0.0f, 0.0f, 0.0f}};
+  if (m.rb == RB_MG) {
+    const int i = (a + 1) >> 1, j = (b + 1) >> 1;
+    cv.v[0] = at(i + 1, j);
+    cv.v[1] = at(i - 1, j);
+    cv.v[2] = at(i, j + 1);
+    cv.v[3] = at(i, j - 1);
+    cv.v[4] = at(i, j);
+    return cv;
+  }
+  const int a1 = (a + 1) >> 1, a2 = a1 + 1 - 2 * (a & 1);
+  const int b1 = (b + 1) >> 1, b2 = b1 + 1 - 2 * (b & 1);
+  cv.v[0] = at(a1, b1);
+  cv.v[1] = at(a2, b1);
+  cv.v[2] = at(a1, b2);
+  return cv;
+}
+
 // k_gsrb_lds for 3D boxes in the colour-split single-precision layout.  A
 // half-sweep reads only the other colour's interior, the face ghosts next to
 // the updated cells and the updated colour's rhs; the updated colour's old
-// values are never read.  The other colour's block is staged in LDS as it
-// lies in the pool (16-byte loads, no reordering), the ghosts in a face
-// array, and each thread updates four consecutive cells of the colour block
-// per pass and stores them as one 16-byte word.  The stencil expression and
-// its operands are those of k_gsrb_lds, so the results are identical; the
-// box's registers and LDS (14 KB) are a fraction of k_gsrb_lds's, so all
-// boxes of a finest level run at once.
-template <int D, int NC, int BS>
-__global__ void __launch_bounds__(BS) k_gsrb_split(const int *__restrict__ ids, const DBox *__restrict__ boxes,
+// values are never read (but for the cells next to faces without a
+// same-level neighbour, whose ghosts use them).  Everything a box reads goes
+// straight from memory into LDS (global_load_lds, no registers held): the
+// other colour's block and the rhs as they lie in the pool (16-byte words),
+// and for each face either the same-level neighbour's cells across it (one
+// value per ghost, already the ghost) or the box's own cells next to it and,
+// at a refinement boundary, a patch of the coarse neighbour's face layer.
+// After one barrier those faces' ghosts are formed, after a second each
+// thread updates four consecutive cells of the colour block per pass and
+// stores them as one 16-byte word.  The stencil and ghost expressions and
+// their operands are those of k_gsrb_lds, so the results are identical.
+// With few registers and 25 KB of LDS, six boxes fit on a CU: all 1536
+// boxes of a 256-CU finest level run at once.
+constexpr int GSRB_SPLIT_NT = 256;
+template <int D, int NC, int NT>
+__global__ void __launch_bounds__(NT) k_gsrb_split(const int *__restrict__ ids, const DBox *__restrict__ boxes,
                                                    float *phi, const float *__restrict__ rhs, int redblack,
                                                    float lambda2, VarBc m, int nw) {
   static_assert(D == 3, "the colour blocks of 3D boxes");
@@ -1232,16 +1267,27 @@ __global__ void __launch_bounds__(BS) k_gsrb_split(const int *__restrict__ ids, 
   using L = Lay<D, NC, float>;
   constexpr int HALF = L::HALF;
   constexpr int NV = HALF / 4;               // 16-byte words of a colour block
-  constexpr int NW = NV / BS;                // per thread
-  constexpr int NGH = D * G::NTAN;           // ghosts next to updated cells: half of each face
-  constexpr int NGT = (NGH + BS - 1) / BS;   // per thread
-  static_assert(NV % BS == 0 && (L::STRIDE * sizeof(float)) % 16 == 0, "16-byte words of whole boxes");
+  constexpr int NW = NV / NT;                // per thread
+  constexpr int NE = G::NTAN / 2;            // ghosts of a face next to updated cells
+  constexpr int NGH = 2 * D * NE;            // of the box
+  constexpr int NGT = (NGH + NT - 1) / NT;   // per thread
+  constexpr int PP = NC / 2 + 2;             // coarse patch of a face: its coarse cells and a ring
+  static_assert(NV % NT == 0 && (L::STRIDE * sizeof(float)) % 16 == 0, "16-byte words of whole boxes");
+  static_assert(NE % 64 == 0 && NT % 64 == 0 && PP * PP <= NE, "a face's ghosts in whole wavefronts");
   __shared__ float4 s_other4[NV];
   __shared__ float4 s_rhs4[NV];
-  __shared__ float s_gh[2 * D][G::NTAN];
+  __shared__ float s_gh[2 * D][NE];  // the ghosts: face nb, item e
+  __shared__ float s_x1[2 * D][NE];  // the box's cells next to a face without a same-level neighbour
+  __shared__ float s_cp[2 * D][NE];  // the coarse patch of a face at a refinement boundary
   const float *s_other = reinterpret_cast<const float *>(s_other4);
   const int t = threadIdx.x, wv = __builtin_amdgcn_readfirstlane(t / 64);  // (the wavefront)
   const int uc = redblack & 1, oc = 1 - uc;  // updated cells: (i + j + k + redblack) even
+  // ghost item e of face nb: tangential (a, b), next to an updated cell
+  auto item_ab = [&](int nb, int e, int &a, int &b) {
+    const int pd = (nb % 2) == 0 ? 1 : NC;
+    b = e / (NC / 2) + 1;
+    a = 2 * (e % (NC / 2)) + 1 + ((uc + pd + b + 1) & 1);  // (pd + a + b + uc) even
+  };
   for (int w = blockIdx.x; w < nw; w += gridDim.x) {
     const int id = ids[w];
     if (id <= 0) continue;  // run masked out
@@ -1250,56 +1296,66 @@ __global__ void __launch_bounds__(BS) k_gsrb_split(const int *__restrict__ ids, 
     float *u = phi + (size_t)(id - 1) * L::STRIDE;
     const float4 *ob = reinterpret_cast<const float4 *>(u + oc * HALF);
     const float4 *rb = reinterpret_cast<const float4 *>(rhs + (size_t)(id - 1) * L::STRIDE + uc * HALF);
-    // the other colour's block and the rhs straight into LDS (each lane
-    // 16 bytes, a wavefront 1 KB; no registers held)
     for (int q = 0; q < NW; q++) {
-      __builtin_amdgcn_global_load_lds((const void *)(ob + t + q * BS),
-                                       (__attribute__((address_space(3))) void *)(s_other4 + 64 * wv + q * BS), 16, 0, 0);
-      __builtin_amdgcn_global_load_lds((const void *)(rb + t + q * BS),
-                                       (__attribute__((address_space(3))) void *)(s_rhs4 + 64 * wv + q * BS), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void *)(ob + t + q * NT),
+                                       (__attribute__((address_space(3))) void *)(s_other4 + 64 * wv + q * NT), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void *)(rb + t + q * NT),
+                                       (__attribute__((address_space(3))) void *)(s_rhs4 + 64 * wv + q * NT), 16, 0, 0);
     }
     int nbid[2 * D];
     for (int nb = 0; nb < 2 * D; nb++) nbid[nb] = bx.nb[nb];
-    // the ghosts next to updated cells, item g: face nb (a face's NTAN / 2
-    // ghosts are whole wavefronts, so nb is uniform), tangential (a, b)
-    auto ghost_pos = [&](int g, int &nb, int &a, int &b) {
-      nb = __builtin_amdgcn_readfirstlane(g / (G::NTAN / 2));
-      const int e = g % (G::NTAN / 2);
-      const int pd = (nb % 2) == 0 ? 1 : NC;
-      b = e / (NC / 2) + 1;
-      a = 2 * (e % (NC / 2)) + 1 + ((uc + pd + b + 1) & 1);  // (pd + a + b + uc) even
-    };
-    // every item's loads first: the same-level neighbour's cell across the
-    // face, or the coarse values and the box's two cells next to the face
-    float x0[NGT], x1[NGT], x2[NGT];
-    CoarseVals<float> cv[NGT];
+    // the faces' values, one per lane: a wavefront's items lie on one face
+    // (its face, its first item and the neighbour there are uniform), and
+    // item e of a wavefront lands at e of the face's arrays
     for (int q = 0; q < NGT; q++) {
-      const int g = t + q * BS;
-      int nb, a, b;
-      ghost_pos(g, nb, a, b);
-      if (g >= NGH) continue;
+      const int g0 = 64 * wv + q * NT;
+      if (g0 >= NGH) continue;
+      const int nb = g0 / NE, e0 = g0 % NE, e = e0 + (t & 63);
+      const int nbv = __builtin_amdgcn_readfirstlane(nbid[nb]);
       const bool low = (nb % 2) == 0;
-      int p[3];
-      if (nbid[nb] > 0) {
+      int a, b, p[3];
+      item_ab(nb, e, a, b);
+      if (nbv > 0) {  // the same-level neighbour's cell across the face: the ghost
         face_pos<D>(nb, a, b, low ? NC : 1, p);
-        x0[q] = phi[(size_t)(nbid[nb] - 1) * L::STRIDE + L::in_at(p[0], p[1], p[2])];
+        const float *src = phi + (size_t)(nbv - 1) * L::STRIDE + L::in_at(p[0], p[1], p[2]);
+        __builtin_amdgcn_global_load_lds((const void *)src, (__attribute__((address_space(3))) void *)&s_gh[nb][e0],
+                                         4, 0, 0);
       } else {
-        cv[q] = ghost_coarse_load<D, NC, float>(bx, phi, m, nb, a, b);
         face_pos<D>(nb, a, b, low ? 1 : NC, p);
-        x1[q] = u[L::in_at(p[0], p[1], p[2])];
-        face_pos<D>(nb, a, b, low ? 2 : NC - 1, p);
-        x2[q] = u[L::in_at(p[0], p[1], p[2])];
+        __builtin_amdgcn_global_load_lds((const void *)(u + L::in_at(p[0], p[1], p[2])),
+                                         (__attribute__((address_space(3))) void *)&s_x1[nb][e0], 4, 0, 0);
+        if (nbv == 0 && e < PP * PP) {  // refinement boundary: coarse patch cell e
+          const int d = nb / 2;
+          int t0, t1;
+          tangential(D, d, t0, t1);
+          int qc[3];
+          qc[d] = low ? NC : 1;
+          qc[t0] = ((bx.ix[t0] - 1) & 1) * (NC / 2) + e % PP;
+          qc[t1] = ((bx.ix[t1] - 1) & 1) * (NC / 2) + e / PP;
+          const float *P = phi + (size_t)(__builtin_amdgcn_readfirstlane(bx.pnb[nb]) - 1) * L::STRIDE;
+          __builtin_amdgcn_global_load_lds((const void *)(P + L::at(qc[0], qc[1], qc[2])),
+                                           (__attribute__((address_space(3))) void *)&s_cp[nb][e0], 4, 0, 0);
+        }
       }
     }
+    __syncthreads();
+    // the ghosts of faces without a same-level neighbour (the box's second
+    // cell along the normal is of the other colour: in the staged block)
     for (int q = 0; q < NGT; q++) {
-      const int g = t + q * BS;
-      int nb, a, b;
-      ghost_pos(g, nb, a, b);
-      if (g >= NGH) continue;
-      s_gh[nb][(a - 1) + NC * (b - 1)] =
-          nbid[nb] > 0 ? x0[q]
-                       : ghost_finish<D, NC, float>(bx, m, nb, ghost_coarse_form<D, NC, float>(bx, m, nb, a, b, cv[q]),
-                                                    x1[q], x2[q]);
+      const int g0 = 64 * wv + q * NT;
+      if (g0 >= NGH) continue;
+      const int nb = g0 / NE, e = g0 % NE + (t & 63);
+      const int nbv = __builtin_amdgcn_readfirstlane(nbid[nb]);
+      if (nbv > 0) continue;
+      const bool low = (nb % 2) == 0;
+      int a, b, p[3];
+      item_ab(nb, e, a, b);
+      face_pos<D>(nb, a, b, low ? 2 : NC - 1, p);
+      const float x2 = s_other[L::in_at(p[0], p[1], p[2]) - oc * HALF];
+      CoarseVals<float> cv = {{0.0f, 0.0f, 0.0f, 0.0f, 0.0f}};
+      if (nbv == 0) cv = coarse_patch_vals<NC>(s_cp[nb], m, a, b);
+      s_gh[nb][e] =
+          ghost_finish<D, NC, float>(bx, m, nb, ghost_coarse_form<D, NC, float>(bx, m, nb, a, b, cv), s_x1[nb][e], x2);
     }
     __syncthreads();
     const float inv_c1 = 1.0f / st.c[0];
@@ -1309,7 +1365,7 @@ __global__ void __launch_bounds__(BS) k_gsrb_split(const int *__restrict__ ids, 
       // i = i0 + 2 (m0 + c); their other-colour neighbours in the row are
       // the five values from B on, in the rows and planes beside it four
       // consecutive values (one 16-byte word), unless they are ghosts
-      const int h0 = 4 * (t + q * BS);
+      const int h0 = 4 * (t + q * NT);
       const int row = h0 / (NC / 2), m0 = h0 % (NC / 2);
       const int j = row % NC + 1, k = row / NC + 1;
       const int i0 = 1 + ((1 + j + k + uc) & 1);
@@ -1319,8 +1375,10 @@ __global__ void __launch_bounds__(BS) k_gsrb_split(const int *__restrict__ ids, 
       // consecutive lanes read consecutive words, free of bank conflicts)
       const bool lo_gh = i0 == 1 && m0 == 0, hi_gh = i0 == 2 && m0 == NC / 2 - 4;
       const float4 xc = s_other4[(R + m0) / 4];
-      const float xe = (i0 == 1) ? (lo_gh ? s_gh[0][row] : s_other[max(B, 0)])
-                                 : (hi_gh ? s_gh[1][row] : s_other[min(B + 4, HALF - 1)]);
+      // (a face's ghosts are stored every other cell: cell r of the face
+      // at r / 2)
+      const float xe = (i0 == 1) ? (lo_gh ? s_gh[0][row >> 1] : s_other[max(B, 0)])
+                                 : (hi_gh ? s_gh[1][row >> 1] : s_other[min(B + 4, HALF - 1)]);
       float xw[5];
       if (i0 == 1) {
         xw[0] = xe; xw[1] = xc.x; xw[2] = xc.y; xw[3] = xc.z; xw[4] = xc.w;
@@ -1330,7 +1388,7 @@ __global__ void __launch_bounds__(BS) k_gsrb_split(const int *__restrict__ ids, 
       float ym[4], yp[4], zm[4], zp[4];
       auto four = [&](float *v, bool ghost, int nb, int r0, int off) {
         if (ghost) {
-          for (int c = 0; c < 4; c++) v[c] = s_gh[nb][r0 + i0 - 1 + 2 * (m0 + c)];
+          for (int c = 0; c < 4; c++) v[c] = s_gh[nb][(r0 + i0 - 1 + 2 * (m0 + c)) >> 1];
         } else {
           const float4 w = s_other4[min(max(R + off + m0, 0), HALF - 4) / 4];
           v[0] = w.x;
@@ -1343,14 +1401,14 @@ __global__ void __launch_bounds__(BS) k_gsrb_split(const int *__restrict__ ids, 
       four(yp, j == NC, 3, NC * (k - 1), NC / 2);
       four(zm, k == 1, 4, NC * (j - 1), -NC * NC / 2);
       four(zp, k == NC, 5, NC * (j - 1), NC * NC / 2);
-      const float4 rv = s_rhs4[t + q * BS];
+      const float4 rv = s_rhs4[t + q * NT];
       const float rr[4] = {rv.x, rv.y, rv.z, rv.w};
       float res[4];
       for (int c = 0; c < 4; c++)
         res[c] = (rr[c] - st.c[1] * xw[c] - st.c[2] * xw[c + 1] - st.c[3] * ym[c] - st.c[4] * yp[c] -
                   st.c[5] * zm[c] - st.c[6] * zp[c]) *
                  inv_c1;
-      ub[t + q * BS] = make_float4(res[0], res[1], res[2], res[3]);
+      ub[t + q * NT] = make_float4(res[0], res[1], res[2], res[3]);
     }
     __syncthreads();  // (the LDS is reused by the next item)
   }
