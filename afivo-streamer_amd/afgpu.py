@@ -39,6 +39,11 @@ def VAR_SPECIES(k):
     """Species k of a reaction list in variable lists (AFGPU_VAR_SPECIES)."""
     return 100 + k
 FC_FLUX, FC_FIELD = 0, 1
+
+
+def VAR_FACE(ifc):
+    """Face variable ifc in the box I/O variable lists (AFGPU_VAR_FACE)."""
+    return 200 + ifc
 OP_POISSON, OP_HELMH = 0, 1
 MAX_MODES = 3
 OPT_GRAPHS = 1
@@ -712,19 +717,26 @@ class Path:
         self._call("get_species", k, state, len(ids), _ip(ids), _dp(out))
         return out
 
+    def box_words(self, ivs):
+        """Words of one box's data for the variable list ivs (a face
+        variable AFGPU_VAR_FACE(ifc) takes nface, any other ncell)."""
+        return sum(self.nface if 200 <= iv < 200 + 2 else self.ncell for iv in ivs)
+
     def get_boxes(self, ivs, ids=None):
-        """Whole boxes of the variables ivs in af_write_tree's order:
-        array (len(ids), len(ivs), ncell)."""
+        """Whole boxes of the variables ivs in af_write_tree's order: array
+        (len(ids), len(ivs), ncell) for cell variables only, else
+        (len(ids), box_words(ivs))."""
         ivs = np.ascontiguousarray(ivs, np.int32)
         ids = self._ids(ids)
-        out = np.zeros((len(ids), len(ivs), self.ncell))
+        cells = self.box_words(ivs) == len(ivs) * self.ncell
+        out = np.zeros((len(ids), len(ivs), self.ncell) if cells else (len(ids), self.box_words(ivs)))
         self._call("get_boxes", len(ivs), _ip(ivs), len(ids), _ip(ids), _dp(out))
         return out
 
     def put_boxes(self, ivs, data, ids=None):
         ivs = np.ascontiguousarray(ivs, np.int32)
         ids = self._ids(ids)
-        data = np.ascontiguousarray(data, np.float64).reshape(len(ids), len(ivs), self.ncell)
+        data = np.ascontiguousarray(data, np.float64).reshape(len(ids), self.box_words(ivs))
         self._call("put_boxes", len(ivs), _ip(ivs), len(ids), _ip(ids), _dp(data))
 
     def get_time_state(self):

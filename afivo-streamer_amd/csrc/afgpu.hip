@@ -3494,11 +3494,32 @@ int afgpu_write_regression_log(afgpu_t *h, const char *filename, int out_cnt, in
   return std::fclose(f) == 0 ? AFGPU_OK : AFGPU_ERR_ARG;
 }
 
-// Box data in af_write_tree's order (afivo/src/m_af_output.f90:90-114)
+// Box data in af_write_tree's order (afivo/src/m_af_output.f90:90-114):
+// cell variables (species included) and face variables (AFGPU_VAR_FACE)
 static int box_io(afgpu_t *h, int n_vars, const int *ivs, int n_ids, const int *ids, double *data, bool put) {
   RCHK(ready(h));
-  VarPtrs vp;
-  RCHK(check_vars(h, n_vars, ivs, vp, put ? "afgpu_put_boxes" : "afgpu_get_boxes"));
+  const char *who = put ? "afgpu_put_boxes" : "afgpu_get_boxes";
+  if (n_vars < 1 || n_vars > AFGPU_DIAG_MAX_VARS || !ivs) {
+    set_err(std::string(who) + ": 1 to AFGPU_DIAG_MAX_VARS variables");
+    return AFGPU_ERR_ARG;
+  }
+  BoxIoVars bv;
+  bv.n = n_vars;
+  bv.total = 0;
+  for (int v = 0; v < n_vars; v++) {
+    const int f = ivs[v] - AFGPU_VAR_FACE(0);
+    if (f >= 0 && f < AFGPU_N_FC) {
+      bv.v[v] = h->fc[f];
+      bv.len[v] = (int)h->nface;
+    } else {
+      VarPtrs one;
+      RCHK(check_vars(h, 1, ivs + v, one, who));
+      bv.v[v] = one.v[0];
+      bv.len[v] = (int)h->ncell;
+    }
+    bv.off[v] = bv.total;
+    bv.total += bv.len[v];
+  }
   if (n_ids < 0 || (n_ids > 0 && (!ids || !data))) return AFGPU_ERR_ARG;
   if (n_ids == 0) return AFGPU_OK;
   for (int q = 0; q < n_ids; q++)
@@ -3506,7 +3527,7 @@ static int box_io(afgpu_t *h, int n_vars, const int *ivs, int n_ids, const int *
       set_err("box I/O: box id out of range");
       return AFGPU_ERR_ARG;
     }
-  const size_t words = (size_t)n_ids * n_vars * h->ncell;
+  const size_t words = (size_t)n_ids * bv.total;
   if (words > h->io_cap) {
     (void)hipFree(h->d_io);
     h->d_io = nullptr;
@@ -3519,7 +3540,7 @@ static int box_io(afgpu_t *h, int n_vars, const int *ivs, int n_ids, const int *
   const int nw = n_ids * n_vars;
   if (put) HIPCHK(hipMemcpyAsync(h->d_io, data, words * sizeof(double), hipMemcpyHostToDevice, h->stream));
   hipLaunchKernelGGL(k_box_io, dim3(std::min(nw, 8 * h->n_cu)), dim3(256), 0, h->stream, (const int *)h->d_ref_ids,
-                     vp, (int)h->ncell, h->d_io, (int)put, nw);
+                     bv, h->d_io, (int)put, nw);
   HIPCHK(hipGetLastError());
   if (!put) HIPCHK(hipMemcpyAsync(data, h->d_io, words * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   return sync_stream(h);

@@ -118,18 +118,25 @@ __global__ void __launch_bounds__(BS) k_diag_fold(const int *__restrict__ ids, c
 }
 
 // Box-major staging of whole boxes (ghost cells included): item w is box
-// w / nv, variable w % nv; to_pool = 0 gathers the pool into buf, 1
-// scatters buf into the pool.  n_cell words per item.
-__global__ void __launch_bounds__(256) k_box_io(const int *__restrict__ ids, VarPtrs vars, int n_cell, double *buf,
-                                                int to_pool, int nw) {
+// w / n, variable w % n; variable v has len[v] words per box in its pool
+// and starts at off[v] within a box's total words in buf.  to_pool = 0
+// gathers the pools into buf, 1 scatters buf into the pools.
+struct BoxIoVars {
+  double *v[DIAG_MAX_VARS];
+  int len[DIAG_MAX_VARS];
+  int off[DIAG_MAX_VARS];
+  int n, total;
+};
+__global__ void __launch_bounds__(256) k_box_io(const int *__restrict__ ids, BoxIoVars vars, double *buf, int to_pool,
+                                                int nw) {
   for (int w = blockIdx.x; w < nw; w += gridDim.x) {
-    const int id = ids[w / vars.n];
-    double *pool = vars.v[w % vars.n] + (size_t)(id - 1) * n_cell;
-    double *b = buf + (size_t)w * n_cell;
+    const int box = w / vars.n, v = w % vars.n, n = vars.len[v];
+    double *pool = vars.v[v] + (size_t)(ids[box] - 1) * n;
+    double *b = buf + (size_t)box * vars.total + vars.off[v];
     if (to_pool)
-      for (int q = threadIdx.x; q < n_cell; q += 256) pool[q] = b[q];
+      for (int q = threadIdx.x; q < n; q += 256) pool[q] = b[q];
     else
-      for (int q = threadIdx.x; q < n_cell; q += 256) b[q] = pool[q];
+      for (int q = threadIdx.x; q < n; q += 256) b[q] = pool[q];
   }
 }
 

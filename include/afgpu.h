@@ -325,11 +325,17 @@ int afgpu_write_regression_log(afgpu_t *h, const char *filename, int out_cnt, in
 
 /* Box data for checkpoints and restarts, in the order af_write_tree writes
  * it (afivo/src/m_af_output.f90:90-114: box by box, and within a box the
- * variables in turn, each as box%cc(DTIMES(:), iv)); af_read_tree reads
- * the same order (m_af_output.f90:269-295).  data holds
- * n_ids * n_vars * (n_cell + 2)^D doubles; at most AFGPU_DIAG_MAX_VARS
- * variables per call.  One gather (or scatter) on the device and one copy
- * over the host link. */
+ * variables in turn, each cell variable as box%cc(DTIMES(:), iv), each
+ * face variable as box%fc(DTIMES(:), :, ifc)); af_read_tree reads the same
+ * order (m_af_output.f90:269-295).  A variable is a cell variable index,
+ * AFGPU_VAR_SPECIES(k), or AFGPU_VAR_FACE(ifc) for the face variable ifc
+ * (AFGPU_FC_FLUX, AFGPU_FC_FIELD); the reference writes a box's cell
+ * variables before its face variables, so a checkpoint lists them in that
+ * order.  data holds, per box, (n_cell + 2)^D doubles per cell variable
+ * and D (n_cell + 1)^D per face variable, the boxes in the order of ids;
+ * at most AFGPU_DIAG_MAX_VARS variables per call.  One gather (or scatter)
+ * on the device and one copy over the host link. */
+#define AFGPU_VAR_FACE(ifc) (200 + (ifc))
 int afgpu_get_boxes(afgpu_t *h, int n_vars, const int *ivs, int n_ids, const int *ids, double *data);
 int afgpu_put_boxes(afgpu_t *h, int n_vars, const int *ivs, int n_ids, const int *ids, const double *data);
 

@@ -34,6 +34,10 @@ module m_afgpu
   integer(c_int), parameter, public :: afgpu_ref_derefine = -2, afgpu_ref_refine = 2, &
        afgpu_ref_max_items = 8
   integer(c_int), parameter, public :: afgpu_diag_max_vars = 8
+  ! variable codes of the diagnostics and box I/O lists: species k of a
+  ! reaction list is afgpu_var_species0 + k (AFGPU_VAR_SPECIES(k)), face
+  ! variable ifc is afgpu_var_face0 + ifc (AFGPU_VAR_FACE(ifc))
+  integer(c_int), parameter, public :: afgpu_var_species0 = 100, afgpu_var_face0 = 200
   integer(c_int), parameter, public :: afgpu_noalloc_scratch_words = 4194304
   integer(c_int), parameter, public :: afgpu_n_kernel_classes = 8
   integer(c_int), parameter, public :: afgpu_chem_max_species = 100, &

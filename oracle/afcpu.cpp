@@ -2730,25 +2730,48 @@ int afcpu_regression_values(afcpu_t *h, int n_vars, const int *ivs, double *out)
   return AFGPU_OK;
 }
 
-// Box data in af_write_tree's order (afivo/src/m_af_output.f90:90-114)
+// Box data in af_write_tree's order (afivo/src/m_af_output.f90:90-114):
+// per box, the variables in turn, box%cc(DTIMES(:), iv) for a cell variable
+// or a species, box%fc(DTIMES(:), :, ifc) for AFGPU_VAR_FACE(ifc)
 static int box_io(afcpu_t *h, int n_vars, const int *ivs, int n_ids, const int *ids, double *data, bool put) {
   CHECK_H(h);
+  Ctx &c = h->c;
   if (n_vars < 1 || n_vars > AFGPU_DIAG_MAX_VARS || !ivs || n_ids < 0 || (n_ids > 0 && (!ids || !data)))
     return AFGPU_ERR_ARG;
-  for (int v = 0; v < n_vars; v++)
-    if (ivs[v] < 0 || ivs[v] >= AFGPU_N_CC) return AFGPU_ERR_ARG;
   for (int q = 0; q < n_ids; q++)
-    if (ids[q] < 1 || ids[q] > h->c.n_boxes) return AFGPU_ERR_ARG;
-  const size_t nc = h->c.ncell;
-  for (int q = 0; q < n_ids; q++)
-    for (int v = 0; v < n_vars; v++) {
-      double *b = data + ((size_t)q * n_vars + v) * nc;
-      double *u = h->c.C(ivs[v], ids[q]);
-      if (put)
-        std::memcpy(u, b, nc * sizeof(double));
-      else
-        std::memcpy(b, u, nc * sizeof(double));
+    if (ids[q] < 1 || ids[q] > c.n_boxes) return AFGPU_ERR_ARG;
+  // each variable's box data: pool, words per box
+  std::vector<std::vector<double> *> pool(n_vars);
+  std::vector<size_t> len(n_vars);
+  size_t total = 0;
+  for (int v = 0; v < n_vars; v++) {
+    const int f = ivs[v] - AFGPU_VAR_FACE(0);
+    const int sv = species_var(c, ivs[v]);
+    if (f >= 0 && f < AFGPU_N_FC) {
+      pool[v] = &c.fc[f];
+      len[v] = (size_t)c.D * c.nface;
+    } else if (sv != AFGPU_N_CC) {
+      pool[v] = sv >= 0 ? &c.cc[sv] : &c.xs[(size_t)(-1 - sv)];
+      len[v] = c.ncell;
+    } else if (ivs[v] >= 0 && ivs[v] < AFGPU_N_CC) {
+      pool[v] = &c.cc[ivs[v]];
+      len[v] = c.ncell;
+    } else {
+      return AFGPU_ERR_ARG;
     }
+    total += len[v];
+  }
+  for (int q = 0; q < n_ids; q++) {
+    double *b = data + (size_t)q * total;
+    for (int v = 0; v < n_vars; v++) {
+      double *u = pool[v]->data() + (size_t)(ids[q] - 1) * len[v];
+      if (put)
+        std::memcpy(u, b, len[v] * sizeof(double));
+      else
+        std::memcpy(b, u, len[v] * sizeof(double));
+      b += len[v];
+    }
+  }
   return AFGPU_OK;
 }
 int afcpu_get_boxes(afcpu_t *h, int n_vars, const int *ivs, int n_ids, const int *ids, double *data) {

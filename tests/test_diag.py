@@ -161,6 +161,46 @@ def test_regression_max_reads_leaf_interiors_only(A, S, name):
     assert p.regression_values([A.CC_NEG_ION])[0, 0, 2] == 7.5
 
 
+def test_box_io_faces_and_species_oracle(A, S, CH):
+    """Box data of cell variables, reaction-list species and face variables
+    (AFGPU_VAR_FACE) in one list, af_write_tree's order
+    (afivo/src/m_af_output.f90:90-114): per box the variables in turn, a
+    cell variable or species (n_cell + 2)^D words, a face variable
+    D (n_cell + 1)^D words as box%fc(DTIMES(:), :, ifc); each block equals
+    the variable's own transfer, and put_boxes writes the same layout back.
+    Species 3 and up live outside the standard pools."""
+    import os
+    from conftest import ROOT
+    case = AMR_CASES["xyz2d"](S)
+    p = oracle_case(A, S, case)
+    N = case.params.gas_number_density
+    E = S.analytic_transport()[0]
+    net = CH.read_network(os.path.join(ROOT, "tests", "golden", "air_chemistry_v1.txt"), {"N2": 0.8, "O2": 0.2}, N,
+                          0.0, E[-1] * 1e21 / N, 1000)
+    ns = len(net.species)
+    chem = CH.Chem("oracle", ns, net.reactions, 0.0, E[-1] * 1e21 / N, net.fld_cols, None, 300.0)
+    p.set_chemistry(chem, net.charges)
+    n = case.extra["export"]["n_boxes"]
+    rng = np.random.default_rng(21)
+    for k in range(ns):
+        p.put_species(k, rng.normal(size=(n, p.ncell)))
+    for f in (A.FC_FLUX, A.FC_FIELD):
+        p.put_fc(f, rng.normal(size=(n, p.nface)))
+    p.put_cc(A.CC_PHI, rng.normal(size=(n, p.ncell)))
+    ivs = [A.CC_PHI, A.VAR_SPECIES(1), A.VAR_SPECIES(5), A.VAR_FACE(A.FC_FLUX), A.VAR_FACE(A.FC_FIELD)]
+    ids = np.array([4, 1, 7, 2], np.int32)
+    got = p.get_boxes(ivs, ids)
+    assert got.shape == (len(ids), 3 * p.ncell + 2 * p.nface)
+    blocks = [p.get_cc(A.CC_PHI, ids), p.get_species(1, ids=ids), p.get_species(5, ids=ids),
+              p.get_fc(A.FC_FLUX, ids), p.get_fc(A.FC_FIELD, ids)]
+    assert np.array_equal(got, np.concatenate(blocks, axis=1))
+    new = rng.normal(size=got.shape)
+    p.put_boxes(ivs, new, ids)
+    assert np.array_equal(p.get_boxes(ivs, ids), new)
+    assert np.array_equal(p.get_species(5, ids=ids), new[:, 2 * p.ncell:3 * p.ncell])
+    assert np.array_equal(p.get_fc(A.FC_FIELD, ids), new[:, 3 * p.ncell + p.nface:])
+
+
 def test_box_io_order_oracle(A, S):
     """get_boxes returns box by box, variable by variable (af_write_tree's
     order); put_boxes writes the same layout back."""

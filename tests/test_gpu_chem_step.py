@@ -341,3 +341,43 @@ def test_autotune_with_air_network_leaves_every_species(A, S, CH, name):
         x, y, z = t.get_species(k), u.get_species(k), o.get_species(k)
         assert np.array_equal(x, y), (name, net.names[k])
         assert np.max(np.abs(x - z)) <= 1e-11 * max(np.max(np.abs(z)), 1e-300), (name, net.names[k])
+
+
+@pytest.mark.parametrize("name", ["cyl2d", "xyz3d_8"])
+def test_box_io_of_species_and_faces_equals_oracle(A, S, CH, name):
+    """Checkpoint box data (afgpu_get_boxes / afgpu_put_boxes,
+    af_write_tree's order) over the air network's species and both face
+    variables: on the device each block equals the variable's own transfer,
+    and device and oracle write and read the same data bit for bit."""
+    import os
+    from conftest import ROOT
+    case = cases(S)[name]
+    N = case.params.gas_number_density
+    E = S.analytic_transport()[0]
+    x_max = E[-1] * 1e21 / N
+    net = CH.read_network(os.path.join(ROOT, "tests", "golden", "air_chemistry_v1.txt"), {"N2": 0.8, "O2": 0.2}, N,
+                          0.0, x_max, 1000)
+    ns = len(net.species)
+    ne0 = np.asarray(case.ne0)
+    ivs = [A.VAR_SPECIES(k) for k in (0, 3, 6, 7)] + [A.CC_PHI, A.VAR_FACE(A.FC_FLUX), A.VAR_FACE(A.FC_FIELD)]
+    hs, keep = {}, []
+    for kind in ("gpu", "oracle"):
+        chem = CH.Chem(kind, ns, net.reactions, 0.0, x_max, net.fld_cols, None, 300.0)
+        keep.append(chem)
+        hs[kind], _ = run(A, S, case, kind, chem, steps=1, charges=net.charges, species={1: 0.8 * ne0, 2: 0.2 * ne0})
+    g, o = hs["gpu"], hs["oracle"]
+    n = g.n_boxes
+    rng = np.random.default_rng(5)
+    ids = np.arange(1, n + 1, dtype=np.int32)
+    rng.shuffle(ids)
+    ids = ids[: max(2, n // 2)]
+    got = g.get_boxes(ivs, ids)
+    blocks = [g.get_species(k, ids=ids) for k in (0, 3, 6, 7)] + [g.get_cc(A.CC_PHI, ids), g.get_fc(A.FC_FLUX, ids),
+                                                                 g.get_fc(A.FC_FIELD, ids)]
+    assert np.array_equal(got, np.concatenate(blocks, axis=1))
+    data = rng.normal(size=got.shape)
+    for p in (g, o):
+        p.put_boxes(ivs, data, ids)
+    assert np.array_equal(g.get_boxes(ivs, ids), o.get_boxes(ivs, ids))
+    assert np.array_equal(g.get_boxes(ivs, ids), data)
+    assert np.array_equal(g.get_fc(A.FC_FIELD, ids), o.get_fc(A.FC_FIELD, ids))
