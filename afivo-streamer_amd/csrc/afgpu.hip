@@ -189,6 +189,7 @@ struct afgpu_ctx {
   int *d_run_host = nullptr;       // runs the host has made active
   int *h_run_active = nullptr;     // pinned staging of the above
   int *d_run_maxlvl = nullptr;     // highest level of every run
+  int *d_run_bits = nullptr;       // per run: levels with leaves (bit l), then levels with refinement-boundary boxes
   bool uneven = false;             // runs of different highest levels
   bool masked = false;             // some list entries are negated
   std::vector<int> base_active;    // runs taking part in the current step (empty: all)
@@ -275,6 +276,7 @@ struct afgpu_ctx {
   // up once per device at afgpu_create
   double *scratch = nullptr;
   int photoi_cycles = 0;  // FMG cycles of the Helmholtz modes in the last photoionization solve
+  std::vector<int> run_pcyc;  // the same per run of a forest
   struct Direct {
     bool set = false;  // lc and the map below are current for the tree
     int lc = 1, N = 0;
@@ -939,6 +941,20 @@ struct Impl : ImplBase {
     return AFGPU_OK;
   }
 
+  // Device side of the masks of a mode's close in a forest (capturable):
+  // the runs made active by the host that would restrict level lvl (kind
+  // 0) or fill the ghosts of level lvl (kind 1) when solved alone.
+  int mask_close(int kind, int lvl, int lc) {
+    hipLaunchKernelGGL(k_runs_close, dim3((c.n_runs + 63) / 64), dim3(64), 0, c.cur, (const int *)c.d_run_host,
+                       (const int *)c.d_run_bits, c.d_run_active, kind, lvl, lc, c.n_runs);
+    HIPCHK(hipGetLastError());
+    const int n = (int)c.h_lists.size();
+    hipLaunchKernelGGL(k_mask_ids, dim3((n + 255) / 256), dim3(256), 0, c.cur, c.d_lists, n,
+                       (const int *)c.d_run_of, (const int *)c.d_run_active);
+    HIPCHK(hipGetLastError());
+    return AFGPU_OK;
+  }
+
   int fmg(int op, bool set_residual, bool have_guess) override {
     RCHK(ensure_factors(op));
     return run_graph(c, {G_FMG, op, (int)set_residual, (int)have_guess},
@@ -1414,14 +1430,32 @@ struct Impl : ImplBase {
     // leaf cells beside refined boxes and by the refinement-boundary ghosts
     // of finer boxes; below the lowest level where either occurs they are
     // not read before mp_finish (a uniformly refined run needs none).
-    RCHK(restrict_levels(iphi, c.highest_lvl, mp_restrict_lo(lc)));
     // The refinement-boundary ghosts of a level-l box (mg_sides_rb) read
     // the level l-1 neighbour including its stored ghost layer, so that
     // layer is refreshed on the coarser levels below such boxes.  The
     // finest leaves' own ghosts are formed inside the residual kernel.
     const VarBc m = var_methods(c, iphi);
-    for (int lvl = lc; lvl < c.highest_lvl; lvl++)
-      if (c.lvl_rb[lvl + 1]) RCHK(gc_var(lvl, c.cc[iphi], m, true));
+    if (c.n_runs > 1 && op != AFGPU_OP_POISSON) {
+      // A photoionization mode in a forest keeps no mp_finish, so its
+      // covered cells stay as its closes leave them: each run restricts
+      // and fills the levels it would alone (the forest's decision, taken
+      // over all runs, would touch cells a run alone leaves as they are).
+      const int top = std::numeric_limits<int>::max();
+      for (int lvl = c.highest_lvl; lvl >= mp_restrict_lo(lc); lvl--) {
+        RCHK(mask_close(0, lvl, lc));
+        RCHK(restrict_levels(iphi, lvl, lvl));
+      }
+      for (int lvl = lc; lvl < c.highest_lvl; lvl++)
+        if (c.lvl_rb[lvl + 1]) {
+          RCHK(mask_close(1, lvl, lc));
+          RCHK(gc_var(lvl, c.cc[iphi], m, true));
+        }
+      RCHK(mask_levels(0, top));
+    } else {
+      RCHK(restrict_levels(iphi, c.highest_lvl, mp_restrict_lo(lc)));
+      for (int lvl = lc; lvl < c.highest_lvl; lvl++)
+        if (c.lvl_rb[lvl + 1]) RCHK(gc_var(lvl, c.cc[iphi], m, true));
+    }
     const int slot = red_slot(op);
     RCHK(set_red(slot, 0ull));
     const List &l = c.l_all_leaves;
@@ -1856,23 +1890,35 @@ struct Impl : ImplBase {
     return rc;
   }
 
+  // photoi_set_src (src/m_photoi.f90:153, src/m_photoi_helmh.f90:162-191)
+  // for the runs of the base set.  In a forest each run stops every mode on
+  // its own test (its own max |rhs|): a run that has converged is masked
+  // out of the mode's further cycles, so every run does exactly the cycles
+  // it does alone.
   int photoi() override {
+    const int R = c.n_runs;
+    RCHK(set_active(nullptr));
     RCHK(with_bb(KC_UPDATE, [&](auto B) {
       constexpr int bb = decltype(B)::value;
       return items<bb>(KC_UPDATE, k_photoi_src<D, NC, BS, bb>, c.l_all_leaves.n, c.list(c.l_all_leaves),
                        (const double *)c.cc[AFGPU_CC_EFIELD], (const double *)c.cc[AFGPU_CC_ELECTRON],
                        c.cc[AFGPU_CC_RHS], c.td, c.p.gas_number_density, c.p.photoi_eta_quench);
     }));
-    double max_rhs;
-    RCHK(maxabs(AFGPU_CC_RHS, &max_rhs));
-    max_rhs = std::max(max_rhs, std::sqrt(std::numeric_limits<double>::epsilon()));
+    std::vector<double> max_rhs;
+    RCHK(maxabs_runs(AFGPU_CC_RHS, max_rhs));
+    for (double &m : max_rhs) m = std::max(m, std::sqrt(std::numeric_limits<double>::epsilon()));
     const int n_modes = c.p.photoi_n_modes;
+    const std::vector<int> base = base_set();
+    c.run_pcyc.assign(R, 0);
+    c.photoi_cycles = 0;
     if (c.precision == AFGPU_PREC_MIXED && n_modes > 0) {
       // Each mode: FP64 defect correction around single-precision FMG
       // cycles on its correction (from e = 0, rhs = the FP64 residual of the
       // previous solution of the mode), stopping on the reference's test
-      // (m_photoi_helmh.f90:184-189).  The modes are independent, so their
-      // cycles run side by side; each mode stops on its own.
+      // (m_photoi_helmh.f90:184-189).  The modes are independent: on a
+      // single run their cycles run side by side, each mode stopping on its
+      // own; a forest's run masks are shared by all modes, so there the
+      // modes take turns, each with its own masks.
       int lc;
       RCHK(direct_level(&lc));
       for (int n = 0; n < n_modes; n++) {
@@ -1880,43 +1926,73 @@ struct Impl : ImplBase {
         RCHK(ensure_direct_factors(AFGPU_OP_HELMH + n));
       }
       RCHK(ensure_mode_pools());
-      int active = (1 << n_modes) - 1;
-      c.photoi_cycles = 0;
-      RCHK(run_graph(c, {G_MP_RES_MODES, active, lc, 0},
-                     [&] { return fork_modes(active, [&](int op) { return mp_close(op, false, lc); }); }));
-      for (int i = 1; i <= c.p.photoi_max_fmg_cycles && active; i++) {
-        c.photoi_cycles += __builtin_popcount(active);
-        RCHK(run_graph(c, {G_MP_FMG_MODES, active, lc, 0}, [&] {
-          return fork_modes(active, [&](int op) {
-            if (c.photoi_inner_vcycle)
-              RCHK(vcycle_body(mg32(op), false, c.highest_lvl, lc));
-            else
-              RCHK(fmg_body(mg32(op), false, false, lc));
-            return mp_close(op, true, lc);
-          });
-        }));
-        RCHK(read_red(N_RED_SLOTS));
-        for (int n = 0; n < n_modes; n++)
-          if ((active >> n) & 1) {
-            const double residu = as_double(c.h_red[RED_MODE + n]);
-            if (residu / max_rhs < c.p.photoi_max_rel_residual) active &= ~(1 << n);
+      const int all = (1 << n_modes) - 1;
+      for (int group = 0; group < (R == 1 ? 1 : n_modes); group++) {
+        const int modes = (R == 1) ? all : (1 << group);
+        int active = modes;
+        std::vector<int> act = base;
+        RCHK(run_graph(c, {G_MP_RES_MODES, active, lc, 0},
+                       [&] { return fork_modes(active, [&](int op) { return mp_close(op, false, lc); }); }));
+        for (int i = 1; i <= c.p.photoi_max_fmg_cycles && active; i++) {
+          c.photoi_cycles += __builtin_popcount(active);
+          for (int r = 0; r < R; r++)
+            if (act[r]) c.run_pcyc[r] += __builtin_popcount(active);
+          RCHK(run_graph(c, {G_MP_FMG_MODES, active, lc, 0}, [&] {
+            return fork_modes(active, [&](int op) {
+              if (c.photoi_inner_vcycle)
+                RCHK(vcycle_body(mg32(op), false, c.highest_lvl, lc));
+              else
+                RCHK(fmg_body(mg32(op), false, false, lc));
+              return mp_close(op, true, lc);
+            });
+          }));
+          RCHK(read_red(N_RED_SLOTS));
+          if (R == 1) {
+            for (int n = 0; n < n_modes; n++)
+              if ((active >> n) & 1) {
+                const double residu = run_red(0, RED_MODE + n);
+                if (residu / max_rhs[0] < c.p.photoi_max_rel_residual) active &= ~(1 << n);
+              }
+          } else {
+            int left = 0;
+            for (int r = 0; r < R; r++) {
+              if (!act[r]) continue;
+              if (run_red(r, RED_MODE + group) / max_rhs[r] < c.p.photoi_max_rel_residual)
+                act[r] = 0;
+              else
+                left++;
+            }
+            if (left == 0) active = 0;
+            RCHK(set_active(&act));
           }
+        }
+        RCHK(set_active(nullptr));
       }
       // No final ghost refresh of the modes: the next solve forms its
       // residual with in-kernel ghosts, and the photoionization source is
       // read on interior cells only (src/m_fluid.f90:437-439).
       return photo_sum();
     }
-    c.photoi_cycles = 0;
     for (int n = 0; n < n_modes; n++) {
       const int op = AFGPU_OP_HELMH + n;
-      for (int i = 1; i <= c.p.photoi_max_fmg_cycles; i++) {
+      std::vector<int> act = base;
+      int left = 0;
+      for (int r = 0; r < R; r++) left += act[r] != 0;
+      for (int i = 1; i <= c.p.photoi_max_fmg_cycles && left > 0; i++) {
         c.photoi_cycles++;
         RCHK(fmg(op, true, true));
-        double residu;
-        RCHK(leaf_residual_max(&residu));
-        if (residu / max_rhs < c.p.photoi_max_rel_residual) break;
+        RCHK(read_red(1));
+        for (int r = 0; r < R; r++) {
+          if (!act[r]) continue;
+          c.run_pcyc[r]++;
+          if (run_red(r, 0) / max_rhs[r] < c.p.photoi_max_rel_residual) {
+            act[r] = 0;
+            left--;
+          }
+        }
+        RCHK(set_active(&act));
       }
+      RCHK(set_active(nullptr));
     }
     return photo_sum();
   }
@@ -2154,6 +2230,23 @@ struct Impl : ImplBase {
       }
       t.it = t.it + 1;
     }
+    // photoionization of the runs whose step count is due
+    // (src/streamer.f90:236-239, as afgpu_step for one run)
+    std::vector<int> photo(R, 0);
+    if (c.p.photoi_enabled && c.p.photoi_per_steps > 0) {
+      int n_photo = 0;
+      for (int r = 0; r < R; r++) {
+        photo[r] = c.rt[r].it % c.p.photoi_per_steps == 0;
+        n_photo += photo[r];
+      }
+      if (n_photo > 0) {
+        if (n_photo < R) c.base_active = photo;
+        const int rc = photoi();
+        c.base_active.clear();
+        RCHK(rc);
+        RCHK(set_active(nullptr));
+      }
+    }
     std::vector<double> dt(R), dt_lim(R, std::numeric_limits<double>::max());
     std::vector<int> todo(R, 1), n_att(R, 0), n_rej(R, 0);
     for (int r = 0; r < R; r++) dt[r] = c.rt[r].dt;
@@ -2349,8 +2442,8 @@ struct Impl : ImplBase {
         f.n_vcycles = c.run_ncyc[r];
         f.residual = c.run_res[r];
         f.rel_residual = max_rhs[r] > 0 ? c.run_res[r] / max_rhs[r] : 0;
-        f.photoi_done = 0;
-        f.photoi_cycles = 0;
+        f.photoi_done = photo[r];
+        f.photoi_cycles = photo[r] ? c.run_pcyc[r] : 0;
         f.leaf_cells = c.run_leaf_cells[r];
       }
     }
@@ -2540,6 +2633,7 @@ int afgpu_destroy(afgpu_t *h) {
   (void)hipFree(h->d_run_active);
   (void)hipFree(h->d_run_host);
   (void)hipFree(h->d_run_maxlvl);
+  (void)hipFree(h->d_run_bits);
   (void)hipFree(h->d_dt_run);
   (void)hipFree(h->d_step);
   (void)hipFree(h->d_run_ones);
@@ -2789,7 +2883,7 @@ int afgpu_sync_tree(afgpu_t *h, int n_boxes, const int *lvl, const int *ix, cons
     if (h->h_dt_run) (void)hipHostFree(h->h_dt_run);
     h->d_red = h->d_red_out = nullptr;
     h->d_run_volt = h->d_dt_run = nullptr;
-    h->d_run_active = h->d_run_host = h->d_run_maxlvl = nullptr;
+    h->d_run_active = h->d_run_host = h->d_run_maxlvl = h->d_run_bits = nullptr;
     h->h_red = nullptr;
     h->h_run_active = nullptr;
     h->h_dt_run = nullptr;
@@ -2802,6 +2896,7 @@ int afgpu_sync_tree(afgpu_t *h, int n_boxes, const int *lvl, const int *ix, cons
     HIPCHK(hipMalloc(&h->d_run_active, nr * sizeof(int)));
     HIPCHK(hipMalloc(&h->d_run_host, nr * sizeof(int)));
     HIPCHK(hipMalloc(&h->d_run_maxlvl, nr * sizeof(int)));
+    HIPCHK(hipMalloc(&h->d_run_bits, 2 * nr * sizeof(int)));
     HIPCHK(hipHostMalloc(&h->h_run_active, nr * sizeof(int), hipHostMallocDefault));
     HIPCHK(hipMalloc(&h->d_dt_run, nr * sizeof(double)));
     HIPCHK(hipHostMalloc(&h->h_dt_run, nr * sizeof(double), hipHostMallocDefault));
@@ -2815,6 +2910,20 @@ int afgpu_sync_tree(afgpu_t *h, int n_boxes, const int *lvl, const int *ix, cons
   HIPCHK(hipMemcpy(h->d_run_volt, h->run_volt.data(), (size_t)n_runs * sizeof(double), hipMemcpyHostToDevice));
   if ((int)h->rt.size() != n_runs) h->rt.assign(n_runs, afgpu_ctx::RunTime());
   HIPCHK(hipMemcpy(h->d_run_maxlvl, maxlvl.data(), (size_t)n_runs * sizeof(int), hipMemcpyHostToDevice));
+  {
+    // per run, the levels with leaves and the levels with refinement-
+    // boundary boxes (the per-run form of the level lists and lvl_rb)
+    std::vector<int> bits(2 * (size_t)n_runs, 0);
+    for (int l = 1; l <= hl; l++) {
+      for (int id : leaves[l]) bits[hb[id - 1].run] |= 1 << l;
+      for (int id : ids[l]) {
+        const DBox &x = hb[id - 1];
+        for (int q = 0; q < nnb; q++)
+          if (x.nb[q] == 0) bits[n_runs + x.run] |= 1 << l;
+      }
+    }
+    HIPCHK(hipMemcpy(h->d_run_bits, bits.data(), bits.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
   {
     std::vector<int> ones(n_runs, 1);
     HIPCHK(hipMemcpy(h->d_run_host, ones.data(), (size_t)n_runs * sizeof(int), hipMemcpyHostToDevice));
@@ -2980,16 +3089,15 @@ int afgpu_set_run_time_state(afgpu_t *h, int run, int it, double time, double gl
   return AFGPU_OK;
 }
 
+static int check_photoi_species(const afgpu_t *h);
+
 int afgpu_step_ensemble(afgpu_t *h, int n_runs, afgpu_step_info_t *info) {
   RCHK(ready(h, true));
   if (n_runs != h->n_runs) {
     set_err("afgpu_step_ensemble: n_runs differs from the forest's run count");
     return AFGPU_ERR_ARG;
   }
-  if (h->p.photoi_enabled) {
-    set_err("afgpu_step_ensemble: photoionization is supported for single runs only");
-    return AFGPU_ERR_UNSUPPORTED;
-  }
+  RCHK(check_photoi_species(h));
   RCHK(h->impl->step_runs(info));
   return sync_stream(h);
 }
@@ -3109,10 +3217,7 @@ int afgpu_advance(afgpu_t *h, double dt, double *dt_lim) {
 
 int afgpu_photoi_set_src(afgpu_t *h) {
   RCHK(ready(h, true));
-  if (h->n_runs > 1) {
-    set_err("afgpu_photoi_set_src: photoionization is supported for single runs only");
-    return AFGPU_ERR_UNSUPPORTED;
-  }
+  RCHK(check_photoi_species(h));
   RCHK(h->impl->photoi());
   return sync_stream(h);
 }

@@ -328,6 +328,25 @@ __global__ void k_runs_in_levels(const int *__restrict__ host_active, const int 
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r < n_runs) active[r] = host_active[r] && maxlvl[r] >= lo && maxlvl[r] <= hi;
 }
+// The runs that, solved alone, would restrict the children on level lvl
+// into their parents in a mixed-precision close (kind 0: the run has
+// leaves below some level l in lc + 1 .. lvl, or refinement-boundary boxes
+// on it), or fill the ghosts of level lvl (kind 1: refinement-boundary
+// boxes on lvl + 1); bits holds per run the levels with leaves (n_runs
+// words) and then the levels with refinement-boundary boxes.
+__global__ void k_runs_close(const int *__restrict__ host_active, const int *__restrict__ bits, int *active, int kind,
+                             int lvl, int lc, int n_runs) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_runs) return;
+  const int leaf = bits[r], rb = bits[n_runs + r];
+  bool on = false;
+  if (kind == 0) {
+    for (int l = lc + 1; l <= lvl; l++) on = on || ((leaf >> (l - 1)) & 1) || ((rb >> l) & 1);
+  } else {
+    on = (rb >> (lvl + 1)) & 1;
+  }
+  active[r] = host_active[r] && on;
+}
 // Box lists of a forest: the id of a box whose run is not active is stored
 // negated, and every list kernel skips it; active runs get their ids back.
 __global__ void k_mask_ids(int *ids, int n, const int *__restrict__ run_of, const int *__restrict__ active) {

@@ -440,7 +440,10 @@ int afgpu_set_option(afgpu_t *h, int option, int value);
  * boxes must share their geometry, their faces all physical boundaries).
  * Every run keeps the reference's semantics: its own applied voltage,
  * time step, rejections and stopping tests, so each run computes what it
- * would compute alone.  Photoionization is supported for single runs only.
+ * would compute alone.  With photoionization each run solves the modes to
+ * its own stopping test (photoi_max_rel_residual of its own max |rhs|) at
+ * its own step count (photoi_per_steps); afgpu_photoi_set_src covers every
+ * run.
  */
 int afgpu_num_runs(afgpu_t *h);
 /* Applied voltage of every run (afgpu_set_voltage sets all runs). */

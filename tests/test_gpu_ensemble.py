@@ -141,6 +141,52 @@ def test_forest_step_graph_equals_single_runs(A, S, prec):
     assert rejected > 0
 
 
+@pytest.mark.parametrize("prec", ["PREC_FP64", "PREC_MIXED"])
+def test_forest_photoionization_equals_single_runs(A, S, prec):
+    """Photoionization in a forest (src/m_photoi_helmh.f90:162-191): each
+    run solves the three Helmholtz modes to its own stopping test (its own
+    max |rhs|) and is masked out of a mode's cycles once it has converged,
+    so every run equals its single-run handle bit for bit: the photo
+    source, the modes, the state and the step records, photoionization
+    cycles included.  Run 2 restarts at another step count, so with
+    photoionization every second step the runs are due at different
+    steps."""
+    ens = S.ensemble(level_list=[5, 6, 5, 5], num_vcycles=20, rel_residual=1e-8, photoi=True)
+    p = params_for(A, ens.cases[0].params)
+    p.photoi_per_steps = 2
+    g, singles = forest_and_singles(A, S, ens, getattr(A, prec), params=p)
+    n_g, r_g = g.field_compute_runs(0, have_guess=False)
+    for r, s in enumerate(singles):
+        assert (int(n_g[r]), float(r_g[r])) == s.field_compute(0, have_guess=False)
+    g.set_run_time_state(2, 1, 0.0, 1e-13)
+    singles[2].set_time_state(1, 0.0, 1e-13)
+    g.photoi_set_src()
+    for s in singles:
+        s.photoi_set_src()
+    photo = (A.CC_PHOTO, A.CC_HELMH, A.CC_HELMH + 1, A.CC_HELMH + 2)
+    for r, s in enumerate(singles):
+        for iv in photo:
+            assert np.array_equal(g.get_cc(iv, ens.run_ids(r)), s.get_cc(iv)), ("set_src", r, iv)
+    keys = STEP_KEYS + ("photoi_done", "photoi_cycles")
+    due = set()
+    for it in range(4):
+        infos = g.step_ensemble()
+        for r, s in enumerate(singles):
+            ref = s.step()
+            got = {k: infos[r][k] for k in keys}
+            exp = {k: ref[k] for k in keys}
+            assert got == exp, (prec, it, r, got, exp)
+            if ref["photoi_done"]:
+                due.add((it, r))
+                assert ref["photoi_cycles"] > 0
+        compare_runs(A, ens, g, singles, "%s photoi step %d" % (prec, it))
+        for r, s in enumerate(singles):
+            for iv in photo:
+                assert np.array_equal(g.get_cc(iv, ens.run_ids(r)), s.get_cc(iv)), (it, r, iv)
+    # both step parities had runs due (run 2 out of phase with the others)
+    assert {it % 2 for it, r in due if r == 2} != {it % 2 for it, r in due if r == 0}
+
+
 def test_forest_mixed_vs_fp64_per_run_error(A, S):
     """Per-run error of the mixed form against the FP64 form (the oracle's
     arithmetic) after a few steps of an uneven ensemble."""
