@@ -104,6 +104,28 @@ def test_transport_tables_equal_reference(A):
     assert len(d["td.x"]) == n
 
 
+def test_chemistry_field_table_equals_reference(CH):
+    """chemistry.py's own table construction (the path of reaction-list
+    files: lt_grid as LT_create's get_x, spaced_data as LT_lin_interp_list
+    in LT_set_col) rebuilds the standard chemistry's field table that the
+    reference's table_set_column made from the same rates on the same
+    range (src/m_table_data.f90:82-118, m_lookup_table.f90:161-181, 240-255)
+    bit for bit."""
+    d = refvec.load("pointwise")
+    x = d["td.x"]
+    n_td = len(x)
+    td = d["td.cols"].reshape(-1, n_td)  # rows_cols (n_points, n_cols), column by column
+    n_chem = d["chem.cols"].size // 2
+    chem = d["chem.cols"].reshape(2, n_chem)
+    townsend_to_si, gas_n = 1.0e-21, d["in.N"][0]
+    grid = CH.lt_grid(x[0], x[-1], n_chem)
+    assert same(np.array([grid[0]]), d["chem.x_min"])
+    for col, k in ((0, 2), (1, 3)):  # ionisation (alpha), attachment (eta)
+        rate = td[k] * td[0] * x * townsend_to_si * gas_n
+        got = CH.spaced_data(x, rate, grid)
+        assert same(got, chem[col]), (col, where_differs(got, chem[col]))
+
+
 def test_table_lookups_equal_reference(lib):
     d = refvec.load("pointwise")
     x = d["look.x"]
