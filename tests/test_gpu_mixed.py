@@ -73,7 +73,8 @@ def test_mixed_poisson_fixed_point(A, S, name):
         m.field_compute(0, True)  # at least one more mixed cycle each call
     e = rel_max(leaf_interior(case, m, A.CC_PHI), leaf_interior(case, f, A.CC_PHI))
     print(name, "fixed point phi %.2e" % e, flush=True)
-    assert e < 1e-11
+    # (achieved on MI355X: at most 1.5e-14)
+    assert e < 2e-13
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
@@ -94,10 +95,14 @@ def test_mixed_steps_match_fp64(A, S, name):
                     ("pos_ion", A.CC_POS_ION)):
         errs[key] = rel_max(leaf_interior(case, m, iv), leaf_interior(case, f, iv))
     print(name, "errors", {k: "%.2e" % v for k, v in errs.items()}, flush=True)
-    assert errs["phi"] < 1e-6
-    assert errs["field"] < 1e-5
-    assert errs["electron"] < 1e-6
-    assert errs["pos_ion"] < 1e-6
+    # about ten times the largest achieved on MI355X over the three cases
+    # (phi 1.6e-11, field 4.5e-10, electrons 5.8e-11, ions 5.0e-15): each
+    # form stops on the same absolute threshold, so the difference is the
+    # stopping tolerance's, not a precision floor
+    assert errs["phi"] < 2e-10
+    assert errs["field"] < 5e-9
+    assert errs["electron"] < 6e-10
+    assert errs["pos_ion"] < 1e-13
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
@@ -112,7 +117,8 @@ def test_mixed_photoi_matches_fp64(A, S, name):
             p.photoi_set_src()
         e = rel_max(leaf_interior(case, m, A.CC_PHOTO), leaf_interior(case, f, A.CC_PHOTO))
         print(name, rep, "photo error %.2e" % e, flush=True)
-        assert e < 1e-7
+        # (achieved on MI355X: at most 8.3e-11)
+        assert e < 1e-9
 
 
 @pytest.mark.parametrize("direct_n", [0, 16, 32, 64])
