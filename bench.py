@@ -115,22 +115,47 @@ def run_gpu(case, precision, warmup, steps, timing_steps, direct_n=None, smooth=
     # inside graphs; the kernels and their inputs are the same)
     stats = {}
     if timing_steps > 0:
-        import ctypes as C
         # (modes in turn, so that no two timed kernels share the GPU)
         p.set_option(A.OPT_BATCH_MODES, 0)
-        p.lib.afgpu_kernel_timing(p.h, 1)
-        for k in range(3):
-            p.lib.afgpu_kernel_stats(p.h, k, 1, None, None, None)
-        for i in range(timing_steps):
-            p.step()
-        for k, name in enumerate(("gsrb", "flux", "update")):
-            ms, n, b = C.c_double(), C.c_longlong(), C.c_double()
-            p.lib.afgpu_kernel_stats(p.h, k, 1, C.byref(ms), C.byref(n), C.byref(b))
-            stats[name] = {"total_ms": ms.value, "launches": n.value, "algo_bytes": b.value}
-        p.lib.afgpu_kernel_timing(p.h, 0)
+        stats = kernel_stats(p, timing_steps, p.step)
         if batch is not None:
             p.set_option(A.OPT_BATCH_MODES, batch)
     return p, infos, t1 - t0, stats
+
+
+def kernel_stats(p, n_steps, step):
+    """Durations of the GSRB / flux / update launches over n_steps further
+    calls of step(), executed eagerly with kernel timing on: {class:
+    {total_ms, launches, algo_bytes}}."""
+    import ctypes as C
+    p.lib.afgpu_kernel_timing(p.h, 1)
+    for k in range(3):
+        p.lib.afgpu_kernel_stats(p.h, k, 1, None, None, None)
+    for i in range(n_steps):
+        step()
+    stats = {}
+    for k, name in enumerate(("gsrb", "flux", "update")):
+        ms, n, b = C.c_double(), C.c_longlong(), C.c_double()
+        p.lib.afgpu_kernel_stats(p.h, k, 1, C.byref(ms), C.byref(n), C.byref(b))
+        stats[name] = {"total_ms": ms.value, "launches": n.value, "algo_bytes": b.value}
+    p.lib.afgpu_kernel_timing(p.h, 0)
+    return stats
+
+
+def roofline_of(stats, timing_steps, traffic=None):
+    """The roofline object of the dominant kernel class in stats."""
+    dom = max(stats, key=lambda k: stats[k]["total_ms"]) if stats else None
+    if not dom:
+        return None
+    st = stats[dom]
+    avg_ms = st["total_ms"] / max(st["launches"], 1)
+    achieved = st["algo_bytes"] / (st["total_ms"] * 1e-3) / 1e9 if st["total_ms"] > 0 else 0.0
+    return {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "avg_launch_ms": round(avg_ms, 5), "launches": st["launches"],
+            "algo_bytes_per_launch": st["algo_bytes"] / max(st["launches"], 1),
+            "timing": "start/stop events recorded by each launch's dispatch (hipExtLaunchKernel) on the "
+                      "library's stream, %d eager steps after the timed region" % timing_steps}
 
 
 def measure_refine(case, p, c=None, reps=10):
@@ -410,7 +435,7 @@ def ensemble_leaf_interior(ens, r, p, iv):
     return a[:, 1:c.nc + 1, 1:c.nc + 1]
 
 
-def run_ensemble(ens, precision, warmup, steps, autotune=True, step_graph=1):
+def run_ensemble(ens, precision, warmup, steps, autotune=True, step_graph=1, timing_steps=0):
     p = S.apply_ensemble(A.Path("gpu", 2, 8, A.CYL, precision), ens)
     p.set_option(A.OPT_STEP_GRAPH, step_graph)
     t0 = time.time()
@@ -425,6 +450,7 @@ def run_ensemble(ens, precision, warmup, steps, autotune=True, step_graph=1):
     for i in range(steps):
         all_infos.append(p.step_ensemble())
     t1 = time.perf_counter()
+    p.kstats = kernel_stats(p, timing_steps, p.step_ensemble) if timing_steps > 0 else {}
     return p, all_infos, t1 - t0
 
 
@@ -438,7 +464,10 @@ def bench_ensemble(args):
     log("workload", ens.name, "built in %.1f s; sizes" % (time.time() - t0),
         {int(np.sqrt(k)): sizes.count(k) for k in sorted(set(sizes))})
     prec = A.PREC_MIXED if args.precision == "mixed" else A.PREC_FP64
-    p, infos, secs = run_ensemble(ens, prec, args.warmup, args.steps, bool(args.autotune), args.step_graph)
+    p, infos, secs = run_ensemble(ens, prec, args.warmup, args.steps, bool(args.autotune), args.step_graph,
+                                  args.timing_steps)
+    if p.kstats:
+        log("  kernel stats", json.dumps(p.kstats))
     cell_steps = sum(f["leaf_cells"] for step in infos for f in step)
     value = cell_steps / secs
     nv = np.array([[f["n_vcycles"] for f in step] for step in infos])
@@ -447,8 +476,10 @@ def bench_ensemble(args):
     extra = {}
     if prec == A.PREC_MIXED and not args.no_fp64_ref:
         log("GPU fp64 ensemble")
+        # (the same eager timing steps, so that both forms are compared at the
+        # same step)
         ref, rinfos, rsecs = run_ensemble(ens, A.PREC_FP64, args.warmup, args.steps, bool(args.autotune),
-                                          args.step_graph)
+                                          args.step_graph, args.timing_steps)
         fp64_value = sum(f["leaf_cells"] for step in rinfos for f in step) / rsecs
         extra["fp64_gpu_value"] = fp64_value
         extra["mixed_over_fp64"] = value / fp64_value
@@ -464,7 +495,8 @@ def bench_ensemble(args):
         extra["error_vs_fp64"] = {"max_rel_over_runs": worst,
                                   "per_run_phi": [round(e["phi"], 14) for e in per_run],
                                   "cells": "leaf interiors of every run after %d steps, max relative "
-                                           "to max |FP64| of that run" % (args.warmup + args.steps)}
+                                           "to max |FP64| of that run"
+                                           % (args.warmup + args.steps + args.timing_steps)}
         log("  fp64 %.4e cell-steps/s; mixed/fp64 %.3f; worst errors %s" % (fp64_value, value / fp64_value, worst))
     cpu = None
     if not args.no_cpu:
@@ -508,7 +540,7 @@ def bench_ensemble(args):
         "config": {"workload": ens.name, "runs": len(ens.cases), "cells": int(sum(sizes)),
                    "box_cells": "8^2", "precision": args.precision, "rel_residual": args.rel_residual,
                    "mean_vcycles": float(nv.mean()), "step_graph": args.step_graph, "shapes": p.shapes},
-        "roofline": None, "cpu_baseline": cpu,
+        "roofline": roofline_of(p.kstats, args.timing_steps), "cpu_baseline": cpu,
     }
     out.update(extra)
     print(json.dumps(out), flush=True)
@@ -611,19 +643,10 @@ def main():
     dom = max(stats, key=lambda k: stats[k]["total_ms"]) if stats else None
     roof = None
     if dom:
-        st = stats[dom]
-        avg_ms = st["total_ms"] / max(st["launches"], 1)
-        achieved = st["algo_bytes"] / (st["total_ms"] * 1e-3) / 1e9 if st["total_ms"] > 0 else 0.0
         traffic = None
         if not args.block_smooth and args.direct_n == 128 and args.mixed_smooth == 0:
             traffic = MEASURED_TRAFFIC.get((case.name, args.precision, dom))
-        roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "avg_launch_ms": round(avg_ms, 5), "launches": st["launches"],
-                "algo_bytes_per_launch": st["algo_bytes"] / max(st["launches"], 1),
-                "timing": "start/stop events recorded by each launch's dispatch (hipExtLaunchKernel) on the "
-                          "library's stream, %d eager steps after the timed region"
-                          % args.timing_steps}
+        roof = roofline_of(stats, args.timing_steps, traffic)
         log("  kernel stats", json.dumps(stats))
 
     photoi = bool(case.params.photoi_enabled)
