@@ -218,7 +218,10 @@ struct afgpu_ctx {
   int lists_bb = 1;  // items per workgroup slot the list order is made for
   std::vector<std::vector<int>> src_ids, src_leaves, src_parents;  // per level, id order
   std::vector<int> src_refb;
-  std::vector<List> l_ids, l_leaves, l_parents;
+  // per level l: the boxes across a refinement-boundary face of a level-(l+1)
+  // box (the coarse neighbours whose stored ghost layer mg_sides_rb reads)
+  std::vector<std::vector<int>> src_rbnb;
+  std::vector<List> l_ids, l_leaves, l_parents, l_rbnb;
   std::vector<char> lvl_rb;  // level has boxes with a refinement-boundary face
   List l_all_leaves, l_all_ids, l_refb;
   // pools
@@ -392,11 +395,13 @@ void build_lists(afgpu_ctx &h, int B) {
   h.l_ids.assign(hl + 1, List());
   h.l_leaves.assign(hl + 1, List());
   h.l_parents.assign(hl + 1, List());
+  h.l_rbnb.assign(hl + 1, List());
   std::vector<int> all_leaves, all_ids;
   for (int l = 1; l <= hl; l++) {
     h.l_ids[l] = add(h.src_ids[l]);
     h.l_leaves[l] = add(h.src_leaves[l]);
     h.l_parents[l] = add(h.src_parents[l]);
+    h.l_rbnb[l] = add(h.src_rbnb[l]);
     all_leaves.insert(all_leaves.end(), h.src_leaves[l].begin(), h.src_leaves[l].end());
     all_ids.insert(all_ids.end(), h.src_ids[l].begin(), h.src_ids[l].end());
   }
@@ -631,7 +636,11 @@ struct Impl : ImplBase {
 
   template <typename T>
   int gc_var(int lvl, T *var, const VarBc &m, bool corners) {
-    const List &l = c.l_ids[lvl];
+    return gc_list(c.l_ids[lvl], var, m, corners);
+  }
+  // the ghost layer of the boxes of list l (all of one level)
+  template <typename T>
+  int gc_list(const List &l, T *var, const VarBc &m, bool corners) {
     const int k = corners ? 1 : bpb(l.n, 2 * D * G::NTAN);
     if (k == 1)
       return with_bb(KC_GC, [&](auto B) {
@@ -1432,8 +1441,9 @@ struct Impl : ImplBase {
     // not read before mp_finish (a uniformly refined run needs none).
     // The refinement-boundary ghosts of a level-l box (mg_sides_rb) read
     // the level l-1 neighbour including its stored ghost layer, so that
-    // layer is refreshed on the coarser levels below such boxes.  The
-    // finest leaves' own ghosts are formed inside the residual kernel.
+    // layer is refreshed on those neighbours (l_rbnb; the other boxes'
+    // stored layers are not read before mp_finish refreshes every level).
+    // The finest leaves' own ghosts are formed inside the residual kernel.
     const VarBc m = var_methods(c, iphi);
     if (c.n_runs > 1 && op != AFGPU_OP_POISSON) {
       // A photoionization mode in a forest keeps no mp_finish, so its
@@ -1448,13 +1458,13 @@ struct Impl : ImplBase {
       for (int lvl = lc; lvl < c.highest_lvl; lvl++)
         if (c.lvl_rb[lvl + 1]) {
           RCHK(mask_close(1, lvl, lc));
-          RCHK(gc_var(lvl, c.cc[iphi], m, true));
+          RCHK(gc_list(c.l_rbnb[lvl], c.cc[iphi], m, true));
         }
       RCHK(mask_levels(0, top));
     } else {
       RCHK(restrict_levels(iphi, c.highest_lvl, mp_restrict_lo(lc)));
       for (int lvl = lc; lvl < c.highest_lvl; lvl++)
-        if (c.lvl_rb[lvl + 1]) RCHK(gc_var(lvl, c.cc[iphi], m, true));
+        if (c.lvl_rb[lvl + 1]) RCHK(gc_list(c.l_rbnb[lvl], c.cc[iphi], m, true));
     }
     const int slot = red_slot(op);
     RCHK(set_red(slot, 0ull));
@@ -2809,6 +2819,20 @@ int afgpu_sync_tree(afgpu_t *h, int n_boxes, const int *lvl, const int *ix, cons
   h->src_leaves = leaves;
   h->src_parents = parents;
   h->src_refb = refb;
+  {
+    std::vector<std::vector<int>> rbnb(hl + 1);
+    for (int l = 2; l <= hl; l++)
+      for (int id : ids[l]) {
+        const DBox &x = hb[id - 1];
+        for (int q = 0; q < nnb; q++)
+          if (x.nb[q] == 0 && x.pnb[q] > 0) rbnb[l - 1].push_back(x.pnb[q]);
+      }
+    for (auto &v : rbnb) {
+      std::sort(v.begin(), v.end());
+      v.erase(std::unique(v.begin(), v.end()), v.end());
+    }
+    h->src_rbnb = rbnb;
+  }
   build_lists(*h, h->shape[KC_GSRB].bb);
   const std::vector<int> &all = h->h_lists;
   HIPCHK(hipStreamSynchronize(h->stream));
