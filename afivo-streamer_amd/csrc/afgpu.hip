@@ -720,6 +720,13 @@ struct Impl : ImplBase {
     }
     for (int n = 1; n <= 2 * n_cycle; n++) {
       const int stored = (stored_first && n == 1 && !Lay<D, NC, T>::SPLIT) ? 1 : 0;
+      if constexpr (D == 2 && Lay<D, NC, T>::SPLIT && NC == 8) {
+        if (k == 1 && !c.box_lds) {  // (a wavefront per eight boxes, four wavefronts per workgroup)
+          RCHK((items_nt<256, 1>(0, bytes, KC_GSRB, k_gsrb_split2d<NC>, (l.n + 31) / 32, c.list(l),
+                                 (const DBox *)c.d_boxes, v.phi, (const T *)v.rhs, n, v.lambda2, cyl, v.m, l.n)));
+          continue;
+        }
+      }
       if constexpr (D == 3 && Lay<D, NC, T>::SPLIT && BS == NC * NC) {
         if (k == 1 && !c.box_lds) {
           RCHK((items_nt<GSRB_SPLIT_NT, 1>(0, bytes, KC_GSRB, k_gsrb_split<D, NC, GSRB_SPLIT_NT>, l.n, c.list(l),

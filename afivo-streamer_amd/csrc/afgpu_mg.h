@@ -1414,6 +1414,171 @@ __global__ void __launch_bounds__(NT) k_gsrb_split(const int *__restrict__ ids, 
   }
 }
 
+// k_gsrb_lds for 2D boxes of 8^2 cells in the colour-split single-precision
+// layout (the ensembles' half-sweep).  A wavefront takes eight boxes, a lane
+// one row of the box's updated colour (four cells, one 16-byte word).  All
+// that the eight boxes read goes straight from memory into the wavefront's
+// part of LDS (global_load_lds, one round, no registers held): each row's
+// other-colour word and rhs word, the row's x ghost (the neighbour's cell
+// across the face), the y ghosts (the neighbour's facing row of the other
+// colour, one word per face), and for a face without a same-level neighbour
+// the box's own cells next to it and the coarse values its ghosts need.
+// Those ghosts are then formed in LDS, and each lane updates its row.  The
+// ghost and stencil expressions and their operands are those of
+// k_gsrb_lds, so the results are identical.  Wavefronts never wait for
+// each other.
+#define LDS_PTR(p) ((__attribute__((address_space(3))) void *)(p))
+template <int NC>
+__global__ void __launch_bounds__(256) k_gsrb_split2d(const int *__restrict__ ids, const DBox *__restrict__ boxes,
+                                                      float *phi, const float *__restrict__ rhs, int redblack,
+                                                      float lambda2, int cyl, VarBc m, int n_boxes, int nw) {
+  constexpr int D = 2;
+  using L = Lay<D, NC, float>;
+  constexpr int HALF = L::HALF;
+  static_assert(NC == 8 && HALF == 32 && (L::STRIDE * sizeof(float)) % 16 == 0, "8^2 boxes: a row is one word");
+  constexpr int NBW = 8;                       // boxes per wavefront
+  __shared__ float4 s_o[4][64];                // other colour, row r of box bl at lane 8 bl + r
+  __shared__ float4 s_r[4][64];                // rhs of the updated colour, the same
+  __shared__ float4 s_y[4][64];                // y ghosts of box bl: face 2 at 8 bl, face 3 at 8 bl + 4
+  __shared__ float s_gx[4][64];                // x ghost of row r
+  __shared__ float s_x1[2][4][64];             // the box's cell next to a face: x item, y item
+  __shared__ float s_cv[2][3][4][64];          // coarse values of an item at a refinement boundary
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x & 63;
+  const int bl = lane >> 3, r = lane & 7;
+  const int uc = redblack & 1, oc = 1 - uc;    // updated cells: (i + j + redblack) even
+  for (int w = blockIdx.x; w < nw; w += gridDim.x) {
+    const int b = (4 * w + wv) * NBW + bl;
+    const int id = b < n_boxes ? ids[b] : 0;
+    const bool on = id > 0;  // (past the end, or a run masked out)
+    const DBox &bx = boxes[on ? id - 1 : 0];
+    float *u = phi + (size_t)(on ? id - 1 : 0) * L::STRIDE;
+    const int j = r + 1, i0 = 1 + ((1 + j + uc) & 1);
+    // x item: the ghost of row j on the side of its first or last cell
+    const int nbx = (i0 == 1) ? 0 : 1;
+    // y item: face 2 (r < 4) or 3, the cell i = i0y + 2 mm of the face row
+    const int fy = r >> 2, mm = r & 3, nby = 2 + fy, jy = fy ? NC : 1;
+    const int iy = 1 + ((1 + jy + uc) & 1) + 2 * mm;
+    int nbvx = 0, nbvy = 0;
+    if (on) {
+      nbvx = bx.nb[nbx];
+      nbvy = bx.nb[nby];
+      __builtin_amdgcn_global_load_lds((const void *)(u + oc * HALF + 4 * r), LDS_PTR(&s_o[wv][0]), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void *)(rhs + (size_t)(id - 1) * L::STRIDE + uc * HALF + 4 * r),
+                                       LDS_PTR(&s_r[wv][0]), 16, 0, 0);
+    }
+    // (a DMA's LDS address is its wavefront's base plus the lane's slot:
+    // every lane's destination below is its own slot of a 64-entry array)
+    if (on && nbvx > 0) {  // the neighbour's cell across the x face
+      const int p = (nbx == 0) ? NC : 1;
+      __builtin_amdgcn_global_load_lds((const void *)(phi + (size_t)(nbvx - 1) * L::STRIDE + L::in_at(p, j, 0)),
+                                       LDS_PTR(&s_gx[wv][0]), 4, 0, 0);
+    }
+    if (on && nbvx <= 0)
+      __builtin_amdgcn_global_load_lds((const void *)(u + L::in_at(nbx == 0 ? 1 : NC, j, 0)), LDS_PTR(&s_x1[0][wv][0]), 4,
+                                       0, 0);
+    if (on && nbvy > 0 && mm == 0) {  // the neighbour's facing row of the other colour
+      const int row = fy ? 1 : NC;
+      __builtin_amdgcn_global_load_lds(
+          (const void *)(phi + (size_t)(nbvy - 1) * L::STRIDE + oc * HALF + 4 * (row - 1)), LDS_PTR(&s_y[wv][0]), 16, 0,
+          0);
+    }
+    if (on && nbvy <= 0)
+      __builtin_amdgcn_global_load_lds((const void *)(u + L::in_at(iy, jy, 0)), LDS_PTR(&s_x1[1][wv][0]), 4, 0, 0);
+    // coarse values (ghost_coarse_load's, in its order) of the items at a
+    // refinement boundary
+    for (int it = 0; it < 2; it++) {
+      const int nb = it ? nby : nbx, nbv = it ? nbvy : nbvx, a = it ? iy : j;
+      if (!(on && nbv == 0)) continue;
+      const int d = nb / 2, t0 = 1 - d;
+      const bool low = (nb % 2) == 0;
+      const float *P = phi + (size_t)(bx.pnb[nb] - 1) * L::STRIDE;
+      const int co = ((bx.ix[t0] - 1) & 1) * (NC / 2), cn = low ? NC : 1;
+      auto at = [&](int qa) {
+        int q[3] = {0, 0, 0};
+        q[d] = cn;
+        q[t0] = qa;
+        return P + L::at(q[0], q[1], q[2]);
+      };
+      int pos[3];
+      int n = 3;
+      if (m.rb == RB_MG) {
+        const int i = (a + 1) >> 1;
+        pos[0] = co + i + 1;
+        pos[1] = co + i - 1;
+        pos[2] = co + i;
+      } else {
+        const int a1 = co + ((a + 1) >> 1);
+        pos[0] = a1;
+        pos[1] = a1 + 1 - 2 * (a & 1);
+        pos[2] = a1;
+        n = 2;
+      }
+      for (int v = 0; v < 3; v++)
+        if (v < n) __builtin_amdgcn_global_load_lds((const void *)at(pos[v]), LDS_PTR(&s_cv[it][v][wv][0]), 4, 0, 0);
+    }
+    // every lane's DMAs complete before any lane reads LDS: a wavefront's
+    // fence orders no DMA, and the compiler does not see the DMAs' LDS
+    // writes, so the wait is explicit and keeps the reads below it
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    gsync<64, 2>();
+    // the ghosts of faces without a same-level neighbour (the box's second
+    // cell from the face is of the other colour: in s_o)
+    if (on && nbvx <= 0) {
+      const float4 o = s_o[wv][lane];
+      const float x2 = (nbx == 0) ? o.x : o.w;  // cell 2 or NC - 1 of the row
+      CoarseVals<float> cv = {{0.0f, 0.0f, 0.0f, 0.0f, 0.0f}};
+      if (nbvx == 0)
+        for (int v = 0; v < 3; v++) cv.v[v] = s_cv[0][v][wv][lane];
+      s_gx[wv][lane] = ghost_finish<D, NC, float>(bx, m, nbx, ghost_coarse_form<D, NC, float>(bx, m, nbx, j, 0, cv),
+                                                  s_x1[0][wv][lane], x2);
+    }
+    if (on && nbvy <= 0) {
+      const float4 o = s_o[wv][8 * bl + (fy ? NC - 2 : 1)];  // row 2 or NC - 1
+      const float x2 = (mm == 0) ? o.x : (mm == 1) ? o.y : (mm == 2) ? o.z : o.w;
+      CoarseVals<float> cv = {{0.0f, 0.0f, 0.0f, 0.0f, 0.0f}};
+      if (nbvy == 0)
+        for (int v = 0; v < 3; v++) cv.v[v] = s_cv[1][v][wv][lane];
+      reinterpret_cast<float *>(&s_y[wv][8 * bl + 4 * fy])[mm] = ghost_finish<D, NC, float>(
+          bx, m, nby, ghost_coarse_form<D, NC, float>(bx, m, nby, iy, 0, cv), s_x1[1][wv][lane], x2);
+    }
+    gsync<64, 2>();
+    if (on) {
+      // four cells of row j at i = i0 + 2 c; the x neighbours are the row's
+      // other-colour word and its x ghost, the y neighbours the words of
+      // rows j -+ 1 or the y ghosts
+      const Sten<D, float> st = lpl_stencil<D, float>(bx, lambda2);
+      const float4 xc = s_o[wv][lane];
+      const float xe = s_gx[wv][lane];
+      float xw[5];
+      if (i0 == 1) {
+        xw[0] = xe; xw[1] = xc.x; xw[2] = xc.y; xw[3] = xc.z; xw[4] = xc.w;
+      } else {
+        xw[0] = xc.x; xw[1] = xc.y; xw[2] = xc.z; xw[3] = xc.w; xw[4] = xe;
+      }
+      const float4 ym4 = (j == 1) ? s_y[wv][8 * bl] : s_o[wv][lane - 1];
+      const float4 yp4 = (j == NC) ? s_y[wv][8 * bl + 4] : s_o[wv][lane + 1];
+      const float ym[4] = {ym4.x, ym4.y, ym4.z, ym4.w}, yp[4] = {yp4.x, yp4.y, yp4.z, yp4.w};
+      const float4 rv = s_r[wv][lane];
+      const float rr[4] = {rv.x, rv.y, rv.z, rv.w};
+      float res[4];
+      if (cyl) {
+        for (int c = 0; c < 4; c++) {
+          float c1, c2, c3;
+          cyl_coeffs<float>(bx, i0 + 2 * c, st.c, c1, c2, c3);
+          const float inv1 = 1.0f / c1;
+          res[c] = (rr[c] - c2 * xw[c] - c3 * xw[c + 1] - st.c[3] * ym[c] - st.c[4] * yp[c]) * inv1;
+        }
+      } else {
+        const float inv_c1 = 1.0f / st.c[0];
+        for (int c = 0; c < 4; c++)
+          res[c] = (rr[c] - st.c[1] * xw[c] - st.c[2] * xw[c + 1] - st.c[3] * ym[c] - st.c[4] * yp[c]) * inv_c1;
+      }
+      reinterpret_cast<float4 *>(u + uc * HALF)[r] = make_float4(res[0], res[1], res[2], res[3]);
+    }
+    gsync<64, 2>();  // (the wavefront's LDS is reused by its next boxes)
+  }
+}
+
 // k_mp_residual with the box staged in LDS.
 template <int D, int NC, int BS, int BB = 1>
 __global__ void __launch_bounds__(BS * BB) k_mp_residual_lds(const int *__restrict__ ids,

@@ -420,9 +420,10 @@ int afgpu_kernel_stats(afgpu_t *h, int kind, int reset, double *total_ms,
                                    (1) or FP64 (0); the FP64 outer correction sets the accuracy
                                    either way (default 0: the coarsest-grid solve stays FP64;
                                    0 whenever AFGPU_OPT_FUSE_DIRECT is 1) */
-#define AFGPU_OPT_BOX_LDS 12 /* mixed precision, 3D boxes of 16^3 cells: the inner cycle's
-                               half-sweeps stage the whole box in LDS (1), or only the other
-                               colour's block as it is stored and the rhs (0); identical
+#define AFGPU_OPT_BOX_LDS 12 /* mixed precision, 3D boxes of 16^3 cells and 2D boxes of 8^2:
+                               the inner cycle's half-sweeps stage the whole box in LDS (1),
+                               or only what they read (the other colour's block as it is
+                               stored, the rhs, the face inputs), by DMA (0); identical
                                results (default 0) */
 #define AFGPU_OPT_DIRECT_FOLD 13 /* mixed precision: the FP64 transforms (128 cells per axis) of
                                    axes whose 1-D operator is symmetric under reversal (the same
