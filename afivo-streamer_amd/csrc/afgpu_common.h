@@ -378,6 +378,15 @@ __device__ inline int lgrp() {
   if (BB == 1) return 0;
   return __builtin_amdgcn_readfirstlane(threadIdx.x / BS);
 }
+// The first work item of workgroup blockIdx.x when the grid's workgroups
+// are dealt to the eight XCDs in turn (workgroup g to XCD g mod 8): XCD x
+// gets a contiguous range of items instead of every eighth one, so boxes
+// that are neighbours in the list share that XCD's L2 and a face line read
+// by two of them is fetched from HBM once.  A bijection of [0, gridDim.x).
+__device__ inline int xcd_item() {
+  const int g = blockIdx.x, n = gridDim.x, x = g & 7, k = g >> 3, per = n >> 3, rem = n & 7;
+  return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + k;
+}
 template <int BS, int BB>
 __device__ inline void gsync() {
   static_assert(BB == 1 || BS == 64, "several groups per workgroup need one wavefront per group");

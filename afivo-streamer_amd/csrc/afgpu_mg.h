@@ -1446,7 +1446,7 @@ __global__ void __launch_bounds__(256) k_gsrb_split2d(const int *__restrict__ id
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x & 63;
   const int bl = lane >> 3, r = lane & 7;
   const int uc = redblack & 1, oc = 1 - uc;    // updated cells: (i + j + redblack) even
-  for (int w = blockIdx.x; w < nw; w += gridDim.x) {
+  for (int w = xcd_item(); w < nw; w += gridDim.x) {
     const int b = (4 * w + wv) * NBW + bl;
     const int id = b < n_boxes ? ids[b] : 0;
     const bool on = id > 0;  // (past the end, or a run masked out)
