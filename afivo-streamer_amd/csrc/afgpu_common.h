@@ -10,22 +10,28 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
 
 namespace afg {
 
-struct DBox {
+// A box's record.  What a smoothing half-sweep reads (neighbours, level,
+// position, coordinates) comes first and the record is aligned to a
+// 128-byte cache line, so those fields cost one line per box: with 8^2
+// boxes a line per box is a third of the half-sweep's own bytes.
+struct alignas(128) DBox {
   int lvl;
   int run;  // index of the independent run (tree of a forest) the box belongs to
-  int parent;
-  int ix[3];
-  int children[8];
   int nb[6];
   int pnb[6];  // faces on a refinement boundary (nb 0): the parent's neighbour there; else 0
-  int nbmat[27];
+  int ix[3];
+  int parent;
   double r_min[3];
   double dr[3];
+  int children[8];
+  int nbmat[27];
 };
+static_assert(offsetof(DBox, dr) + 3 * sizeof(double) <= 128, "the fields a half-sweep reads share one line");
 
 enum BcType { BC_DIRICHLET = 0, BC_NEUMANN = 1, BC_DIRICHLET_COPY = 2 };
 enum RbType { RB_NONE = 0, RB_MG = 1, RB_INTERP = 2, RB_INTERP_LIM = 3 };
